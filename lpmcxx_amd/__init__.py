@@ -1,0 +1,281 @@
+"""lpmcxx_amd -- MI355X-native spherical Biot-Savart / Poisson sum.
+
+Python access to the two native libraries built in-tree:
+
+* ``lib/liblpm_gpu.so`` -- HIP kernels for gfx950 behind the C ABI in
+  ``include/lpm_gpu.h`` (direct sum, octree, treecode, RK4, order model).
+* ``lib/liblpm.so`` -- the C++ host library (namespace ``Lpm``, the
+  reference's API) and the mesh builder C ABI in ``include/lpm_mesh.h``.
+
+There is no CPU fallback: every compute entry point goes through the HIP
+library and raises if it is missing or fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+LIBDIR = ROOT / "lib"
+
+LPM_K_BIOT_SAVART_SPHERE = 1
+LPM_K_GREENS_SPHERE = 2
+LPM_F64 = 0
+LPM_F32C = 1
+LPM_SEED_ICOS_TRI = 0
+LPM_SEED_CUBED = 1
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+
+class LpmError(RuntimeError):
+    """Raised when a native entry point returns a non-zero lpm_status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"lpm status {status}: {message}")
+        self.status = status
+
+
+_gpu = None
+_host = None
+
+
+def _load(name: str) -> C.CDLL:
+    path = LIBDIR / name
+    if not path.exists():
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() or python -m lpmcxx_amd.build")
+    return C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+
+
+def gpu_lib() -> C.CDLL:
+    """The HIP library (loaded once)."""
+    global _gpu
+    if _gpu is None:
+        lib = _load("liblpm_gpu.so")
+        lib.lpm_create.argtypes = [C.POINTER(_vp), C.c_int]
+        lib.lpm_destroy.argtypes = [_vp]
+        lib.lpm_destroy.restype = None
+        lib.lpm_status_string.argtypes = [C.c_int]
+        lib.lpm_status_string.restype = C.c_char_p
+        lib.lpm_last_error.argtypes = [_vp]
+        lib.lpm_last_error.restype = C.c_char_p
+        lib.lpm_synchronize.argtypes = [_vp]
+        lib.lpm_stream.argtypes = [_vp]
+        lib.lpm_stream.restype = _vp
+        ds_args = [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64,
+                   _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]
+        lib.lpm_direct_sum.argtypes = ds_args
+        lib.lpm_direct_sum_dev.argtypes = ds_args
+        _declare_extra(lib)
+        _gpu = lib
+    return _gpu
+
+
+def _declare_extra(lib: C.CDLL) -> None:
+    """Signatures of the tree / RK4 / order-model entry points (when present)."""
+    sigs = {
+        "lpm_tree_build": [_vp, _vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, _i64p, C.POINTER(C.c_int)],
+        "lpm_tree_build_dev": [_vp, _vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, _i64p, C.POINTER(C.c_int)],
+        "lpm_tree_moments": [_vp, _vp, C.c_int, C.c_int],
+        "lpm_tree_moments_dev": [_vp, _vp, C.c_int, C.c_int],
+        "lpm_tree_eval": [_vp, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _i64p],
+        "lpm_tree_eval_dev": [_vp, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _i64p],
+        "lpm_tree_info": [_vp, _i64p, C.POINTER(C.c_int), _i64p],
+        "lpm_tree_export": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    }
+    for name, args in sigs.items():
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = args
+
+
+def host_lib() -> C.CDLL:
+    """The C++ host library with the mesh C ABI (loaded once)."""
+    global _host
+    if _host is None:
+        gpu_lib()
+        lib = _load("liblpm.so")
+        lib.lpm_mesh_create.argtypes = [C.POINTER(_vp), C.c_int, C.c_int, C.c_double]
+        lib.lpm_mesh_destroy.argtypes = [_vp]
+        lib.lpm_mesh_destroy.restype = None
+        lib.lpm_mesh_sizes.argtypes = [_vp, _i64p, _i64p, _i64p]
+        lib.lpm_mesh_vertices.argtypes = [_vp, _vp, _vp, _vp]
+        lib.lpm_mesh_leaf_faces.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        lib.lpm_mesh_faces.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        lib.lpm_mesh_max_radius.argtypes = [_vp]
+        lib.lpm_mesh_max_radius.restype = C.c_double
+        _host = lib
+    return _host
+
+
+def _ptr(a) -> int | None:
+    """Address of a contiguous numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+# ---------------------------------------------------------------------- mesh
+class Mesh:
+    """Vertices and leaf faces of a refined spherical mesh (reference index order)."""
+
+    def __init__(self, seed: int, level: int, radius: float = 1.0):
+        lib = host_lib()
+        h = _vp()
+        st = lib.lpm_mesh_create(C.byref(h), seed, level, radius)
+        if st != 0:
+            raise LpmError(st, f"mesh seed={seed} level={level}")
+        try:
+            nv, nf, nl = C.c_int64(), C.c_int64(), C.c_int64()
+            lib.lpm_mesh_sizes(h, C.byref(nv), C.byref(nf), C.byref(nl))
+            self.n_vertices, self.n_faces, self.n_leaves = nv.value, nf.value, nl.value
+            self.vx = np.empty(self.n_vertices)
+            self.vy = np.empty(self.n_vertices)
+            self.vz = np.empty(self.n_vertices)
+            lib.lpm_mesh_vertices(h, _ptr(self.vx), _ptr(self.vy), _ptr(self.vz))
+            self.cx = np.empty(self.n_leaves)
+            self.cy = np.empty(self.n_leaves)
+            self.cz = np.empty(self.n_leaves)
+            self.area = np.empty(self.n_leaves)
+            self.face_index = np.empty(self.n_leaves, dtype=np.int64)
+            lib.lpm_mesh_leaf_faces(h, _ptr(self.cx), _ptr(self.cy), _ptr(self.cz), _ptr(self.area),
+                                    _ptr(self.face_index))
+            self.max_radius = lib.lpm_mesh_max_radius(h)
+        finally:
+            lib.lpm_mesh_destroy(h)
+        self.seed, self.level, self.radius = seed, level, radius
+
+    def targets(self):
+        """Targets of meshSolve: all vertices, then all leaf faces."""
+        return (np.concatenate([self.vx, self.cx]), np.concatenate([self.vy, self.cy]),
+                np.concatenate([self.vz, self.cz]))
+
+    def face_self_skip(self) -> np.ndarray:
+        """Per-target source index to exclude (leaf face i is source i); -1 for vertices."""
+        return np.concatenate([np.full(self.n_vertices, -1, np.int64), np.arange(self.n_leaves, dtype=np.int64)])
+
+
+# ---------------------------------------------------------------------- GPU context
+class Context:
+    """One lpm_ctx: a HIP stream, its workspace and at most one source octree."""
+
+    def __init__(self, device: int = 0):
+        self.lib = gpu_lib()
+        h = _vp()
+        st = self.lib.lpm_create(C.byref(h), device)
+        if st != 0:
+            raise LpmError(st, self.lib.lpm_status_string(st).decode())
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.lpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int) -> None:
+        if st != 0:
+            raise LpmError(st, f"{self.lib.lpm_status_string(st).decode()}: {self.lib.lpm_last_error(self.h).decode()}")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.lpm_stream(self.h)
+
+    def synchronize(self) -> None:
+        self.check(self.lib.lpm_synchronize(self.h))
+
+    def direct_sum(self, targets, sources, gamma, *, kernels=LPM_K_BIOT_SAVART_SPHERE, prec=LPM_F64,
+                   radius=1.0, delta=0.0, skip=None):
+        """Host-array direct sum; returns (ux, uy, uz) and/or psi as float64 arrays."""
+        tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+        sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        nt, ns = tx.size, sx.size
+        sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.int64)
+        vel = bool(kernels & LPM_K_BIOT_SAVART_SPHERE)
+        pot = bool(kernels & LPM_K_GREENS_SPHERE)
+        ux = np.zeros(nt) if vel else None
+        uy = np.zeros(nt) if vel else None
+        uz = np.zeros(nt) if vel else None
+        psi = np.zeros(nt) if pot else None
+        self.check(self.lib.lpm_direct_sum(self.h, kernels, prec, radius, delta, _ptr(tx), _ptr(ty), _ptr(tz), nt,
+                                           _ptr(sx), _ptr(sy), _ptr(sz), _ptr(g), ns, _ptr(sk),
+                                           _ptr(ux), _ptr(uy), _ptr(uz), _ptr(psi)))
+        if vel and pot:
+            return (ux, uy, uz), psi
+        return (ux, uy, uz) if vel else psi
+
+    def direct_sum_dev(self, tx, ty, tz, sx, sy, sz, gamma, out, *, kernels=LPM_K_BIOT_SAVART_SPHERE,
+                       prec=LPM_F64, radius=1.0, delta=0.0, skip=None, psi=None):
+        """Device-tensor direct sum enqueued on the context stream (no synchronize)."""
+        ux, uy, uz = out if out is not None else (None, None, None)
+        self.check(self.lib.lpm_direct_sum_dev(self.h, kernels, prec, radius, delta, _ptr(tx), _ptr(ty), _ptr(tz),
+                                               tx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(gamma), sx.numel(),
+                                               _ptr(skip), _ptr(ux), _ptr(uy), _ptr(uz), _ptr(psi)))
+
+
+def native_libraries() -> list[str]:
+    """Paths of the native libraries this process has loaded."""
+    out = []
+    for lib in (_gpu, _host):
+        if lib is not None:
+            out.append(lib._name)
+    return out
+
+
+# ---------------------------------------------------------------------- analytic fields
+def sphere_harmonic54(x, y, z):
+    """zeta = 30 cos(4 lambda) z (z^2-1)^2 (src/LpmAnalyticFunctions.h:87-103)."""
+    lon = np.mod(np.arctan2(y, x), 2 * np.pi)
+    return 30.0 * np.cos(4.0 * lon) * z * (z * z - 1.0) * (z * z - 1.0)
+
+
+def rh54_velocity(x, y, z):
+    """Exact velocity for sphere_harmonic54 (src/LpmAnalyticFunctions.h:105-120)."""
+    lat = np.arctan2(z, np.sqrt(x * x + y * y))
+    lon = np.mod(np.arctan2(y, x), 2 * np.pi)
+    u = 0.5 * np.cos(4 * lon) * np.cos(lat) ** 3 * (5 * np.cos(2 * lat) - 3)
+    v = 4 * np.cos(lat) ** 3 * np.sin(lat) * np.sin(4 * lon)
+    return (-u * np.sin(lon) - v * np.sin(lat) * np.cos(lon), u * np.cos(lon) - v * np.sin(lat) * np.sin(lon),
+            v * np.cos(lat))
+
+
+def gaussian_vortex(x, y, z, area, lat0_deg=60.0, lon0_deg=0.0, beta=16.0, amp=4 * np.pi):
+    """zeta = amp exp(-beta |x - x0|^2) minus its area-weighted mean."""
+    la, lo = np.radians(lat0_deg), np.radians(lon0_deg)
+    x0 = np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)
+    z_ = amp * np.exp(-beta * ((x - x0[0]) ** 2 + (y - x0[1]) ** 2 + (z - x0[2]) ** 2))
+    return z_ - np.sum(z_ * area) / np.sum(area)
+
+
+def random_vortices(x, y, z, area, n=8, seed=1234):
+    """Sum of n Gaussian vortices (centers uniform on the sphere, amplitude U(-2,2),
+    width U(4,16)) minus the area-weighted mean; numpy PCG64 seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    zeta = np.zeros_like(x)
+    for _ in range(n):
+        zc = rng.uniform(-1.0, 1.0)
+        lam = rng.uniform(0.0, 2 * np.pi)
+        a = rng.uniform(-2.0, 2.0)
+        b = rng.uniform(4.0, 16.0)
+        r = np.sqrt(1 - zc * zc)
+        c = (r * np.cos(lam), r * np.sin(lam), zc)
+        zeta += a * np.exp(-b * ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2))
+    return zeta - np.sum(zeta * area) / np.sum(area)

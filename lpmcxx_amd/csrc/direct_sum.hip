@@ -1,0 +1,403 @@
+// Direct O(N*M) spherical Biot-Savart / Green's-function sums on gfx950.
+//
+// Replaces the pair loops of DirectSum::meshSolve (src/LpmDirectSum.cpp:25-173)
+// with the kernels BiotSavartSphere (src/LpmVectorKernel.h:19-39) and
+// SphereGreensFn (src/LpmScalarKernel.h:31-45).
+//
+// Layout and mapping
+//   * Targets stay in registers: each lane owns TPL targets, a wave owns
+//     64*TPL consecutive targets.
+//   * Sources are read with wave-uniform indices, so every source value is a
+//     scalar (SGPR) operand fetched once per wave through the scalar cache;
+//     no LDS and no barriers are needed.
+//   * Grid = (target groups) x (source splits).  Each (group, split) task
+//     writes one FP64 partial per target; ds_reduce adds the splits in
+//     ascending order in FP64 and applies the cross product and the
+//     -1/(4 pi R) multiplier.  Splits exist so that small problems still fill
+//     256 CUs.
+//
+// FP64 form (LPM_F64): the reference's per-pair arithmetic in its operation
+// order -- cross product x*y, denominator (R*R - x.y) + delta*delta, zero when
+// |den| <= 1e-13, reciprocal by IEEE division, then scaled by the strength.
+// The library is compiled with -ffp-contract=off so each pair value is
+// bit-identical to the reference's; only the summation order differs.
+//
+// FP32C form (LPM_F32C): R^2 - x.y is formed as |d|^2/2 with d = x - y,
+// u = 1/(4 pi R) x * sum_j [2 gamma_j / (|d|^2 + 2 delta^2)] d_j.
+// Positions enter as FP32 offsets from FP64 centers (per wave of targets and
+// per 256-source tile); d = (fp32(c_t - c_s) + o_t) - o_s keeps the relative
+// error of near-pair separations at the offset scale, not at |x| ~ 1.
+// Plain FP32 sums run inside a tile; tile sums enter a TwoSum-compensated
+// running sum; partials leave the kernel in FP64.
+// Exactly coincident pairs (d == 0, e.g. the face-target self pair) give a
+// zero velocity term because the reciprocal is taken of |d|^2 + 2 delta^2 +
+// 1e-24 and multiplied by d == 0.
+#include "lpm_internal.hpp"
+
+#include <cmath>
+
+namespace lpm {
+namespace {
+
+constexpr int kTile = 256;          // sources per tile
+constexpr int kWavesPerBlock = 4;   // 256 threads
+constexpr float kTiny = 1e-24f;     // keeps 1/r2 finite for coincident pairs
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline void two_sum_acc(float& S, float& C, float a) {
+    const float s = S + a;
+    const float bp = s - S;
+    const float e = (S - (s - bp)) + (a - bp);
+    S = s;
+    C += e;
+}
+
+// ---------------------------------------------------------------- source prep
+// One block per 256-source tile: FP64 mean center and FP32 offsets + strength.
+__global__ __launch_bounds__(256) void ds_prep_src_f32c(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                        const double* __restrict__ sz, const double* __restrict__ g,
+                                                        int64_t ns, float4* __restrict__ out,
+                                                        double* __restrict__ centers) {
+    __shared__ double red[4][kWavesPerBlock];
+    const int t = threadIdx.x;
+    const int64_t j = int64_t(blockIdx.x) * kTile + t;
+    const bool v = j < ns;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (v) { x = sx[j]; y = sy[j]; z = sz[j]; }
+    double a = wave_sum(x), b = wave_sum(y), c = wave_sum(z), n = wave_sum(v ? 1.0 : 0.0);
+    if ((t & 63) == 0) { red[0][t >> 6] = a; red[1][t >> 6] = b; red[2][t >> 6] = c; red[3][t >> 6] = n; }
+    __syncthreads();
+    double cx = 0.0, cy = 0.0, cz = 0.0, cn = 0.0;
+    for (int w = 0; w < kWavesPerBlock; ++w) { cx += red[0][w]; cy += red[1][w]; cz += red[2][w]; cn += red[3][w]; }
+    cx /= cn; cy /= cn; cz /= cn;
+    if (t == 0) {
+        centers[4 * blockIdx.x + 0] = cx;
+        centers[4 * blockIdx.x + 1] = cy;
+        centers[4 * blockIdx.x + 2] = cz;
+        centers[4 * blockIdx.x + 3] = 0.0;
+    }
+    out[j] = v ? make_float4(float(x - cx), float(y - cy), float(z - cz), float(g[j])) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void ds_prep_src_f64(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                       const double* __restrict__ sz, const double* __restrict__ g,
+                                                       int64_t ns, int64_t nsp, double4* __restrict__ out) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nsp) return;
+    out[j] = (j < ns) ? make_double4(sx[j], sy[j], sz[j], g[j]) : make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// Sum of all strengths (potential correction term), one block.
+__global__ __launch_bounds__(1024) void ds_gamma_total(const double* __restrict__ g, int64_t ns, double* out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t j = threadIdx.x; j < ns; j += 1024) s += g[j];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; ++w) r += red[w];
+        *out = r;
+    }
+}
+
+// ---------------------------------------------------------------- FP32C pair kernel
+template <int TPL, bool VEL, bool PSI>
+__global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, const double* __restrict__ ty,
+                                               const double* __restrict__ tz, int64_t nt,
+                                               const float4* __restrict__ src, const double* __restrict__ scen,
+                                               int ntiles, int tiles_per_split, float eps2v,
+                                               const int64_t* __restrict__ skip, double* __restrict__ part,
+                                               int64_t ntp) {
+    const int lane = threadIdx.x & 63;
+    const int64_t group = int64_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t base = group * (64 * TPL);
+
+    double px[TPL], py[TPL], pz[TPL];
+    double sxw = 0.0, syw = 0.0, szw = 0.0, cnt = 0.0;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int64_t i = base + k * 64 + lane;
+        const bool v = i < nt;
+        px[k] = v ? tx[i] : 0.0;
+        py[k] = v ? ty[i] : 0.0;
+        pz[k] = v ? tz[i] : 0.0;
+        sxw += px[k]; syw += py[k]; szw += pz[k]; cnt += v ? 1.0 : 0.0;
+    }
+    cnt = wave_sum(cnt);
+    if (cnt == 0.0) return;  // wave entirely past the end (uniform)
+    const double cwx = wave_sum(sxw) / cnt, cwy = wave_sum(syw) / cnt, cwz = wave_sum(szw) / cnt;
+
+    float ox[TPL], oy[TPL], oz[TPL];
+    float Sx[TPL], Sy[TPL], Sz[TPL], Cx[TPL], Cy[TPL], Cz[TPL], Sp[TPL], Cp[TPL];
+    int sk[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        ox[k] = float(px[k] - cwx); oy[k] = float(py[k] - cwy); oz[k] = float(pz[k] - cwz);
+        Sx[k] = Sy[k] = Sz[k] = Cx[k] = Cy[k] = Cz[k] = Sp[k] = Cp[k] = 0.f;
+        sk[k] = -1;
+        if (PSI && skip) {
+            const int64_t i = base + k * 64 + lane;
+            if (i < nt) sk[k] = int(skip[i]);
+        }
+    }
+
+    const int t0 = blockIdx.y * tiles_per_split;
+    const int t1 = min(t0 + tiles_per_split, ntiles);
+    for (int tile = t0; tile < t1; ++tile) {
+        const float dcx = float(cwx - scen[4 * tile + 0]);
+        const float dcy = float(cwy - scen[4 * tile + 1]);
+        const float dcz = float(cwz - scen[4 * tile + 2]);
+        float qx[TPL], qy[TPL], qz[TPL], ax[TPL], ay[TPL], az[TPL], ap[TPL];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            qx[k] = dcx + ox[k]; qy[k] = dcy + oy[k]; qz[k] = dcz + oz[k];
+            ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+        }
+        const float4* s = src + int64_t(tile) * kTile;
+        const int jbase = tile * kTile;
+#pragma unroll 8
+        for (int j = 0; j < kTile; ++j) {
+            const float4 q = s[j];
+#pragma unroll
+            for (int k = 0; k < TPL; ++k) {
+                const float dx = qx[k] - q.x;
+                const float dy = qy[k] - q.y;
+                const float dz = qz[k] - q.z;
+                if (VEL) {
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
+                    const float w = q.w * __builtin_amdgcn_rcpf(r2);
+                    ax[k] = fmaf(w, dx, ax[k]);
+                    ay[k] = fmaf(w, dy, ay[k]);
+                    az[k] = fmaf(w, dz, az[k]);
+                }
+                if (PSI) {
+                    const float r2p = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float l = q.w * __builtin_amdgcn_logf(r2p);
+                    ap[k] += (jbase + j == sk[k]) ? 0.f : l;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            if (VEL) {
+                two_sum_acc(Sx[k], Cx[k], ax[k]);
+                two_sum_acc(Sy[k], Cy[k], ay[k]);
+                two_sum_acc(Sz[k], Cz[k], az[k]);
+            }
+            if (PSI) two_sum_acc(Sp[k], Cp[k], ap[k]);
+        }
+    }
+
+    double* p = part + int64_t(blockIdx.y) * 4 * ntp;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int64_t i = base + k * 64 + lane;
+        if (i >= nt) continue;
+        if (VEL) {
+            p[i] = double(Sx[k]) + double(Cx[k]);
+            p[ntp + i] = double(Sy[k]) + double(Cy[k]);
+            p[2 * ntp + i] = double(Sz[k]) + double(Cz[k]);
+        }
+        if (PSI) p[3 * ntp + i] = double(Sp[k]) + double(Cp[k]);
+    }
+}
+
+// ---------------------------------------------------------------- FP64 pair kernel
+template <int TPL, bool VEL, bool PSI>
+__global__ __launch_bounds__(256) void ds_f64(const double* __restrict__ tx, const double* __restrict__ ty,
+                                              const double* __restrict__ tz, int64_t nt,
+                                              const double4* __restrict__ src, int64_t nsp, int64_t per_split,
+                                              double R2, double d2, const int64_t* __restrict__ skip,
+                                              double* __restrict__ part, int64_t ntp) {
+    const int lane = threadIdx.x & 63;
+    const int64_t group = int64_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t base = group * (64 * TPL);
+    if (base >= nt) return;
+
+    double px[TPL], py[TPL], pz[TPL], ax[TPL], ay[TPL], az[TPL], ap[TPL];
+    int64_t sk[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int64_t i = base + k * 64 + lane;
+        const bool v = i < nt;
+        px[k] = v ? tx[i] : 0.0;
+        py[k] = v ? ty[i] : 0.0;
+        pz[k] = v ? tz[i] : 0.0;
+        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+        sk[k] = (PSI && skip && v) ? skip[i] : -1;
+    }
+    const int64_t j0 = int64_t(blockIdx.y) * per_split;
+    const int64_t j1 = min(j0 + per_split, nsp);
+#pragma unroll 4
+    for (int64_t j = j0; j < j1; ++j) {
+        const double4 s = src[j];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            if (VEL) {
+                const double cx = py[k] * s.z - pz[k] * s.y;
+                const double cy = pz[k] * s.x - px[k] * s.z;
+                const double cz = px[k] * s.y - py[k] * s.x;
+                const double dot = px[k] * s.x + py[k] * s.y + pz[k] * s.z;
+                const double den = R2 - dot + d2;
+                const double inv = 1.0 / den;
+                const bool ok = fabs(den) > 1.0e-13;
+                ax[k] += ok ? (cx * inv) * s.w : 0.0;
+                ay[k] += ok ? (cy * inv) * s.w : 0.0;
+                az[k] += ok ? (cz * inv) * s.w : 0.0;
+            }
+            if (PSI) {
+                const double dot = s.x * px[k] + s.y * py[k] + s.z * pz[k];
+                const double l = log(R2 - dot) * s.w;
+                ap[k] += (j == sk[k]) ? 0.0 : l;
+            }
+        }
+    }
+    double* p = part + int64_t(blockIdx.y) * 4 * ntp;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int64_t i = base + k * 64 + lane;
+        if (i >= nt) continue;
+        if (VEL) { p[i] = ax[k]; p[ntp + i] = ay[k]; p[2 * ntp + i] = az[k]; }
+        if (PSI) p[3 * ntp + i] = ap[k];
+    }
+}
+
+// ---------------------------------------------------------------- reduction over splits
+// f32c: u = vel_scale * (x * S), psi = psi_scale * (A - (Gtot - gamma[skip]))
+// f64 : u = vel_scale * S,       psi = psi_scale * A
+template <bool F32C>
+__global__ __launch_bounds__(256) void ds_reduce(const double* __restrict__ part, int nsplit, int64_t ntp, int64_t nt,
+                                                 const double* __restrict__ tx, const double* __restrict__ ty,
+                                                 const double* __restrict__ tz, int vel, int psi, double vel_scale,
+                                                 double psi_scale, const double* __restrict__ gtot,
+                                                 const double* __restrict__ gamma, const int64_t* __restrict__ skip,
+                                                 double* ux, double* uy, double* uz, double* upsi) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        const double* p = part + int64_t(s) * 4 * ntp;
+        if (vel) { sx += p[i]; sy += p[ntp + i]; sz += p[2 * ntp + i]; }
+        if (psi) sp += p[3 * ntp + i];
+    }
+    if (vel) {
+        if (F32C) {
+            const double x = tx[i], y = ty[i], z = tz[i];
+            ux[i] = vel_scale * (y * sz - z * sy);
+            uy[i] = vel_scale * (z * sx - x * sz);
+            uz[i] = vel_scale * (x * sy - y * sx);
+        } else {
+            ux[i] = sx * vel_scale;
+            uy[i] = sy * vel_scale;
+            uz[i] = sz * vel_scale;
+        }
+    }
+    if (psi) {
+        if (F32C) {
+            double b = *gtot;
+            if (skip && skip[i] >= 0) b -= gamma[skip[i]];
+            upsi[i] = psi_scale * (sp - b);
+        } else {
+            upsi[i] = sp * psi_scale;
+        }
+    }
+}
+
+template <int TPL, bool VEL, bool PSI>
+void launch_f32c(dim3 g, hipStream_t st, const double* tx, const double* ty, const double* tz, int64_t nt,
+                 const float4* src, const double* scen, int ntiles, int tps, float eps2v, const int64_t* skip,
+                 double* part, int64_t ntp) {
+    hipLaunchKernelGGL((ds_f32c<TPL, VEL, PSI>), g, dim3(256), 0, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v,
+                       skip, part, ntp);
+}
+
+template <int TPL, bool VEL, bool PSI>
+void launch_f64(dim3 g, hipStream_t st, const double* tx, const double* ty, const double* tz, int64_t nt,
+                const double4* src, int64_t nsp, int64_t per, double R2, double d2, const int64_t* skip, double* part,
+                int64_t ntp) {
+    hipLaunchKernelGGL((ds_f64<TPL, VEL, PSI>), g, dim3(256), 0, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip,
+                       part, ntp);
+}
+
+constexpr int kTplF32 = 4;
+constexpr int kTplF64 = 2;
+
+}  // namespace
+
+int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                      const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                      const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy,
+                      double* uz, double* psi) {
+    const bool vel = (kernels & LPM_K_BIOT_SAVART_SPHERE) != 0;
+    const bool pot = (kernels & LPM_K_GREENS_SPHERE) != 0;
+    if (nt == 0) return LPM_OK;
+    hipStream_t st = c->stream;
+    if (ns == 0) {  // empty source set: zero outputs
+        if (vel) {
+            LPM_HIP(c, hipMemsetAsync(ux, 0, nt * sizeof(double), st));
+            LPM_HIP(c, hipMemsetAsync(uy, 0, nt * sizeof(double), st));
+            LPM_HIP(c, hipMemsetAsync(uz, 0, nt * sizeof(double), st));
+        }
+        if (pot) LPM_HIP(c, hipMemsetAsync(psi, 0, nt * sizeof(double), st));
+        return LPM_OK;
+    }
+    const int ntiles = int((ns + kTile - 1) / kTile);
+    const int64_t nsp = int64_t(ntiles) * kTile;
+    const int tpl = (prec == LPM_F32C) ? kTplF32 : kTplF64;
+    const int64_t per_block = int64_t(kWavesPerBlock) * 64 * tpl;
+    const int64_t gx = (nt + per_block - 1) / per_block;
+    // enough (group, split) tasks for 8 blocks per CU on 256 CUs
+    int nsplit = int(std::min<int64_t>(ntiles, std::max<int64_t>(1, (2048 + gx - 1) / gx)));
+    const int tps = (ntiles + nsplit - 1) / nsplit;
+    nsplit = (ntiles + tps - 1) / tps;
+    const int64_t ntp = nt;
+
+    LPM_CHECK(c->ds_part.reserve(size_t(nsplit) * 4 * ntp * sizeof(double)));
+    LPM_CHECK(c->ds_misc.reserve(64));
+    double* part = c->ds_part.as<double>();
+    double* gtot = c->ds_misc.as<double>();
+
+    const double pi = 3.1415926535897932384626433832795027975;
+    if (prec == LPM_F32C) {
+        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * sizeof(float4) + size_t(ntiles) * 4 * sizeof(double)));
+        float4* src = c->ds_src.as<float4>();
+        double* scen = reinterpret_cast<double*>(src + nsp);
+        hipLaunchKernelGGL(ds_prep_src_f32c, dim3(ntiles), dim3(256), 0, st, sx, sy, sz, gamma, ns, src, scen);
+        if (pot) hipLaunchKernelGGL(ds_gamma_total, dim3(1), dim3(1024), 0, st, gamma, ns, gtot);
+        const float eps2v = float(2.0 * delta * delta) + kTiny;
+        const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
+        if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
+        else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
+        else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
+        LPM_HIP(c, hipGetLastError());
+        hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp, nt,
+                           tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),
+                           -std::log(2.0) / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
+    } else {
+        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * sizeof(double4)));
+        double4* src = c->ds_src.as<double4>();
+        hipLaunchKernelGGL(ds_prep_src_f64, dim3(unsigned((nsp + 255) / 256)), dim3(256), 0, st, sx, sy, sz, gamma,
+                           ns, nsp, src);
+        const int64_t per = int64_t(tps) * kTile;
+        const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
+        const double R2 = radius * radius, d2 = delta * delta;
+        if (vel && pot) launch_f64<kTplF64, true, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        else if (vel) launch_f64<kTplF64, true, false>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        else launch_f64<kTplF64, false, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        LPM_HIP(c, hipGetLastError());
+        hipLaunchKernelGGL(ds_reduce<false>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp,
+                           nt, tx, ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius),
+                           -1.0 / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
+    }
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
+}
+
+}  // namespace lpm

@@ -1,0 +1,173 @@
+// C ABI of liblpm_gpu: contexts, status codes, host-pointer wrappers.
+#include "lpm_internal.hpp"
+
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+namespace lpm {
+
+int DevBuf::reserve(size_t n) {
+    if (n <= bytes) return LPM_OK;
+    if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+    if (hipMalloc(&ptr, n) != hipSuccess) { ptr = nullptr; (void)hipGetLastError(); return LPM_ERR_ALLOC; }
+    bytes = n;
+    return LPM_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int fail(lpm_ctx* c, int status, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return status;
+}
+
+int hip_fail(lpm_ctx* c, hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory) return fail(c, LPM_ERR_ALLOC, "%s: %s", what, hipGetErrorString(e));
+    return fail(c, LPM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace lpm
+
+using namespace lpm;
+
+extern "C" {
+
+int lpm_create(lpm_ctx** out, int device) {
+    if (!out) return LPM_ERR_NULL_PTR;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return LPM_ERR_HIP; }
+    if (device < 0 || device >= n) return LPM_ERR_INVALID_ARG;
+    lpm_ctx* c = new (std::nothrow) lpm_ctx();
+    if (!c) return LPM_ERR_ALLOC;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LPM_ERR_HIP;
+    }
+    *out = c;
+    return LPM_OK;
+}
+
+void lpm_destroy(lpm_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    tree_free(c->tree);
+    c->tree = nullptr;
+    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out}) b->release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lpm_status_string(int s) {
+    switch (s) {
+        case LPM_OK: return "ok";
+        case LPM_ERR_INVALID_ARG: return "invalid argument";
+        case LPM_ERR_NULL_PTR: return "null pointer";
+        case LPM_ERR_HIP: return "HIP runtime error";
+        case LPM_ERR_ALLOC: return "device allocation failed";
+        case LPM_ERR_STATE: return "invalid call order";
+        case LPM_ERR_TREE_DEPTH: return "octree depth limit exceeded";
+        case LPM_ERR_KERNEL: return "invalid kernel";
+        default: return "unknown status";
+    }
+}
+
+const char* lpm_last_error(const lpm_ctx* c) { return c ? c->err.c_str() : ""; }
+
+int lpm_synchronize(lpm_ctx* c) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+void* lpm_stream(lpm_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+static int check_direct_args(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                             const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                             const double* sz, const double* gamma, int64_t ns, double* ux, double* uy, double* uz,
+                             double* psi) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (kernels <= 0 || (kernels & ~(LPM_K_BIOT_SAVART_SPHERE | LPM_K_GREENS_SPHERE)))
+        return fail(c, LPM_ERR_KERNEL, "invalid kernel mask %d", kernels);
+    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
+    if (nt < 0 || ns < 0 || ns > (int64_t(1) << 31) - 4096 || nt > (int64_t(1) << 40))
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid sizes nt=%lld ns=%lld", (long long)nt, (long long)ns);
+    if (!(radius > 0.0) || !(delta >= 0.0) || !std::isfinite(radius) || !std::isfinite(delta))
+        return fail(c, LPM_ERR_INVALID_ARG, "radius must be > 0 and delta >= 0");
+    if (nt > 0 && (!tx || !ty || !tz)) return fail(c, LPM_ERR_NULL_PTR, "target coordinates are NULL");
+    if (ns > 0 && (!sx || !sy || !sz || !gamma)) return fail(c, LPM_ERR_NULL_PTR, "source arrays are NULL");
+    if (nt > 0 && (kernels & LPM_K_BIOT_SAVART_SPHERE) && (!ux || !uy || !uz))
+        return fail(c, LPM_ERR_NULL_PTR, "velocity outputs are NULL");
+    if (nt > 0 && (kernels & LPM_K_GREENS_SPHERE) && !psi) return fail(c, LPM_ERR_NULL_PTR, "potential output is NULL");
+    return LPM_OK;
+}
+
+int lpm_direct_sum_dev(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                       const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                       const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy,
+                       double* uz, double* psi) {
+    LPM_CHECK(check_direct_args(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, ux, uy, uz, psi));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return direct_sum_device(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz,
+                             psi);
+}
+
+int lpm_direct_sum(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx, const double* ty,
+                   const double* tz, int64_t nt, const double* sx, const double* sy, const double* sz,
+                   const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy, double* uz,
+                   double* psi) {
+    LPM_CHECK(check_direct_args(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, ux, uy, uz, psi));
+    LPM_HIP(c, hipSetDevice(c->device));
+    if (nt == 0) return LPM_OK;
+    const bool vel = kernels & LPM_K_BIOT_SAVART_SPHERE, pot = kernels & LPM_K_GREENS_SPHERE;
+    const size_t in_bytes = sizeof(double) * (3 * size_t(nt) + 4 * size_t(ns)) + (skip ? sizeof(int64_t) * nt : 0);
+    LPM_CHECK(c->h_in.reserve(in_bytes));
+    LPM_CHECK(c->h_out.reserve(sizeof(double) * 4 * size_t(nt)));
+    double* d = c->h_in.as<double>();
+    double *dtx = d, *dty = d + nt, *dtz = d + 2 * nt;
+    double *dsx = d + 3 * nt, *dsy = dsx + ns, *dsz = dsy + ns, *dg = dsz + ns;
+    int64_t* dskip = skip ? reinterpret_cast<int64_t*>(dg + ns) : nullptr;
+    double* o = c->h_out.as<double>();
+    hipStream_t st = c->stream;
+    auto h2d = [&](double* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    };
+    LPM_HIP(c, h2d(dtx, tx, nt * sizeof(double)));
+    LPM_HIP(c, h2d(dty, ty, nt * sizeof(double)));
+    LPM_HIP(c, h2d(dtz, tz, nt * sizeof(double)));
+    if (ns > 0) {
+        LPM_HIP(c, h2d(dsx, sx, ns * sizeof(double)));
+        LPM_HIP(c, h2d(dsy, sy, ns * sizeof(double)));
+        LPM_HIP(c, h2d(dsz, sz, ns * sizeof(double)));
+        LPM_HIP(c, h2d(dg, gamma, ns * sizeof(double)));
+    }
+    if (skip) LPM_HIP(c, hipMemcpyAsync(dskip, skip, nt * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    LPM_CHECK(direct_sum_device(c, kernels, prec, radius, delta, dtx, dty, dtz, nt, dsx, dsy, dsz, dg, ns, dskip,
+                                vel ? o : nullptr, vel ? o + nt : nullptr, vel ? o + 2 * nt : nullptr,
+                                pot ? o + 3 * nt : nullptr));
+    if (vel) {
+        LPM_HIP(c, hipMemcpyAsync(ux, o, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(uy, o + nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(uz, o + 2 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (pot) LPM_HIP(c, hipMemcpyAsync(psi, o + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    return LPM_OK;
+}
+
+}  // extern "C"

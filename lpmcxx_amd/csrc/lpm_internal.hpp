@@ -1,0 +1,66 @@
+// Internal declarations shared by the HIP translation units of liblpm_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/lpm_gpu.h"
+
+namespace lpm {
+
+constexpr int kWave = 64;
+
+// Growable device buffer owned by a context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t n);   // grows (never shrinks); returns lpm_status
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Tree;  // octree state (octree.hip)
+
+}  // namespace lpm
+
+struct lpm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // direct-sum workspace
+    lpm::DevBuf ds_src, ds_tgt, ds_part, ds_misc;
+    // staging for the host-pointer entry points
+    lpm::DevBuf h_in, h_out;
+    lpm::Tree* tree = nullptr;
+};
+
+namespace lpm {
+
+int fail(lpm_ctx* c, int status, const char* fmt, ...);
+int hip_fail(lpm_ctx* c, hipError_t e, const char* what);
+
+#define LPM_HIP(ctx, call)                                  \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return ::lpm::hip_fail(ctx, _e, #call); \
+    } while (0)
+
+#define LPM_CHECK(x)                 \
+    do {                             \
+        int _s = (x);                \
+        if (_s != LPM_OK) return _s; \
+    } while (0)
+
+// direct sum (direct_sum.hip); device pointers, enqueued on ctx->stream
+int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                      const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                      const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux,
+                      double* uy, double* uz, double* psi);
+
+void tree_free(Tree* t);
+
+}  // namespace lpm

@@ -1,0 +1,155 @@
+"""GPU parity of the direct sum (liblpm_gpu, include/lpm_gpu.h) against the oracle.
+
+Bars: FP64 relative L2 and max error <= 1e-12 against the oracle's
+reference-order FP64 sum; FP32C relative L2 <= 1e-5 (velocity) with the
+measured error printed.  Inputs are the mesh particle sets of the workloads
+(icosahedral level 4, cubed-sphere levels 5 and 7) plus ragged, empty,
+smoothed and scaled cases.
+"""
+import numpy as np
+import pytest
+
+import lpmcxx_amd as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VEL, PSI = L.LPM_K_BIOT_SAVART_SPHERE, L.LPM_K_GREENS_SPHERE
+
+
+def _setup(m, field="rh"):
+    if field == "rh":
+        zeta = L.sphere_harmonic54(m.cx, m.cy, m.cz)
+    else:
+        zeta = L.gaussian_vortex(m.cx, m.cy, m.cz, m.area)
+    return m.targets(), (m.cx, m.cy, m.cz), zeta * m.area, m.face_self_skip()
+
+
+@pytest.mark.parametrize("seed,level", [(L.LPM_SEED_ICOS_TRI, 4), (L.LPM_SEED_CUBED, 5)])
+def test_full_mesh_parity(ctx, mesh, seed, level):
+    m = mesh(seed, level)
+    tgt, src, g, skip = _setup(m)
+    u_ref = O.bs_direct(tgt, src, g, skip=skip)
+    p_ref = O.greens_direct(tgt, src, g, skip=skip)
+    (u64, p64) = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F64, skip=skip)
+    assert O.rel_l2(u64, u_ref) <= 1e-12 and O.rel_max(u64, u_ref) <= 1e-12
+    assert O.rel_l2(p64, p_ref) <= 1e-12 and O.rel_max(p64, p_ref) <= 1e-12
+    (u32, p32) = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip)
+    e_u, e_p = O.rel_l2(u32, u_ref), O.rel_l2(p32, p_ref)
+    print(f"\nseed={seed} level={level} f32c rel-L2 velocity {e_u:.3e} potential {e_p:.3e} "
+          f"max {O.rel_max(u32, u_ref):.3e}")
+    assert e_u <= 1e-5 and e_p <= 1e-5
+    # velocity-only and potential-only launches agree with the fused one
+    u_only = ctx.direct_sum(tgt, src, g, kernels=VEL, prec=L.LPM_F32C)
+    p_only = ctx.direct_sum(tgt, src, g, kernels=PSI, prec=L.LPM_F32C, skip=skip)
+    assert O.rel_l2(u_only, u32) < 1e-12 and O.rel_l2(p_only, p32) < 1e-12
+
+
+def test_cubed_level7_sampled_targets(ctx, mesh):
+    """Full GPU sum at 98,306 + 98,304 targets; oracle on 2,048 sampled targets."""
+    m = mesh(L.LPM_SEED_CUBED, 7)
+    tgt, src, g, skip = _setup(m)
+    u64 = ctx.direct_sum(tgt, src, g, prec=L.LPM_F64)
+    u32 = ctx.direct_sum(tgt, src, g, prec=L.LPM_F32C)
+    idx = np.sort(np.random.default_rng(11).choice(len(tgt[0]), 2048, replace=False))
+    sub = tuple(t[idx] for t in tgt)
+    u_ref = O.bs_direct(sub, src, g, skip=skip[idx])
+    u64s = tuple(u[idx] for u in u64)
+    u32s = tuple(u[idx] for u in u32)
+    assert O.rel_l2(u64s, u_ref) <= 1e-12 and O.rel_max(u64s, u_ref) <= 1e-12
+    e = O.rel_l2(u32s, u_ref)
+    print(f"\ncubed L7 f32c rel-L2 {e:.3e} max {O.rel_max(u32s, u_ref):.3e}")
+    assert e <= 1e-5
+    # the analytic Rossby-Haurwitz velocity is approached (first-order method)
+    assert O.rel_l2(u64, L.rh54_velocity(*tgt)) < 2e-3
+
+
+@pytest.mark.parametrize("nt,ns", [(1, 1), (1, 257), (300, 1), (65, 513), (1000, 255), (4097, 3001)])
+def test_ragged_sizes(ctx, nt, ns):
+    rng = np.random.default_rng(nt * 7 + ns)
+    t = rng.normal(size=(3, nt)); t /= np.linalg.norm(t, axis=0)
+    s = rng.normal(size=(3, ns)); s /= np.linalg.norm(s, axis=0)
+    g = rng.normal(size=ns)
+    ref = O.bs_direct(tuple(t), tuple(s), g)
+    pref = O.greens_direct(tuple(t), tuple(s), g)
+    u, p = ctx.direct_sum(tuple(t), tuple(s), g, kernels=VEL | PSI, prec=L.LPM_F64)
+    assert O.rel_l2(u, ref) <= 1e-12 and O.rel_l2(p, pref) <= 1e-12
+    u, p = ctx.direct_sum(tuple(t), tuple(s), g, kernels=VEL | PSI, prec=L.LPM_F32C)
+    assert O.rel_l2(u, ref) <= 1e-5 and O.rel_l2(p, pref) <= 1e-5
+
+
+def test_smoothing_and_radius(ctx, mesh):
+    m = mesh(L.LPM_SEED_ICOS_TRI, 3)
+    tgt, src, g, skip = _setup(m, "gauss")
+    for R, delta in [(1.0, 0.05), (2.0, 0.0), (6371.22, 10.0)]:
+        t = tuple(R * a for a in tgt)
+        s = tuple(R * a for a in src)
+        ref = O.bs_direct(t, s, g, radius=R, delta=delta)
+        u = ctx.direct_sum(t, s, g, prec=L.LPM_F64, radius=R, delta=delta)
+        assert O.rel_l2(u, ref) <= 1e-12, (R, delta)
+        u = ctx.direct_sum(t, s, g, prec=L.LPM_F32C, radius=R, delta=delta)
+        assert O.rel_l2(u, ref) <= 1e-5, (R, delta)
+
+
+def test_empty_inputs(ctx):
+    one = (np.array([1.0]), np.array([0.0]), np.array([0.0]))
+    u = ctx.direct_sum(one, (np.empty(0),) * 3, np.empty(0))
+    assert all(np.array_equal(a, [0.0]) for a in u)
+    u = ctx.direct_sum((np.empty(0),) * 3, one, np.array([1.0]))
+    assert all(a.size == 0 for a in u)
+
+
+def test_coincident_sources_give_zero_velocity(ctx):
+    # every target coincides with one source (the face self pair) and the
+    # reference's |den| <= 1e-13 rule zeroes it; duplicates do not produce NaN
+    rng = np.random.default_rng(3)
+    s = rng.normal(size=(3, 100)); s /= np.linalg.norm(s, axis=0)
+    g = rng.normal(size=100)
+    ref = O.bs_direct(tuple(s), tuple(s), g)
+    for prec, tol in [(L.LPM_F64, 1e-12), (L.LPM_F32C, 1e-5)]:
+        u = ctx.direct_sum(tuple(s), tuple(s), g, prec=prec)
+        assert all(np.isfinite(a).all() for a in u)
+        assert O.rel_l2(u, ref) <= tol
+
+
+def test_deterministic(ctx, mesh):
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    tgt, src, g, skip = _setup(m)
+    for prec in (L.LPM_F64, L.LPM_F32C):
+        a = ctx.direct_sum(tgt, src, g, prec=prec, kernels=VEL | PSI, skip=skip)
+        b = ctx.direct_sum(tgt, src, g, prec=prec, kernels=VEL | PSI, skip=skip)
+        assert all(np.array_equal(x, y) for x, y in zip(a[0], b[0])) and np.array_equal(a[1], b[1])
+
+
+def test_device_pointer_entry(ctx, mesh):
+    import torch
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    tgt, src, g, skip = _setup(m)
+    host = ctx.direct_sum(tgt, src, g, prec=L.LPM_F32C)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    T = [dev(a) for a in tgt]
+    S = [dev(a) for a in src]
+    G = dev(g)
+    out = [torch.empty_like(T[0]) for _ in range(3)]
+    torch.cuda.synchronize()
+    ctx.direct_sum_dev(*T, *S, G, out, prec=L.LPM_F32C)
+    ctx.synchronize()
+    for a, b in zip(out, host):
+        assert np.array_equal(a.cpu().numpy(), b)
+
+
+def test_argument_errors(ctx):
+    one = (np.array([1.0]), np.array([0.0]), np.array([0.0]))
+    with pytest.raises(L.LpmError) as e:
+        ctx.direct_sum(one, one, np.array([1.0]), kernels=4)
+    assert e.value.status == -7  # LPM_ERR_KERNEL, the reference's "invalid kernel"
+    with pytest.raises(L.LpmError) as e:
+        ctx.direct_sum(one, one, np.array([1.0]), prec=9)
+    assert e.value.status == -1
+    with pytest.raises(L.LpmError) as e:
+        ctx.direct_sum(one, one, np.array([1.0]), radius=-1.0)
+    assert e.value.status == -1
+    lib = L.gpu_lib()
+    st = lib.lpm_direct_sum(ctx.h, VEL, 0, 1.0, 0.0, None, None, None, 5, None, None, None, None, 5,
+                            None, None, None, None, None)
+    assert st == -2
