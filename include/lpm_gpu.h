@@ -44,7 +44,18 @@ enum lpm_status {
 #define LPM_F32C 1 /* FP32 pair math on tile-local offsets, |x-y|^2/2 denominator,
                       compensated FP32 tile sums, FP64 final reduction */
 
+/* far-field series of the treecode */
+#define LPM_SERIES_REFERENCE 0 /* SumNode::calc_coeffs: Taylor series of 1/(1 + delta^2 - t.y) about the box
+                                  centroid (src/LpmTreeSum.cpp:149-171); converges only while
+                                  |t.(y - c)| < |1 + delta^2 - t.c| */
+#define LPM_SERIES_CARTESIAN 1 /* Taylor series of the same kernel on the sphere, 2/(|t - y|^2 + 2 delta^2),
+                                  about the box centroid; converges for maxRadius/|t - c| < 1 */
+
 #define LPM_TREE_MAX_DEPTH 21
+
+/* tree depth control (Tree::TREE_DEPTH_CONTROL, src/LpmOctree.h:51) */
+#define LPM_TREE_MAX_PER_NODE 0      /* split iff a node holds more than param sources */
+#define LPM_TREE_MAX_DEPTH_CONTROL 1 /* split every non-empty node above level param */
 
 /* ------------------------------------------------------------------ context */
 int lpm_create(lpm_ctx** ctx, int device);
@@ -83,6 +94,63 @@ int lpm_direct_sum_dev(lpm_ctx* ctx, int kernels, int prec, double radius, doubl
                        const double* sx, const double* sy, const double* sz, const double* gamma, int64_t ns,
                        const int64_t* tgt_self_src,
                        double* ux, double* uy, double* uz, double* psi);
+
+/* ------------------------------------------------------------------ treecode
+ * Source octree (TreeSum mesh constructor, src/LpmTreeSum.cpp:221-238 and
+ * generateTreeFromMesh :266-290): root [-max_radius-1e-5, max_radius+1e-5)^3,
+ * a node splits into its non-empty octants iff it holds more than leaf_max
+ * sources.  Node boxes, leaf lists and the octant order equal the reference's.
+ * Returns LPM_ERR_INVALID_ARG if a source lies outside the root box and
+ * LPM_ERR_TREE_DEPTH if more than leaf_max sources coincide.
+ * Replaces: TreeSum::TreeSum(int, int, shared_ptr<MeshedParticles>, ...) src/LpmTreeSum.h:52-53. */
+int lpm_tree_build(lpm_ctx* ctx, const double* sx, const double* sy, const double* sz, int64_t ns,
+                   double max_radius, int leaf_max, int64_t* n_nodes, int* depth);
+int lpm_tree_build_dev(lpm_ctx* ctx, const double* sx, const double* sy, const double* sz, int64_t ns,
+                       double max_radius, int leaf_max, int64_t* n_nodes, int* depth);
+/* General root box (Tree(crds) + buildTree, src/LpmOctree.cpp:16-36,73-180):
+ * box_min/box_max[3] are the root bounds, control/param as above.  Children
+ * are the non-empty octants, as for the mesh tree.  A source outside the root
+ * box is an error (the reference keeps such a source in the root's list and
+ * drops it from every child).  Host pointers.
+ * Replaces: Tree::Tree(shared_ptr<Coords>, ...) and Tree::buildTree(TREE_DEPTH_CONTROL,
+ * index_type, bool) src/LpmOctree.h:53,68 (do_shrink = false only). */
+int lpm_tree_build_box(lpm_ctx* ctx, const double* sx, const double* sy, const double* sz, int64_t ns,
+                       const double* box_min, const double* box_max, int control, int param,
+                       int64_t* n_nodes, int* depth);
+/* Moments m_k = sum_j gamma_j y_j h_j^k for every node and every |k| < order,
+ * h_j = moment_sign * (y_j - centroid); moment_sign = -1 reproduces the
+ * reference (h = centroid - y, src/LpmTreeSum.cpp:140), +1 is the expansion
+ * the coefficients assume.  order in [1, 12].
+ * Replaces: TreeSum::computeMeshMoments() src/LpmTreeSum.h:59. */
+int lpm_tree_moments(lpm_ctx* ctx, const double* gamma, int order, int moment_sign);
+int lpm_tree_moments_dev(lpm_ctx* ctx, const double* gamma, int order, int moment_sign);
+/* Velocity at nt targets: multipole acceptance maxRadius/|t - c| < theta
+ * (src/LpmTreeSum.cpp:22-24), far field by `series`, near field by the
+ * kernel at unaccepted leaves; result scaled by -1/(4 pi).  mac_count (may be
+ * NULL) receives the number of accepted (target, node) pairs, the reference's
+ * macCounter.  LPM_SERIES_CARTESIAN requires moments with moment_sign = +1.
+ * Replaces: TreeSum::meshSolve(verts, faces, mp_tol, macCounter) src/LpmTreeSum.h:67. */
+int lpm_tree_eval(lpm_ctx* ctx, int prec, int series, double theta, double delta,
+                  const double* tx, const double* ty, const double* tz, int64_t nt,
+                  double* ux, double* uy, double* uz, int64_t* mac_count);
+int lpm_tree_eval_dev(lpm_ctx* ctx, int prec, int series, double theta, double delta,
+                      const double* tx, const double* ty, const double* tz, int64_t nt,
+                      double* ux, double* uy, double* uz, int64_t* mac_count);
+/* node count, depth (root = 0) and leaf count of the current tree */
+int lpm_tree_info(lpm_ctx* ctx, int64_t* n_nodes, int* depth, int64_t* n_leaves);
+/* Node table in breadth-first order (children of a node are consecutive):
+ * box [nn][6] = (xmin, ymin, zmin, xmax, ymax, zmax), cent [nn][4] =
+ * (centroid, maxRadius), level, parent (-1 at the root), first_kid (-1 at a
+ * leaf), n_kids, range [nn][2] = [begin, end) into perm, perm [ns] = source
+ * index at each tree position (leaves list ascending indices).  Host pointers;
+ * any may be NULL.  Replaces: Tree::writeToVtk data, src/LpmOctree.cpp:245-268. */
+int lpm_tree_export(lpm_ctx* ctx, double* box, double* cent, int32_t* level, int32_t* parent,
+                    int32_t* first_kid, int32_t* n_kids, int32_t* range, int32_t* perm);
+/* Moments of the last lpm_tree_moments call, [nn][nterms(order)][3] in the
+ * node order of lpm_tree_export and lex order of the multi-indices
+ * (nterms(p) = p(p+1)(p+2)/6).  Host pointer.
+ * Replaces: SumNode::moments / momentString(), src/LpmTreeSum.h:36-44. */
+int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,8 @@ LPM_K_GREENS_SPHERE = 2
 LPM_F64 = 0
 LPM_F32C = 1
 LPM_SEED_ICOS_TRI = 0
+LPM_SERIES_REFERENCE = 0
+LPM_SERIES_CARTESIAN = 1
 LPM_SEED_CUBED = 1
 
 _dp = C.POINTER(C.c_double)
@@ -83,10 +85,15 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_tree_build_dev": [_vp, _vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, _i64p, C.POINTER(C.c_int)],
         "lpm_tree_moments": [_vp, _vp, C.c_int, C.c_int],
         "lpm_tree_moments_dev": [_vp, _vp, C.c_int, C.c_int],
-        "lpm_tree_eval": [_vp, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _i64p],
-        "lpm_tree_eval_dev": [_vp, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _i64p],
+        "lpm_tree_eval": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp,
+                          _vp, _i64p],
+        "lpm_tree_eval_dev": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp,
+                              _vp, _i64p],
         "lpm_tree_info": [_vp, _i64p, C.POINTER(C.c_int), _i64p],
-        "lpm_tree_export": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "lpm_tree_build_box": [_vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int, C.c_int, _i64p,
+                               C.POINTER(C.c_int)],
+        "lpm_tree_export": [_vp] * 9,
+        "lpm_tree_export_moments": [_vp, _vp],
     }
     for name, args in sigs.items():
         if hasattr(lib, name):

@@ -29,9 +29,11 @@
 // error of near-pair separations at the offset scale, not at |x| ~ 1.
 // Plain FP32 sums run inside a tile; tile sums enter a TwoSum-compensated
 // running sum; partials leave the kernel in FP64.
-// Exactly coincident pairs (d == 0, e.g. the face-target self pair) give a
-// zero velocity term because the reciprocal is taken of |d|^2 + 2 delta^2 +
-// 1e-24 and multiplied by d == 0.
+// The reference zeroes a pair when |R^2 - x.y + delta^2| <= 1e-13, i.e. when
+// |d|^2 + 2 delta^2 <= 2e-13 (this covers the face-target self pair).  Only
+// tiles whose bounding sphere comes within 1e-6 of the wave's targets can hold
+// such pairs; those tiles run a loop with the test, all others the plain
+// 11-instruction pair loop.
 #include "lpm_internal.hpp"
 
 #include <cmath>
@@ -41,11 +43,18 @@ namespace {
 
 constexpr int kTile = 256;          // sources per tile
 constexpr int kWavesPerBlock = 4;   // 256 threads
-constexpr float kTiny = 1e-24f;     // keeps 1/r2 finite for coincident pairs
+constexpr float kZeroTol2 = 2e-13f; // |d|^2 + 2 delta^2 at or below this contributes zero
+constexpr double kNearPad = 1e-6;   // tiles closer than this to a wave run the tested loop
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -75,11 +84,19 @@ __global__ __launch_bounds__(256) void ds_prep_src_f32c(const double* __restrict
     double cx = 0.0, cy = 0.0, cz = 0.0, cn = 0.0;
     for (int w = 0; w < kWavesPerBlock; ++w) { cx += red[0][w]; cy += red[1][w]; cz += red[2][w]; cn += red[3][w]; }
     cx /= cn; cy /= cn; cz /= cn;
+    // tile radius: max distance of a source from the center
+    double r = v ? sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz)) : 0.0;
+    r = wave_max(r);
+    __syncthreads();
+    if ((t & 63) == 0) red[0][t >> 6] = r;
+    __syncthreads();
     if (t == 0) {
+        double rm = 0.0;
+        for (int w = 0; w < kWavesPerBlock; ++w) rm = fmax(rm, red[0][w]);
         centers[4 * blockIdx.x + 0] = cx;
         centers[4 * blockIdx.x + 1] = cy;
         centers[4 * blockIdx.x + 2] = cz;
-        centers[4 * blockIdx.x + 3] = 0.0;
+        centers[4 * blockIdx.x + 3] = rm;
     }
     out[j] = v ? make_float4(float(x - cx), float(y - cy), float(z - cz), float(g[j])) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -137,6 +154,15 @@ __global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, co
     float ox[TPL], oy[TPL], oz[TPL];
     float Sx[TPL], Sy[TPL], Sz[TPL], Cx[TPL], Cy[TPL], Cz[TPL], Sp[TPL], Cp[TPL];
     int sk[TPL];
+    double rw = 0.0;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int64_t i = base + k * 64 + lane;
+        if (i < nt)
+            rw = fmax(rw, sqrt((px[k] - cwx) * (px[k] - cwx) + (py[k] - cwy) * (py[k] - cwy) +
+                               (pz[k] - cwz) * (pz[k] - cwz)));
+    }
+    rw = wave_max(rw);
 #pragma unroll
     for (int k = 0; k < TPL; ++k) {
         ox[k] = float(px[k] - cwx); oy[k] = float(py[k] - cwy); oz[k] = float(pz[k] - cwz);
@@ -162,25 +188,51 @@ __global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, co
         }
         const float4* s = src + int64_t(tile) * kTile;
         const int jbase = tile * kTile;
+        const double ex = cwx - scen[4 * tile + 0], ey = cwy - scen[4 * tile + 1], ez = cwz - scen[4 * tile + 2];
+        const bool near = sqrt(ex * ex + ey * ey + ez * ez) <= rw + scen[4 * tile + 3] + kNearPad;
+        if (!near) {
 #pragma unroll 8
-        for (int j = 0; j < kTile; ++j) {
-            const float4 q = s[j];
+            for (int j = 0; j < kTile; ++j) {
+                const float4 q = s[j];
 #pragma unroll
-            for (int k = 0; k < TPL; ++k) {
-                const float dx = qx[k] - q.x;
-                const float dy = qy[k] - q.y;
-                const float dz = qz[k] - q.z;
-                if (VEL) {
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
-                    const float w = q.w * __builtin_amdgcn_rcpf(r2);
-                    ax[k] = fmaf(w, dx, ax[k]);
-                    ay[k] = fmaf(w, dy, ay[k]);
-                    az[k] = fmaf(w, dz, az[k]);
+                for (int k = 0; k < TPL; ++k) {
+                    const float dx = qx[k] - q.x;
+                    const float dy = qy[k] - q.y;
+                    const float dz = qz[k] - q.z;
+                    if (VEL) {
+                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
+                        const float w = q.w * __builtin_amdgcn_rcpf(r2);
+                        ax[k] = fmaf(w, dx, ax[k]);
+                        ay[k] = fmaf(w, dy, ay[k]);
+                        az[k] = fmaf(w, dz, az[k]);
+                    }
+                    if (PSI) {
+                        const float r2p = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const float l = q.w * __builtin_amdgcn_logf(r2p);
+                        ap[k] += (jbase + j == sk[k]) ? 0.f : l;
+                    }
                 }
-                if (PSI) {
-                    const float r2p = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const float l = q.w * __builtin_amdgcn_logf(r2p);
-                    ap[k] += (jbase + j == sk[k]) ? 0.f : l;
+            }
+        } else {
+            for (int j = 0; j < kTile; ++j) {
+                const float4 q = s[j];
+#pragma unroll
+                for (int k = 0; k < TPL; ++k) {
+                    const float dx = qx[k] - q.x;
+                    const float dy = qy[k] - q.y;
+                    const float dz = qz[k] - q.z;
+                    if (VEL) {
+                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
+                        const float w = (r2 > kZeroTol2) ? q.w * __builtin_amdgcn_rcpf(r2) : 0.f;
+                        ax[k] = fmaf(w, dx, ax[k]);
+                        ay[k] = fmaf(w, dy, ay[k]);
+                        az[k] = fmaf(w, dz, az[k]);
+                    }
+                    if (PSI) {
+                        const float r2p = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const float l = q.w * __builtin_amdgcn_logf(r2p);
+                        ap[k] += (jbase + j == sk[k]) ? 0.f : l;
+                    }
                 }
             }
         }
@@ -371,7 +423,7 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         double* scen = reinterpret_cast<double*>(src + nsp);
         hipLaunchKernelGGL(ds_prep_src_f32c, dim3(ntiles), dim3(256), 0, st, sx, sy, sz, gamma, ns, src, scen);
         if (pot) hipLaunchKernelGGL(ds_gamma_total, dim3(1), dim3(1024), 0, st, gamma, ns, gtot);
-        const float eps2v = float(2.0 * delta * delta) + kTiny;
+        const float eps2v = float(2.0 * delta * delta);
         const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
         if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
         else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);

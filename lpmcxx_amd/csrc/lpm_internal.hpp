@@ -62,5 +62,7 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
                       double* uy, double* uz, double* psi);
 
 void tree_free(Tree* t);
+int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n,
+                      const double* box_min, const double* box_max, int control, int param);
 
 }  // namespace lpm

@@ -1,0 +1,41 @@
+// Device octree over the sources of the treecode (shared by octree.hip and
+// treecode.hip).
+#pragma once
+
+#include "lpm_internal.hpp"
+
+namespace lpm {
+
+constexpr int kMaxDepth = LPM_TREE_MAX_DEPTH;  // 21 levels x 3 bits = 63-bit keys
+constexpr int kMaxOrder = 16;                  // series order limit (|k| < p)
+
+struct Tree {
+    int64_t ns = 0;       // sources
+    int64_t nn = 0;       // nodes (BFS order; children of a node are consecutive)
+    int64_t nleaves = 0;
+    int depth = 0;
+    int leaf_max = 0;
+    double root_box[6] = {0, 0, 0, 0, 0, 0};  // xmin ymin zmin xmax ymax zmax
+    // source data in tree order (device)
+    DevBuf perm;      // int32 [ns]  tree position -> source index
+    DevBuf spos;      // double4 [ns] (x, y, z, gamma) in tree order (gamma set by moments)
+    DevBuf keys;      // uint64 [ns] Morton keys in tree order
+    // node table (device)
+    DevBuf nbeg, nend, nfirst, nkids, nlevel, nparent;  // int32 [cap]
+    DevBuf nprefix;   // uint64 [cap] key prefix (top 3*level bits)
+    DevBuf ncent;     // double4 [nn] centroid + maxRadius
+    DevBuf nbox;      // double [nn][6] lo3 hi3
+    int64_t cap = 0;
+    // moments
+    int order = 0;        // p (|k| < p), 0 = none
+    int moment_sign = 1;
+    DevBuf mom;           // double [nn][nterms][3]
+    DevBuf mom32;         // float  [nn][nterms][4] (FP32 copy for the FP32C evaluation)
+    DevBuf soff32;        // float4 [ns] FP32 source offsets from their leaf centroid + gamma
+    // scratch
+    DevBuf tmp0, tmp1, tmp2, tmp3, sort_tmp;
+};
+
+inline int nterms(int p) { return p * (p + 1) * (p + 2) / 6; }
+
+}  // namespace lpm

@@ -1,0 +1,619 @@
+// Treecode moments and evaluation on gfx950 (TreeSum, src/LpmTreeSum.cpp).
+//
+// K7 tree_moments  per node, for every particle of the node (no
+//                  moment-to-moment shifts, as in the reference):
+//                  m_k[c] += (gamma_j y_j[c]) h^k, |k| < p, lex order of k,
+//                  h = moment_sign * (y_j - centroid).  The reference uses
+//                  h = centroid - y (moment_sign = -1), which flips the sign of
+//                  every odd-order term relative to the coefficients; +1 is the
+//                  consistent expansion.  Work item = (node, chunk of 128
+//                  particles); one thread per multi-index with the chunk's
+//                  powers h^a staged in LDS (particle-major, conflict free);
+//                  chunk partials reduced per node in chunk order
+//                  (deterministic).
+// K8-K11 tree_eval one wave per 64 Morton-sorted targets, depth-first over the
+//                  node table with a wave-shared stack of (node, lane mask):
+//                  every lane evaluates the reference MAC
+//                  maxRadius / |t - c| < theta for itself; accepting lanes run
+//                  the far field, the remaining lanes descend, and at a leaf
+//                  they sum the kernel over the leaf's particles.  Each lane
+//                  therefore visits exactly the nodes the reference recursion
+//                  visits for that target, in the same order, and the MAC
+//                  count is exact.
+//                  Far field: coefficients of 1/(1 + delta^2 - t.c - t.h)
+//                  a_k = |k|!/k! t^k / D^{|k|+1} by the reference's recurrence
+//                  (each a_k from its last lex-order writer: k - e3, else
+//                  k - e2, else k - e1), streamed with three running values so
+//                  no coefficient array is stored, contracted with the node
+//                  moments (wave-uniform scalar loads) as S = sum_k a_k m_k,
+//                  then t x S.  FP64 throughout.
+//                  Near field: LPM_F64 uses the reference's pair formula;
+//                  LPM_F32C uses FP32 offsets from the leaf centroid,
+//                  |d|^2/2 denominators and an FP32 per-leaf sum.
+#include "lpm_tree.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+namespace lpm {
+
+namespace {
+
+constexpr int kChunk = 128;
+constexpr int kStack = 160;   // >= 7 * kMaxDepth + 1
+__constant__ double c_rinv[kMaxOrder + 1] = {0.0,       1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
+                                             1.0 / 6,   1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
+                                             1.0 / 12,  1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16};
+
+inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
+
+// Lex index of the multi-index (a, b, c), |k| < P: terms with first index < a,
+// then with first index a and second < b, then c.
+__host__ __device__ constexpr int lex_index(int P, int a, int b, int c) {
+    int idx = 0;
+    for (int i = 0; i < a; ++i) idx += (P - i) * (P - i + 1) / 2;
+    for (int j = 0; j < b; ++j) idx += P - a - j;
+    return idx + c;
+}
+
+// Cartesian far field: with d = t - c and R2 = |d|^2 + 2 delta^2 the
+// coefficients of 2 / (|d - h|^2 + 2 delta^2) = sum_k b_k h^k satisfy
+// b_0 = 2/R2 and b_k = (2 sum_i d_i b_{k-e_i} - sum_i b_{k-2e_i}) / R2.
+// Computed degree shell by degree shell (only the last two shells are live)
+// and contracted with the node moments m_k = sum_j gamma_j y_j h_j^k:
+// S = sum_k b_k m_k.  Converges for maxRadius / |d| < 1.
+// Graded (degree-by-degree) visiting order of the multi-indices |k| < P with
+// the lex positions of each term and of its k - e_i and k - 2 e_i neighbours,
+// built at compile time so every coefficient lives in a named register.
+template <int P>
+struct CartTab {
+    static constexpr int NT = P * (P + 1) * (P + 2) / 6;
+    int id[NT] = {};
+    int p1[NT][3] = {};
+    int p2[NT][3] = {};
+    constexpr CartTab() {
+        int pos = 0;
+        for (int n = 0; n < P; ++n)
+            for (int k1 = n; k1 >= 0; --k1)
+                for (int k2 = n - k1; k2 >= 0; --k2) {
+                    const int k3 = n - k1 - k2;
+                    id[pos] = lex_index(P, k1, k2, k3);
+                    p1[pos][0] = k1 >= 1 ? lex_index(P, k1 - 1, k2, k3) : -1;
+                    p1[pos][1] = k2 >= 1 ? lex_index(P, k1, k2 - 1, k3) : -1;
+                    p1[pos][2] = k3 >= 1 ? lex_index(P, k1, k2, k3 - 1) : -1;
+                    p2[pos][0] = k1 >= 2 ? lex_index(P, k1 - 2, k2, k3) : -1;
+                    p2[pos][1] = k2 >= 2 ? lex_index(P, k1, k2 - 2, k3) : -1;
+                    p2[pos][2] = k3 >= 2 ? lex_index(P, k1, k2, k3 - 2) : -1;
+                    ++pos;
+                }
+    }
+};
+template <int P>
+inline constexpr CartTab<P> kCartTab{};
+
+template <int P, int I>
+__device__ __forceinline__ void cart_step(double* b, double e0, double e1, double e2, double iR2,
+                                          const double* __restrict__ mm, double& S0, double& S1, double& S2) {
+    constexpr int id = kCartTab<P>.id[I];
+    constexpr int a0 = kCartTab<P>.p1[I][0], a1 = kCartTab<P>.p1[I][1], a2 = kCartTab<P>.p1[I][2];
+    constexpr int c0 = kCartTab<P>.p2[I][0], c1 = kCartTab<P>.p2[I][1], c2 = kCartTab<P>.p2[I][2];
+    double s = 0.0;
+    if constexpr (a0 >= 0) s = fma(e0, b[a0], s);
+    if constexpr (a1 >= 0) s = fma(e1, b[a1], s);
+    if constexpr (a2 >= 0) s = fma(e2, b[a2], s);
+    if constexpr (c0 >= 0) s -= b[c0];
+    if constexpr (c1 >= 0) s -= b[c1];
+    if constexpr (c2 >= 0) s -= b[c2];
+    b[id] = s * iR2;
+    S0 = fma(b[id], mm[3 * id + 0], S0);
+    S1 = fma(b[id], mm[3 * id + 1], S1);
+    S2 = fma(b[id], mm[3 * id + 2], S2);
+}
+
+template <int P, int... I>
+__device__ __forceinline__ void cart_all(std::integer_sequence<int, I...>, double* b, double e0, double e1,
+                                         double e2, double iR2, const double* __restrict__ mm, double& S0,
+                                         double& S1, double& S2) {
+    (cart_step<P, I + 1>(b, e0, e1, e2, iR2, mm, S0, S1, S2), ...);
+}
+
+// Cartesian far field: with d = t - c and R2 = |d|^2 + 2 delta^2 the
+// coefficients of 2 / (|d - h|^2 + 2 delta^2) = sum_k b_k h^k satisfy
+// b_0 = 2/R2 and b_k = (2 sum_i d_i b_{k-e_i} - sum_i b_{k-2e_i}) / R2
+// (from Q grad F = -F grad Q for F = 1/Q).  Computed degree shell by degree
+// shell and contracted with the node moments m_k = sum_j gamma_j y_j h_j^k:
+// S = sum_k b_k m_k.  Converges for maxRadius / |d| < 1.
+template <int P>
+__device__ __forceinline__ void far_cartesian(double d0, double d1, double d2, double delta2x2,
+                                              const double* __restrict__ mm, double& S0, double& S1, double& S2) {
+    constexpr int NT = CartTab<P>::NT;
+    const double R2 = d0 * d0 + d1 * d1 + d2 * d2 + delta2x2;
+    const double iR2 = 1.0 / R2;
+    double b[NT];
+    b[0] = 2.0 * iR2;
+    S0 = b[0] * mm[0];
+    S1 = b[0] * mm[1];
+    S2 = b[0] * mm[2];
+    cart_all<P>(std::make_integer_sequence<int, NT - 1>{}, b, 2.0 * d0, 2.0 * d1, 2.0 * d2, iR2, mm, S0, S1, S2);
+}
+
+// Reference far field (SumNode::calc_coeffs, src/LpmTreeSum.cpp:149-171):
+// coefficients of 1/(1 + delta^2 - t.c - t.h) streamed in lex order.
+__device__ __forceinline__ void far_reference(int p, double t0, double t1, double t2, double cx, double cy,
+                                              double cz, double d2, const double* __restrict__ mm, double& S0,
+                                              double& S1, double& S2) {
+    const double D = 1.0 + d2 - (t0 * cx + t1 * cy + t2 * cz);
+    const double invD = 1.0 / D;
+    S0 = S1 = S2 = 0.0;
+    double A1 = invD;
+    int idx = 0;
+    for (int k1 = 0; k1 < p; ++k1) {
+        if (k1 > 0) A1 = ((double(k1) * invD) * t0) * A1 * c_rinv[k1];
+        double A2 = A1;
+        for (int k2 = 0; k2 < p - k1; ++k2) {
+            if (k2 > 0) A2 = ((double(k1 + k2) * invD) * t1) * A2 * c_rinv[k2];
+            double A3 = A2;
+            for (int k3 = 0; k3 < p - k1 - k2; ++k3) {
+                if (k3 > 0) A3 = ((double(k1 + k2 + k3) * invD) * t2) * A3 * c_rinv[k3];
+                S0 = fma(A3, mm[3 * idx + 0], S0);
+                S1 = fma(A3, mm[3 * idx + 1], S1);
+                S2 = fma(A3, mm[3 * idx + 2], S2);
+                ++idx;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- moments
+__global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict__ chunk_node,
+                                                          const int* __restrict__ chunk_beg,
+                                                          const int* __restrict__ chunk_end,
+                                                          const double4* __restrict__ ncent,
+                                                          const double4* __restrict__ spos, int p, int nterm,
+                                                          double sign, double* __restrict__ part) {
+    extern __shared__ double lds[];
+    double* pw = lds;                       // [kChunk][3][p] powers
+    double* gy = lds + kChunk * 3 * p;      // [kChunk][3]   gamma * y
+    const int ch = blockIdx.x;
+    const int node = chunk_node[ch], b = chunk_beg[ch], e = chunk_end[ch];
+    const int cnt = e - b;
+    const double4 c = ncent[node];
+    for (int q = threadIdx.x; q < kChunk; q += blockDim.x) {
+        double hx = 0.0, hy = 0.0, hz = 0.0, g = 0.0, yx = 0.0, yy = 0.0, yz = 0.0;
+        if (q < cnt) {
+            const double4 s = spos[b + q];
+            yx = s.x; yy = s.y; yz = s.z; g = s.w;
+            hx = sign * (s.x - c.x); hy = sign * (s.y - c.y); hz = sign * (s.z - c.z);
+        }
+        double* w = pw + q * 3 * p;
+        double ax = 1.0, ay = 1.0, az = 1.0;
+        for (int a = 0; a < p; ++a) {
+            w[a] = ax; w[p + a] = ay; w[2 * p + a] = az;
+            ax *= hx; ay *= hy; az *= hz;
+        }
+        gy[3 * q + 0] = g * yx;
+        gy[3 * q + 1] = g * yy;
+        gy[3 * q + 2] = g * yz;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
+        // decode lex index t -> (k1, k2, k3)
+        int r = t, k1 = 0, k2 = 0;
+        for (;;) {
+            const int m = p - k1;                 // terms with this k1: m(m+1)/2
+            const int blk = m * (m + 1) / 2;
+            if (r < blk) break;
+            r -= blk;
+            ++k1;
+        }
+        for (;;) {
+            const int m = p - k1 - k2;
+            if (r < m) break;
+            r -= m;
+            ++k2;
+        }
+        const int k3 = r;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            const double* w = pw + q * 3 * p;
+            const double P = w[k1] * w[p + k2] * w[2 * p + k3];
+            m0 += gy[3 * q + 0] * P;
+            m1 += gy[3 * q + 1] * P;
+            m2 += gy[3 * q + 2] * P;
+        }
+        double* o = part + (int64_t(ch) * nterm + t) * 3;
+        o[0] = m0; o[1] = m1; o[2] = m2;
+    }
+}
+
+__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, const int* __restrict__ node_chunk0,
+                                                           const int* __restrict__ node_nchunk,
+                                                           const double* __restrict__ part, double* __restrict__ mom,
+                                                           float* __restrict__ mom32) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(nn) * nterm * 3) return;
+    const int node = int(i / (int64_t(nterm) * 3));
+    const int rem = int(i - int64_t(node) * nterm * 3);
+    const int c0 = node_chunk0[node], nc = node_nchunk[node];
+    double s = 0.0;
+    for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * 3 + rem];
+    mom[i] = s;
+    mom32[i] = float(s);
+}
+
+__global__ __launch_bounds__(256) void tree_set_gamma(const double* __restrict__ gamma, const int* __restrict__ perm,
+                                                      int64_t n, double4* __restrict__ spos) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) spos[i].w = gamma[perm[i]];
+}
+
+// FP32 offsets of every particle from its leaf's centroid (FP32C near field).
+__global__ __launch_bounds__(256) void tree_leaf_offsets(int nn, const int* __restrict__ nkids,
+                                                         const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                         const double4* __restrict__ ncent,
+                                                         const double4* __restrict__ spos, float4* __restrict__ soff) {
+    const int i = blockIdx.x;
+    if (i >= nn || nkids[i] != 0) return;
+    const double4 c = ncent[i];
+    for (int q = nbeg[i] + threadIdx.x; q < nend[i]; q += blockDim.x) {
+        const double4 s = spos[q];
+        soff[q] = make_float4(float(s.x - c.x), float(s.y - c.y), float(s.z - c.z), float(s.w));
+    }
+}
+
+// ---------------------------------------------------------------- target ordering
+__global__ __launch_bounds__(256) void target_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, int64_t n, double x0, double y0,
+                                                   double z0, double x1, double y1, double z1,
+                                                   uint64_t* __restrict__ keys, int* __restrict__ vals) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // clamp into the root box: the order only groups nearby targets
+    const double px = fmin(fmax(x[i], x0), x1), py = fmin(fmax(y[i], y0), y1), pz = fmin(fmax(z[i], z0), z1);
+    double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
+    uint64_t key = 0;
+    for (int l = 0; l < kMaxDepth; ++l) {
+        const double mx = 0.5 * (xl + xh), my = 0.5 * (yl + yh), mz = 0.5 * (zl + zh);
+        const int bx = px >= mx, by = py >= my, bz = pz >= mz;
+        if (bx) xl = mx; else xh = mx;
+        if (by) yl = my; else yh = my;
+        if (bz) zl = mz; else zh = mz;
+        key = (key << 3) | uint64_t(bx | (by << 1) | (bz << 2));
+    }
+    keys[i] = key;
+    vals[i] = int(i);
+}
+
+// ---------------------------------------------------------------- evaluation
+__device__ inline uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+template <int P, bool F32NEAR>
+__global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, const double* __restrict__ ty,
+                                                 const double* __restrict__ tz, const int* __restrict__ tperm,
+                                                 int64_t nt, const double4* __restrict__ ncent,
+                                                 const int* __restrict__ nfirst, const int* __restrict__ nkids,
+                                                 const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                 const double4* __restrict__ spos, const float4* __restrict__ soff,
+                                                 const double* __restrict__ mom, int p, int nterm, double theta,
+                                                 double delta, double R2, double d2, float eps2v, double mult,
+                                                 double* __restrict__ ux, double* __restrict__ uy,
+                                                 double* __restrict__ uz, unsigned long long* __restrict__ mac_total) {
+    __shared__ int s_node[4][kStack];
+    __shared__ uint64_t s_mask[4][kStack];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t it = (int64_t(blockIdx.x) * 4 + w) * 64 + lane;
+    const bool valid = it < nt;
+    const int orig = valid ? tperm[it] : 0;
+    const double t0 = valid ? tx[orig] : 0.0, t1 = valid ? ty[orig] : 0.0, t2 = valid ? tz[orig] : 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    unsigned mac = 0;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    int sp = 0;
+    s_node[w][0] = 0;
+    s_mask[w][0] = vmask;
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
+        const uint64_t m = uniform64(s_mask[w][sp]);
+        const bool act = (m >> lane) & 1ull;
+        const double4 c = ncent[n];
+        bool acc = false;
+        if (act) {
+            const double dist = sqrt((t0 - c.x) * (t0 - c.x) + (t1 - c.y) * (t1 - c.y) + (t2 - c.z) * (t2 - c.z));
+            acc = c.w / dist < theta;
+        }
+        const uint64_t am = __ballot(acc);
+        if (am) {
+            if (acc) {
+                const double* mm = mom + int64_t(n) * nterm * 3;
+                double S0, S1, S2;
+                if (P == 0) far_reference(p, t0, t1, t2, c.x, c.y, c.z, d2, mm, S0, S1, S2);
+                else far_cartesian<(P > 0 ? P : 1)>(t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2, mm, S0, S1, S2);
+                s0 += t1 * S2 - t2 * S1;
+                s1 += t2 * S0 - t0 * S2;
+                s2 += t0 * S1 - t1 * S0;
+                ++mac;
+            }
+        }
+        const uint64_t rest = m & ~am;
+        if (rest == 0) continue;
+        const int nk = nkids[n];
+        if (nk == 0) {
+            const bool mine = (rest >> lane) & 1ull;
+            const int b = nbeg[n], e = nend[n];
+            if (F32NEAR) {
+                const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+                if (mine) {
+                    for (int j = b; j < e; ++j) {
+                        const float4 o = soff[j];
+                        const float dx = q0 - o.x, dy = q1 - o.y, dz = q2 - o.z;
+                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
+                        // the reference's |den| <= 1e-13 rule: |d|^2 + 2 delta^2 <= 2e-13 contributes 0
+                        const float wgt = (r2 > 2e-13f) ? o.w * __builtin_amdgcn_rcpf(r2) : 0.f;
+                        a0 = fmaf(wgt, dx, a0);
+                        a1 = fmaf(wgt, dy, a1);
+                        a2 = fmaf(wgt, dz, a2);
+                    }
+                    // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2)
+                    const double b0 = a0, b1 = a1, b2 = a2;
+                    s0 -= 2.0 * (t1 * b2 - t2 * b1);
+                    s1 -= 2.0 * (t2 * b0 - t0 * b2);
+                    s2 -= 2.0 * (t0 * b1 - t1 * b0);
+                }
+            } else if (mine) {
+                for (int j = b; j < e; ++j) {
+                    const double4 s = spos[j];
+                    const double cx = t1 * s.z - t2 * s.y;
+                    const double cy = t2 * s.x - t0 * s.z;
+                    const double cz = t0 * s.y - t1 * s.x;
+                    const double den = R2 - (t0 * s.x + t1 * s.y + t2 * s.z) + d2;
+                    if (fabs(den) > 1.0e-13) {
+                        const double inv = 1.0 / den;
+                        s0 += (cx * inv) * s.w;
+                        s1 += (cy * inv) * s.w;
+                        s2 += (cz * inv) * s.w;
+                    }
+                }
+            }
+        } else {
+            const int f = nfirst[n];
+            for (int q = nk - 1; q >= 0; --q) {
+                s_node[w][sp] = f + q;
+                s_mask[w][sp] = rest;
+                ++sp;
+            }
+        }
+    }
+    if (valid) {
+        ux[orig] = s0 * mult;
+        uy[orig] = s1 * mult;
+        uz[orig] = s2 * mult;
+    }
+    // wave-reduce the acceptance count, one atomic per wave
+    unsigned long long tot = mac;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+    if (lane == 0) atomicAdd(mac_total, tot);
+}
+
+}  // namespace
+
+int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
+    Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    const int nterm = nterms(p);
+    const int64_t nn = t->nn;
+    // chunk list from the node ranges
+    std::vector<int> hb(nn), he(nn);
+    LPM_HIP(c, hipMemcpyAsync(hb.data(), t->nbeg.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(he.data(), t->nend.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    std::vector<int> cn, cb, ce, n0(nn), nc(nn);
+    for (int64_t i = 0; i < nn; ++i) {
+        n0[i] = int(cn.size());
+        for (int b = hb[i]; b < he[i]; b += kChunk) {
+            cn.push_back(int(i));
+            cb.push_back(b);
+            ce.push_back(std::min(he[i], b + kChunk));
+        }
+        nc[i] = int(cn.size()) - n0[i];
+    }
+    const int64_t nch = int64_t(cn.size());
+    LPM_CHECK(t->tmp3.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
+    int* d_cn = t->tmp3.as<int>();
+    int* d_cb = d_cn + nch;
+    int* d_ce = d_cb + nch;
+    int* d_n0 = d_ce + nch;
+    int* d_nc = d_n0 + nn;
+    if (nch > 0) {
+        LPM_HIP(c, hipMemcpyAsync(d_cn, cn.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
+        LPM_HIP(c, hipMemcpyAsync(d_cb, cb.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
+        LPM_HIP(c, hipMemcpyAsync(d_ce, ce.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    LPM_HIP(c, hipMemcpyAsync(d_n0, n0.data(), nn * sizeof(int), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d_nc, nc.data(), nn * sizeof(int), hipMemcpyHostToDevice, st));
+    LPM_CHECK(t->tmp2.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * 3 * sizeof(double)));
+    LPM_CHECK(t->mom.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
+    LPM_CHECK(t->mom32.reserve(size_t(nn) * nterm * 3 * sizeof(float)));
+    LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
+    if (t->ns > 0) {
+        hipLaunchKernelGGL(tree_set_gamma, dim3(blocks(t->ns)), dim3(256), 0, st, gamma, t->perm.as<int>(), t->ns,
+                           t->spos.as<double4>());
+    }
+    if (nch > 0) {
+        const size_t lds = size_t(kChunk) * (3 * p + 3) * sizeof(double);
+        hipLaunchKernelGGL(tree_moments_chunk, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
+                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->tmp2.as<double>());
+    }
+    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), nterm, d_n0, d_nc,
+                       t->tmp2.as<double>(), t->mom.as<double>(), t->mom32.as<float>());
+    hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+                       t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
+                       t->soff32.as<float4>());
+    LPM_HIP(c, hipGetLastError());
+    t->order = p;
+    t->moment_sign = sign;
+    return LPM_OK;
+}
+
+int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
+                     const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
+                     int64_t* mac_out, double radius) {
+    Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    if (nt == 0) { if (mac_out) *mac_out = 0; return LPM_OK; }
+    const int64_t na = nt;
+    LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + 64));
+    uint64_t* k_in = c->ds_tgt.as<uint64_t>();
+    uint64_t* k_out = k_in + na;
+    int* v_in = reinterpret_cast<int*>(k_out + na);
+    int* v_out = v_in + na;
+    unsigned long long* d_mac = reinterpret_cast<unsigned long long*>(v_out + na);
+    LPM_HIP(c, hipMemsetAsync(d_mac, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(target_keys, dim3(blocks(nt)), dim3(256), 0, st, tx, ty, tz, nt, t->root_box[0], t->root_box[1],
+                       t->root_box[2], t->root_box[3], t->root_box[4], t->root_box[5], k_in, v_in);
+    size_t bytes = 0;
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0, 3 * kMaxDepth,
+                                                  st));
+    LPM_CHECK(t->sort_tmp.reserve(bytes + 256));
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0,
+                                                  3 * kMaxDepth, st));
+    const double pi = 3.1415926535897932384626433832795027975;
+    const double mult = -1.0 / (4.0 * pi * radius);
+    const float eps2v = float(2.0 * delta * delta);
+    const int p = t->order, nterm = nterms(p);
+    const unsigned nb = unsigned((nt + 255) / 256);
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tx, ty, tz, v_out, nt, t->ncent.as<double4>(),
+                           t->nfirst.as<int>(), t->nkids.as<int>(), t->nbeg.as<int>(), t->nend.as<int>(),
+                           t->spos.as<double4>(), t->soff32.as<float4>(), t->mom.as<double>(), p, nterm, theta, delta,
+                           radius * radius, delta * delta, eps2v, mult, ux, uy, uz, d_mac);
+    };
+    if (series == LPM_SERIES_REFERENCE) {
+        if (prec == LPM_F32C) args(tree_eval<0, true>);
+        else args(tree_eval<0, false>);
+    } else {
+        switch (p) {
+#define LPM_CASE(PP)                                      \
+    case PP:                                              \
+        if (prec == LPM_F32C) args(tree_eval<PP, true>);  \
+        else args(tree_eval<PP, false>);                  \
+        break;
+            LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
+            LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
+#undef LPM_CASE
+            default: return fail(c, LPM_ERR_INVALID_ARG, "series order %d unsupported", p);
+        }
+    }
+    LPM_HIP(c, hipGetLastError());
+    if (mac_out) {
+        unsigned long long h = 0;
+        LPM_HIP(c, hipMemcpyAsync(&h, d_mac, sizeof(h), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipStreamSynchronize(st));
+        *mac_out = int64_t(h);
+    }
+    return LPM_OK;
+}
+
+}  // namespace lpm
+
+using namespace lpm;
+
+extern "C" {
+
+static int check_moments(lpm_ctx* c, const double* gamma, int p, int sign) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (!c->tree) return fail(c, LPM_ERR_STATE, "lpm_tree_moments before lpm_tree_build");
+    if (p < 1 || p > 12) return fail(c, LPM_ERR_INVALID_ARG, "series order must be in [1, 12]");
+    if (sign != 1 && sign != -1) return fail(c, LPM_ERR_INVALID_ARG, "moment_sign must be +1 or -1");
+    if (c->tree->ns > 0 && !gamma) return fail(c, LPM_ERR_NULL_PTR, "gamma is NULL");
+    return LPM_OK;
+}
+
+int lpm_tree_moments_dev(lpm_ctx* c, const double* gamma, int p, int sign) {
+    LPM_CHECK(check_moments(c, gamma, p, sign));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_moments_device(c, gamma, p, sign);
+}
+
+int lpm_tree_moments(lpm_ctx* c, const double* gamma, int p, int sign) {
+    LPM_CHECK(check_moments(c, gamma, p, sign));
+    LPM_HIP(c, hipSetDevice(c->device));
+    const int64_t n = std::max<int64_t>(c->tree->ns, 1);
+    LPM_CHECK(c->h_out.reserve(n * sizeof(double)));
+    if (c->tree->ns > 0)
+        LPM_HIP(c, hipMemcpyAsync(c->h_out.ptr, gamma, c->tree->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    LPM_CHECK(tree_moments_device(c, c->h_out.as<double>(), p, sign));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_tree_export_moments(lpm_ctx* c, double* out) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    if (!c->tree || c->tree->order == 0) return fail(c, LPM_ERR_STATE, "no moments computed");
+    if (!out) return fail(c, LPM_ERR_NULL_PTR, "output is NULL");
+    LPM_HIP(c, hipSetDevice(c->device));
+    const size_t n = size_t(c->tree->nn) * nterms(c->tree->order) * 3;
+    LPM_HIP(c, hipMemcpyAsync(out, c->tree->mom.ptr, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+static int check_eval(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
+                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (!c->tree) return fail(c, LPM_ERR_STATE, "lpm_tree_eval before lpm_tree_build");
+    if (c->tree->order == 0) return fail(c, LPM_ERR_STATE, "lpm_tree_eval before lpm_tree_moments");
+    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
+    if (series != LPM_SERIES_REFERENCE && series != LPM_SERIES_CARTESIAN)
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid series %d", series);
+    if (series == LPM_SERIES_CARTESIAN && c->tree->moment_sign != 1)
+        return fail(c, LPM_ERR_INVALID_ARG, "the Cartesian series needs moments with moment_sign = +1");
+    if (nt < 0 || nt > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid target count");
+    if (!(delta >= 0.0) || !std::isfinite(theta)) return fail(c, LPM_ERR_INVALID_ARG, "invalid theta/delta");
+    if (nt > 0 && (!tx || !ty || !tz || !ux || !uy || !uz)) return fail(c, LPM_ERR_NULL_PTR, "NULL target/output");
+    return LPM_OK;
+}
+
+int lpm_tree_eval_dev(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
+                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
+                      int64_t* mac_count) {
+    LPM_CHECK(check_eval(c, prec, series, theta, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_eval_device(c, prec, series, theta, delta, tx, ty, tz, nt, ux, uy, uz, mac_count, 1.0);
+}
+
+int lpm_tree_eval(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx, const double* ty,
+                  const double* tz, int64_t nt, double* ux, double* uy, double* uz, int64_t* mac_count) {
+    LPM_CHECK(check_eval(c, prec, series, theta, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    if (nt == 0) { if (mac_count) *mac_count = 0; return LPM_OK; }
+    LPM_CHECK(c->h_in.reserve(6 * size_t(nt) * sizeof(double)));
+    double* d = c->h_in.as<double>();
+    hipStream_t st = c->stream;
+    LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    int64_t mac = 0;
+    LPM_CHECK(tree_eval_device(c, prec, series, theta, delta, d, d + nt, d + 2 * nt, nt, d + 3 * nt, d + 4 * nt,
+                               d + 5 * nt, &mac, 1.0));
+    LPM_HIP(c, hipMemcpyAsync(ux, d + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(uy, d + 4 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(uz, d + 5 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    if (mac_count) *mac_count = mac;
+    return LPM_OK;
+}
+
+}  // extern "C"

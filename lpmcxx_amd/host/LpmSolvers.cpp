@@ -1,0 +1,445 @@
+// Implementation of the host API in LpmSolvers.hpp.
+#include "LpmSolvers.hpp"
+
+#include <algorithm>
+#include <fstream>
+#include <limits>
+#include <new>
+#include <sstream>
+
+namespace Lpm {
+
+// ---------------------------------------------------------------- Box3d
+void Box3d::finish() {
+    centroid = XyzVector(0.5 * (xmax + xmin), 0.5 * (ymax + ymin), 0.5 * (zmax + zmin));
+    scalar_type r = 0.0;
+    for (const XyzVector& c : corners()) {
+        const scalar_type t = distance(centroid, c);
+        if (t > r) r = t;
+    }
+    maxRadius = r;
+    const XyzVector fc[6] = {XyzVector(0.5 * (xmin + xmax), ymin, 0.5 * (zmin + zmax)),
+                             XyzVector(xmax, 0.5 * (ymin + ymax), 0.5 * (zmin + zmax)),
+                             XyzVector(0.5 * (xmin + xmax), ymax, 0.5 * (zmin + zmax)),
+                             XyzVector(xmin, 0.5 * (ymin + ymax), 0.5 * (zmin + zmax)),
+                             XyzVector(0.5 * (xmin + xmax), 0.5 * (ymin + ymax), zmin),
+                             XyzVector(0.5 * (xmin + xmax), 0.5 * (ymin + ymax), zmax)};
+    scalar_type m = std::numeric_limits<scalar_type>::max();
+    for (const XyzVector& c : fc) {
+        const scalar_type t = distance(centroid, c);
+        if (t < m) m = t;
+    }
+    minRadius = m;
+}
+
+// corner order (xmin,ymin,zmin), (xmin,ymax,zmin), (xmin,ymin,zmax), ... (src/LpmBox3d.cpp:55-66)
+std::vector<XyzVector> Box3d::corners() const {
+    return {XyzVector(xmin, ymin, zmin), XyzVector(xmin, ymax, zmin), XyzVector(xmin, ymin, zmax),
+            XyzVector(xmin, ymax, zmax), XyzVector(xmax, ymin, zmin), XyzVector(xmax, ymax, zmin),
+            XyzVector(xmax, ymin, zmax), XyzVector(xmax, ymax, zmax)};
+}
+
+// children c = ix + 2 iy + 4 iz, planes 0.5*(min+max) (src/LpmBox3d.cpp:140-151)
+std::vector<Box3d> Box3d::bisectAll() const {
+    const scalar_type mx = 0.5 * (xmin + xmax), my = 0.5 * (ymin + ymax), mz = 0.5 * (zmin + zmax);
+    std::vector<Box3d> r;
+    r.reserve(8);
+    for (int c = 0; c < 8; ++c)
+        r.emplace_back((c & 1) ? mx : xmin, (c & 1) ? xmax : mx, (c & 2) ? my : ymin, (c & 2) ? ymax : my,
+                       (c & 4) ? mz : zmin, (c & 4) ? zmax : mz);
+    return r;
+}
+
+scalar_type Box3d::longestEdge() const { return std::max(xmax - xmin, std::max(ymax - ymin, zmax - zmin)); }
+
+std::string Box3d::infoString() const {
+    std::stringstream ss;
+    ss << "x in [" << xmin << ", " << xmax << "], y in [" << ymin << ", " << ymax << "], z in [" << zmin << ", "
+       << zmax << "]" << std::endl;
+    return ss.str();
+}
+
+// ---------------------------------------------------------------- MPIReplicatedData
+// start = r floor(n/P), end = (r+1) floor(n/P) - 1, the last rank ends at n-1
+// (src/LpmMPIReplicatedData.cpp:12-23).
+MPIReplicatedData::MPIReplicatedData(const index_type nItems, const int rank, const int nProcs)
+    : _nItems(nItems), _nProcs(nProcs), _rank(rank), _start(nProcs), _end(nProcs), _size(nProcs) {
+    if (nProcs < 1 || rank < 0 || rank >= nProcs || nItems < 0)
+        throw std::invalid_argument("MPIReplicatedData: bad rank/size");
+    const index_type chunk = nItems / nProcs;
+    for (int i = 0; i < nProcs; ++i) {
+        _start[i] = i * chunk;
+        _end[i] = (i + 1) * chunk - 1;
+    }
+    _end[nProcs - 1] = nItems - 1;
+    for (int i = 0; i < nProcs; ++i) _size[i] = _end[i] - _start[i] + 1;
+}
+
+// ---------------------------------------------------------------- Coords
+scalar_type Coords::minX() const { return *std::min_element(x.begin(), x.end()); }
+scalar_type Coords::maxX() const { return *std::max_element(x.begin(), x.end()); }
+scalar_type Coords::minY() const { return *std::min_element(y.begin(), y.end()); }
+scalar_type Coords::maxY() const { return *std::max_element(y.begin(), y.end()); }
+scalar_type Coords::minZ() const { return *std::min_element(z.begin(), z.end()); }
+scalar_type Coords::maxZ() const { return *std::max_element(z.begin(), z.end()); }
+scalar_type Coords::maxR() const {
+    scalar_type r = 0.0;
+    for (index_type i = 0; i < n(); ++i) {
+        const scalar_type t = square(x[i]) + square(y[i]) + square(z[i]);
+        if (t > r) r = t;
+    }
+    return std::sqrt(r);
+}
+
+// ---------------------------------------------------------------- GpuContext
+GpuContext::GpuContext(int device) {
+    const int st = lpm_create(&_ctx, device);
+    if (st != LPM_OK) throw std::runtime_error(std::string("lpm_create: ") + lpm_status_string(st));
+}
+
+GpuContext::~GpuContext() { lpm_destroy(_ctx); }
+
+void GpuContext::check(int st, const char* where) const {
+    if (st == LPM_OK) return;
+    const std::string msg = std::string(where) + ": " + lpm_status_string(st) + ": " + lpm_last_error(_ctx);
+    switch (st) {
+        case LPM_ERR_ALLOC: throw std::bad_alloc();
+        case LPM_ERR_KERNEL:
+        case LPM_ERR_STATE: throw std::logic_error(msg);
+        case LPM_ERR_INVALID_ARG:
+        case LPM_ERR_NULL_PTR: throw std::invalid_argument(msg);
+        default: throw std::runtime_error(msg);
+    }
+}
+
+GpuContext& GpuContext::threadDefault() {
+    thread_local std::unique_ptr<GpuContext> ctx;
+    if (!ctx) ctx.reset(new GpuContext(0));
+    return *ctx;
+}
+
+// ---------------------------------------------------------------- Tree
+Tree::Tree(const std::shared_ptr<Coords> crds, const scalar_type, const int) : _crds(crds) {
+    _gpu = &GpuContext::threadDefault();
+    _nnodes = 1;
+}
+
+void Tree::buildTree(const TREE_DEPTH_CONTROL type, const index_type param, const bool do_shrink) {
+    if (do_shrink) throw std::invalid_argument("Tree::buildTree: do_shrink is not supported");
+    auto c = _crds.lock();
+    if (!c) throw std::logic_error("Tree::buildTree: coordinates expired");
+    // root box padded relatively by BOX_PADDING_FACTOR (src/LpmOctree.cpp:20-26)
+    double lo[3] = {c->minX() - BOX_PADDING_FACTOR * c->minX(), c->minY() - BOX_PADDING_FACTOR * c->minY(),
+                    c->minZ() - BOX_PADDING_FACTOR * c->minZ()};
+    double hi[3] = {c->maxX() + BOX_PADDING_FACTOR * c->maxX(), c->maxY() + BOX_PADDING_FACTOR * c->maxY(),
+                    c->maxZ() + BOX_PADDING_FACTOR * c->maxZ()};
+    int64_t nn = 0;
+    int depth = 0;
+    _gpu->check(lpm_tree_build_box(_gpu->get(), c->x.data(), c->y.data(), c->z.data(), c->n(), lo, hi,
+                                   type == MAX_COORDS_PER_NODE ? LPM_TREE_MAX_PER_NODE : LPM_TREE_MAX_DEPTH_CONTROL,
+                                   int(param), &nn, &depth),
+                "Tree::buildTree");
+    _nnodes = nn;
+    _depth = depth;
+    importFromGpu(c->n());
+}
+
+void Tree::importFromGpu(index_type ns) {
+    int64_t nn = 0;
+    int depth = 0;
+    int64_t nl = 0;
+    _gpu->check(lpm_tree_info(_gpu->get(), &nn, &depth, &nl), "Tree::importFromGpu");
+    std::vector<double> box(6 * nn), cent(4 * nn);
+    std::vector<int32_t> level(nn), parent(nn), first(nn), nk(nn), range(2 * nn), perm(std::max<index_type>(ns, 1));
+    _gpu->check(lpm_tree_export(_gpu->get(), box.data(), cent.data(), level.data(), parent.data(), first.data(),
+                                nk.data(), range.data(), perm.data()),
+                "Tree::importFromGpu");
+    // depth-first ids, children in octant order
+    std::vector<index_type> order, dfs_id(nn);
+    std::vector<index_type> stack = {0};
+    while (!stack.empty()) {
+        const index_type i = stack.back();
+        stack.pop_back();
+        dfs_id[i] = index_type(order.size());
+        order.push_back(i);
+        for (int k = nk[i] - 1; k >= 0; --k) stack.push_back(first[i] + k);
+    }
+    _nodes.assign(nn, Node());
+    for (index_type d = 0; d < nn; ++d) {
+        const index_type i = order[d];
+        Node& n = _nodes[d];
+        n.box = Box3d(box[6 * i + 0], box[6 * i + 3], box[6 * i + 1], box[6 * i + 4], box[6 * i + 2], box[6 * i + 5]);
+        n.level = level[i];
+        n.parent = parent[i] < 0 ? -1 : dfs_id[parent[i]];
+        for (int k = 0; k < nk[i]; ++k) n.kids.push_back(dfs_id[first[i] + k]);
+        n.coordsContained.assign(perm.begin() + range[2 * i], perm.begin() + range[2 * i + 1]);
+        std::sort(n.coordsContained.begin(), n.coordsContained.end());
+    }
+    _nnodes = nn;
+    _depth = depth;
+}
+
+std::string Tree::infoString() const {
+    std::stringstream ss;
+    ss << "Tree info:" << std::endl << "\tnNodes = " << _nnodes << std::endl << "\tdepth = " << _depth << std::endl;
+    return ss.str();
+}
+
+void Tree::writeToVtk(const std::string& filename, const std::string& desc) const {
+    std::ofstream fs(filename);
+    if (!fs.is_open()) throw std::ios_base::failure("file write error");
+    fs << "# vtk DataFile Version 2.0" << std::endl << desc << std::endl << "ASCII" << std::endl;
+    fs << "DATASET UNSTRUCTURED_GRID" << std::endl;
+    fs << "POINTS " << 8 * _nnodes << " double" << std::endl;
+    for (const Node& n : _nodes) {
+        const Box3d& b = n.box;
+        fs << b.xmin << " " << b.ymin << " " << b.zmin << "\n" << b.xmax << " " << b.ymin << " " << b.zmin << "\n"
+           << b.xmax << " " << b.ymax << " " << b.zmin << "\n" << b.xmin << " " << b.ymax << " " << b.zmin << "\n"
+           << b.xmin << " " << b.ymin << " " << b.zmax << "\n" << b.xmax << " " << b.ymin << " " << b.zmax << "\n"
+           << b.xmax << " " << b.ymax << " " << b.zmax << "\n" << b.xmin << " " << b.ymax << " " << b.zmax << "\n";
+    }
+    fs << "CELLS " << _nnodes << " " << 9 * _nnodes << std::endl;
+    for (index_type i = 0; i < _nnodes; ++i) {
+        fs << 8 << " ";
+        for (int k = 0; k < 8; ++k) fs << 8 * i + k << " ";
+        fs << "\n";
+    }
+    fs << "CELL_TYPES " << _nnodes << std::endl;
+    for (index_type i = 0; i < _nnodes; ++i) fs << "12\n";
+    fs << "CELL_DATA " << _nnodes << std::endl << "SCALARS tree_level int 1" << std::endl;
+    fs << "LOOKUP_TABLE default" << std::endl;
+    for (const Node& n : _nodes) fs << n.level << "\n";
+}
+
+// ---------------------------------------------------------------- DirectSum
+DirectSum::DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_ptr<ScalarKernel> kernel,
+                     const std::string tgt, const std::string src)
+    : _kernel(kernel), _mesh(pmesh), _vertTgt(pmesh->getVertexFieldPtr(tgt)), _faceSrc(pmesh->getFaceFieldPtr(src)),
+      _faceTgt(pmesh->getFaceFieldPtr(tgt)) {}
+
+DirectSum::DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_ptr<VectorKernel> kernel,
+                     const std::string tgt, const std::string src)
+    : _vecKernel(kernel), _mesh(pmesh), _vertTgt(pmesh->getVertexFieldPtr(tgt)), _faceSrc(pmesh->getFaceFieldPtr(src)),
+      _faceTgt(pmesh->getFaceFieldPtr(tgt)) {}
+
+namespace {
+// Leaf-face sources (ascending face index) with strength zeta * area and the
+// targets of one rank's blocks: vertices [vs, ve] then leaf faces [fs, fe].
+struct MeshProblem {
+    std::vector<scalar_type> sx, sy, sz, g, tx, ty, tz;
+    std::vector<int64_t> skip;
+    std::vector<index_type> vtx, face;  // target -> vertex / face index
+};
+
+MeshProblem materialize(const MeshedParticles& pm, const Field& src, const MPIReplicatedData& mv,
+                        const MPIReplicatedData& mf) {
+    MeshProblem p;
+    std::vector<index_type> leafOf(pm.nFaces(), -1);
+    for (index_type j = 0; j < pm.nFaces(); ++j) {
+        if (pm.faceIsDivided(j)) continue;
+        leafOf[j] = index_type(p.sx.size());
+        const XyzVector c = pm.getFaceCentroid(j);
+        p.sx.push_back(c.x); p.sy.push_back(c.y); p.sz.push_back(c.z);
+        p.g.push_back(src.getScalar(j) * pm.getFaceArea(j));
+    }
+    const int r = mv.getRank();
+    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+        const XyzVector v = pm.getVertex(i);
+        p.tx.push_back(v.x); p.ty.push_back(v.y); p.tz.push_back(v.z);
+        p.skip.push_back(-1);
+        p.vtx.push_back(i);
+    }
+    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+        if (pm.faceIsDivided(i)) continue;
+        p.tx.push_back(p.sx[leafOf[i]]); p.ty.push_back(p.sy[leafOf[i]]); p.tz.push_back(p.sz[leafOf[i]]);
+        p.skip.push_back(leafOf[i]);
+        p.face.push_back(i);
+    }
+    return p;
+}
+}  // namespace
+
+void DirectSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf) const {
+    auto pm = _mesh.lock();
+    auto src = _faceSrc.lock();
+    auto vt = _vertTgt.lock();
+    auto ft = _faceTgt.lock();
+    if (!pm || !src || !vt || !ft) throw std::logic_error("DirectSum::meshSolve: mesh or fields expired");
+    GpuContext& gpu = GpuContext::threadDefault();
+    MeshProblem p = materialize(*pm, *src, mv, mf);
+    const int64_t nt = int64_t(p.tx.size()), ns = int64_t(p.sx.size());
+    const index_type nv = index_type(p.vtx.size());
+    if (auto k = _kernel.lock()) {
+        auto gk = std::dynamic_pointer_cast<SphereGreensFn>(k);
+        if (!gk) throw std::logic_error("invalid kernel");
+        std::vector<scalar_type> psi(nt);
+        gpu.check(lpm_direct_sum(gpu.get(), LPM_K_GREENS_SPHERE, _prec, gk->radius(), 0.0, p.tx.data(), p.ty.data(),
+                                 p.tz.data(), nt, p.sx.data(), p.sy.data(), p.sz.data(), p.g.data(), ns,
+                                 p.skip.data(), nullptr, nullptr, nullptr, psi.data()),
+                  "DirectSum::meshSolve");
+        for (index_type i = 0; i < nv; ++i) vt->replace(p.vtx[i], psi[i]);
+        for (size_t i = 0; i < p.face.size(); ++i) ft->replace(p.face[i], psi[nv + i]);
+    } else if (auto k = _vecKernel.lock()) {
+        auto bk = std::dynamic_pointer_cast<BiotSavartSphere>(k);
+        if (!bk) throw std::logic_error("invalid kernel");
+        std::vector<scalar_type> ux(nt), uy(nt), uz(nt);
+        gpu.check(lpm_direct_sum(gpu.get(), LPM_K_BIOT_SAVART_SPHERE, _prec, bk->radius(), bk->delta(), p.tx.data(),
+                                 p.ty.data(), p.tz.data(), nt, p.sx.data(), p.sy.data(), p.sz.data(), p.g.data(), ns,
+                                 nullptr, ux.data(), uy.data(), uz.data(), nullptr),
+                  "DirectSum::meshSolve");
+        for (index_type i = 0; i < nv; ++i) vt->replace(p.vtx[i], XyzVector(ux[i], uy[i], uz[i]));
+        for (size_t i = 0; i < p.face.size(); ++i)
+            ft->replace(p.face[i], XyzVector(ux[nv + i], uy[nv + i], uz[nv + i]));
+    } else {
+        throw std::logic_error("invalid kernel");
+    }
+}
+
+// ---------------------------------------------------------------- TreeSum
+TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
+                 const std::shared_ptr<VectorKernel> kernel, const std::string tgt, const std::string src)
+    : _maxSeriesOrder(max_series_order), _maxParticlesPerNode(max_particles_per_box), _mesh(pmesh),
+      _vertTgt(pmesh->getVertexFieldPtr(tgt)), _faceSrc(pmesh->getFaceFieldPtr(src)),
+      _faceTgt(pmesh->getFaceFieldPtr(tgt)), vKern(kernel.get()) {
+    auto bk = dynamic_cast<BiotSavartSphere*>(vKern);
+    if (!bk) throw std::logic_error("invalid kernel");
+    if (bk->radius() != 1.0) throw std::invalid_argument("TreeSum: the series assumes the unit sphere");
+    _smooth = bk->delta();
+    _gpu = &GpuContext::threadDefault();
+    std::vector<scalar_type> x, y, z;
+    for (index_type i = 0; i < pmesh->nFaces(); ++i) {
+        if (pmesh->faceIsDivided(i)) continue;
+        const XyzVector c = pmesh->getFaceCentroid(i);
+        x.push_back(c.x); y.push_back(c.y); z.push_back(c.z);
+        _leafFaces.push_back(i);
+    }
+    int64_t nn = 0;
+    int depth = 0;
+    _gpu->check(lpm_tree_build(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), pmesh->maxR(),
+                               max_particles_per_box, &nn, &depth),
+                "TreeSum::TreeSum");
+    _nnodes = nn;
+    _depth = depth;
+    importFromGpu(index_type(x.size()));
+    // node lists hold source positions among the leaf faces; report face indices
+    for (Node& n : _nodes)
+        for (index_type& j : n.coordsContained) j = _leafFaces[j];
+}
+
+void TreeSum::computeMeshMoments() {
+    auto pm = _mesh.lock();
+    auto src = _faceSrc.lock();
+    if (!pm || !src) throw std::logic_error("TreeSum::computeMeshMoments: mesh or field expired");
+    std::vector<scalar_type> g;
+    g.reserve(_leafFaces.size());
+    for (index_type f : _leafFaces) g.push_back(src->getScalar(f) * pm->getFaceArea(f));
+    _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), _maxSeriesOrder, _momentSign), "TreeSum::computeMeshMoments");
+    _haveMoments = true;
+}
+
+void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, const scalar_type mp_tol,
+                        int& macCounter) {
+    auto pm = _mesh.lock();
+    auto vt = _vertTgt.lock();
+    auto ft = _faceTgt.lock();
+    if (!pm || !vt || !ft) throw std::logic_error("TreeSum::meshSolve: mesh or fields expired");
+    if (!_haveMoments) computeMeshMoments();
+    std::vector<scalar_type> tx, ty, tz;
+    std::vector<index_type> vtx, face;
+    const int r = mv.getRank();
+    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+        const XyzVector v = pm->getVertex(i);
+        tx.push_back(v.x); ty.push_back(v.y); tz.push_back(v.z);
+        vtx.push_back(i);
+    }
+    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+        if (pm->faceIsDivided(i)) continue;
+        const XyzVector c = pm->getFaceCentroid(i);
+        tx.push_back(c.x); ty.push_back(c.y); tz.push_back(c.z);
+        face.push_back(i);
+    }
+    const int64_t nt = int64_t(tx.size());
+    std::vector<scalar_type> ux(nt), uy(nt), uz(nt);
+    int64_t mac = 0;
+    _gpu->check(lpm_tree_eval(_gpu->get(), _prec, _series, mp_tol, _smooth, tx.data(), ty.data(), tz.data(), nt,
+                              ux.data(), uy.data(), uz.data(), &mac),
+                "TreeSum::meshSolve");
+    const size_t nv = vtx.size();
+    for (size_t i = 0; i < nv; ++i) vt->replace(vtx[i], XyzVector(ux[i], uy[i], uz[i]));
+    for (size_t i = 0; i < face.size(); ++i) ft->replace(face[i], XyzVector(ux[nv + i], uy[nv + i], uz[nv + i]));
+    macCounter = int(mac);
+}
+
+// ---------------------------------------------------------------- TaylorSeries3d
+TaylorSeries3d::TaylorSeries3d(const int maxSeriesOrder) : maxP(maxSeriesOrder) {
+    const scalar_type fill = std::numeric_limits<scalar_type>::max();
+    for (int a = 0; a <= maxP; ++a)
+        for (int b = 0; a + b <= maxP; ++b)
+            for (int c = 0; a + b + c <= maxP; ++c) {
+                coeffs.emplace(MultiIndex(a, b, c), fill);
+                moments.emplace(MultiIndex(a, b, c), fill);
+            }
+}
+
+scalar_type TaylorSeries3d::sum() const {
+    scalar_type r = 0.0;
+    for (const auto& e : coeffs) r += e.second * moments.at(e.first);
+    return r;
+}
+
+std::string TaylorSeries3d::infoString() const {
+    std::stringstream ss;
+    ss << "TaylorSeries info:" << std::endl << "\t(multi-index) -- coeff -- moment : " << std::endl;
+    for (const auto& e : coeffs) ss << "\tindex " << e.first << " -- " << e.second << " -- " << moments.at(e.first) << "\n";
+    return ss.str();
+}
+
+// The loop order of src/LpmTaylorSeries3d.cpp:63-117.
+void SphereGreensSeries::computeCoeffs(const XyzVector& t, const XyzVector& s, const scalar_type) {
+    const scalar_type D = 1.0 - t.dotProduct(s);
+    auto A = [&](int a, int b, int c) -> scalar_type& { return coeffs.at(MultiIndex(a, b, c)); };
+    A(0, 0, 0) = 1.0 / D;
+    for (int k1 = 1; k1 <= maxP; ++k1) A(k1, 0, 0) = t.x * A(k1 - 1, 0, 0) / D;
+    for (int k2 = 1; k2 <= maxP; ++k2) A(0, k2, 0) = t.y * A(0, k2 - 1, 0) / D;
+    for (int k1 = 1; k1 <= maxP; ++k1)
+        for (int k2 = 1; k2 <= maxP - k1; ++k2)
+            A(k1, k2, 0) = t.y * A(k1, k2 - 1, 0) * (scalar_type(k1 + k2) / k2) / D;
+    for (int k3 = 1; k3 <= maxP; ++k3) {
+        A(0, 0, k3) = t.z * A(0, 0, k3 - 1) / D;
+        for (int k1 = 1; k1 <= maxP - k3; ++k1) A(k1, 0, k3) = t.x * A(k1 - 1, 0, k3) * (scalar_type(k1 + k3) / k1) / D;
+        for (int k2 = 1; k2 <= maxP - k3; ++k2) A(0, k2, k3) = t.y * A(0, k2 - 1, k3) * (scalar_type(k2 + k3) / k2) / D;
+        for (int k1 = 1; k1 <= maxP - k3; ++k1)
+            for (int k2 = 1; k2 <= maxP - k3 - k1; ++k2)
+                A(k1, k2, k3) = t.y * A(k1, k2 - 1, k3) * (scalar_type(k1 + k2 + k3) / k2) / D;
+    }
+}
+
+// m_k = sum_i (y_i - c)^k * strength_i (src/LpmTaylorSeries3d.cpp:119-154)
+void TaylorSeries3d::computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& inds,
+                                    const XyzVector& cntd, const std::shared_ptr<Field> str) {
+    for (auto& e : moments) {
+        e.second = 0.0;
+        for (index_type i : inds) e.second += e.first.vectorPower(crds->getVec(i) - cntd) * str->getScalar(i);
+    }
+}
+
+void TaylorSeries3d::computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& inds,
+                                    const XyzVector& cntd, const std::shared_ptr<Field> vals,
+                                    const std::shared_ptr<Field> wts) {
+    for (auto& e : moments) {
+        e.second = 0.0;
+        for (index_type i : inds)
+            e.second += e.first.vectorPower(crds->getVec(i) - cntd) * (vals->getScalar(i) * wts->getScalar(i));
+    }
+}
+
+void TaylorSeries3d::computeMoments(const std::shared_ptr<PolyMesh2d> faces, const std::vector<index_type>& inds,
+                                    const XyzVector& cntd, const std::shared_ptr<Field> vals) {
+    for (auto& e : moments) {
+        e.second = 0.0;
+        for (index_type i : inds)
+            e.second += e.first.vectorPower(faces->faceCentroid(i) - cntd) * vals->getScalar(i) * faces->faceArea(i);
+    }
+}
+
+}  // namespace Lpm

@@ -1,0 +1,324 @@
+// Host C++ API of the spherical Biot-Savart / Poisson sum, namespace Lpm.
+//
+// Keeps the reference's class names, constructor arguments and method
+// signatures on this path so calling code moves over unchanged:
+//   VectorKernel / BiotSavartSphere        src/LpmVectorKernel.h:11-39
+//   ScalarKernel / SphereGreensFn          src/LpmScalarKernel.h:11-45
+//   MultiIndex                             src/LpmMultiIndex.h:12-72
+//   Box3d                                  src/LpmBox3d.h:12-91
+//   MPIReplicatedData                      src/LpmMPIReplicatedData.h:11-37
+//   Coords (SoA positions)                 src/LpmCoords.h:16-78
+//   Tree / Node                            src/LpmOctree.h:21-98
+//   DirectSum                              src/LpmDirectSum.h:18-56
+//   TreeSum                                src/LpmTreeSum.h:47-97
+//   TaylorSeries3d / SphereGreensSeries    src/LpmTaylorSeries3d.h:20-63
+// The sums run on the GPU through the C ABI of include/lpm_gpu.h; failures
+// come back as the reference's exception types (std::logic_error for an
+// unusable kernel, std::bad_alloc for allocation, std::invalid_argument /
+// std::runtime_error otherwise).  One process is one rank: ranks of a
+// replicated-data partition are blocks of targets solved locally, and the
+// broadcast step has nothing to exchange.
+#pragma once
+
+#include "LpmCore.hpp"
+#include "LpmMesh.hpp"
+#include "lpm_gpu.h"
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace Lpm {
+
+// ---------------------------------------------------------------- kernels
+class VectorKernel {
+  public:
+    virtual ~VectorKernel() {}
+    virtual XyzVector evaluate(const XyzVector& tgtLoc, const XyzVector& srcLoc) const = 0;
+    virtual bool isSingular() const = 0;
+    virtual scalar_type sumMultiplier() const { return 1.0; }
+};
+
+class BiotSavartSphere : public VectorKernel {
+  public:
+    BiotSavartSphere(const scalar_type delta = 0.0, const scalar_type sphRadius = 1.0)
+        : _delta(delta), _sphRadius(sphRadius) {}
+    // (x * y) / (R^2 - x.y + delta^2), zero when |den| <= 1e-13
+    XyzVector evaluate(const XyzVector& t, const XyzVector& s) const override {
+        const XyzVector c = t.crossProduct(s);
+        const scalar_type den = _sphRadius * _sphRadius - t.dotProduct(s) + _delta * _delta;
+        return std::abs(den) > ZERO_TOL ? c.scalarMultiply(1.0 / den) : XyzVector(0.0, 0.0, 0.0);
+    }
+    scalar_type sumMultiplier() const override { return -1.0 / (4.0 * PI * _sphRadius); }
+    bool isSingular() const override { return _delta == 0.0; }
+    scalar_type delta() const { return _delta; }
+    scalar_type radius() const { return _sphRadius; }
+
+  protected:
+    scalar_type _delta, _sphRadius;
+};
+
+class ScalarKernel {
+  public:
+    virtual ~ScalarKernel() {}
+    virtual scalar_type evaluate(const XyzVector& tgtLoc, const XyzVector& srcLoc) const = 0;
+    virtual bool isSingular() const = 0;
+};
+
+class SphereGreensFn : public ScalarKernel {
+  public:
+    SphereGreensFn(const scalar_type r) : _radius(r) {}
+    // -log(R^2 - y.x) / (4 pi R)
+    scalar_type evaluate(const XyzVector& t, const XyzVector& s) const override {
+        return -std::log(_radius * _radius - s.dotProduct(t)) / (4.0 * PI * _radius);
+    }
+    bool isSingular() const override { return true; }
+    scalar_type radius() const { return _radius; }
+
+  protected:
+    scalar_type _radius;
+};
+
+// ---------------------------------------------------------------- multi-index
+inline index_type factorial(const index_type n) { return (n == 1 || n == 0) ? 1 : n * factorial(n - 1); }
+
+struct MultiIndex {
+    MultiIndex(const unsigned i1 = 0, const unsigned i2 = 0, const unsigned i3 = 0) : k1(i1), k2(i2), k3(i3) {}
+    index_type k1, k2, k3;
+    index_type magnitude() const { return k1 + k2 + k3; }
+    scalar_type vectorPower(const XyzVector& v) const {
+        return std::pow(v.x, double(k1)) * std::pow(v.y, double(k2)) * std::pow(v.z, double(k3));
+    }
+    index_type factorial() const { return Lpm::factorial(k1) * Lpm::factorial(k2) * Lpm::factorial(k3); }
+};
+inline bool operator<(const MultiIndex& a, const MultiIndex& b) {
+    if (a.k1 != b.k1) return a.k1 < b.k1;
+    if (a.k2 != b.k2) return a.k2 < b.k2;
+    return a.k3 < b.k3;
+}
+inline bool operator==(const MultiIndex& a, const MultiIndex& b) { return a.k1 == b.k1 && a.k2 == b.k2 && a.k3 == b.k3; }
+inline bool operator!=(const MultiIndex& a, const MultiIndex& b) { return !(a == b); }
+inline bool operator>(const MultiIndex& a, const MultiIndex& b) { return b < a; }
+inline bool operator<=(const MultiIndex& a, const MultiIndex& b) { return !(b < a); }
+inline bool operator>=(const MultiIndex& a, const MultiIndex& b) { return !(a < b); }
+inline std::ostream& operator<<(std::ostream& os, const MultiIndex& k) {
+    return os << "(" << k.k1 << ", " << k.k2 << ", " << k.k3 << ")";
+}
+
+// ---------------------------------------------------------------- Box3d
+#define BOX_PADDING_FACTOR 0.00001
+
+class Box3d {
+  public:
+    Box3d() : xmin(0), xmax(0), ymin(0), ymax(0), zmin(0), zmax(0), minRadius(0), maxRadius(0) {}
+    Box3d(scalar_type x0, scalar_type x1, scalar_type y0, scalar_type y1, scalar_type z0, scalar_type z1)
+        : xmin(x0), xmax(x1), ymin(y0), ymax(y1), zmin(z0), zmax(z1) { finish(); }
+    Box3d(scalar_type maxr, bool pad = true) : xmin(-maxr), xmax(maxr), ymin(-maxr), ymax(maxr), zmin(-maxr), zmax(maxr) {
+        if (pad) {
+            xmin -= BOX_PADDING_FACTOR; xmax += BOX_PADDING_FACTOR;
+            ymin -= BOX_PADDING_FACTOR; ymax += BOX_PADDING_FACTOR;
+            zmin -= BOX_PADDING_FACTOR; zmax += BOX_PADDING_FACTOR;
+        }
+        finish();
+    }
+    scalar_type volume() const { return (xmax - xmin) * (ymax - ymin) * (zmax - zmin); }
+    bool containsPoint(const XyzVector& v) const {
+        return (xmin <= v.x && v.x < xmax) && (ymin <= v.y && v.y < ymax) && (zmin <= v.z && v.z < zmax);
+    }
+    std::vector<XyzVector> corners() const;
+    std::vector<Box3d> bisectAll() const;
+    scalar_type edgeLength(int dim) const { return dim == 0 ? xmax - xmin : dim == 1 ? ymax - ymin : zmax - zmin; }
+    scalar_type longestEdge() const;
+    std::string infoString() const;
+
+    scalar_type xmin, xmax, ymin, ymax, zmin, zmax;
+    scalar_type minRadius, maxRadius;
+    XyzVector centroid;
+
+  private:
+    void finish();
+};
+
+// ---------------------------------------------------------------- replicated-data partition
+class MPIReplicatedData {
+  public:
+    MPIReplicatedData(const index_type nItems, const int rank = 0, const int nProcs = 1);
+    index_type startIndex(const int rank) const { return _start.at(rank); }
+    index_type endIndex(const int rank) const { return _end.at(rank); }
+    index_type msgSize(const int rank) const { return _size.at(rank); }
+    int getRank() const { return _rank; }
+    int getSize() const { return _nProcs; }
+    index_type nItems() const { return _nItems; }
+
+  private:
+    index_type _nItems;
+    int _nProcs, _rank;
+    std::vector<index_type> _start, _end, _size;
+};
+
+// ---------------------------------------------------------------- Coords
+class Coords {
+  public:
+    explicit Coords(index_type nMax = 0) { x.reserve(nMax); y.reserve(nMax); z.reserve(nMax); }
+    index_type n() const { return index_type(x.size()); }
+    void insert(const XyzVector& v) { x.push_back(v.x); y.push_back(v.y); z.push_back(v.z); }
+    void insert(scalar_type a, scalar_type b, scalar_type c) { x.push_back(a); y.push_back(b); z.push_back(c); }
+    XyzVector getVec(index_type i) const { return XyzVector(x[i], y[i], z[i]); }
+    scalar_type minX() const; scalar_type maxX() const;
+    scalar_type minY() const; scalar_type maxY() const;
+    scalar_type minZ() const; scalar_type maxZ() const;
+    scalar_type maxR() const;
+    std::vector<scalar_type> x, y, z;
+};
+
+// ---------------------------------------------------------------- GPU context
+// One lpm_ctx per host thread, created on first use.
+class GpuContext {
+  public:
+    explicit GpuContext(int device = 0);
+    ~GpuContext();
+    GpuContext(const GpuContext&) = delete;
+    GpuContext& operator=(const GpuContext&) = delete;
+    lpm_ctx* get() const { return _ctx; }
+    // Throws the exception type matching an lpm_status.
+    void check(int status, const char* where) const;
+    static GpuContext& threadDefault();
+
+  private:
+    lpm_ctx* _ctx = nullptr;
+};
+
+// ---------------------------------------------------------------- octree
+struct Node {
+    Box3d box;
+    int level = 0;
+    index_type parent = -1;                  // node id, -1 at the root
+    std::vector<index_type> kids;            // node ids in octant order
+    std::vector<index_type> coordsContained; // source indices (ascending in leaves)
+    bool isLeaf() const { return kids.empty(); }
+    bool hasKids() const { return !kids.empty(); }
+    index_type nCoords() const { return index_type(coordsContained.size()); }
+};
+
+class Tree {
+  public:
+    enum TREE_DEPTH_CONTROL { MAX_COORDS_PER_NODE, MAX_DEPTH };
+
+    // Point-cloud tree: root box [min - 1e-5 min, max + 1e-5 max) per axis
+    // (src/LpmOctree.cpp:16-36).
+    Tree(const std::shared_ptr<Coords> crds, const scalar_type maxAspectRatio = 1.0, const int prank = 0);
+    virtual ~Tree() {}
+
+    // Builds on the GPU.  do_shrink = true is not supported (invalid_argument).
+    virtual void buildTree(const TREE_DEPTH_CONTROL depth_type, const index_type intParam, const bool do_shrink = false);
+    int depth() const { return _depth; }
+    index_type nNodes() const { return _nnodes; }
+    index_type recursiveNNodes() const { return _nnodes; }
+    const Node& node(index_type i) const { return _nodes.at(i); }
+    const Node& root() const { return _nodes.at(0); }
+    std::string infoString() const;
+    // Legacy VTK unstructured grid of every node box, depth-first
+    // (src/LpmOctree.cpp:245-331).
+    virtual void writeToVtk(const std::string& filename, const std::string& desc = "") const;
+
+  protected:
+    Tree() {}
+    // Reads the GPU node table of the context into _nodes (depth-first ids).
+    void importFromGpu(index_type nSources);
+
+    int _depth = 0;
+    index_type _nnodes = 0;
+    std::weak_ptr<Coords> _crds;
+    std::vector<Node> _nodes;
+    GpuContext* _gpu = nullptr;
+};
+
+// ---------------------------------------------------------------- direct sum
+class DirectSum {
+  public:
+    DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_ptr<ScalarKernel> kernel,
+              const std::string tgtFieldName, const std::string srcFieldName);
+    DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_ptr<VectorKernel> kernel,
+              const std::string tgtFieldName, const std::string srcFieldName);
+
+    void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces) const;
+    // Every rank's block is computed in this process: nothing to exchange.
+    void meshBroadcast(const MPIReplicatedData&, const MPIReplicatedData&) const {}
+
+    // LPM_F64 (default, the reference's pair formula) or LPM_F32C.
+    void setPrecision(int prec) { _prec = prec; }
+
+  protected:
+    std::weak_ptr<ScalarKernel> _kernel;
+    std::weak_ptr<VectorKernel> _vecKernel;
+    std::weak_ptr<MeshedParticles> _mesh;
+    std::weak_ptr<Field> _vertTgt, _faceSrc, _faceTgt;
+    int _prec = LPM_F64;
+};
+
+// ---------------------------------------------------------------- treecode
+class TreeSum : public Tree {
+  public:
+    TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
+            const std::shared_ptr<VectorKernel> kernel, const std::string tgtFieldName,
+            const std::string srcFieldName);
+
+    void computeMeshMoments();
+    void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces, const scalar_type mp_tol,
+                   int& macCounter);
+    void meshBroadcast(const MPIReplicatedData&, const MPIReplicatedData&) const {}
+
+    // Defaults reproduce the reference: moment sign -1 (h = centroid - y),
+    // the reference series, FP64.  setMomentSign(+1) with
+    // setSeries(LPM_SERIES_CARTESIAN) selects the convergent expansion.
+    void setMomentSign(int s) { _momentSign = s; }
+    void setSeries(int s) { _series = s; }
+    void setPrecision(int p) { _prec = p; }
+    void setSmoothing(scalar_type d) { _smooth = d; }
+    int maxSeriesOrder() const { return _maxSeriesOrder; }
+
+  protected:
+    int _maxSeriesOrder, _maxParticlesPerNode;
+    scalar_type _smooth = 0.0;
+    int _momentSign = -1, _series = LPM_SERIES_REFERENCE, _prec = LPM_F64;
+    bool _haveMoments = false;
+    std::weak_ptr<MeshedParticles> _mesh;
+    std::weak_ptr<Field> _vertTgt, _faceSrc, _faceTgt;
+    VectorKernel* vKern;
+    std::vector<index_type> _leafFaces;  // source j -> face index
+};
+
+// ---------------------------------------------------------------- Taylor series (host, FP64)
+class TaylorSeries3d {
+  public:
+    virtual ~TaylorSeries3d() {}
+    scalar_type getCoeff(const MultiIndex& k) const { return coeffs.at(k); }
+    scalar_type getMoment(const MultiIndex& k) const { return moments.at(k); }
+    scalar_type sum() const;
+    std::string infoString() const;
+    virtual void computeCoeffs(const XyzVector& tgtVec, const XyzVector& srcVec, const scalar_type param = 0.0) = 0;
+    void computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& crdInds,
+                        const XyzVector& cntd, const std::shared_ptr<Field> srcStr);
+    void computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& crdInds,
+                        const XyzVector& cntd, const std::shared_ptr<Field> srcVals,
+                        const std::shared_ptr<Field> srcWeights);
+    void computeMoments(const std::shared_ptr<PolyMesh2d> faces, const std::vector<index_type>& crdInds,
+                        const XyzVector& cntd, const std::shared_ptr<Field> srcVals);
+    int maxOrder() const { return maxP; }
+
+  protected:
+    explicit TaylorSeries3d(const int maxSeriesOrder);
+    std::map<MultiIndex, scalar_type> coeffs, moments;
+    int maxP;
+};
+
+// Coefficients of 1/(1 - x.c - x.h) for |k| <= maxP (src/LpmTaylorSeries3d.cpp:63-117).
+class SphereGreensSeries : public TaylorSeries3d {
+  public:
+    explicit SphereGreensSeries(const int maxSeriesOrder) : TaylorSeries3d(maxSeriesOrder) {}
+    void computeCoeffs(const XyzVector& tgtVec, const XyzVector& srcVec, const scalar_type param = 0.0) override;
+};
+
+}  // namespace Lpm

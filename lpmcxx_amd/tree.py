@@ -1,0 +1,126 @@
+"""Treecode entry points (lpm_tree_* in include/lpm_gpu.h) for Python callers.
+
+Mirrors TreeSum's mesh path (src/LpmTreeSum.cpp:221-238 constructor,
+:182-197 computeMeshMoments, :55-86 meshSolve):
+    build(ctx, sources, max_radius, leaf_max)     TreeSum(p, leaf_max, mesh, ...)
+    moments(ctx, gamma, order, moment_sign)       computeMeshMoments()
+    evaluate(ctx, targets, theta, ...)            meshSolve(..., mp_tol=theta, macCounter)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import LPM_F64, LPM_SERIES_CARTESIAN, Context, _ptr
+
+
+@dataclass
+class TreeResult:
+    velocity: tuple
+    mac_count: int
+    n_nodes: int
+    depth: int
+
+
+def build(ctx: Context, sources, max_radius: float, leaf_max: int):
+    sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
+    nn, depth = C.c_int64(), C.c_int()
+    ctx.check(ctx.lib.lpm_tree_build(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.size, max_radius, leaf_max,
+                                     C.byref(nn), C.byref(depth)))
+    return nn.value, depth.value
+
+
+def build_box(ctx: Context, sources, box_min, box_max, control: int, param: int):
+    """General root box and depth control (control 0: max per node, 1: max depth)."""
+    sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
+    lo = np.ascontiguousarray(box_min, dtype=np.float64)
+    hi = np.ascontiguousarray(box_max, dtype=np.float64)
+    nn, depth = C.c_int64(), C.c_int()
+    ctx.check(ctx.lib.lpm_tree_build_box(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.size, _ptr(lo), _ptr(hi), control,
+                                         param, C.byref(nn), C.byref(depth)))
+    return nn.value, depth.value
+
+
+def build_dev(ctx: Context, sx, sy, sz, max_radius: float, leaf_max: int):
+    nn, depth = C.c_int64(), C.c_int()
+    ctx.check(ctx.lib.lpm_tree_build_dev(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.numel(), max_radius, leaf_max,
+                                         C.byref(nn), C.byref(depth)))
+    return nn.value, depth.value
+
+
+def moments(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    ctx.check(ctx.lib.lpm_tree_moments(ctx.h, _ptr(g), order, moment_sign))
+
+
+def moments_dev(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
+    ctx.check(ctx.lib.lpm_tree_moments_dev(ctx.h, _ptr(gamma), order, moment_sign))
+
+
+def evaluate(ctx: Context, targets, theta: float, delta: float = 0.0, prec: int = LPM_F64,
+             series: int = LPM_SERIES_CARTESIAN):
+    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    n = tx.size
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    mac = C.c_int64()
+    ctx.check(ctx.lib.lpm_tree_eval(ctx.h, prec, series, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), n,
+                                    _ptr(ux), _ptr(uy), _ptr(uz), C.byref(mac)))
+    return (ux, uy, uz), mac.value
+
+
+def evaluate_dev(ctx: Context, tx, ty, tz, out, theta: float, delta: float = 0.0, prec: int = LPM_F64,
+                 series: int = LPM_SERIES_CARTESIAN, want_mac: bool = False):
+    mac = C.c_int64()
+    ctx.check(ctx.lib.lpm_tree_eval_dev(ctx.h, prec, series, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
+                                        _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                        C.byref(mac) if want_mac else None))
+    return mac.value
+
+
+def info(ctx: Context):
+    nn, depth, nl = C.c_int64(), C.c_int(), C.c_int64()
+    ctx.check(ctx.lib.lpm_tree_info(ctx.h, C.byref(nn), C.byref(depth), C.byref(nl)))
+    return nn.value, depth.value, nl.value
+
+
+def export(ctx: Context, n_sources: int) -> dict:
+    nn, _, _ = info(ctx)
+    out = {
+        "box": np.empty((nn, 6)), "cent": np.empty((nn, 4)), "level": np.empty(nn, np.int32),
+        "parent": np.empty(nn, np.int32), "first_kid": np.empty(nn, np.int32), "n_kids": np.empty(nn, np.int32),
+        "range": np.empty((nn, 2), np.int32), "perm": np.empty(n_sources, np.int32),
+    }
+    ctx.check(ctx.lib.lpm_tree_export(ctx.h, _ptr(out["box"]), _ptr(out["cent"]), _ptr(out["level"]),
+                                      _ptr(out["parent"]), _ptr(out["first_kid"]), _ptr(out["n_kids"]),
+                                      _ptr(out["range"]), _ptr(out["perm"])))
+    return out
+
+
+def export_moments(ctx: Context, order: int) -> np.ndarray:
+    nn, _, _ = info(ctx)
+    out = np.empty((nn, order * (order + 1) * (order + 2) // 6, 3))
+    ctx.check(ctx.lib.lpm_tree_export_moments(ctx.h, _ptr(out)))
+    return out
+
+
+def dfs_order(first_kid, n_kids) -> np.ndarray:
+    """Node ids in depth-first pre-order with children in octant order (the
+    reference's traversal order, src/LpmTreeSum.cpp:132-134)."""
+    out, stack = [], [0]
+    while stack:
+        i = stack.pop()
+        out.append(i)
+        k, f = int(n_kids[i]), int(first_kid[i])
+        stack.extend(range(f + k - 1, f - 1, -1))
+    return np.array(out, dtype=np.int64)
+
+
+def tree_velocity(ctx: Context, sources, gamma, targets, *, max_radius: float, leaf_max: int, order: int,
+                  theta: float, moment_sign: int = 1, prec: int = LPM_F64, delta: float = 0.0,
+                  series: int = LPM_SERIES_CARTESIAN) -> TreeResult:
+    nn, depth = build(ctx, sources, max_radius, leaf_max)
+    moments(ctx, gamma, order, moment_sign)
+    u, mac = evaluate(ctx, targets, theta, delta, prec, series)
+    return TreeResult(u, mac, nn, depth)

@@ -38,6 +38,8 @@ def _declare_tree(L: C.CDLL) -> None:
     if hasattr(L, "oracle_tree_build"):
         L.oracle_tree_build.argtypes = [_vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, C.c_int]
         L.oracle_tree_build.restype = C.c_int64
+        L.oracle_tree_build_box.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int, C.c_int]
+        L.oracle_tree_build_box.restype = C.c_int64
         L.oracle_tree_get.argtypes = [_vp] * 9
         L.oracle_tree_get.restype = None
         L.oracle_tree_moments.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp]
@@ -47,6 +49,9 @@ def _declare_tree(L: C.CDLL) -> None:
         L.oracle_tree_eval.restype = C.c_int64
         L.oracle_tree_free.argtypes = []
         L.oracle_tree_free.restype = None
+        L.oracle_node_far.argtypes = [C.c_double] * 6 + [_vp] * 4 + [C.c_int64, C.c_int, C.c_int, C.c_int,
+                                                                     C.c_double, _vp]
+        L.oracle_node_far.restype = None
         L.oracle_coeffs_lex.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int, _vp]
         L.oracle_coeffs_lex.restype = C.c_int
@@ -104,3 +109,93 @@ def rel_max(a, b) -> float:
         nb = np.sqrt(sum(y ** 2 for y in b))
         return float(da.max() / nb.max())
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+# ---------------------------------------------------------------------- treecode
+_last_nn = [0]
+
+
+def nterms(p):
+    return p * (p + 1) * (p + 2) // 6
+
+
+def tree_build(sources, max_r, leaf_max, box=None, control=0):
+    """Reference recursive octree (module state); returns a dict of node arrays
+    with root 0 and the children of every node consecutive.  box=(lo3, hi3)
+    and control (0 = max per node, 1 = max depth) select the general build."""
+    L = lib()
+    sx, sy, sz = (_c(a) for a in sources)
+    if box is None:
+        nn = L.oracle_tree_build(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, sx.size, max_r, leaf_max, 0)
+    else:
+        lo, hi = _c(box[0]), _c(box[1])
+        nn = L.oracle_tree_build_box(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, sx.size, lo.ctypes.data,
+                                     hi.ctypes.data, control, leaf_max)
+    if nn < 0:
+        raise ValueError("oracle tree build failed")
+    _last_nn[0] = nn
+    out = {"box": np.empty((nn, 8)), "cent": np.empty((nn, 4))}
+    for k in ("level", "parent", "first_kid", "n_kids", "begin", "end"):
+        out[k] = np.empty(nn, np.int64)
+    out["perm"] = np.empty(sx.size, np.int64)
+    L.oracle_tree_get(out["box"].ctypes.data, out["cent"].ctypes.data, out["level"].ctypes.data,
+                      out["parent"].ctypes.data, out["first_kid"].ctypes.data, out["n_kids"].ctypes.data,
+                      out["begin"].ctypes.data, out["end"].ctypes.data, out["perm"].ctypes.data)
+    out["box"] = out["box"][:, :6].copy()
+    return out
+
+
+def tree_moments(sources, gamma, p, moment_sign):
+    """Moments of the last built oracle tree: [n_nodes, nterms(p), 3]."""
+    sx, sy, sz = (_c(a) for a in sources)
+    g = _c(gamma)
+    out = np.empty((_last_nn[0], nterms(p), 3))
+    lib().oracle_tree_moments(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, g.ctypes.data, p, moment_sign,
+                              out.ctypes.data)
+    return out
+
+
+def tree_eval(sources, gamma, moments, p, theta, targets, delta=0.0, radius=1.0, series=0):
+    """Reference traversal; series 0 = reference expansion, 1 = Cartesian
+    expansion of 2/(|t-y|^2 + 2 delta^2).  Returns ((ux, uy, uz), mac_total, mac_per_target)."""
+    L = lib()
+    sx, sy, sz = (_c(a) for a in sources)
+    g = _c(gamma)
+    m = _c(moments)
+    tx, ty, tz = (_c(a) for a in targets)
+    n = tx.size
+    ux, uy, uz = np.empty(n), np.empty(n), np.empty(n)
+    mac = np.empty(n, np.int64)
+    tot = L.oracle_tree_eval(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, g.ctypes.data, m.ctypes.data, p,
+                             theta, delta, radius, tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, n,
+                             ux.ctypes.data, uy.ctypes.data, uz.ctypes.data, mac.ctypes.data, series)
+    return (ux, uy, uz), int(tot), mac
+
+
+def coeffs_lex(t, c, delta, p):
+    """SumNode::calc_coeffs restated: lex-ordered coefficients for |k| < p."""
+    out = np.empty(nterms(p))
+    lib().oracle_coeffs_lex(t[0], t[1], t[2], c[0], c[1], c[2], delta, p, out.ctypes.data)
+    return out
+
+
+def coeffs_series(t, s, max_p):
+    """SphereGreensSeries::computeCoeffs restated: lex-ordered for |k| <= max_p."""
+    out = np.empty(nterms(max_p + 1))
+    lib().oracle_coeffs_series(t[0], t[1], t[2], s[0], s[1], s[2], max_p, out.ctypes.data)
+    return out
+
+
+def multi_indices(p):
+    """(k1, k2, k3) with |k| < p in lex order."""
+    return [(a, b, c) for a in range(p) for b in range(p) for c in range(p) if a + b + c < p]
+
+
+def node_far(c, t, sources, gamma, p, moment_sign, series, delta=0.0):
+    """Unscaled far field of one box about centroid c (sum before -1/(4 pi R))."""
+    sx, sy, sz = (_c(a) for a in sources)
+    g = _c(gamma)
+    out = np.empty(3)
+    lib().oracle_node_far(c[0], c[1], c[2], t[0], t[1], t[2], sx.ctypes.data, sy.ctypes.data, sz.ctypes.data,
+                          g.ctypes.data, sx.size, p, moment_sign, series, delta, out.ctypes.data)
+    return out

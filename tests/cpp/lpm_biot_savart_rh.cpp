@@ -1,0 +1,102 @@
+// Rossby-Haurwitz wave velocity by direct sum and treecode through the Lpm
+// API on the GPU; the asserting counterpart of
+// tests/LpmBiotSavartRHWaveTests.cpp:46-148 (icosahedral levels, zeta = Y_5^4,
+// exact velocity rossbyHaurwitz54Velocity).  Exit status = failures.
+#include "LpmSolvers.hpp"
+
+#include <cstdio>
+
+using namespace Lpm;
+
+static int failures = 0;
+#define CHECK(cond)                                                                \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            ++failures;                                                            \
+        }                                                                          \
+    } while (0)
+
+static double maxRelErr(const Field& a, const Field& b, const MeshedParticles& pm, bool faces) {
+    double num = 0.0, den = 0.0;
+    for (index_type i = 0; i < a.n(); ++i) {
+        if (faces && pm.faceIsDivided(i)) continue;
+        const XyzVector d = a.getVec(i) - b.getVec(i);
+        num = std::max(num, d.magnitude());
+        den = std::max(den, b.getVec(i).magnitude());
+    }
+    return num / den;
+}
+
+int main() {
+    IcosTriSphereSeed seed;
+    SphereHarmonic54 y54;
+    RossbyHaurwitz54Velocity rh;
+    auto kernel = std::make_shared<BiotSavartSphere>();
+    double prev = 1e9;
+    for (int k = 2; k <= 5; ++k) {
+        auto pm = std::make_shared<MeshedParticles>(seed, k, true, 1.0, 0);
+        for (const char* n : {"velocity", "treeVelocity", "exact", "treeVelocityRef"}) {
+            pm->createVertexField(n, "length/time", 3);
+            pm->createFaceField(n, "length/time", 3);
+        }
+        pm->createVertexField("vorticity", "1/time", 1);
+        pm->createFaceField("vorticity", "1/time", 1);
+        pm->initializeVertexFieldWithFunction("vorticity", &y54);
+        pm->initializeFaceFieldWithFunction("vorticity", &y54);
+        pm->initializeVertexFieldWithFunction("exact", &rh);
+        pm->initializeFaceFieldWithFunction("exact", &rh);
+        MPIReplicatedData mpiVerts(pm->nVertices(), 0, 1);
+        MPIReplicatedData mpiFaces(pm->nFaces(), 0, 1);
+
+        DirectSum solver(pm, kernel, "velocity", "vorticity");
+        solver.meshSolve(mpiVerts, mpiFaces);
+        solver.meshBroadcast(mpiVerts, mpiFaces);
+        const double err = maxRelErr(*pm->getFaceFieldPtr("velocity"), *pm->getFaceFieldPtr("exact"), *pm, true);
+
+        // reference defaults: moment sign -1, reference series (parity mode)
+        TreeSum refTree(6, 500, pm, kernel, "treeVelocityRef", "vorticity");
+        refTree.computeMeshMoments();
+        int macRef = 0;
+        refTree.meshSolve(mpiVerts, mpiFaces, 0.33, macRef);
+
+        // convergent expansion
+        TreeSum tree(8, 64, pm, kernel, "treeVelocity", "vorticity");
+        tree.setMomentSign(1);
+        tree.setSeries(LPM_SERIES_CARTESIAN);
+        tree.computeMeshMoments();
+        int mac = 0;
+        tree.meshSolve(mpiVerts, mpiFaces, 0.5, mac);
+        const double terr = maxRelErr(*pm->getFaceFieldPtr("treeVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
+        std::printf("icos level %d: faces %ld, max rel err direct vs exact %.3e, tree(p=8,theta=0.5) vs direct %.3e, "
+                    "mac %d, reference-mode mac %d, nodes %ld depth %d\n",
+                    k, (long)pm->nLeafFaces(), err, terr, mac, macRef, (long)tree.nNodes(), tree.depth());
+        CHECK(err < prev);
+        prev = err;
+        CHECK(terr < 1e-4);
+        if (k >= 4) CHECK(mac > 0);
+        // the tree's node lists partition the leaf faces
+        index_type covered = 0;
+        for (index_type i = 0; i < tree.nNodes(); ++i)
+            if (tree.node(i).isLeaf()) covered += tree.node(i).nCoords();
+        CHECK(covered == pm->nLeafFaces());
+    }
+    CHECK(prev < 0.02);
+    // a kernel the GPU cannot evaluate is the reference's "invalid kernel"
+    struct Other : VectorKernel {
+        XyzVector evaluate(const XyzVector&, const XyzVector&) const override { return XyzVector(); }
+        bool isSingular() const override { return false; }
+    };
+    auto pm = std::make_shared<MeshedParticles>(seed, 1);
+    pm->createVertexField("u", "", 3);
+    pm->createFaceField("u", "", 3);
+    pm->createFaceField("z", "", 1);
+    bool threw = false;
+    try {
+        DirectSum bad(pm, std::shared_ptr<VectorKernel>(new Other()), "u", "z");
+        bad.meshSolve(MPIReplicatedData(pm->nVertices()), MPIReplicatedData(pm->nFaces()));
+    } catch (const std::logic_error&) { threw = true; }
+    CHECK(threw);
+    std::printf("lpm_biot_savart_rh: %d failure(s)\n", failures);
+    return failures;
+}

@@ -1,0 +1,74 @@
+// Streamfunction by direct sum through the Lpm API: psi = zeta/30 for
+// zeta = Y_5^4 on cubed-sphere and icosahedral meshes; the asserting
+// counterpart of tests/LpmPoissonSolverTests.cpp:254-401.  Also runs the
+// point-cloud Tree of tests/LpmOctreeUnitTests.cpp:35-127 and writes its VTK.
+#include "LpmSolvers.hpp"
+
+#include <cstdio>
+#include <random>
+
+using namespace Lpm;
+
+static int failures = 0;
+#define CHECK(cond)                                                                \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            ++failures;                                                            \
+        }                                                                          \
+    } while (0)
+
+int main(int argc, char** argv) {
+    SphereHarmonic54 y54;
+    auto kernel = std::make_shared<SphereGreensFn>(1.0);
+    CubedSphereSeed cubed;
+    IcosTriSphereSeed icos;
+    for (const MeshSeed* seed : {static_cast<const MeshSeed*>(&cubed), static_cast<const MeshSeed*>(&icos)}) {
+        double prev = 1e9;
+        for (int k = 2; k <= 5; ++k) {
+            auto pm = std::make_shared<MeshedParticles>(*seed, k);
+            pm->createVertexField("psi", "", 1);
+            pm->createFaceField("psi", "", 1);
+            pm->createFaceField("vorticity", "", 1);
+            pm->initializeFaceFieldWithFunction("vorticity", &y54);
+            DirectSum solver(pm, kernel, "psi", "vorticity");
+            solver.meshSolve(MPIReplicatedData(pm->nVertices()), MPIReplicatedData(pm->nFaces()));
+            double num = 0.0, den = 0.0;
+            auto psi = pm->getFaceFieldPtr("psi");
+            for (index_type i = 0; i < pm->nFaces(); ++i) {
+                if (pm->faceIsDivided(i)) continue;
+                const double ex = y54.evaluateScalar(pm->getFaceCentroid(i)) / 30.0;
+                num = std::max(num, std::abs(psi->getScalar(i) - ex));
+                den = std::max(den, std::abs(ex));
+            }
+            std::printf("%s level %d: psi max rel err %.3e\n", seed->idString().c_str(), k, num / den);
+            CHECK(num / den < prev);
+            prev = num / den;
+        }
+        CHECK(prev < 0.06);
+    }
+    // standalone octree on random points (leaf <= 10), as the reference's octree test
+    auto crds = std::make_shared<Coords>(8000);
+    std::mt19937_64 gen(1234);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int i = 0; i < 8000; ++i) crds->insert(U(gen), U(gen), U(gen));
+    Tree tree(crds);
+    tree.buildTree(Tree::MAX_COORDS_PER_NODE, 10);
+    index_type covered = 0;
+    for (index_type i = 0; i < tree.nNodes(); ++i) {
+        const Node& n = tree.node(i);
+        if (!n.isLeaf()) continue;
+        CHECK(n.nCoords() <= 10);
+        for (index_type j : n.coordsContained) CHECK(n.box.containsPoint(crds->getVec(j)));
+        covered += n.nCoords();
+    }
+    // points at the extremes can fall outside the relatively padded root box
+    std::printf("octree: %ld nodes, depth %d, %ld of 8000 points in leaves\n", (long)tree.nNodes(), tree.depth(),
+                (long)covered);
+    if (argc > 1) tree.writeToVtk(argv[1], "octree of 8000 random points");
+    Tree byDepth(crds);
+    byDepth.buildTree(Tree::MAX_DEPTH, 4);
+    CHECK(byDepth.depth() == 4);
+    std::printf("lpm_poisson: %d failure(s)\n", failures);
+    return failures;
+}
