@@ -136,6 +136,9 @@ int lpm_tree_eval(lpm_ctx* ctx, int prec, int series, double theta, double delta
 int lpm_tree_eval_dev(lpm_ctx* ctx, int prec, int series, double theta, double delta,
                       const double* tx, const double* ty, const double* tz, int64_t nt,
                       double* ux, double* uy, double* uz, int64_t* mac_count);
+/* Work of the last lpm_tree_eval: accepted (target, node) pairs (far field)
+ * and (target, source) pairs summed at leaves (near field).  Synchronizes. */
+int lpm_tree_last_stats(lpm_ctx* ctx, int64_t* far_pairs, int64_t* near_pairs);
 /* node count, depth (root = 0) and leaf count of the current tree */
 int lpm_tree_info(lpm_ctx* ctx, int64_t* n_nodes, int* depth, int64_t* n_leaves);
 /* Node table in breadth-first order (children of a node are consecutive):
@@ -151,6 +154,36 @@ int lpm_tree_export(lpm_ctx* ctx, double* box, double* cent, int32_t* level, int
  * (nterms(p) = p(p+1)(p+2)/6).  Host pointer.
  * Replaces: SumNode::moments / momentString(), src/LpmTreeSum.h:36-44. */
 int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
+
+/* ------------------------------------------------------------------ time stepping
+ * Velocity solver used by the RK4 step. */
+#define LPM_METHOD_DIRECT 0
+#define LPM_METHOD_TREE 1
+typedef struct lpm_solver_config {
+    int method;              /* LPM_METHOD_DIRECT or LPM_METHOD_TREE */
+    int prec;                /* LPM_F64 or LPM_F32C */
+    int series;              /* tree: LPM_SERIES_CARTESIAN (moments use sign +1) or LPM_SERIES_REFERENCE */
+    int order;               /* tree: series order p, |k| < p, in [1, 12] */
+    int leaf_max;            /* tree: sources per leaf */
+    double theta;            /* tree: multipole acceptance tolerance */
+    double delta;            /* smoothing length (0 = singular kernel) */
+    double root_box_min[3];  /* tree: root box; must contain every particle at every stage */
+    double root_box_max[3];
+} lpm_solver_config;
+/* direct FP64, tree defaults p = 8, leaf 128, theta 0.5, Cartesian series, root box [-1.00101, 1.00101)^3 */
+void lpm_solver_config_default(lpm_solver_config* cfg);
+/* One classical RK4 step of the barotropic vorticity equation on the rotating
+ * unit sphere for n Lagrangian particles (no counterpart in the reference C++,
+ * whose tests only evaluate velocities).  Per stage the relative circulation is
+ * gamma = (zeta_abs - 2 omega z) area, the velocity comes from the configured
+ * solver, and stage/final positions are renormalized to |x| = 1 in FP64.
+ * (x, y, z) are updated in place; zeta_abs and area are conserved.  Device
+ * pointers; enqueued on the context stream. */
+int lpm_rk4_step_dev(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int64_t n,
+                     double* x, double* y, double* z, const double* zeta_abs, const double* area);
+/* nsteps RK4 steps on host arrays (copied in once, out once). */
+int lpm_rk4_steps(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n,
+                  double* x, double* y, double* z, const double* zeta_abs, const double* area);
 
 #ifdef __cplusplus
 }

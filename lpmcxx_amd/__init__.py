@@ -78,6 +78,25 @@ def gpu_lib() -> C.CDLL:
     return _gpu
 
 
+class SolverConfig(C.Structure):
+    """lpm_solver_config (include/lpm_gpu.h): velocity solver of the RK4 step."""
+    _fields_ = [("method", C.c_int), ("prec", C.c_int), ("series", C.c_int), ("order", C.c_int),
+                ("leaf_max", C.c_int), ("theta", C.c_double), ("delta", C.c_double),
+                ("root_box_min", C.c_double * 3), ("root_box_max", C.c_double * 3)]
+
+    @classmethod
+    def default(cls, **kw) -> "SolverConfig":
+        cfg = cls()
+        gpu_lib().lpm_solver_config_default(C.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        return cfg
+
+
+LPM_METHOD_DIRECT = 0
+LPM_METHOD_TREE = 1
+
+
 def _declare_extra(lib: C.CDLL) -> None:
     """Signatures of the tree / RK4 / order-model entry points (when present)."""
     sigs = {
@@ -94,10 +113,16 @@ def _declare_extra(lib: C.CDLL) -> None:
                                C.POINTER(C.c_int)],
         "lpm_tree_export": [_vp] * 9,
         "lpm_tree_export_moments": [_vp, _vp],
+        "lpm_tree_last_stats": [_vp, _i64p, _i64p],
+        "lpm_solver_config_default": [_vp],
+        "lpm_rk4_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, _vp, _vp, _vp, _vp, _vp],
+        "lpm_rk4_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, _vp, _vp, _vp, _vp, _vp],
     }
     for name, args in sigs.items():
         if hasattr(lib, name):
             getattr(lib, name).argtypes = args
+    if hasattr(lib, "lpm_solver_config_default"):
+        lib.lpm_solver_config_default.restype = None
 
 
 def host_lib() -> C.CDLL:
@@ -236,6 +261,26 @@ class Context:
         self.check(self.lib.lpm_direct_sum_dev(self.h, kernels, prec, radius, delta, _ptr(tx), _ptr(ty), _ptr(tz),
                                                tx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(gamma), sx.numel(),
                                                _ptr(skip), _ptr(ux), _ptr(uy), _ptr(uz), _ptr(psi)))
+
+
+def _rk4_steps(self, cfg, dt, omega, nsteps, x, y, z, zeta_abs, area):
+    """nsteps RK4 steps on host arrays; returns new (x, y, z) copies."""
+    x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in (x, y, z))
+    za = np.ascontiguousarray(zeta_abs, dtype=np.float64)
+    ar = np.ascontiguousarray(area, dtype=np.float64)
+    self.check(self.lib.lpm_rk4_steps(self.h, C.byref(cfg), dt, omega, nsteps, x.size, _ptr(x), _ptr(y), _ptr(z),
+                                      _ptr(za), _ptr(ar)))
+    return x, y, z
+
+
+def _rk4_step_dev(self, cfg, dt, omega, x, y, z, zeta_abs, area):
+    """One RK4 step on device tensors, in place, enqueued on the context stream."""
+    self.check(self.lib.lpm_rk4_step_dev(self.h, C.byref(cfg), dt, omega, x.numel(), _ptr(x), _ptr(y), _ptr(z),
+                                         _ptr(zeta_abs), _ptr(area)))
+
+
+Context.rk4_steps = _rk4_steps
+Context.rk4_step_dev = _rk4_step_dev
 
 
 def native_libraries() -> list[str]:

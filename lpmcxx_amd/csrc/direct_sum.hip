@@ -29,11 +29,14 @@
 // error of near-pair separations at the offset scale, not at |x| ~ 1.
 // Plain FP32 sums run inside a tile; tile sums enter a TwoSum-compensated
 // running sum; partials leave the kernel in FP64.
-// The reference zeroes a pair when |R^2 - x.y + delta^2| <= 1e-13, i.e. when
-// |d|^2 + 2 delta^2 <= 2e-13 (this covers the face-target self pair).  Only
-// tiles whose bounding sphere comes within 1e-6 of the wave's targets can hold
-// such pairs; those tiles run a loop with the test, all others the plain
-// 11-instruction pair loop.
+// Tiles whose bounding sphere comes within 1e-6 of the wave's targets ("near"
+// tiles, a fraction of a percent on mesh-ordered inputs) run their pairs in
+// FP64 from an FP64 copy of the tile: d = x - y is then exactly zero for a
+// coincident pair (the face-target self pair), close pairs keep full accuracy,
+// and the reference's zero rule |R^2 - x.y + delta^2| <= 1e-13, i.e.
+// |d|^2 + 2 delta^2 <= 2e-13, applies.  All other tiles run the plain
+// 11-instruction FP32 pair loop.  Inputs whose order is not spatially coherent
+// make every tile near: results stay accurate, at FP64 speed.
 #include "lpm_internal.hpp"
 
 #include <cmath>
@@ -43,7 +46,6 @@ namespace {
 
 constexpr int kTile = 256;          // sources per tile
 constexpr int kWavesPerBlock = 4;   // 256 threads
-constexpr float kZeroTol2 = 2e-13f; // |d|^2 + 2 delta^2 at or below this contributes zero
 constexpr double kNearPad = 1e-6;   // tiles closer than this to a wave run the tested loop
 
 __device__ inline double wave_sum(double v) {
@@ -129,6 +131,7 @@ template <int TPL, bool VEL, bool PSI>
 __global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, const double* __restrict__ ty,
                                                const double* __restrict__ tz, int64_t nt,
                                                const float4* __restrict__ src, const double* __restrict__ scen,
+                                               const double4* __restrict__ src64, double d2x2,
                                                int ntiles, int tiles_per_split, float eps2v,
                                                const int64_t* __restrict__ skip, double* __restrict__ part,
                                                int64_t ntp) {
@@ -214,23 +217,22 @@ __global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, co
                 }
             }
         } else {
+            const double4* s64 = src64 + int64_t(tile) * kTile;
             for (int j = 0; j < kTile; ++j) {
-                const float4 q = s[j];
+                const double4 q = s64[j];
 #pragma unroll
                 for (int k = 0; k < TPL; ++k) {
-                    const float dx = qx[k] - q.x;
-                    const float dy = qy[k] - q.y;
-                    const float dz = qz[k] - q.z;
+                    const double dx = px[k] - q.x, dy = py[k] - q.y, dz = pz[k] - q.z;
+                    const double dd = dx * dx + dy * dy + dz * dz;
                     if (VEL) {
-                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
-                        const float w = (r2 > kZeroTol2) ? q.w * __builtin_amdgcn_rcpf(r2) : 0.f;
-                        ax[k] = fmaf(w, dx, ax[k]);
-                        ay[k] = fmaf(w, dy, ay[k]);
-                        az[k] = fmaf(w, dz, az[k]);
+                        const double r2 = dd + d2x2;
+                        const double w = (r2 > 2e-13) ? q.w / r2 : 0.0;
+                        ax[k] += float(w * dx);
+                        ay[k] += float(w * dy);
+                        az[k] += float(w * dz);
                     }
                     if (PSI) {
-                        const float r2p = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        const float l = q.w * __builtin_amdgcn_logf(r2p);
+                        const float l = float(q.w) * __builtin_amdgcn_logf(float(dd));
                         ap[k] += (jbase + j == sk[k]) ? 0.f : l;
                     }
                 }
@@ -364,10 +366,10 @@ __global__ __launch_bounds__(256) void ds_reduce(const double* __restrict__ part
 
 template <int TPL, bool VEL, bool PSI>
 void launch_f32c(dim3 g, hipStream_t st, const double* tx, const double* ty, const double* tz, int64_t nt,
-                 const float4* src, const double* scen, int ntiles, int tps, float eps2v, const int64_t* skip,
-                 double* part, int64_t ntp) {
-    hipLaunchKernelGGL((ds_f32c<TPL, VEL, PSI>), g, dim3(256), 0, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v,
-                       skip, part, ntp);
+                 const float4* src, const double* scen, const double4* src64, double d2x2, int ntiles, int tps,
+                 float eps2v, const int64_t* skip, double* part, int64_t ntp) {
+    hipLaunchKernelGGL((ds_f32c<TPL, VEL, PSI>), g, dim3(256), 0, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles,
+                       tps, eps2v, skip, part, ntp);
 }
 
 template <int TPL, bool VEL, bool PSI>
@@ -418,16 +420,21 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
 
     const double pi = 3.1415926535897932384626433832795027975;
     if (prec == LPM_F32C) {
-        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * sizeof(float4) + size_t(ntiles) * 4 * sizeof(double)));
-        float4* src = c->ds_src.as<float4>();
+        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * (sizeof(float4) + sizeof(double4)) +
+                                    size_t(ntiles) * 4 * sizeof(double)));
+        double4* src64 = c->ds_src.as<double4>();
+        float4* src = reinterpret_cast<float4*>(src64 + nsp);
         double* scen = reinterpret_cast<double*>(src + nsp);
         hipLaunchKernelGGL(ds_prep_src_f32c, dim3(ntiles), dim3(256), 0, st, sx, sy, sz, gamma, ns, src, scen);
+        hipLaunchKernelGGL(ds_prep_src_f64, dim3(unsigned((nsp + 255) / 256)), dim3(256), 0, st, sx, sy, sz, gamma,
+                           ns, nsp, src64);
+        const double d2x2 = 2.0 * delta * delta;
         if (pot) hipLaunchKernelGGL(ds_gamma_total, dim3(1), dim3(1024), 0, st, gamma, ns, gtot);
         const float eps2v = float(2.0 * delta * delta);
         const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
-        if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
-        else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
-        else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, ntiles, tps, eps2v, skip, part, ntp);
+        if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
+        else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
+        else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         LPM_HIP(c, hipGetLastError());
         hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp, nt,
                            tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),

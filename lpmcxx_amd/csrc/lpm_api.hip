@@ -67,7 +67,8 @@ void lpm_destroy(lpm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     tree_free(c->tree);
     c->tree = nullptr;
-    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out}) b->release();
+    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work})
+        b->release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

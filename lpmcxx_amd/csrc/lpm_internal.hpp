@@ -35,7 +35,10 @@ struct lpm_ctx {
     lpm::DevBuf ds_src, ds_tgt, ds_part, ds_misc;
     // staging for the host-pointer entry points
     lpm::DevBuf h_in, h_out;
+    // RK4 stage storage
+    lpm::DevBuf rk_work;
     lpm::Tree* tree = nullptr;
+    unsigned long long* stats_ptr = nullptr;  // [far pairs, near pairs] of the last tree evaluation (device)
 };
 
 namespace lpm {
@@ -64,5 +67,9 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
 void tree_free(Tree* t);
 int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n,
                       const double* box_min, const double* box_max, int control, int param);
+int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign);
+int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
+                     const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
+                     int64_t* mac_out, double radius);
 
 }  // namespace lpm

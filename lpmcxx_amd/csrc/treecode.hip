@@ -305,6 +305,7 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
                                                  double delta, double R2, double d2, float eps2v, double mult,
                                                  double* __restrict__ ux, double* __restrict__ uy,
                                                  double* __restrict__ uz, unsigned long long* __restrict__ mac_total) {
+    // mac_total[0] += accepted (target, node) pairs, mac_total[1] += near-field (target, source) pairs
     __shared__ int s_node[4][kStack];
     __shared__ uint64_t s_mask[4][kStack];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -314,6 +315,7 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
     const double t0 = valid ? tx[orig] : 0.0, t1 = valid ? ty[orig] : 0.0, t2 = valid ? tz[orig] : 0.0;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     unsigned mac = 0;
+    unsigned long long npairs = 0;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
     int sp = 0;
@@ -350,6 +352,7 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
         if (nk == 0) {
             const bool mine = (rest >> lane) & 1ull;
             const int b = nbeg[n], e = nend[n];
+            npairs += mine ? (unsigned long long)(e - b) : 0ull;
             if (F32NEAR) {
                 const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
                 float a0 = 0.f, a1 = 0.f, a2 = 0.f;
@@ -402,7 +405,11 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
     // wave-reduce the acceptance count, one atomic per wave
     unsigned long long tot = mac;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-    if (lane == 0) atomicAdd(mac_total, tot);
+    for (int o = 32; o > 0; o >>= 1) npairs += __shfl_xor(npairs, o, 64);
+    if (lane == 0) {
+        atomicAdd(mac_total, tot);
+        atomicAdd(mac_total + 1, npairs);
+    }
 }
 
 }  // namespace
@@ -473,12 +480,13 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
     if (nt == 0) { if (mac_out) *mac_out = 0; return LPM_OK; }
     const int64_t na = nt;
     LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + 64));
+    c->stats_ptr = reinterpret_cast<unsigned long long*>(c->ds_tgt.as<char>() + size_t(na) * (16 + 8));
     uint64_t* k_in = c->ds_tgt.as<uint64_t>();
     uint64_t* k_out = k_in + na;
     int* v_in = reinterpret_cast<int*>(k_out + na);
     int* v_out = v_in + na;
     unsigned long long* d_mac = reinterpret_cast<unsigned long long*>(v_out + na);
-    LPM_HIP(c, hipMemsetAsync(d_mac, 0, sizeof(unsigned long long), st));
+    LPM_HIP(c, hipMemsetAsync(d_mac, 0, 2 * sizeof(unsigned long long), st));
     hipLaunchKernelGGL(target_keys, dim3(blocks(nt)), dim3(256), 0, st, tx, ty, tz, nt, t->root_box[0], t->root_box[1],
                        t->root_box[2], t->root_box[3], t->root_box[4], t->root_box[5], k_in, v_in);
     size_t bytes = 0;
@@ -555,6 +563,18 @@ int lpm_tree_moments(lpm_ctx* c, const double* gamma, int p, int sign) {
         LPM_HIP(c, hipMemcpyAsync(c->h_out.ptr, gamma, c->tree->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     LPM_CHECK(tree_moments_device(c, c->h_out.as<double>(), p, sign));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_tree_last_stats(lpm_ctx* c, int64_t* far_pairs, int64_t* near_pairs) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    if (!c->stats_ptr) return fail(c, LPM_ERR_STATE, "no tree evaluation yet");
+    unsigned long long h[2] = {0, 0};
+    LPM_HIP(c, hipSetDevice(c->device));
+    LPM_HIP(c, hipMemcpyAsync(h, c->stats_ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (far_pairs) *far_pairs = int64_t(h[0]);
+    if (near_pairs) *near_pairs = int64_t(h[1]);
     return LPM_OK;
 }
 

@@ -128,20 +128,37 @@ void Tree::buildTree(const TREE_DEPTH_CONTROL type, const index_type param, cons
     if (do_shrink) throw std::invalid_argument("Tree::buildTree: do_shrink is not supported");
     auto c = _crds.lock();
     if (!c) throw std::logic_error("Tree::buildTree: coordinates expired");
-    // root box padded relatively by BOX_PADDING_FACTOR (src/LpmOctree.cpp:20-26)
+    // root box padded relatively by BOX_PADDING_FACTOR (src/LpmOctree.cpp:20-26);
+    // for a negative minimum this moves the bound inward, and such extreme
+    // points stay in the root's list but in no child, as in the reference.
     double lo[3] = {c->minX() - BOX_PADDING_FACTOR * c->minX(), c->minY() - BOX_PADDING_FACTOR * c->minY(),
                     c->minZ() - BOX_PADDING_FACTOR * c->minZ()};
     double hi[3] = {c->maxX() + BOX_PADDING_FACTOR * c->maxX(), c->maxY() + BOX_PADDING_FACTOR * c->maxY(),
                     c->maxZ() + BOX_PADDING_FACTOR * c->maxZ()};
+    const Box3d rootBox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    std::vector<index_type> inside;
+    std::vector<scalar_type> x, y, z;
+    for (index_type i = 0; i < c->n(); ++i) {
+        if (!rootBox.containsPoint(c->getVec(i))) continue;
+        inside.push_back(i);
+        x.push_back(c->x[i]); y.push_back(c->y[i]); z.push_back(c->z[i]);
+    }
     int64_t nn = 0;
     int depth = 0;
-    _gpu->check(lpm_tree_build_box(_gpu->get(), c->x.data(), c->y.data(), c->z.data(), c->n(), lo, hi,
+    _gpu->check(lpm_tree_build_box(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), lo, hi,
                                    type == MAX_COORDS_PER_NODE ? LPM_TREE_MAX_PER_NODE : LPM_TREE_MAX_DEPTH_CONTROL,
                                    int(param), &nn, &depth),
                 "Tree::buildTree");
     _nnodes = nn;
     _depth = depth;
-    importFromGpu(c->n());
+    importFromGpu(index_type(x.size()));
+    for (Node& n : _nodes)
+        for (index_type& j : n.coordsContained) j = inside[j];
+    if (index_type(inside.size()) != c->n()) {
+        std::vector<index_type>& all = _nodes[0].coordsContained;
+        all.resize(c->n());
+        for (index_type i = 0; i < c->n(); ++i) all[i] = i;
+    }
 }
 
 void Tree::importFromGpu(index_type ns) {

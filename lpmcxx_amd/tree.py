@@ -79,6 +79,13 @@ def evaluate_dev(ctx: Context, tx, ty, tz, out, theta: float, delta: float = 0.0
     return mac.value
 
 
+def last_stats(ctx: Context):
+    """(far pairs, near pairs) of the last evaluation."""
+    f, n = C.c_int64(), C.c_int64()
+    ctx.check(ctx.lib.lpm_tree_last_stats(ctx.h, C.byref(f), C.byref(n)))
+    return f.value, n.value
+
+
 def info(ctx: Context):
     nn, depth, nl = C.c_int64(), C.c_int(), C.c_int64()
     ctx.check(ctx.lib.lpm_tree_info(ctx.h, C.byref(nn), C.byref(depth), C.byref(nl)))

@@ -199,3 +199,26 @@ def node_far(c, t, sources, gamma, p, moment_sign, series, delta=0.0):
     lib().oracle_node_far(c[0], c[1], c[2], t[0], t[1], t[2], sx.ctypes.data, sy.ctypes.data, sz.ctypes.data,
                           g.ctypes.data, sx.size, p, moment_sign, series, delta, out.ctypes.data)
     return out
+
+
+def rk4_steps(x, y, z, zeta_abs, area, dt, omega, nsteps):
+    """FP64 RK4 of the Lagrangian vorticity equation with the oracle direct sum:
+    gamma = (zeta_abs - 2 omega z) area per stage, stage and final positions
+    renormalized to the unit sphere.  Returns (x, y, z)."""
+    x, y, z = (np.array(a, dtype=np.float64) for a in (x, y, z))
+
+    def vel(px, py, pz):
+        g = (zeta_abs - 2.0 * omega * pz) * area
+        return np.stack(bs_direct((px, py, pz), (px, py, pz), g))
+
+    def nrm(p):
+        return p / np.sqrt((p * p).sum(0))
+
+    for _ in range(nsteps):
+        x0 = np.stack([x, y, z])
+        k1 = vel(*x0)
+        k2 = vel(*nrm(x0 + 0.5 * dt * k1))
+        k3 = vel(*nrm(x0 + 0.5 * dt * k2))
+        k4 = vel(*nrm(x0 + dt * k3))
+        x, y, z = nrm(x0 + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4))
+    return x, y, z

@@ -62,7 +62,9 @@ int main(int argc, char** argv) {
         for (index_type j : n.coordsContained) CHECK(n.box.containsPoint(crds->getVec(j)));
         covered += n.nCoords();
     }
-    // points at the extremes can fall outside the relatively padded root box
+    // points at the extremes can fall outside the relatively padded root box;
+    // they stay in the root's list only
+    CHECK(covered <= 8000 && covered > 7990 && tree.root().nCoords() == 8000);
     std::printf("octree: %ld nodes, depth %d, %ld of 8000 points in leaves\n", (long)tree.nNodes(), tree.depth(),
                 (long)covered);
     if (argc > 1) tree.writeToVtk(argv[1], "octree of 8000 random points");

@@ -1,0 +1,57 @@
+"""RK4 time step (lpm_rk4_*): parity with the oracle's FP64 RK4 and the
+Rossby-Haurwitz phase speed on the rotating sphere.
+
+For zeta_rel = 2 alpha z + 30 cos(4 lambda) z (z^2 - 1)^2 (solid body plus the
+degree-5 wavenumber-4 Rossby-Haurwitz wave of src/LpmAnalyticFunctions.h:87-120)
+the vorticity pattern of the barotropic vorticity equation rotates rigidly
+with nu = (R(R+3) alpha - 2 Omega) / ((R+1)(R+2)), R = 4 (Haurwitz 1940).
+Absolute vorticity zeta_abs = zeta_rel + 2 Omega z is carried by the particles.
+"""
+import numpy as np
+import pytest
+
+import lpmcxx_amd as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rh(x, y, z, alpha):
+    return 2 * alpha * z + L.sphere_harmonic54(x, y, z)
+
+
+def test_rk4_matches_oracle(ctx, mesh):
+    m = mesh(L.LPM_SEED_ICOS_TRI, 4)
+    omega, alpha, dt = 2 * np.pi, 1.0, 0.01
+    za = _rh(m.cx, m.cy, m.cz, alpha) + 2 * omega * m.cz
+    cfg = L.SolverConfig.default(method=L.LPM_METHOD_DIRECT, prec=L.LPM_F64)
+    g = ctx.rk4_steps(cfg, dt, omega, 2, m.cx, m.cy, m.cz, za, m.area)
+    r = O.rk4_steps(m.cx, m.cy, m.cz, za, m.area, dt, omega, 2)
+    d = np.sqrt(sum((a - b) ** 2 for a, b in zip(g, r)))
+    assert d.max() < 1e-12
+    assert np.abs(np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2) - 1).max() < 1e-15
+    moved = np.sqrt((g[0] - m.cx) ** 2 + (g[1] - m.cy) ** 2 + (g[2] - m.cz) ** 2)
+    assert moved.max() > 1e-3          # particles did move
+
+
+@pytest.mark.parametrize("method", [L.LPM_METHOD_DIRECT, L.LPM_METHOD_TREE])
+def test_rossby_haurwitz_phase_speed(ctx, mesh, method):
+    m = mesh(L.LPM_SEED_CUBED, 6)
+    omega, alpha, dt, nsteps = 2 * np.pi, 1.0, 0.01, 10
+    za = _rh(m.cx, m.cy, m.cz, alpha) + 2 * omega * m.cz
+    cfg = L.SolverConfig.default(method=method, prec=L.LPM_F32C, order=10, theta=0.5, leaf_max=64)
+    x, y, z = ctx.rk4_steps(cfg, dt, omega, nsteps, m.cx, m.cy, m.cz, za, m.area)
+    t = dt * nsteps
+    nu = (28 * alpha - 2 * omega) / 30
+    zeta_now = za - 2 * omega * z                     # relative vorticity carried to time t
+    lam = np.arctan2(y, x)
+    rot = lambda ang: (np.cos(ang) * x + np.sin(ang) * y, -np.sin(ang) * x + np.cos(ang) * y, z)
+    exact = _rh(*rot(nu * t), alpha)                  # pattern rotated by nu t
+    frozen = _rh(x, y, z, alpha)                      # pattern not moving
+    wrong = _rh(*rot(-nu * t), alpha)                 # moving the wrong way
+    e = np.linalg.norm(zeta_now - exact) / np.linalg.norm(exact)
+    e0 = np.linalg.norm(zeta_now - frozen) / np.linalg.norm(exact)
+    e1 = np.linalg.norm(zeta_now - wrong) / np.linalg.norm(exact)
+    print(f"\nmethod {method}: rel err vs propagated pattern {e:.3e}, vs frozen {e0:.3e}, vs reversed {e1:.3e}")
+    assert e < 0.02 and e0 > 5 * e and e1 > 5 * e
+    assert np.abs(np.sqrt(x * x + y * y + z * z) - 1).max() < 1e-14
