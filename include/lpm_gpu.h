@@ -155,6 +155,92 @@ int lpm_tree_export(lpm_ctx* ctx, double* box, double* cent, int32_t* level, int
  * Replaces: SumNode::moments / momentString(), src/LpmTreeSum.h:36-44. */
 int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
 
+/* Counters of the last tree evaluation (any of the lpm_tree_eval* calls):
+ * out[0] far-field (target, node) pairs, out[1] near-field (target, source)
+ * pairs, out[2] far-field series terms summed over the far pairs
+ * (p(p+1)(p+2)/6 each), out[3] order-model evaluations.  Synchronizes.
+ * No counterpart in the reference (its macCounter is out[0] of lpm_tree_eval). */
+#define LPM_STAT_FAR 0
+#define LPM_STAT_NEAR 1
+#define LPM_STAT_TERMS 2
+#define LPM_STAT_MODEL 3
+int lpm_tree_stats(lpm_ctx* ctx, int64_t* out);
+
+/* ------------------------------------------------------------------ batch treecode
+ * The treecode velocity of lpm_tree_eval with the acceptance decided per
+ * batch of 64 Morton-sorted targets: node n is accepted for the batch when
+ * maxRadius_n + r_b < theta |c_b - c_n| (c_b, r_b: centroid and radius of the
+ * batch), with the Cartesian series of order `order` (<= the moments' order,
+ * moment_sign +1).  prec LPM_F64: FP64 throughout, reference pair formula in
+ * the near field; LPM_F32C: FP32 far field from FP32 moments and FP32
+ * near field with |d|^2/2 denominators, FP64 per-target sums.  The result
+ * differs from lpm_tree_eval only by the series truncation of the pairs each
+ * accepts.  Replaces: TreeSum::meshSolve src/LpmTreeSum.h:67 (same role,
+ * batch acceptance instead of the per-target criterion of
+ * src/LpmTreeSum.cpp:22-24). */
+int lpm_tree_eval_batch(lpm_ctx* ctx, int prec, int order, double theta, double delta,
+                        const double* tx, const double* ty, const double* tz, int64_t nt,
+                        double* ux, double* uy, double* uz);
+int lpm_tree_eval_batch_dev(lpm_ctx* ctx, int prec, int order, double theta, double delta,
+                            const double* tx, const double* ty, const double* tz, int64_t nt,
+                            double* ux, double* uy, double* uz);
+
+/* ------------------------------------------------------------------ learned order and acceptance
+ * No counterpart in the reference.  The order model is a 16-64-64-16 ReLU
+ * perceptron (10 used inputs, 13 classes) with BF16 weights and FP32
+ * accumulation on the MFMA units; class 0 = descend, class q = accept with
+ * order q.  Parameters are a flat FP32 array of LPM_MODEL_NPARAMS values:
+ * W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16] (row = input).
+ * Inputs are normalized by (x - mean) * inv_std (16 values each). */
+#define LPM_MODEL_NPARAMS 6288
+#define LPM_MODEL_INPUTS 16
+#define LPM_MODEL_CLASSES 13
+#define LPM_LABEL_ORDERS 12
+/* He-initialized weights from `seed`, zero biases, identity normalization. */
+int lpm_model_init(lpm_ctx* ctx, uint64_t seed);
+/* Set the FP32 master parameters and the normalization (mean / inv_std may be
+ * NULL: 0 / 1); resets the Adam state. */
+int lpm_model_set_params(lpm_ctx* ctx, const float* params, const float* mean, const float* inv_std);
+int lpm_model_get_params(lpm_ctx* ctx, float* params, float* mean, float* inv_std);
+/* Gradient of the mean softmax cross entropy over n rows (x[n][16] raw
+ * inputs, y[n] classes) with respect to the parameters, computed by the
+ * forward and backward MFMA kernels; loss may be NULL. */
+int lpm_model_gradient(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n, float* grad, float* loss);
+/* Adam training (beta 0.9 / 0.999, eps 1e-8) for `epochs` shuffled passes in
+ * mini-batches of `batch` rows; final_loss (may be NULL) is the mean loss of
+ * the last epoch. */
+int lpm_model_train(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n, int epochs, int batch,
+                    float lr, uint64_t seed, float* final_loss);
+/* Logits [n][16] and argmax classes [n] (either may be NULL). */
+int lpm_model_forward(lpm_ctx* ctx, const float* x, int64_t n, float* logits, int32_t* cls);
+/* Training data from the FP64 treecode: for every batch_stride-th target
+ * batch the wave walks the tree; a node passing maxRadius_n + r_b <
+ * theta_max |c_b - c_n| is a candidate pair when at most frontier_levels of
+ * its ancestors passed the same test (frontier_levels < 0: any), and a
+ * candidate is kept with probability keep_rate (times 65536 / particles for
+ * larger nodes).  For each kept pair, features[16] and errors[12] are
+ * written: the model inputs and, for orders 1..12, the largest error over the
+ * batch's targets of the far-field velocity against the exact sum over the
+ * node's particles, divided by u_ref (|du| / u_ref); ids (may be NULL)
+ * receives [n][2] = (batch index, node index of lpm_tree_export).  Needs
+ * moments of order >= 12 with moment_sign +1.  Host pointers; n_samples
+ * receives the number written (<= max_samples). */
+int lpm_tree_labels(lpm_ctx* ctx, double theta_max, double delta, double u_ref, int64_t batch_stride,
+                    double keep_rate, int frontier_levels, uint64_t seed, int64_t max_samples,
+                    const double* tx, const double* ty, const double* tz, int64_t nt,
+                    float* features, float* errors, int32_t* ids, int64_t* n_samples);
+/* Batch treecode with the order model deciding every pair that passes
+ * maxRadius_n + r_b < theta_max |c_b - c_n|: class 0 (or a class above
+ * max_order) descends, class q accepts with order min(q + margin, max_order).
+ * u_ref scales the moment feature (the value used for the labels).  Needs
+ * moments of order >= max_order (<= 12) with moment_sign +1 and a model. */
+int lpm_tree_eval_learned(lpm_ctx* ctx, int prec, double theta_max, int margin, int max_order, double u_ref,
+                          double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
+                          double* ux, double* uy, double* uz);
+int lpm_tree_eval_learned_dev(lpm_ctx* ctx, int prec, double theta_max, int margin, int max_order,
+                              double u_ref, double delta, const double* tx, const double* ty,
+                              const double* tz, int64_t nt, double* ux, double* uy, double* uz);
+
 /* ------------------------------------------------------------------ time stepping
  * Velocity solver used by the RK4 step. */
 #define LPM_METHOD_DIRECT 0

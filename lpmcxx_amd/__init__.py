@@ -114,6 +114,23 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_tree_export": [_vp] * 9,
         "lpm_tree_export_moments": [_vp, _vp],
         "lpm_tree_last_stats": [_vp, _i64p, _i64p],
+        "lpm_tree_stats": [_vp, _vp],
+        "lpm_tree_eval_batch": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp,
+                                _vp, _vp],
+        "lpm_tree_eval_batch_dev": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64,
+                                    _vp, _vp, _vp],
+        "lpm_tree_eval_learned": [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp,
+                                  _vp, C.c_int64, _vp, _vp, _vp],
+        "lpm_tree_eval_learned_dev": [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, _vp,
+                                      _vp, _vp, C.c_int64, _vp, _vp, _vp],
+        "lpm_tree_labels": [_vp, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_int, C.c_uint64,
+                            C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _i64p],
+        "lpm_model_init": [_vp, C.c_uint64],
+        "lpm_model_set_params": [_vp, _vp, _vp, _vp],
+        "lpm_model_get_params": [_vp, _vp, _vp, _vp],
+        "lpm_model_gradient": [_vp, _vp, _vp, C.c_int64, _vp, _vp],
+        "lpm_model_train": [_vp, _vp, _vp, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint64, _vp],
+        "lpm_model_forward": [_vp, _vp, C.c_int64, _vp, _vp],
         "lpm_solver_config_default": [_vp],
         "lpm_rk4_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, _vp, _vp, _vp, _vp, _vp],
         "lpm_rk4_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, _vp, _vp, _vp, _vp, _vp],

@@ -66,6 +66,7 @@ void lpm_destroy(lpm_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     tree_free(c->tree);
+    model_free(c->model);
     c->tree = nullptr;
     for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work})
         b->release();

@@ -23,7 +23,12 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
-struct Tree;  // octree state (octree.hip)
+struct Tree;   // octree state (octree.hip)
+struct Model;  // order model state (mlp.hip)
+
+// counters of the last tree evaluation: far (target, node) pairs, near
+// (target, source) pairs, far-field series terms, order-model calls
+constexpr int kStats = 4;
 
 }  // namespace lpm
 
@@ -38,7 +43,8 @@ struct lpm_ctx {
     // RK4 stage storage
     lpm::DevBuf rk_work;
     lpm::Tree* tree = nullptr;
-    unsigned long long* stats_ptr = nullptr;  // [far pairs, near pairs] of the last tree evaluation (device)
+    lpm::Model* model = nullptr;
+    unsigned long long* stats_ptr = nullptr;  // kStats counters of the last tree evaluation (device)
 };
 
 namespace lpm {
@@ -71,5 +77,20 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign);
 int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);
+// Morton order of the targets in the tree's root box (tperm: sorted position
+// -> target index, device); also clears the evaluation counters.
+int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                          int** tperm);
+// batch-MAC treecode (treecode_batch.hip): prec LPM_F64, or LPM_F32C with
+// FP32 far field and FP32C near field; order <= tree order
+int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
+                           const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz);
+int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
+                             double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
+                             double* ux, double* uy, double* uz);
+
+void model_free(Model* m);
+const unsigned short* model_weights(lpm_ctx* c);  // device BF16 weights + FP32 biases, or nullptr
+const float* model_norm(lpm_ctx* c, int which);   // host feature mean (0) / inverse std (1)
 
 }  // namespace lpm

@@ -223,7 +223,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
 void tree_free(Tree* t) {
     if (!t) return;
     for (DevBuf* b : {&t->perm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
-                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->mom, &t->mom32, &t->soff32, &t->tmp0,
+                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->mom, &t->momg, &t->mom32, &t->soff32, &t->nstat, &t->tmp0,
                       &t->tmp1, &t->tmp2, &t->tmp3, &t->sort_tmp})
         b->release();
     delete t;

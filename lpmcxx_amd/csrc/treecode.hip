@@ -31,6 +31,8 @@
 //                  LPM_F32C uses FP32 offsets from the leaf centroid,
 //                  |d|^2/2 denominators and an FP32 per-leaf sum.
 #include "lpm_tree.hpp"
+#include "lpm_mlp.hpp"
+#include "lpm_series.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -44,128 +46,8 @@ namespace {
 
 constexpr int kChunk = 128;
 constexpr int kStack = 160;   // >= 7 * kMaxDepth + 1
-__constant__ double c_rinv[kMaxOrder + 1] = {0.0,       1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
-                                             1.0 / 6,   1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
-                                             1.0 / 12,  1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16};
 
 inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
-
-// Lex index of the multi-index (a, b, c), |k| < P: terms with first index < a,
-// then with first index a and second < b, then c.
-__host__ __device__ constexpr int lex_index(int P, int a, int b, int c) {
-    int idx = 0;
-    for (int i = 0; i < a; ++i) idx += (P - i) * (P - i + 1) / 2;
-    for (int j = 0; j < b; ++j) idx += P - a - j;
-    return idx + c;
-}
-
-// Cartesian far field: with d = t - c and R2 = |d|^2 + 2 delta^2 the
-// coefficients of 2 / (|d - h|^2 + 2 delta^2) = sum_k b_k h^k satisfy
-// b_0 = 2/R2 and b_k = (2 sum_i d_i b_{k-e_i} - sum_i b_{k-2e_i}) / R2.
-// Computed degree shell by degree shell (only the last two shells are live)
-// and contracted with the node moments m_k = sum_j gamma_j y_j h_j^k:
-// S = sum_k b_k m_k.  Converges for maxRadius / |d| < 1.
-// Graded (degree-by-degree) visiting order of the multi-indices |k| < P with
-// the lex positions of each term and of its k - e_i and k - 2 e_i neighbours,
-// built at compile time so every coefficient lives in a named register.
-template <int P>
-struct CartTab {
-    static constexpr int NT = P * (P + 1) * (P + 2) / 6;
-    int id[NT] = {};
-    int p1[NT][3] = {};
-    int p2[NT][3] = {};
-    constexpr CartTab() {
-        int pos = 0;
-        for (int n = 0; n < P; ++n)
-            for (int k1 = n; k1 >= 0; --k1)
-                for (int k2 = n - k1; k2 >= 0; --k2) {
-                    const int k3 = n - k1 - k2;
-                    id[pos] = lex_index(P, k1, k2, k3);
-                    p1[pos][0] = k1 >= 1 ? lex_index(P, k1 - 1, k2, k3) : -1;
-                    p1[pos][1] = k2 >= 1 ? lex_index(P, k1, k2 - 1, k3) : -1;
-                    p1[pos][2] = k3 >= 1 ? lex_index(P, k1, k2, k3 - 1) : -1;
-                    p2[pos][0] = k1 >= 2 ? lex_index(P, k1 - 2, k2, k3) : -1;
-                    p2[pos][1] = k2 >= 2 ? lex_index(P, k1, k2 - 2, k3) : -1;
-                    p2[pos][2] = k3 >= 2 ? lex_index(P, k1, k2, k3 - 2) : -1;
-                    ++pos;
-                }
-    }
-};
-template <int P>
-inline constexpr CartTab<P> kCartTab{};
-
-template <int P, int I>
-__device__ __forceinline__ void cart_step(double* b, double e0, double e1, double e2, double iR2,
-                                          const double* __restrict__ mm, double& S0, double& S1, double& S2) {
-    constexpr int id = kCartTab<P>.id[I];
-    constexpr int a0 = kCartTab<P>.p1[I][0], a1 = kCartTab<P>.p1[I][1], a2 = kCartTab<P>.p1[I][2];
-    constexpr int c0 = kCartTab<P>.p2[I][0], c1 = kCartTab<P>.p2[I][1], c2 = kCartTab<P>.p2[I][2];
-    double s = 0.0;
-    if constexpr (a0 >= 0) s = fma(e0, b[a0], s);
-    if constexpr (a1 >= 0) s = fma(e1, b[a1], s);
-    if constexpr (a2 >= 0) s = fma(e2, b[a2], s);
-    if constexpr (c0 >= 0) s -= b[c0];
-    if constexpr (c1 >= 0) s -= b[c1];
-    if constexpr (c2 >= 0) s -= b[c2];
-    b[id] = s * iR2;
-    S0 = fma(b[id], mm[3 * id + 0], S0);
-    S1 = fma(b[id], mm[3 * id + 1], S1);
-    S2 = fma(b[id], mm[3 * id + 2], S2);
-}
-
-template <int P, int... I>
-__device__ __forceinline__ void cart_all(std::integer_sequence<int, I...>, double* b, double e0, double e1,
-                                         double e2, double iR2, const double* __restrict__ mm, double& S0,
-                                         double& S1, double& S2) {
-    (cart_step<P, I + 1>(b, e0, e1, e2, iR2, mm, S0, S1, S2), ...);
-}
-
-// Cartesian far field: with d = t - c and R2 = |d|^2 + 2 delta^2 the
-// coefficients of 2 / (|d - h|^2 + 2 delta^2) = sum_k b_k h^k satisfy
-// b_0 = 2/R2 and b_k = (2 sum_i d_i b_{k-e_i} - sum_i b_{k-2e_i}) / R2
-// (from Q grad F = -F grad Q for F = 1/Q).  Computed degree shell by degree
-// shell and contracted with the node moments m_k = sum_j gamma_j y_j h_j^k:
-// S = sum_k b_k m_k.  Converges for maxRadius / |d| < 1.
-template <int P>
-__device__ __forceinline__ void far_cartesian(double d0, double d1, double d2, double delta2x2,
-                                              const double* __restrict__ mm, double& S0, double& S1, double& S2) {
-    constexpr int NT = CartTab<P>::NT;
-    const double R2 = d0 * d0 + d1 * d1 + d2 * d2 + delta2x2;
-    const double iR2 = 1.0 / R2;
-    double b[NT];
-    b[0] = 2.0 * iR2;
-    S0 = b[0] * mm[0];
-    S1 = b[0] * mm[1];
-    S2 = b[0] * mm[2];
-    cart_all<P>(std::make_integer_sequence<int, NT - 1>{}, b, 2.0 * d0, 2.0 * d1, 2.0 * d2, iR2, mm, S0, S1, S2);
-}
-
-// Reference far field (SumNode::calc_coeffs, src/LpmTreeSum.cpp:149-171):
-// coefficients of 1/(1 + delta^2 - t.c - t.h) streamed in lex order.
-__device__ __forceinline__ void far_reference(int p, double t0, double t1, double t2, double cx, double cy,
-                                              double cz, double d2, const double* __restrict__ mm, double& S0,
-                                              double& S1, double& S2) {
-    const double D = 1.0 + d2 - (t0 * cx + t1 * cy + t2 * cz);
-    const double invD = 1.0 / D;
-    S0 = S1 = S2 = 0.0;
-    double A1 = invD;
-    int idx = 0;
-    for (int k1 = 0; k1 < p; ++k1) {
-        if (k1 > 0) A1 = ((double(k1) * invD) * t0) * A1 * c_rinv[k1];
-        double A2 = A1;
-        for (int k2 = 0; k2 < p - k1; ++k2) {
-            if (k2 > 0) A2 = ((double(k1 + k2) * invD) * t1) * A2 * c_rinv[k2];
-            double A3 = A2;
-            for (int k3 = 0; k3 < p - k1 - k2; ++k3) {
-                if (k3 > 0) A3 = ((double(k1 + k2 + k3) * invD) * t2) * A3 * c_rinv[k3];
-                S0 = fma(A3, mm[3 * idx + 0], S0);
-                S1 = fma(A3, mm[3 * idx + 1], S1);
-                S2 = fma(A3, mm[3 * idx + 2], S2);
-                ++idx;
-            }
-        }
-    }
-}
 
 // ---------------------------------------------------------------- moments
 __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict__ chunk_node,
@@ -229,10 +111,18 @@ __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, const int* __restrict__ node_chunk0,
+// graded position of the lex-order term r of order p
+__device__ inline int lex_to_graded(int p, int r) {
+    int a = 0, b = 0;
+    while (r >= (p - a) * (p - a + 1) / 2) { r -= (p - a) * (p - a + 1) / 2; ++a; }
+    while (r >= p - a - b) { r -= p - a - b; ++b; }
+    return graded_index(a, b, r);
+}
+
+__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int p, int nterm, const int* __restrict__ node_chunk0,
                                                            const int* __restrict__ node_nchunk,
                                                            const double* __restrict__ part, double* __restrict__ mom,
-                                                           float* __restrict__ mom32) {
+                                                           double* __restrict__ momg, float* __restrict__ mom32) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= int64_t(nn) * nterm * 3) return;
     const int node = int(i / (int64_t(nterm) * 3));
@@ -241,7 +131,25 @@ __global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, co
     double s = 0.0;
     for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * 3 + rem];
     mom[i] = s;
-    mom32[i] = float(s);
+    const int64_t g = int64_t(node) * nterm * 3 + 3 * lex_to_graded(p, rem / 3) + rem % 3;
+    momg[g] = s;
+    mom32[g] = float(s);
+}
+
+// Norms of the degree-0, 1 and 2 moment shells (graded positions 0, 1..3,
+// 4..9) and the particle count of every node.
+__global__ __launch_bounds__(256) void tree_node_stats(int nn, int p, int nterm, const double* __restrict__ momg,
+                                                       const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                       float4* __restrict__ nstat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const double* m = momg + int64_t(i) * nterm * 3;
+    double sh[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < nterm && t < 10; ++t) {
+        const int d = t == 0 ? 0 : (t < 4 ? 1 : 2);
+        sh[d] += m[3 * t] * m[3 * t] + m[3 * t + 1] * m[3 * t + 1] + m[3 * t + 2] * m[3 * t + 2];
+    }
+    nstat[i] = make_float4(float(sqrt(sh[0])), float(sqrt(sh[1])), float(sqrt(sh[2])), float(nend[i] - nbeg[i]));
 }
 
 __global__ __launch_bounds__(256) void tree_set_gamma(const double* __restrict__ gamma, const int* __restrict__ perm,
@@ -301,7 +209,8 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
                                                  const int* __restrict__ nfirst, const int* __restrict__ nkids,
                                                  const int* __restrict__ nbeg, const int* __restrict__ nend,
                                                  const double4* __restrict__ spos, const float4* __restrict__ soff,
-                                                 const double* __restrict__ mom, int p, int nterm, double theta,
+                                                 const double* __restrict__ mom, const double* __restrict__ momg,
+                                                 int p, int nterm, double theta,
                                                  double delta, double R2, double d2, float eps2v, double mult,
                                                  double* __restrict__ ux, double* __restrict__ uy,
                                                  double* __restrict__ uz, unsigned long long* __restrict__ mac_total) {
@@ -336,10 +245,10 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
         const uint64_t am = __ballot(acc);
         if (am) {
             if (acc) {
-                const double* mm = mom + int64_t(n) * nterm * 3;
                 double S0, S1, S2;
-                if (P == 0) far_reference(p, t0, t1, t2, c.x, c.y, c.z, d2, mm, S0, S1, S2);
-                else far_cartesian<(P > 0 ? P : 1)>(t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2, mm, S0, S1, S2);
+                if (P == 0) far_reference(p, t0, t1, t2, c.x, c.y, c.z, d2, mom + int64_t(n) * nterm * 3, S0, S1, S2);
+                else far_cartesian<(P > 0 ? P : 1)>(t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2,
+                                                    momg + int64_t(n) * nterm * 3, S0, S1, S2);
                 s0 += t1 * S2 - t2 * S1;
                 s1 += t2 * S0 - t0 * S2;
                 s2 += t0 * S1 - t1 * S0;
@@ -450,8 +359,10 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     LPM_HIP(c, hipMemcpyAsync(d_nc, nc.data(), nn * sizeof(int), hipMemcpyHostToDevice, st));
     LPM_CHECK(t->tmp2.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * 3 * sizeof(double)));
     LPM_CHECK(t->mom.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
+    LPM_CHECK(t->momg.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
     LPM_CHECK(t->mom32.reserve(size_t(nn) * nterm * 3 * sizeof(float)));
     LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
+    LPM_CHECK(t->nstat.reserve(size_t(nn) * sizeof(float4)));
     if (t->ns > 0) {
         hipLaunchKernelGGL(tree_set_gamma, dim3(blocks(t->ns)), dim3(256), 0, st, gamma, t->perm.as<int>(), t->ns,
                            t->spos.as<double4>());
@@ -461,8 +372,10 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
         hipLaunchKernelGGL(tree_moments_chunk, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
                            t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->tmp2.as<double>());
     }
-    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), nterm, d_n0, d_nc,
-                       t->tmp2.as<double>(), t->mom.as<double>(), t->mom32.as<float>());
+    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm, d_n0,
+                       d_nc, t->tmp2.as<double>(), t->mom.as<double>(), t->momg.as<double>(), t->mom32.as<float>());
+    hipLaunchKernelGGL(tree_node_stats, dim3(blocks(nn)), dim3(256), 0, st, int(nn), p, nterm, t->momg.as<double>(),
+                       t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
                        t->soff32.as<float4>());
@@ -472,21 +385,20 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     return LPM_OK;
 }
 
-int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
-                     const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
-                     int64_t* mac_out, double radius) {
+int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                          int** tperm) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
-    if (nt == 0) { if (mac_out) *mac_out = 0; return LPM_OK; }
-    const int64_t na = nt;
+    const int64_t na = std::max<int64_t>(nt, 1);
     LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + 64));
-    c->stats_ptr = reinterpret_cast<unsigned long long*>(c->ds_tgt.as<char>() + size_t(na) * (16 + 8));
     uint64_t* k_in = c->ds_tgt.as<uint64_t>();
     uint64_t* k_out = k_in + na;
     int* v_in = reinterpret_cast<int*>(k_out + na);
     int* v_out = v_in + na;
-    unsigned long long* d_mac = reinterpret_cast<unsigned long long*>(v_out + na);
-    LPM_HIP(c, hipMemsetAsync(d_mac, 0, 2 * sizeof(unsigned long long), st));
+    c->stats_ptr = reinterpret_cast<unsigned long long*>(v_out + na);
+    LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
+    *tperm = v_out;
+    if (nt == 0) return LPM_OK;
     hipLaunchKernelGGL(target_keys, dim3(blocks(nt)), dim3(256), 0, st, tx, ty, tz, nt, t->root_box[0], t->root_box[1],
                        t->root_box[2], t->root_box[3], t->root_box[4], t->root_box[5], k_in, v_in);
     size_t bytes = 0;
@@ -495,6 +407,18 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
     LPM_CHECK(t->sort_tmp.reserve(bytes + 256));
     LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0,
                                                   3 * kMaxDepth, st));
+    return LPM_OK;
+}
+
+int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
+                     const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
+                     int64_t* mac_out, double radius) {
+    Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    int* v_out = nullptr;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &v_out));
+    if (nt == 0) { if (mac_out) *mac_out = 0; return LPM_OK; }
+    unsigned long long* d_mac = c->stats_ptr;
     const double pi = 3.1415926535897932384626433832795027975;
     const double mult = -1.0 / (4.0 * pi * radius);
     const float eps2v = float(2.0 * delta * delta);
@@ -503,7 +427,7 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tx, ty, tz, v_out, nt, t->ncent.as<double4>(),
                            t->nfirst.as<int>(), t->nkids.as<int>(), t->nbeg.as<int>(), t->nend.as<int>(),
-                           t->spos.as<double4>(), t->soff32.as<float4>(), t->mom.as<double>(), p, nterm, theta, delta,
+                           t->spos.as<double4>(), t->soff32.as<float4>(), t->mom.as<double>(), t->momg.as<double>(), p, nterm, theta, delta,
                            radius * radius, delta * delta, eps2v, mult, ux, uy, uz, d_mac);
     };
     if (series == LPM_SERIES_REFERENCE) {

@@ -1,0 +1,684 @@
+// Batch-MAC treecode and the learned order / acceptance (the TreeSum
+// evaluation of src/LpmTreeSum.cpp:232-311, reorganised for wave64).
+//
+// A target batch is 64 consecutive Morton-sorted targets, one wave.  The wave
+// walks the node table depth-first with a wave-uniform stack, and every
+// decision is taken for the whole batch (c_b, r_b: centroid and radius of the
+// batch's targets; c_n, r_n: centroid and maxRadius of the node):
+//   plain    accept when r_n + r_b < theta |c_b - c_n|; every lane then
+//            evaluates the order-p Cartesian far field.  Otherwise descend,
+//            and at a leaf every lane sums the leaf's particles, which all
+//            lanes read at one address (scalar loads, one broadcast per
+//            source).  Because no lane idles, the far and near loops run at
+//            full SIMD width, unlike the per-target MAC of the reference
+//            (which tree_eval keeps for parity).
+//   learned  pairs with r_n + r_b < theta_max |c_b - c_n| go through the order
+//            model on the MFMA units (mlp_classify_row): class 0 descends,
+//            class q accepts with order min(q + margin, max_order).
+//   labels   for a sample of batches the wave visits every node; for a sample
+//            of the pairs that pass theta_max it records the ten features and
+//            the error of the order-1..12 far field against the exact sum over
+//            the node's particles (max over the batch, in units of u_ref).
+// Precision: LPM_F64 evaluates everything in FP64 (reference pair formula in
+// the near field).  LPM_F32C evaluates the far field in FP32 from FP32
+// moments (graded order) with d = t - c_n rounded once, and the near field in
+// FP32 from offsets to the leaf centroid with |d|^2 / 2 for 1 - x.y; the
+// per-node results are added to FP64 per-target sums.
+#include "lpm_mlp.hpp"
+#include "lpm_series.hpp"
+#include "lpm_tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <utility>
+
+namespace lpm {
+namespace {
+
+constexpr int kStackB = 160;  // >= 7 * kMaxDepth + 1
+constexpr int kLabelOrder = 12;
+constexpr int kMaxLearnedOrder = 12;
+
+__device__ inline double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// true in every lane when cond holds in lane 0 (cond is wave-uniform by construction)
+__device__ inline bool uniform(bool cond) { return (__ballot(cond) & 1ull) != 0; }
+
+struct Batch {
+    double cx, cy, cz, r;
+    int nvalid;
+};
+
+// Centroid and radius of the valid targets of the wave.  The butterfly
+// reductions leave identical values in every lane.
+__device__ inline Batch batch_of(bool valid, double t0, double t1, double t2) {
+    Batch b;
+    b.nvalid = __popcll(__ballot(valid));
+    const double inv = 1.0 / double(b.nvalid);
+    b.cx = wsum(valid ? t0 : 0.0) * inv;
+    b.cy = wsum(valid ? t1 : 0.0) * inv;
+    b.cz = wsum(valid ? t2 : 0.0) * inv;
+    const double dx = t0 - b.cx, dy = t1 - b.cy, dz = t2 - b.cz;
+    b.r = sqrt(wmax(valid ? dx * dx + dy * dy + dz * dz : 0.0));
+    return b;
+}
+
+// Ten features of a (batch, node) pair, wave-uniform:
+//   log2(r_n/s), log2(r_b/s), (r_n + r_b)/s, log2(M0/(u_ref s)),
+//   log2(M1/(M0 r_n)), log2(M2/(M0 r_n^2)), log2(npart), nvalid/64, log2 s,
+//   leaf flag, with s = |c_b - c_n| and M0, M1, M2 the moment shell norms.
+__device__ inline void pair_features(float* f, double rn, double rb, double s, float4 st, bool leaf, int nvalid,
+                                     double u_ref) {
+    const double floor_log = -100.0;
+    auto lg = [&](double v) { return v > 0.0 ? fmax(log2(v), floor_log) : floor_log; };
+    const double m0 = st.x, m1 = st.y, m2 = st.z;
+    f[0] = float(lg(fmax(rn, 1e-12 * s) / s));
+    f[1] = float(lg(fmax(rb, 1e-12 * s) / s));
+    f[2] = float((rn + rb) / s);
+    f[3] = float(lg(m0 / (u_ref * s)));
+    f[4] = float(m0 * rn > 0.0 ? lg(m1 / (m0 * rn)) : floor_log);
+    f[5] = float(m0 * rn > 0.0 ? lg(m2 / (m0 * rn * rn)) : floor_log);
+    f[6] = float(lg(double(st.w)));
+    f[7] = float(nvalid) / 64.f;
+    f[8] = float(lg(s));
+    f[9] = leaf ? 1.f : 0.f;
+#pragma unroll
+    for (int k = kFeatures; k < kIn; ++k) f[k] = 0.f;
+}
+
+template <typename T, int... Ps>
+__device__ __forceinline__ void far_switch(std::integer_sequence<int, Ps...>, int p, T d0, T d1, T d2, T dd,
+                                           const T* __restrict__ mm, T& S0, T& S1, T& S2) {
+    ((p == Ps + 1 ? (far_cartesian<Ps + 1>(d0, d1, d2, dd, mm, S0, S1, S2), 0) : 0), ...);
+}
+
+// Far field of node n at every lane's target: t x S, added to (s0, s1, s2).
+template <int P, bool F32, bool RUNTIME_P>
+__device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, double t1, double t2, const double4& c,
+                                         double d2, const double* __restrict__ momg,
+                                         const float* __restrict__ mom32, double& s0, double& s1, double& s2) {
+    double S0, S1, S2;
+    if constexpr (F32) {
+        float A0 = 0.f, A1 = 0.f, A2 = 0.f;
+        const float e0 = float(t0 - c.x), e1 = float(t1 - c.y), e2 = float(t2 - c.z), dd = float(2.0 * d2);
+        const float* mm = mom32 + int64_t(n) * nterm * 3;
+        if constexpr (RUNTIME_P) far_switch<float>(std::make_integer_sequence<int, P>{}, p, e0, e1, e2, dd, mm, A0, A1, A2);
+        else far_cartesian<P>(e0, e1, e2, dd, mm, A0, A1, A2);
+        S0 = A0; S1 = A1; S2 = A2;
+    } else {
+        S0 = S1 = S2 = 0.0;
+        const double* mm = momg + int64_t(n) * nterm * 3;
+        if constexpr (RUNTIME_P)
+            far_switch<double>(std::make_integer_sequence<int, P>{}, p, t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2, mm, S0,
+                               S1, S2);
+        else far_cartesian<P>(t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2, mm, S0, S1, S2);
+    }
+    s0 += t1 * S2 - t2 * S1;
+    s1 += t2 * S0 - t0 * S2;
+    s2 += t0 * S1 - t1 * S0;
+}
+
+// Leaf [b, e) summed directly at every lane's target; the sources are
+// wave-uniform loads.
+template <bool F32>
+__device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, double t2, const double4& c, double d2,
+                                          float eps2v, const double4* __restrict__ spos,
+                                          const float4* __restrict__ soff, double& s0, double& s1, double& s2) {
+    if constexpr (F32) {
+        const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 4
+        for (int j = b; j < e; ++j) {
+            const float4 o = soff[j];
+            const float dx = q0 - o.x, dy = q1 - o.y, dz = q2 - o.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
+            // the reference's |den| <= 1e-13 rule: |d|^2 + 2 delta^2 <= 2e-13 contributes 0
+            const float wgt = (r2 > 2e-13f) ? o.w * __builtin_amdgcn_rcpf(r2) : 0.f;
+            a0 = fmaf(wgt, dx, a0);
+            a1 = fmaf(wgt, dy, a1);
+            a2 = fmaf(wgt, dz, a2);
+        }
+        // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2)
+        const double b0 = a0, b1 = a1, b2 = a2;
+        s0 -= 2.0 * (t1 * b2 - t2 * b1);
+        s1 -= 2.0 * (t2 * b0 - t0 * b2);
+        s2 -= 2.0 * (t0 * b1 - t1 * b0);
+    } else {
+        for (int j = b; j < e; ++j) {
+            const double4 s = spos[j];
+            const double cx = t1 * s.z - t2 * s.y;
+            const double cy = t2 * s.x - t0 * s.z;
+            const double cz = t0 * s.y - t1 * s.x;
+            const double den = 1.0 - (t0 * s.x + t1 * s.y + t2 * s.z) + d2;
+            if (fabs(den) > 1.0e-13) {
+                const double inv = 1.0 / den;
+                s0 += (cx * inv) * s.w;
+                s1 += (cy * inv) * s.w;
+                s2 += (cz * inv) * s.w;
+            }
+        }
+    }
+}
+
+struct TreeView {
+    const double4* ncent;
+    const int *nfirst, *nkids, *nbeg, *nend;
+    const float4* nstat;
+    const double4* spos;
+    const float4* soff;
+    const double* momg;
+    const float* mom32;
+    int nterm;
+};
+
+struct Targets {
+    const double *x, *y, *z;
+    const int* perm;
+    int64_t n;
+};
+
+__device__ inline void load_target(const Targets& tg, int64_t it, bool& valid, int& orig, double& t0, double& t1,
+                                   double& t2) {
+    valid = it < tg.n;
+    orig = valid ? tg.perm[it] : 0;
+    t0 = valid ? tg.x[orig] : 0.0;
+    t1 = valid ? tg.y[orig] : 0.0;
+    t2 = valid ? tg.z[orig] : 0.0;
+}
+
+__device__ inline void flush_stats(unsigned long long* stats, unsigned long long far, unsigned long long near,
+                                   unsigned long long terms, unsigned long long calls) {
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(stats + 0, far);
+        atomicAdd(stats + 1, near);
+        atomicAdd(stats + 2, terms);
+        atomicAdd(stats + 3, calls);
+    }
+}
+
+// ---------------------------------------------------------------- plain batch treecode
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, double theta, double d2, float eps2v,
+                                                       double mult, double* __restrict__ ux, double* __restrict__ uy,
+                                                       double* __restrict__ uz,
+                                                       unsigned long long* __restrict__ stats) {
+    __shared__ int s_node[4][kStackB];
+    const int w = threadIdx.x >> 6;
+    const int64_t it = (int64_t(blockIdx.x) * 4 + w) * 64 + (threadIdx.x & 63);
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    load_target(tg, it, valid, orig, t0, t1, t2);
+    if (__ballot(valid) == 0) return;
+    const Batch bt = batch_of(valid, t0, t1, t2);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    unsigned long long nfar = 0, nnear = 0;
+    constexpr unsigned long long NT = unsigned(P * (P + 1) * (P + 2) / 6);
+    int sp = 1;
+    s_node[w][0] = 0;
+    while (sp > 0) {
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][--sp]);
+        const double4 c = tv.ncent[n];
+        const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
+        const double s = sqrt(dx * dx + dy * dy + dz * dz);
+        if (uniform(c.w + bt.r < theta * s)) {
+            far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2);
+            ++nfar;
+            continue;
+        }
+        const int nk = tv.nkids[n];
+        if (nk == 0) {
+            const int b = tv.nbeg[n], e = tv.nend[n];
+            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
+            nnear += unsigned(e - b);
+        } else {
+            const int f = tv.nfirst[n];
+            for (int q = nk - 1; q >= 0; --q) s_node[w][sp++] = f + q;
+        }
+    }
+    if (valid) {
+        ux[orig] = s0 * mult;
+        uy[orig] = s1 * mult;
+        uz[orig] = s2 * mult;
+    }
+    const unsigned long long nv = unsigned(bt.nvalid);
+    flush_stats(stats, nfar * nv, nnear * nv, nfar * nv * NT, 0ull);
+}
+
+// ---------------------------------------------------------------- learned batch treecode
+template <int MAXP, bool F32>
+__global__ __launch_bounds__(256) void tree_eval_learned(Targets tg, TreeView tv, double theta_max, int margin,
+                                                         int max_order, double u_ref, double d2, float eps2v,
+                                                         double mult, const unsigned short* __restrict__ wts,
+                                                         MlpNorm norm, double* __restrict__ ux,
+                                                         double* __restrict__ uy, double* __restrict__ uz,
+                                                         unsigned long long* __restrict__ stats) {
+    __shared__ int s_node[4][kStackB];
+    __shared__ float s_h[4][144];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t it = (int64_t(blockIdx.x) * 4 + w) * 64 + lane;
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    load_target(tg, it, valid, orig, t0, t1, t2);
+    if (__ballot(valid) == 0) return;
+    const Batch bt = batch_of(valid, t0, t1, t2);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    unsigned long long nfar = 0, nnear = 0, nterms = 0, ncalls = 0;
+    int sp = 1;
+    s_node[w][0] = 0;
+    while (sp > 0) {
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][--sp]);
+        const double4 c = tv.ncent[n];
+        const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
+        const double s = sqrt(dx * dx + dy * dy + dz * dz);
+        const int nk = tv.nkids[n];
+        if (uniform(c.w + bt.r < theta_max * s)) {
+            float f[kIn];
+            pair_features(f, c.w, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
+            float v = 0.f;
+#pragma unroll
+            for (int k = 0; k < kIn; ++k)
+                if (lane == k) v = (f[k] - norm.mean[k]) * norm.inv_std[k];
+            if (lane < kIn) s_h[w][lane] = v;
+            const int cls = mlp_classify_row(wts, s_h[w]);
+            ++ncalls;
+            if (cls > 0 && cls <= max_order) {
+                const int p = min(cls + margin, max_order);
+                far_node<MAXP, F32, true>(p, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2);
+                ++nfar;
+                nterms += unsigned(p * (p + 1) * (p + 2) / 6);
+                continue;
+            }
+        }
+        if (nk == 0) {
+            const int b = tv.nbeg[n], e = tv.nend[n];
+            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
+            nnear += unsigned(e - b);
+        } else {
+            const int f = tv.nfirst[n];
+            for (int q = nk - 1; q >= 0; --q) s_node[w][sp++] = f + q;
+        }
+    }
+    if (valid) {
+        ux[orig] = s0 * mult;
+        uy[orig] = s1 * mult;
+        uz[orig] = s2 * mult;
+    }
+    const unsigned long long nv = unsigned(bt.nvalid);
+    flush_stats(stats, nfar * nv, nnear * nv, nterms * nv, ncalls);
+}
+
+// ---------------------------------------------------------------- labels
+__device__ inline uint32_t mix32(uint32_t h) {  // MurmurHash3 finalizer
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView tv, int64_t nsel, int64_t bstride,
+                                                          double theta_max, double d2, double u_ref,
+                                                          uint32_t keep_threshold, int big, uint32_t seed, int frontier,
+                                                          int64_t max_samples, unsigned long long* __restrict__ counter,
+                                                          float* __restrict__ feat_out, float* __restrict__ err_out,
+                                                          int* __restrict__ id_out) {
+    __shared__ int s_node[4][kStackB];
+    __shared__ int s_prop[4][kStackB];  // proposed strict ancestors of the entry
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t sel = int64_t(blockIdx.x) * 4 + w;
+    if (sel >= nsel) return;
+    const int64_t bi = sel * bstride;
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    load_target(tg, bi * 64 + lane, valid, orig, t0, t1, t2);
+    if (__ballot(valid) == 0) return;
+    const Batch bt = batch_of(valid, t0, t1, t2);
+    const double inv = 1.0 / (4.0 * 3.14159265358979323846 * u_ref);
+    int sp = 1;
+    s_node[w][0] = 0;
+    s_prop[w][0] = 0;
+    while (sp > 0) {
+        --sp;
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
+        const int a = __builtin_amdgcn_readfirstlane(s_prop[w][sp]);
+        const double4 c = tv.ncent[n];
+        const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
+        const double s = sqrt(dx * dx + dy * dy + dz * dz);
+        const int nk = tv.nkids[n];
+        const int b = tv.nbeg[n], e = tv.nend[n];
+        const bool prop = uniform(c.w + bt.r < theta_max * s);
+        if (prop && (frontier < 0 || a <= frontier)) {
+            const uint32_t h = mix32(uint32_t(bi) * 0x9E3779B1u ^ mix32(uint32_t(n) + seed));
+            const uint32_t h2 = mix32(h ^ 0x27d4eb2fu);
+            const bool keep = h <= keep_threshold && (e - b <= big || int(h2 % uint32_t(e - b)) < big);
+            if (keep) {
+                unsigned long long slot = 0;
+                if (lane == 0) slot = atomicAdd(counter, 1ull);
+                slot = __shfl(slot, 0, 64);
+                if (int64_t(slot) < max_samples) {
+                    float f[kIn];
+                    pair_features(f, c.w, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
+                    if (lane == 0) {
+                        for (int k = 0; k < kIn; ++k) feat_out[slot * kIn + k] = f[k];
+                        id_out[2 * slot] = int(bi);
+                        id_out[2 * slot + 1] = n;
+                    }
+                    // exact S = sum_j gamma_j y_j / (1 - t.y_j + delta^2) over the node
+                    double E0 = 0.0, E1 = 0.0, E2 = 0.0;
+                    for (int j = b; j < e; ++j) {
+                        const double4 y = tv.spos[j];
+                        const double den = 1.0 - (t0 * y.x + t1 * y.y + t2 * y.z) + d2;
+                        if (fabs(den) > 1.0e-13) {
+                            const double g = y.w / den;
+                            E0 = fma(g, y.x, E0);
+                            E1 = fma(g, y.y, E1);
+                            E2 = fma(g, y.z, E2);
+                        }
+                    }
+                    double err[kLabelOrder];
+                    auto on_shell = [&](int q, double S0, double S1, double S2) {
+                        const double a0 = S0 - E0, a1 = S1 - E1, a2 = S2 - E2;
+                        const double v0 = t1 * a2 - t2 * a1, v1 = t2 * a0 - t0 * a2, v2 = t0 * a1 - t1 * a0;
+                        err[q] = sqrt(v0 * v0 + v1 * v1 + v2 * v2) * inv;
+                    };
+                    far_cartesian_shells<kLabelOrder>(t0 - c.x, t1 - c.y, t2 - c.z, 2.0 * d2,
+                                                      tv.momg + int64_t(n) * tv.nterm * 3, on_shell);
+#pragma unroll
+                    for (int q = 0; q < kLabelOrder; ++q) {
+                        const double m = wmax(valid ? err[q] : 0.0);
+                        if (lane == 0) err_out[slot * kLabelOrder + q] = float(fmin(m, 3.0e38));
+                    }
+                }
+            }
+        }
+        const int a2 = a + (prop ? 1 : 0);
+        if (nk > 0 && (frontier < 0 || a2 <= frontier)) {
+            const int f = tv.nfirst[n];
+            for (int q = nk - 1; q >= 0; --q) {
+                s_node[w][sp] = f + q;
+                s_prop[w][sp] = a2;
+                ++sp;
+            }
+        }
+    }
+}
+
+TreeView view(const Tree* t) {
+    TreeView v;
+    v.ncent = t->ncent.as<double4>();
+    v.nfirst = t->nfirst.as<int>();
+    v.nkids = t->nkids.as<int>();
+    v.nbeg = t->nbeg.as<int>();
+    v.nend = t->nend.as<int>();
+    v.nstat = t->nstat.as<float4>();
+    v.spos = t->spos.as<double4>();
+    v.soff = t->soff32.as<float4>();
+    v.momg = t->momg.as<double>();
+    v.mom32 = t->mom32.as<float>();
+    v.nterm = nterms(t->order);
+    return v;
+}
+
+constexpr double kPi = 3.1415926535897932384626433832795027975;
+
+}  // namespace
+
+int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
+                           const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
+    Tree* t = c->tree;
+    int* tperm = nullptr;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
+    if (nt == 0) return LPM_OK;
+    const Targets tg{tx, ty, tz, tperm, nt};
+    const TreeView tv = view(t);
+    const unsigned nb = unsigned((nt + 255) / 256);
+    const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);
+    const float eps2v = float(2.0 * d2);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, tg, tv, theta, d2, eps2v, mult, ux, uy, uz,
+                           c->stats_ptr);
+    };
+    const bool f32 = prec == LPM_F32C;
+    switch (order) {
+#define LPM_CASE(PP)                                  \
+    case PP:                                          \
+        if (f32) go(tree_eval_batch<PP, true>);       \
+        else go(tree_eval_batch<PP, false>);          \
+        break;
+        LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
+        LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
+#undef LPM_CASE
+        default: return fail(c, LPM_ERR_INVALID_ARG, "series order %d unsupported", order);
+    }
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
+}
+
+int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
+                             double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
+                             double* ux, double* uy, double* uz) {
+    Tree* t = c->tree;
+    int* tperm = nullptr;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
+    if (nt == 0) return LPM_OK;
+    const Targets tg{tx, ty, tz, tperm, nt};
+    const TreeView tv = view(t);
+    MlpNorm norm;
+    for (int k = 0; k < kIn; ++k) {
+        norm.mean[k] = model_norm(c, 0)[k];
+        norm.inv_std[k] = model_norm(c, 1)[k];
+    }
+    const unsigned nb = unsigned((nt + 255) / 256);
+    const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);
+    const float eps2v = float(2.0 * d2);
+    const unsigned short* w = model_weights(c);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, tg, tv, theta_max, margin, max_order, u_ref, d2,
+                           eps2v, mult, w, norm, ux, uy, uz, c->stats_ptr);
+    };
+    const bool f32 = prec == LPM_F32C;
+    if (max_order <= 8) {
+        if (f32) go(tree_eval_learned<8, true>);
+        else go(tree_eval_learned<8, false>);
+    } else if (max_order <= 10) {
+        if (f32) go(tree_eval_learned<10, true>);
+        else go(tree_eval_learned<10, false>);
+    } else {
+        if (f32) go(tree_eval_learned<12, true>);
+        else go(tree_eval_learned<12, false>);
+    }
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
+}
+
+static int tree_labels_device(lpm_ctx* c, double theta_max, double delta, double u_ref, int64_t bstride,
+                              double keep_rate, int frontier, uint32_t seed, int64_t max_samples, const double* tx,
+                              const double* ty, const double* tz, int64_t nt, float* feat, float* err, int* ids,
+                              int64_t* n_out) {
+    Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    int* tperm = nullptr;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
+    const Targets tg{tx, ty, tz, tperm, nt};
+    const TreeView tv = view(t);
+    const int64_t nbatch = (nt + 63) / 64;
+    const int64_t nsel = (nbatch + bstride - 1) / bstride;
+    // the first evaluation counter serves as the sample counter
+    unsigned long long* counter = c->stats_ptr;
+    const uint32_t thr = keep_rate >= 1.0 ? 0xFFFFFFFFu : uint32_t(keep_rate * 4294967295.0);
+    hipLaunchKernelGGL(tree_labels_kernel, dim3(unsigned((nsel + 3) / 4)), dim3(256), 0, st, tg, tv, nsel, bstride,
+                       theta_max, delta * delta, u_ref, thr, 1 << 16, seed, frontier, max_samples, counter, feat, err,
+                       ids);
+    LPM_HIP(c, hipGetLastError());
+    unsigned long long h = 0;
+    LPM_HIP(c, hipMemcpyAsync(&h, counter, sizeof(h), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    *n_out = std::min<int64_t>(int64_t(h), max_samples);
+    return LPM_OK;
+}
+
+}  // namespace lpm
+
+using namespace lpm;
+
+namespace {
+
+static int check_targets(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, double* ux,
+                         double* uy, double* uz) {
+    if (nt < 0 || nt > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid target count");
+    if (nt > 0 && (!tx || !ty || !tz || !ux || !uy || !uz)) return fail(c, LPM_ERR_NULL_PTR, "NULL target/output");
+    return LPM_OK;
+}
+
+static int check_tree_for_batch(lpm_ctx* c, int order) {
+    if (!c->tree) return fail(c, LPM_ERR_STATE, "no tree (lpm_tree_build first)");
+    if (c->tree->order < order) return fail(c, LPM_ERR_STATE, "moments of order >= %d needed", order);
+    if (c->tree->moment_sign != 1) return fail(c, LPM_ERR_STATE, "the Cartesian series needs moment_sign +1");
+    return LPM_OK;
+}
+
+// Host-pointer wrapper: stage the targets, run `fn` on device copies, copy the velocity back.
+template <class F>
+static int with_staged(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, double* ux,
+                       double* uy, double* uz, F fn) {
+    hipStream_t st = c->stream;
+    LPM_CHECK(c->h_in.reserve(6 * size_t(nt) * sizeof(double)));
+    double* d = c->h_in.as<double>();
+    LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_CHECK(fn(d, d + nt, d + 2 * nt, d + 3 * nt, d + 4 * nt, d + 5 * nt));
+    LPM_HIP(c, hipMemcpyAsync(ux, d + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(uy, d + 4 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(uz, d + 5 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    return LPM_OK;
+}
+
+static int check_batch(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
+                       const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
+    if (order < 1 || order > kMaxLearnedOrder) return fail(c, LPM_ERR_INVALID_ARG, "order %d unsupported", order);
+    if (!(theta >= 0.0) || !(delta >= 0.0)) return fail(c, LPM_ERR_INVALID_ARG, "theta and delta must be >= 0");
+    LPM_CHECK(check_tree_for_batch(c, order));
+    return check_targets(c, tx, ty, tz, nt, ux, uy, uz);
+}
+
+static int check_learned(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
+                         double delta, const double* tx, const double* ty, const double* tz, int64_t nt, double* ux,
+                         double* uy, double* uz) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
+    if (max_order < 1 || max_order > kMaxLearnedOrder)
+        return fail(c, LPM_ERR_INVALID_ARG, "max_order must be in 1..%d", kMaxLearnedOrder);
+    if (margin < 0 || !(u_ref > 0.0) || !(delta >= 0.0) || !(theta_max >= 0.0) || !std::isfinite(theta_max))
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid learned-evaluation arguments");
+    LPM_CHECK(check_tree_for_batch(c, max_order));
+    if (!model_weights(c)) return fail(c, LPM_ERR_STATE, "no order model (lpm_model_init or lpm_model_set_params)");
+    return check_targets(c, tx, ty, tz, nt, ux, uy, uz);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_tree_eval_batch_dev(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
+                            const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
+    LPM_CHECK(check_batch(c, prec, order, theta, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_eval_batch_device(c, prec, order, theta, delta, tx, ty, tz, nt, ux, uy, uz);
+}
+
+int lpm_tree_eval_batch(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
+                        const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
+    LPM_CHECK(check_batch(c, prec, order, theta, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    if (nt == 0) return tree_eval_batch_device(c, prec, order, theta, delta, tx, ty, tz, 0, ux, uy, uz);
+    return with_staged(c, tx, ty, tz, nt, ux, uy, uz,
+                       [&](const double* x, const double* y, const double* z, double* u, double* v, double* w) {
+                           return tree_eval_batch_device(c, prec, order, theta, delta, x, y, z, nt, u, v, w);
+                       });
+}
+
+int lpm_tree_eval_learned_dev(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
+                              double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
+                              double* ux, double* uy, double* uz) {
+    LPM_CHECK(check_learned(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, nt, ux, uy, uz);
+}
+
+int lpm_tree_eval_learned(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
+                          double delta, const double* tx, const double* ty, const double* tz, int64_t nt, double* ux,
+                          double* uy, double* uz) {
+    LPM_CHECK(check_learned(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, nt, ux, uy, uz));
+    LPM_HIP(c, hipSetDevice(c->device));
+    if (nt == 0)
+        return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, 0, ux, uy, uz);
+    return with_staged(c, tx, ty, tz, nt, ux, uy, uz,
+                       [&](const double* x, const double* y, const double* z, double* u, double* v, double* w) {
+                           return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, x, y,
+                                                           z, nt, u, v, w);
+                       });
+}
+
+int lpm_tree_labels(lpm_ctx* c, double theta_max, double delta, double u_ref, int64_t batch_stride,
+                    double keep_rate, int frontier_levels, uint64_t seed, int64_t max_samples, const double* tx, const double* ty,
+                    const double* tz, int64_t nt, float* features, float* errors, int32_t* ids,
+                    int64_t* n_samples) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    LPM_CHECK(check_tree_for_batch(c, kLabelOrder));
+    if (!tx || !ty || !tz || !features || !errors || !n_samples) return fail(c, LPM_ERR_NULL_PTR, "NULL argument");
+    if (nt < 1 || nt > (int64_t(1) << 30) || batch_stride < 1 || max_samples < 1 || !(keep_rate > 0.0) ||
+        !(u_ref > 0.0) || !(delta >= 0.0) || !(theta_max > 0.0))
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid label arguments");
+    LPM_HIP(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    LPM_CHECK(c->h_in.reserve(3 * size_t(nt) * sizeof(double)));
+    LPM_CHECK(c->h_out.reserve(size_t(max_samples) * (kIn + kLabelOrder + 2) * sizeof(float)));
+    double* d = c->h_in.as<double>();
+    float* df = c->h_out.as<float>();
+    float* de = df + size_t(max_samples) * kIn;
+    int* di = reinterpret_cast<int*>(de + size_t(max_samples) * kLabelOrder);
+    LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    int64_t ns = 0;
+    LPM_CHECK(tree_labels_device(c, theta_max, delta, u_ref, batch_stride, keep_rate, frontier_levels,
+                                 uint32_t(seed ^ (seed >> 32)),
+                                 max_samples, d, d + nt, d + 2 * nt, nt, df, de, di, &ns));
+    if (ns > 0) {
+        LPM_HIP(c, hipMemcpyAsync(features, df, size_t(ns) * kIn * sizeof(float), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(errors, de, size_t(ns) * kLabelOrder * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (ids) LPM_HIP(c, hipMemcpyAsync(ids, di, size_t(ns) * 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    }
+    LPM_HIP(c, hipStreamSynchronize(st));
+    *n_samples = ns;
+    return LPM_OK;
+}
+
+int lpm_tree_stats(lpm_ctx* c, int64_t* out) {
+    if (!c || !out) return LPM_ERR_NULL_PTR;
+    if (!c->stats_ptr) return fail(c, LPM_ERR_STATE, "no tree evaluation yet");
+    unsigned long long h[kStats];
+    LPM_HIP(c, hipMemcpyAsync(h, c->stats_ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < kStats; ++k) out[k] = int64_t(h[k]);
+    return LPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+"""Learned Taylor order and acceptance for the batch treecode.
+
+Python access to the order-model entry points of ``include/lpm_gpu.h``
+(``lpm_model_*``, ``lpm_tree_labels``, ``lpm_tree_eval_learned``) and the
+training pipeline built on them:
+
+1. ``labels``: the FP64 treecode scores every order p = 1..12 on a sample of
+   (target batch, node) pairs against the exact sum over the node;
+2. ``order_labels``: class = lowest order meeting the per-pair error bound
+   ``tau`` (0 = descend), with leaves kept in the near field when that is
+   cheaper than the series;
+3. ``OrderModel.train``: Adam on the GPU through the forward/backward MFMA
+   kernels;
+4. ``evaluate``: the batch treecode with the model choosing p and acceptance.
+
+Nothing here has a counterpart in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import LPM_F32C, LPM_F64, Context, _ptr
+
+NPARAMS = 6288
+NIN = 16
+NCLASSES = 13
+NORDERS = 12
+NFEATURES = 10
+FEATURE_NAMES = ("log2 r_n/s", "log2 r_b/s", "(r_n+r_b)/s", "log2 M0/(u_ref s)", "log2 M1/(M0 r_n)",
+                 "log2 M2/(M0 r_n^2)", "log2 npart", "nvalid/64", "log2 s", "leaf")
+# parameter layout (row = input): W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16]
+LAYOUT = (("W1", (16, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 16)), ("b3", (16,)))
+
+
+def unpack(params: np.ndarray) -> dict:
+    out, o = {}, 0
+    for name, shape in LAYOUT:
+        n = int(np.prod(shape))
+        out[name] = params[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def pack(parts: dict) -> np.ndarray:
+    return np.concatenate([np.asarray(parts[name], np.float32).ravel() for name, _ in LAYOUT]).astype(np.float32)
+
+
+def nterms(p) -> np.ndarray:
+    p = np.asarray(p, np.int64)
+    return p * (p + 1) * (p + 2) // 6
+
+
+class OrderModel:
+    """The order model of a context (BF16 MFMA kernels, FP32 master weights)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def init(self, seed: int = 1) -> None:
+        self.ctx.check(self.ctx.lib.lpm_model_init(self.ctx.h, seed))
+
+    def set_params(self, params, mean=None, inv_std=None) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == NPARAMS
+        m = None if mean is None else np.ascontiguousarray(mean, np.float32)
+        s = None if inv_std is None else np.ascontiguousarray(inv_std, np.float32)
+        self.ctx.check(self.ctx.lib.lpm_model_set_params(self.ctx.h, _ptr(p), _ptr(m), _ptr(s)))
+
+    def get_params(self):
+        p = np.empty(NPARAMS, np.float32)
+        m, s = np.empty(NIN, np.float32), np.empty(NIN, np.float32)
+        self.ctx.check(self.ctx.lib.lpm_model_get_params(self.ctx.h, _ptr(p), _ptr(m), _ptr(s)))
+        return p, m, s
+
+    def gradient(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        g = np.empty(NPARAMS, np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.ctx.lib.lpm_model_gradient(self.ctx.h, _ptr(x), _ptr(y), len(y), _ptr(g),
+                                                       C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, y, epochs: int = 10, batch: int = 4096, lr: float = 2e-3, seed: int = 1) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        loss = C.c_float()
+        self.ctx.check(self.ctx.lib.lpm_model_train(self.ctx.h, _ptr(x), _ptr(y), len(y), epochs, batch,
+                                                    C.c_float(lr), seed, C.byref(loss)))
+        return loss.value
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n = x.shape[0]
+        logits = np.empty((n, NIN), np.float32)
+        cls = np.empty(n, np.int32)
+        self.ctx.check(self.ctx.lib.lpm_model_forward(self.ctx.h, _ptr(x), n, _ptr(logits), _ptr(cls)))
+        return logits, cls
+
+
+@dataclass
+class LabelSet:
+    features: np.ndarray  # [n][16] raw model inputs
+    errors: np.ndarray    # [n][12] |du| / u_ref of the order-1..12 far field (max over the batch)
+    ids: np.ndarray       # [n][2] (batch, node)
+
+
+def labels(ctx: Context, targets, theta_max: float, u_ref: float, delta: float = 0.0, batch_stride: int = 1,
+           keep_rate: float = 1.0, frontier_levels: int = 2, seed: int = 1, max_samples: int = 1 << 20) -> LabelSet:
+    """Training data from the FP64 treecode (lpm_tree_labels); the current tree
+    needs moments of order >= 12 with moment_sign +1.  frontier_levels limits
+    the candidates to the first levels at and below the first node of each
+    branch that passes theta_max: the pairs the learned traversal meets."""
+    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    f = np.empty((max_samples, NIN), np.float32)
+    e = np.empty((max_samples, NORDERS), np.float32)
+    ids = np.empty((max_samples, 2), np.int32)
+    n = C.c_int64()
+    ctx.check(ctx.lib.lpm_tree_labels(ctx.h, theta_max, delta, u_ref, batch_stride, keep_rate, frontier_levels,
+                                      seed, max_samples,
+                                      _ptr(tx), _ptr(ty), _ptr(tz), tx.size, _ptr(f), _ptr(e), _ptr(ids),
+                                      C.byref(n)))
+    k = n.value
+    return LabelSet(f[:k].copy(), e[:k].copy(), ids[:k].copy())
+
+
+# relative cost of one far-field series term and of one near-field pair
+# (vector instructions per lane: ~9 for a term, ~11 for a pair)
+FAR_TERM_COST = 9.0
+NEAR_PAIR_COST = 11.0
+
+
+def order_labels(ls: LabelSet, tau: float, max_order: int) -> np.ndarray:
+    """Class of every sample: the lowest order p <= max_order whose error is at
+    most tau, or 0 (descend) when there is none or, at a leaf, when the
+    series would cost more than summing the leaf directly."""
+    ok = ls.errors[:, :max_order] <= tau
+    has = ok.any(axis=1)
+    p = np.where(has, ok.argmax(axis=1) + 1, 0)
+    leaf = ls.features[:, 9] > 0.5
+    npart = np.exp2(ls.features[:, 6])
+    dearer = leaf & (FAR_TERM_COST * nterms(p) >= NEAR_PAIR_COST * npart)
+    return np.where(dearer, 0, p).astype(np.int32)
+
+
+def normalization(features: np.ndarray):
+    """Per-input mean and inverse standard deviation (padded inputs: 0 / 1)."""
+    mean = np.zeros(NIN, np.float32)
+    inv = np.ones(NIN, np.float32)
+    x = features[:, :NFEATURES].astype(np.float64)
+    mean[:NFEATURES] = x.mean(axis=0)
+    sd = x.std(axis=0)
+    inv[:NFEATURES] = 1.0 / np.maximum(sd, 1e-3)
+    return mean, inv
+
+
+def evaluate(ctx: Context, targets, theta_max: float, margin: int, max_order: int, u_ref: float,
+             delta: float = 0.0, prec: int = LPM_F32C):
+    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    n = tx.size
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    ctx.check(ctx.lib.lpm_tree_eval_learned(ctx.h, prec, theta_max, margin, max_order, u_ref, delta, _ptr(tx),
+                                            _ptr(ty), _ptr(tz), n, _ptr(ux), _ptr(uy), _ptr(uz)))
+    return ux, uy, uz
+
+
+def evaluate_dev(ctx: Context, tx, ty, tz, out, theta_max: float, margin: int, max_order: int, u_ref: float,
+                 delta: float = 0.0, prec: int = LPM_F32C) -> None:
+    ctx.check(ctx.lib.lpm_tree_eval_learned_dev(ctx.h, prec, theta_max, margin, max_order, u_ref, delta, _ptr(tx),
+                                                _ptr(ty), _ptr(tz), tx.numel(), _ptr(out[0]), _ptr(out[1]),
+                                                _ptr(out[2])))
+
+
+def constant_model(ctx: Context, cls: int) -> None:
+    """A model that answers `cls` for every input (zero weights, one large
+    output bias): the learned traversal then reduces to the plain one."""
+    parts = {name: np.zeros(shape, np.float32) for name, shape in LAYOUT}
+    parts["b3"][cls] = 10.0
+    OrderModel(ctx).set_params(pack(parts))
+
+
+def u_ref_of(gamma) -> float:
+    """Velocity scale sum |gamma| / (4 pi) used by the features and labels."""
+    return float(np.abs(np.asarray(gamma, np.float64)).sum() / (4.0 * np.pi))
+
+
+@dataclass
+class FitResult:
+    n_samples: int
+    class_counts: np.ndarray
+    final_loss: float
+    accuracy: float
+    mean: np.ndarray
+    inv_std: np.ndarray
+
+
+def fit(ctx: Context, ls: LabelSet, tau: float, max_order: int, *, epochs: int = 20, batch: int = 4096,
+        lr: float = 2e-3, seed: int = 1) -> FitResult:
+    """Train the context's order model on a label set for the bound tau."""
+    y = order_labels(ls, tau, max_order)
+    mean, inv = normalization(ls.features)
+    m = OrderModel(ctx)
+    m.init(seed)
+    p, _, _ = m.get_params()
+    m.set_params(p, mean, inv)
+    loss = m.train(ls.features, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    _, cls = m.forward(ls.features)
+    return FitResult(len(y), np.bincount(y, minlength=NCLASSES), loss, float((cls == y).mean()), mean, inv)

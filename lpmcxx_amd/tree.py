@@ -86,6 +86,34 @@ def last_stats(ctx: Context):
     return f.value, n.value
 
 
+LPM_STAT_FAR, LPM_STAT_NEAR, LPM_STAT_TERMS, LPM_STAT_MODEL = 0, 1, 2, 3
+
+
+def stats(ctx: Context) -> dict:
+    """Counters of the last evaluation: far pairs, near pairs, far-field series
+    terms, order-model calls."""
+    out = np.zeros(4, np.int64)
+    ctx.check(ctx.lib.lpm_tree_stats(ctx.h, _ptr(out)))
+    return {"far": int(out[0]), "near": int(out[1]), "terms": int(out[2]), "model": int(out[3])}
+
+
+def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float = 0.0, prec: int = LPM_F64):
+    """Batch-acceptance treecode (lpm_tree_eval_batch): one decision per 64
+    Morton-sorted targets, Cartesian series of the given order."""
+    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    n = tx.size
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    ctx.check(ctx.lib.lpm_tree_eval_batch(ctx.h, prec, order, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), n,
+                                          _ptr(ux), _ptr(uy), _ptr(uz)))
+    return ux, uy, uz
+
+
+def evaluate_batch_dev(ctx: Context, tx, ty, tz, out, theta: float, order: int, delta: float = 0.0,
+                       prec: int = LPM_F64) -> None:
+    ctx.check(ctx.lib.lpm_tree_eval_batch_dev(ctx.h, prec, order, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz),
+                                              tx.numel(), _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
+
+
 def info(ctx: Context):
     nn, depth, nl = C.c_int64(), C.c_int(), C.c_int64()
     ctx.check(ctx.lib.lpm_tree_info(ctx.h, C.byref(nn), C.byref(depth), C.byref(nl)))

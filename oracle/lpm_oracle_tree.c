@@ -514,3 +514,242 @@ void oracle_node_far(double cx, double cy, double cz, double tx, double ty, doub
     out3[0] = sol[0]; out3[1] = sol[1]; out3[2] = sol[2];
     free(coef); free(mom); free(ks); free(succ); free(pred); free(id);
 }
+
+/* ---------------------------------------------------------------- batch acceptance
+ * Batch-MAC treecode (no counterpart in the reference; the specification of
+ * lpm_tree_eval_batch in include/lpm_gpu.h).  Targets are ordered by 63-bit
+ * Morton keys of 21 bisections of the root box (clamped into it), ties by
+ * index; batches are 64 consecutive targets.  Batch centroid and radius are
+ * formed with the xor-butterfly summation order of a 64-lane wave reduction
+ * so the acceptance decisions reproduce the GPU's exactly. */
+typedef struct {
+    uint64_t key;
+    int64_t idx;
+} keyed;
+
+static int cmp_keyed(const void* a, const void* b) {
+    const keyed *x = (const keyed*)a, *y = (const keyed*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+/* Morton order of the targets in the current tree's root box. */
+void oracle_target_order(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order) {
+    keyed* k = (keyed*)malloc(sizeof(keyed) * (n > 0 ? n : 1));
+    const onode* r = &g_nodes[0];
+    for (int64_t i = 0; i < n; ++i) {
+        const double p[3] = {fmin(fmax(tx[i], r->lo[0]), r->hi[0]), fmin(fmax(ty[i], r->lo[1]), r->hi[1]),
+                             fmin(fmax(tz[i], r->lo[2]), r->hi[2])};
+        double lo[3] = {r->lo[0], r->lo[1], r->lo[2]}, hi[3] = {r->hi[0], r->hi[1], r->hi[2]};
+        uint64_t key = 0;
+        for (int l = 0; l < 21; ++l) {
+            int bits = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double m = 0.5 * (lo[d] + hi[d]);
+                const int b = p[d] >= m;
+                if (b) lo[d] = m; else hi[d] = m;
+                bits |= b << d;
+            }
+            key = (key << 3) | (uint64_t)bits;
+        }
+        k[i].key = key;
+        k[i].idx = i;
+    }
+    qsort(k, (size_t)n, sizeof(keyed), cmp_keyed);
+    for (int64_t i = 0; i < n; ++i) order[i] = k[i].idx;
+    free(k);
+}
+
+static double butterfly(double* v, int is_max) {
+    double w[64];
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int l = 0; l < 64; ++l) w[l] = is_max ? fmax(v[l], v[l ^ o]) : v[l] + v[l ^ o];
+        memcpy(v, w, sizeof(w));
+    }
+    return v[0];
+}
+
+/* Centroid (c[0..2]) and radius (c[3]) of a batch of nv <= 64 targets t[64][3]. */
+static void batch_geometry(const double (*t)[3], int nv, double c[4]) {
+    double v[64];
+    for (int d = 0; d < 3; ++d) {
+        for (int l = 0; l < 64; ++l) v[l] = l < nv ? t[l][d] : 0.0;
+        c[d] = butterfly(v, 0) * (1.0 / (double)nv);
+    }
+    for (int l = 0; l < 64; ++l) {
+        const double dx = t[l][0] - c[0], dy = t[l][1] - c[1], dz = t[l][2] - c[2];
+        v[l] = l < nv ? dx * dx + dy * dy + dz * dz : 0.0;
+    }
+    c[3] = sqrt(butterfly(v, 1));
+}
+
+/* Lex-order Cartesian setup shared by the batch functions. */
+static void cartesian_env(eval_env* e, int p) {
+    const int nt = nterms(p);
+    int (*ks)[3] = malloc(sizeof(int[3]) * nt);
+    int* id = malloc(sizeof(int) * p * p * p);
+    int* pred = malloc(sizeof(int) * 6 * nt);
+    int t = 0;
+    for (int a = 0; a < p; ++a)
+        for (int b = 0; b < p; ++b)
+            for (int c = 0; c < p; ++c) {
+                id[(a * p + b) * p + c] = (a + b + c < p) ? t : -1;
+                if (a + b + c < p) { ks[t][0] = a; ks[t][1] = b; ks[t][2] = c; ++t; }
+            }
+    for (int k = 0; k < nt; ++k)
+        for (int d = 0; d < 3; ++d) {
+            int n1[3] = {ks[k][0], ks[k][1], ks[k][2]};
+            n1[d] -= 1;
+            pred[6 * k + d] = (n1[d] >= 0) ? id[(n1[0] * p + n1[1]) * p + n1[2]] : -1;
+            n1[d] -= 1;
+            pred[6 * k + 3 + d] = (n1[d] >= 0) ? id[(n1[0] * p + n1[1]) * p + n1[2]] : -1;
+        }
+    free(id);
+    e->p = p;
+    e->nt = nt;
+    e->ks = (const int(*)[3])ks;
+    e->pred = pred;
+    e->succ = NULL;
+    e->series = 1;
+}
+
+static void cartesian_env_free(eval_env* e) {
+    free((void*)e->ks);
+    free((void*)e->pred);
+}
+
+/* Batch treecode velocity (FP64, Cartesian series of order p with lex
+ * moments of order p and sign +1).  stats[0] += accepted pairs x batch
+ * size, stats[1] += near-field pairs x batch size. */
+void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz, const double* gamma,
+                            const double* moments, int p, double theta, double delta, const double* tx,
+                            const double* ty, const double* tz, int64_t ntg, double* ux, double* uy, double* uz,
+                            int64_t* stats) {
+    eval_env e;
+    memset(&e, 0, sizeof(e));
+    e.sx = sx; e.sy = sy; e.sz = sz; e.g = gamma; e.mom = moments;
+    e.theta = theta; e.delta = delta; e.radius = 1.0;
+    cartesian_env(&e, p);
+    int64_t* order = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
+    oracle_target_order(tx, ty, tz, ntg, order);
+    const int64_t nb = (ntg + 63) / 64;
+    const double mult = -1.0 / (4.0 * kPi);
+    int64_t far_tot = 0, near_tot = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : far_tot, near_tot)
+    for (int64_t b = 0; b < nb; ++b) {
+        double t[64][3], sol[64][3], bc[4];
+        double* coef = malloc(sizeof(double) * e.nt);
+        int64_t* stack = malloc(sizeof(int64_t) * 8 * 72); /* >= 7 x depth + 1 */
+        const int nv = (int)((ntg - 64 * b) < 64 ? (ntg - 64 * b) : 64);
+        for (int l = 0; l < 64; ++l) {
+            const int64_t i = l < nv ? order[64 * b + l] : 0;
+            t[l][0] = l < nv ? tx[i] : 0.0;
+            t[l][1] = l < nv ? ty[i] : 0.0;
+            t[l][2] = l < nv ? tz[i] : 0.0;
+            sol[l][0] = sol[l][1] = sol[l][2] = 0.0;
+        }
+        batch_geometry((const double(*)[3])t, nv, bc);
+        int64_t sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const onode* n = &g_nodes[stack[--sp]];
+            const int64_t id = n - g_nodes;
+            const double dx = bc[0] - n->c[0], dy = bc[1] - n->c[1], dz = bc[2] - n->c[2];
+            const double s = sqrt(dx * dx + dy * dy + dz * dz);
+            if (n->max_radius + bc[3] < theta * s) {
+                const double* m = moments + (size_t)id * e.nt * 3;
+                for (int l = 0; l < nv; ++l) {
+                    const double d[3] = {t[l][0] - n->c[0], t[l][1] - n->c[1], t[l][2] - n->c[2]};
+                    coeffs_cartesian(&e, d, coef);
+                    double S[3] = {0.0, 0.0, 0.0};
+                    for (int k = 0; k < e.nt; ++k)
+                        for (int c = 0; c < 3; ++c) S[c] += coef[k] * m[3 * k + c];
+                    sol[l][0] += t[l][1] * S[2] - t[l][2] * S[1];
+                    sol[l][1] += t[l][2] * S[0] - t[l][0] * S[2];
+                    sol[l][2] += t[l][0] * S[1] - t[l][1] * S[0];
+                }
+                far_tot += nv;
+                continue;
+            }
+            if (n->n_kids == 0) {
+                const double d2 = delta * delta;
+                for (int l = 0; l < nv; ++l)
+                    for (int64_t q = n->begin; q < n->end; ++q) {
+                        const int64_t j = g_perm[q];
+                        const double den = 1.0 - (t[l][0] * sx[j] + t[l][1] * sy[j] + t[l][2] * sz[j]) + d2;
+                        if (fabs(den) > kZeroTol) {
+                            const double inv = 1.0 / den;
+                            sol[l][0] += ((t[l][1] * sz[j] - t[l][2] * sy[j]) * inv) * gamma[j];
+                            sol[l][1] += ((t[l][2] * sx[j] - t[l][0] * sz[j]) * inv) * gamma[j];
+                            sol[l][2] += ((t[l][0] * sy[j] - t[l][1] * sx[j]) * inv) * gamma[j];
+                        }
+                    }
+                near_tot += (int64_t)nv * (n->end - n->begin);
+                continue;
+            }
+            for (int64_t k = n->first_kid + n->n_kids - 1; k >= n->first_kid; --k) stack[sp++] = k;
+        }
+        for (int l = 0; l < nv; ++l) {
+            const int64_t i = order[64 * b + l];
+            ux[i] = sol[l][0] * mult;
+            uy[i] = sol[l][1] * mult;
+            uz[i] = sol[l][2] * mult;
+        }
+        free(coef);
+        free(stack);
+    }
+    if (stats) { stats[0] = far_tot; stats[1] = near_tot; }
+    free(order);
+    cartesian_env_free(&e);
+}
+
+/* Label of one (batch, node) pair: for every order q = 1..p the largest, over
+ * the batch's targets, of |t x (S_q - S_exact)| / (4 pi u_ref), where S_q is
+ * the Cartesian far field truncated to |k| < q (lex moments of order p, sign
+ * +1) and S_exact = sum_j gamma_j y_j / (1 - t.y_j + delta^2) over the node. */
+void oracle_pair_errors(const double* sx, const double* sy, const double* sz, const double* gamma,
+                        const double* moments, int p, double delta, double u_ref, const double* tx,
+                        const double* ty, const double* tz, int64_t ntg, int64_t batch, int64_t node,
+                        double* err /* [p] */) {
+    eval_env e;
+    memset(&e, 0, sizeof(e));
+    e.delta = delta;
+    cartesian_env(&e, p);
+    int64_t* order = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
+    oracle_target_order(tx, ty, tz, ntg, order);
+    const onode* n = &g_nodes[node];
+    const double* m = moments + (size_t)node * e.nt * 3;
+    double* coef = malloc(sizeof(double) * e.nt);
+    for (int q = 0; q < p; ++q) err[q] = 0.0;
+    const int nv = (int)((ntg - 64 * batch) < 64 ? (ntg - 64 * batch) : 64);
+    for (int l = 0; l < nv; ++l) {
+        const int64_t i = order[64 * batch + l];
+        const double t[3] = {tx[i], ty[i], tz[i]};
+        double E[3] = {0.0, 0.0, 0.0};
+        for (int64_t qq = n->begin; qq < n->end; ++qq) {
+            const int64_t j = g_perm[qq];
+            const double den = 1.0 - (t[0] * sx[j] + t[1] * sy[j] + t[2] * sz[j]) + delta * delta;
+            if (fabs(den) > kZeroTol) {
+                E[0] += gamma[j] / den * sx[j];
+                E[1] += gamma[j] / den * sy[j];
+                E[2] += gamma[j] / den * sz[j];
+            }
+        }
+        const double d[3] = {t[0] - n->c[0], t[1] - n->c[1], t[2] - n->c[2]};
+        coeffs_cartesian(&e, d, coef);
+        for (int q = 1; q <= p; ++q) {
+            double S[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < e.nt; ++k) {
+                if (e.ks[k][0] + e.ks[k][1] + e.ks[k][2] >= q) continue;
+                for (int c = 0; c < 3; ++c) S[c] += coef[k] * m[3 * k + c];
+            }
+            const double a[3] = {S[0] - E[0], S[1] - E[1], S[2] - E[2]};
+            const double v[3] = {t[1] * a[2] - t[2] * a[1], t[2] * a[0] - t[0] * a[2], t[0] * a[1] - t[1] * a[0]};
+            const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / (4.0 * kPi * u_ref);
+            if (r > err[q - 1]) err[q - 1] = r;
+        }
+    }
+    free(coef);
+    free(order);
+    cartesian_env_free(&e);
+}

@@ -58,6 +58,14 @@ def _declare_tree(L: C.CDLL) -> None:
         L.oracle_coeffs_series.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_int, _vp]
         L.oracle_coeffs_series.restype = C.c_int
+        L.oracle_target_order.argtypes = [_vp, _vp, _vp, C.c_int64, _vp]
+        L.oracle_target_order.restype = None
+        L.oracle_tree_eval_batch.argtypes = [_vp] * 5 + [C.c_int, C.c_double, C.c_double] + [_vp] * 3 + \
+            [C.c_int64] + [_vp] * 4
+        L.oracle_tree_eval_batch.restype = None
+        L.oracle_pair_errors.argtypes = [_vp] * 5 + [C.c_int, C.c_double, C.c_double] + [_vp] * 3 + \
+            [C.c_int64, C.c_int64, C.c_int64, _vp]
+        L.oracle_pair_errors.restype = None
 
 
 def _c(a, dtype=np.float64):
@@ -198,6 +206,42 @@ def node_far(c, t, sources, gamma, p, moment_sign, series, delta=0.0):
     out = np.empty(3)
     lib().oracle_node_far(c[0], c[1], c[2], t[0], t[1], t[2], sx.ctypes.data, sy.ctypes.data, sz.ctypes.data,
                           g.ctypes.data, sx.size, p, moment_sign, series, delta, out.ctypes.data)
+    return out
+
+
+def target_order(targets):
+    """Morton order of the targets in the last built oracle tree's root box."""
+    tx, ty, tz = (_c(a) for a in targets)
+    out = np.empty(tx.size, np.int64)
+    lib().oracle_target_order(tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, tx.size, out.ctypes.data)
+    return out
+
+
+def tree_eval_batch(sources, gamma, moments, p, theta, targets, delta=0.0):
+    """Batch-acceptance treecode on the last built oracle tree (lex moments of
+    order p, sign +1).  Returns ((ux, uy, uz), far_pairs, near_pairs)."""
+    sx, sy, sz = (_c(a) for a in sources)
+    g, m = _c(gamma), _c(moments)
+    tx, ty, tz = (_c(a) for a in targets)
+    n = tx.size
+    ux, uy, uz = np.empty(n), np.empty(n), np.empty(n)
+    st = np.zeros(2, np.int64)
+    lib().oracle_tree_eval_batch(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, g.ctypes.data, m.ctypes.data, p,
+                                 theta, delta, tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, n, ux.ctypes.data,
+                                 uy.ctypes.data, uz.ctypes.data, st.ctypes.data)
+    return (ux, uy, uz), int(st[0]), int(st[1])
+
+
+def pair_errors(sources, gamma, moments, p, targets, batch, node, u_ref, delta=0.0):
+    """Order-1..p far-field errors of one (batch, node) pair: max over the
+    batch of |du| / u_ref (the label definition)."""
+    sx, sy, sz = (_c(a) for a in sources)
+    g, m = _c(gamma), _c(moments)
+    tx, ty, tz = (_c(a) for a in targets)
+    out = np.empty(p)
+    lib().oracle_pair_errors(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, g.ctypes.data, m.ctypes.data, p,
+                             delta, u_ref, tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, tx.size, batch, node,
+                             out.ctypes.data)
     return out
 
 

@@ -1,0 +1,236 @@
+"""GPU tests of the batch treecode, the order-model kernels and the learned
+treecode.
+
+Bars: batch treecode (FP64) equal to the oracle's batch traversal within
+1e-12 with identical far/near pair counts; FP32C within 1e-6 of FP64; model
+gradients of the BF16 MFMA kernels within 5e-3 (relative L2) of the oracle's
+BF16-emulating backward pass; labels within 1e-5 of the oracle's per-pair
+errors; the learned traversal reducing exactly to the plain one for a
+constant model.
+"""
+import numpy as np
+import pytest
+
+import lpmcxx_amd as L
+from lpmcxx_amd import learned as LN
+from lpmcxx_amd import tree as T
+from oracle import mlp_oracle as MO
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(mesh, level, leaf, order, seed=L.LPM_SEED_CUBED):
+    m = mesh(seed, level)
+    src = (m.cx, m.cy, m.cz)
+    g = L.random_vortices(m.cx, m.cy, m.cz, m.area) * m.area
+    return m, src, g, m.targets()
+
+
+def _gpu_tree(ctx, m, src, g, leaf, order):
+    T.build(ctx, src, m.max_radius, leaf)
+    T.moments(ctx, g, order, 1)
+
+
+# ---------------------------------------------------------------- batch treecode
+@pytest.mark.parametrize("level,leaf,p,theta", [(5, 64, 6, 0.5), (5, 32, 8, 0.7), (4, 16, 4, 0.9)])
+def test_batch_matches_oracle(ctx, mesh, level, leaf, p, theta):
+    m, src, g, tgt = _field(mesh, level, leaf, p)
+    _gpu_tree(ctx, m, src, g, leaf, p)
+    u = T.evaluate_batch(ctx, tgt, theta, p, prec=L.LPM_F64)
+    st = T.stats(ctx)
+    O.tree_build(src, m.max_radius, leaf)
+    mom = O.tree_moments(src, g, p, 1)
+    ur, far, near = O.tree_eval_batch(src, g, mom, p, theta, tgt)
+    assert O.rel_max(np.stack(u), np.stack(ur)) < 1e-12
+    assert (st["far"], st["near"]) == (far, near)
+    assert st["terms"] == far * p * (p + 1) * (p + 2) // 6
+    u32 = T.evaluate_batch(ctx, tgt, theta, p, prec=L.LPM_F32C)
+    assert O.rel_l2(np.stack(u32), np.stack(u)) < 1e-6
+
+
+def test_batch_theta_zero_is_direct_sum(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 4, 32, 4)
+    _gpu_tree(ctx, m, src, g, 32, 4)
+    u = T.evaluate_batch(ctx, tgt, 0.0, 4)
+    ud = O.bs_direct(tgt, src, g)
+    assert O.rel_max(np.stack(u), np.stack(ud)) < 1e-12
+    assert T.stats(ctx)["far"] == 0
+
+
+def test_batch_converges_and_ragged(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 6, 128, 12)
+    _gpu_tree(ctx, m, src, g, 128, 12)
+    ud = np.stack(ctx.direct_sum(tgt, src, g))
+    errs = [O.rel_l2(np.stack(T.evaluate_batch(ctx, tgt, 0.5, p)), ud) for p in (2, 4, 6, 8, 10, 12)]
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 1e-7
+    # ragged batch counts and an empty target set
+    for n in (1, 63, 65, 1000):
+        sub = tuple(a[:n].copy() for a in tgt)
+        us = T.evaluate_batch(ctx, sub, 0.5, 8)
+        assert O.rel_l2(np.stack(us), ud[:, :n]) < 1e-5
+    T.evaluate_batch(ctx, tuple(a[:0].copy() for a in tgt), 0.5, 8)
+
+
+def test_batch_errors(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 3, 32, 4)
+    _gpu_tree(ctx, m, src, g, 32, 4)
+    with pytest.raises(L.LpmError):            # order above the moments' order
+        T.evaluate_batch(ctx, tgt, 0.5, 6)
+    with pytest.raises(L.LpmError):
+        T.evaluate_batch(ctx, tgt, 0.5, 4, prec=7)
+    T.moments(ctx, g, 4, -1)
+    with pytest.raises(L.LpmError):            # reference moment sign
+        T.evaluate_batch(ctx, tgt, 0.5, 4)
+
+
+# ---------------------------------------------------------------- model kernels
+def _rand_model(ctx, seed):
+    mod = LN.OrderModel(ctx)
+    mod.init(seed)
+    params, _, _ = mod.get_params()
+    rng = np.random.default_rng(seed)
+    params = params + rng.normal(0, 0.05, params.size).astype(np.float32)  # nonzero biases too
+    mod.set_params(params)
+    return mod, params
+
+
+def test_model_gradient_matches_oracle(ctx):
+    mod, params = _rand_model(ctx, 3)
+    rng = np.random.default_rng(4)
+    for n in (64, 333):
+        x = np.zeros((n, 16), np.float32)
+        x[:, :10] = rng.normal(size=(n, 10))
+        y = rng.integers(0, 13, n).astype(np.int32)
+        g, loss = mod.gradient(x, y)
+        lr, gr = MO.loss_and_grad(params, x, y, bf=True)
+        assert abs(loss - lr) < 1e-4 * max(1.0, abs(lr))
+        assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 5e-3
+        # every layer separately
+        for name, a in LN.unpack(g).items():
+            b = MO.unpack(gr)[name]
+            assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-7, name
+
+
+def test_model_forward_matches_oracle(ctx):
+    mod, params = _rand_model(ctx, 5)
+    rng = np.random.default_rng(6)
+    x = np.zeros((517, 16), np.float32)
+    x[:, :10] = rng.normal(size=(517, 10))
+    logits, cls = mod.forward(x)
+    z, _ = MO.forward(params, x, bf=True)
+    assert np.max(np.abs(logits - z)) < 1e-3 * max(1.0, np.abs(z).max())
+    assert np.mean(cls == np.argmax(z[:, :13], axis=1)) > 0.99
+
+
+def test_model_params_roundtrip_and_init(ctx):
+    mod = LN.OrderModel(ctx)
+    mod.init(11)
+    a, _, _ = mod.get_params()
+    mod.init(11)
+    b, _, _ = mod.get_params()
+    assert np.array_equal(a, b)
+    mean = np.arange(16, dtype=np.float32)
+    inv = np.full(16, 0.5, np.float32)
+    mod.set_params(a * 2, mean, inv)
+    c, m2, i2 = mod.get_params()
+    assert np.array_equal(c, a * 2) and np.array_equal(m2, mean) and np.array_equal(i2, inv)
+    with pytest.raises(L.LpmError):
+        mod.gradient(np.zeros((4, 16), np.float32), np.array([0, 1, 2, 13], np.int32))
+
+
+def test_model_training_learns_rule(ctx):
+    """Adam through the GPU kernels learns a fixed piecewise rule of the inputs."""
+    rng = np.random.default_rng(7)
+    n = 20000
+    x = np.zeros((n, 16), np.float32)
+    x[:, :10] = rng.normal(size=(n, 10))
+    y = np.clip(np.floor(2.0 * x[:, 0] + 6.0), 0, 12).astype(np.int32)
+    mod = LN.OrderModel(ctx)
+    mod.init(1)
+    p, _, _ = mod.get_params()
+    mean, inv = LN.normalization(x)
+    mod.set_params(p, mean, inv)
+    _, l0 = mod.gradient(x[:4096], y[:4096])
+    loss = mod.train(x, y, epochs=15, batch=1024, lr=3e-3, seed=2)
+    _, cls = mod.forward(x)
+    assert loss < 0.5 * l0
+    assert np.mean(np.abs(cls - y) <= 1) > 0.97
+
+
+# ---------------------------------------------------------------- labels and learned traversal
+def test_labels_match_oracle(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 5, 64, 12)
+    _gpu_tree(ctx, m, src, g, 64, 12)
+    u_ref = LN.u_ref_of(g)
+    ls = LN.labels(ctx, tgt, 0.8, u_ref, batch_stride=7, keep_rate=1.0, max_samples=1 << 16)
+    assert len(ls.ids) > 100
+    assert np.median(ls.errors[:, -1] / np.maximum(ls.errors[:, 0], 1e-300)) < 0.1  # the series converges
+    O.tree_build(src, m.max_radius, 64)
+    mom = O.tree_moments(src, g, 12, 1)
+    G = T.export(ctx, len(src[0]))
+    R = O.tree_build(src, m.max_radius, 64)
+    gi = T.dfs_order(G["first_kid"], G["n_kids"])
+    ri = T.dfs_order(R["first_kid"], R["n_kids"])
+    g2o = np.empty(len(gi), np.int64)
+    g2o[gi] = ri
+    rng = np.random.default_rng(0)
+    for k in rng.choice(len(ls.ids), 25, replace=False):
+        b, node = ls.ids[k]
+        ref = O.pair_errors(src, g, mom, 12, tgt, int(b), int(g2o[node]), u_ref)
+        got = ls.errors[k].astype(np.float64)
+        big = ref > 1e-12
+        assert np.all(np.abs(got[big] - ref[big]) <= 1e-5 * ref[big]), (k, got, ref)
+        assert np.all(np.abs(got[~big] - ref[~big]) <= 2e-12)
+        # features: node geometry from the oracle tree
+        c = R["cent"][g2o[node]]
+        assert ls.features[k, 9] == (1.0 if R["n_kids"][g2o[node]] == 0 else 0.0)
+        assert abs(np.exp2(ls.features[k, 6]) - (R["end"][g2o[node]] - R["begin"][g2o[node]])) < 1e-3
+        order = O.target_order(tgt)[64 * b:64 * b + 64]
+        T_b = np.stack([a[order] for a in tgt], 1)
+        cb = T_b.mean(0)
+        s = np.linalg.norm(cb - c[:3])
+        assert abs(ls.features[k, 8] - np.log2(s)) < 1e-5
+        assert abs(ls.features[k, 0] - np.log2(c[3] / s)) < 1e-5
+
+
+def test_learned_constant_model_reduces_to_plain(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 5, 64, 8)
+    _gpu_tree(ctx, m, src, g, 64, 8)
+    u_ref = LN.u_ref_of(g)
+    for prec in (L.LPM_F64, L.LPM_F32C):
+        LN.constant_model(ctx, 6)
+        ul = LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=prec)
+        sl = T.stats(ctx)
+        ub = T.evaluate_batch(ctx, tgt, 0.6, 6, prec=prec)
+        sb = T.stats(ctx)
+        assert O.rel_max(np.stack(ul), np.stack(ub)) < 1e-14
+        assert (sl["far"], sl["near"], sl["terms"]) == (sb["far"], sb["near"], sb["terms"])
+        assert sl["model"] > 0
+        # margin raises the order; class 0 (always descend) is the direct sum
+        LN.constant_model(ctx, 4)
+        um = LN.evaluate(ctx, tgt, 0.6, 2, 8, u_ref, prec=prec)
+        assert O.rel_max(np.stack(um), np.stack(ub)) < 1e-14
+    LN.constant_model(ctx, 0)
+    u0 = LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=L.LPM_F64)
+    assert T.stats(ctx)["far"] == 0
+    assert O.rel_max(np.stack(u0), np.stack(O.bs_direct(tgt, src, g))) < 1e-12
+    # a class above max_order descends as well
+    LN.constant_model(ctx, 9)
+    u9 = LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=L.LPM_F64)
+    assert T.stats(ctx)["far"] == 0 and O.rel_max(np.stack(u9), np.stack(u0)) < 1e-14
+
+
+def test_learned_errors(ctx, mesh):
+    m, src, g, tgt = _field(mesh, 3, 32, 4)
+    _gpu_tree(ctx, m, src, g, 32, 4)
+    LN.constant_model(ctx, 2)
+    with pytest.raises(L.LpmError):          # max_order above the moments' order
+        LN.evaluate(ctx, tgt, 0.6, 0, 6, 1.0)
+    with pytest.raises(L.LpmError):
+        LN.evaluate(ctx, tgt, 0.6, -1, 4, 1.0)
+    with pytest.raises(L.LpmError):
+        LN.evaluate(ctx, tgt, 0.6, 0, 4, 0.0)
+    with pytest.raises(L.LpmError):          # labels need order-12 moments
+        LN.labels(ctx, tgt, 0.8, 1.0)

@@ -176,3 +176,83 @@ def test_cartesian_series_converges_in_p(mesh):
         errs.append(O.rel_l2(u, ref))
     assert all(a > 3 * b for a, b in zip(errs, errs[1:]))
     assert errs[-1] < 1e-5
+
+
+# ---------------------------------------------------------------- batch acceptance and labels
+def _morton_keys(P, lo, hi):
+    """Independent restatement of the 21-level target keys (clamped into the box)."""
+    p = np.clip(P, lo, hi)
+    l, h = np.tile(lo, (len(P), 1)), np.tile(hi, (len(P), 1))
+    key = np.zeros(len(P), np.uint64)
+    for _ in range(21):
+        m = 0.5 * (l + h)
+        b = p >= m
+        l = np.where(b, m, l)
+        h = np.where(b, h, m)
+        key = (key << np.uint64(3)) | (b[:, 0] | (b[:, 1] << 1) | (b[:, 2] << 2)).astype(np.uint64)
+    return key
+
+
+def test_target_order_is_stable_morton(mesh):
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    src = (m.cx, m.cy, m.cz)
+    R = O.tree_build(src, m.max_radius, 32)
+    rng = np.random.default_rng(1)
+    P = rng.normal(size=(3000, 3)) * 0.7
+    P[:20] = P[20]                               # ties keep index order
+    order = O.target_order(tuple(P.T.copy()))
+    lo, hi = R["box"][0, :3], R["box"][0, 3:]
+    key = _morton_keys(P, lo, hi)
+    assert np.array_equal(order, np.argsort(key, kind="stable"))
+
+
+def test_batch_oracle_limits(mesh):
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    src = (m.cx, m.cy, m.cz)
+    g = L.random_vortices(m.cx, m.cy, m.cz, m.area) * m.area
+    tgt = m.targets()
+    O.tree_build(src, m.max_radius, 32)
+    mom = O.tree_moments(src, g, 8, 1)
+    ud = np.stack(O.bs_direct(tgt, src, g))
+    u0, far, near = O.tree_eval_batch(src, g, mom, 8, 0.0, tgt)
+    assert far == 0 and near == len(tgt[0]) * len(src[0])
+    assert O.rel_max(np.stack(u0), ud) < 1e-13
+    errs = []
+    for p in (2, 4, 6, 8):
+        mp = O.tree_moments(src, g, p, 1)
+        u, far, near = O.tree_eval_batch(src, g, mp, p, 0.6, tgt)
+        assert far > 0
+        errs.append(O.rel_l2(np.stack(u), ud))
+    assert all(b < 0.5 * a for a, b in zip(errs, errs[1:])), errs
+    # the batch criterion is stricter than the per-target one: more accurate at equal (p, theta)
+    (up, _, _) = O.tree_eval(src, g, mom, 8, 0.6, tgt, series=1)
+    assert errs[-1] <= O.rel_l2(np.stack(up), ud)
+
+
+def test_pair_errors_match_single_box_series(mesh):
+    """Label errors equal |t x (S_q - S_exact)| / (4 pi u_ref) from the
+    single-box far field (independent moment code) and the direct sum."""
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    src = (m.cx, m.cy, m.cz)
+    g = L.random_vortices(m.cx, m.cy, m.cz, m.area) * m.area
+    tgt = m.targets()
+    R = O.tree_build(src, m.max_radius, 32)
+    mom = O.tree_moments(src, g, 12, 1)
+    order = O.target_order(tgt)
+    u_ref = float(np.abs(g).sum() / (4 * np.pi))
+    node = int(np.flatnonzero(R["level"] == 2)[3])
+    batch = 37
+    err = O.pair_errors(src, g, mom, 12, tgt, batch, node, u_ref)
+    ids = R["perm"][R["begin"][node]:R["end"][node]]
+    ps = tuple(a[ids] for a in src)
+    c = R["cent"][node, :3]
+    ref = np.zeros(12)
+    for i in order[64 * batch:64 * batch + 64]:
+        t = np.array([tgt[0][i], tgt[1][i], tgt[2][i]])
+        y = np.stack(ps, 1)
+        ex = np.cross(t, (g[ids][:, None] * y / (1.0 - y @ t)[:, None]).sum(0))
+        for q in range(1, 13):
+            far = O.node_far(c, t, ps, g[ids], q, 1, 1)
+            ref[q - 1] = max(ref[q - 1], np.linalg.norm(far - ex) / (4 * np.pi * u_ref))
+    assert np.allclose(err, ref, rtol=1e-9, atol=1e-16)
+    assert err[-1] < 1e-3 * err[0]
