@@ -244,7 +244,9 @@ int lpm_tree_eval_learned_dev(lpm_ctx* ctx, int prec, double theta_max, int marg
 /* ------------------------------------------------------------------ time stepping
  * Velocity solver used by the RK4 step. */
 #define LPM_METHOD_DIRECT 0
-#define LPM_METHOD_TREE 1
+#define LPM_METHOD_TREE 1          /* lpm_tree_eval: per-target acceptance */
+#define LPM_METHOD_TREE_BATCH 2    /* lpm_tree_eval_batch (Cartesian series) */
+#define LPM_METHOD_TREE_LEARNED 3  /* lpm_tree_eval_learned: theta = theta_max, order = max_order */
 typedef struct lpm_solver_config {
     int method;              /* LPM_METHOD_DIRECT or LPM_METHOD_TREE */
     int prec;                /* LPM_F64 or LPM_F32C */
@@ -255,6 +257,8 @@ typedef struct lpm_solver_config {
     double delta;            /* smoothing length (0 = singular kernel) */
     double root_box_min[3];  /* tree: root box; must contain every particle at every stage */
     double root_box_max[3];
+    int margin;              /* learned: orders added to the model's class */
+    double u_ref;            /* learned: velocity scale of the model's features */
 } lpm_solver_config;
 /* direct FP64, tree defaults p = 8, leaf 128, theta 0.5, Cartesian series, root box [-1.00101, 1.00101)^3 */
 void lpm_solver_config_default(lpm_solver_config* cfg);

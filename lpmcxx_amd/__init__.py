@@ -82,7 +82,8 @@ class SolverConfig(C.Structure):
     """lpm_solver_config (include/lpm_gpu.h): velocity solver of the RK4 step."""
     _fields_ = [("method", C.c_int), ("prec", C.c_int), ("series", C.c_int), ("order", C.c_int),
                 ("leaf_max", C.c_int), ("theta", C.c_double), ("delta", C.c_double),
-                ("root_box_min", C.c_double * 3), ("root_box_max", C.c_double * 3)]
+                ("root_box_min", C.c_double * 3), ("root_box_max", C.c_double * 3), ("margin", C.c_int),
+                ("u_ref", C.c_double)]
 
     @classmethod
     def default(cls, **kw) -> "SolverConfig":
@@ -95,6 +96,8 @@ class SolverConfig(C.Structure):
 
 LPM_METHOD_DIRECT = 0
 LPM_METHOD_TREE = 1
+LPM_METHOD_TREE_BATCH = 2
+LPM_METHOD_TREE_LEARNED = 3
 
 
 def _declare_extra(lib: C.CDLL) -> None:

@@ -68,7 +68,8 @@ void lpm_destroy(lpm_ctx* c) {
     tree_free(c->tree);
     model_free(c->model);
     c->tree = nullptr;
-    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work})
+    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work,
+                      &c->tb_work})
         b->release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -42,6 +42,8 @@ struct lpm_ctx {
     lpm::DevBuf h_in, h_out;
     // RK4 stage storage
     lpm::DevBuf rk_work;
+    // target batches of the batch treecode
+    lpm::DevBuf tb_work;
     lpm::Tree* tree = nullptr;
     lpm::Model* model = nullptr;
     unsigned long long* stats_ptr = nullptr;  // kStats counters of the last tree evaluation (device)
@@ -81,6 +83,12 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
 // -> target index, device); also clears the evaluation counters.
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
                           int** tperm);
+// Cell-aligned target batches: the Morton-sorted targets split at the
+// boundaries of the octree cells of level L (the shallowest level with at
+// least nt / 256 occupied cells) and cut into runs of at most 64.
+// bstart[b] = first sorted position of batch b, nb batches.
+int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
+                        int** bstart, int64_t* nb);
 // batch-MAC treecode (treecode_batch.hip): prec LPM_F64, or LPM_F32C with
 // FP32 far field and FP32C near field; order <= tree order
 int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,

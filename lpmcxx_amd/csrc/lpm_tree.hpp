@@ -25,6 +25,7 @@ struct Tree {
     DevBuf nprefix;   // uint64 [cap] key prefix (top 3*level bits)
     DevBuf ncent;     // double4 [nn] centroid + maxRadius
     DevBuf nbox;      // double [nn][6] lo3 hi3
+    DevBuf nrad;      // double [nn] max |y - centroid| over the node's sources (0 if empty)
     int64_t cap = 0;
     // moments
     int order = 0;        // p (|k| < p), 0 = none

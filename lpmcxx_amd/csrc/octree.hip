@@ -171,6 +171,30 @@ __global__ __launch_bounds__(256) void iota_i32(int* v, int64_t n) {
     if (i < n) v[i] = int(i);
 }
 
+// Largest distance of a node's sources from its box centroid (one block per node).
+__global__ __launch_bounds__(256) void tree_node_radius(int nn, const int* __restrict__ nbeg,
+                                                        const int* __restrict__ nend,
+                                                        const double4* __restrict__ ncent,
+                                                        const double4* __restrict__ spos, double* __restrict__ nrad) {
+    __shared__ double red[256];
+    const int i = blockIdx.x;
+    if (i >= nn) return;
+    const double4 c = ncent[i];
+    double r2 = 0.0;
+    for (int q = nbeg[i] + threadIdx.x; q < nend[i]; q += blockDim.x) {
+        const double4 s = spos[q];
+        const double dx = s.x - c.x, dy = s.y - c.y, dz = s.z - c.z;
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    red[threadIdx.x] = r2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) nrad[i] = sqrt(red[0]);
+}
+
 __global__ __launch_bounds__(256) void gather_pos(const double* __restrict__ x, const double* __restrict__ y,
                                                   const double* __restrict__ z, const int* __restrict__ perm,
                                                   int64_t n, double4* __restrict__ out) {
@@ -223,7 +247,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
 void tree_free(Tree* t) {
     if (!t) return;
     for (DevBuf* b : {&t->perm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
-                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->mom, &t->momg, &t->mom32, &t->soff32, &t->nstat, &t->tmp0,
+                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32, &t->soff32, &t->nstat, &t->tmp0,
                       &t->tmp1, &t->tmp2, &t->tmp3, &t->sort_tmp})
         b->release();
     delete t;
@@ -323,6 +347,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
                            t->spos.as<double4>());
     }
+    LPM_CHECK(t->nrad.reserve(size_t(nn) * sizeof(double)));
+    hipLaunchKernelGGL(tree_node_radius, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nbeg.as<int>(),
+                       t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(), t->nrad.as<double>());
     LPM_HIP(c, hipGetLastError());
     std::vector<int> hk(nn);
     LPM_HIP(c, hipMemcpyAsync(hk.data(), t->nkids.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));

@@ -67,6 +67,11 @@ static int velocity(lpm_ctx* c, const lpm_solver_config& cfg, int64_t n, const d
     LPM_CHECK(tree_build_device(c, x, y, z, n, cfg.root_box_min, cfg.root_box_max, LPM_TREE_MAX_PER_NODE,
                                 cfg.leaf_max));
     LPM_CHECK(tree_moments_device(c, gamma, cfg.order, 1));
+    if (cfg.method == LPM_METHOD_TREE_BATCH)
+        return tree_eval_batch_device(c, cfg.prec, cfg.order, cfg.theta, cfg.delta, x, y, z, n, ux, uy, uz);
+    if (cfg.method == LPM_METHOD_TREE_LEARNED)
+        return tree_eval_learned_device(c, cfg.prec, cfg.theta, cfg.margin, cfg.order, cfg.u_ref, cfg.delta, x, y, z,
+                                        n, ux, uy, uz);
     return tree_eval_device(c, cfg.prec, cfg.series, cfg.theta, cfg.delta, x, y, z, n, ux, uy, uz, nullptr, 1.0);
 }
 
@@ -110,10 +115,15 @@ static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_
     if (n < 0 || n > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid particle count");
     if (n > 0 && (!x || !y || !z || !zeta_abs || !area)) return fail(c, LPM_ERR_NULL_PTR, "NULL particle array");
     if (!std::isfinite(dt)) return fail(c, LPM_ERR_INVALID_ARG, "invalid dt");
-    if (cfg->method != LPM_METHOD_DIRECT && cfg->method != LPM_METHOD_TREE)
+    if (cfg->method < LPM_METHOD_DIRECT || cfg->method > LPM_METHOD_TREE_LEARNED)
         return fail(c, LPM_ERR_INVALID_ARG, "invalid velocity method");
     if (cfg->prec != LPM_F64 && cfg->prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
-    if (cfg->method == LPM_METHOD_TREE) {
+    if (cfg->method == LPM_METHOD_TREE_LEARNED) {
+        if (!model_weights(c)) return fail(c, LPM_ERR_STATE, "no order model (lpm_model_set_params)");
+        if (!(cfg->u_ref > 0.0) || cfg->margin < 0)
+            return fail(c, LPM_ERR_INVALID_ARG, "learned treecode needs u_ref > 0 and margin >= 0");
+    }
+    if (cfg->method != LPM_METHOD_DIRECT) {
         if (cfg->order < 1 || cfg->order > 12 || cfg->leaf_max < 1)
             return fail(c, LPM_ERR_INVALID_ARG, "tree order must be in [1, 12] and leaf_max >= 1");
         if (cfg->series != LPM_SERIES_CARTESIAN && cfg->series != LPM_SERIES_REFERENCE)
@@ -134,6 +144,8 @@ void lpm_solver_config_default(lpm_solver_config* cfg) {
     cfg->leaf_max = 128;
     cfg->theta = 0.5;
     cfg->delta = 0.0;
+    cfg->margin = 0;
+    cfg->u_ref = 0.0;
     for (int d = 0; d < 3; ++d) {
         cfg->root_box_min[d] = -1.0 - 1e-5 - 1e-3;
         cfg->root_box_max[d] = 1.0 + 1e-5 + 1e-3;

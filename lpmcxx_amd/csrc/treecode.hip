@@ -195,6 +195,38 @@ __global__ __launch_bounds__(256) void target_keys(const double* __restrict__ x,
     vals[i] = int(i);
 }
 
+// ---------------------------------------------------------------- target batches
+// Number of leading equal octree levels of two 63-bit keys.
+__device__ inline int key_lcp(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (x == 0) return kMaxDepth;
+    return (62 - (63 - __clzll(x))) / 3;
+}
+
+__global__ __launch_bounds__(256) void batch_lcp_hist(const uint64_t* __restrict__ k, int64_t n,
+                                                      unsigned* __restrict__ hist) {
+    __shared__ unsigned h[kMaxDepth + 1];
+    if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 1 && i < n) atomicAdd(&h[key_lcp(k[i - 1], k[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x <= kMaxDepth && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// cs[i] = i where a level-L cell starts, else 0 (then a running max gives the cell start)
+__global__ __launch_bounds__(256) void batch_cell_start(const uint64_t* __restrict__ k, int64_t n, int L,
+                                                        int* __restrict__ cs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) cs[i] = (i == 0 || key_lcp(k[i - 1], k[i]) < L) ? int(i) : 0;
+}
+
+__global__ __launch_bounds__(256) void batch_flags(const int* __restrict__ cs, int64_t n,
+                                                   unsigned char* __restrict__ f) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = ((i - cs[i]) % kWave) == 0;
+}
+
 // ---------------------------------------------------------------- evaluation
 __device__ inline uint64_t uniform64(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
@@ -407,6 +439,58 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
     LPM_CHECK(t->sort_tmp.reserve(bytes + 256));
     LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0,
                                                   3 * kMaxDepth, st));
+    return LPM_OK;
+}
+
+int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
+                        int** bstart, int64_t* nb) {
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm));
+    *nb = 0;
+    if (nt == 0) return LPM_OK;
+    hipStream_t st = c->stream;
+    const uint64_t* keys = c->ds_tgt.as<uint64_t>() + nt;  // sorted keys (k_out)
+    // workspace: hist[32] | nsel (int64) | cs[nt] | flags[nt] | bstart[nt] | scan/select temp
+    size_t scan_bytes = 0, sel_bytes = 0;
+    int* cs0 = nullptr;
+    LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(nullptr, scan_bytes, cs0, cs0, hipcub::Max(), int(nt), st));
+    LPM_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<int>(0),
+                                             static_cast<unsigned char*>(nullptr), cs0, static_cast<int*>(nullptr),
+                                             int(nt), st));
+    const size_t tmp_bytes = std::max(scan_bytes, sel_bytes) + 256;
+    const size_t off_cs = 256, off_f = off_cs + size_t(nt) * 4, off_b = off_f + ((size_t(nt) + 255) & ~size_t(255)),
+                 off_t = off_b + size_t(nt) * 4 + 256;
+    LPM_CHECK(c->tb_work.reserve(off_t + tmp_bytes));
+    char* w = c->tb_work.as<char>();
+    unsigned* hist = reinterpret_cast<unsigned*>(w);
+    int* nsel = reinterpret_cast<int*>(w + 128);
+    int* cs = reinterpret_cast<int*>(w + off_cs);
+    unsigned char* f = reinterpret_cast<unsigned char*>(w + off_f);
+    int* bs = reinterpret_cast<int*>(w + off_b);
+    void* tmp = w + off_t;
+    LPM_HIP(c, hipMemsetAsync(hist, 0, 128, st));
+    hipLaunchKernelGGL(batch_lcp_hist, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, hist);
+    unsigned h[kMaxDepth + 1];
+    LPM_HIP(c, hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    const int64_t want = std::max<int64_t>(1, nt / 256);
+    int L = kMaxDepth;
+    int64_t cells = 1;
+    for (int l = 0; l <= kMaxDepth; ++l) {
+        if (cells >= want) { L = l; break; }
+        if (l < kMaxDepth) cells += h[l];
+    }
+    hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, L, cs);
+    size_t b1 = tmp_bytes;
+    LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(tmp, b1, cs, cs, hipcub::Max(), int(nt), st));
+    hipLaunchKernelGGL(batch_flags, dim3(blocks(nt)), dim3(256), 0, st, cs, nt, f);
+    size_t b2 = tmp_bytes;
+    LPM_HIP(c, hipcub::DeviceSelect::Flagged(tmp, b2, hipcub::CountingInputIterator<int>(0), f, bs, nsel, int(nt),
+                                             st));
+    int hn = 0;
+    LPM_HIP(c, hipMemcpyAsync(&hn, nsel, sizeof(int), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    *bstart = bs;
+    *nb = hn;
     return LPM_OK;
 }
 

@@ -168,6 +168,7 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
 
 struct TreeView {
     const double4* ncent;
+    const double* nrad;
     const int *nfirst, *nkids, *nbeg, *nend;
     const float4* nstat;
     const double4* spos;
@@ -179,13 +180,18 @@ struct TreeView {
 
 struct Targets {
     const double *x, *y, *z;
-    const int* perm;
+    const int* perm;    // sorted position -> target index
     int64_t n;
+    const int* bstart;  // first sorted position of every batch
+    int64_t nb;
 };
 
-__device__ inline void load_target(const Targets& tg, int64_t it, bool& valid, int& orig, double& t0, double& t1,
+// Target of this lane in batch b (b < nb).
+__device__ inline void load_target(const Targets& tg, int64_t b, bool& valid, int& orig, double& t0, double& t1,
                                    double& t2) {
-    valid = it < tg.n;
+    const int64_t b0 = tg.bstart[b], b1 = b + 1 < tg.nb ? tg.bstart[b + 1] : tg.n;
+    const int64_t it = b0 + (threadIdx.x & 63);
+    valid = it < b1;
     orig = valid ? tg.perm[it] : 0;
     t0 = valid ? tg.x[orig] : 0.0;
     t1 = valid ? tg.y[orig] : 0.0;
@@ -210,12 +216,12 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
                                                        unsigned long long* __restrict__ stats) {
     __shared__ int s_node[4][kStackB];
     const int w = threadIdx.x >> 6;
-    const int64_t it = (int64_t(blockIdx.x) * 4 + w) * 64 + (threadIdx.x & 63);
+    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    if (bi >= tg.nb) return;
     bool valid;
     int orig;
     double t0, t1, t2;
-    load_target(tg, it, valid, orig, t0, t1, t2);
-    if (__ballot(valid) == 0) return;
+    load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     unsigned long long nfar = 0, nnear = 0;
@@ -227,7 +233,8 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
         const double4 c = tv.ncent[n];
         const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
         const double s = sqrt(dx * dx + dy * dy + dz * dz);
-        if (uniform(c.w + bt.r < theta * s)) {
+        const double rn = tv.nrad[n];
+        if (uniform(rn + bt.r < theta * s)) {
             far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2);
             ++nfar;
             continue;
@@ -262,12 +269,12 @@ __global__ __launch_bounds__(256) void tree_eval_learned(Targets tg, TreeView tv
     __shared__ int s_node[4][kStackB];
     __shared__ float s_h[4][144];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t it = (int64_t(blockIdx.x) * 4 + w) * 64 + lane;
+    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    if (bi >= tg.nb) return;
     bool valid;
     int orig;
     double t0, t1, t2;
-    load_target(tg, it, valid, orig, t0, t1, t2);
-    if (__ballot(valid) == 0) return;
+    load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     unsigned long long nfar = 0, nnear = 0, nterms = 0, ncalls = 0;
@@ -279,9 +286,10 @@ __global__ __launch_bounds__(256) void tree_eval_learned(Targets tg, TreeView tv
         const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
         const double s = sqrt(dx * dx + dy * dy + dz * dz);
         const int nk = tv.nkids[n];
-        if (uniform(c.w + bt.r < theta_max * s)) {
+        const double rn = tv.nrad[n];
+        if (uniform(rn + bt.r < theta_max * s)) {
             float f[kIn];
-            pair_features(f, c.w, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
+            pair_features(f, rn, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
             float v = 0.f;
 #pragma unroll
             for (int k = 0; k < kIn; ++k)
@@ -340,8 +348,7 @@ __global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView t
     bool valid;
     int orig;
     double t0, t1, t2;
-    load_target(tg, bi * 64 + lane, valid, orig, t0, t1, t2);
-    if (__ballot(valid) == 0) return;
+    load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
     const double inv = 1.0 / (4.0 * 3.14159265358979323846 * u_ref);
     int sp = 1;
@@ -356,7 +363,8 @@ __global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView t
         const double s = sqrt(dx * dx + dy * dy + dz * dz);
         const int nk = tv.nkids[n];
         const int b = tv.nbeg[n], e = tv.nend[n];
-        const bool prop = uniform(c.w + bt.r < theta_max * s);
+        const double rn = tv.nrad[n];
+        const bool prop = uniform(rn + bt.r < theta_max * s);
         if (prop && (frontier < 0 || a <= frontier)) {
             const uint32_t h = mix32(uint32_t(bi) * 0x9E3779B1u ^ mix32(uint32_t(n) + seed));
             const uint32_t h2 = mix32(h ^ 0x27d4eb2fu);
@@ -367,7 +375,7 @@ __global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView t
                 slot = __shfl(slot, 0, 64);
                 if (int64_t(slot) < max_samples) {
                     float f[kIn];
-                    pair_features(f, c.w, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
+                    pair_features(f, rn, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
                     if (lane == 0) {
                         for (int k = 0; k < kIn; ++k) feat_out[slot * kIn + k] = f[k];
                         id_out[2 * slot] = int(bi);
@@ -416,6 +424,7 @@ __global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView t
 TreeView view(const Tree* t) {
     TreeView v;
     v.ncent = t->ncent.as<double4>();
+    v.nrad = t->nrad.as<double>();
     v.nfirst = t->nfirst.as<int>();
     v.nkids = t->nkids.as<int>();
     v.nbeg = t->nbeg.as<int>();
@@ -436,12 +445,13 @@ constexpr double kPi = 3.1415926535897932384626433832795027975;
 int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
                            const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
     Tree* t = c->tree;
-    int* tperm = nullptr;
-    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
+    int *tperm = nullptr, *bstart = nullptr;
+    int64_t nbatch = 0;
+    LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch));
     if (nt == 0) return LPM_OK;
-    const Targets tg{tx, ty, tz, tperm, nt};
+    const Targets tg{tx, ty, tz, tperm, nt, bstart, nbatch};
     const TreeView tv = view(t);
-    const unsigned nb = unsigned((nt + 255) / 256);
+    const unsigned nb = unsigned((nbatch + 3) / 4);
     const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);
     const float eps2v = float(2.0 * d2);
     auto go = [&](auto kern) {
@@ -468,17 +478,18 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
                              double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
                              double* ux, double* uy, double* uz) {
     Tree* t = c->tree;
-    int* tperm = nullptr;
-    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
+    int *tperm = nullptr, *bstart = nullptr;
+    int64_t nbatch = 0;
+    LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch));
     if (nt == 0) return LPM_OK;
-    const Targets tg{tx, ty, tz, tperm, nt};
+    const Targets tg{tx, ty, tz, tperm, nt, bstart, nbatch};
     const TreeView tv = view(t);
     MlpNorm norm;
     for (int k = 0; k < kIn; ++k) {
         norm.mean[k] = model_norm(c, 0)[k];
         norm.inv_std[k] = model_norm(c, 1)[k];
     }
-    const unsigned nb = unsigned((nt + 255) / 256);
+    const unsigned nb = unsigned((nbatch + 3) / 4);
     const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);
     const float eps2v = float(2.0 * d2);
     const unsigned short* w = model_weights(c);
@@ -507,11 +518,11 @@ static int tree_labels_device(lpm_ctx* c, double theta_max, double delta, double
                               int64_t* n_out) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
-    int* tperm = nullptr;
-    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &tperm));
-    const Targets tg{tx, ty, tz, tperm, nt};
+    int *tperm = nullptr, *bstart = nullptr;
+    int64_t nbatch = 0;
+    LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch));
+    const Targets tg{tx, ty, tz, tperm, nt, bstart, nbatch};
     const TreeView tv = view(t);
-    const int64_t nbatch = (nt + 63) / 64;
     const int64_t nsel = (nbatch + bstride - 1) / bstride;
     // the first evaluation counter serves as the sample counter
     unsigned long long* counter = c->stats_ptr;

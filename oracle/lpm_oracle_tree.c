@@ -519,7 +519,10 @@ void oracle_node_far(double cx, double cy, double cz, double tx, double ty, doub
  * Batch-MAC treecode (no counterpart in the reference; the specification of
  * lpm_tree_eval_batch in include/lpm_gpu.h).  Targets are ordered by 63-bit
  * Morton keys of 21 bisections of the root box (clamped into it), ties by
- * index; batches are 64 consecutive targets.  Batch centroid and radius are
+ * index; batches are runs of at most 64 sorted targets inside one octree
+ * cell (oracle_target_batches); node n is accepted when r_n + r_b <
+ * theta |c_b - c_n| with r_n the largest distance of the node's particles
+ * from its box centroid.  Batch centroid and radius are
  * formed with the xor-butterfly summation order of a 64-lane wave reduction
  * so the acceptance decisions reproduce the GPU's exactly. */
 typedef struct {
@@ -533,8 +536,10 @@ static int cmp_keyed(const void* a, const void* b) {
     return (x->idx > y->idx) - (x->idx < y->idx);
 }
 
-/* Morton order of the targets in the current tree's root box. */
-void oracle_target_order(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order) {
+/* Morton keys of the targets in the current tree's root box, sorted
+ * (order[i] = target at sorted position i, keys[i] its key; keys may be NULL). */
+static void sorted_keys(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order,
+                        uint64_t* keys) {
     keyed* k = (keyed*)malloc(sizeof(keyed) * (n > 0 ? n : 1));
     const onode* r = &g_nodes[0];
     for (int64_t i = 0; i < n; ++i) {
@@ -556,8 +561,48 @@ void oracle_target_order(const double* tx, const double* ty, const double* tz, i
         k[i].idx = i;
     }
     qsort(k, (size_t)n, sizeof(keyed), cmp_keyed);
-    for (int64_t i = 0; i < n; ++i) order[i] = k[i].idx;
+    for (int64_t i = 0; i < n; ++i) {
+        order[i] = k[i].idx;
+        if (keys) keys[i] = k[i].key;
+    }
     free(k);
+}
+
+void oracle_target_order(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order) {
+    sorted_keys(tx, ty, tz, n, order, NULL);
+}
+
+/* leading equal octree levels of two 63-bit keys */
+static int key_lcp(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (!x) return 21;
+    return (62 - (63 - __builtin_clzll(x))) / 3;
+}
+
+/* Target batches: the sorted targets split where the level-L cell changes,
+ * L the shallowest level with at least max(1, n / 256) occupied cells, and
+ * cut into runs of at most 64.  Fills order and bstart; returns the count. */
+int64_t oracle_target_batches(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order,
+                              int64_t* bstart) {
+    if (n == 0) return 0;
+    uint64_t* keys = malloc(sizeof(uint64_t) * n);
+    sorted_keys(tx, ty, tz, n, order, keys);
+    int64_t hist[22] = {0};
+    for (int64_t i = 1; i < n; ++i) hist[key_lcp(keys[i - 1], keys[i])]++;
+    const int64_t want = n / 256 > 1 ? n / 256 : 1;
+    int L = 21;
+    int64_t cells = 1;
+    for (int l = 0; l <= 21; ++l) {
+        if (cells >= want) { L = l; break; }
+        if (l < 21) cells += hist[l];
+    }
+    int64_t nb = 0, cs = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i == 0 || key_lcp(keys[i - 1], keys[i]) < L) cs = i;
+        if ((i - cs) % 64 == 0) bstart[nb++] = i;
+    }
+    free(keys);
+    return nb;
 }
 
 static double butterfly(double* v, int is_max) {
@@ -581,6 +626,17 @@ static void batch_geometry(const double (*t)[3], int nv, double c[4]) {
         v[l] = l < nv ? dx * dx + dy * dy + dz * dz : 0.0;
     }
     c[3] = sqrt(butterfly(v, 1));
+}
+
+/* Largest distance of a node's particles from its box centroid. */
+static double node_radius(const onode* n, const double* sx, const double* sy, const double* sz) {
+    double r2 = 0.0;
+    for (int64_t q = n->begin; q < n->end; ++q) {
+        const int64_t j = g_perm[q];
+        const double dx = sx[j] - n->c[0], dy = sy[j] - n->c[1], dz = sz[j] - n->c[2];
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    return sqrt(r2);
 }
 
 /* Lex-order Cartesian setup shared by the batch functions. */
@@ -631,8 +687,10 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
     e.theta = theta; e.delta = delta; e.radius = 1.0;
     cartesian_env(&e, p);
     int64_t* order = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
-    oracle_target_order(tx, ty, tz, ntg, order);
-    const int64_t nb = (ntg + 63) / 64;
+    int64_t* bstart = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
+    const int64_t nb = oracle_target_batches(tx, ty, tz, ntg, order, bstart);
+    double* nrad = malloc(sizeof(double) * (g_nn > 0 ? g_nn : 1));
+    for (int64_t i = 0; i < g_nn; ++i) nrad[i] = node_radius(&g_nodes[i], sx, sy, sz);
     const double mult = -1.0 / (4.0 * kPi);
     int64_t far_tot = 0, near_tot = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : far_tot, near_tot)
@@ -640,9 +698,10 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
         double t[64][3], sol[64][3], bc[4];
         double* coef = malloc(sizeof(double) * e.nt);
         int64_t* stack = malloc(sizeof(int64_t) * 8 * 72); /* >= 7 x depth + 1 */
-        const int nv = (int)((ntg - 64 * b) < 64 ? (ntg - 64 * b) : 64);
+        const int64_t b0 = bstart[b], b1 = b + 1 < nb ? bstart[b + 1] : ntg;
+        const int nv = (int)(b1 - b0);
         for (int l = 0; l < 64; ++l) {
-            const int64_t i = l < nv ? order[64 * b + l] : 0;
+            const int64_t i = l < nv ? order[b0 + l] : 0;
             t[l][0] = l < nv ? tx[i] : 0.0;
             t[l][1] = l < nv ? ty[i] : 0.0;
             t[l][2] = l < nv ? tz[i] : 0.0;
@@ -656,7 +715,7 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
             const int64_t id = n - g_nodes;
             const double dx = bc[0] - n->c[0], dy = bc[1] - n->c[1], dz = bc[2] - n->c[2];
             const double s = sqrt(dx * dx + dy * dy + dz * dz);
-            if (n->max_radius + bc[3] < theta * s) {
+            if (nrad[id] + bc[3] < theta * s) {
                 const double* m = moments + (size_t)id * e.nt * 3;
                 for (int l = 0; l < nv; ++l) {
                     const double d[3] = {t[l][0] - n->c[0], t[l][1] - n->c[1], t[l][2] - n->c[2]};
@@ -690,7 +749,7 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
             for (int64_t k = n->first_kid + n->n_kids - 1; k >= n->first_kid; --k) stack[sp++] = k;
         }
         for (int l = 0; l < nv; ++l) {
-            const int64_t i = order[64 * b + l];
+            const int64_t i = order[b0 + l];
             ux[i] = sol[l][0] * mult;
             uy[i] = sol[l][1] * mult;
             uz[i] = sol[l][2] * mult;
@@ -700,6 +759,8 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
     }
     if (stats) { stats[0] = far_tot; stats[1] = near_tot; }
     free(order);
+    free(bstart);
+    free(nrad);
     cartesian_env_free(&e);
 }
 
@@ -716,14 +777,16 @@ void oracle_pair_errors(const double* sx, const double* sy, const double* sz, co
     e.delta = delta;
     cartesian_env(&e, p);
     int64_t* order = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
-    oracle_target_order(tx, ty, tz, ntg, order);
+    int64_t* bstart = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
+    const int64_t nb = oracle_target_batches(tx, ty, tz, ntg, order, bstart);
     const onode* n = &g_nodes[node];
     const double* m = moments + (size_t)node * e.nt * 3;
     double* coef = malloc(sizeof(double) * e.nt);
     for (int q = 0; q < p; ++q) err[q] = 0.0;
-    const int nv = (int)((ntg - 64 * batch) < 64 ? (ntg - 64 * batch) : 64);
+    const int64_t b0 = bstart[batch], b1 = batch + 1 < nb ? bstart[batch + 1] : ntg;
+    const int nv = (int)(b1 - b0);
     for (int l = 0; l < nv; ++l) {
-        const int64_t i = order[64 * batch + l];
+        const int64_t i = order[b0 + l];
         const double t[3] = {tx[i], ty[i], tz[i]};
         double E[3] = {0.0, 0.0, 0.0};
         for (int64_t qq = n->begin; qq < n->end; ++qq) {
@@ -751,5 +814,6 @@ void oracle_pair_errors(const double* sx, const double* sy, const double* sz, co
     }
     free(coef);
     free(order);
+    free(bstart);
     cartesian_env_free(&e);
 }

@@ -60,6 +60,8 @@ def _declare_tree(L: C.CDLL) -> None:
         L.oracle_coeffs_series.restype = C.c_int
         L.oracle_target_order.argtypes = [_vp, _vp, _vp, C.c_int64, _vp]
         L.oracle_target_order.restype = None
+        L.oracle_target_batches.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp]
+        L.oracle_target_batches.restype = C.c_int64
         L.oracle_tree_eval_batch.argtypes = [_vp] * 5 + [C.c_int, C.c_double, C.c_double] + [_vp] * 3 + \
             [C.c_int64] + [_vp] * 4
         L.oracle_tree_eval_batch.restype = None
@@ -215,6 +217,17 @@ def target_order(targets):
     out = np.empty(tx.size, np.int64)
     lib().oracle_target_order(tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, tx.size, out.ctypes.data)
     return out
+
+
+def target_batches(targets):
+    """(order, bstart): Morton order and the first sorted position of every
+    target batch of the batch treecode (last built oracle tree's root box)."""
+    tx, ty, tz = (_c(a) for a in targets)
+    order = np.empty(tx.size, np.int64)
+    bstart = np.empty(max(tx.size, 1), np.int64)
+    nb = lib().oracle_target_batches(tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, tx.size, order.ctypes.data,
+                                     bstart.ctypes.data)
+    return order, bstart[:nb].copy()
 
 
 def tree_eval_batch(sources, gamma, moments, p, theta, targets, delta=0.0):

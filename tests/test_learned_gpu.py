@@ -69,7 +69,7 @@ def test_batch_converges_and_ragged(ctx, mesh):
     for n in (1, 63, 65, 1000):
         sub = tuple(a[:n].copy() for a in tgt)
         us = T.evaluate_batch(ctx, sub, 0.5, 8)
-        assert O.rel_l2(np.stack(us), ud[:, :n]) < 1e-5
+        assert O.rel_l2(np.stack(us), ud[:, :n]) < 1e-4
     T.evaluate_batch(ctx, tuple(a[:0].copy() for a in tgt), 0.5, 8)
 
 
@@ -187,12 +187,15 @@ def test_labels_match_oracle(ctx, mesh):
         c = R["cent"][g2o[node]]
         assert ls.features[k, 9] == (1.0 if R["n_kids"][g2o[node]] == 0 else 0.0)
         assert abs(np.exp2(ls.features[k, 6]) - (R["end"][g2o[node]] - R["begin"][g2o[node]])) < 1e-3
-        order = O.target_order(tgt)[64 * b:64 * b + 64]
-        T_b = np.stack([a[order] for a in tgt], 1)
+        order, bstart = O.target_batches(tgt)
+        b1 = bstart[b + 1] if b + 1 < len(bstart) else len(order)
+        T_b = np.stack([a[order[bstart[b]:b1]] for a in tgt], 1)
         cb = T_b.mean(0)
         s = np.linalg.norm(cb - c[:3])
         assert abs(ls.features[k, 8] - np.log2(s)) < 1e-5
-        assert abs(ls.features[k, 0] - np.log2(c[3] / s)) < 1e-5
+        ids = R["perm"][R["begin"][g2o[node]]:R["end"][g2o[node]]]
+        rn = np.sqrt(((np.stack([a[ids] for a in src], 1) - c[:3]) ** 2).sum(1).max())
+        assert abs(ls.features[k, 0] - np.log2(rn / s)) < 1e-5
 
 
 def test_learned_constant_model_reduces_to_plain(ctx, mesh):

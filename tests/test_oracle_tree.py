@@ -206,6 +206,30 @@ def test_target_order_is_stable_morton(mesh):
     assert np.array_equal(order, np.argsort(key, kind="stable"))
 
 
+def test_target_batches_are_cell_aligned(mesh):
+    m = mesh(L.LPM_SEED_CUBED, 5)
+    src = (m.cx, m.cy, m.cz)
+    R = O.tree_build(src, m.max_radius, 32)
+    tgt = m.targets()
+    n = len(tgt[0])
+    order, bstart = O.target_batches(tgt)
+    assert np.array_equal(order, O.target_order(tgt))
+    sizes = np.diff(np.append(bstart, n))
+    assert bstart[0] == 0 and np.all(sizes >= 1) and np.all(sizes <= 64)
+    key = _morton_keys(np.stack(tgt, 1)[order], R["box"][0, :3], R["box"][0, 3:])
+    # the level of the split: shallowest with >= n/256 occupied cells
+    lcp = np.array([21 if a == b else (62 - (int(a ^ b).bit_length() - 1)) // 3 for a, b in zip(key[:-1], key[1:])])
+    cells = [1 + np.sum(lcp < l) for l in range(22)]
+    lev = next(l for l in range(22) if cells[l] >= max(1, n // 256))
+    cell_start = np.maximum.accumulate(np.where(np.append(True, lcp < lev), np.arange(n), 0))
+    expect = np.flatnonzero((np.arange(n) - cell_start) % 64 == 0)
+    assert np.array_equal(bstart, expect)
+    # every batch inside one level-L cell, and compact
+    shift = np.uint64(3 * (21 - lev))
+    for b0, sz in zip(bstart, sizes):
+        assert len(set((key[b0:b0 + sz] >> shift).tolist())) == 1
+
+
 def test_batch_oracle_limits(mesh):
     m = mesh(L.LPM_SEED_CUBED, 4)
     src = (m.cx, m.cy, m.cz)
@@ -238,7 +262,7 @@ def test_pair_errors_match_single_box_series(mesh):
     tgt = m.targets()
     R = O.tree_build(src, m.max_radius, 32)
     mom = O.tree_moments(src, g, 12, 1)
-    order = O.target_order(tgt)
+    order, bstart = O.target_batches(tgt)
     u_ref = float(np.abs(g).sum() / (4 * np.pi))
     node = int(np.flatnonzero(R["level"] == 2)[3])
     batch = 37
@@ -247,7 +271,7 @@ def test_pair_errors_match_single_box_series(mesh):
     ps = tuple(a[ids] for a in src)
     c = R["cent"][node, :3]
     ref = np.zeros(12)
-    for i in order[64 * batch:64 * batch + 64]:
+    for i in order[bstart[batch]:bstart[batch + 1]]:
         t = np.array([tgt[0][i], tgt[1][i], tgt[2][i]])
         y = np.stack(ps, 1)
         ex = np.cross(t, (g[ids][:, None] * y / (1.0 - y @ t)[:, None]).sum(0))
