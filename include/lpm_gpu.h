@@ -155,6 +155,12 @@ int lpm_tree_export(lpm_ctx* ctx, double* box, double* cent, int32_t* level, int
  * Replaces: SumNode::moments / momentString(), src/LpmTreeSum.h:36-44. */
 int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
 
+/* Duration of the last pair or evaluation kernel (ds_f64 / ds_f32c of
+ * lpm_direct_sum*, the evaluation kernel of lpm_tree_eval*), measured with
+ * HIP events recorded on the context stream around that kernel alone.
+ * Waits for the kernel.  No counterpart in the reference. */
+int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
+
 /* Counters of the last tree evaluation (any of the lpm_tree_eval* calls):
  * out[0] far-field (target, node) pairs, out[1] near-field (target, source)
  * pairs, out[2] far-field series terms summed over the far pairs

@@ -118,6 +118,7 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_tree_export_moments": [_vp, _vp],
         "lpm_tree_last_stats": [_vp, _i64p, _i64p],
         "lpm_tree_stats": [_vp, _vp],
+        "lpm_last_kernel_ms": [_vp, _vp],
         "lpm_tree_eval_batch": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp,
                                 _vp, _vp],
         "lpm_tree_eval_batch_dev": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64,
@@ -249,6 +250,12 @@ class Context:
     @property
     def stream(self) -> int:
         return self.lib.lpm_stream(self.h)
+
+    def last_kernel_ms(self) -> float:
+        """Duration of the last pair / evaluation kernel (HIP events on the context stream)."""
+        ms = C.c_float()
+        self.check(self.lib.lpm_last_kernel_ms(self.h, C.byref(ms)))
+        return ms.value
 
     def synchronize(self) -> None:
         self.check(self.lib.lpm_synchronize(self.h))

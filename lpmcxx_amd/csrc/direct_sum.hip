@@ -432,9 +432,11 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         if (pot) hipLaunchKernelGGL(ds_gamma_total, dim3(1), dim3(1024), 0, st, gamma, ns, gtot);
         const float eps2v = float(2.0 * delta * delta);
         const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
+        mark_begin(c);
         if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
+        mark_end(c);
         LPM_HIP(c, hipGetLastError());
         hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp, nt,
                            tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),
@@ -447,9 +449,11 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         const int64_t per = int64_t(tps) * kTile;
         const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
         const double R2 = radius * radius, d2 = delta * delta;
+        mark_begin(c);
         if (vel && pot) launch_f64<kTplF64, true, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
         else if (vel) launch_f64<kTplF64, true, false>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
         else launch_f64<kTplF64, false, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        mark_end(c);
         LPM_HIP(c, hipGetLastError());
         hipLaunchKernelGGL(ds_reduce<false>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp,
                            nt, tx, ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius),

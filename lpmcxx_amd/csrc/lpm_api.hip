@@ -53,7 +53,10 @@ int lpm_create(lpm_ctx** out, int device) {
     lpm_ctx* c = new (std::nothrow) lpm_ctx();
     if (!c) return LPM_ERR_ALLOC;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_beg) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return LPM_ERR_HIP;
     }
@@ -69,10 +72,21 @@ void lpm_destroy(lpm_ctx* c) {
     model_free(c->model);
     c->tree = nullptr;
     for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work,
-                      &c->tb_work})
+                      &c->tb_work, &c->tl_work})
         b->release();
+    if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
+    if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int lpm_last_kernel_ms(lpm_ctx* c, float* ms) {
+    if (!c || !ms) return LPM_ERR_NULL_PTR;
+    if (!c->ev_valid) return fail(c, LPM_ERR_STATE, "no timed kernel yet");
+    LPM_HIP(c, hipSetDevice(c->device));
+    LPM_HIP(c, hipEventSynchronize(c->ev_end));
+    LPM_HIP(c, hipEventElapsedTime(ms, c->ev_beg, c->ev_end));
+    return LPM_OK;
 }
 
 const char* lpm_status_string(int s) {

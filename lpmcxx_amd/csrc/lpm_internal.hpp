@@ -42,16 +42,31 @@ struct lpm_ctx {
     lpm::DevBuf h_in, h_out;
     // RK4 stage storage
     lpm::DevBuf rk_work;
-    // target batches of the batch treecode
-    lpm::DevBuf tb_work;
+    // target batches of the batch treecode, interaction lists of the learned one
+    lpm::DevBuf tb_work, tl_work;
+    int far_cap = 1024, near_cap = 512;
     lpm::Tree* tree = nullptr;
     lpm::Model* model = nullptr;
     unsigned long long* stats_ptr = nullptr;  // kStats counters of the last tree evaluation (device)
+    // events around the last pair / evaluation kernel (lpm_last_kernel_ms)
+    hipEvent_t ev_beg = nullptr, ev_end = nullptr;
+    bool ev_valid = false;
 };
 
 namespace lpm {
 
 int fail(lpm_ctx* c, int status, const char* fmt, ...);
+
+// bracket the main kernel of an entry point with the context's timing events
+inline void mark_begin(lpm_ctx* c) {
+    if (c->ev_beg) (void)hipEventRecord(c->ev_beg, c->stream);
+}
+inline void mark_end(lpm_ctx* c) {
+    if (c->ev_end) {
+        (void)hipEventRecord(c->ev_end, c->stream);
+        c->ev_valid = true;
+    }
+}
 int hip_fail(lpm_ctx* c, hipError_t e, const char* what);
 
 #define LPM_HIP(ctx, call)                                  \

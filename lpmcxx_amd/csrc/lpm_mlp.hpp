@@ -57,28 +57,65 @@ __device__ inline void wave_sync() {
 // A[m][k]: lane holds A[c][4g + i]; B[k][n]: lane holds B[4g + i][c];
 // D[m][n]: lane holds D[4g + i][c], i = 0..3.
 
-// One row (wave-uniform, normalized features in h[0..15]) through the network
-// on the MFMA units, the row placed in tile row 0; returns the argmax class.
-// w: BF16 weights + FP32 biases (Model::wbf layout); h: 144 floats of
-// wave-private LDS (inputs 0..15, hidden layers 16..79 and 80..143).
-__device__ inline int mlp_classify_row(const unsigned short* __restrict__ w, float* h) {
-    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+// Weights of the model as MFMA B fragments in LDS: 24 tiles x 64 lanes x 4
+// BF16 (W1: tiles 0..3 = n-tiles, W2: 4 + 4 kt + nt, W3: 20 + kt), then the
+// FP32 biases b1, b2, b3.
+constexpr int kFragTiles = 24;
+constexpr int kFragBytes = kFragTiles * 64 * 8;
+constexpr int kModelLds = kFragBytes + (2 * kHid + kOut) * 4;
+
+// Fill the LDS model image from the device copy (BF16 weights + FP32
+// biases); all threads of the block take part, the caller synchronizes.
+__device__ inline void mlp_stage_weights(const unsigned short* __restrict__ w, unsigned char* lds) {
+    uint2* frag = reinterpret_cast<uint2*>(lds);
     const unsigned short* W1 = w;
     const unsigned short* W2 = w + kIn * kHid;
     const unsigned short* W3 = W2 + kHid * kHid;
+    for (int f = threadIdx.x; f < kFragTiles * 64; f += blockDim.x) {
+        const int tile = f >> 6, l = f & 63, g = l >> 4, c = l & 15;
+        unsigned short e[4];
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * g + i;
+            if (tile < 4) e[i] = W1[k * kHid + tile * 16 + c];
+            else if (tile < 20) e[i] = W2[(((tile - 4) >> 2) * 16 + k) * kHid + ((tile - 4) & 3) * 16 + c];
+            else e[i] = W3[((tile - 20) * 16 + k) * kOut + c];
+        }
+        frag[f] = make_uint2(unsigned(e[0]) | (unsigned(e[1]) << 16), unsigned(e[2]) | (unsigned(e[3]) << 16));
+    }
     const float* fb = reinterpret_cast<const float*>(w + kBfWeights);
+    float* lb = reinterpret_cast<float*>(lds + kFragBytes);
+    for (int i = threadIdx.x; i < 2 * kHid + kOut; i += blockDim.x) lb[i] = fb[i];
+}
+
+__device__ inline bf16x4 as_bf16x4(uint2 v) {
+    bf16x4 r;
+    r[0] = short(v.x & 0xFFFFu);
+    r[1] = short(v.x >> 16);
+    r[2] = short(v.y & 0xFFFFu);
+    r[3] = short(v.y >> 16);
+    return r;
+}
+
+// Up to 16 rows (normalized inputs as BF16 in X[16][16], wave-private LDS)
+// through the network on the MFMA units; the argmax class of row r goes to
+// cls[r].  H1, H2: wave-private LDS scratch of 16 x 64 BF16 each.  The row
+// arithmetic is that of the training forward pass (same instruction, same
+// k order), so the classes equal lpm_model_forward's.
+__device__ inline void mlp_classify_rows(const unsigned char* model, const unsigned short* X, unsigned short* H1,
+                                         unsigned short* H2, int* cls) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const uint2* frag = reinterpret_cast<const uint2*>(model);
+    const float* b1 = reinterpret_cast<const float*>(model + kFragBytes);
+    const float* b2 = b1 + kHid;
+    const float* b3 = b2 + kHid;
     wave_sync();
-    bf16x4 a;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = (c == 0) ? short(f2bf(h[4 * g + i])) : short(0);
+    const bf16x4 a1 = as_bf16x4(*reinterpret_cast<const uint2*>(X + c * kIn + 4 * g));
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        bf16x4 b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) b[i] = short(W1[(4 * g + i) * kHid + nt * 16 + c]);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(a, b, acc);
-        if (g == 0) h[16 + nt * 16 + c] = fmaxf(acc[0] + fb[nt * 16 + c], 0.f);
+        acc = mfma16(a1, as_bf16x4(frag[nt * 64 + lane]), acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) H1[(4 * g + i) * kHid + nt * 16 + c] = f2bf(fmaxf(acc[i] + b1[nt * 16 + c], 0.f));
     }
     wave_sync();
 #pragma unroll
@@ -86,38 +123,32 @@ __device__ inline int mlp_classify_row(const unsigned short* __restrict__ w, flo
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
-            bf16x4 b;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = (c == 0) ? short(f2bf(h[16 + kt * 16 + 4 * g + i])) : short(0);
-                b[i] = short(W2[(kt * 16 + 4 * g + i) * kHid + nt * 16 + c]);
-            }
-            acc = mfma16(a, b, acc);
+            const bf16x4 a = as_bf16x4(*reinterpret_cast<const uint2*>(H1 + c * kHid + kt * 16 + 4 * g));
+            acc = mfma16(a, as_bf16x4(frag[(4 + 4 * kt + nt) * 64 + lane]), acc);
         }
-        if (g == 0) h[80 + nt * 16 + c] = fmaxf(acc[0] + fb[kHid + nt * 16 + c], 0.f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) H2[(4 * g + i) * kHid + nt * 16 + c] = f2bf(fmaxf(acc[i] + b2[nt * 16 + c], 0.f));
     }
     wave_sync();
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kt = 0; kt < 4; ++kt) {
-        bf16x4 b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            a[i] = (c == 0) ? short(f2bf(h[80 + kt * 16 + 4 * g + i])) : short(0);
-            b[i] = short(W3[(kt * 16 + 4 * g + i) * kOut + c]);
-        }
-        acc = mfma16(a, b, acc);
+        const bf16x4 a = as_bf16x4(*reinterpret_cast<const uint2*>(H2 + c * kHid + kt * 16 + 4 * g));
+        acc = mfma16(a, as_bf16x4(frag[(20 + kt) * 64 + lane]), acc);
     }
-    float v = (g == 0 && c < kClasses) ? acc[0] + fb[2 * kHid + c] : -3.0e38f;
-    int best = c;
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(v, o, 64);
-        const int ob = __shfl_xor(best, o, 64);
-        if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
+    for (int i = 0; i < 4; ++i) {
+        float v = (c < kClasses) ? acc[i] + b3[c] : -3.0e38f;
+        int best = c;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(v, o, 64);
+            const int ob = __shfl_xor(best, o, 64);
+            if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
+        }
+        if (c == 0) cls[4 * g + i] = best;
     }
     wave_sync();
-    return __builtin_amdgcn_readfirstlane(best);
 }
 
 }  // namespace lpm

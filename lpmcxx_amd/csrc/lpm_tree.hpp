@@ -4,6 +4,8 @@
 
 #include "lpm_internal.hpp"
 
+#include <vector>
+
 namespace lpm {
 
 constexpr int kMaxDepth = LPM_TREE_MAX_DEPTH;  // 21 levels x 3 bits = 63-bit keys
@@ -25,7 +27,9 @@ struct Tree {
     DevBuf nprefix;   // uint64 [cap] key prefix (top 3*level bits)
     DevBuf ncent;     // double4 [nn] centroid + maxRadius
     DevBuf nbox;      // double [nn][6] lo3 hi3
-    DevBuf nrad;      // double [nn] max |y - centroid| over the node's sources (0 if empty)
+    DevBuf nrad;      // double [nn] source radius about the box centroid: exact at leaves,
+                      // max over children of |c_child - c| + r_child above (0 if empty)
+    std::vector<int64_t> level_start;  // host: first node of every level, then nn
     int64_t cap = 0;
     // moments
     int order = 0;        // p (|k| < p), 0 = none

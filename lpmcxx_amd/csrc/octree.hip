@@ -171,14 +171,16 @@ __global__ __launch_bounds__(256) void iota_i32(int* v, int64_t n) {
     if (i < n) v[i] = int(i);
 }
 
-// Largest distance of a node's sources from its box centroid (one block per node).
-__global__ __launch_bounds__(256) void tree_node_radius(int nn, const int* __restrict__ nbeg,
-                                                        const int* __restrict__ nend,
+// Source radius of every leaf about its box centroid: the largest distance
+// of its sources (one block per node; internal nodes are set by
+// tree_radius_up).
+__global__ __launch_bounds__(256) void tree_radius_leaf(int nn, const int* __restrict__ nkids,
+                                                        const int* __restrict__ nbeg, const int* __restrict__ nend,
                                                         const double4* __restrict__ ncent,
                                                         const double4* __restrict__ spos, double* __restrict__ nrad) {
     __shared__ double red[256];
     const int i = blockIdx.x;
-    if (i >= nn) return;
+    if (i >= nn || nkids[i] != 0) return;
     const double4 c = ncent[i];
     double r2 = 0.0;
     for (int q = nbeg[i] + threadIdx.x; q < nend[i]; q += blockDim.x) {
@@ -193,6 +195,25 @@ __global__ __launch_bounds__(256) void tree_node_radius(int nn, const int* __res
         __syncthreads();
     }
     if (threadIdx.x == 0) nrad[i] = sqrt(red[0]);
+}
+
+// Internal nodes of one level: r = max over children of |c_child - c| + r_child.
+__global__ __launch_bounds__(256) void tree_radius_up(int first, int count, const int* __restrict__ nfirst,
+                                                      const int* __restrict__ nkids,
+                                                      const double4* __restrict__ ncent, double* __restrict__ nrad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int i = first + t;
+    const int nk = nkids[i];
+    if (nk == 0) return;
+    const double4 c = ncent[i];
+    double r = 0.0;
+    for (int k = nfirst[i]; k < nfirst[i] + nk; ++k) {
+        const double4 ck = ncent[k];
+        const double dx = ck.x - c.x, dy = ck.y - c.y, dz = ck.z - c.z;
+        r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz) + nrad[k]);
+    }
+    nrad[i] = r;
 }
 
 __global__ __launch_bounds__(256) void gather_pos(const double* __restrict__ x, const double* __restrict__ y,
@@ -296,6 +317,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     LPM_HIP(c, hipStreamSynchronize(st));
     int64_t nn = 1, lvl_start = 0, lvl_count = 1;
     int depth = 0;
+    t->level_start.assign(1, 0);
     std::vector<int> hcnt(2);
     for (int level = 0;; ++level) {
         LPM_CHECK(t->tmp3.reserve(size_t(2 * lvl_count + 2) * sizeof(int)));
@@ -327,7 +349,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         lvl_count = total;
         nn += total;
         depth = level + 1;
+        t->level_start.push_back(lvl_start);
     }
+    t->level_start.push_back(nn);
     t->nn = nn;
     t->depth = depth;
 
@@ -348,8 +372,15 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                            t->spos.as<double4>());
     }
     LPM_CHECK(t->nrad.reserve(size_t(nn) * sizeof(double)));
-    hipLaunchKernelGGL(tree_node_radius, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nbeg.as<int>(),
-                       t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(), t->nrad.as<double>());
+    LPM_HIP(c, hipMemsetAsync(t->nrad.ptr, 0, size_t(nn) * sizeof(double), st));
+    hipLaunchKernelGGL(tree_radius_leaf, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+                       t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
+                       t->nrad.as<double>());
+    for (int l = depth - 1; l >= 0; --l) {
+        const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
+        hipLaunchKernelGGL(tree_radius_up, dim3(blocks(cnt)), dim3(256), 0, st, int(f), int(cnt),
+                           t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), t->nrad.as<double>());
+    }
     LPM_HIP(c, hipGetLastError());
     std::vector<int> hk(nn);
     LPM_HIP(c, hipMemcpyAsync(hk.data(), t->nkids.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));

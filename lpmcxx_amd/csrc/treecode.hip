@@ -119,10 +119,9 @@ __device__ inline int lex_to_graded(int p, int r) {
     return graded_index(a, b, r);
 }
 
-__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int p, int nterm, const int* __restrict__ node_chunk0,
+__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, const int* __restrict__ node_chunk0,
                                                            const int* __restrict__ node_nchunk,
-                                                           const double* __restrict__ part, double* __restrict__ mom,
-                                                           double* __restrict__ momg, float* __restrict__ mom32) {
+                                                           const double* __restrict__ part, double* __restrict__ mom) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= int64_t(nn) * nterm * 3) return;
     const int node = int(i / (int64_t(nterm) * 3));
@@ -131,7 +130,80 @@ __global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int p, int nt
     double s = 0.0;
     for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * 3 + rem];
     mom[i] = s;
+}
+
+struct BinomTab {
+    double v[kMaxOrder][kMaxOrder];
+    constexpr BinomTab() : v() {
+        for (int n = 0; n < kMaxOrder; ++n) {
+            v[n][0] = 1.0;
+            for (int k = 1; k <= n; ++k) v[n][k] = v[n - 1][k - 1] + (k < n ? v[n - 1][k] : 0.0);
+        }
+    }
+};
+__constant__ BinomTab c_binom{};
+
+// Upward pass for the internal nodes of one level (moments about the box
+// centroid, h = sign (y - c)): with d = sign (c_child - c),
+//   m_k(node) = sum_children sum_{j <= k} C(k, j) d^(k - j) m_j(child)
+// (componentwise binomials), the same sums as the particle-by-particle
+// definition of the leaves, regrouped.
+__global__ __launch_bounds__(256) void tree_m2m(int first, int count, int p, int nterm, double sign,
+                                                const int* __restrict__ nfirst, const int* __restrict__ nkids,
+                                                const double4* __restrict__ ncent, double* __restrict__ mom) {
+    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (tid >= int64_t(count) * nterm) return;
+    const int i = first + int(tid / nterm);
+    const int t = int(tid - int64_t(i - first) * nterm);
+    const int nk = nkids[i];
+    if (nk == 0) return;
+    int r = t, k1 = 0, k2 = 0;
+    while (r >= (p - k1) * (p - k1 + 1) / 2) { r -= (p - k1) * (p - k1 + 1) / 2; ++k1; }
+    while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
+    const int k3 = r;
+    const double4 c = ncent[i];
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int ch = nfirst[i]; ch < nfirst[i] + nk; ++ch) {
+        const double4 cc = ncent[ch];
+        const double dx = sign * (cc.x - c.x), dy = sign * (cc.y - c.y), dz = sign * (cc.z - c.z);
+        const double* mc = mom + int64_t(ch) * nterm * 3;
+        double pa = 1.0;
+        for (int a = k1; a >= 0; --a) {
+            const double ca = c_binom.v[k1][a] * pa;
+            double pb = 1.0;
+            for (int b = k2; b >= 0; --b) {
+                const double cb = ca * c_binom.v[k2][b] * pb;
+                double pc = 1.0;
+                // lex position of (a, b, 0), then + cz
+                int idx = 0;
+                for (int q = 0; q < a; ++q) idx += (p - q) * (p - q + 1) / 2;
+                for (int q = 0; q < b; ++q) idx += p - a - q;
+                for (int cz = k3; cz >= 0; --cz) {
+                    const double w = cb * c_binom.v[k3][cz] * pc;
+                    const double* mj = mc + 3 * (idx + cz);
+                    m0 = fma(w, mj[0], m0);
+                    m1 = fma(w, mj[1], m1);
+                    m2 = fma(w, mj[2], m2);
+                    pc *= dz;
+                }
+                pb *= dy;
+            }
+            pa *= dx;
+        }
+    }
+    double* o = mom + (int64_t(i) * nterm + t) * 3;
+    o[0] = m0; o[1] = m1; o[2] = m2;
+}
+
+// graded FP64 and FP32 copies of the lex moments
+__global__ __launch_bounds__(256) void tree_moments_graded(int nn, int p, int nterm, const double* __restrict__ mom,
+                                                           double* __restrict__ momg, float* __restrict__ mom32) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(nn) * nterm * 3) return;
+    const int node = int(i / (int64_t(nterm) * 3));
+    const int rem = int(i - int64_t(node) * nterm * 3);
     const int64_t g = int64_t(node) * nterm * 3 + 3 * lex_to_graded(p, rem / 3) + rem % 3;
+    const double s = mom[i];
     momg[g] = s;
     mom32[g] = float(s);
 }
@@ -360,19 +432,21 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     hipStream_t st = c->stream;
     const int nterm = nterms(p);
     const int64_t nn = t->nn;
-    // chunk list from the node ranges
-    std::vector<int> hb(nn), he(nn);
+    // chunk list of the leaves (internal nodes come from the upward pass)
+    std::vector<int> hb(nn), he(nn), hk(nn);
     LPM_HIP(c, hipMemcpyAsync(hb.data(), t->nbeg.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipMemcpyAsync(he.data(), t->nend.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(hk.data(), t->nkids.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipStreamSynchronize(st));
     std::vector<int> cn, cb, ce, n0(nn), nc(nn);
     for (int64_t i = 0; i < nn; ++i) {
         n0[i] = int(cn.size());
-        for (int b = hb[i]; b < he[i]; b += kChunk) {
-            cn.push_back(int(i));
-            cb.push_back(b);
-            ce.push_back(std::min(he[i], b + kChunk));
-        }
+        if (hk[i] == 0)
+            for (int b = hb[i]; b < he[i]; b += kChunk) {
+                cn.push_back(int(i));
+                cb.push_back(b);
+                ce.push_back(std::min(he[i], b + kChunk));
+            }
         nc[i] = int(cn.size()) - n0[i];
     }
     const int64_t nch = int64_t(cn.size());
@@ -404,8 +478,18 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
         hipLaunchKernelGGL(tree_moments_chunk, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
                            t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->tmp2.as<double>());
     }
-    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm, d_n0,
-                       d_nc, t->tmp2.as<double>(), t->mom.as<double>(), t->momg.as<double>(), t->mom32.as<float>());
+    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), nterm, d_n0,
+                       d_nc, t->tmp2.as<double>(), t->mom.as<double>());
+    // upward pass, deepest internal level first
+    for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
+        const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(tree_m2m, dim3(blocks(cnt * nterm)), dim3(256), 0, st, int(f), int(cnt), p, nterm,
+                           double(sign), t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(),
+                           t->mom.as<double>());
+    }
+    hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm,
+                       t->mom.as<double>(), t->momg.as<double>(), t->mom32.as<float>());
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(nn)), dim3(256), 0, st, int(nn), p, nterm, t->momg.as<double>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
@@ -514,6 +598,7 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
                            t->spos.as<double4>(), t->soff32.as<float4>(), t->mom.as<double>(), t->momg.as<double>(), p, nterm, theta, delta,
                            radius * radius, delta * delta, eps2v, mult, ux, uy, uz, d_mac);
     };
+    mark_begin(c);
     if (series == LPM_SERIES_REFERENCE) {
         if (prec == LPM_F32C) args(tree_eval<0, true>);
         else args(tree_eval<0, false>);
@@ -530,6 +615,7 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
             default: return fail(c, LPM_ERR_INVALID_ARG, "series order %d unsupported", p);
         }
     }
+    mark_end(c);
     LPM_HIP(c, hipGetLastError());
     if (mac_out) {
         unsigned long long h = 0;

@@ -13,8 +13,9 @@
 //            full SIMD width, unlike the per-target MAC of the reference
 //            (which tree_eval keeps for parity).
 //   learned  pairs with r_n + r_b < theta_max |c_b - c_n| go through the order
-//            model on the MFMA units (mlp_classify_row): class 0 descends,
-//            class q accepts with order min(q + margin, max_order).
+//            model on the MFMA units (mlp_classify_rows, all children of a
+//            node in one pass): class 0 descends, class q <= max_order
+//            accepts with order min(q + margin, max_order).
 //   labels   for a sample of batches the wave visits every node; for a sample
 //            of the pairs that pass theta_max it records the ten features and
 //            the error of the order-1..12 far field against the exact sum over
@@ -186,9 +187,9 @@ struct Targets {
     int64_t nb;
 };
 
-// Target of this lane in batch b (b < nb).
-__device__ inline void load_target(const Targets& tg, int64_t b, bool& valid, int& orig, double& t0, double& t1,
-                                   double& t2) {
+// Target of this lane in batch b (b < nb); pos = its sorted position.
+__device__ inline int64_t load_target(const Targets& tg, int64_t b, bool& valid, int& orig, double& t0, double& t1,
+                                      double& t2) {
     const int64_t b0 = tg.bstart[b], b1 = b + 1 < tg.nb ? tg.bstart[b + 1] : tg.n;
     const int64_t it = b0 + (threadIdx.x & 63);
     valid = it < b1;
@@ -196,6 +197,7 @@ __device__ inline void load_target(const Targets& tg, int64_t b, bool& valid, in
     t0 = valid ? tg.x[orig] : 0.0;
     t1 = valid ? tg.y[orig] : 0.0;
     t2 = valid ? tg.z[orig] : 0.0;
+    return it;
 }
 
 __device__ inline void flush_stats(unsigned long long* stats, unsigned long long far, unsigned long long near,
@@ -259,15 +261,40 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
 }
 
 // ---------------------------------------------------------------- learned batch treecode
-template <int MAXP, bool F32>
-__global__ __launch_bounds__(256) void tree_eval_learned(Targets tg, TreeView tv, double theta_max, int margin,
-                                                         int max_order, double u_ref, double d2, float eps2v,
-                                                         double mult, const unsigned short* __restrict__ wts,
-                                                         MlpNorm norm, double* __restrict__ ux,
-                                                         double* __restrict__ uy, double* __restrict__ uz,
-                                                         unsigned long long* __restrict__ stats) {
+// Two passes.  tree_plan_learned walks the tree per batch and decides every
+// pair with the order model; the result is an interaction list per batch:
+// far entries (node | p << 27) and near leaves.  When a wave descends into
+// node n, the children of n are classified together: lane k < nkids computes
+// the features of child k, the rows go through the model in one pass, and
+// each child is pushed with its class (0 = descend, also for children failing
+// theta_max).  The lists are then evaluated by tree_near_list (writes the
+// per-target FP64 sums) and tree_far_list<P> for P = 1..max_order (adds the
+// entries of order P), each at the occupancy of its own register footprint;
+// tree_list_finish scales and scatters.
+constexpr int kNodeBits = 27;
+
+struct Lists {
+    int* far;
+    int* near;
+    int* far_cnt;
+    int* near_cnt;
+    int far_cap, near_cap;
+    int* overflow;
+};
+
+__global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
+                                                         int max_order, double u_ref,
+                                                         const unsigned short* __restrict__ wts, MlpNorm norm,
+                                                         Lists ls, unsigned long long* __restrict__ stats) {
+    __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackB];
-    __shared__ float s_h[4][144];
+    __shared__ int s_cls[4][kStackB];
+    __shared__ __align__(16) unsigned short s_x[4][16 * kIn];
+    __shared__ __align__(16) unsigned short s_h1[4][16 * kHid];
+    __shared__ __align__(16) unsigned short s_h2[4][16 * kHid];
+    __shared__ int s_row[4][16];
+    mlp_stage_weights(wts, s_model);
+    __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t bi = int64_t(blockIdx.x) * 4 + w;
     if (bi >= tg.nb) return;
@@ -276,51 +303,144 @@ __global__ __launch_bounds__(256) void tree_eval_learned(Targets tg, TreeView tv
     double t0, t1, t2;
     load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    unsigned long long nfar = 0, nnear = 0, nterms = 0, ncalls = 0;
+    int* far = ls.far + bi * ls.far_cap;
+    int* near = ls.near + bi * ls.near_cap;
+    int nfar = 0, nnear = 0;
+    unsigned long long near_pairs = 0, nterms = 0, ncalls = 0;
     int sp = 1;
     s_node[w][0] = 0;
+    s_cls[w][0] = 0;
     while (sp > 0) {
-        const int n = __builtin_amdgcn_readfirstlane(s_node[w][--sp]);
-        const double4 c = tv.ncent[n];
-        const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
-        const double s = sqrt(dx * dx + dy * dy + dz * dz);
+        --sp;
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
+        const int cls = __builtin_amdgcn_readfirstlane(s_cls[w][sp]);
+        if (cls > 0) {
+            const int p = min(cls + margin, max_order);
+            if (lane == 0 && nfar < ls.far_cap) far[nfar] = n | (p << kNodeBits);
+            ++nfar;
+            nterms += unsigned(p * (p + 1) * (p + 2) / 6);
+            continue;
+        }
         const int nk = tv.nkids[n];
-        const double rn = tv.nrad[n];
-        if (uniform(rn + bt.r < theta_max * s)) {
-            float f[kIn];
-            pair_features(f, rn, bt.r, s, tv.nstat[n], nk == 0, bt.nvalid, u_ref);
-            float v = 0.f;
+        if (nk == 0) {
+            if (lane == 0 && nnear < ls.near_cap) near[nnear] = n;
+            ++nnear;
+            near_pairs += unsigned(tv.nend[n] - tv.nbeg[n]);
+            continue;
+        }
+        // classify the children: lane k < nk takes child f + k
+        const int f = tv.nfirst[n];
+        bool prop = false;
+        float x[kIn];
 #pragma unroll
-            for (int k = 0; k < kIn; ++k)
-                if (lane == k) v = (f[k] - norm.mean[k]) * norm.inv_std[k];
-            if (lane < kIn) s_h[w][lane] = v;
-            const int cls = mlp_classify_row(wts, s_h[w]);
-            ++ncalls;
-            if (cls > 0 && cls <= max_order) {
-                const int p = min(cls + margin, max_order);
-                far_node<MAXP, F32, true>(p, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2);
-                ++nfar;
-                nterms += unsigned(p * (p + 1) * (p + 2) / 6);
-                continue;
+        for (int k = 0; k < kIn; ++k) x[k] = 0.f;
+        if (lane < nk) {
+            const int ch = f + lane;
+            const double4 cc = tv.ncent[ch];
+            const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
+            const double sd = sqrt(dx * dx + dy * dy + dz * dz);
+            const double rn = tv.nrad[ch];
+            prop = rn + bt.r < theta_max * sd;
+            if (prop) {
+                pair_features(x, rn, bt.r, sd, tv.nstat[ch], tv.nkids[ch] == 0, bt.nvalid, u_ref);
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) x[k] = (x[k] - norm.mean[k]) * norm.inv_std[k];
             }
         }
-        if (nk == 0) {
-            const int b = tv.nbeg[n], e = tv.nend[n];
-            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
-            nnear += unsigned(e - b);
-        } else {
-            const int f = tv.nfirst[n];
-            for (int q = nk - 1; q >= 0; --q) s_node[w][sp++] = f + q;
+        int my_cls = 0;
+        if (__ballot(prop)) {
+            if (lane < 16) {
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) s_x[w][lane * kIn + k] = f2bf(x[k]);
+            }
+            mlp_classify_rows(s_model, s_x[w], s_h1[w], s_h2[w], s_row[w]);
+            ++ncalls;
+            if (prop) {
+                const int q = s_row[w][lane];
+                my_cls = (q <= max_order) ? q : 0;
+            }
         }
+        if (lane < nk) {
+            s_node[w][sp + nk - 1 - lane] = f + lane;
+            s_cls[w][sp + nk - 1 - lane] = my_cls;
+        }
+        sp += nk;
+        wave_sync();
     }
-    if (valid) {
-        ux[orig] = s0 * mult;
-        uy[orig] = s1 * mult;
-        uz[orig] = s2 * mult;
+    if (lane == 0) {
+        ls.far_cnt[bi] = nfar;
+        ls.near_cnt[bi] = nnear;
+        if (nfar > ls.far_cap || nnear > ls.near_cap) atomicOr(ls.overflow, 1);
     }
     const unsigned long long nv = unsigned(bt.nvalid);
-    flush_stats(stats, nfar * nv, nnear * nv, nterms * nv, ncalls);
+    flush_stats(stats, (unsigned long long)nfar * nv, near_pairs * nv, nterms * nv, ncalls);
+}
+
+// acc[3][nt] (sorted positions) = near-field sums of the batch's leaf list
+template <bool F32>
+__global__ __launch_bounds__(256) void tree_near_list(Targets tg, TreeView tv, Lists ls, double d2, float eps2v,
+                                                      double* __restrict__ acc) {
+    const int w = threadIdx.x >> 6;
+    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    if (bi >= tg.nb) return;
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
+    const int* near = ls.near + bi * ls.near_cap;
+    const int cnt = ls.near_cnt[bi];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < cnt; ++j) {
+        const int n = near[j];
+        near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
+    }
+    if (valid) {
+        acc[pos] = s0;
+        acc[tg.n + pos] = s1;
+        acc[2 * tg.n + pos] = s2;
+    }
+}
+
+// acc += far fields of the batch's entries of order P
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void tree_far_list(Targets tg, TreeView tv, Lists ls, double d2,
+                                                     double* __restrict__ acc) {
+    const int w = threadIdx.x >> 6;
+    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    if (bi >= tg.nb) return;
+    const int* far = ls.far + bi * ls.far_cap;
+    const int cnt = ls.far_cnt[bi];
+    int j = 0;
+    while (j < cnt && (far[j] >> kNodeBits) != P) ++j;
+    if (j == cnt) return;
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (; j < cnt; ++j) {
+        const int e = far[j];
+        if ((e >> kNodeBits) != P) continue;
+        const int n = e & ((1 << kNodeBits) - 1);
+        far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0, s1, s2);
+    }
+    if (valid) {
+        acc[pos] += s0;
+        acc[tg.n + pos] += s1;
+        acc[2 * tg.n + pos] += s2;
+    }
+}
+
+__global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ perm, int64_t n,
+                                                        const double* __restrict__ acc, double mult,
+                                                        double* __restrict__ ux, double* __restrict__ uy,
+                                                        double* __restrict__ uz) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    ux[o] = acc[i] * mult;
+    uy[o] = acc[n + i] * mult;
+    uz[o] = acc[2 * n + i] * mult;
 }
 
 // ---------------------------------------------------------------- labels
@@ -459,6 +579,7 @@ int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double
                            c->stats_ptr);
     };
     const bool f32 = prec == LPM_F32C;
+    mark_begin(c);
     switch (order) {
 #define LPM_CASE(PP)                                  \
     case PP:                                          \
@@ -470,6 +591,7 @@ int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double
 #undef LPM_CASE
         default: return fail(c, LPM_ERR_INVALID_ARG, "series order %d unsupported", order);
     }
+    mark_end(c);
     LPM_HIP(c, hipGetLastError());
     return LPM_OK;
 }
@@ -478,6 +600,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
                              double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
                              double* ux, double* uy, double* uz) {
     Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    if (t->nn >= (int64_t(1) << kNodeBits)) return fail(c, LPM_ERR_INVALID_ARG, "too many tree nodes");
     int *tperm = nullptr, *bstart = nullptr;
     int64_t nbatch = 0;
     LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch));
@@ -493,21 +617,54 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);
     const float eps2v = float(2.0 * d2);
     const unsigned short* w = model_weights(c);
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, tg, tv, theta_max, margin, max_order, u_ref, d2,
-                           eps2v, mult, w, norm, ux, uy, uz, c->stats_ptr);
-    };
-    const bool f32 = prec == LPM_F32C;
-    if (max_order <= 8) {
-        if (f32) go(tree_eval_learned<8, true>);
-        else go(tree_eval_learned<8, false>);
-    } else if (max_order <= 10) {
-        if (f32) go(tree_eval_learned<10, true>);
-        else go(tree_eval_learned<10, false>);
-    } else {
-        if (f32) go(tree_eval_learned<12, true>);
-        else go(tree_eval_learned<12, false>);
+    // interaction lists (grown and planned again if a batch overflows)
+    Lists ls;
+    for (int attempt = 0;; ++attempt) {
+        const size_t bytes = size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + 2) * sizeof(int) + 64 +
+                             size_t(3 * nt) * sizeof(double);
+        LPM_CHECK(c->tl_work.reserve(bytes));
+        double* acc = c->tl_work.as<double>();
+        ls.far = reinterpret_cast<int*>(acc + 3 * nt);
+        ls.near = ls.far + size_t(nbatch) * c->far_cap;
+        ls.far_cnt = ls.near + size_t(nbatch) * c->near_cap;
+        ls.near_cnt = ls.far_cnt + nbatch;
+        ls.overflow = ls.near_cnt + nbatch;
+        ls.far_cap = c->far_cap;
+        ls.near_cap = c->near_cap;
+        LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, sizeof(int), st));
+        if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
+        mark_begin(c);
+        hipLaunchKernelGGL(tree_plan_learned, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin, max_order, u_ref,
+                           w, norm, ls, c->stats_ptr);
+        LPM_HIP(c, hipGetLastError());
+        int ovf = 0;
+        LPM_HIP(c, hipMemcpyAsync(&ovf, ls.overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipStreamSynchronize(st));
+        if (!ovf) break;
+        if (attempt > 0) return fail(c, LPM_ERR_KERNEL, "interaction lists overflow after growing");
+        std::vector<int> fc(nbatch), ncnt(nbatch);
+        LPM_HIP(c, hipMemcpyAsync(fc.data(), ls.far_cnt, nbatch * sizeof(int), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(ncnt.data(), ls.near_cnt, nbatch * sizeof(int), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipStreamSynchronize(st));
+        c->far_cap = std::max(c->far_cap, *std::max_element(fc.begin(), fc.end()));
+        c->near_cap = std::max(c->near_cap, *std::max_element(ncnt.begin(), ncnt.end()));
     }
+    double* acc = c->tl_work.as<double>();
+    const bool f32 = prec == LPM_F32C;
+    if (f32) hipLaunchKernelGGL(tree_near_list<true>, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc);
+    else hipLaunchKernelGGL(tree_near_list<false>, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc);
+    auto far = [&](auto k32, auto k64) {
+        if (f32) hipLaunchKernelGGL(k32, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, acc);
+        else hipLaunchKernelGGL(k64, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, acc);
+    };
+#define LPM_FAR(PP) \
+    if (max_order >= PP) far(tree_far_list<PP, true>, tree_far_list<PP, false>);
+    LPM_FAR(1) LPM_FAR(2) LPM_FAR(3) LPM_FAR(4) LPM_FAR(5) LPM_FAR(6)
+    LPM_FAR(7) LPM_FAR(8) LPM_FAR(9) LPM_FAR(10) LPM_FAR(11) LPM_FAR(12)
+#undef LPM_FAR
+    hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult, ux,
+                       uy, uz);
+    mark_end(c);
     LPM_HIP(c, hipGetLastError());
     return LPM_OK;
 }

@@ -521,8 +521,8 @@ void oracle_node_far(double cx, double cy, double cz, double tx, double ty, doub
  * Morton keys of 21 bisections of the root box (clamped into it), ties by
  * index; batches are runs of at most 64 sorted targets inside one octree
  * cell (oracle_target_batches); node n is accepted when r_n + r_b <
- * theta |c_b - c_n| with r_n the largest distance of the node's particles
- * from its box centroid.  Batch centroid and radius are
+ * theta |c_b - c_n| with r_n the source radius of node_radius.  Batch
+ * centroid and radius are
  * formed with the xor-butterfly summation order of a 64-lane wave reduction
  * so the acceptance decisions reproduce the GPU's exactly. */
 typedef struct {
@@ -628,15 +628,28 @@ static void batch_geometry(const double (*t)[3], int nv, double c[4]) {
     c[3] = sqrt(butterfly(v, 1));
 }
 
-/* Largest distance of a node's particles from its box centroid. */
-static double node_radius(const onode* n, const double* sx, const double* sy, const double* sz) {
-    double r2 = 0.0;
-    for (int64_t q = n->begin; q < n->end; ++q) {
-        const int64_t j = g_perm[q];
-        const double dx = sx[j] - n->c[0], dy = sy[j] - n->c[1], dz = sz[j] - n->c[2];
-        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+/* Source radius about the box centroid: at a leaf the largest distance of
+ * its particles, above it max over children of |c_child - c| + r_child. */
+static double node_radius(int64_t id, const double* sx, const double* sy, const double* sz, double* nrad) {
+    const onode* n = &g_nodes[id];
+    double r = 0.0;
+    if (n->n_kids == 0) {
+        double r2 = 0.0;
+        for (int64_t q = n->begin; q < n->end; ++q) {
+            const int64_t j = g_perm[q];
+            const double dx = sx[j] - n->c[0], dy = sy[j] - n->c[1], dz = sz[j] - n->c[2];
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+        r = sqrt(r2);
+    } else {
+        for (int64_t k = n->first_kid; k < n->first_kid + n->n_kids; ++k) {
+            const double rk = node_radius(k, sx, sy, sz, nrad);
+            const double dx = g_nodes[k].c[0] - n->c[0], dy = g_nodes[k].c[1] - n->c[1], dz = g_nodes[k].c[2] - n->c[2];
+            r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz) + rk);
+        }
     }
-    return sqrt(r2);
+    nrad[id] = r;
+    return r;
 }
 
 /* Lex-order Cartesian setup shared by the batch functions. */
@@ -690,7 +703,7 @@ void oracle_tree_eval_batch(const double* sx, const double* sy, const double* sz
     int64_t* bstart = malloc(sizeof(int64_t) * (ntg > 0 ? ntg : 1));
     const int64_t nb = oracle_target_batches(tx, ty, tz, ntg, order, bstart);
     double* nrad = malloc(sizeof(double) * (g_nn > 0 ? g_nn : 1));
-    for (int64_t i = 0; i < g_nn; ++i) nrad[i] = node_radius(&g_nodes[i], sx, sy, sz);
+    if (g_nn > 0) node_radius(0, sx, sy, sz, nrad);
     const double mult = -1.0 / (4.0 * kPi);
     int64_t far_tot = 0, near_tot = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : far_tot, near_tot)

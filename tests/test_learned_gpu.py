@@ -65,11 +65,15 @@ def test_batch_converges_and_ragged(ctx, mesh):
     errs = [O.rel_l2(np.stack(T.evaluate_batch(ctx, tgt, 0.5, p)), ud) for p in (2, 4, 6, 8, 10, 12)]
     assert all(b < a for a, b in zip(errs, errs[1:])), errs
     assert errs[-1] < 1e-7
-    # ragged batch counts and an empty target set
+    # ragged batch counts (one target, partial waves) and an empty target set, against the oracle
+    O.tree_build(src, m.max_radius, 128)
+    mom = O.tree_moments(src, g, 8, 1)
     for n in (1, 63, 65, 1000):
         sub = tuple(a[:n].copy() for a in tgt)
         us = T.evaluate_batch(ctx, sub, 0.5, 8)
-        assert O.rel_l2(np.stack(us), ud[:, :n]) < 1e-4
+        ur, far, near = O.tree_eval_batch(src, g, mom, 8, 0.5, sub)
+        assert O.rel_max(np.stack(us), np.stack(ur)) < 1e-12
+        assert (T.stats(ctx)["far"], T.stats(ctx)["near"]) == (far, near)
     T.evaluate_batch(ctx, tuple(a[:0].copy() for a in tgt), 0.5, 8)
 
 
@@ -160,6 +164,18 @@ def test_model_training_learns_rule(ctx):
 
 
 # ---------------------------------------------------------------- labels and learned traversal
+def _radius(R, src, i):
+    """Source radius of oracle node i: exact at a leaf, max over children of
+    |c_child - c| + r_child above."""
+    c = R["cent"][i, :3]
+    if R["n_kids"][i] == 0:
+        ids = R["perm"][R["begin"][i]:R["end"][i]]
+        return float(np.sqrt(((np.stack([a[ids] for a in src], 1) - c) ** 2).sum(1).max()))
+    f = R["first_kid"][i]
+    return max(float(np.linalg.norm(R["cent"][k, :3] - c)) + _radius(R, src, k)
+               for k in range(f, f + R["n_kids"][i]))
+
+
 def test_labels_match_oracle(ctx, mesh):
     m, src, g, tgt = _field(mesh, 5, 64, 12)
     _gpu_tree(ctx, m, src, g, 64, 12)
@@ -193,8 +209,7 @@ def test_labels_match_oracle(ctx, mesh):
         cb = T_b.mean(0)
         s = np.linalg.norm(cb - c[:3])
         assert abs(ls.features[k, 8] - np.log2(s)) < 1e-5
-        ids = R["perm"][R["begin"][g2o[node]]:R["end"][g2o[node]]]
-        rn = np.sqrt(((np.stack([a[ids] for a in src], 1) - c[:3]) ** 2).sum(1).max())
+        rn = _radius(R, src, int(g2o[node]))
         assert abs(ls.features[k, 0] - np.log2(rn / s)) < 1e-5
 
 
