@@ -1,24 +1,27 @@
 """Benchmark of the spherical Biot-Savart sum on one MI355X.
 
-Workload: cubed-sphere level 9 -- 1,572,864 leaf-face particles (targets =
-sources = leaf faces, the face self pair excluded); zeta = sum of 8 seeded
-Gaussian vortices minus its mean; gamma = zeta * area.  Inputs are resident
-in HBM before timing starts.
+Workload: cubed-sphere level 9, 1,572,864 leaf-face particles (targets =
+sources = leaf faces, the face self pair excluded); zeta = sum of 8 Gaussian
+vortices (centers uniform on the sphere, amplitudes U(-2, 2), widths U(4, 16),
+numpy PCG64 seed 1234) minus its mean; gamma = zeta * area.  Inputs are
+resident in HBM before timing starts.
 
-One step = one velocity evaluation of all particles by the treecode with the
-FP32C near field (tree build + moments + evaluation, the work a time step
-repeats at every stage), value = N (N - 1) / step time ("effective"
-interactions/s), valid only if its relative L2 error against the FP64 direct
-sum of the same run is <= 1e-5 (otherwise the FP32C direct sum is the step).
+One step = one velocity evaluation of all particles by the learned-order
+FP32C treecode: tree build + moments + model-planned evaluation, the work an
+RK4 stage repeats.  value = N (N - 1) / step time (effective interactions/s),
+valid only if the step's relative L2 error against the FP64 direct sum of
+this run is <= 1e-5; otherwise the fixed-order FP32C batch treecode is the
+step (the fallback), and if that misses too, the FP32C direct sum.
 
-Also reported, each measured in this run:
-  * FP64 direct sum (the error judge) and FP32C direct sum rates and errors,
-    with the roofline of their pair kernels (ds_f64 / ds_f32c);
-  * the plain FP64 treecode (same tree, FP64 near field) time and error;
-  * RK4 steps/s at this size with the treecode velocity;
-  * the roofline of the step's dominant kernel (tree_eval) from HIP events on
-    the library's stream and its algorithmic flop count (near pairs x 18 +
-    far pairs x (8 C(p+2,3) + 16));
+Measured in the same run:
+  * FP64 direct sum (the error judge) and FP32C direct sum, with the roofline
+    of their pair kernels;
+  * the plain FP64 GPU treecode (reference acceptance per target, Cartesian
+    series, fixed p and theta) and the fixed-order FP32C batch treecode;
+  * training of the order model on GPU-generated labels (time reported);
+  * RK4 steps/s with the learned treecode and with the plain FP64 treecode;
+  * the roofline of the step's evaluation kernels from HIP events recorded by
+    the library on its stream (lpm_last_kernel_ms);
   * a CPU baseline: the oracle (reference-order FP64 direct sum, OpenMP) on a
     bounded sample of targets.
 """
@@ -34,6 +37,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK = 157.3   # TFLOP/s, MI355X FP32 vector (packed)
+FP32_NONPACKED = 78.6
 FP64_PEAK = 78.6    # TFLOP/s, MI355X FP64 vector
 DIRECT_FLOPS = 18   # velocity pair: d 3, |d|^2 5, q 2, rcp 1, w 1, acc 6
 METRIC = "Biot-Savart interactions/s + RK4 steps/s at 1.57M particles, rel L2 err ≤1e-5"
@@ -73,12 +77,26 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--level", type=int, default=9)
-    ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--order", type=int, default=10)
+    # learned treecode
     ap.add_argument("--leaf", type=int, default=128)
+    ap.add_argument("--theta-max", type=float, default=0.85)
+    ap.add_argument("--tau", type=float, default=1e-6)
+    ap.add_argument("--max-order", type=int, default=10)
+    ap.add_argument("--margin", type=int, default=1)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--label-stride", type=int, default=16)
+    ap.add_argument("--label-keep", type=float, default=0.3)
+    # plain FP64 treecode (reference acceptance) and fixed-order batch treecode
+    ap.add_argument("--base-order", type=int, default=6)
+    ap.add_argument("--base-theta", type=float, default=0.35)
+    ap.add_argument("--base-leaf", type=int, default=128)
+    ap.add_argument("--batch-order", type=int, default=7)
+    ap.add_argument("--batch-theta", type=float, default=0.4)
+    ap.add_argument("--batch-leaf", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
+    ap.add_argument("--no-direct64", action="store_true", help="judge errors against the FP32C direct sum")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -88,6 +106,7 @@ def main() -> int:
     import numpy as np
     import torch
     import lpmcxx_amd as L
+    from lpmcxx_amd import learned as LN
     from lpmcxx_amd import tree as T
 
     log(f"[bench] cubed-sphere level {args.level} mesh")
@@ -98,9 +117,9 @@ def main() -> int:
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     X, Y, Z, G = dev(m.cx), dev(m.cy), dev(m.cz), dev(g)
     ctx = L.Context(0)
-    stream = torch.cuda.ExternalStream(ctx.stream)
     torch.cuda.synchronize()
     pairs = n * (n - 1)
+    u_ref = LN.u_ref_of(g)
 
     if args.copy_probe:
         a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
@@ -109,103 +128,150 @@ def main() -> int:
         torch.cuda.synchronize()
         log("[bench] copy probe: 256 MiB device-to-device")
 
-    def timed(fn, reps=1):
-        """(mean event ms per call on the library stream, wall s per call)"""
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    def wall(fn, reps=1):
         ctx.synchronize()
         t0 = time.perf_counter()
-        for e0, e1 in evs:
-            e0.record(stream)
+        for _ in range(reps):
             fn()
-            e1.record(stream)
         ctx.synchronize()
-        wall = (time.perf_counter() - t0) / reps
-        return float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs])), wall
+        return (time.perf_counter() - t0) / reps
 
     def rel_l2(u, ref):
         return float(torch.linalg.norm(torch.stack(u) - torch.stack(ref)) / torch.linalg.norm(torch.stack(ref)))
 
-    # --- FP64 direct sum: the error judge
-    U64 = [torch.empty_like(X) for _ in range(3)]
-    log("[bench] FP64 direct sum")
-    ms64, _ = timed(lambda: ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, U64, prec=L.LPM_F64))
-
-    # --- FP32C direct sum
+    # --- direct sums
     U32 = [torch.empty_like(X) for _ in range(3)]
     f32c = lambda: ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, U32, prec=L.LPM_F32C)
     f32c()
-    ms32, wall32 = timed(f32c, reps=max(1, min(args.steps, 3)))
+    k32 = []
+    for _ in range(max(1, min(args.steps, 3))):
+        f32c()
+        k32.append(ctx.last_kernel_ms())
+    ms32 = float(np.mean(k32))
+    if args.no_direct64:
+        U64, ms64 = U32, None
+    else:
+        log("[bench] FP64 direct sum")
+        U64 = [torch.empty_like(X) for _ in range(3)]
+        ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, U64, prec=L.LPM_F64)
+        ms64 = ctx.last_kernel_ms()
     err32 = rel_l2(U32, U64)
-    log(f"[bench] direct: f64 {ms64:.1f} ms, f32c {ms32:.1f} ms, f32c rel-L2 {err32:.2e}")
+    log(f"[bench] direct: f64 {ms64} ms, f32c {ms32:.1f} ms (pair kernels), f32c rel-L2 {err32:.2e}")
 
-    # --- treecode
     UT = [torch.empty_like(X) for _ in range(3)]
-    p, theta, leaf = args.order, args.theta, args.leaf
-    ev_ms = {}
 
-    def tree_step(prec):
-        T.build_dev(ctx, X, Y, Z, m.max_radius, leaf)
-        T.moments_dev(ctx, G, p, 1)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        T.evaluate_dev(ctx, X, Y, Z, UT, theta, prec=prec, series=L.LPM_SERIES_CARTESIAN)
-        e1.record(stream)
-        ev_ms.setdefault(prec, []).append((e0, e1))
+    # --- plain FP64 GPU treecode: reference acceptance per target, Cartesian series
+    def base_step():
+        T.build_dev(ctx, X, Y, Z, m.max_radius, args.base_leaf)
+        T.moments_dev(ctx, G, args.base_order, 1)
+        T.evaluate_dev(ctx, X, Y, Z, UT, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
 
-    log("[bench] FP64 treecode")
-    _, wall_t64 = timed(lambda: tree_step(L.LPM_F64))
-    err_t64 = rel_l2(UT, U64)
-    far, near = T.last_stats(ctx)
-    log(f"[bench] warmup {args.warmup}, {args.steps} timed FP32C treecode step(s)")
+    base_step()
+    base_s = wall(base_step, 3)
+    base_eval_ms = ctx.last_kernel_ms()
+    base_err = rel_l2(UT, U64)
+    log(f"[bench] plain FP64 treecode: {1e3 * base_s:.1f} ms/step (eval {base_eval_ms:.1f}), rel-L2 {base_err:.2e}")
+
+    # --- fixed-order FP32C batch treecode (the fallback)
+    def batch_step():
+        T.build_dev(ctx, X, Y, Z, m.max_radius, args.batch_leaf)
+        T.moments_dev(ctx, G, args.batch_order, 1)
+        T.evaluate_batch_dev(ctx, X, Y, Z, UT, args.batch_theta, args.batch_order, prec=L.LPM_F32C)
+
+    batch_step()
+    batch_s = wall(batch_step, 3)
+    batch_eval_ms = ctx.last_kernel_ms()
+    batch_err = rel_l2(UT, U64)
+    batch_stats = T.stats(ctx)
+    log(f"[bench] FP32C batch treecode: {1e3 * batch_s:.1f} ms/step (eval {batch_eval_ms:.1f}), rel-L2 {batch_err:.2e}")
+
+    # --- learned order model: labels from the FP64 treecode, training on the GPU
+    t0 = time.perf_counter()
+    T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+    T.moments_dev(ctx, G, 12, 1)
+    ls = LN.labels(ctx, (m.cx, m.cy, m.cz), args.theta_max, u_ref, batch_stride=args.label_stride,
+                   keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
+    t_label = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fr = LN.fit(ctx, ls, args.tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    log(f"[bench] labels {len(ls.ids)} in {t_label:.1f} s, training {t_train:.1f} s, loss {fr.final_loss:.3f}, "
+        f"accuracy {fr.accuracy:.3f}")
+
+    def learned_step():
+        T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+        T.moments_dev(ctx, G, args.max_order, 1)
+        LN.evaluate_dev(ctx, X, Y, Z, UT, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
+
+    log(f"[bench] warmup {args.warmup}, {args.steps} timed learned-treecode step(s)")
     for _ in range(args.warmup):
-        tree_step(L.LPM_F32C)
+        learned_step()
     ctx.synchronize()
-    ev_ms[L.LPM_F32C] = []
     torch.cuda.synchronize()
+    eval_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        tree_step(L.LPM_F32C)
+        learned_step()
+        eval_ms.append(ctx.last_kernel_ms())
     ctx.synchronize()
     torch.cuda.synchronize()
-    wall_tree = (time.perf_counter() - t0) / args.steps
-    err_tree = rel_l2(UT, U64)
-    eval_ms = float(np.mean([a.elapsed_time(b) for a, b in ev_ms[L.LPM_F32C]]))
-    eval64_ms = float(np.mean([a.elapsed_time(b) for a, b in ev_ms[L.LPM_F64]]))
-    far, near = T.last_stats(ctx)
+    learned_s = (time.perf_counter() - t0) / args.steps
+    learned_err = rel_l2(UT, U64)
+    st = T.stats(ctx)
+    learned_eval_ms = float(np.mean(eval_ms))
     nn, depth, nleaves = T.info(ctx)
-    log(f"[bench] tree f32c: {1e3 * wall_tree:.1f} ms/step (eval {eval_ms:.1f} ms), rel-L2 {err_tree:.2e}; "
-        f"f64 tree {1e3 * wall_t64:.1f} ms, rel-L2 {err_t64:.2e}; far {far} near {near}")
+    log(f"[bench] learned FP32C treecode: {1e3 * learned_s:.1f} ms/step (eval {learned_eval_ms:.1f}), "
+        f"rel-L2 {learned_err:.2e}; far {st['far']} near {st['near']} terms {st['terms']} model {st['model']}")
 
-    use_tree = err_tree <= 1e-5
-    step_s = wall_tree if use_tree else wall32
+    if learned_err <= 1e-5:
+        step_name, step_s, step_err = "learned", learned_s, learned_err
+    elif batch_err <= 1e-5:
+        step_name, step_s, step_err = "batch", batch_s, batch_err
+    else:
+        step_name, step_s, step_err = "direct_f32c", ms32 / 1e3, err32
     value = pairs / step_s
-    tree_flops = near * DIRECT_FLOPS + far * (8 * nterms(p) + 16)
-    tree_roof = {"bound": "valu", "kernel": "tree_eval", "achieved": tree_flops / (eval_ms / 1e3) / 1e12,
-                 "peak": FP64_PEAK, "unit": "TFLOP/s", "traffic": None, "flops_per_launch": tree_flops,
-                 "launch_ms": eval_ms, "far_pairs": far, "near_pairs": near}
-    tree_roof["frac"] = tree_roof["achieved"] / FP64_PEAK
-    direct_roof = {"bound": "valu", "kernel": "ds_f32c", "achieved": pairs * DIRECT_FLOPS / (ms32 / 1e3) / 1e12,
-                   "peak": FP32_PEAK, "unit": "TFLOP/s", "traffic": None, "flops_per_launch": pairs * DIRECT_FLOPS,
+
+    # roofline of the step's evaluation kernels (FP32 VALU work)
+    def tree_roof(stats, ms, kernel):
+        flops = stats["near"] * DIRECT_FLOPS + stats["terms"] * 8 + stats["far"] * 16
+        ach = flops / (ms / 1e3) / 1e12
+        return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
+                "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
+                "flops_per_launch": flops, "launch_ms": ms, "far_pairs": stats["far"], "near_pairs": stats["near"],
+                "far_terms": stats["terms"]}
+
+    direct_flops = pairs * DIRECT_FLOPS
+    direct_roof = {"bound": "valu", "kernel": "ds_f32c", "achieved": direct_flops / (ms32 / 1e3) / 1e12,
+                   "peak": FP32_PEAK, "unit": "TFLOP/s", "traffic": None, "flops_per_launch": direct_flops,
                    "launch_ms": ms32}
     direct_roof["frac"] = direct_roof["achieved"] / FP32_PEAK
+    direct_roof["frac_nonpacked"] = direct_roof["achieved"] / FP32_NONPACKED
+    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_plan_learned + tree_eval_lists + tree_list_finish"),
+            "batch": tree_roof(batch_stats, batch_eval_ms, "tree_eval_batch"),
+            "direct_f32c": direct_roof}[step_name]
 
-    # --- RK4 steps/s with the treecode velocity (4 evaluations per step)
+    # --- RK4 steps/s (4 treecode evaluations per step), learned and plain FP64
     rk4 = None
     if not args.no_rk4:
-        log("[bench] RK4 step (treecode, FP32C near field)")
-        cfg = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F32C, order=p, theta=theta, leaf_max=leaf)
         za = dev(zeta + 2 * (2 * np.pi) * m.cz)
         A = dev(m.area)
-        Xs, Ys, Zs = X.clone(), Y.clone(), Z.clone()
-        ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A)
-        ctx.synchronize()
-        rk4_s = time.perf_counter() - t0
-        rk4 = {"rk4_steps_per_s": 1.0 / rk4_s, "rk4_ms_per_step": 1e3 * rk4_s,
-               "rk4_solver": f"treecode p={p} theta={theta} leaf={leaf} FP32C near field, dt=0.01, Omega=2pi"}
-        log(f"[bench] RK4: {1e3 * rk4_s:.1f} ms/step")
+
+        def rk4_rate(cfg, reps=2):
+            Xs, Ys, Zs = X.clone(), Y.clone(), Z.clone()
+            ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A)
+            return 1.0 / wall(lambda: ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A), reps)
+
+        log("[bench] RK4 steps")
+        cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
+                                       theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
+        cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
+                                       theta=args.base_theta, leaf_max=args.base_leaf)
+        rl, rb = rk4_rate(cfg_l), rk4_rate(cfg_b)
+        rk4 = {"rk4_steps_per_s": rl, "rk4_ms_per_step": 1e3 / rl, "rk4_plain_fp64_steps_per_s": rb,
+               "rk4_speedup_vs_plain_fp64": rl / rb,
+               "rk4_solver": f"learned FP32C treecode (theta_max {args.theta_max}, max order {args.max_order}, "
+                             f"leaf {args.leaf}), dt 0.01, Omega 2 pi, FP64 update and renormalization"}
+        log(f"[bench] RK4: learned {1e3 / rl:.1f} ms/step, plain FP64 treecode {1e3 / rb:.1f} ms/step")
 
     cpu = None
     if not args.no_cpu:
@@ -218,21 +284,30 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
         "config": {"workload": f"velocity of all {n} particles of a cubed-sphere level-{args.level} mesh, "
                                "8 Gaussian vortices", "particles": n, "pairs_per_step": pairs,
-                   "step": ("treecode (build + moments + evaluation), Cartesian series "
-                            f"p={p}, theta={theta}, leaf={leaf}, FP64 far field, FP32C near field")
-                   if use_tree else "FP32C direct sum",
-                   "rel_l2_err_vs_fp64_direct": err_tree if use_tree else err32},
-        "roofline": tree_roof if use_tree else direct_roof,
+                   "step": {"learned": f"learned-order FP32C treecode (build + moments + evaluation), leaf "
+                                       f"{args.leaf}, theta_max {args.theta_max}, tau {args.tau}, max order "
+                                       f"{args.max_order}, margin {args.margin}",
+                            "batch": f"fixed-order FP32C batch treecode p={args.batch_order} theta="
+                                     f"{args.batch_theta} leaf {args.batch_leaf}",
+                            "direct_f32c": "FP32C direct sum"}[step_name],
+                   "rel_l2_err_vs_fp64_direct": step_err},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "direct_f32c": {"interactions_per_s": pairs / (ms32 / 1e3), "ms": ms32, "rel_l2_err": err32,
+        "learned": {"ms_per_step": 1e3 * learned_s, "eval_ms": learned_eval_ms, "rel_l2_err": learned_err,
+                    "speedup_vs_plain_fp64_tree": base_s / learned_s, "label_samples": int(len(ls.ids)),
+                    "label_s": t_label, "train_s": t_train, "train_loss": fr.final_loss,
+                    "train_accuracy": fr.accuracy, "stats": st, "nodes": nn, "depth": depth, "leaves": nleaves},
+        "plain_fp64_tree": {"ms_per_step": 1e3 * base_s, "eval_ms": base_eval_ms, "rel_l2_err": base_err,
+                            "order": args.base_order, "theta": args.base_theta, "leaf": args.base_leaf,
+                            "effective_interactions_per_s": pairs / base_s},
+        "batch_f32c_tree": {"ms_per_step": 1e3 * batch_s, "eval_ms": batch_eval_ms, "rel_l2_err": batch_err,
+                            "order": args.batch_order, "theta": args.batch_theta, "leaf": args.batch_leaf,
+                            "speedup_vs_plain_fp64_tree": base_s / batch_s, "stats": batch_stats},
+        "direct_f32c": {"interactions_per_s": pairs / (ms32 / 1e3), "kernel_ms": ms32, "rel_l2_err": err32,
                         "roofline": direct_roof},
-        "direct_f64": {"interactions_per_s": pairs / (ms64 / 1e3), "ms": ms64,
-                       "frac_of_fp64_peak": pairs * DIRECT_FLOPS / (ms64 / 1e3) / (FP64_PEAK * 1e12)},
-        "tree_f64": {"ms_per_step": 1e3 * wall_t64, "eval_ms": eval64_ms, "rel_l2_err": err_t64,
-                     "effective_interactions_per_s": pairs / wall_t64},
-        "tree_f32c": {"ms_per_step": 1e3 * wall_tree, "eval_ms": eval_ms, "rel_l2_err": err_tree,
-                      "effective_interactions_per_s": pairs / wall_tree, "nodes": nn, "depth": depth,
-                      "leaves": nleaves, "speedup_vs_tree_f64": wall_t64 / wall_tree},
+        "direct_f64": None if ms64 is None else
+        {"interactions_per_s": pairs / (ms64 / 1e3), "kernel_ms": ms64,
+         "frac_of_fp64_peak": direct_flops / (ms64 / 1e3) / (FP64_PEAK * 1e12)},
     }
     if rk4:
         out.update(rk4)

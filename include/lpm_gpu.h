@@ -155,6 +155,13 @@ int lpm_tree_export(lpm_ctx* ctx, double* box, double* cent, int32_t* level, int
  * Replaces: SumNode::moments / momentString(), src/LpmTreeSum.h:36-44. */
 int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
 
+/* Tuning options of a context (no counterpart in the reference).
+ * LPM_OPT_BATCH_CELL_TARGETS: target batches of the batch and learned
+ * treecodes are cut at the octree cells of the shallowest level with at least
+ * n_targets / value occupied cells (default 256). */
+#define LPM_OPT_BATCH_CELL_TARGETS 1
+int lpm_set_option(lpm_ctx* ctx, int option, double value);
+
 /* Duration of the last pair or evaluation kernel (ds_f64 / ds_f32c of
  * lpm_direct_sum*, the evaluation kernel of lpm_tree_eval*), measured with
  * HIP events recorded on the context stream around that kernel alone.

@@ -94,6 +94,7 @@ class SolverConfig(C.Structure):
         return cfg
 
 
+LPM_OPT_BATCH_CELL_TARGETS = 1
 LPM_METHOD_DIRECT = 0
 LPM_METHOD_TREE = 1
 LPM_METHOD_TREE_BATCH = 2
@@ -119,6 +120,7 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_tree_last_stats": [_vp, _i64p, _i64p],
         "lpm_tree_stats": [_vp, _vp],
         "lpm_last_kernel_ms": [_vp, _vp],
+        "lpm_set_option": [_vp, C.c_int, C.c_double],
         "lpm_tree_eval_batch": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp,
                                 _vp, _vp],
         "lpm_tree_eval_batch_dev": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64,
@@ -250,6 +252,9 @@ class Context:
     @property
     def stream(self) -> int:
         return self.lib.lpm_stream(self.h)
+
+    def set_option(self, option: int, value: float) -> None:
+        self.check(self.lib.lpm_set_option(self.h, option, value))
 
     def last_kernel_ms(self) -> float:
         """Duration of the last pair / evaluation kernel (HIP events on the context stream)."""

@@ -80,6 +80,18 @@ void lpm_destroy(lpm_ctx* c) {
     delete c;
 }
 
+int lpm_set_option(lpm_ctx* c, int option, double value) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    switch (option) {
+        case LPM_OPT_BATCH_CELL_TARGETS:
+            if (!(value >= 1.0 && value <= 1e9)) return fail(c, LPM_ERR_INVALID_ARG, "batch cell targets out of range");
+            c->batch_cell_targets = int(value);
+            return LPM_OK;
+        default: return fail(c, LPM_ERR_INVALID_ARG, "unknown option %d", option);
+    }
+}
+
 int lpm_last_kernel_ms(lpm_ctx* c, float* ms) {
     if (!c || !ms) return LPM_ERR_NULL_PTR;
     if (!c->ev_valid) return fail(c, LPM_ERR_STATE, "no timed kernel yet");

@@ -45,6 +45,7 @@ struct lpm_ctx {
     // target batches of the batch treecode, interaction lists of the learned one
     lpm::DevBuf tb_work, tl_work;
     int far_cap = 1024, near_cap = 512;
+    int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
     lpm::Tree* tree = nullptr;
     lpm::Model* model = nullptr;
     unsigned long long* stats_ptr = nullptr;  // kStats counters of the last tree evaluation (device)

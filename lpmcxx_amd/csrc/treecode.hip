@@ -143,56 +143,70 @@ struct BinomTab {
 };
 __constant__ BinomTab c_binom{};
 
+__host__ __device__ constexpr int tet(int m) { return m * (m + 1) * (m + 2) / 6; }
+
+// lex position of (a, b, c) among |k| < p
+__device__ inline int lex_pos(int p, int a, int b, int c) {
+    return tet(p) - tet(p - a) + b * (p - a) - b * (b - 1) / 2 + c;
+}
+
 // Upward pass for the internal nodes of one level (moments about the box
 // centroid, h = sign (y - c)): with d = sign (c_child - c),
-//   m_k(node) = sum_children sum_{j <= k} C(k, j) d^(k - j) m_j(child)
-// (componentwise binomials), the same sums as the particle-by-particle
-// definition of the leaves, regrouped.
-__global__ __launch_bounds__(256) void tree_m2m(int first, int count, int p, int nterm, double sign,
+//   m_k(node) = sum_children sum_{j <= k} C(k, j) d^(k - j) m_j(child),
+// the same sums as the particle-by-particle definition of the leaves,
+// regrouped.  Since (h + d)^k factorizes over the axes, the translation is
+// applied as three one-dimensional binomial passes (x, then y, then z) on a
+// child's moments staged in LDS: O(p) work per term instead of O(p^3).
+// One block per node.
+__global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, double sign,
                                                 const int* __restrict__ nfirst, const int* __restrict__ nkids,
                                                 const double4* __restrict__ ncent, double* __restrict__ mom) {
-    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (tid >= int64_t(count) * nterm) return;
-    const int i = first + int(tid / nterm);
-    const int t = int(tid - int64_t(i - first) * nterm);
+    extern __shared__ double lds[];
+    double* A = lds;                  // [nterm][3]
+    double* B = lds + 3 * nterm;      // [nterm][3]
+    double* acc = lds + 6 * nterm;    // [nterm][3]
+    const int i = first + blockIdx.x;
     const int nk = nkids[i];
     if (nk == 0) return;
-    int r = t, k1 = 0, k2 = 0;
-    while (r >= (p - k1) * (p - k1 + 1) / 2) { r -= (p - k1) * (p - k1 + 1) / 2; ++k1; }
-    while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
-    const int k3 = r;
     const double4 c = ncent[i];
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) acc[q] = 0.0;
     for (int ch = nfirst[i]; ch < nfirst[i] + nk; ++ch) {
         const double4 cc = ncent[ch];
-        const double dx = sign * (cc.x - c.x), dy = sign * (cc.y - c.y), dz = sign * (cc.z - c.z);
+        const double d[3] = {sign * (cc.x - c.x), sign * (cc.y - c.y), sign * (cc.z - c.z)};
         const double* mc = mom + int64_t(ch) * nterm * 3;
-        double pa = 1.0;
-        for (int a = k1; a >= 0; --a) {
-            const double ca = c_binom.v[k1][a] * pa;
-            double pb = 1.0;
-            for (int b = k2; b >= 0; --b) {
-                const double cb = ca * c_binom.v[k2][b] * pb;
-                double pc = 1.0;
-                // lex position of (a, b, 0), then + cz
-                int idx = 0;
-                for (int q = 0; q < a; ++q) idx += (p - q) * (p - q + 1) / 2;
-                for (int q = 0; q < b; ++q) idx += p - a - q;
-                for (int cz = k3; cz >= 0; --cz) {
-                    const double w = cb * c_binom.v[k3][cz] * pc;
-                    const double* mj = mc + 3 * (idx + cz);
-                    m0 = fma(w, mj[0], m0);
-                    m1 = fma(w, mj[1], m1);
-                    m2 = fma(w, mj[2], m2);
-                    pc *= dz;
+        __syncthreads();
+        for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) A[q] = mc[q];
+        __syncthreads();
+        for (int axis = 0; axis < 3; ++axis) {
+            const double* src = (axis == 1) ? B : A;
+            double* dst = (axis == 1) ? A : B;
+            for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
+                int r = t, k1 = 0, k2 = 0;
+                while (r >= (p - k1) * (p - k1 + 1) / 2) { r -= (p - k1) * (p - k1 + 1) / 2; ++k1; }
+                while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
+                const int k3 = r;
+                const int kk = axis == 0 ? k1 : (axis == 1 ? k2 : k3);
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, pw = 1.0;
+                for (int a = kk; a >= 0; --a) {
+                    const int j = axis == 0 ? lex_pos(p, a, k2, k3) : (axis == 1 ? lex_pos(p, k1, a, k3)
+                                                                                 : lex_pos(p, k1, k2, a));
+                    const double w = c_binom.v[kk][a] * pw;
+                    s0 = fma(w, src[3 * j], s0);
+                    s1 = fma(w, src[3 * j + 1], s1);
+                    s2 = fma(w, src[3 * j + 2], s2);
+                    pw *= d[axis];
                 }
-                pb *= dy;
+                if (axis < 2) {
+                    dst[3 * t] = s0; dst[3 * t + 1] = s1; dst[3 * t + 2] = s2;
+                } else {
+                    acc[3 * t] += s0; acc[3 * t + 1] += s1; acc[3 * t + 2] += s2;
+                }
             }
-            pa *= dx;
+            __syncthreads();
         }
     }
-    double* o = mom + (int64_t(i) * nterm + t) * 3;
-    o[0] = m0; o[1] = m1; o[2] = m2;
+    double* o = mom + int64_t(i) * nterm * 3;
+    for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) o[q] = acc[q];
 }
 
 // graded FP64 and FP32 copies of the lex moments
@@ -484,8 +498,8 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(tree_m2m, dim3(blocks(cnt * nterm)), dim3(256), 0, st, int(f), int(cnt), p, nterm,
-                           double(sign), t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(),
+        hipLaunchKernelGGL(tree_m2m, dim3(unsigned(cnt)), dim3(128), size_t(9) * nterm * sizeof(double), st, int(f),
+                           p, nterm, double(sign), t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(),
                            t->mom.as<double>());
     }
     hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm,
@@ -556,7 +570,7 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     unsigned h[kMaxDepth + 1];
     LPM_HIP(c, hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipStreamSynchronize(st));
-    const int64_t want = std::max<int64_t>(1, nt / 256);
+    const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
     int L = kMaxDepth;
     int64_t cells = 1;
     for (int l = 0; l <= kMaxDepth; ++l) {

@@ -48,6 +48,12 @@ __device__ inline double wmax(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Global index of this wave (4 waves per block), marked wave-uniform so that
+// loads addressed by it become scalar loads.
+__device__ inline int64_t wave_index() {
+    return int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+}
+
 // true in every lane when cond holds in lane 0 (cond is wave-uniform by construction)
 __device__ inline bool uniform(bool cond) { return (__ballot(cond) & 1ull) != 0; }
 
@@ -70,24 +76,32 @@ __device__ inline Batch batch_of(bool valid, double t0, double t1, double t2) {
     return b;
 }
 
-// Ten features of a (batch, node) pair, wave-uniform:
+// Ten features of a (batch, node) pair:
 //   log2(r_n/s), log2(r_b/s), (r_n + r_b)/s, log2(M0/(u_ref s)),
 //   log2(M1/(M0 r_n)), log2(M2/(M0 r_n^2)), log2(npart), nvalid/64, log2 s,
 //   leaf flag, with s = |c_b - c_n| and M0, M1, M2 the moment shell norms.
+// FP32 with the hardware log2 and reciprocal: the label pass and the
+// evaluation call this same function, so training and inference see
+// identical inputs.
+__device__ inline float flog2(float v) {
+    return v > 0.f ? fmaxf(__builtin_amdgcn_logf(v), -100.f) : -100.f;
+}
 __device__ inline void pair_features(float* f, double rn, double rb, double s, float4 st, bool leaf, int nvalid,
                                      double u_ref) {
-    const double floor_log = -100.0;
-    auto lg = [&](double v) { return v > 0.0 ? fmax(log2(v), floor_log) : floor_log; };
-    const double m0 = st.x, m1 = st.y, m2 = st.z;
-    f[0] = float(lg(fmax(rn, 1e-12 * s) / s));
-    f[1] = float(lg(fmax(rb, 1e-12 * s) / s));
-    f[2] = float((rn + rb) / s);
-    f[3] = float(lg(m0 / (u_ref * s)));
-    f[4] = float(m0 * rn > 0.0 ? lg(m1 / (m0 * rn)) : floor_log);
-    f[5] = float(m0 * rn > 0.0 ? lg(m2 / (m0 * rn * rn)) : floor_log);
-    f[6] = float(lg(double(st.w)));
-    f[7] = float(nvalid) / 64.f;
-    f[8] = float(lg(s));
+    const float sf = float(s);
+    const float is = __builtin_amdgcn_rcpf(sf);
+    const float rnf = float(fmax(rn, 1e-12 * s)), rbf = float(fmax(rb, 1e-12 * s));
+    const float m0 = st.x, m1 = st.y, m2 = st.z;
+    f[0] = flog2(rnf * is);
+    f[1] = flog2(rbf * is);
+    f[2] = (rnf + rbf) * is;
+    f[3] = flog2(m0 * is * __builtin_amdgcn_rcpf(float(u_ref)));
+    const float mr = m0 * rnf;
+    f[4] = mr > 0.f ? flog2(m1 * __builtin_amdgcn_rcpf(mr)) : -100.f;
+    f[5] = mr > 0.f ? flog2(m2 * __builtin_amdgcn_rcpf(mr * rnf)) : -100.f;
+    f[6] = flog2(st.w);
+    f[7] = float(nvalid) * (1.f / 64.f);
+    f[8] = flog2(sf);
     f[9] = leaf ? 1.f : 0.f;
 #pragma unroll
     for (int k = kFeatures; k < kIn; ++k) f[k] = 0.f;
@@ -218,7 +232,7 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
                                                        unsigned long long* __restrict__ stats) {
     __shared__ int s_node[4][kStackB];
     const int w = threadIdx.x >> 6;
-    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    const int64_t bi = wave_index();
     if (bi >= tg.nb) return;
     bool valid;
     int orig;
@@ -267,16 +281,22 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
 // node n, the children of n are classified together: lane k < nkids computes
 // the features of child k, the rows go through the model in one pass, and
 // each child is pushed with its class (0 = descend, also for children failing
-// theta_max).  The lists are then evaluated by tree_near_list (writes the
-// per-target FP64 sums) and tree_far_list<P> for P = 1..max_order (adds the
-// entries of order P), each at the occupancy of its own register footprint;
-// tree_list_finish scales and scatters.
+// theta_max).  tree_eval_lists then evaluates each batch's list (near
+// leaves and low-order far entries in one launch, high orders in a second)
+// without the model's registers and LDS, and tree_list_finish scales and
+// scatters.
 constexpr int kNodeBits = 27;
+
+// Far entries of order <= kLowOrder fill a batch's far list from the front,
+// higher orders from the back, so two evaluation kernels with different
+// register footprints each read a contiguous range.
+constexpr int kLowOrder = 7;
 
 struct Lists {
     int* far;
     int* near;
-    int* far_cnt;
+    int* far_cnt;     // low-order far entries (front)
+    int* far_hi_cnt;  // high-order far entries (back)
     int* near_cnt;
     int far_cap, near_cap;
     int* overflow;
@@ -296,7 +316,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     mlp_stage_weights(wts, s_model);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+    const int64_t bi = wave_index();
     if (bi >= tg.nb) return;
     bool valid;
     int orig;
@@ -305,7 +325,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     const Batch bt = batch_of(valid, t0, t1, t2);
     int* far = ls.far + bi * ls.far_cap;
     int* near = ls.near + bi * ls.near_cap;
-    int nfar = 0, nnear = 0;
+    int nfar = 0, nhi = 0, nnear = 0;
     unsigned long long near_pairs = 0, nterms = 0, ncalls = 0;
     int sp = 1;
     s_node[w][0] = 0;
@@ -316,8 +336,13 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         const int cls = __builtin_amdgcn_readfirstlane(s_cls[w][sp]);
         if (cls > 0) {
             const int p = min(cls + margin, max_order);
-            if (lane == 0 && nfar < ls.far_cap) far[nfar] = n | (p << kNodeBits);
-            ++nfar;
+            if (p <= kLowOrder) {
+                if (lane == 0 && nfar + nhi < ls.far_cap) far[nfar] = n | (p << kNodeBits);
+                ++nfar;
+            } else {
+                if (lane == 0 && nfar + nhi < ls.far_cap) far[ls.far_cap - 1 - nhi] = n | (p << kNodeBits);
+                ++nhi;
+            }
             nterms += unsigned(p * (p + 1) * (p + 2) / 6);
             continue;
         }
@@ -369,65 +394,60 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     }
     if (lane == 0) {
         ls.far_cnt[bi] = nfar;
+        ls.far_hi_cnt[bi] = nhi;
         ls.near_cnt[bi] = nnear;
-        if (nfar > ls.far_cap || nnear > ls.near_cap) atomicOr(ls.overflow, 1);
+        if (nfar + nhi > ls.far_cap || nnear > ls.near_cap) atomicOr(ls.overflow, 1);
     }
     const unsigned long long nv = unsigned(bt.nvalid);
-    flush_stats(stats, (unsigned long long)nfar * nv, near_pairs * nv, nterms * nv, ncalls);
+    flush_stats(stats, (unsigned long long)(nfar + nhi) * nv, near_pairs * nv, nterms * nv, ncalls);
 }
 
-// acc[3][nt] (sorted positions) = near-field sums of the batch's leaf list
-template <bool F32>
-__global__ __launch_bounds__(256) void tree_near_list(Targets tg, TreeView tv, Lists ls, double d2, float eps2v,
-                                                      double* __restrict__ acc) {
-    const int w = threadIdx.x >> 6;
-    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
-    if (bi >= tg.nb) return;
-    bool valid;
-    int orig;
-    double t0, t1, t2;
-    const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
-    const int* near = ls.near + bi * ls.near_cap;
-    const int cnt = ls.near_cnt[bi];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int j = 0; j < cnt; ++j) {
-        const int n = near[j];
-        near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
-    }
-    if (valid) {
-        acc[pos] = s0;
-        acc[tg.n + pos] = s1;
-        acc[2 * tg.n + pos] = s2;
-    }
-}
-
-// acc += far fields of the batch's entries of order P
-template <int P, bool F32>
-__global__ __launch_bounds__(256) void tree_far_list(Targets tg, TreeView tv, Lists ls, double d2,
-                                                     double* __restrict__ acc) {
-    const int w = threadIdx.x >> 6;
-    const int64_t bi = int64_t(blockIdx.x) * 4 + w;
+// The batch's interaction list.  LOW: near leaves and the far entries of
+// order <= kLowOrder, written to acc[3][nt] (sorted positions, FP64); else
+// the far entries of order kLowOrder + 1 .. MAXP, added to acc.  The order is
+// wave-uniform per entry.
+template <int MAXP, bool LOW, bool F32>
+__global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, double d2, float eps2v,
+                                                       double* __restrict__ acc) {
+    const int64_t bi = wave_index();
     if (bi >= tg.nb) return;
     const int* far = ls.far + bi * ls.far_cap;
-    const int cnt = ls.far_cnt[bi];
-    int j = 0;
-    while (j < cnt && (far[j] >> kNodeBits) != P) ++j;
-    if (j == cnt) return;
+    int j0 = 0, j1 = __builtin_amdgcn_readfirstlane(ls.far_cnt[bi]);
+    if (!LOW) {
+        j1 = ls.far_cap;
+        j0 = j1 - __builtin_amdgcn_readfirstlane(ls.far_hi_cnt[bi]);
+        if (j0 == j1) return;
+    }
     bool valid;
     int orig;
     double t0, t1, t2;
     const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (; j < cnt; ++j) {
-        const int e = far[j];
-        if ((e >> kNodeBits) != P) continue;
+    if (LOW) {
+        const int* near = ls.near + bi * ls.near_cap;
+        const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
+        for (int j = 0; j < nnear; ++j) {
+            const int n = __builtin_amdgcn_readfirstlane(near[j]);
+            near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1,
+                           s2);
+        }
+    }
+    for (int j = j0; j < j1; ++j) {
+        const int e = __builtin_amdgcn_readfirstlane(far[j]);
         const int n = e & ((1 << kNodeBits) - 1);
-        far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0, s1, s2);
+        far_node<MAXP, F32, true>(e >> kNodeBits, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0,
+                                  s1, s2);
     }
     if (valid) {
-        acc[pos] += s0;
-        acc[tg.n + pos] += s1;
-        acc[2 * tg.n + pos] += s2;
+        if (LOW) {
+            acc[pos] = s0;
+            acc[tg.n + pos] = s1;
+            acc[2 * tg.n + pos] = s2;
+        } else {
+            acc[pos] += s0;
+            acc[tg.n + pos] += s1;
+            acc[2 * tg.n + pos] += s2;
+        }
     }
 }
 
@@ -462,7 +482,7 @@ __global__ __launch_bounds__(256) void tree_labels_kernel(Targets tg, TreeView t
     __shared__ int s_node[4][kStackB];
     __shared__ int s_prop[4][kStackB];  // proposed strict ancestors of the entry
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t sel = int64_t(blockIdx.x) * 4 + w;
+    const int64_t sel = wave_index();
     if (sel >= nsel) return;
     const int64_t bi = sel * bstride;
     bool valid;
@@ -620,14 +640,15 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     // interaction lists (grown and planned again if a batch overflows)
     Lists ls;
     for (int attempt = 0;; ++attempt) {
-        const size_t bytes = size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + 2) * sizeof(int) + 64 +
+        const size_t bytes = size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + 3) * sizeof(int) + 64 +
                              size_t(3 * nt) * sizeof(double);
         LPM_CHECK(c->tl_work.reserve(bytes));
         double* acc = c->tl_work.as<double>();
         ls.far = reinterpret_cast<int*>(acc + 3 * nt);
         ls.near = ls.far + size_t(nbatch) * c->far_cap;
         ls.far_cnt = ls.near + size_t(nbatch) * c->near_cap;
-        ls.near_cnt = ls.far_cnt + nbatch;
+        ls.far_hi_cnt = ls.far_cnt + nbatch;
+        ls.near_cnt = ls.far_hi_cnt + nbatch;
         ls.overflow = ls.near_cnt + nbatch;
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
@@ -642,26 +663,30 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         LPM_HIP(c, hipStreamSynchronize(st));
         if (!ovf) break;
         if (attempt > 0) return fail(c, LPM_ERR_KERNEL, "interaction lists overflow after growing");
-        std::vector<int> fc(nbatch), ncnt(nbatch);
+        std::vector<int> fc(nbatch), fh(nbatch), ncnt(nbatch);
         LPM_HIP(c, hipMemcpyAsync(fc.data(), ls.far_cnt, nbatch * sizeof(int), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(fh.data(), ls.far_hi_cnt, nbatch * sizeof(int), hipMemcpyDeviceToHost, st));
         LPM_HIP(c, hipMemcpyAsync(ncnt.data(), ls.near_cnt, nbatch * sizeof(int), hipMemcpyDeviceToHost, st));
         LPM_HIP(c, hipStreamSynchronize(st));
-        c->far_cap = std::max(c->far_cap, *std::max_element(fc.begin(), fc.end()));
+        int need = 0;
+        for (int64_t b = 0; b < nbatch; ++b) need = std::max(need, fc[b] + fh[b]);
+        c->far_cap = std::max(c->far_cap, need);
         c->near_cap = std::max(c->near_cap, *std::max_element(ncnt.begin(), ncnt.end()));
     }
     double* acc = c->tl_work.as<double>();
     const bool f32 = prec == LPM_F32C;
-    if (f32) hipLaunchKernelGGL(tree_near_list<true>, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc);
-    else hipLaunchKernelGGL(tree_near_list<false>, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc);
-    auto far = [&](auto k32, auto k64) {
-        if (f32) hipLaunchKernelGGL(k32, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, acc);
-        else hipLaunchKernelGGL(k64, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, acc);
-    };
-#define LPM_FAR(PP) \
-    if (max_order >= PP) far(tree_far_list<PP, true>, tree_far_list<PP, false>);
-    LPM_FAR(1) LPM_FAR(2) LPM_FAR(3) LPM_FAR(4) LPM_FAR(5) LPM_FAR(6)
-    LPM_FAR(7) LPM_FAR(8) LPM_FAR(9) LPM_FAR(10) LPM_FAR(11) LPM_FAR(12)
-#undef LPM_FAR
+    auto lists = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc); };
+    if (f32) lists(tree_eval_lists<kLowOrder, true, true>);
+    else lists(tree_eval_lists<kLowOrder, true, false>);
+    if (max_order > kLowOrder) {
+        if (max_order <= 10) {
+            if (f32) lists(tree_eval_lists<10, false, true>);
+            else lists(tree_eval_lists<10, false, false>);
+        } else {
+            if (f32) lists(tree_eval_lists<12, false, true>);
+            else lists(tree_eval_lists<12, false, false>);
+        }
+    }
     hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult, ux,
                        uy, uz);
     mark_end(c);
