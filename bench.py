@@ -80,9 +80,10 @@ def main() -> int:
     # learned treecode
     ap.add_argument("--leaf", type=int, default=128)
     ap.add_argument("--theta-max", type=float, default=0.85)
-    ap.add_argument("--tau", type=float, default=1e-6)
-    ap.add_argument("--max-order", type=int, default=10)
-    ap.add_argument("--margin", type=int, default=1)
+    ap.add_argument("--tau", type=float, default=3e-7)
+    ap.add_argument("--max-order", type=int, default=8)
+    ap.add_argument("--margin", type=int, default=0)
+    ap.add_argument("--learned-cell", type=int, default=1024, help="targets per batch cell (learned treecode)")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--label-stride", type=int, default=16)
     ap.add_argument("--label-keep", type=float, default=0.3)
@@ -93,6 +94,7 @@ def main() -> int:
     ap.add_argument("--batch-order", type=int, default=7)
     ap.add_argument("--batch-theta", type=float, default=0.4)
     ap.add_argument("--batch-leaf", type=int, default=256)
+    ap.add_argument("--batch-cell", type=int, default=256, help="targets per batch cell (batch treecode)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
@@ -174,6 +176,7 @@ def main() -> int:
 
     # --- fixed-order FP32C batch treecode (the fallback)
     def batch_step():
+        ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.batch_cell)
         T.build_dev(ctx, X, Y, Z, m.max_radius, args.batch_leaf)
         T.moments_dev(ctx, G, args.batch_order, 1)
         T.evaluate_batch_dev(ctx, X, Y, Z, UT, args.batch_theta, args.batch_order, prec=L.LPM_F32C)
@@ -186,6 +189,7 @@ def main() -> int:
     log(f"[bench] FP32C batch treecode: {1e3 * batch_s:.1f} ms/step (eval {batch_eval_ms:.1f}), rel-L2 {batch_err:.2e}")
 
     # --- learned order model: labels from the FP64 treecode, training on the GPU
+    ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
     t0 = time.perf_counter()
     T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, 12, 1)
@@ -246,7 +250,7 @@ def main() -> int:
                    "launch_ms": ms32}
     direct_roof["frac"] = direct_roof["achieved"] / FP32_PEAK
     direct_roof["frac_nonpacked"] = direct_roof["achieved"] / FP32_NONPACKED
-    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_plan_learned + tree_eval_lists + tree_list_finish"),
+    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists (low-order + high-order launch)"),
             "batch": tree_roof(batch_stats, batch_eval_ms, "tree_eval_batch"),
             "direct_f32c": direct_roof}[step_name]
 

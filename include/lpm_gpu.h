@@ -162,10 +162,12 @@ int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
 #define LPM_OPT_BATCH_CELL_TARGETS 1
 int lpm_set_option(lpm_ctx* ctx, int option, double value);
 
-/* Duration of the last pair or evaluation kernel (ds_f64 / ds_f32c of
- * lpm_direct_sum*, the evaluation kernel of lpm_tree_eval*), measured with
- * HIP events recorded on the context stream around that kernel alone.
- * Waits for the kernel.  No counterpart in the reference. */
+/* Duration of the last pair or evaluation kernel, measured with HIP events
+ * recorded on the context stream around it alone: ds_f64 / ds_f32c of
+ * lpm_direct_sum*, tree_eval of lpm_tree_eval*, tree_eval_batch of
+ * lpm_tree_eval_batch*, and the interaction-list kernels (tree_eval_lists,
+ * not the planning pass) of lpm_tree_eval_learned*.  Waits for the kernel.
+ * No counterpart in the reference. */
 int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
 
 /* Counters of the last tree evaluation (any of the lpm_tree_eval* calls):

@@ -96,59 +96,69 @@ __device__ inline bf16x4 as_bf16x4(uint2 v) {
     return r;
 }
 
-// Up to 16 rows (normalized inputs as BF16 in X[16][16], wave-private LDS)
-// through the network on the MFMA units; the argmax class of row r goes to
-// cls[r].  H1, H2: wave-private LDS scratch of 16 x 64 BF16 each.  The row
-// arithmetic is that of the training forward pass (same instruction, same
-// k order), so the classes equal lpm_model_forward's.
-__device__ inline void mlp_classify_rows(const unsigned char* model, const unsigned short* X, unsigned short* H1,
-                                         unsigned short* H2, int* cls) {
-    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+
+// four floats to BF16 (round to nearest even, the hardware conversion)
+__device__ inline bf16x4 cvt_bf16x4(float a, float b, float c, float d) {
+    const bf16x2v lo = __builtin_convertvector((f32x2v){a, b}, bf16x2v);
+    const bf16x2v hi = __builtin_convertvector((f32x2v){c, d}, bf16x2v);
+    const unsigned ulo = __builtin_bit_cast(unsigned, lo), uhi = __builtin_bit_cast(unsigned, hi);
+    bf16x4 r;
+    r[0] = short(ulo & 0xFFFFu);
+    r[1] = short(ulo >> 16);
+    r[2] = short(uhi & 0xFFFFu);
+    r[3] = short(uhi >> 16);
+    return r;
+}
+
+// Register-only inference for up to 16 rows.  Every layer is computed
+// transposed, H_out^T = W^T H_in^T, so that the MFMA result layout (lane
+// (g, c) holds row c, columns 4g..4g+3) is directly the operand layout of
+// the next layer and no LDS transposes are needed.  x: this lane's four
+// normalized inputs, row (lane % 16), columns 4g..4g+3 (g = lane / 16), as
+// BF16.  Returns the argmax class of row (lane % 16) in every lane.
+__device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint2* frag = reinterpret_cast<const uint2*>(model);
     const float* b1 = reinterpret_cast<const float*>(model + kFragBytes);
     const float* b2 = b1 + kHid;
     const float* b3 = b2 + kHid;
-    wave_sync();
-    const bf16x4 a1 = as_bf16x4(*reinterpret_cast<const uint2*>(X + c * kIn + 4 * g));
+    bf16x4 h1[4], h2[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(a1, as_bf16x4(frag[nt * 64 + lane]), acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) H1[(4 * g + i) * kHid + nt * 16 + c] = f2bf(fmaxf(acc[i] + b1[nt * 16 + c], 0.f));
+        acc = mfma16(as_bf16x4(frag[nt * 64 + lane]), x, acc);
+        const float4 b = *reinterpret_cast<const float4*>(b1 + nt * 16 + 4 * g);
+        h1[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
+                            fmaxf(acc[3] + b.w, 0.f));
     }
-    wave_sync();
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
-            const bf16x4 a = as_bf16x4(*reinterpret_cast<const uint2*>(H1 + c * kHid + kt * 16 + 4 * g));
-            acc = mfma16(a, as_bf16x4(frag[(4 + 4 * kt + nt) * 64 + lane]), acc);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) H2[(4 * g + i) * kHid + nt * 16 + c] = f2bf(fmaxf(acc[i] + b2[nt * 16 + c], 0.f));
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(as_bf16x4(frag[(4 + 4 * kt + nt) * 64 + lane]), h1[kt], acc);
+        const float4 b = *reinterpret_cast<const float4*>(b2 + nt * 16 + 4 * g);
+        h2[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
+                            fmaxf(acc[3] + b.w, 0.f));
     }
-    wave_sync();
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
-        const bf16x4 a = as_bf16x4(*reinterpret_cast<const uint2*>(H2 + c * kHid + kt * 16 + 4 * g));
-        acc = mfma16(a, as_bf16x4(frag[(20 + kt) * 64 + lane]), acc);
-    }
+    for (int kt = 0; kt < 4; ++kt) acc = mfma16(as_bf16x4(frag[(20 + kt) * 64 + lane]), h2[kt], acc);
+    const float4 b = *reinterpret_cast<const float4*>(b3 + 4 * g);
+    const float z[4] = {acc[0] + b.x, acc[1] + b.y, acc[2] + b.z, acc[3] + b.w};
+    float v = -3.0e38f;
+    int best = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float v = (c < kClasses) ? acc[i] + b3[c] : -3.0e38f;
-        int best = c;
+    for (int i = 0; i < 4; ++i)
+        if (4 * g + i < kClasses && z[i] > v) { v = z[i]; best = 4 * g + i; }
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(v, o, 64);
-            const int ob = __shfl_xor(best, o, 64);
-            if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
-        }
-        if (c == 0) cls[4 * g + i] = best;
+    for (int o = 16; o < 64; o <<= 1) {
+        const float ov = __shfl_xor(v, o, 64);
+        const int ob = __shfl_xor(best, o, 64);
+        if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
     }
-    wave_sync();
+    return best;
 }
 
 }  // namespace lpm

@@ -13,7 +13,7 @@
 //            full SIMD width, unlike the per-target MAC of the reference
 //            (which tree_eval keeps for parity).
 //   learned  pairs with r_n + r_b < theta_max |c_b - c_n| go through the order
-//            model on the MFMA units (mlp_classify_rows, all children of a
+//            model on the MFMA units (mlp_classify_regs, all children of a
 //            node in one pass): class 0 descends, class q <= max_order
 //            accepts with order min(q + margin, max_order).
 //   labels   for a sample of batches the wave visits every node; for a sample
@@ -309,10 +309,6 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackB];
     __shared__ int s_cls[4][kStackB];
-    __shared__ __align__(16) unsigned short s_x[4][16 * kIn];
-    __shared__ __align__(16) unsigned short s_h1[4][16 * kHid];
-    __shared__ __align__(16) unsigned short s_h2[4][16 * kHid];
-    __shared__ int s_row[4][16];
     mlp_stage_weights(wts, s_model);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -353,37 +349,35 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
             near_pairs += unsigned(tv.nend[n] - tv.nbeg[n]);
             continue;
         }
-        // classify the children: lane k < nk takes child f + k
+        // classify the children: lane l works on child f + (l % 16); the
+        // lanes of one row hold its inputs 4g..4g+3 (g = l / 16)
         const int f = tv.nfirst[n];
+        const int row = lane & 15, g = lane >> 4;
         bool prop = false;
-        float x[kIn];
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) x[k] = 0.f;
-        if (lane < nk) {
-            const int ch = f + lane;
+        float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+        if (row < nk) {
+            const int ch = f + row;
             const double4 cc = tv.ncent[ch];
             const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
             const double sd = sqrt(dx * dx + dy * dy + dz * dz);
             const double rn = tv.nrad[ch];
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
-                pair_features(x, rn, bt.r, sd, tv.nstat[ch], tv.nkids[ch] == 0, bt.nvalid, u_ref);
+                float fe[kIn];
+                pair_features(fe, rn, bt.r, sd, tv.nstat[ch], tv.nkids[ch] == 0, bt.nvalid, u_ref);
 #pragma unroll
-                for (int k = 0; k < kIn; ++k) x[k] = (x[k] - norm.mean[k]) * norm.inv_std[k];
+                for (int k = 0; k < kIn; ++k) fe[k] = (fe[k] - norm.mean[k]) * norm.inv_std[k];
+                x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
+                x1 = g == 0 ? fe[1] : g == 1 ? fe[5] : g == 2 ? fe[9] : fe[13];
+                x2 = g == 0 ? fe[2] : g == 1 ? fe[6] : g == 2 ? fe[10] : fe[14];
+                x3 = g == 0 ? fe[3] : g == 1 ? fe[7] : g == 2 ? fe[11] : fe[15];
             }
         }
         int my_cls = 0;
         if (__ballot(prop)) {
-            if (lane < 16) {
-#pragma unroll
-                for (int k = 0; k < kIn; ++k) s_x[w][lane * kIn + k] = f2bf(x[k]);
-            }
-            mlp_classify_rows(s_model, s_x[w], s_h1[w], s_h2[w], s_row[w]);
+            const int q = mlp_classify_regs(s_model, cvt_bf16x4(x0, x1, x2, x3));
             ++ncalls;
-            if (prop) {
-                const int q = s_row[w][lane];
-                my_cls = (q <= max_order) ? q : 0;
-            }
+            if (prop) my_cls = (q <= max_order) ? q : 0;
         }
         if (lane < nk) {
             s_node[w][sp + nk - 1 - lane] = f + lane;
@@ -654,7 +648,6 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         ls.near_cap = c->near_cap;
         LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, sizeof(int), st));
         if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
-        mark_begin(c);
         hipLaunchKernelGGL(tree_plan_learned, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin, max_order, u_ref,
                            w, norm, ls, c->stats_ptr);
         LPM_HIP(c, hipGetLastError());
@@ -676,6 +669,7 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     double* acc = c->tl_work.as<double>();
     const bool f32 = prec == LPM_F32C;
     auto lists = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc); };
+    mark_begin(c);
     if (f32) lists(tree_eval_lists<kLowOrder, true, true>);
     else lists(tree_eval_lists<kLowOrder, true, false>);
     if (max_order > kLowOrder) {
@@ -687,9 +681,9 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
             else lists(tree_eval_lists<12, false, false>);
         }
     }
+    mark_end(c);
     hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult, ux,
                        uy, uz);
-    mark_end(c);
     LPM_HIP(c, hipGetLastError());
     return LPM_OK;
 }

@@ -2,6 +2,7 @@
 #include "LpmSolvers.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <limits>
 #include <new>
@@ -343,15 +344,82 @@ TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, st
         for (index_type& j : n.coordsContained) j = _leafFaces[j];
 }
 
-void TreeSum::computeMeshMoments() {
+void TreeSum::computeMeshMoments() { ensureMoments(_maxSeriesOrder); }
+
+void TreeSum::ensureMoments(int order) {
+    if (_haveMoments && _momentOrder == order) return;
     auto pm = _mesh.lock();
     auto src = _faceSrc.lock();
     if (!pm || !src) throw std::logic_error("TreeSum::computeMeshMoments: mesh or field expired");
     std::vector<scalar_type> g;
     g.reserve(_leafFaces.size());
-    for (index_type f : _leafFaces) g.push_back(src->getScalar(f) * pm->getFaceArea(f));
-    _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), _maxSeriesOrder, _momentSign), "TreeSum::computeMeshMoments");
+    _uref = 0.0;
+    for (index_type f : _leafFaces) {
+        g.push_back(src->getScalar(f) * pm->getFaceArea(f));
+        _uref += std::abs(g.back());
+    }
+    _uref /= 4.0 * PI;
+    _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), order, _momentSign), "TreeSum::computeMeshMoments");
     _haveMoments = true;
+    _momentOrder = order;
+}
+
+float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
+    auto pm = _mesh.lock();
+    if (!pm) throw std::logic_error("TreeSum::trainOrderModel: mesh expired");
+    if (cfg.maxOrder < 1 || cfg.maxOrder > 12 || !(cfg.tau > 0.0))
+        throw std::invalid_argument("TreeSum::trainOrderModel: maxOrder in 1..12 and tau > 0");
+    if (_momentSign != 1) setMomentSign(1);
+    ensureMoments(12);
+    std::vector<scalar_type> tx, ty, tz;
+    for (index_type f : _leafFaces) {
+        const XyzVector c = pm->getFaceCentroid(f);
+        tx.push_back(c.x); ty.push_back(c.y); tz.push_back(c.z);
+    }
+    const int64_t cap = int64_t(1) << 21;
+    std::vector<float> feat(size_t(cap) * LPM_MODEL_INPUTS), err(size_t(cap) * LPM_LABEL_ORDERS);
+    int64_t ns = 0;
+    _gpu->check(lpm_tree_labels(_gpu->get(), cfg.thetaMax, _smooth, _uref, cfg.labelStride, cfg.keepRate, 2, cfg.seed,
+                                cap, tx.data(), ty.data(), tz.data(), int64_t(tx.size()), feat.data(), err.data(),
+                                nullptr, &ns),
+                "TreeSum::trainOrderModel");
+    if (ns == 0) throw std::runtime_error("TreeSum::trainOrderModel: no candidate pairs (mesh too small?)");
+    // class = lowest order meeting tau; 0 = descend, also at a leaf whose
+    // direct sum is cheaper than the series (9 instructions per term, 11 per pair)
+    std::vector<int32_t> y(ns);
+    for (int64_t i = 0; i < ns; ++i) {
+        int p = 0;
+        for (int q = 1; q <= cfg.maxOrder; ++q)
+            if (err[size_t(i) * LPM_LABEL_ORDERS + q - 1] <= cfg.tau) { p = q; break; }
+        const float* f = &feat[size_t(i) * LPM_MODEL_INPUTS];
+        const double npart = std::exp2(double(f[6]));
+        if (p > 0 && f[9] > 0.5f && 9.0 * (p * (p + 1) * (p + 2) / 6) >= 11.0 * npart) p = 0;
+        y[i] = p;
+    }
+    float mean[LPM_MODEL_INPUTS] = {}, inv[LPM_MODEL_INPUTS];
+    for (int k = 0; k < LPM_MODEL_INPUTS; ++k) inv[k] = 1.0f;
+    for (int k = 0; k < 10; ++k) {
+        double s = 0.0, s2 = 0.0;
+        for (int64_t i = 0; i < ns; ++i) {
+            const double v = feat[size_t(i) * LPM_MODEL_INPUTS + k];
+            s += v;
+            s2 += v * v;
+        }
+        const double m = s / double(ns), var = std::max(0.0, s2 / double(ns) - m * m);
+        mean[k] = float(m);
+        inv[k] = float(1.0 / std::max(std::sqrt(var), 1e-3));
+    }
+    _gpu->check(lpm_model_init(_gpu->get(), cfg.seed), "TreeSum::trainOrderModel");
+    std::vector<float> params(LPM_MODEL_NPARAMS);
+    _gpu->check(lpm_model_get_params(_gpu->get(), params.data(), nullptr, nullptr), "TreeSum::trainOrderModel");
+    _gpu->check(lpm_model_set_params(_gpu->get(), params.data(), mean, inv), "TreeSum::trainOrderModel");
+    float loss = 0.f;
+    _gpu->check(lpm_model_train(_gpu->get(), feat.data(), y.data(), ns, cfg.epochs, 4096, 2e-3f, cfg.seed, &loss),
+                "TreeSum::trainOrderModel");
+    _model = cfg;
+    _trained = true;
+    _learned = true;
+    return loss;
 }
 
 void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, const scalar_type mp_tol,
@@ -360,7 +428,8 @@ void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf
     auto vt = _vertTgt.lock();
     auto ft = _faceTgt.lock();
     if (!pm || !vt || !ft) throw std::logic_error("TreeSum::meshSolve: mesh or fields expired");
-    if (!_haveMoments) computeMeshMoments();
+    if (_learned && !_trained) throw std::logic_error("TreeSum::meshSolve: trainOrderModel first");
+    ensureMoments(_learned ? std::max(_model.maxOrder, 1) : _maxSeriesOrder);
     std::vector<scalar_type> tx, ty, tz;
     std::vector<index_type> vtx, face;
     const int r = mv.getRank();
@@ -378,9 +447,25 @@ void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf
     const int64_t nt = int64_t(tx.size());
     std::vector<scalar_type> ux(nt), uy(nt), uz(nt);
     int64_t mac = 0;
-    _gpu->check(lpm_tree_eval(_gpu->get(), _prec, _series, mp_tol, _smooth, tx.data(), ty.data(), tz.data(), nt,
-                              ux.data(), uy.data(), uz.data(), &mac),
-                "TreeSum::meshSolve");
+    if (_learned) {
+        _gpu->check(lpm_tree_eval_learned(_gpu->get(), _prec, _model.thetaMax, _model.margin, _model.maxOrder, _uref,
+                                          _smooth, tx.data(), ty.data(), tz.data(), nt, ux.data(), uy.data(),
+                                          uz.data()),
+                    "TreeSum::meshSolve");
+    } else if (_batch) {
+        _gpu->check(lpm_tree_eval_batch(_gpu->get(), _prec, _maxSeriesOrder, mp_tol, _smooth, tx.data(), ty.data(),
+                                        tz.data(), nt, ux.data(), uy.data(), uz.data()),
+                    "TreeSum::meshSolve");
+    } else {
+        _gpu->check(lpm_tree_eval(_gpu->get(), _prec, _series, mp_tol, _smooth, tx.data(), ty.data(), tz.data(), nt,
+                                  ux.data(), uy.data(), uz.data(), &mac),
+                    "TreeSum::meshSolve");
+    }
+    if (_learned || _batch) {
+        int64_t st[4] = {0, 0, 0, 0};
+        _gpu->check(lpm_tree_stats(_gpu->get(), st), "TreeSum::meshSolve");
+        mac = st[LPM_STAT_FAR];
+    }
     const size_t nv = vtx.size();
     for (size_t i = 0; i < nv; ++i) vt->replace(vtx[i], XyzVector(ux[i], uy[i], uz[i]));
     for (size_t i = 0; i < face.size(); ++i) ft->replace(face[i], XyzVector(ux[nv + i], uy[nv + i], uz[nv + i]));

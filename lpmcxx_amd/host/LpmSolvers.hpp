@@ -259,6 +259,19 @@ class DirectSum {
 };
 
 // ---------------------------------------------------------------- treecode
+// Settings of the learned Taylor order and acceptance of TreeSum (no
+// counterpart in the reference).
+struct OrderModelConfig {
+    scalar_type thetaMax = 0.85;  // pairs passing r_n + r_b < thetaMax |c_b - c_n| go to the model
+    scalar_type tau = 3e-7;       // per-pair error bound, in units of sum |gamma| / (4 pi)
+    int maxOrder = 8;
+    int margin = 0;               // orders added to the model's class
+    int epochs = 8;
+    int labelStride = 16;         // every labelStride-th target batch is labelled
+    scalar_type keepRate = 0.3;   // fraction of its candidate pairs kept
+    uint64_t seed = 1;
+};
+
 class TreeSum : public Tree {
   public:
     TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
@@ -273,17 +286,36 @@ class TreeSum : public Tree {
     // Defaults reproduce the reference: moment sign -1 (h = centroid - y),
     // the reference series, FP64.  setMomentSign(+1) with
     // setSeries(LPM_SERIES_CARTESIAN) selects the convergent expansion.
-    void setMomentSign(int s) { _momentSign = s; }
+    void setMomentSign(int s) { _momentSign = s; _haveMoments = false; }
     void setSeries(int s) { _series = s; }
     void setPrecision(int p) { _prec = p; }
     void setSmoothing(scalar_type d) { _smooth = d; }
     int maxSeriesOrder() const { return _maxSeriesOrder; }
+
+    // Acceptance per batch of <= 64 Morton-sorted targets (lpm_tree_eval_batch):
+    // node n is accepted for the whole batch when r_n + r_b < mp_tol |c_b - c_n|;
+    // Cartesian series of order max_series_order (moment sign +1 is set).
+    // macCounter then counts accepted (target, node) pairs.
+    void setBatchAcceptance(bool on) {
+        _batch = on;
+        if (on && _momentSign != 1) setMomentSign(1);
+    }
+    // Labels from the FP64 treecode at the leaf faces, then GPU training of
+    // the order model; meshSolve uses it afterwards (lpm_tree_eval_learned),
+    // mp_tol being ignored.  Returns the final training loss.
+    float trainOrderModel(const OrderModelConfig& cfg = OrderModelConfig());
+    void setLearned(bool on) { _learned = on; }
 
   protected:
     int _maxSeriesOrder, _maxParticlesPerNode;
     scalar_type _smooth = 0.0;
     int _momentSign = -1, _series = LPM_SERIES_REFERENCE, _prec = LPM_F64;
     bool _haveMoments = false;
+    int _momentOrder = 0;
+    bool _batch = false, _learned = false, _trained = false;
+    OrderModelConfig _model;
+    scalar_type _uref = 0.0;
+    void ensureMoments(int order);
     std::weak_ptr<MeshedParticles> _mesh;
     std::weak_ptr<Field> _vertTgt, _faceSrc, _faceTgt;
     VectorKernel* vKern;

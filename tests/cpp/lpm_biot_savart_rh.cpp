@@ -82,6 +82,40 @@ int main() {
         CHECK(covered == pm->nLeafFaces());
     }
     CHECK(prev < 0.02);
+    {
+        // batch acceptance and the learned order model through the same API
+        CubedSphereSeed cseed;
+        auto pm = std::make_shared<MeshedParticles>(cseed, 6, true, 1.0, 0);
+        for (const char* n : {"velocity", "batchVelocity", "learnedVelocity"}) {
+            pm->createVertexField(n, "length/time", 3);
+            pm->createFaceField(n, "length/time", 3);
+        }
+        pm->createVertexField("vorticity", "1/time", 1);
+        pm->createFaceField("vorticity", "1/time", 1);
+        pm->initializeVertexFieldWithFunction("vorticity", &y54);
+        pm->initializeFaceFieldWithFunction("vorticity", &y54);
+        MPIReplicatedData mpiVerts(pm->nVertices(), 0, 1), mpiFaces(pm->nFaces(), 0, 1);
+        DirectSum ds(pm, kernel, "velocity", "vorticity");
+        ds.meshSolve(mpiVerts, mpiFaces);
+        TreeSum bt(8, 64, pm, kernel, "batchVelocity", "vorticity");
+        bt.setBatchAcceptance(true);
+        int macB = 0;
+        bt.meshSolve(mpiVerts, mpiFaces, 0.5, macB);
+        const double berr = maxRelErr(*pm->getFaceFieldPtr("batchVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
+        TreeSum lt(8, 64, pm, kernel, "learnedVelocity", "vorticity");
+        OrderModelConfig cfg;
+        cfg.labelStride = 2;
+        cfg.keepRate = 1.0;
+        const float loss = lt.trainOrderModel(cfg);
+        int macL = 0;
+        lt.meshSolve(mpiVerts, mpiFaces, 0.0, macL);
+        const double lerr = maxRelErr(*pm->getFaceFieldPtr("learnedVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
+        std::printf("cubed level 6: batch tree vs direct %.3e (far pairs %d), learned vs direct %.3e (far pairs %d, "
+                    "training loss %.3f)\n", berr, macB, lerr, macL, loss);
+        CHECK(berr < 1e-4 && macB > 0);
+        CHECK(lerr < 1e-3 && macL > 0);
+        CHECK(loss < 1.5f);
+    }
     // a kernel the GPU cannot evaluate is the reference's "invalid kernel"
     struct Other : VectorKernel {
         XyzVector evaluate(const XyzVector&, const XyzVector&) const override { return XyzVector(); }

@@ -252,3 +252,29 @@ def test_learned_errors(ctx, mesh):
         LN.evaluate(ctx, tgt, 0.6, 0, 4, 0.0)
     with pytest.raises(L.LpmError):          # labels need order-12 moments
         LN.labels(ctx, tgt, 0.8, 1.0)
+
+
+def test_learned_gain_over_fixed_order(ctx, mesh):
+    """The trained model does less far- and near-field work than every
+    fixed-(p, theta) batch treecode that is at least as accurate (cubed
+    level 7, 8 vortices; work = 8.5 vector instructions per series term + 11
+    per near pair, the kernels' instruction counts)."""
+    m, src, g, _ = _field(mesh, 7, 64, 12)
+    _gpu_tree(ctx, m, src, g, 64, 12)
+    u_ref = LN.u_ref_of(g)
+    ud = np.stack(ctx.direct_sum(src, src, g))   # the self pair contributes 0 (|den| <= 1e-13)
+    ls = LN.labels(ctx, src, 0.85, u_ref, batch_stride=2, keep_rate=1.0, max_samples=1 << 21)
+    fr = LN.fit(ctx, ls, 3e-7, 8, epochs=10, seed=3)
+    assert fr.accuracy > 0.6
+    work = lambda st: 8.5 * st["terms"] + 11.0 * st["near"]
+    ul = LN.evaluate(ctx, src, 0.85, 0, 8, u_ref)
+    err_l, work_l = O.rel_l2(np.stack(ul), ud), work(T.stats(ctx))
+    assert err_l < 1e-5
+    best = None
+    for p in (4, 5, 6, 7, 8):
+        for th in (0.25, 0.3, 0.35, 0.4, 0.5, 0.6, 0.7):
+            ub = T.evaluate_batch(ctx, src, th, p, prec=L.LPM_F32C)
+            if O.rel_l2(np.stack(ub), ud) <= err_l:
+                w = work(T.stats(ctx))
+                best = w if best is None else min(best, w)
+    assert best is not None and work_l < 0.8 * best, (work_l, best)
