@@ -240,6 +240,54 @@ DirectSum::DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_p
     : _vecKernel(kernel), _mesh(pmesh), _vertTgt(pmesh->getVertexFieldPtr(tgt)), _faceSrc(pmesh->getFaceFieldPtr(src)),
       _faceTgt(pmesh->getFaceFieldPtr(tgt)) {}
 
+DirectSum::DirectSum(std::shared_ptr<Particles> particles, const std::shared_ptr<ScalarKernel> kernel,
+                     const std::string tgt, const std::string src, const std::string weight)
+    : _kernel(kernel), _srcLocs(particles->getCoordPtr()), _srcVals(particles->getFieldPtr(src)),
+      _tgtVals(particles->getFieldPtr(tgt)), _useWeights(!weight.empty()) {
+    if (_useWeights) _srcWeights = particles->getFieldPtr(weight);
+    if (_srcVals.expired() || _tgtVals.expired() || (_useWeights && _srcWeights.expired()))
+        throw std::invalid_argument("DirectSum: unknown particle field");
+}
+
+void Particles::initializeFieldWithFunction(const std::string& name, const AnalyticFunction* fn) {
+    auto f = getFieldPtr(name);
+    if (!f) throw std::invalid_argument("Particles: unknown field " + name);
+    for (index_type i = 0; i < n(); ++i) {
+        const XyzVector v = _coords->getVec(i);
+        if (f->nDim() == 1) f->replace(i, fn->evaluateScalar(v));
+        else f->replace(i, fn->evaluateVector(v));
+    }
+}
+
+// src/LpmDirectSum.cpp:175-254.  The singular branch there stops at
+// endIndex(i) instead of endIndex(myRank); the block here is the rank's.
+void DirectSum::meshfreeSolve(const MPIReplicatedData& mp) const {
+    auto crd = _srcLocs.lock();
+    auto src = _srcVals.lock();
+    auto tgt = _tgtVals.lock();
+    auto k = _kernel.lock();
+    if (!crd || !src || !tgt || !k) throw std::logic_error("DirectSum::meshfreeSolve: particles or fields expired");
+    auto gk = std::dynamic_pointer_cast<SphereGreensFn>(k);
+    if (!gk) throw std::logic_error("invalid kernel");
+    auto w = _useWeights ? _srcWeights.lock() : nullptr;
+    if (_useWeights && !w) throw std::logic_error("DirectSum::meshfreeSolve: weight field expired");
+    const int64_t ns = int64_t(crd->n());
+    std::vector<scalar_type> g(ns);
+    for (int64_t j = 0; j < ns; ++j) g[j] = src->getScalar(j) * (w ? w->getScalar(j) : 1.0);
+    const int r = mp.getRank();
+    const index_type b = mp.startIndex(r), e = mp.endIndex(r);
+    const int64_t nt = std::max<int64_t>(0, int64_t(e - b + 1));
+    std::vector<int64_t> skip(nt);
+    for (int64_t i = 0; i < nt; ++i) skip[i] = gk->isSingular() ? int64_t(b + i) : -1;
+    std::vector<scalar_type> psi(nt);
+    GpuContext& gpu = GpuContext::threadDefault();
+    gpu.check(lpm_direct_sum(gpu.get(), LPM_K_GREENS_SPHERE, _prec, gk->radius(), 0.0, crd->x.data() + b,
+                             crd->y.data() + b, crd->z.data() + b, nt, crd->x.data(), crd->y.data(), crd->z.data(),
+                             g.data(), ns, skip.data(), nullptr, nullptr, nullptr, psi.data()),
+              "DirectSum::meshfreeSolve");
+    for (int64_t i = 0; i < nt; ++i) tgt->replace(b + i, psi[i]);
+}
+
 namespace {
 // Leaf-face sources (ascending face index) with strength zeta * area and the
 // targets of one rank's blocks: vertices [vs, ve] then leaf faces [fs, fe].

@@ -173,6 +173,30 @@ class Coords {
     std::vector<scalar_type> x, y, z;
 };
 
+// ---------------------------------------------------------------- particles
+// Point particles with named fields for the meshfree direct sum (Particles,
+// src/LpmParticles.h:17-79: construction from a coordinate set, fields by
+// name).
+class Particles {
+  public:
+    explicit Particles(std::shared_ptr<Coords> crds) : _coords(std::move(crds)) {}
+    std::shared_ptr<Coords> getCoordPtr(const bool = false) const { return _coords; }
+    index_type n() const { return _coords->n(); }
+    void createField(const std::string& name, const std::string& units, const int dim) {
+        _fields[name] = std::make_shared<Field>(n(), dim, name, units);
+    }
+    // null if there is no such field (the reference logs a warning and returns null)
+    std::shared_ptr<Field> getFieldPtr(const std::string& name) const {
+        auto it = _fields.find(name);
+        return it == _fields.end() ? nullptr : it->second;
+    }
+    void initializeFieldWithFunction(const std::string& name, const AnalyticFunction* fn);
+
+  private:
+    std::shared_ptr<Coords> _coords;
+    std::map<std::string, std::shared_ptr<Field>> _fields;
+};
+
 // ---------------------------------------------------------------- GPU context
 // One lpm_ctx per host thread, created on first use.
 class GpuContext {
@@ -243,9 +267,19 @@ class DirectSum {
     DirectSum(std::shared_ptr<MeshedParticles> pmesh, const std::shared_ptr<VectorKernel> kernel,
               const std::string tgtFieldName, const std::string srcFieldName);
 
+    // Meshfree direct sum over point particles (src/LpmDirectSum.h:22-23):
+    // target field = sum_j kernel(x_i, x_j) src_j [weight_j], j != i for a
+    // singular kernel, over the rank block [startIndex, endIndex] of the
+    // particles.  The GPU evaluates SphereGreensFn.
+    DirectSum(std::shared_ptr<Particles> particles, const std::shared_ptr<ScalarKernel> kernel,
+              const std::string tgtFieldName, const std::string srcFieldName,
+              const std::string srcWeightName = std::string());
+
     void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces) const;
+    void meshfreeSolve(const MPIReplicatedData& mpiParticles) const;
     // Every rank's block is computed in this process: nothing to exchange.
     void meshBroadcast(const MPIReplicatedData&, const MPIReplicatedData&) const {}
+    void meshfreeBroadcast(const MPIReplicatedData&) const {}
 
     // LPM_F64 (default, the reference's pair formula) or LPM_F32C.
     void setPrecision(int prec) { _prec = prec; }
@@ -255,6 +289,9 @@ class DirectSum {
     std::weak_ptr<VectorKernel> _vecKernel;
     std::weak_ptr<MeshedParticles> _mesh;
     std::weak_ptr<Field> _vertTgt, _faceSrc, _faceTgt;
+    std::weak_ptr<Coords> _srcLocs;
+    std::weak_ptr<Field> _srcVals, _tgtVals, _srcWeights;
+    bool _useWeights = false;
     int _prec = LPM_F64;
 };
 

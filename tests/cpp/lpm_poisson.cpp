@@ -44,6 +44,40 @@ int main(int argc, char** argv) {
             std::printf("%s level %d: psi max rel err %.3e\n", seed->idString().c_str(), k, num / den);
             CHECK(num / den < prev);
             prev = num / den;
+            if (k == 4) {
+                // meshfree sum over the leaf-face particles weighted by area equals the mesh sum at the faces
+                auto crd = std::make_shared<Coords>();
+                std::vector<index_type> ids;
+                for (index_type i = 0; i < pm->nFaces(); ++i) {
+                    if (pm->faceIsDivided(i)) continue;
+                    crd->insert(pm->getFaceCentroid(i));
+                    ids.push_back(i);
+                }
+                auto parts = std::make_shared<Particles>(crd);
+                parts->createField("zeta", "", 1);
+                parts->createField("area", "", 1);
+                parts->createField("psi", "", 1);
+                parts->initializeFieldWithFunction("zeta", &y54);
+                for (size_t j = 0; j < ids.size(); ++j)
+                    parts->getFieldPtr("area")->replace(index_type(j), pm->getFaceArea(ids[j]));
+                DirectSum mf(parts, kernel, "psi", "zeta", "area");
+                const MPIReplicatedData blocks(parts->n(), 0, 1);
+                mf.meshfreeSolve(blocks);
+                mf.meshfreeBroadcast(blocks);
+                double d = 0.0, m = 0.0;
+                for (size_t j = 0; j < ids.size(); ++j) {
+                    d = std::max(d, std::abs(parts->getFieldPtr("psi")->getScalar(index_type(j)) -
+                                             psi->getScalar(ids[j])));
+                    m = std::max(m, std::abs(psi->getScalar(ids[j])));
+                }
+                std::printf("  meshfree vs mesh psi: max rel diff %.3e\n", d / m);
+                CHECK(d / m < 1e-12);
+                bool threw = false;
+                try {
+                    DirectSum bad(parts, kernel, "psi", "nosuchfield");
+                } catch (const std::invalid_argument&) { threw = true; }
+                CHECK(threw);
+            }
         }
         CHECK(prev < 0.06);
     }
