@@ -71,6 +71,97 @@ def cpu_baseline(src, g, target_seconds: float):
             "sample": f"{k} random targets x all {n} sources, FP64 reference-order oracle, {dt:.1f} s"}
 
 
+def rk4_level10(ctx, args):
+    """RK4 on the rotating sphere at cubed-sphere level 10 (6,291,456
+    particles): zeta_rel = 2 alpha z + 30 cos(4 lambda) z (z^2 - 1)^2 (solid
+    body plus the Rossby-Haurwitz wave), alpha = 1, Omega = 2 pi, dt = 0.01.
+    The order model is trained on this field's own labels at t = 0; then
+    `l10_steps` steps run with the learned FP32C treecode and with the plain
+    FP64 treecode from the same start.  Reported: ms per step of each, the
+    velocity error of both at t = 0 on 4096 random targets against the FP64
+    direct sum, and the learned trajectory's distance from the plain one."""
+    import numpy as np
+    import torch
+    import lpmcxx_amd as L
+    from lpmcxx_amd import learned as LN
+    from lpmcxx_amd import tree as T
+
+    log(f"[bench] level-{args.l10_level} RK4: mesh")
+    m = L.Mesh(L.LPM_SEED_CUBED, args.l10_level)
+    n = m.n_leaves
+    omega, alpha, dt = 2 * np.pi, 1.0, 0.01
+    zrel = 2 * alpha * m.cz + L.sphere_harmonic54(m.cx, m.cy, m.cz)
+    g = zrel * m.area
+    u_ref = LN.u_ref_of(g)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X, Y, Z, G = dev(m.cx), dev(m.cy), dev(m.cz), dev(g)
+    ZA, A = dev(zrel + 2 * omega * m.cz), dev(m.area)
+
+    ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
+    t0 = time.perf_counter()
+    T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+    T.moments_dev(ctx, G, 12, 1)
+    ls = LN.labels(ctx, (m.cx, m.cy, m.cz), args.theta_max, u_ref, batch_stride=args.l10_label_stride,
+                   keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
+    fr = LN.fit(ctx, ls, args.l10_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+    log(f"[bench] level-{args.l10_level} labels {len(ls.ids)}, labels + training {t_fit:.1f} s")
+
+    # velocity error at t = 0 on a sample of targets
+    idx = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False))
+    I = dev(idx.astype(np.int64))
+    S = [dev(a[idx]) for a in (m.cx, m.cy, m.cz)]
+    ref = [torch.empty_like(S[0]) for _ in range(3)]
+    ctx.direct_sum_dev(*S, X, Y, Z, G, ref, prec=L.LPM_F64)
+    U = [torch.empty_like(X) for _ in range(3)]
+
+    def sample_err():
+        ctx.synchronize()
+        d = torch.stack([u[I] for u in U]) - torch.stack(ref)
+        return float(torch.linalg.norm(d) / torch.linalg.norm(torch.stack(ref)))
+
+    T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+    T.moments_dev(ctx, G, args.max_order, 1)
+    LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
+    err_l = sample_err()
+    T.build_dev(ctx, X, Y, Z, m.max_radius, args.base_leaf)
+    T.moments_dev(ctx, G, args.base_order, 1)
+    T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
+    err_b = sample_err()
+
+    cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
+                                   theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
+    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
+                                   theta=args.base_theta, leaf_max=args.base_leaf)
+
+    def run(cfg):
+        warm = [X.clone(), Y.clone(), Z.clone()]
+        ctx.rk4_step_dev(cfg, dt, omega, *warm, ZA, A)
+        P = [X.clone(), Y.clone(), Z.clone()]
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.l10_steps):
+            ctx.rk4_step_dev(cfg, dt, omega, *P, ZA, A)
+        ctx.synchronize()
+        return (time.perf_counter() - t) / args.l10_steps, P
+
+    log(f"[bench] level-{args.l10_level} RK4: {args.l10_steps} learned steps, {args.l10_steps} plain FP64 steps")
+    s_l, P_l = run(cfg_l)
+    s_b, P_b = run(cfg_b)
+    P0 = torch.stack([X, Y, Z])
+    Pl, Pb = torch.stack(P_l), torch.stack(P_b)
+    traj = float(torch.linalg.norm(Pl - Pb) / torch.linalg.norm(Pb - P0))
+    log(f"[bench] level-{args.l10_level} RK4: learned {1e3 * s_l:.1f} ms/step (t=0 err {err_l:.2e}), plain FP64 "
+        f"{1e3 * s_b:.1f} ms/step (t=0 err {err_b:.2e}), trajectory difference {traj:.2e}")
+    return {"particles": n, "steps": args.l10_steps, "dt": dt, "omega": omega,
+            "field": "solid body (alpha 1) + Rossby-Haurwitz wave 30 cos(4 lambda) z (z^2-1)^2",
+            "learned_ms_per_step": 1e3 * s_l, "plain_fp64_ms_per_step": 1e3 * s_b, "speedup": s_b / s_l,
+            "learned_rel_l2_err_t0": err_l, "plain_fp64_rel_l2_err_t0": err_b,
+            "err_sample": "4096 random targets vs the FP64 direct sum over all sources",
+            "trajectory_rel_diff": traj, "tau": args.l10_tau, "label_samples": int(len(ls.ids)),
+            "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +172,7 @@ def main() -> int:
     ap.add_argument("--leaf", type=int, default=128)
     ap.add_argument("--theta-max", type=float, default=0.85)
     ap.add_argument("--tau", type=float, default=3e-7)
-    ap.add_argument("--max-order", type=int, default=8)
+    ap.add_argument("--max-order", type=int, default=7)
     ap.add_argument("--margin", type=int, default=0)
     ap.add_argument("--learned-cell", type=int, default=1024, help="targets per batch cell (learned treecode)")
     ap.add_argument("--epochs", type=int, default=8)
@@ -100,6 +191,11 @@ def main() -> int:
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-direct64", action="store_true", help="judge errors against the FP32C direct sum")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--no-rk4-l10", action="store_true", help="skip the level-10 RK4 comparison")
+    ap.add_argument("--l10-level", type=int, default=10)
+    ap.add_argument("--l10-steps", type=int, default=10)
+    ap.add_argument("--l10-tau", type=float, default=1.5e-7)
+    ap.add_argument("--l10-label-stride", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
@@ -224,7 +320,8 @@ def main() -> int:
     st = T.stats(ctx)
     learned_eval_ms = float(np.mean(eval_ms))
     nn, depth, nleaves = T.info(ctx)
-    log(f"[bench] learned FP32C treecode: {1e3 * learned_s:.1f} ms/step (eval {learned_eval_ms:.1f}), "
+    log(f"[bench] learned FP32C treecode: {1e3 * learned_s:.1f} ms/step (low-order list kernel "
+        f"{learned_eval_ms:.2f} ms), "
         f"rel-L2 {learned_err:.2e}; far {st['far']} near {st['near']} terms {st['terms']} model {st['model']}")
 
     if learned_err <= 1e-5:
@@ -235,14 +332,18 @@ def main() -> int:
         step_name, step_s, step_err = "direct_f32c", ms32 / 1e3, err32
     value = pairs / step_s
 
-    # roofline of the step's evaluation kernels (FP32 VALU work)
+    # roofline of the step's dominant evaluation kernel (FP32 VALU work); for
+    # the learned step that is the low-order list kernel, so the far pairs and
+    # terms of the order 8..12 list kernel are left out
     def tree_roof(stats, ms, kernel):
-        flops = stats["near"] * DIRECT_FLOPS + stats["terms"] * 8 + stats["far"] * 16
+        far = stats["far"] - stats.get("far_high", 0)
+        terms = stats["terms"] - stats.get("terms_high", 0)
+        flops = stats["near"] * DIRECT_FLOPS + terms * 8 + far * 16
         ach = flops / (ms / 1e3) / 1e12
         return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
                 "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
-                "flops_per_launch": flops, "launch_ms": ms, "far_pairs": stats["far"], "near_pairs": stats["near"],
-                "far_terms": stats["terms"]}
+                "flops_per_launch": flops, "launch_ms": ms, "far_pairs": far, "near_pairs": stats["near"],
+                "far_terms": terms}
 
     direct_flops = pairs * DIRECT_FLOPS
     direct_roof = {"bound": "valu", "kernel": "ds_f32c", "achieved": direct_flops / (ms32 / 1e3) / 1e12,
@@ -250,7 +351,7 @@ def main() -> int:
                    "launch_ms": ms32}
     direct_roof["frac"] = direct_roof["achieved"] / FP32_PEAK
     direct_roof["frac_nonpacked"] = direct_roof["achieved"] / FP32_NONPACKED
-    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists (low-order + high-order launch)"),
+    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists<7, true, true> (near leaves + far p<=7)"),
             "batch": tree_roof(batch_stats, batch_eval_ms, "tree_eval_batch"),
             "direct_f32c": direct_roof}[step_name]
 
@@ -277,6 +378,10 @@ def main() -> int:
                              f"leaf {args.leaf}), dt 0.01, Omega 2 pi, FP64 update and renormalization"}
         log(f"[bench] RK4: learned {1e3 / rl:.1f} ms/step, plain FP64 treecode {1e3 / rb:.1f} ms/step")
 
+    rk4_l10 = None
+    if not args.no_rk4_l10:
+        rk4_l10 = rk4_level10(ctx, args)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
@@ -297,7 +402,7 @@ def main() -> int:
                    "rel_l2_err_vs_fp64_direct": step_err},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "learned": {"ms_per_step": 1e3 * learned_s, "eval_ms": learned_eval_ms, "rel_l2_err": learned_err,
+        "learned": {"ms_per_step": 1e3 * learned_s, "low_list_kernel_ms": learned_eval_ms, "rel_l2_err": learned_err,
                     "speedup_vs_plain_fp64_tree": base_s / learned_s, "label_samples": int(len(ls.ids)),
                     "label_s": t_label, "train_s": t_train, "train_loss": fr.final_loss,
                     "train_accuracy": fr.accuracy, "stats": st, "nodes": nn, "depth": depth, "leaves": nleaves},
@@ -315,6 +420,8 @@ def main() -> int:
     }
     if rk4:
         out.update(rk4)
+    if rk4_l10:
+        out["rk4_level10"] = rk4_l10
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

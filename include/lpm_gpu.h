@@ -165,20 +165,28 @@ int lpm_set_option(lpm_ctx* ctx, int option, double value);
 /* Duration of the last pair or evaluation kernel, measured with HIP events
  * recorded on the context stream around it alone: ds_f64 / ds_f32c of
  * lpm_direct_sum*, tree_eval of lpm_tree_eval*, tree_eval_batch of
- * lpm_tree_eval_batch*, and the interaction-list kernels (tree_eval_lists,
- * not the planning pass) of lpm_tree_eval_learned*.  Waits for the kernel.
+ * lpm_tree_eval_batch*, and the low-order interaction-list kernel
+ * (tree_eval_lists of the near leaves and the far entries of order <= 7; not
+ * the planning pass, not the order 8..12 list kernel) of
+ * lpm_tree_eval_learned*.  Waits for the kernel.
  * No counterpart in the reference. */
 int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
 
 /* Counters of the last tree evaluation (any of the lpm_tree_eval* calls):
  * out[0] far-field (target, node) pairs, out[1] near-field (target, source)
  * pairs, out[2] far-field series terms summed over the far pairs
- * (p(p+1)(p+2)/6 each), out[3] order-model evaluations.  Synchronizes.
+ * (p(p+1)(p+2)/6 each), out[3] order-model evaluations, and for
+ * lpm_tree_eval_learned* the part of out[0] (out[4]) and of out[2] (out[5])
+ * evaluated by the order 8..12 list kernel (zero otherwise).  out holds
+ * LPM_TREE_NSTATS values.  Synchronizes.
  * No counterpart in the reference (its macCounter is out[0] of lpm_tree_eval). */
 #define LPM_STAT_FAR 0
 #define LPM_STAT_NEAR 1
 #define LPM_STAT_TERMS 2
 #define LPM_STAT_MODEL 3
+#define LPM_STAT_FAR_HIGH 4
+#define LPM_STAT_TERMS_HIGH 5
+#define LPM_TREE_NSTATS 6
 int lpm_tree_stats(lpm_ctx* ctx, int64_t* out);
 
 /* ------------------------------------------------------------------ batch treecode

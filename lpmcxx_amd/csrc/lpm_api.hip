@@ -54,8 +54,11 @@ int lpm_create(lpm_ctx** out, int device) {
     if (!c) return LPM_ERR_ALLOC;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_beg) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        hipEventCreate(&c->ev_beg) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->pin), kPinWords * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
         if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
+        if (c->ev_end) (void)hipEventDestroy(c->ev_end);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return LPM_ERR_HIP;
@@ -76,6 +79,7 @@ void lpm_destroy(lpm_ctx* c) {
         b->release();
     if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
+    if (c->pin) (void)hipHostFree(c->pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -28,7 +28,7 @@ struct Model;  // order model state (mlp.hip)
 
 // counters of the last tree evaluation: far (target, node) pairs, near
 // (target, source) pairs, far-field series terms, order-model calls
-constexpr int kStats = 4;
+constexpr int kStats = 6;
 
 }  // namespace lpm
 
@@ -52,7 +52,13 @@ struct lpm_ctx {
     // events around the last pair / evaluation kernel (lpm_last_kernel_ms)
     hipEvent_t ev_beg = nullptr, ev_end = nullptr;
     bool ev_valid = false;
+    // pinned host words for small device-to-host reads (build table, flags)
+    int* pin = nullptr;
 };
+
+namespace lpm {
+constexpr int kPinWords = 256;
+}
 
 namespace lpm {
 
@@ -101,10 +107,14 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
                           int** tperm);
 // Cell-aligned target batches: the Morton-sorted targets split at the
 // boundaries of the octree cells of level L (the shallowest level with at
-// least nt / 256 occupied cells) and cut into runs of at most 64.
-// bstart[b] = first sorted position of batch b, nb batches.
+// least nt / batch_cell_targets occupied cells) and cut into runs of at most
+// 64.  Enqueued without waiting: bstart[b] = first sorted position of batch
+// b, bstart[nb] = nt; the count nb stays in device memory (*nb_dev) and
+// *nb_bound >= nb bounds it for launch sizes.
 int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
-                        int** bstart, int64_t* nb);
+                        int** bstart, int64_t* nb_bound, const int** nb_dev);
+// waits for the batches and reads their count
+int read_batch_count(lpm_ctx* c, const int* nb_dev, int64_t* nb);
 // batch-MAC treecode (treecode_batch.hip): prec LPM_F64, or LPM_F32C with
 // FP32 far field and FP32C near field; order <= tree order
 int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,

@@ -10,6 +10,17 @@ namespace lpm {
 
 constexpr int kMaxDepth = LPM_TREE_MAX_DEPTH;  // 21 levels x 3 bits = 63-bit keys
 constexpr int kMaxOrder = 16;                  // series order limit (|k| < p)
+constexpr int kChunk = 128;                    // leaf particles per block of the leaf moment kernel
+
+// Build table in device memory (int32): first node and node count of levels
+// 0 .. kMaxDepth + 1, then flags and counters written by the build kernels.
+constexpr int kLvWords = 2 * (kMaxDepth + 2);
+constexpr int kLvOutside = kLvWords;      // a source lies outside the root box
+constexpr int kLvTooDeep = kLvWords + 1;  // a node at depth kMaxDepth must split
+constexpr int kLvKeys = kLvWords + 2;     // a node below the sorted key levels must split
+constexpr int kLvLeaves = kLvWords + 3;   // number of leaves
+constexpr int kLvChunks = kLvWords + 4;   // leaf chunks of kChunk particles
+constexpr int kLvTotal = kLvWords + 5;
 
 struct Tree {
     int64_t ns = 0;       // sources
@@ -31,6 +42,9 @@ struct Tree {
                       // max over children of |c_child - c| + r_child above (0 if empty)
     std::vector<int64_t> level_start;  // host: first node of every level, then nn
     int64_t cap = 0;
+    DevBuf lvt;           // int32 [kLvTotal] build table
+    DevBuf chunks;        // int32: chunk node [nch], begin [nch], end [nch], first chunk [nn], count [nn]
+    int64_t nch = 0;      // leaf chunks (moment kernel blocks)
     // moments
     int order = 0;        // p (|k| < p), 0 = none
     int moment_sign = 1;
@@ -40,7 +54,8 @@ struct Tree {
     DevBuf soff32;        // float4 [ns] FP32 source offsets from their leaf centroid + gamma
     DevBuf nstat;         // float4 [nn] |m| of the degree-0, 1, 2 moment shells, particle count
     // scratch
-    DevBuf tmp0, tmp1, tmp2, tmp3, sort_tmp;
+    DevBuf mpart;         // double [nch][nterms][3] leaf-chunk moment partials
+    DevBuf tmp0, tmp1, tmp2, tmp3, sort_tmp, scan_tmp;
 };
 
 inline int nterms(int p) { return p * (p + 1) * (p + 2) / 6; }

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void tree_keys(const double* __restrict__ x, c
                                                  const double* __restrict__ z, int64_t n, double x0, double x1,
                                                  double y0, double y1, double z0, double z1,
                                                  uint64_t* __restrict__ keys, int* __restrict__ vals,
-                                                 int* __restrict__ outside) {
+                                                 int* __restrict__ lvt) {
+    int* outside = lvt + kLvOutside;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = x[i], py = y[i], pz = z[i];
@@ -68,41 +69,80 @@ __device__ inline int lower_code(const uint64_t* keys, int b, int e, int level, 
     return b;
 }
 
+// Root node and an empty level table (level 0 = the root).  One thread.
+__global__ void tree_root(int n, int* nbeg, int* nend, int* nlevel, int* nparent, uint64_t* nprefix,
+                          int* __restrict__ lvt) {
+    nbeg[0] = 0;
+    nend[0] = n;
+    nlevel[0] = 0;
+    nparent[0] = -1;
+    nprefix[0] = 0;
+    for (int w = 0; w < kLvWords; ++w) lvt[w] = 0;
+    lvt[1] = 1;
+    lvt[kLvTooDeep] = 0;
+    lvt[kLvKeys] = 0;
+    lvt[kLvLeaves] = 0;
+    lvt[kLvChunks] = 0;
+}
+
+// Children count of every node of `level` (first node and count from the
+// level table); cnt[t] = 0 for t in [count, ub) so a scan over ub entries is
+// exact.  Leaves are counted with their moment chunks.
 // control 0 (MAX_COORDS_PER_NODE): split iff count > param
 // control 1 (MAX_DEPTH):           split iff level < param (src/LpmOctree.cpp:145-180)
-__global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ keys, int lvl_start, int lvl_count,
-                                                  int level, int control, int param, const int* __restrict__ nbeg,
-                                                  const int* __restrict__ nend, int* __restrict__ cnt,
-                                                  int* __restrict__ nfirst, int* __restrict__ nkids) {
+__global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ keys, int level, int sorted_levels,
+                                                  int control, int param, int ub, int* __restrict__ lvt,
+                                                  const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                  int* __restrict__ cnt, int* __restrict__ nfirst,
+                                                  int* __restrict__ nkids) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= lvl_count) return;
-    const int i = lvl_start + t;
+    if (t >= ub) return;
+    const int start = lvt[2 * level], count = lvt[2 * level + 1];
+    if (t >= count) {
+        cnt[t] = 0;
+        return;
+    }
+    const int i = start + t;
     const int b = nbeg[i], e = nend[i];
     const bool want = (control == 0) ? (e - b > param) : (level < param && e > b);
     int c = 0;
-    if (want && level < kMaxDepth) {
-        int prev = b;
-        for (int code = 1; code <= 8; ++code) {
-            const int nxt = (code == 8) ? e : lower_code(keys, prev, e, level, code);
-            c += (nxt > prev);
-            prev = nxt;
+    if (want) {
+        if (level >= kMaxDepth) atomicOr(lvt + kLvTooDeep, 1);         // cannot split further
+        else if (level >= sorted_levels) atomicOr(lvt + kLvKeys, 1);   // keys sorted too coarsely
+        else {
+            int prev = b;
+            for (int code = 1; code <= 8; ++code) {
+                const int nxt = (code == 8) ? e : lower_code(keys, prev, e, level, code);
+                c += (nxt > prev);
+                prev = nxt;
+            }
         }
     }
     cnt[t] = c;
     nfirst[i] = -1;
     nkids[i] = 0;
-    if (want && level >= kMaxDepth) cnt[t] = -1;  // cannot split further
+    if (c == 0) {
+        atomicAdd(lvt + kLvLeaves, 1);
+        atomicAdd(lvt + kLvChunks, (e - b + kChunk - 1) / kChunk);
+    }
 }
 
-__global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ keys, int lvl_start, int lvl_count,
-                                                 int level, const int* __restrict__ cnt, const int* __restrict__ off,
-                                                 int base, int* nbeg, int* nend, int* nfirst, int* nkids,
-                                                 int* nlevel, int* nparent, uint64_t* nprefix) {
+// Children of the nodes of `level` at first + count + off[t] (off = exclusive
+// scan of cnt), and the next level's entry of the level table.
+__global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ keys, int level, int* lvt,
+                                                 const int* __restrict__ cnt, const int* __restrict__ off, int* nbeg,
+                                                 int* nend, int* nfirst, int* nkids, int* nlevel, int* nparent,
+                                                 uint64_t* nprefix) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= lvl_count || cnt[t] <= 0) return;
-    const int i = lvl_start + t;
+    const int start = lvt[2 * level], count = lvt[2 * level + 1];
+    if (t == 0) {
+        lvt[2 * level + 2] = start + count;
+        lvt[2 * level + 3] = count > 0 ? off[count - 1] + cnt[count - 1] : 0;
+    }
+    if (t >= count || cnt[t] <= 0) return;
+    const int i = start + t;
     const int b = nbeg[i], e = nend[i];
-    int k = base + off[t];
+    int k = start + count + off[t];
     nfirst[i] = k;
     nkids[i] = cnt[t];
     int prev = b;
@@ -119,6 +159,29 @@ __global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ ke
             ++k;
         }
         prev = nxt;
+    }
+}
+
+// Leaf chunk list of the moment kernel: node i owns chunks n0[i] .. n0[i] +
+// nc[i] - 1 (n0 = exclusive scan of nc), each of at most kChunk particles.
+__global__ __launch_bounds__(256) void chunk_count(int nn, const int* __restrict__ nkids,
+                                                   const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                   int* __restrict__ nc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) nc[i] = nkids[i] == 0 ? (nend[i] - nbeg[i] + kChunk - 1) / kChunk : 0;
+}
+
+__global__ __launch_bounds__(256) void chunk_fill(int nn, const int* __restrict__ n0, const int* __restrict__ nc,
+                                                  const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                  int* __restrict__ cn, int* __restrict__ cb, int* __restrict__ ce) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int b = nbeg[i], e = nend[i];
+    for (int k = 0; k < nc[i]; ++k) {
+        const int q = n0[i] + k;
+        cn[q] = i;
+        cb[q] = b + k * kChunk;
+        ce[q] = min(e, b + (k + 1) * kChunk);
     }
 }
 
@@ -253,23 +316,29 @@ int grow_nodes(lpm_ctx* c, Tree* t, int64_t need) {
     return LPM_OK;
 }
 
+// stable sort of (key, value) pairs by key bits [begin_bit, 3 kMaxDepth)
 template <class K, class V>
-int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout, int64_t n, int end_bit) {
+int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout, int64_t n, int begin_bit) {
     size_t bytes = 0;
-    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, int(n), 0, end_bit, c->stream));
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, int(n), begin_bit,
+                                                  3 * kMaxDepth, c->stream));
     LPM_CHECK(t->sort_tmp.reserve(bytes + 256));
-    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, kin, kout, vin, vout, int(n), 0, end_bit,
-                                                  c->stream));
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, kin, kout, vin, vout, int(n), begin_bit,
+                                                  3 * kMaxDepth, c->stream));
     return LPM_OK;
 }
+
+constexpr int kFastLevels = 12;  // key levels sorted before the first level loop
+constexpr int kLevelGroup = 4;   // levels built between two reads of the build table
 
 }  // namespace
 
 void tree_free(Tree* t) {
     if (!t) return;
     for (DevBuf* b : {&t->perm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
-                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32, &t->soff32, &t->nstat, &t->tmp0,
-                      &t->tmp1, &t->tmp2, &t->tmp3, &t->sort_tmp})
+                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32,
+                      &t->soff32, &t->nstat, &t->mpart, &t->tmp0, &t->tmp1, &t->tmp2, &t->tmp3, &t->sort_tmp, &t->scan_tmp,
+                      &t->lvt, &t->chunks})
         b->release();
     delete t;
 }
@@ -291,69 +360,88 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     LPM_CHECK(t->perm.reserve(nalloc * sizeof(int)));
     LPM_CHECK(t->spos.reserve(nalloc * sizeof(double4)));
     LPM_CHECK(t->tmp0.reserve(nalloc * sizeof(uint64_t)));   // unsorted keys / truncated keys
-    LPM_CHECK(t->tmp1.reserve(nalloc * sizeof(int) + 64));   // unsorted values / flags
+    LPM_CHECK(t->tmp1.reserve(nalloc * sizeof(int) + 64));   // unsorted values
     LPM_CHECK(t->tmp2.reserve(nalloc * sizeof(uint64_t)));
-    int* outside = reinterpret_cast<int*>(t->tmp1.as<char>() + nalloc * sizeof(int));
-    LPM_HIP(c, hipMemsetAsync(outside, 0, sizeof(int), st));
-    if (n > 0) {
-        hipLaunchKernelGGL(tree_keys, dim3(blocks(n)), dim3(256), 0, st, x, y, z, n, blo[0], bhi[0], blo[1], bhi[1],
-                           blo[2], bhi[2], t->tmp0.as<uint64_t>(), t->tmp1.as<int>(), outside);
-        LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->keys.as<uint64_t>(), t->tmp1.as<int>(),
-                             t->perm.as<int>(), n, 3 * kMaxDepth));
-    }
-    int h_out = 0;
-    LPM_HIP(c, hipMemcpyAsync(&h_out, outside, sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    if (h_out) return fail(c, LPM_ERR_INVALID_ARG, "a source lies outside the root box");
-
-    // root
+    LPM_CHECK(t->tmp3.reserve(size_t(2 * nalloc + 2) * sizeof(int)));  // children counts | their scan
+    LPM_CHECK(t->lvt.reserve(kLvTotal * sizeof(int)));
+    size_t scan_bytes = 0;
+    int* cnt = t->tmp3.as<int>();
+    int* off = cnt + nalloc + 1;
+    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, int(nalloc), st));
+    LPM_CHECK(t->scan_tmp.reserve(scan_bytes + 256));
+    int* lvt = t->lvt.as<int>();
     LPM_CHECK(grow_nodes(c, t, std::max<int64_t>(1024, 4 * (n / std::max(1, leaf_max)) + 64)));
-    const int root_vals[6] = {0, int(n), -1, 0, 0, -1};
-    LPM_HIP(c, hipMemcpyAsync(t->nbeg.as<int>(), &root_vals[0], sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(t->nend.as<int>(), &root_vals[1], sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(t->nlevel.as<int>(), &root_vals[3], sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(t->nparent.as<int>(), &root_vals[5], sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemsetAsync(t->nprefix.as<uint64_t>(), 0, sizeof(uint64_t), st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    int64_t nn = 1, lvl_start = 0, lvl_count = 1;
+    LPM_HIP(c, hipMemsetAsync(lvt, 0, kLvTotal * sizeof(int), st));
+    if (n > 0)
+        hipLaunchKernelGGL(tree_keys, dim3(blocks(n)), dim3(256), 0, st, x, y, z, n, blo[0], bhi[0], blo[1], bhi[1],
+                           blo[2], bhi[2], t->tmp0.as<uint64_t>(), t->tmp1.as<int>(), lvt);
+
+    // Level by level: count children, scan, emit -- kLevelGroup levels between
+    // reads of the build table.  The keys are first sorted on their top
+    // kFastLevels levels only; a node below them that must split sends the
+    // build round again with fully sorted keys.
+    const int64_t split_bound = (control == 0) ? std::min<int64_t>(n, 8 * (n / (int64_t(leaf_max) + 1))) : n;
+    auto bound_after = [&](int64_t known, int steps) {  // node count bound `steps` levels below a known level
+        int64_t b = known;
+        for (int s2 = 0; s2 < steps; ++s2) b = std::min<int64_t>(split_bound, 8 * b);
+        return b;
+    };
+    int* h = c->pin;
+    int sorted_levels = std::min(kFastLevels, kMaxDepth);
+    int64_t nn = 1;
+    for (;;) {
+        if (n > 0)
+            LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->keys.as<uint64_t>(), t->tmp1.as<int>(),
+                                 t->perm.as<int>(), n, 3 * (kMaxDepth - sorted_levels)));
+        hipLaunchKernelGGL(tree_root, dim3(1), dim3(1), 0, st, int(n), t->nbeg.as<int>(), t->nend.as<int>(),
+                           t->nlevel.as<int>(), t->nparent.as<int>(), t->nprefix.as<uint64_t>(), lvt);
+        int lk = 0;
+        int64_t ck = 1;
+        nn = 1;
+        bool resort = false;
+        for (;;) {
+            const int G = std::min(kLevelGroup, kMaxDepth + 1 - lk);
+            int64_t need = nn;
+            for (int j = 1; j <= G; ++j) need += bound_after(ck, j);
+            LPM_CHECK(grow_nodes(c, t, need));
+            for (int j = 0; j < G; ++j) {
+                const int level = lk + j;
+                const int ub = int(std::max<int64_t>(1, j == 0 ? ck : bound_after(ck, j)));
+                hipLaunchKernelGGL(tree_count, dim3(blocks(ub)), dim3(256), 0, st, t->keys.as<uint64_t>(), level,
+                                   sorted_levels, control, param, ub, lvt, t->nbeg.as<int>(), t->nend.as<int>(), cnt,
+                                   t->nfirst.as<int>(), t->nkids.as<int>());
+                size_t sb = scan_bytes;
+                LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, cnt, off, ub, st));
+                hipLaunchKernelGGL(tree_emit, dim3(blocks(ub)), dim3(256), 0, st, t->keys.as<uint64_t>(), level, lvt,
+                                   cnt, off, t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(),
+                                   t->nkids.as<int>(), t->nlevel.as<int>(), t->nparent.as<int>(),
+                                   t->nprefix.as<uint64_t>());
+            }
+            LPM_HIP(c, hipGetLastError());
+            LPM_HIP(c, hipMemcpyAsync(h, lvt, kLvTotal * sizeof(int), hipMemcpyDeviceToHost, st));
+            LPM_HIP(c, hipStreamSynchronize(st));
+            if (h[kLvOutside]) return fail(c, LPM_ERR_INVALID_ARG, "a source lies outside the root box");
+            if (h[kLvTooDeep]) return fail(c, LPM_ERR_TREE_DEPTH, "tree deeper than %d levels (coincident sources)", kMaxDepth);
+            if (h[kLvKeys]) { resort = true; break; }
+            lk += G;
+            ck = h[2 * lk + 1];
+            nn = int64_t(h[2 * lk]) + ck;
+            if (ck == 0 || lk > kMaxDepth) break;
+        }
+        if (!resort) break;
+        sorted_levels = kMaxDepth;
+    }
     int depth = 0;
     t->level_start.assign(1, 0);
-    std::vector<int> hcnt(2);
-    for (int level = 0;; ++level) {
-        LPM_CHECK(t->tmp3.reserve(size_t(2 * lvl_count + 2) * sizeof(int)));
-        int* cnt = t->tmp3.as<int>();
-        int* off = cnt + lvl_count + 1;
-        hipLaunchKernelGGL(tree_count, dim3(blocks(lvl_count)), dim3(256), 0, st, t->keys.as<uint64_t>(),
-                           int(lvl_start), int(lvl_count), level, control, param, t->nbeg.as<int>(),
-                           t->nend.as<int>(), cnt, t->nfirst.as<int>(), t->nkids.as<int>());
-        // detect nodes that cannot be split (count > leaf_max at the depth limit)
-        std::vector<int> hc(lvl_count);
-        LPM_HIP(c, hipMemcpyAsync(hc.data(), cnt, lvl_count * sizeof(int), hipMemcpyDeviceToHost, st));
-        LPM_HIP(c, hipStreamSynchronize(st));
-        int64_t total = 0;
-        std::vector<int> ho(lvl_count);
-        for (int64_t i = 0; i < lvl_count; ++i) {
-            if (hc[i] < 0) return fail(c, LPM_ERR_TREE_DEPTH, "tree deeper than %d levels (coincident sources)", kMaxDepth);
-            ho[i] = int(total);
-            total += hc[i];
-        }
-        if (total == 0) break;
-        LPM_HIP(c, hipMemcpyAsync(off, ho.data(), lvl_count * sizeof(int), hipMemcpyHostToDevice, st));
-        LPM_CHECK(grow_nodes(c, t, nn + total));
-        hipLaunchKernelGGL(tree_emit, dim3(blocks(lvl_count)), dim3(256), 0, st, t->keys.as<uint64_t>(),
-                           int(lvl_start), int(lvl_count), level, cnt, off, int(nn), t->nbeg.as<int>(),
-                           t->nend.as<int>(), t->nfirst.as<int>(), t->nkids.as<int>(), t->nlevel.as<int>(),
-                           t->nparent.as<int>(), t->nprefix.as<uint64_t>());
-        LPM_HIP(c, hipGetLastError());
-        lvl_start = nn;
-        lvl_count = total;
-        nn += total;
-        depth = level + 1;
-        t->level_start.push_back(lvl_start);
+    for (int l = 1; l <= kMaxDepth && h[2 * l + 1] > 0; ++l) {
+        t->level_start.push_back(h[2 * l]);
+        depth = l;
     }
     t->level_start.push_back(nn);
     t->nn = nn;
     t->depth = depth;
+    t->nleaves = h[kLvLeaves];
+    t->nch = h[kLvChunks];
 
     LPM_CHECK(t->ncent.reserve(size_t(nn) * sizeof(double4)));
     LPM_CHECK(t->nbox.reserve(size_t(nn) * 6 * sizeof(double)));
@@ -361,13 +449,13 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                        bhi[2], t->nlevel.as<int>(), t->nprefix.as<uint64_t>(), t->ncent.as<double4>(),
                        t->nbox.as<double>());
     if (n > 0) {
-        // leaf-local ascending source order
+        // leaf-local ascending source order: keys truncated to the leaf depth, sorted from source order
         hipLaunchKernelGGL(tree_leaf_keys, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
                            t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
                            t->perm.as<int>(), t->tmp0.as<uint64_t>());
         hipLaunchKernelGGL(iota_i32, dim3(blocks(n)), dim3(256), 0, st, t->tmp1.as<int>(), n);
         LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->tmp2.as<uint64_t>(), t->tmp1.as<int>(),
-                             t->perm.as<int>(), n, 3 * kMaxDepth));
+                             t->perm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
         hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
                            t->spos.as<double4>());
     }
@@ -377,15 +465,24 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                        t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
                        t->nrad.as<double>());
     for (int l = depth - 1; l >= 0; --l) {
-        const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
-        hipLaunchKernelGGL(tree_radius_up, dim3(blocks(cnt)), dim3(256), 0, st, int(f), int(cnt),
+        const int64_t f = t->level_start[l], lc = t->level_start[l + 1] - f;
+        hipLaunchKernelGGL(tree_radius_up, dim3(blocks(lc)), dim3(256), 0, st, int(f), int(lc),
                            t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), t->nrad.as<double>());
     }
+    // leaf chunk list of the moment kernel
+    const int64_t nch = t->nch;
+    LPM_CHECK(t->chunks.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
+    int* ch = t->chunks.as<int>();
+    int *cn = ch, *cb = ch + nch, *ce = ch + 2 * nch, *n0 = ch + 3 * nch, *nc = n0 + nn;
+    hipLaunchKernelGGL(chunk_count, dim3(blocks(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(), t->nbeg.as<int>(),
+                       t->nend.as<int>(), nc);
+    size_t sb = 0;
+    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, nc, n0, int(nn), st));
+    LPM_CHECK(t->scan_tmp.reserve(sb + 256));  // the level scans have finished (table read above)
+    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, nc, n0, int(nn), st));
+    hipLaunchKernelGGL(chunk_fill, dim3(blocks(nn)), dim3(256), 0, st, int(nn), n0, nc, t->nbeg.as<int>(),
+                       t->nend.as<int>(), cn, cb, ce);
     LPM_HIP(c, hipGetLastError());
-    std::vector<int> hk(nn);
-    LPM_HIP(c, hipMemcpyAsync(hk.data(), t->nkids.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    t->nleaves = std::count(hk.begin(), hk.end(), 0);
     return LPM_OK;
 }
 

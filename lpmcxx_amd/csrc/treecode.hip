@@ -44,7 +44,6 @@ namespace lpm {
 
 namespace {
 
-constexpr int kChunk = 128;
 constexpr int kStack = 160;   // >= 7 * kMaxDepth + 1
 
 inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
@@ -301,9 +300,10 @@ __global__ __launch_bounds__(256) void batch_lcp_hist(const uint64_t* __restrict
 }
 
 // cs[i] = i where a level-L cell starts, else 0 (then a running max gives the cell start)
-__global__ __launch_bounds__(256) void batch_cell_start(const uint64_t* __restrict__ k, int64_t n, int L,
-                                                        int* __restrict__ cs) {
+__global__ __launch_bounds__(256) void batch_cell_start(const uint64_t* __restrict__ k, int64_t n,
+                                                        const int* __restrict__ level, int* __restrict__ cs) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int L = *level;
     if (i < n) cs[i] = (i == 0 || key_lcp(k[i - 1], k[i]) < L) ? int(i) : 0;
 }
 
@@ -312,6 +312,22 @@ __global__ __launch_bounds__(256) void batch_flags(const int* __restrict__ cs, i
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) f[i] = ((i - cs[i]) % kWave) == 0;
 }
+
+// The batching level L: the shallowest level with at least `want` occupied
+// cells (hist[l] = adjacent sorted keys whose common prefix is l levels).
+// One thread.
+__global__ void batch_level(const unsigned* __restrict__ hist, int64_t want, int* __restrict__ level) {
+    int L = kMaxDepth;
+    int64_t cells = 1;
+    for (int l = 0; l <= kMaxDepth; ++l) {
+        if (cells >= want) { L = l; break; }
+        if (l < kMaxDepth) cells += hist[l];
+    }
+    *level = L;
+}
+
+// bstart[nb] = n closes the last batch.  One thread.
+__global__ void batch_close(int* __restrict__ bs, const int* __restrict__ nb, int n) { bs[*nb] = n; }
 
 // ---------------------------------------------------------------- evaluation
 __device__ inline uint64_t uniform64(uint64_t v) {
@@ -446,38 +462,14 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     hipStream_t st = c->stream;
     const int nterm = nterms(p);
     const int64_t nn = t->nn;
-    // chunk list of the leaves (internal nodes come from the upward pass)
-    std::vector<int> hb(nn), he(nn), hk(nn);
-    LPM_HIP(c, hipMemcpyAsync(hb.data(), t->nbeg.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipMemcpyAsync(he.data(), t->nend.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipMemcpyAsync(hk.data(), t->nkids.ptr, nn * sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    std::vector<int> cn, cb, ce, n0(nn), nc(nn);
-    for (int64_t i = 0; i < nn; ++i) {
-        n0[i] = int(cn.size());
-        if (hk[i] == 0)
-            for (int b = hb[i]; b < he[i]; b += kChunk) {
-                cn.push_back(int(i));
-                cb.push_back(b);
-                ce.push_back(std::min(he[i], b + kChunk));
-            }
-        nc[i] = int(cn.size()) - n0[i];
-    }
-    const int64_t nch = int64_t(cn.size());
-    LPM_CHECK(t->tmp3.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
-    int* d_cn = t->tmp3.as<int>();
-    int* d_cb = d_cn + nch;
-    int* d_ce = d_cb + nch;
-    int* d_n0 = d_ce + nch;
-    int* d_nc = d_n0 + nn;
-    if (nch > 0) {
-        LPM_HIP(c, hipMemcpyAsync(d_cn, cn.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
-        LPM_HIP(c, hipMemcpyAsync(d_cb, cb.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
-        LPM_HIP(c, hipMemcpyAsync(d_ce, ce.data(), nch * sizeof(int), hipMemcpyHostToDevice, st));
-    }
-    LPM_HIP(c, hipMemcpyAsync(d_n0, n0.data(), nn * sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(d_nc, nc.data(), nn * sizeof(int), hipMemcpyHostToDevice, st));
-    LPM_CHECK(t->tmp2.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * 3 * sizeof(double)));
+    // chunk list of the leaves, made by the build (internal nodes come from the upward pass)
+    const int64_t nch = t->nch;
+    const int* d_cn = t->chunks.as<int>();
+    const int* d_cb = d_cn + nch;
+    const int* d_ce = d_cb + nch;
+    const int* d_n0 = d_ce + nch;
+    const int* d_nc = d_n0 + nn;
+    LPM_CHECK(t->mpart.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * 3 * sizeof(double)));
     LPM_CHECK(t->mom.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
     LPM_CHECK(t->momg.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
     LPM_CHECK(t->mom32.reserve(size_t(nn) * nterm * 3 * sizeof(float)));
@@ -490,10 +482,10 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     if (nch > 0) {
         const size_t lds = size_t(kChunk) * (3 * p + 3) * sizeof(double);
         hipLaunchKernelGGL(tree_moments_chunk, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
-                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->tmp2.as<double>());
+                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->mpart.as<double>());
     }
     hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), nterm, d_n0,
-                       d_nc, t->tmp2.as<double>(), t->mom.as<double>());
+                       d_nc, t->mpart.as<double>(), t->mom.as<double>());
     // upward pass, deepest internal level first
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
@@ -541,13 +533,13 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
 }
 
 int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
-                        int** bstart, int64_t* nb) {
+                        int** bstart, int64_t* nb_bound, const int** nb_dev) {
     LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm));
-    *nb = 0;
+    *nb_bound = 0;
     if (nt == 0) return LPM_OK;
     hipStream_t st = c->stream;
     const uint64_t* keys = c->ds_tgt.as<uint64_t>() + nt;  // sorted keys (k_out)
-    // workspace: hist[32] | nsel (int64) | cs[nt] | flags[nt] | bstart[nt] | scan/select temp
+    // workspace: hist[32] | L | nsel | cs[nt] | flags[nt] | bstart[nt + 1] | scan/select temp
     size_t scan_bytes = 0, sel_bytes = 0;
     int* cs0 = nullptr;
     LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(nullptr, scan_bytes, cs0, cs0, hipcub::Max(), int(nt), st));
@@ -560,35 +552,37 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     LPM_CHECK(c->tb_work.reserve(off_t + tmp_bytes));
     char* w = c->tb_work.as<char>();
     unsigned* hist = reinterpret_cast<unsigned*>(w);
-    int* nsel = reinterpret_cast<int*>(w + 128);
+    int* level = reinterpret_cast<int*>(w + 128);
+    int* nsel = reinterpret_cast<int*>(w + 132);
     int* cs = reinterpret_cast<int*>(w + off_cs);
     unsigned char* f = reinterpret_cast<unsigned char*>(w + off_f);
     int* bs = reinterpret_cast<int*>(w + off_b);
     void* tmp = w + off_t;
+    const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
     LPM_HIP(c, hipMemsetAsync(hist, 0, 128, st));
     hipLaunchKernelGGL(batch_lcp_hist, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, hist);
-    unsigned h[kMaxDepth + 1];
-    LPM_HIP(c, hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
-    int L = kMaxDepth;
-    int64_t cells = 1;
-    for (int l = 0; l <= kMaxDepth; ++l) {
-        if (cells >= want) { L = l; break; }
-        if (l < kMaxDepth) cells += h[l];
-    }
-    hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, L, cs);
+    hipLaunchKernelGGL(batch_level, dim3(1), dim3(1), 0, st, hist, want, level);
+    hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, level, cs);
     size_t b1 = tmp_bytes;
     LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(tmp, b1, cs, cs, hipcub::Max(), int(nt), st));
     hipLaunchKernelGGL(batch_flags, dim3(blocks(nt)), dim3(256), 0, st, cs, nt, f);
     size_t b2 = tmp_bytes;
     LPM_HIP(c, hipcub::DeviceSelect::Flagged(tmp, b2, hipcub::CountingInputIterator<int>(0), f, bs, nsel, int(nt),
                                              st));
-    int hn = 0;
-    LPM_HIP(c, hipMemcpyAsync(&hn, nsel, sizeof(int), hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
+    hipLaunchKernelGGL(batch_close, dim3(1), dim3(1), 0, st, bs, nsel, int(nt));
+    LPM_HIP(c, hipGetLastError());
+    // every level-L cell of s targets makes ceil(s / 64) batches; the level
+    // below L has fewer than `want` cells, so L has fewer than 8 want
     *bstart = bs;
-    *nb = hn;
+    *nb_bound = std::min<int64_t>(nt, (nt + kWave - 1) / kWave + 8 * want);
+    *nb_dev = nsel;
+    return LPM_OK;
+}
+
+int read_batch_count(lpm_ctx* c, const int* nb_dev, int64_t* nb) {
+    LPM_HIP(c, hipMemcpyAsync(c->pin, nb_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    *nb = c->pin[0];
     return LPM_OK;
 }
 

@@ -510,7 +510,7 @@ void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf
                     "TreeSum::meshSolve");
     }
     if (_learned || _batch) {
-        int64_t st[4] = {0, 0, 0, 0};
+        int64_t st[LPM_TREE_NSTATS] = {};
         _gpu->check(lpm_tree_stats(_gpu->get(), st), "TreeSum::meshSolve");
         mac = st[LPM_STAT_FAR];
     }

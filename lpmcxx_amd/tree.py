@@ -87,14 +87,17 @@ def last_stats(ctx: Context):
 
 
 LPM_STAT_FAR, LPM_STAT_NEAR, LPM_STAT_TERMS, LPM_STAT_MODEL = 0, 1, 2, 3
+LPM_STAT_FAR_HIGH, LPM_STAT_TERMS_HIGH, LPM_TREE_NSTATS = 4, 5, 6
 
 
 def stats(ctx: Context) -> dict:
     """Counters of the last evaluation: far pairs, near pairs, far-field series
-    terms, order-model calls."""
-    out = np.zeros(4, np.int64)
+    terms, order-model calls, and (learned evaluation) the far pairs and terms
+    handled by the order 8..12 list kernel."""
+    out = np.zeros(LPM_TREE_NSTATS, np.int64)
     ctx.check(ctx.lib.lpm_tree_stats(ctx.h, _ptr(out)))
-    return {"far": int(out[0]), "near": int(out[1]), "terms": int(out[2]), "model": int(out[3])}
+    return {"far": int(out[0]), "near": int(out[1]), "terms": int(out[2]), "model": int(out[3]),
+            "far_high": int(out[4]), "terms_high": int(out[5])}
 
 
 def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float = 0.0, prec: int = LPM_F64):
