@@ -115,6 +115,63 @@ __device__ __forceinline__ void far_cartesian(T d0, T d1, T d2, T delta2x2, cons
     cart_all<P>(std::make_integer_sequence<int, NT - 1>{}, b, T(2) * d0, T(2) * d1, T(2) * d2, iR2, mm, S0, S1, S2);
 }
 
+// FP32 far field with node-scaled coefficients and moments.  With rho the
+// node's box radius, the dimensionless coefficients c_k = b_k rho^|k| R2
+// satisfy c_0 = 2 and c_k = sum_i a_i c_{k-e_i} - q sum_i c_{k-2e_i} with
+// a_i = 2 d_i rho / R2 and q = rho^2 / R2 (no multiply per term), and
+// S = (1 / R2) sum_k c_k mh_k with the scaled moments mh_k = m_k / rho^|k|.
+// Every quantity stays O(1) whatever the node size, so FP32 neither
+// overflows nor underflows at high order.
+template <int P, int I>
+__device__ __forceinline__ void cart_step_scaled(float* c, float a0, float a1, float a2, float q,
+                                                 const float* __restrict__ mm, float& S0, float& S1, float& S2) {
+    constexpr int i0 = kCartTab<P>.p1[I][0], i1 = kCartTab<P>.p1[I][1], i2 = kCartTab<P>.p1[I][2];
+    constexpr int j0 = kCartTab<P>.p2[I][0], j1 = kCartTab<P>.p2[I][1], j2 = kCartTab<P>.p2[I][2];
+    float s = 0.f;
+    if constexpr (i0 >= 0) s = fmadd(a0, c[i0], s);
+    if constexpr (i1 >= 0) s = fmadd(a1, c[i1], s);
+    if constexpr (i2 >= 0) s = fmadd(a2, c[i2], s);
+    if constexpr (j0 >= 0 || j1 >= 0 || j2 >= 0) {
+        float t = 0.f;
+        if constexpr (j0 >= 0) t += c[j0];
+        if constexpr (j1 >= 0) t += c[j1];
+        if constexpr (j2 >= 0) t += c[j2];
+        s = fmadd(-q, t, s);
+    }
+    c[I] = s;
+    S0 = fmadd(s, mm[3 * I + 0], S0);
+    S1 = fmadd(s, mm[3 * I + 1], S1);
+    S2 = fmadd(s, mm[3 * I + 2], S2);
+}
+
+template <int P, int... I>
+__device__ __forceinline__ void cart_all_scaled(std::integer_sequence<int, I...>, float* c, float a0, float a1,
+                                                float a2, float q, const float* __restrict__ mm, float& S0,
+                                                float& S1, float& S2) {
+    (cart_step_scaled<P, I + 1>(c, a0, a1, a2, q, mm, S0, S1, S2), ...);
+}
+
+template <int P>
+__device__ __forceinline__ void far_cartesian_scaled(float d0, float d1, float d2, float delta2x2, float rho,
+                                                     const float* __restrict__ mm, float& S0, float& S1,
+                                                     float& S2) {
+    constexpr int NT = CartTab<P>::NT;
+    const float R2 = fmadd(d0, d0, fmadd(d1, d1, fmadd(d2, d2, delta2x2)));
+    const float iR2 = __builtin_amdgcn_rcpf(R2);
+    const float rs = rho * iR2;
+    const float q = rho * rs, rs2 = 2.f * rs;
+    float c[NT];
+    c[0] = 2.f;
+    S0 = 2.f * mm[0];
+    S1 = 2.f * mm[1];
+    S2 = 2.f * mm[2];
+    cart_all_scaled<P>(std::make_integer_sequence<int, NT - 1>{}, c, rs2 * d0, rs2 * d1, rs2 * d2, q, mm, S0, S1,
+                       S2);
+    S0 *= iR2;
+    S1 *= iR2;
+    S2 *= iR2;
+}
+
 // Same series in FP64, calling on_shell(n, S0, S1, S2) after every degree
 // shell n: S is then the sum over |k| <= n, the order-(n + 1) far field.
 template <int P, int I, class F>

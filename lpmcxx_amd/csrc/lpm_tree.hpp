@@ -50,7 +50,7 @@ struct Tree {
     int moment_sign = 1;
     DevBuf mom;           // double [nn][nterms][3], lex order of k (the reference's layout)
     DevBuf momg;          // double [nn][nterms][3], graded order of k (Cartesian series)
-    DevBuf mom32;         // float  [nn][nterms][3], graded order (FP32 far field)
+    DevBuf mom32;         // float  [nn][nterms][3], graded order scaled by rho^-|k|, rho = box radius (FP32 far field)
     DevBuf soff32;        // float4 [ns] FP32 source offsets from their leaf centroid + gamma
     DevBuf nstat;         // float4 [nn] |m| of the degree-0, 1, 2 moment shells, particle count
     // scratch

@@ -209,16 +209,23 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, dou
 }
 
 // graded FP64 and FP32 copies of the lex moments
+// Graded copies of the lex-order moments: FP64 as they are, and FP32 scaled
+// by rho^-|k| (rho = the node's box radius) for the FP32 far field.
 __global__ __launch_bounds__(256) void tree_moments_graded(int nn, int p, int nterm, const double* __restrict__ mom,
+                                                           const double4* __restrict__ ncent,
                                                            double* __restrict__ momg, float* __restrict__ mom32) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= int64_t(nn) * nterm * 3) return;
     const int node = int(i / (int64_t(nterm) * 3));
     const int rem = int(i - int64_t(node) * nterm * 3);
-    const int64_t g = int64_t(node) * nterm * 3 + 3 * lex_to_graded(p, rem / 3) + rem % 3;
+    const int gi = lex_to_graded(p, rem / 3);
+    const int64_t g = int64_t(node) * nterm * 3 + 3 * gi + rem % 3;
     const double s = mom[i];
     momg[g] = s;
-    mom32[g] = float(s);
+    const double irho = 1.0 / ncent[node].w;
+    double sc = 1.0;
+    for (int d = shell_of(gi); d > 0; --d) sc *= irho;
+    mom32[g] = float(s * sc);
 }
 
 // Norms of the degree-0, 1 and 2 moment shells (graded positions 0, 1..3,
@@ -495,7 +502,7 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
                            t->mom.as<double>());
     }
     hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm,
-                       t->mom.as<double>(), t->momg.as<double>(), t->mom32.as<float>());
+                       t->mom.as<double>(), t->ncent.as<double4>(), t->momg.as<double>(), t->mom32.as<float>());
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(nn)), dim3(256), 0, st, int(nn), p, nterm, t->momg.as<double>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),

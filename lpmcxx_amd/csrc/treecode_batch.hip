@@ -37,6 +37,9 @@ namespace lpm {
 namespace {
 
 constexpr int kStackB = 160;  // >= 7 * kMaxDepth + 1
+// the planning pass pops two nodes and pushes up to 16 children: at most 14
+// left behind per level
+constexpr int kStackPlan = 14 * kMaxDepth + 2 + 16;
 constexpr int kLabelOrder = 12;
 constexpr int kMaxLearnedOrder = 12;
 
@@ -113,6 +116,13 @@ __device__ __forceinline__ void far_switch(std::integer_sequence<int, Ps...>, in
     ((p == Ps + 1 ? (far_cartesian<Ps + 1>(d0, d1, d2, dd, mm, S0, S1, S2), 0) : 0), ...);
 }
 
+template <int... Ps>
+__device__ __forceinline__ void far_switch_scaled(std::integer_sequence<int, Ps...>, int p, float d0, float d1,
+                                                  float d2, float dd, float rho, const float* __restrict__ mm,
+                                                  float& S0, float& S1, float& S2) {
+    ((p == Ps + 1 ? (far_cartesian_scaled<Ps + 1>(d0, d1, d2, dd, rho, mm, S0, S1, S2), 0) : 0), ...);
+}
+
 // Far field of node n at every lane's target: t x S, added to (s0, s1, s2).
 template <int P, bool F32, bool RUNTIME_P>
 __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, double t1, double t2, const double4& c,
@@ -122,9 +132,11 @@ __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, dou
     if constexpr (F32) {
         float A0 = 0.f, A1 = 0.f, A2 = 0.f;
         const float e0 = float(t0 - c.x), e1 = float(t1 - c.y), e2 = float(t2 - c.z), dd = float(2.0 * d2);
+        const float rho = float(c.w);  // box radius: the moments' scale
         const float* mm = mom32 + int64_t(n) * nterm * 3;
-        if constexpr (RUNTIME_P) far_switch<float>(std::make_integer_sequence<int, P>{}, p, e0, e1, e2, dd, mm, A0, A1, A2);
-        else far_cartesian<P>(e0, e1, e2, dd, mm, A0, A1, A2);
+        if constexpr (RUNTIME_P)
+            far_switch_scaled(std::make_integer_sequence<int, P>{}, p, e0, e1, e2, dd, rho, mm, A0, A1, A2);
+        else far_cartesian_scaled<P>(e0, e1, e2, dd, rho, mm, A0, A1, A2);
         S0 = A0; S1 = A1; S2 = A2;
     } else {
         S0 = S1 = S2 = 0.0;
@@ -314,8 +326,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
                                                          Lists ls, unsigned long long* __restrict__ stats) {
     __shared__ __align__(16) unsigned char s_model[kModelLds];
-    __shared__ int s_node[4][kStackB];
-    __shared__ int s_cls[4][kStackB];
+    __shared__ int s_node[4][kStackPlan];
     mlp_stage_weights(wts, s_model);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -330,49 +341,44 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     int* near = ls.near + bi * ls.near_cap;
     int nfar = 0, nhi = 0, nnear = 0;
     unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0;
-    int sp = 1;
-    s_node[w][0] = 0;
-    s_cls[w][0] = 0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int sp = 0;
+    if (tv.nkids[0] == 0) {  // the root is a leaf
+        if (lane == 0 && ls.near_cap > 0) near[0] = 0;
+        nnear = 1;
+        near_pairs = unsigned(tv.nend[0] - tv.nbeg[0]);
+    } else {
+        s_node[w][0] = 0;
+        sp = 1;
+    }
+    // The stack holds internal nodes to descend into.  Up to two are popped at
+    // a time and all their children classified in one model call (rows 0..7:
+    // children of the first, rows 8..15: of the second); lane l < 16 then
+    // files its child: accepted -> far list (order p), leaf -> near list,
+    // else -> the stack.
     while (sp > 0) {
-        --sp;
-        const int n = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
-        const int cls = __builtin_amdgcn_readfirstlane(s_cls[w][sp]);
-        if (cls > 0) {
-            const int p = min(cls + margin, max_order);
-            if (p <= kLowOrder) {
-                if (lane == 0 && nfar + nhi < ls.far_cap) far[nfar] = n | (p << kNodeBits);
-                ++nfar;
-            } else {
-                if (lane == 0 && nfar + nhi < ls.far_cap) far[ls.far_cap - 1 - nhi] = n | (p << kNodeBits);
-                ++nhi;
-                nterms_hi += unsigned(p * (p + 1) * (p + 2) / 6);
-            }
-            nterms += unsigned(p * (p + 1) * (p + 2) / 6);
-            continue;
-        }
-        const int nk = tv.nkids[n];
-        if (nk == 0) {
-            if (lane == 0 && nnear < ls.near_cap) near[nnear] = n;
-            ++nnear;
-            near_pairs += unsigned(tv.nend[n] - tv.nbeg[n]);
-            continue;
-        }
-        // classify the children: lane l works on child f + (l % 16); the
+        const int na = __builtin_amdgcn_readfirstlane(s_node[w][sp - 1]);
+        const int nb2 = sp > 1 ? __builtin_amdgcn_readfirstlane(s_node[w][sp - 2]) : -1;
+        sp -= nb2 >= 0 ? 2 : 1;
+        const int nka = tv.nkids[na], fa = tv.nfirst[na];
+        const int nkb = nb2 >= 0 ? tv.nkids[nb2] : 0, fb = nb2 >= 0 ? tv.nfirst[nb2] : 0;
+        // lane l works on row r = l % 16 (child r % 8 of node r / 8); the
         // lanes of one row hold its inputs 4g..4g+3 (g = l / 16)
-        const int f = tv.nfirst[n];
         const int row = lane & 15, g = lane >> 4;
+        const int rk = row & 7;
+        const bool has = row < 8 ? rk < nka : rk < nkb;
+        const int chr = (row < 8 ? fa : fb) + rk;
         bool prop = false;
         float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
-        if (row < nk) {
-            const int ch = f + row;
-            const double4 cc = tv.ncent[ch];
+        if (has) {
+            const double4 cc = tv.ncent[chr];
             const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
             const double sd = sqrt(dx * dx + dy * dy + dz * dz);
-            const double rn = tv.nrad[ch];
+            const double rn = tv.nrad[chr];
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
                 float fe[kIn];
-                pair_features(fe, rn, bt.r, sd, tv.nstat[ch], tv.nkids[ch] == 0, bt.nvalid, u_ref);
+                pair_features(fe, rn, bt.r, sd, tv.nstat[chr], tv.nkids[chr] == 0, bt.nvalid, u_ref);
 #pragma unroll
                 for (int k = 0; k < kIn; ++k) fe[k] = (fe[k] - norm.mean[k]) * norm.inv_std[k];
                 x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
@@ -381,17 +387,47 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
                 x3 = g == 0 ? fe[3] : g == 1 ? fe[7] : g == 2 ? fe[11] : fe[15];
             }
         }
-        int my_cls = 0;
+        int p = 0;
         if (__ballot(prop)) {
             const int q = mlp_classify_regs(s_model, cvt_bf16x4(x0, x1, x2, x3));
             ++ncalls;
-            if (prop) my_cls = (q <= max_order) ? q : 0;
+            if (prop && q > 0 && q <= max_order) p = min(q + margin, max_order);
         }
-        if (lane < nk) {
-            s_node[w][sp + nk - 1 - lane] = f + lane;
-            s_cls[w][sp + nk - 1 - lane] = my_cls;
+        const bool mine = lane < 16 && has;
+        const int cb = mine ? tv.nbeg[chr] : 0, ce = mine ? tv.nend[chr] : 0;
+        const bool leaf = mine && tv.nkids[chr] == 0;
+        const bool is_lo = mine && p > 0 && p <= kLowOrder, is_hi = mine && p > kLowOrder;
+        const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
+        const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi);
+        const uint64_t m_near = __ballot(is_near), m_desc = __ballot(is_desc);
+        const int c_lo = __popcll(m_lo), c_hi = __popcll(m_hi), c_near = __popcll(m_near), c_desc = __popcll(m_desc);
+        // entries are written only while the batch's lists fit (else the
+        // batch is reported as overflowing at the end)
+        if (nfar + c_lo + nhi + c_hi <= ls.far_cap) {
+            const int e = chr | (p << kNodeBits);
+            if (is_lo) far[nfar + __popcll(m_lo & below)] = e;
+            if (is_hi) far[ls.far_cap - 1 - nhi - __popcll(m_hi & below)] = e;
         }
-        sp += nk;
+        if (nnear + c_near <= ls.near_cap && is_near) near[nnear + __popcll(m_near & below)] = chr;
+        if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = chr;
+        nfar += c_lo;
+        nhi += c_hi;
+        nnear += c_near;
+        sp += c_desc;
+        // terms of the accepted children and sources of the near leaves
+        // (lanes 0..15: an xor butterfly over 16 lanes sums them)
+        unsigned t_all = (is_lo || is_hi) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
+        unsigned t_hi = is_hi ? t_all : 0u;
+        unsigned np = is_near ? unsigned(ce - cb) : 0u;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            t_all += __shfl_xor(t_all, o, 64);
+            t_hi += __shfl_xor(t_hi, o, 64);
+            np += __shfl_xor(np, o, 64);
+        }
+        nterms += __builtin_amdgcn_readfirstlane(t_all);
+        nterms_hi += __builtin_amdgcn_readfirstlane(t_hi);
+        near_pairs += __builtin_amdgcn_readfirstlane(np);
         wave_sync();
     }
     if (lane == 0) {

@@ -240,6 +240,26 @@ def test_learned_constant_model_reduces_to_plain(ctx, mesh):
     assert T.stats(ctx)["far"] == 0 and O.rel_max(np.stack(u9), np.stack(u0)) < 1e-14
 
 
+@pytest.mark.parametrize("order", [6, 8, 10, 12])
+def test_learned_constant_model_all_list_kernels(ctx, mesh, order):
+    """Orders 8..12 go through the second list kernel: a constant model of
+    order p must reproduce the fixed-order batch treecode in both precisions."""
+    m, src, g, tgt = _field(mesh, 5, 64, 12)
+    _gpu_tree(ctx, m, src, g, 64, 12)
+    u_ref = LN.u_ref_of(g)
+    for prec in (L.LPM_F64, L.LPM_F32C):
+        LN.constant_model(ctx, order)
+        ul = LN.evaluate(ctx, tgt, 0.9, 0, order, u_ref, prec=prec)
+        sl = T.stats(ctx)
+        ub = T.evaluate_batch(ctx, tgt, 0.9, order, prec=prec)
+        sb = T.stats(ctx)
+        d = O.rel_max(np.stack(ul), np.stack(ub))
+        print(f"\norder {order} prec {prec}: learned(const) vs batch rel max {d:.3e}, far {sl['far']} high {sl['far_high']}")
+        assert (sl["far"], sl["near"], sl["terms"]) == (sb["far"], sb["near"], sb["terms"])
+        assert sl["far_high"] == (sl["far"] if order > 7 else 0)
+        assert d < (1e-13 if prec == L.LPM_F64 else 1e-6)
+
+
 def test_learned_errors(ctx, mesh):
     m, src, g, tgt = _field(mesh, 3, 32, 4)
     _gpu_tree(ctx, m, src, g, 32, 4)
