@@ -162,6 +162,20 @@ def rk4_level10(ctx, args):
             "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy}
 
 
+def measured_traffic(kernel: str, args) -> tuple:
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    the default level-9 learned step (FETCH_SIZE / WRITE_SIZE passes scaled by
+    a 256 MiB copy); None for other workloads."""
+    defaults = (args.level, args.leaf, args.theta_max, args.max_order, args.learned_cell) == (9, 128, 0.85, 7, 1024)
+    f = ROOT / "profiles" / "tree_eval_lists_l9_traffic.json"
+    if not defaults or not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    if d.get("kernel") != kernel:
+        return None, None
+    return d["traffic_bytes_per_launch"], "profiles/tree_eval_lists_l9_traffic.md"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,9 +365,11 @@ def main() -> int:
                    "launch_ms": ms32}
     direct_roof["frac"] = direct_roof["achieved"] / FP32_PEAK
     direct_roof["frac_nonpacked"] = direct_roof["achieved"] / FP32_NONPACKED
-    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists<7, true, true> (near leaves + far p<=7)"),
+    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists<7, true, true>"),
             "batch": tree_roof(batch_stats, batch_eval_ms, "tree_eval_batch"),
             "direct_f32c": direct_roof}[step_name]
+    if step_name == "learned":
+        roof["traffic"], roof["traffic_source"] = measured_traffic(roof["kernel"], args)
 
     # --- RK4 steps/s (4 treecode evaluations per step), learned and plain FP64
     rk4 = None

@@ -8,7 +8,11 @@
  * and Coords containers expose it (src/LpmField.h:34-51 getPtrToData).
  * Entry points without the _dev suffix take HOST pointers, copy to device,
  * run, copy back and synchronize.  The _dev variants take DEVICE pointers,
- * enqueue on the context's stream and return without synchronizing.
+ * enqueue on the context's stream and return without synchronizing, with
+ * two exceptions: lpm_tree_build_dev waits for the device to size the node
+ * table (once per 4 tree levels; it returns the node count and depth), and
+ * lpm_tree_eval_learned_dev waits once at its end to check that every
+ * batch's interaction lists fit (they are grown and planned again if not).
  *
  * A context owns one HIP stream, one growable device workspace and at most
  * one source octree.  Use one context per host thread.
