@@ -302,6 +302,20 @@ int lpm_rk4_step_dev(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, doub
 int lpm_rk4_steps(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n,
                   double* x, double* y, double* z, const double* zeta_abs, const double* area);
 
+/* Barotropic vorticity step of a meshed particle set: the RK4 step above for
+ * n_src vortex particles (mesh faces: zeta_abs[n_src], area[n_src]) followed
+ * in (x, y, z) by n_tracer passive particles (mesh vertices) that move with
+ * the velocity the n_src particles induce and carry no circulation.  Their
+ * absolute vorticity is conserved along trajectories like the faces'.
+ * No counterpart in the reference C++; its Fortran drivers advance faces and
+ * vertices of a SphereBVE mesh with a SphereBVESolver (README.md:68-81).
+ * Device pointers; enqueued on the context stream. */
+int lpm_bve_step_dev(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int64_t n_src,
+                     int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area);
+/* nsteps BVE steps on host arrays (copied in once, out once). */
+int lpm_bve_steps(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n_src,
+                  int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area);
+
 #ifdef __cplusplus
 }
 #endif

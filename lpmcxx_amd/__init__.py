@@ -140,6 +140,9 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_solver_config_default": [_vp],
         "lpm_rk4_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, _vp, _vp, _vp, _vp, _vp],
         "lpm_rk4_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, _vp, _vp, _vp, _vp, _vp],
+        "lpm_bve_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp],
+        "lpm_bve_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp,
+                          _vp],
     }
     for name, args in sigs.items():
         if hasattr(lib, name):
@@ -311,8 +314,34 @@ def _rk4_step_dev(self, cfg, dt, omega, x, y, z, zeta_abs, area):
                                          _ptr(zeta_abs), _ptr(area)))
 
 
+def _bve_steps(self, cfg, dt, omega, nsteps, src, zeta_abs, area, tracers=None):
+    """nsteps barotropic-vorticity RK4 steps of vortex particles src = (x, y, z)
+    (with zeta_abs, area) and passive tracers = (x, y, z) on host arrays;
+    returns the new (src, tracers) positions."""
+    nt = 0 if tracers is None else len(tracers[0])
+    ns = len(src[0])
+    if nt:
+        x, y, z = (np.concatenate([np.asarray(a, np.float64), np.asarray(b, np.float64)]) for a, b in zip(src, tracers))
+    else:
+        x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in src)
+    za = np.ascontiguousarray(zeta_abs, dtype=np.float64)
+    ar = np.ascontiguousarray(area, dtype=np.float64)
+    self.check(self.lib.lpm_bve_steps(self.h, C.byref(cfg), dt, omega, nsteps, ns, nt, _ptr(x), _ptr(y), _ptr(z),
+                                      _ptr(za), _ptr(ar)))
+    return (x[:ns], y[:ns], z[:ns]), ((x[ns:], y[ns:], z[ns:]) if nt else None)
+
+
+def _bve_step_dev(self, cfg, dt, omega, n_src, x, y, z, zeta_abs, area):
+    """One BVE step on device tensors (first n_src entries are vortex particles,
+    the rest tracers), in place, enqueued on the context stream."""
+    self.check(self.lib.lpm_bve_step_dev(self.h, C.byref(cfg), dt, omega, n_src, x.numel() - n_src, _ptr(x), _ptr(y),
+                                         _ptr(z), _ptr(zeta_abs), _ptr(area)))
+
+
 Context.rk4_steps = _rk4_steps
 Context.rk4_step_dev = _rk4_step_dev
+Context.bve_steps = _bve_steps
+Context.bve_step_dev = _bve_step_dev
 
 
 def native_libraries() -> list[str]:

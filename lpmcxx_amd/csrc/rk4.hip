@@ -58,13 +58,14 @@ __global__ __launch_bounds__(256) void rk4_final(int64_t n, double* __restrict__
 
 }  // namespace
 
-// velocity of the particles at (x, y, z) with circulation gamma, into (ux, uy, uz)
-static int velocity(lpm_ctx* c, const lpm_solver_config& cfg, int64_t n, const double* x, const double* y,
-                    const double* z, const double* gamma, double* ux, double* uy, double* uz) {
+// velocity at all n particles (x, y, z) induced by the first ns of them with
+// circulation gamma[ns], into (ux, uy, uz)
+static int velocity(lpm_ctx* c, const lpm_solver_config& cfg, int64_t n, int64_t ns, const double* x,
+                    const double* y, const double* z, const double* gamma, double* ux, double* uy, double* uz) {
     if (cfg.method == LPM_METHOD_DIRECT)
-        return direct_sum_device(c, LPM_K_BIOT_SAVART_SPHERE, cfg.prec, 1.0, cfg.delta, x, y, z, n, x, y, z, gamma, n,
+        return direct_sum_device(c, LPM_K_BIOT_SAVART_SPHERE, cfg.prec, 1.0, cfg.delta, x, y, z, n, x, y, z, gamma, ns,
                                  nullptr, ux, uy, uz, nullptr);
-    LPM_CHECK(tree_build_device(c, x, y, z, n, cfg.root_box_min, cfg.root_box_max, LPM_TREE_MAX_PER_NODE,
+    LPM_CHECK(tree_build_device(c, x, y, z, ns, cfg.root_box_min, cfg.root_box_max, LPM_TREE_MAX_PER_NODE,
                                 cfg.leaf_max));
     LPM_CHECK(tree_moments_device(c, gamma, cfg.order, 1));
     if (cfg.method == LPM_METHOD_TREE_BATCH)
@@ -75,8 +76,12 @@ static int velocity(lpm_ctx* c, const lpm_solver_config& cfg, int64_t n, const d
     return tree_eval_device(c, cfg.prec, cfg.series, cfg.theta, cfg.delta, x, y, z, n, ux, uy, uz, nullptr, 1.0);
 }
 
-int rk4_step_device(lpm_ctx* c, const lpm_solver_config& cfg, double dt, double omega, int64_t n, double* x,
-                    double* y, double* z, const double* zeta_abs, const double* area) {
+// One RK4 step of n = ns + nt particles: the first ns carry circulation
+// (zeta_abs[ns], area[ns]), the last nt are passive tracers moved by the same
+// velocity field.
+int rk4_step_device(lpm_ctx* c, const lpm_solver_config& cfg, double dt, double omega, int64_t ns, int64_t nt,
+                    double* x, double* y, double* z, const double* zeta_abs, const double* area) {
+    const int64_t n = ns + nt;
     if (n == 0) return LPM_OK;
     hipStream_t st = c->stream;
     LPM_CHECK(c->rk_work.reserve(size_t(n) * 13 * sizeof(double)));
@@ -91,8 +96,9 @@ int rk4_step_device(lpm_ctx* c, const lpm_solver_config& cfg, double dt, double 
     const double* py = y;
     const double* pz = z;
     for (int s = 0; s < 4; ++s) {
-        hipLaunchKernelGGL(rk4_gamma, dim3(blocks(n)), dim3(256), 0, st, n, pz, zeta_abs, area, 2.0 * omega, g);
-        LPM_CHECK(velocity(c, cfg, n, px, py, pz, g, kx, ky, kz));
+        if (ns > 0)
+            hipLaunchKernelGGL(rk4_gamma, dim3(blocks(ns)), dim3(256), 0, st, ns, pz, zeta_abs, area, 2.0 * omega, g);
+        LPM_CHECK(velocity(c, cfg, n, ns, px, py, pz, g, kx, ky, kz));
         hipLaunchKernelGGL(rk4_stage, dim3(blocks(n)), dim3(256), 0, st, n, x, y, z, kx, ky, kz, h[s], wt[s],
                            int(s == 0), ax, ay, az, s < 3 ? sx : nullptr, sy, sz);
         px = sx; py = sy; pz = sz;
@@ -108,12 +114,13 @@ using namespace lpm;
 
 extern "C" {
 
-static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_t n, const double* x,
+static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_t ns, int64_t nt, const double* x,
                      const double* y, const double* z, const double* zeta_abs, const double* area) {
     if (!c || !cfg) return LPM_ERR_NULL_PTR;
     c->err.clear();
-    if (n < 0 || n > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid particle count");
-    if (n > 0 && (!x || !y || !z || !zeta_abs || !area)) return fail(c, LPM_ERR_NULL_PTR, "NULL particle array");
+    if (ns < 0 || nt < 0 || ns + nt > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid particle count");
+    if (ns + nt > 0 && (!x || !y || !z)) return fail(c, LPM_ERR_NULL_PTR, "NULL particle array");
+    if (ns > 0 && (!zeta_abs || !area)) return fail(c, LPM_ERR_NULL_PTR, "NULL vorticity or area array");
     if (!std::isfinite(dt)) return fail(c, LPM_ERR_INVALID_ARG, "invalid dt");
     if (cfg->method < LPM_METHOD_DIRECT || cfg->method > LPM_METHOD_TREE_LEARNED)
         return fail(c, LPM_ERR_INVALID_ARG, "invalid velocity method");
@@ -132,6 +139,32 @@ static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_
             if (!(cfg->root_box_min[d] < cfg->root_box_max[d]))
                 return fail(c, LPM_ERR_INVALID_ARG, "tree root box must have min < max");
     }
+    return LPM_OK;
+}
+
+static int steps_host(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t ns,
+                      int64_t nt, double* x, double* y, double* z, const double* zeta_abs, const double* area) {
+    if (nsteps < 0) return fail(c, LPM_ERR_INVALID_ARG, "nsteps must be >= 0");
+    LPM_HIP(c, hipSetDevice(c->device));
+    const int64_t n = ns + nt;
+    if (n == 0) return LPM_OK;
+    LPM_CHECK(c->h_in.reserve(size_t(3 * n + 2 * std::max<int64_t>(ns, 1)) * sizeof(double)));
+    double* d = c->h_in.as<double>();
+    double *dz = d + 3 * n, *da = dz + std::max<int64_t>(ns, 1);
+    hipStream_t st = c->stream;
+    const size_t b = size_t(n) * sizeof(double), bs = size_t(ns) * sizeof(double);
+    LPM_HIP(c, hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + n, y, b, hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + 2 * n, z, b, hipMemcpyHostToDevice, st));
+    if (ns > 0) {
+        LPM_HIP(c, hipMemcpyAsync(dz, zeta_abs, bs, hipMemcpyHostToDevice, st));
+        LPM_HIP(c, hipMemcpyAsync(da, area, bs, hipMemcpyHostToDevice, st));
+    }
+    for (int s = 0; s < nsteps; ++s) LPM_CHECK(rk4_step_device(c, *cfg, dt, omega, ns, nt, d, d + n, d + 2 * n, dz, da));
+    LPM_HIP(c, hipMemcpyAsync(x, d, b, hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(y, d + n, b, hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipMemcpyAsync(z, d + 2 * n, b, hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
     return LPM_OK;
 }
 
@@ -154,33 +187,28 @@ void lpm_solver_config_default(lpm_solver_config* cfg) {
 
 int lpm_rk4_step_dev(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int64_t n, double* x,
                      double* y, double* z, const double* zeta_abs, const double* area) {
-    LPM_CHECK(check_rk4(c, cfg, dt, n, x, y, z, zeta_abs, area));
+    LPM_CHECK(check_rk4(c, cfg, dt, n, 0, x, y, z, zeta_abs, area));
     LPM_HIP(c, hipSetDevice(c->device));
-    return rk4_step_device(c, *cfg, dt, omega, n, x, y, z, zeta_abs, area);
+    return rk4_step_device(c, *cfg, dt, omega, n, 0, x, y, z, zeta_abs, area);
 }
 
 int lpm_rk4_steps(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n,
                   double* x, double* y, double* z, const double* zeta_abs, const double* area) {
-    LPM_CHECK(check_rk4(c, cfg, dt, n, x, y, z, zeta_abs, area));
-    if (nsteps < 0) return fail(c, LPM_ERR_INVALID_ARG, "nsteps must be >= 0");
+    LPM_CHECK(check_rk4(c, cfg, dt, n, 0, x, y, z, zeta_abs, area));
+    return steps_host(c, cfg, dt, omega, nsteps, n, 0, x, y, z, zeta_abs, area);
+}
+
+int lpm_bve_step_dev(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int64_t n_src,
+                     int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area) {
+    LPM_CHECK(check_rk4(c, cfg, dt, n_src, n_tracer, x, y, z, zeta_abs, area));
     LPM_HIP(c, hipSetDevice(c->device));
-    if (n == 0) return LPM_OK;
-    LPM_CHECK(c->h_in.reserve(size_t(n) * 5 * sizeof(double)));
-    double* d = c->h_in.as<double>();
-    hipStream_t st = c->stream;
-    const size_t b = size_t(n) * sizeof(double);
-    LPM_HIP(c, hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(d + n, y, b, hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(d + 2 * n, z, b, hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(d + 3 * n, zeta_abs, b, hipMemcpyHostToDevice, st));
-    LPM_HIP(c, hipMemcpyAsync(d + 4 * n, area, b, hipMemcpyHostToDevice, st));
-    for (int s = 0; s < nsteps; ++s)
-        LPM_CHECK(rk4_step_device(c, *cfg, dt, omega, n, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n));
-    LPM_HIP(c, hipMemcpyAsync(x, d, b, hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipMemcpyAsync(y, d + n, b, hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipMemcpyAsync(z, d + 2 * n, b, hipMemcpyDeviceToHost, st));
-    LPM_HIP(c, hipStreamSynchronize(st));
-    return LPM_OK;
+    return rk4_step_device(c, *cfg, dt, omega, n_src, n_tracer, x, y, z, zeta_abs, area);
+}
+
+int lpm_bve_steps(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n_src,
+                  int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area) {
+    LPM_CHECK(check_rk4(c, cfg, dt, n_src, n_tracer, x, y, z, zeta_abs, area));
+    return steps_host(c, cfg, dt, omega, nsteps, n_src, n_tracer, x, y, z, zeta_abs, area);
 }
 
 }  // extern "C"

@@ -204,6 +204,11 @@ class MeshedParticles {
     void leafFaceArrays(std::vector<scalar_type>& cx, std::vector<scalar_type>& cy, std::vector<scalar_type>& cz,
                         std::vector<scalar_type>& area, std::vector<index_type>& faceIndex) const;
 
+    // Legacy VTK polydata of the mesh with every vertex field as point data
+    // and every face field as cell data (src/LpmMeshedParticles.cpp:131-153;
+    // writer in LpmVtk.cpp).
+    void writeToVtkFile(const std::string& fname, const std::string& desc = "") const;
+
   private:
     std::shared_ptr<PolyMesh2d> mesh2d;
     std::map<std::string, std::shared_ptr<Field>> _vertexFields, _faceFields;

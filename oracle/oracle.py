@@ -258,6 +258,29 @@ def pair_errors(sources, gamma, moments, p, targets, batch, node, u_ref, delta=0
     return out
 
 
+def bve_steps(src, zeta_abs, area, tracers, dt, omega, nsteps):
+    """The RK4 of rk4_steps for vortex particles src = (x, y, z) plus passive
+    tracers moved by the velocity the vortex particles induce (oracle direct
+    sum, FP64).  Returns (src, tracers) positions."""
+    ns = len(src[0])
+    P = np.concatenate([np.stack(src), np.stack(tracers)], axis=1).astype(np.float64)
+
+    def vel(p):
+        g = (zeta_abs - 2.0 * omega * p[2, :ns]) * area
+        return np.stack(bs_direct((p[0], p[1], p[2]), (p[0, :ns], p[1, :ns], p[2, :ns]), g))
+
+    def nrm(p):
+        return p / np.sqrt((p * p).sum(0))
+
+    for _ in range(nsteps):
+        k1 = vel(P)
+        k2 = vel(nrm(P + 0.5 * dt * k1))
+        k3 = vel(nrm(P + 0.5 * dt * k2))
+        k4 = vel(nrm(P + dt * k3))
+        P = nrm(P + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4))
+    return tuple(P[:, :ns]), tuple(P[:, ns:])
+
+
 def rk4_steps(x, y, z, zeta_abs, area, dt, omega, nsteps):
     """FP64 RK4 of the Lagrangian vorticity equation with the oracle direct sum:
     gamma = (zeta_abs - 2 omega z) area per stage, stage and final positions

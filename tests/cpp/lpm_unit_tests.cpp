@@ -4,6 +4,9 @@
 // tests/LpmMPIReplicatedDataUnitTests.cpp and tests/LpmTaylorSeries3dUnitTests.cpp
 // of the reference, with assertions.  Exit status = number of failures.
 #include "LpmSolvers.hpp"
+#include "LpmVtk.hpp"
+
+#include <fstream>
 
 #include <cstdio>
 #include <iostream>
@@ -21,7 +24,7 @@ static int failures = 0;
 
 static double relerr(double a, double b) { return std::abs(a - b) / std::abs(b); }
 
-int main() {
+int main(int argc, char** argv) {
     // MultiIndex known answers (tests/LpmMultiIndexUnitTests.cpp:28-35)
     const MultiIndex k(4, 3, 1);
     CHECK(k.magnitude() == 8);
@@ -94,6 +97,20 @@ int main() {
     double area = 0.0;
     for (index_type i = 0; i < pm->nFaces(); ++i) area += pm->getFaceArea(i);
     CHECK(std::abs(area - 4 * PI) < 1e-12);
+
+    // VTK polydata and CSV output of the mesh (argv[1], argv[2])
+    pm->createVertexField("streamFn", "m^2/s", 1);
+    pm->initializeVertexFieldWithFunction("streamFn", &y54);
+    pm->createFaceField("velocity", "m/s", 3);
+    RossbyHaurwitz54Velocity rh;
+    pm->initializeFaceFieldWithFunction("velocity", &rh);
+    if (argc > 1) pm->writeToVtkFile(argv[1], "cubed sphere level 2");
+    if (argc > 2) {
+        std::ofstream csv(argv[2]);
+        const PolyMesh2d& m = *pm->meshPtr();
+        VtkWriter().writePointsAndFieldsToCSV(csv, m.vx.data(), m.vy.data(), m.vz.data(), m.nVertices(),
+                                              {pm->getVertexFieldPtr("streamFn")});
+    }
 
     std::printf("lpm_unit_tests: %d failure(s)\n", failures);
     return failures;
