@@ -193,6 +193,28 @@ int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
 #define LPM_TREE_NSTATS 6
 int lpm_tree_stats(lpm_ctx* ctx, int64_t* out);
 
+/* ------------------------------------------------------------------ streamfunction treecode
+ * psi(t) = -(1/4 pi) sum_j gamma_j log(1 - t.y_j) (SphereGreensFn,
+ * src/LpmScalarKernel.h:31-45) by treecode.  The reference declares the
+ * scalar tree sum (TreeSum::initScalarKernel, src/LpmTreeSum.h:55-56) but
+ * never implements it (computeScalarSum is commented out, :61-65); this is
+ * that sum.  Moments: mu_k = sum_j gamma_j (y_j - c)^k about every node's box
+ * centroid, |k| < p, on the current tree (moment sign +1; gamma also becomes
+ * the strength of the tree's near fields).  Evaluation: the batch acceptance
+ * of lpm_tree_eval_batch (maxRadius_n + r_b < theta |c_b - c_n| for batches
+ * of <= 64 Morton-sorted targets); far nodes by the Taylor series of
+ * log(|t - y|^2) of order `order` (<= the moments' order), near leaves
+ * summed directly; pairs at zero distance contribute nothing.  prec LPM_F64:
+ * FP64 with the reference pair formula in the near field; LPM_F32C: FP32 far
+ * field from node-scaled FP32 moments, FP32 log2 near field, FP64 per-target
+ * sums. */
+int lpm_tree_moments_psi(lpm_ctx* ctx, const double* gamma, int p);
+int lpm_tree_moments_psi_dev(lpm_ctx* ctx, const double* gamma, int p);
+int lpm_tree_eval_psi(lpm_ctx* ctx, int prec, int order, double theta, const double* tx, const double* ty,
+                      const double* tz, int64_t nt, double* psi);
+int lpm_tree_eval_psi_dev(lpm_ctx* ctx, int prec, int order, double theta, const double* tx, const double* ty,
+                          const double* tz, int64_t nt, double* psi);
+
 /* ------------------------------------------------------------------ batch treecode
  * The treecode velocity of lpm_tree_eval with the acceptance decided per
  * batch of 64 Morton-sorted targets: node n is accepted for the batch when

@@ -107,6 +107,10 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_tree_build": [_vp, _vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, _i64p, C.POINTER(C.c_int)],
         "lpm_tree_build_dev": [_vp, _vp, _vp, _vp, C.c_int64, C.c_double, C.c_int, _i64p, C.POINTER(C.c_int)],
         "lpm_tree_moments": [_vp, _vp, C.c_int, C.c_int],
+        "lpm_tree_moments_psi": [_vp, _vp, C.c_int],
+        "lpm_tree_moments_psi_dev": [_vp, _vp, C.c_int],
+        "lpm_tree_eval_psi": [_vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, C.c_int64, _vp],
+        "lpm_tree_eval_psi_dev": [_vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, C.c_int64, _vp],
         "lpm_tree_moments_dev": [_vp, _vp, C.c_int, C.c_int],
         "lpm_tree_eval": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int64, _vp, _vp,
                           _vp, _i64p],

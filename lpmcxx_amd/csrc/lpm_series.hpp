@@ -172,6 +172,141 @@ __device__ __forceinline__ void far_cartesian_scaled(float d0, float d1, float d
     S2 *= iR2;
 }
 
+// ---------------------------------------------------------------- log kernel
+// Streamfunction far field: log(|d - h|^2 + eps) = sum_k a_k h^k about the
+// node centre.  From grad_h log Q = (grad_h Q) / Q with Q = |d - h|^2 + eps:
+// k_i a_k = -2 d_i g_{k-e_i} + 2 g_{k-2e_i} for any i with k_i > 0, where g_k
+// are the coefficients of 1/Q (the b_k above are those of 2/Q, so
+// a_k = (-d_i b_{k-e_i} + b_{k-2e_i}) / k_i), and a_0 = log R2.  The first
+// non-zero component of k is used.  Terms |k| < P need b up to degree P - 2.
+template <int P>
+struct LogTab {
+    static constexpr int NT = P * (P + 1) * (P + 2) / 6;
+    int first[NT] = {};  // first non-zero component of k (-1 for k = 0)
+    int kf[NT] = {};     // its value
+    constexpr LogTab() {
+        int pos = 0;
+        for (int n = 0; n < P; ++n)
+            for (int k1 = n; k1 >= 0; --k1)
+                for (int k2 = n - k1; k2 >= 0; --k2) {
+                    const int k3 = n - k1 - k2;
+                    first[pos] = k1 > 0 ? 0 : (k2 > 0 ? 1 : (k3 > 0 ? 2 : -1));
+                    kf[pos] = k1 > 0 ? k1 : (k2 > 0 ? k2 : k3);
+                    ++pos;
+                }
+    }
+};
+template <int P>
+inline constexpr LogTab<P> kLogTab{};
+
+// coefficient recurrence only (b_I from lower degrees), FP64 form of cart_step
+template <int P, int I, typename T>
+__device__ __forceinline__ void coef_step(T* b, T e0, T e1, T e2, T iR2) {
+    constexpr int a0 = kCartTab<P>.p1[I][0], a1 = kCartTab<P>.p1[I][1], a2 = kCartTab<P>.p1[I][2];
+    constexpr int c0 = kCartTab<P>.p2[I][0], c1 = kCartTab<P>.p2[I][1], c2 = kCartTab<P>.p2[I][2];
+    T s = T(0);
+    if constexpr (a0 >= 0) s = fmadd(e0, b[a0], s);
+    if constexpr (a1 >= 0) s = fmadd(e1, b[a1], s);
+    if constexpr (a2 >= 0) s = fmadd(e2, b[a2], s);
+    if constexpr (c0 >= 0) s -= b[c0];
+    if constexpr (c1 >= 0) s -= b[c1];
+    if constexpr (c2 >= 0) s -= b[c2];
+    b[I] = s * iR2;
+}
+
+// term I of the log series: a_I mu_I added to S (FP64, unscaled moments)
+template <int P, int I>
+__device__ __forceinline__ void log_term(double* b, double e0, double e1, double e2, double iR2, double d0, double d1,
+                                         double d2, const double* __restrict__ mu, double& S) {
+    constexpr int NTb = CartTab<P - 1>::NT;  // coefficients of degree <= P - 2
+    if constexpr (I < NTb) coef_step<P, I>(b, e0, e1, e2, iR2);
+    constexpr int i = kLogTab<P>.first[I];
+    constexpr int j1 = kCartTab<P>.p1[I][i], j2 = kCartTab<P>.p2[I][i];
+    constexpr double inv = 1.0 / double(kLogTab<P>.kf[I]);
+    const double di = i == 0 ? d0 : (i == 1 ? d1 : d2);
+    double a = -di * b[j1];
+    if constexpr (j2 >= 0) a += b[j2];
+    S = fma(a * inv, mu[I], S);
+}
+
+template <int P, int... I>
+__device__ __forceinline__ void log_all(std::integer_sequence<int, I...>, double* b, double e0, double e1, double e2,
+                                        double iR2, double d0, double d1, double d2, const double* __restrict__ mu,
+                                        double& S) {
+    (log_term<P, I + 1>(b, e0, e1, e2, iR2, d0, d1, d2, mu, S), ...);
+}
+
+// sum_{|k| < P} a_k mu_k for the target offset d = t - c (FP64)
+template <int P>
+__device__ __forceinline__ double far_log(double d0, double d1, double d2, double eps, const double* __restrict__ mu) {
+    constexpr int NT = CartTab<P>::NT;
+    constexpr int NTb = CartTab<P - 1>::NT > 0 ? CartTab<P - 1>::NT : 1;
+    const double R2 = d0 * d0 + d1 * d1 + d2 * d2 + eps;
+    const double iR2 = 1.0 / R2;
+    double b[NTb];
+    b[0] = 2.0 * iR2;
+    double S = log(R2) * mu[0];
+    if constexpr (NT > 1)
+        log_all<P>(std::make_integer_sequence<int, NT - 1>{}, b, 2.0 * d0, 2.0 * d1, 2.0 * d2, iR2, d0, d1, d2, mu, S);
+    return S;
+}
+
+// FP32 with node-scaled coefficients c_k = b_k rho^|k| R2 and moments
+// mu_k / rho^|k|: a_k rho^|k| = (-h_i c_{k-e_i} + q c_{k-2e_i}) / k_i with
+// h_i = d_i rho / R2 and q = rho^2 / R2.
+template <int P, int I>
+__device__ __forceinline__ void log_term_scaled(float* c, float a0, float a1, float a2, float q, float h0, float h1,
+                                                float h2, const float* __restrict__ mu, float& S) {
+    constexpr int NTb = CartTab<P - 1>::NT;
+    if constexpr (I < NTb) {
+        constexpr int i0 = kCartTab<P>.p1[I][0], i1 = kCartTab<P>.p1[I][1], i2 = kCartTab<P>.p1[I][2];
+        constexpr int k0 = kCartTab<P>.p2[I][0], k1 = kCartTab<P>.p2[I][1], k2 = kCartTab<P>.p2[I][2];
+        float s = 0.f;
+        if constexpr (i0 >= 0) s = fmadd(a0, c[i0], s);
+        if constexpr (i1 >= 0) s = fmadd(a1, c[i1], s);
+        if constexpr (i2 >= 0) s = fmadd(a2, c[i2], s);
+        if constexpr (k0 >= 0 || k1 >= 0 || k2 >= 0) {
+            float t = 0.f;
+            if constexpr (k0 >= 0) t += c[k0];
+            if constexpr (k1 >= 0) t += c[k1];
+            if constexpr (k2 >= 0) t += c[k2];
+            s = fmadd(-q, t, s);
+        }
+        c[I] = s;
+    }
+    constexpr int i = kLogTab<P>.first[I];
+    constexpr int j1 = kCartTab<P>.p1[I][i], j2 = kCartTab<P>.p2[I][i];
+    constexpr float inv = 1.f / float(kLogTab<P>.kf[I]);
+    const float hi = i == 0 ? h0 : (i == 1 ? h1 : h2);
+    float a = -hi * c[j1];
+    if constexpr (j2 >= 0) a = fmadd(q, c[j2], a);
+    S = fmadd(a * inv, mu[I], S);
+}
+
+template <int P, int... I>
+__device__ __forceinline__ void log_all_scaled(std::integer_sequence<int, I...>, float* c, float a0, float a1,
+                                               float a2, float q, float h0, float h1, float h2,
+                                               const float* __restrict__ mu, float& S) {
+    (log_term_scaled<P, I + 1>(c, a0, a1, a2, q, h0, h1, h2, mu, S), ...);
+}
+
+template <int P>
+__device__ __forceinline__ float far_log_scaled(float d0, float d1, float d2, float eps, float rho,
+                                                const float* __restrict__ mu) {
+    constexpr int NT = CartTab<P>::NT;
+    constexpr int NTb = CartTab<P - 1>::NT > 0 ? CartTab<P - 1>::NT : 1;
+    const float R2 = fmadd(d0, d0, fmadd(d1, d1, fmadd(d2, d2, eps)));
+    const float iR2 = __builtin_amdgcn_rcpf(R2);
+    const float rs = rho * iR2, q = rho * rs;
+    float c[NTb];
+    c[0] = 2.f;
+    float S = 0.69314718055994531f * __builtin_amdgcn_logf(R2) * mu[0];
+    if constexpr (NT > 1)
+        log_all_scaled<P>(std::make_integer_sequence<int, NT - 1>{}, c, 2.f * rs * d0, 2.f * rs * d1, 2.f * rs * d2,
+                          q, rs * d0, rs * d1, rs * d2, mu, S);
+    return S;
+}
+
 // Same series in FP64, calling on_shell(n, S0, S1, S2) after every degree
 // shell n: S is then the sum over |k| <= n, the order-(n + 1) far field.
 template <int P, int I, class F>

@@ -53,6 +53,9 @@ struct Tree {
     DevBuf mom32;         // float  [nn][nterms][3], graded order scaled by rho^-|k|, rho = box radius (FP32 far field)
     DevBuf soff32;        // float4 [ns] FP32 source offsets from their leaf centroid + gamma
     DevBuf nstat;         // float4 [nn] |m| of the degree-0, 1, 2 moment shells, particle count
+    // streamfunction moments mu_k = sum_j gamma_j h_j^k (one component): lex, graded, FP32 graded scaled
+    int psi_order = 0;    // p of the streamfunction moments, 0 = none
+    DevBuf pmom, pmomg, pmom32;
     // scratch
     DevBuf mpart;         // double [nch][nterms][3] leaf-chunk moment partials
     DevBuf tmp0, tmp1, tmp2, tmp3, sort_tmp, scan_tmp;

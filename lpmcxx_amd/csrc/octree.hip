@@ -337,8 +337,8 @@ void tree_free(Tree* t) {
     if (!t) return;
     for (DevBuf* b : {&t->perm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
                       &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32,
-                      &t->soff32, &t->nstat, &t->mpart, &t->tmp0, &t->tmp1, &t->tmp2, &t->tmp3, &t->sort_tmp, &t->scan_tmp,
-                      &t->lvt, &t->chunks})
+                      &t->soff32, &t->nstat, &t->mpart, &t->pmom, &t->pmomg, &t->pmom32, &t->tmp0, &t->tmp1,
+                      &t->tmp2, &t->tmp3, &t->sort_tmp, &t->scan_tmp, &t->lvt, &t->chunks})
         b->release();
     delete t;
 }
@@ -351,6 +351,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     t->ns = n;
     t->nn = 0;
     t->order = 0;
+    t->psi_order = 0;
     t->leaf_max = (control == 0) ? param : 0;
     for (int d = 0; d < 3; ++d) { t->root_box[d] = blo[d]; t->root_box[3 + d] = bhi[d]; }
     const int leaf_max = (control == 0) ? param : 1;

@@ -49,6 +49,9 @@ constexpr int kStack = 160;   // >= 7 * kMaxDepth + 1
 inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
 
 // ---------------------------------------------------------------- moments
+// Moments of NC components per multi-index: NC = 3, m_k = sum_j gamma_j y_j
+// h_j^k (velocity); NC = 1, m_k = sum_j gamma_j h_j^k (streamfunction).
+template <int NC>
 __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict__ chunk_node,
                                                           const int* __restrict__ chunk_beg,
                                                           const int* __restrict__ chunk_end,
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict_
                                                           double sign, double* __restrict__ part) {
     extern __shared__ double lds[];
     double* pw = lds;                       // [kChunk][3][p] powers
-    double* gy = lds + kChunk * 3 * p;      // [kChunk][3]   gamma * y
+    double* gy = lds + kChunk * 3 * p;      // [kChunk][NC]  gamma * y, or gamma
     const int ch = blockIdx.x;
     const int node = chunk_node[ch], b = chunk_beg[ch], e = chunk_end[ch];
     const int cnt = e - b;
@@ -75,9 +78,13 @@ __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict_
             w[a] = ax; w[p + a] = ay; w[2 * p + a] = az;
             ax *= hx; ay *= hy; az *= hz;
         }
-        gy[3 * q + 0] = g * yx;
-        gy[3 * q + 1] = g * yy;
-        gy[3 * q + 2] = g * yz;
+        if (NC == 3) {
+            gy[3 * q + 0] = g * yx;
+            gy[3 * q + 1] = g * yy;
+            gy[3 * q + 2] = g * yz;
+        } else {
+            gy[q] = g;
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
@@ -97,16 +104,16 @@ __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict_
             ++k2;
         }
         const int k3 = r;
-        double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        double m[NC] = {};
         for (int q = 0; q < cnt; ++q) {
             const double* w = pw + q * 3 * p;
             const double P = w[k1] * w[p + k2] * w[2 * p + k3];
-            m0 += gy[3 * q + 0] * P;
-            m1 += gy[3 * q + 1] * P;
-            m2 += gy[3 * q + 2] * P;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) m[cc] += gy[NC * q + cc] * P;
         }
-        double* o = part + (int64_t(ch) * nterm + t) * 3;
-        o[0] = m0; o[1] = m1; o[2] = m2;
+        double* o = part + (int64_t(ch) * nterm + t) * NC;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) o[cc] = m[cc];
     }
 }
 
@@ -118,16 +125,17 @@ __device__ inline int lex_to_graded(int p, int r) {
     return graded_index(a, b, r);
 }
 
-__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, const int* __restrict__ node_chunk0,
+__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, int ncomp,
+                                                           const int* __restrict__ node_chunk0,
                                                            const int* __restrict__ node_nchunk,
                                                            const double* __restrict__ part, double* __restrict__ mom) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= int64_t(nn) * nterm * 3) return;
-    const int node = int(i / (int64_t(nterm) * 3));
-    const int rem = int(i - int64_t(node) * nterm * 3);
+    if (i >= int64_t(nn) * nterm * ncomp) return;
+    const int node = int(i / (int64_t(nterm) * ncomp));
+    const int rem = int(i - int64_t(node) * nterm * ncomp);
     const int c0 = node_chunk0[node], nc = node_nchunk[node];
     double s = 0.0;
-    for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * 3 + rem];
+    for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * ncomp + rem];
     mom[i] = s;
 }
 
@@ -157,24 +165,25 @@ __device__ inline int lex_pos(int p, int a, int b, int c) {
 // applied as three one-dimensional binomial passes (x, then y, then z) on a
 // child's moments staged in LDS: O(p) work per term instead of O(p^3).
 // One block per node.
+template <int NC>
 __global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, double sign,
                                                 const int* __restrict__ nfirst, const int* __restrict__ nkids,
                                                 const double4* __restrict__ ncent, double* __restrict__ mom) {
     extern __shared__ double lds[];
-    double* A = lds;                  // [nterm][3]
-    double* B = lds + 3 * nterm;      // [nterm][3]
-    double* acc = lds + 6 * nterm;    // [nterm][3]
+    double* A = lds;                   // [nterm][NC]
+    double* B = lds + NC * nterm;      // [nterm][NC]
+    double* acc = lds + 2 * NC * nterm;  // [nterm][NC]
     const int i = first + blockIdx.x;
     const int nk = nkids[i];
     if (nk == 0) return;
     const double4 c = ncent[i];
-    for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) acc[q] = 0.0;
+    for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) acc[q] = 0.0;
     for (int ch = nfirst[i]; ch < nfirst[i] + nk; ++ch) {
         const double4 cc = ncent[ch];
         const double d[3] = {sign * (cc.x - c.x), sign * (cc.y - c.y), sign * (cc.z - c.z)};
-        const double* mc = mom + int64_t(ch) * nterm * 3;
+        const double* mc = mom + int64_t(ch) * nterm * NC;
         __syncthreads();
-        for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) A[q] = mc[q];
+        for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) A[q] = mc[q];
         __syncthreads();
         for (int axis = 0; axis < 3; ++axis) {
             const double* src = (axis == 1) ? B : A;
@@ -185,41 +194,42 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, dou
                 while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
                 const int k3 = r;
                 const int kk = axis == 0 ? k1 : (axis == 1 ? k2 : k3);
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, pw = 1.0;
+                double sm[NC] = {};
+                double pw = 1.0;
                 for (int a = kk; a >= 0; --a) {
                     const int j = axis == 0 ? lex_pos(p, a, k2, k3) : (axis == 1 ? lex_pos(p, k1, a, k3)
                                                                                  : lex_pos(p, k1, k2, a));
                     const double w = c_binom.v[kk][a] * pw;
-                    s0 = fma(w, src[3 * j], s0);
-                    s1 = fma(w, src[3 * j + 1], s1);
-                    s2 = fma(w, src[3 * j + 2], s2);
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) sm[q] = fma(w, src[NC * j + q], sm[q]);
                     pw *= d[axis];
                 }
-                if (axis < 2) {
-                    dst[3 * t] = s0; dst[3 * t + 1] = s1; dst[3 * t + 2] = s2;
-                } else {
-                    acc[3 * t] += s0; acc[3 * t + 1] += s1; acc[3 * t + 2] += s2;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    if (axis < 2) dst[NC * t + q] = sm[q];
+                    else acc[NC * t + q] += sm[q];
                 }
             }
             __syncthreads();
         }
     }
-    double* o = mom + int64_t(i) * nterm * 3;
-    for (int q = threadIdx.x; q < 3 * nterm; q += blockDim.x) o[q] = acc[q];
+    double* o = mom + int64_t(i) * nterm * NC;
+    for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) o[q] = acc[q];
 }
 
 // graded FP64 and FP32 copies of the lex moments
 // Graded copies of the lex-order moments: FP64 as they are, and FP32 scaled
 // by rho^-|k| (rho = the node's box radius) for the FP32 far field.
-__global__ __launch_bounds__(256) void tree_moments_graded(int nn, int p, int nterm, const double* __restrict__ mom,
+__global__ __launch_bounds__(256) void tree_moments_graded(int nn, int p, int nterm, int ncomp,
+                                                           const double* __restrict__ mom,
                                                            const double4* __restrict__ ncent,
                                                            double* __restrict__ momg, float* __restrict__ mom32) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= int64_t(nn) * nterm * 3) return;
-    const int node = int(i / (int64_t(nterm) * 3));
-    const int rem = int(i - int64_t(node) * nterm * 3);
-    const int gi = lex_to_graded(p, rem / 3);
-    const int64_t g = int64_t(node) * nterm * 3 + 3 * gi + rem % 3;
+    if (i >= int64_t(nn) * nterm * ncomp) return;
+    const int node = int(i / (int64_t(nterm) * ncomp));
+    const int rem = int(i - int64_t(node) * nterm * ncomp);
+    const int gi = lex_to_graded(p, rem / ncomp);
+    const int64_t g = int64_t(node) * nterm * ncomp + ncomp * gi + rem % ncomp;
     const double s = mom[i];
     momg[g] = s;
     const double irho = 1.0 / ncent[node].w;
@@ -464,7 +474,10 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
 
 }  // namespace
 
-int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
+// P2M over the build's leaf chunks, chunk reduction, upward pass level by
+// level, graded FP64 copy and FP32 copy scaled by rho^-|k|: NC components.
+template <int NC>
+static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, DevBuf& mom32) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     const int nterm = nterms(p);
@@ -476,41 +489,63 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     const int* d_ce = d_cb + nch;
     const int* d_n0 = d_ce + nch;
     const int* d_nc = d_n0 + nn;
-    LPM_CHECK(t->mpart.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * 3 * sizeof(double)));
-    LPM_CHECK(t->mom.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
-    LPM_CHECK(t->momg.reserve(size_t(nn) * nterm * 3 * sizeof(double)));
-    LPM_CHECK(t->mom32.reserve(size_t(nn) * nterm * 3 * sizeof(float)));
-    LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
-    LPM_CHECK(t->nstat.reserve(size_t(nn) * sizeof(float4)));
-    if (t->ns > 0) {
-        hipLaunchKernelGGL(tree_set_gamma, dim3(blocks(t->ns)), dim3(256), 0, st, gamma, t->perm.as<int>(), t->ns,
-                           t->spos.as<double4>());
-    }
+    LPM_CHECK(t->mpart.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * NC * sizeof(double)));
+    LPM_CHECK(mom.reserve(size_t(nn) * nterm * NC * sizeof(double)));
+    LPM_CHECK(momg.reserve(size_t(nn) * nterm * NC * sizeof(double)));
+    LPM_CHECK(mom32.reserve(size_t(nn) * nterm * NC * sizeof(float)));
     if (nch > 0) {
-        const size_t lds = size_t(kChunk) * (3 * p + 3) * sizeof(double);
-        hipLaunchKernelGGL(tree_moments_chunk, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
+        const size_t lds = size_t(kChunk) * (3 * p + NC) * sizeof(double);
+        hipLaunchKernelGGL(tree_moments_chunk<NC>, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
                            t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->mpart.as<double>());
     }
-    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), nterm, d_n0,
-                       d_nc, t->mpart.as<double>(), t->mom.as<double>());
+    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), nterm, NC, d_n0,
+                       d_nc, t->mpart.as<double>(), mom.as<double>());
     // upward pass, deepest internal level first
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(tree_m2m, dim3(unsigned(cnt)), dim3(128), size_t(9) * nterm * sizeof(double), st, int(f),
-                           p, nterm, double(sign), t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(),
-                           t->mom.as<double>());
+        hipLaunchKernelGGL(tree_m2m<NC>, dim3(unsigned(cnt)), dim3(128), size_t(3 * NC) * nterm * sizeof(double), st,
+                           int(f), p, nterm, double(sign), t->nfirst.as<int>(), t->nkids.as<int>(),
+                           t->ncent.as<double4>(), mom.as<double>());
     }
-    hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * 3)), dim3(256), 0, st, int(nn), p, nterm,
-                       t->mom.as<double>(), t->ncent.as<double4>(), t->momg.as<double>(), t->mom32.as<float>());
-    hipLaunchKernelGGL(tree_node_stats, dim3(blocks(nn)), dim3(256), 0, st, int(nn), p, nterm, t->momg.as<double>(),
-                       t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
-    hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+    hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), p, nterm, NC,
+                       mom.as<double>(), t->ncent.as<double4>(), momg.as<double>(), mom32.as<float>());
+    return LPM_OK;
+}
+
+static int set_strengths(lpm_ctx* c, const double* gamma) {
+    Tree* t = c->tree;
+    hipStream_t st = c->stream;
+    LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
+    if (t->ns > 0)
+        hipLaunchKernelGGL(tree_set_gamma, dim3(blocks(t->ns)), dim3(256), 0, st, gamma, t->perm.as<int>(), t->ns,
+                           t->spos.as<double4>());
+    hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(t->nn)), dim3(256), 0, st, int(t->nn), t->nkids.as<int>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
                        t->soff32.as<float4>());
+    return LPM_OK;
+}
+
+int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
+    Tree* t = c->tree;
+    const int nterm = nterms(p);
+    LPM_CHECK(t->nstat.reserve(size_t(t->nn) * sizeof(float4)));
+    LPM_CHECK(set_strengths(c, gamma));
+    LPM_CHECK(moments_pass<3>(c, p, sign, t->mom, t->momg, t->mom32));
+    hipLaunchKernelGGL(tree_node_stats, dim3(blocks(t->nn)), dim3(256), 0, c->stream, int(t->nn), p, nterm,
+                       t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
     t->order = p;
     t->moment_sign = sign;
+    return LPM_OK;
+}
+
+int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
+    Tree* t = c->tree;
+    LPM_CHECK(set_strengths(c, gamma));
+    LPM_CHECK(moments_pass<1>(c, p, 1, t->pmom, t->pmomg, t->pmom32));
+    LPM_HIP(c, hipGetLastError());
+    t->psi_order = p;
     return LPM_OK;
 }
 
@@ -671,6 +706,24 @@ int lpm_tree_moments(lpm_ctx* c, const double* gamma, int p, int sign) {
     if (c->tree->ns > 0)
         LPM_HIP(c, hipMemcpyAsync(c->h_out.ptr, gamma, c->tree->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
     LPM_CHECK(tree_moments_device(c, c->h_out.as<double>(), p, sign));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_tree_moments_psi_dev(lpm_ctx* c, const double* gamma, int p) {
+    LPM_CHECK(check_moments(c, gamma, p, 1));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_moments_psi_device(c, gamma, p);
+}
+
+int lpm_tree_moments_psi(lpm_ctx* c, const double* gamma, int p) {
+    LPM_CHECK(check_moments(c, gamma, p, 1));
+    LPM_HIP(c, hipSetDevice(c->device));
+    const int64_t n = std::max<int64_t>(c->tree->ns, 1);
+    LPM_CHECK(c->h_out.reserve(n * sizeof(double)));
+    if (c->tree->ns > 0)
+        LPM_HIP(c, hipMemcpyAsync(c->h_out.ptr, gamma, c->tree->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    LPM_CHECK(tree_moments_psi_device(c, c->h_out.as<double>(), p));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
     return LPM_OK;
 }

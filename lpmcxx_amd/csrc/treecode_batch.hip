@@ -509,6 +509,84 @@ __global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ 
     uz[o] = acc[2 * n + i] * mult;
 }
 
+// ---------------------------------------------------------------- streamfunction
+// psi(t) = -(1/4 pi) sum_j gamma_j log(1 - t.y_j) (src/LpmScalarKernel.h:31-45)
+// with the batch acceptance of tree_eval_batch.  Far nodes: the log series
+// of order P about the node centre from the scalar moments, minus ln 2 times
+// the node's total strength (log(1 - t.y) = log(|t - y|^2 / 2) on the unit
+// sphere).  Near leaves: summed directly -- FP64: the reference's pair
+// log(1 - t.y); FP32: log2 |t - y|^2 of the FP32 offsets.  Pairs at zero
+// distance contribute nothing (the singular kernel has no self term).
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void tree_eval_psi(Targets tg, TreeView tv, const double* __restrict__ pmomg,
+                                                     const float* __restrict__ pmom32, double theta, double mult,
+                                                     double* __restrict__ psi, unsigned long long* __restrict__ stats) {
+    __shared__ int s_node[4][kStackB];
+    constexpr double kLn2 = 0.69314718055994530942;
+    const int w = threadIdx.x >> 6;
+    const int64_t bi = wave_index();
+    if (!is_batch(tg, bi)) return;
+    bool valid;
+    int orig;
+    double t0, t1, t2;
+    load_target(tg, bi, valid, orig, t0, t1, t2);
+    const Batch bt = batch_of(valid, t0, t1, t2);
+    double acc = 0.0;
+    unsigned long long nfar = 0, nnear = 0;
+    const int nterm = tv.nterm;
+    int sp = 1;
+    s_node[w][0] = 0;
+    while (sp > 0) {
+        const int n = __builtin_amdgcn_readfirstlane(s_node[w][--sp]);
+        const double4 c = tv.ncent[n];
+        const double dx = bt.cx - c.x, dy = bt.cy - c.y, dz = bt.cz - c.z;
+        const double s = sqrt(dx * dx + dy * dy + dz * dz);
+        const double rn = tv.nrad[n];
+        if (uniform(rn + bt.r < theta * s)) {
+            const double* mu = pmomg + int64_t(n) * nterm;
+            double S;
+            if constexpr (F32)
+                S = far_log_scaled<P>(float(t0 - c.x), float(t1 - c.y), float(t2 - c.z), 0.f, float(c.w),
+                                      pmom32 + int64_t(n) * nterm);
+            else
+                S = far_log<P>(t0 - c.x, t1 - c.y, t2 - c.z, 0.0, mu);
+            acc += S - kLn2 * mu[0];
+            ++nfar;
+            continue;
+        }
+        const int nk = tv.nkids[n];
+        if (nk == 0) {
+            const int b = tv.nbeg[n], e = tv.nend[n];
+            if constexpr (F32) {
+                const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
+                float l2 = 0.f, gs = 0.f;
+                for (int j = b; j < e; ++j) {
+                    const float4 o = tv.soff[j];
+                    const float ex = q0 - o.x, ey = q1 - o.y, ez = q2 - o.z;
+                    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                    const bool ok = r2 > 0.f;
+                    l2 = fmaf(ok ? o.w : 0.f, __builtin_amdgcn_logf(ok ? r2 : 1.f), l2);
+                    gs += ok ? o.w : 0.f;
+                }
+                acc += kLn2 * (double(l2) - double(gs));
+            } else {
+                for (int j = b; j < e; ++j) {
+                    const double4 y = tv.spos[j];
+                    const double ex = t0 - y.x, ey = t1 - y.y, ez = t2 - y.z;
+                    if (ex * ex + ey * ey + ez * ez > 0.0) acc += log(1.0 - (t0 * y.x + t1 * y.y + t2 * y.z)) * y.w;
+                }
+            }
+            nnear += unsigned(e - b);
+        } else {
+            const int f = tv.nfirst[n];
+            for (int q = nk - 1; q >= 0; --q) s_node[w][sp++] = f + q;
+        }
+    }
+    if (valid) psi[orig] = acc * mult;
+    const unsigned long long nv = unsigned(bt.nvalid);
+    flush_stats(stats, nfar * nv, nnear * nv, nfar * nv * unsigned(P * (P + 1) * (P + 2) / 6), 0ull);
+}
+
 // ---------------------------------------------------------------- labels
 __device__ inline uint32_t mix32(uint32_t h) {  // MurmurHash3 finalizer
     h ^= h >> 16;
@@ -652,6 +730,41 @@ int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double
     case PP:                                          \
         if (f32) go(tree_eval_batch<PP, true>);       \
         else go(tree_eval_batch<PP, false>);          \
+        break;
+        LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
+        LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
+#undef LPM_CASE
+        default: return fail(c, LPM_ERR_INVALID_ARG, "series order %d unsupported", order);
+    }
+    mark_end(c);
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
+}
+
+int tree_eval_psi_device(lpm_ctx* c, int prec, int order, double theta, const double* tx, const double* ty,
+                         const double* tz, int64_t nt, double* psi) {
+    Tree* t = c->tree;
+    int *tperm = nullptr, *bstart = nullptr;
+    int64_t nbatch = 0;
+    const int* nbd = nullptr;
+    LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch, &nbd));
+    if (nt == 0) return LPM_OK;
+    const Targets tg{tx, ty, tz, tperm, nt, bstart, nbatch, nbd};
+    TreeView tv = view(t);
+    tv.nterm = nterms(t->psi_order);
+    const unsigned nb = unsigned((nbatch + 3) / 4);
+    const double mult = -1.0 / (4.0 * kPi);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, tg, tv, t->pmomg.as<double>(),
+                           t->pmom32.as<float>(), theta, mult, psi, c->stats_ptr);
+    };
+    const bool f32 = prec == LPM_F32C;
+    mark_begin(c);
+    switch (order) {
+#define LPM_CASE(PP)                                \
+    case PP:                                        \
+        if (f32) go(tree_eval_psi<PP, true>);       \
+        else go(tree_eval_psi<PP, false>);          \
         break;
         LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
         LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
@@ -849,6 +962,45 @@ int lpm_tree_eval_batch(lpm_ctx* c, int prec, int order, double theta, double de
                        [&](const double* x, const double* y, const double* z, double* u, double* v, double* w) {
                            return tree_eval_batch_device(c, prec, order, theta, delta, x, y, z, nt, u, v, w);
                        });
+}
+
+static int check_psi(lpm_ctx* c, int prec, int order, double theta, const double* tx, const double* ty,
+                     const double* tz, int64_t nt, double* psi) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    c->err.clear();
+    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
+    if (order < 1 || order > kMaxLearnedOrder) return fail(c, LPM_ERR_INVALID_ARG, "order %d unsupported", order);
+    if (!(theta >= 0.0)) return fail(c, LPM_ERR_INVALID_ARG, "theta must be >= 0");
+    if (!c->tree) return fail(c, LPM_ERR_STATE, "no tree (lpm_tree_build first)");
+    if (c->tree->psi_order < order)
+        return fail(c, LPM_ERR_STATE, "streamfunction moments of order >= %d needed (lpm_tree_moments_psi)", order);
+    if (nt < 0 || nt > (int64_t(1) << 30)) return fail(c, LPM_ERR_INVALID_ARG, "invalid target count");
+    if (nt > 0 && (!tx || !ty || !tz || !psi)) return fail(c, LPM_ERR_NULL_PTR, "NULL target/output");
+    return LPM_OK;
+}
+
+int lpm_tree_eval_psi_dev(lpm_ctx* c, int prec, int order, double theta, const double* tx, const double* ty,
+                          const double* tz, int64_t nt, double* psi) {
+    LPM_CHECK(check_psi(c, prec, order, theta, tx, ty, tz, nt, psi));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return tree_eval_psi_device(c, prec, order, theta, tx, ty, tz, nt, psi);
+}
+
+int lpm_tree_eval_psi(lpm_ctx* c, int prec, int order, double theta, const double* tx, const double* ty,
+                      const double* tz, int64_t nt, double* psi) {
+    LPM_CHECK(check_psi(c, prec, order, theta, tx, ty, tz, nt, psi));
+    LPM_HIP(c, hipSetDevice(c->device));
+    if (nt == 0) return LPM_OK;
+    hipStream_t st = c->stream;
+    LPM_CHECK(c->h_in.reserve(4 * size_t(nt) * sizeof(double)));
+    double* d = c->h_in.as<double>();
+    LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_CHECK(tree_eval_psi_device(c, prec, order, theta, d, d + nt, d + 2 * nt, nt, d + 3 * nt));
+    LPM_HIP(c, hipMemcpyAsync(psi, d + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    LPM_HIP(c, hipStreamSynchronize(st));
+    return LPM_OK;
 }
 
 int lpm_tree_eval_learned_dev(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,

@@ -371,18 +371,34 @@ TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, st
     if (!bk) throw std::logic_error("invalid kernel");
     if (bk->radius() != 1.0) throw std::invalid_argument("TreeSum: the series assumes the unit sphere");
     _smooth = bk->delta();
+    buildFromMesh(*pmesh, max_particles_per_box);
+}
+
+TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
+                 const std::shared_ptr<ScalarKernel> kernel, const std::string tgt, const std::string src)
+    : _maxSeriesOrder(max_series_order), _maxParticlesPerNode(max_particles_per_box), _mesh(pmesh),
+      _vertTgt(pmesh->getVertexFieldPtr(tgt)), _faceSrc(pmesh->getFaceFieldPtr(src)),
+      _faceTgt(pmesh->getFaceFieldPtr(tgt)), sKern(kernel.get()) {
+    auto gk = dynamic_cast<SphereGreensFn*>(sKern);
+    if (!gk) throw std::logic_error("invalid kernel");
+    if (gk->radius() != 1.0) throw std::invalid_argument("TreeSum: the series assumes the unit sphere");
+    _momentSign = 1;
+    buildFromMesh(*pmesh, max_particles_per_box);
+}
+
+void TreeSum::buildFromMesh(const MeshedParticles& pm, int leafMax) {
     _gpu = &GpuContext::threadDefault();
     std::vector<scalar_type> x, y, z;
-    for (index_type i = 0; i < pmesh->nFaces(); ++i) {
-        if (pmesh->faceIsDivided(i)) continue;
-        const XyzVector c = pmesh->getFaceCentroid(i);
+    for (index_type i = 0; i < pm.nFaces(); ++i) {
+        if (pm.faceIsDivided(i)) continue;
+        const XyzVector c = pm.getFaceCentroid(i);
         x.push_back(c.x); y.push_back(c.y); z.push_back(c.z);
         _leafFaces.push_back(i);
     }
     int64_t nn = 0;
     int depth = 0;
-    _gpu->check(lpm_tree_build(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), pmesh->maxR(),
-                               max_particles_per_box, &nn, &depth),
+    _gpu->check(lpm_tree_build(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), pm.maxR(), leafMax, &nn,
+                               &depth),
                 "TreeSum::TreeSum");
     _nnodes = nn;
     _depth = depth;
@@ -390,6 +406,45 @@ TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, st
     // node lists hold source positions among the leaf faces; report face indices
     for (Node& n : _nodes)
         for (index_type& j : n.coordsContained) j = _leafFaces[j];
+}
+
+void TreeSum::scalarSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, const scalar_type mp_tol,
+                          int& macCounter) {
+    auto pm = _mesh.lock();
+    auto vt = _vertTgt.lock();
+    auto ft = _faceTgt.lock();
+    auto src = _faceSrc.lock();
+    if (!pm || !vt || !ft || !src) throw std::logic_error("TreeSum::meshSolve: mesh or fields expired");
+    std::vector<scalar_type> g;
+    g.reserve(_leafFaces.size());
+    for (index_type f : _leafFaces) g.push_back(src->getScalar(f) * pm->getFaceArea(f));
+    _gpu->check(lpm_tree_moments_psi(_gpu->get(), g.data(), _maxSeriesOrder), "TreeSum::meshSolve");
+    _haveMoments = false;  // the near-field strengths were reset
+    std::vector<scalar_type> tx, ty, tz;
+    std::vector<index_type> vtx, face;
+    const int r = mv.getRank();
+    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+        const XyzVector v = pm->getVertex(i);
+        tx.push_back(v.x); ty.push_back(v.y); tz.push_back(v.z);
+        vtx.push_back(i);
+    }
+    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+        if (pm->faceIsDivided(i)) continue;
+        const XyzVector c = pm->getFaceCentroid(i);
+        tx.push_back(c.x); ty.push_back(c.y); tz.push_back(c.z);
+        face.push_back(i);
+    }
+    const int64_t nt = int64_t(tx.size());
+    std::vector<scalar_type> psi(nt);
+    _gpu->check(lpm_tree_eval_psi(_gpu->get(), _prec, _maxSeriesOrder, mp_tol, tx.data(), ty.data(), tz.data(), nt,
+                                  psi.data()),
+                "TreeSum::meshSolve");
+    int64_t st[LPM_TREE_NSTATS] = {};
+    _gpu->check(lpm_tree_stats(_gpu->get(), st), "TreeSum::meshSolve");
+    const size_t nv = vtx.size();
+    for (size_t i = 0; i < nv; ++i) vt->replace(vtx[i], psi[i]);
+    for (size_t i = 0; i < face.size(); ++i) ft->replace(face[i], psi[nv + i]);
+    macCounter = int(st[LPM_STAT_FAR]);
 }
 
 void TreeSum::computeMeshMoments() { ensureMoments(_maxSeriesOrder); }
@@ -472,6 +527,8 @@ float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
 
 void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, const scalar_type mp_tol,
                         int& macCounter) {
+    if (sKern) return scalarSolve(mv, mf, mp_tol, macCounter);
+    if (!vKern) throw std::logic_error("TreeSum::meshSolve: no kernel");
     auto pm = _mesh.lock();
     auto vt = _vertTgt.lock();
     auto ft = _faceTgt.lock();

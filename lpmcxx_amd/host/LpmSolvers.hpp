@@ -314,6 +314,17 @@ class TreeSum : public Tree {
     TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
             const std::shared_ptr<VectorKernel> kernel, const std::string tgtFieldName,
             const std::string srcFieldName);
+    // Scalar kernel (SphereGreensFn): the streamfunction tree sum the
+    // reference declares (initScalarKernel, src/LpmTreeSum.h:55-56) but never
+    // implements (computeScalarSum is commented out, :61-65).  meshSolve then
+    // fills the scalar target field by lpm_tree_eval_psi: batch acceptance with
+    // tolerance mp_tol, log series of order max_series_order (moment sign +1);
+    // macCounter counts accepted (target, node) pairs.
+    TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
+            const std::shared_ptr<ScalarKernel> kernel, const std::string tgtFieldName,
+            const std::string srcFieldName);
+    void initVectorKernel(VectorKernel* kernel) { vKern = kernel; sKern = nullptr; }
+    void initScalarKernel(ScalarKernel* kernel) { sKern = kernel; vKern = nullptr; }
 
     void computeMeshMoments();
     void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces, const scalar_type mp_tol,
@@ -355,8 +366,11 @@ class TreeSum : public Tree {
     void ensureMoments(int order);
     std::weak_ptr<MeshedParticles> _mesh;
     std::weak_ptr<Field> _vertTgt, _faceSrc, _faceTgt;
-    VectorKernel* vKern;
+    VectorKernel* vKern = nullptr;
+    ScalarKernel* sKern = nullptr;
     std::vector<index_type> _leafFaces;  // source j -> face index
+    void buildFromMesh(const MeshedParticles& pm, int leafMax);
+    void scalarSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, scalar_type mp_tol, int& macCounter);
 };
 
 // ---------------------------------------------------------------- Taylor series (host, FP64)

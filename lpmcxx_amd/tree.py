@@ -117,6 +117,30 @@ def evaluate_batch_dev(ctx: Context, tx, ty, tz, out, theta: float, order: int, 
                                               tx.numel(), _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
 
 
+def moments_psi(ctx: Context, gamma, order: int) -> None:
+    """Streamfunction moments sum_j gamma_j (y_j - c)^k of the current tree."""
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    ctx.check(ctx.lib.lpm_tree_moments_psi(ctx.h, _ptr(g), order))
+
+
+def moments_psi_dev(ctx: Context, gamma, order: int) -> None:
+    ctx.check(ctx.lib.lpm_tree_moments_psi_dev(ctx.h, _ptr(gamma), order))
+
+
+def evaluate_psi(ctx: Context, targets, theta: float, order: int, prec: int = LPM_F64) -> np.ndarray:
+    """Streamfunction treecode (lpm_tree_eval_psi), batch acceptance, log
+    series of the given order."""
+    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    psi = np.zeros(tx.size)
+    ctx.check(ctx.lib.lpm_tree_eval_psi(ctx.h, prec, order, theta, _ptr(tx), _ptr(ty), _ptr(tz), tx.size, _ptr(psi)))
+    return psi
+
+
+def evaluate_psi_dev(ctx: Context, tx, ty, tz, psi, theta: float, order: int, prec: int = LPM_F64) -> None:
+    ctx.check(ctx.lib.lpm_tree_eval_psi_dev(ctx.h, prec, order, theta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
+                                            _ptr(psi)))
+
+
 def info(ctx: Context):
     nn, depth, nl = C.c_int64(), C.c_int(), C.c_int64()
     ctx.check(ctx.lib.lpm_tree_info(ctx.h, C.byref(nn), C.byref(depth), C.byref(nl)))

@@ -77,6 +77,27 @@ int main(int argc, char** argv) {
                     DirectSum bad(parts, kernel, "psi", "nosuchfield");
                 } catch (const std::invalid_argument&) { threw = true; }
                 CHECK(threw);
+                // streamfunction treecode (TreeSum with the scalar kernel) against the direct sum
+                pm->createVertexField("psiTree", "", 1);
+                pm->createFaceField("psiTree", "", 1);
+                TreeSum tree(10, 64, pm, kernel, "psiTree", "vorticity");
+                int mac = 0;
+                tree.meshSolve(MPIReplicatedData(pm->nVertices()), MPIReplicatedData(pm->nFaces()), 0.5, mac);
+                double td = 0.0, tm = 0.0;
+                auto pt = pm->getFaceFieldPtr("psiTree");
+                auto vt = pm->getVertexFieldPtr("psiTree"), vd = pm->getVertexFieldPtr("psi");
+                for (index_type i = 0; i < pm->nFaces(); ++i) {
+                    if (pm->faceIsDivided(i)) continue;
+                    td = std::max(td, std::abs(pt->getScalar(i) - psi->getScalar(i)));
+                    tm = std::max(tm, std::abs(psi->getScalar(i)));
+                }
+                for (index_type i = 0; i < pm->nVertices(); ++i) {
+                    td = std::max(td, std::abs(vt->getScalar(i) - vd->getScalar(i)));
+                    tm = std::max(tm, std::abs(vd->getScalar(i)));
+                }
+                std::printf("  psi treecode (p 10, theta 0.5) vs direct: max rel diff %.3e, %d far pairs\n", td / tm,
+                            mac);
+                CHECK(td / tm < 1e-5 && mac > 0);
             }
         }
         CHECK(prev < 0.06);
