@@ -24,6 +24,10 @@ Measured in the same run:
     the library on its stream (lpm_last_kernel_ms);
   * a CPU baseline: the oracle (reference-order FP64 direct sum, OpenMP) on a
     bounded sample of targets.
+
+--workload mesh prints the line of another path instead: GPU mesh generation
+of the cubed sphere at --mesh-level (default 10), one step = one whole
+generation, against the host mesh builder (see workload_mesh).
 """
 from __future__ import annotations
 
@@ -39,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 FP32_PEAK = 157.3   # TFLOP/s, MI355X FP32 vector (packed)
 FP32_NONPACKED = 78.6
 FP64_PEAK = 78.6    # TFLOP/s, MI355X FP64 vector
+HBM_PEAK = 8000.0   # GB/s, MI355X HBM3E
 DIRECT_FLOPS = 18   # velocity pair: d 3, |d|^2 5, q 2, rcp 1, w 1, acc 6
 METRIC = "Biot-Savart interactions/s + RK4 steps/s at 1.57M particles, rel L2 err ≤1e-5"
 
@@ -162,6 +167,83 @@ def rk4_level10(ctx, args):
             "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy}
 
 
+def workload_mesh(args) -> dict:
+    """GPU mesh generation (lpm_mesh_generate_dev), cubed sphere: one step =
+    the whole refinement from the seed to the leaf centroids and areas, into
+    device arrays allocated before timing.  The CPU baseline is the host
+    builder (liblpm's PolyMesh2d: the reference's sequential division loop)
+    on the same level; its arrays double as a full-size parity check
+    (vertices and centroids bit for bit, areas to a few ulp)."""
+    import numpy as np
+    import torch
+    import lpmcxx_amd as L
+
+    seed, level = L.LPM_SEED_CUBED, args.mesh_level
+    nv, nf, nl = L.mesh_counts(seed, level)
+    ctx = L.Context(0)
+    out = ctx.mesh_generate_dev(seed, level)
+    for _ in range(args.warmup):
+        ctx.mesh_generate_dev(seed, level, out=out)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.mesh_generate_dev(seed, level, out=out)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / args.steps
+    leaf_ms = []
+    for _ in range(args.steps):
+        ctx.mesh_generate_dev(seed, level, out=out)
+        leaf_ms.append(ctx.last_kernel_ms())
+    leaf_ms = float(np.mean(leaf_ms))
+    log(f"[bench] mesh level {level}: {1e3 * step_s:.3f} ms per generation, leaf kernel {leaf_ms:.3f} ms")
+
+    # algorithmic HBM bytes of the leaf kernel: corner edge lists (4 ints per
+    # leaf), edges read once (orig, dest, left: 3 ints), vertices read once
+    # (3 doubles), centroid and area written (4 doubles)
+    ne = nl + nv - 2
+    alg_bytes = 16 * nl + 12 * ne + 24 * nv + 32 * nl
+    roof = {"bound": "valu", "kernel": "mesh_leaves<4>", "launch_ms": leaf_ms, "algorithmic_bytes": alg_bytes,
+            "hbm_achieved_GBps": alg_bytes / (leaf_ms / 1e3) / 1e9, "achieved": None, "peak": FP64_PEAK,
+            "unit": "TFLOP/s", "frac": None, "traffic": None}
+    roof["hbm_frac"] = roof["hbm_achieved_GBps"] / HBM_PEAK
+    f = ROOT / "profiles" / "mesh_leaves_l10_counters.json"
+    if level == 10 and f.exists():
+        d = json.loads(f.read_text())
+        flops = d["fp64_flops_per_launch"]
+        roof.update({"achieved": flops / (leaf_ms / 1e3) / 1e12, "fp64_flops_per_launch": flops,
+                     "flops_source": "executed FP64 VALU flops (2 FMA + ADD + MUL, x64 lanes) per launch from "
+                                     "profiles/mesh_leaves_l10_counters.md",
+                     "traffic": d.get("traffic_bytes_per_launch")})
+        roof["frac"] = roof["achieved"] / FP64_PEAK
+
+    cpu, parity = None, None
+    if not args.no_cpu:
+        log("[bench] host builder (CPU baseline) and full-size parity")
+        t0 = time.perf_counter()
+        h = L.Mesh(seed, level)
+        dt = time.perf_counter() - t0
+        cpu = {"value": nl / dt, "unit": "leaf faces/s", "cores": 1, "kind": "port",
+               "sample": f"whole level-{level} cubed sphere by the host PolyMesh2d builder (liblpm, the "
+                         f"reference's sequential division loop in C++), {dt:.2f} s"}
+        g = {k: v.cpu().numpy() for k, v in out.items() if v is not None}
+        parity = {"vertices_bit_exact": all(np.array_equal(g[k], getattr(h, k)) for k in ("vx", "vy", "vz")),
+                  "centroids_bit_exact": all(np.array_equal(g[k], getattr(h, k)) for k in ("cx", "cy", "cz")),
+                  "area_max_rel_diff": float(np.max(np.abs(g["area"] - h.area) / h.area))}
+    ctx.close()
+    return {
+        "metric": f"GPU mesh generation: leaf faces/s (cubed sphere level {level})", "value": nl / step_s,
+        "unit": "leaf faces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * step_s, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"uniform refinement of the cubed-sphere seed to level {level}: {nv} vertices, "
+                               f"{nl} leaf faces ({nf} faces with parents), radius 1",
+                   "vertices": nv, "leaf_faces": nl},
+        "roofline": roof, "cpu_baseline": cpu, "parity_vs_host_builder": parity,
+    }
+
+
 def measured_traffic(kernel: str, args) -> tuple:
     """HBM bytes per launch of `kernel` from the committed counter summary of
     the default level-9 learned step (FETCH_SIZE / WRITE_SIZE passes scaled by
@@ -210,10 +292,17 @@ def main() -> int:
     ap.add_argument("--l10-steps", type=int, default=10)
     ap.add_argument("--l10-tau", type=float, default=1.5e-7)
     ap.add_argument("--l10-label-stride", type=int, default=64)
+    ap.add_argument("--workload", choices=["learned", "mesh"], default="learned",
+                    help="learned: the velocity step (default); mesh: GPU mesh generation")
+    ap.add_argument("--mesh-level", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
+    if args.workload != "learned":
+        out = {"mesh": workload_mesh}[args.workload](args)
+        print(json.dumps(out), flush=True)
+        return 0
 
     import numpy as np
     import torch

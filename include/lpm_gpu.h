@@ -172,7 +172,8 @@ int lpm_set_option(lpm_ctx* ctx, int option, double value);
  * lpm_tree_eval_batch*, and the low-order interaction-list kernel
  * (tree_eval_lists of the near leaves and the far entries of order <= 7; not
  * the planning pass, not the order 8..12 list kernel) of
- * lpm_tree_eval_learned*.  Waits for the kernel.
+ * lpm_tree_eval_learned*, and the leaf kernel (centroids and areas of the
+ * last level) of lpm_mesh_generate*.  Waits for the kernel.
  * No counterpart in the reference. */
 int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
 
@@ -337,6 +338,32 @@ int lpm_bve_step_dev(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, doub
 /* nsteps BVE steps on host arrays (copied in once, out once). */
 int lpm_bve_steps(lpm_ctx* ctx, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t n_src,
                   int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area);
+
+/* ------------------------------------------------------------------ mesh generation
+ * Uniform refinement of a seed mesh on the GPU: the particle sets of
+ * MeshedParticles(seed, level, ..., radius) (src/LpmMeshedParticles.cpp:12-21,
+ * PolyMesh2d src/LpmPolyMesh2d.cpp:20-93 dividing faces by
+ * src/LpmQuadFaces.cpp:14-146 / src/LpmTriFaces.cpp:13-142 and edges by
+ * src/LpmEdges.cpp:99-120), in the same index order as the host builder of
+ * lpm_mesh.h: all vertices, and the leaf faces in ascending face index
+ * (centroid, area).  Vertices and centroids equal the host builder's bit for
+ * bit; areas agree to a few units in the last place (device atan2 / tan).
+ * seed: LPM_SEED_ICOS_TRI (0) or LPM_SEED_CUBED (1); level 0..12. */
+#ifndef LPM_SEED_ICOS_TRI
+#define LPM_SEED_ICOS_TRI 0  /* as in lpm_mesh.h */
+#define LPM_SEED_CUBED 1
+#endif
+/* vertex, face (parents included) and leaf-face counts of seed / level
+ * (src/LpmMeshSeed.cpp:233-284); LPM_ERR_INVALID_ARG out of range */
+int lpm_mesh_counts(int seed, int level, int64_t* n_vertices, int64_t* n_faces, int64_t* n_leaf_faces);
+/* Device pointers: vx, vy, vz[n_vertices]; cx, cy, cz, area[n_leaf_faces];
+ * leaf_vertices[n_leaf_faces * (3 or 4)] the corners of each leaf face
+ * counter-clockwise (may be NULL).  Enqueued on the context stream. */
+int lpm_mesh_generate_dev(lpm_ctx* ctx, int seed, int level, double radius, double* vx, double* vy, double* vz,
+                          double* cx, double* cy, double* cz, double* area, int64_t* leaf_vertices);
+/* The same into host arrays (returns when they are filled). */
+int lpm_mesh_generate(lpm_ctx* ctx, int seed, int level, double radius, double* vx, double* vy, double* vz,
+                      double* cx, double* cy, double* cz, double* area, int64_t* leaf_vertices);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,9 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_bve_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp],
         "lpm_bve_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp,
                           _vp],
+        "lpm_mesh_counts": [C.c_int, C.c_int, _i64p, _i64p, _i64p],
+        "lpm_mesh_generate": [_vp, C.c_int, C.c_int, C.c_double] + [_vp] * 8,
+        "lpm_mesh_generate_dev": [_vp, C.c_int, C.c_int, C.c_double] + [_vp] * 8,
     }
     for name, args in sigs.items():
         if hasattr(lib, name):
@@ -240,6 +243,7 @@ class Context:
         if st != 0:
             raise LpmError(st, self.lib.lpm_status_string(st).decode())
         self.h = h
+        self.device = device
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -342,10 +346,65 @@ def _bve_step_dev(self, cfg, dt, omega, n_src, x, y, z, zeta_abs, area):
                                          _ptr(z), _ptr(zeta_abs), _ptr(area)))
 
 
+def mesh_counts(seed: int, level: int) -> tuple[int, int, int]:
+    """(vertices, faces with parents, leaf faces) of seed refined to level."""
+    nv, nf, nl = C.c_int64(), C.c_int64(), C.c_int64()
+    st = gpu_lib().lpm_mesh_counts(seed, level, C.byref(nv), C.byref(nf), C.byref(nl))
+    if st != 0:
+        raise LpmError(st, f"mesh seed={seed} level={level}")
+    return nv.value, nf.value, nl.value
+
+
+class GpuMesh(Mesh):
+    """The particle sets of Mesh(seed, level, radius) generated on the GPU
+    (lpm_mesh_generate): same attributes, same order; leaf_vertices (n_leaves
+    x 3 or 4 corner indices) when asked for."""
+
+    def __init__(self, ctx: "Context", seed: int, level: int, radius: float = 1.0, leaf_vertices: bool = False):
+        self.n_vertices, self.n_faces, self.n_leaves = mesh_counts(seed, level)
+        nv, nl, nepf = self.n_vertices, self.n_leaves, 4 if seed == LPM_SEED_CUBED else 3
+        self.vx, self.vy, self.vz = np.empty(nv), np.empty(nv), np.empty(nv)
+        self.cx, self.cy, self.cz, self.area = np.empty(nl), np.empty(nl), np.empty(nl), np.empty(nl)
+        self.leaf_vertices = np.empty((nl, nepf), np.int64) if leaf_vertices else None
+        ctx.check(ctx.lib.lpm_mesh_generate(ctx.h, seed, level, radius, _ptr(self.vx), _ptr(self.vy), _ptr(self.vz),
+                                            _ptr(self.cx), _ptr(self.cy), _ptr(self.cz), _ptr(self.area),
+                                            _ptr(self.leaf_vertices)))
+        # leaves are the faces of the last level, the last n_leaves face indices
+        self.face_index = np.arange(self.n_faces - nl, self.n_faces, dtype=np.int64)
+        self.max_radius = float(np.sqrt(np.max(self.vx * self.vx + self.vy * self.vy + self.vz * self.vz)))
+        self.seed, self.level, self.radius = seed, level, radius
+
+
+def _mesh_generate_dev(self, seed, level, radius=1.0, leaf_vertices=False, out=None):
+    """Mesh generated into device tensors, enqueued on the context stream:
+    dict of vx, vy, vz, cx, cy, cz, area (float64) and leaf_vertices (int64,
+    or None); `out` (a dict returned by an earlier call of the same seed and
+    level) is filled again instead of allocating."""
+    import torch
+    nv, _, nl = mesh_counts(seed, level)
+    nepf = 4 if seed == LPM_SEED_CUBED else 3
+    if out is None:
+        dev = torch.device("cuda", self.device)
+        out = {k: torch.empty(nv, dtype=torch.float64, device=dev) for k in ("vx", "vy", "vz")}
+        out.update({k: torch.empty(nl, dtype=torch.float64, device=dev) for k in ("cx", "cy", "cz", "area")})
+        out["leaf_vertices"] = torch.empty((nl, nepf), dtype=torch.int64, device=dev) if leaf_vertices else None
+    else:
+        sizes = {"vx": nv, "vy": nv, "vz": nv, "cx": nl, "cy": nl, "cz": nl, "area": nl}
+        if any(out[k].numel() != n or out[k].dtype != torch.float64 for k, n in sizes.items()):
+            raise ValueError("out does not match this seed and level")
+        lv = out.get("leaf_vertices")
+        if lv is not None and (lv.numel() != nl * nepf or lv.dtype != torch.int64):
+            raise ValueError("out['leaf_vertices'] does not match this seed and level")
+    self.check(self.lib.lpm_mesh_generate_dev(self.h, seed, level, radius, *(_ptr(out[k]) for k in
+                                              ("vx", "vy", "vz", "cx", "cy", "cz", "area", "leaf_vertices"))))
+    return out
+
+
 Context.rk4_steps = _rk4_steps
 Context.rk4_step_dev = _rk4_step_dev
 Context.bve_steps = _bve_steps
 Context.bve_step_dev = _bve_step_dev
+Context.mesh_generate_dev = _mesh_generate_dev
 
 
 def native_libraries() -> list[str]:

@@ -75,7 +75,7 @@ void lpm_destroy(lpm_ctx* c) {
     model_free(c->model);
     c->tree = nullptr;
     for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work,
-                      &c->tb_work, &c->tl_work})
+                      &c->tb_work, &c->tl_work, &c->mesh_work})
         b->release();
     if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);

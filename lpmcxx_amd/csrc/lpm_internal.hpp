@@ -44,6 +44,8 @@ struct lpm_ctx {
     lpm::DevBuf rk_work;
     // target batches of the batch treecode, interaction lists of the learned one
     lpm::DevBuf tb_work, tl_work;
+    // level tables of the GPU mesh generator
+    lpm::DevBuf mesh_work;
     int far_cap = 1024, near_cap = 512;
     int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
     lpm::Tree* tree = nullptr;

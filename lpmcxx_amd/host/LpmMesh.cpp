@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cstdlib>
 
+#include "../csrc/lpm_seeds.hpp"
+
 namespace Lpm {
 
 scalar_type Field::maxMagnitude() const {
@@ -19,46 +21,8 @@ scalar_type Field::maxMagnitude() const {
 }
 
 // ---------------------------------------------------------------- seed tables
-// Coordinates are the decimal strings of the seed files
-// (meshSeeds/icosTriSphereSeed.dat:2-13, meshSeeds/cubedSphereSeed.dat:2-9),
-// parsed with strtod so the doubles equal the reference's stream-parsed ones.
-namespace {
-const char* kIcosXyz[12][3] = {
-    {"0.0", "0.0", "1.0"},
-    {"0.723606797749978969640917366873", "0.525731112119133606025669084848", "0.447213595499957939281834733746"},
-    {"-0.276393202250021030359082633126", "0.850650808352039932181540497063", "0.447213595499957939281834733746"},
-    {"-0.894427190999915878563669467492", "0.0", "0.447213595499957939281834733746"},
-    {"-0.276393202250021030359082633127", "-0.850650808352039932181540497063", "0.447213595499957939281834733746"},
-    {"0.723606797749978969640917366873", "-0.525731112119133606025669084848", "0.447213595499957939281834733746"},
-    {"0.894427190999915878563669467492", "0.0", "-0.447213595499957939281834733746"},
-    {"0.276393202250021030359082633127", "0.850650808352039932181540497063", "-0.447213595499957939281834733746"},
-    {"-0.723606797749978969640917366873", "0.525731112119133606025669084848", "-0.447213595499957939281834733746"},
-    {"-0.723606797749978969640917366873", "-0.525731112119133606025669084848", "-0.447213595499957939281834733746"},
-    {"0.276393202250021030359082633127", "-0.850650808352039932181540497063", "-0.447213595499957939281834733746"},
-    {"0.0", "0.0", "-1.0"}};
-// orig, dest, left face, right face (meshSeeds/icosTriSphereSeed.dat:15-44)
-const int kIcosEdges[30][4] = {
-    {0, 1, 0, 4},    {1, 2, 0, 6},    {2, 0, 0, 1},    {2, 3, 1, 8},    {0, 3, 2, 1},    {3, 4, 2, 10},
-    {4, 0, 2, 3},    {4, 5, 3, 12},   {5, 0, 3, 4},    {5, 1, 4, 14},   {1, 6, 5, 14},   {6, 7, 5, 15},
-    {7, 1, 5, 6},    {7, 2, 6, 7},    {7, 8, 7, 16},   {8, 2, 7, 8},    {8, 3, 8, 9},    {8, 9, 9, 17},
-    {3, 9, 10, 9},   {9, 4, 10, 11},  {9, 10, 11, 18}, {10, 4, 11, 12}, {10, 5, 12, 13}, {10, 6, 13, 19},
-    {6, 5, 13, 14},  {11, 6, 19, 15}, {11, 7, 15, 16}, {8, 11, 17, 16}, {11, 9, 17, 18}, {10, 11, 19, 18}};
-// face edge lists (meshSeeds/icosTriSphereSeed.dat:67-86)
-const int kIcosFaces[20][3] = {{0, 1, 2},    {2, 3, 4},    {4, 5, 6},    {6, 7, 8},    {8, 9, 0},
-                               {10, 11, 12}, {13, 1, 12},  {13, 14, 15}, {16, 3, 15},  {16, 17, 18},
-                               {19, 5, 18},  {19, 20, 21}, {22, 7, 21},  {22, 23, 24}, {10, 9, 24},
-                               {26, 11, 25}, {27, 14, 26}, {28, 17, 27}, {29, 20, 28}, {25, 23, 29}};
-
-const char* kCubeA = "0.577350269189626";
-// sign pattern of the 8 cube corners (meshSeeds/cubedSphereSeed.dat:2-9)
-const int kCubeSigns[8][3] = {{1, -1, 1},  {1, -1, -1}, {1, 1, -1},  {1, 1, 1},
-                              {-1, 1, -1}, {-1, 1, 1},  {-1, -1, -1}, {-1, -1, 1}};
-// meshSeeds/cubedSphereSeed.dat:17-28
-const int kCubeEdges[12][4] = {{0, 1, 0, 3}, {1, 2, 0, 5}, {2, 3, 0, 1}, {3, 0, 0, 4}, {2, 4, 1, 5}, {4, 5, 1, 2},
-                               {5, 3, 1, 4}, {4, 6, 2, 5}, {6, 7, 2, 3}, {7, 5, 2, 4}, {6, 1, 3, 5}, {0, 7, 3, 4}};
-// meshSeeds/cubedSphereSeed.dat:37-42
-const int kCubeFaces[6][4] = {{0, 1, 2, 3}, {2, 4, 5, 6}, {5, 7, 8, 9}, {8, 10, 0, 11}, {11, 3, 6, 9}, {10, 7, 4, 1}};
-}  // namespace
+// The tables live in lpm_seeds.hpp, shared with the GPU mesh generator.
+using namespace lpm_seed;
 
 void IcosTriSphereSeed::tables(std::vector<XyzVector>& v, std::vector<index_type>& e,
                                std::vector<index_type>& f) const {

@@ -134,3 +134,16 @@ def test_children_are_consecutive_and_nested(mesh):
     c = np.stack([m2.cx, m2.cy, m2.cz], 1)
     owner = cKDTree(p).query(c)[1]
     assert np.array_equal(owner, np.repeat(np.arange(len(p)), 4))
+
+
+def test_gpu_generator_counts_match_host_builder(mesh):
+    """lpm_mesh_counts (the GPU generator's closed forms, no device needed)
+    against the host builder's sizes; out-of-range arguments rejected."""
+    for seed in (L.LPM_SEED_CUBED, L.LPM_SEED_ICOS_TRI):
+        for level in range(0, 6):
+            m = mesh(seed, level)
+            assert L.mesh_counts(seed, level) == (m.n_vertices, m.n_faces, m.n_leaves)
+        assert L.mesh_counts(seed, 12)[2] == (6 if seed == L.LPM_SEED_CUBED else 20) * 4 ** 12
+    for seed, level in ((2, 3), (1, 13), (0, -1)):
+        with pytest.raises(L.LpmError):
+            L.mesh_counts(seed, level)
