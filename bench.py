@@ -45,6 +45,7 @@ FP32_NONPACKED = 78.6
 FP64_PEAK = 78.6    # TFLOP/s, MI355X FP64 vector
 HBM_PEAK = 8000.0   # GB/s, MI355X HBM3E
 DIRECT_FLOPS = 18   # velocity pair: d 3, |d|^2 5, q 2, rcp 1, w 1, acc 6
+PSI_NEAR_FLOPS = 12  # streamfunction pair (FP32): d 3, |d|^2 5, log 1, gamma * log 2, strength sum 1
 METRIC = "Biot-Savart interactions/s + RK4 steps/s at 1.57M particles, rel L2 err ≤1e-5"
 
 
@@ -165,6 +166,259 @@ def rk4_level10(ctx, args):
             "err_sample": "4096 random targets vs the FP64 direct sum over all sources",
             "trajectory_rel_diff": traj, "tau": args.l10_tau, "label_samples": int(len(ls.ids)),
             "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy}
+
+
+def flop_roof(kernel: str, ms: float, near: int, far: int, terms: int, near_flops: int, far_flops: int = 16,
+              term_flops: int = 8) -> dict:
+    """FP32 VALU roofline of one treecode evaluation kernel from its pair and
+    term counts and its launch time (HIP events on the library stream)."""
+    flops = near * near_flops + far * far_flops + terms * term_flops
+    ach = flops / (ms / 1e3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
+            "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
+            "flops_per_launch": flops, "launch_ms": ms, "far_pairs": far, "near_pairs": near, "far_terms": terms}
+
+
+def sample_rate(fn, n_src: int, n_tgt: int, target_seconds: float, seed: int = 5):
+    """Time the oracle fn(idx) on a growing random sample of targets until it
+    takes target_seconds; returns (pairs/s, sample size, seconds)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    k = 16
+    while True:
+        idx = np.sort(rng.choice(n_tgt, k, replace=False))
+        t0 = time.perf_counter()
+        fn(idx)
+        dt = time.perf_counter() - t0
+        if dt >= target_seconds or k >= n_tgt:
+            return k * (n_src - 1) / dt, k, dt
+        k = min(n_tgt, int(k * max(2.0, 1.2 * target_seconds / max(dt, 1e-3))))
+
+
+def workload_psi(args) -> dict:
+    """Streamfunction treecode (lpm_tree_*_psi) on the particles of the
+    default workload (cubed L9, 8 Gaussian vortices): one step = tree build +
+    streamfunction moments + FP32C batch evaluation of psi at every particle.
+    Judged against the FP64 direct streamfunction on 8,192 random targets
+    (self pair excluded); the FP64 streamfunction treecode and the FP32C
+    direct sum are timed beside it."""
+    import numpy as np
+    import torch
+    import lpmcxx_amd as L
+    from lpmcxx_amd import tree as T
+    from oracle import oracle as O
+
+    PSI_K = L.LPM_K_GREENS_SPHERE
+    m = L.Mesh(L.LPM_SEED_CUBED, args.level)
+    n = m.n_leaves
+    g = L.random_vortices(m.cx, m.cy, m.cz, m.area, n=8, seed=1234) * m.area
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X, Y, Z, G = dev(m.cx), dev(m.cy), dev(m.cz), dev(g)
+    ctx = L.Context(0)
+    idx = np.sort(np.random.default_rng(7).choice(n, 8192, replace=False))
+    I = dev(idx.astype(np.int64))
+    ref = torch.empty(len(idx), dtype=torch.float64, device="cuda")
+    ctx.direct_sum_dev(*(dev(a[idx]) for a in (m.cx, m.cy, m.cz)), X, Y, Z, G, None, kernels=PSI_K,
+                       prec=L.LPM_F64, skip=I, psi=ref)
+    PSI = torch.empty_like(X)
+
+    def err():
+        ctx.synchronize()
+        return float(torch.linalg.norm(PSI[I] - ref) / torch.linalg.norm(ref))
+
+    def step(leaf, p, theta, prec):
+        T.build_dev(ctx, X, Y, Z, m.max_radius, leaf)
+        T.moments_psi_dev(ctx, G, p)
+        T.evaluate_psi_dev(ctx, X, Y, Z, PSI, theta, p, prec=prec)
+
+    cfg = (args.psi_leaf, args.psi_order, args.psi_theta, L.LPM_F32C)
+    for _ in range(args.warmup):
+        step(*cfg)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(*cfg)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / args.steps
+    e32 = err()
+    kms = []
+    for _ in range(args.steps):
+        step(*cfg)
+        kms.append(ctx.last_kernel_ms())
+    st = T.stats(ctx)
+    kms = float(np.mean(kms))
+    roof = flop_roof(f"tree_eval_psi<{args.psi_order}, true>", kms, st["near"], st["far"], st["terms"],
+                     PSI_NEAR_FLOPS)
+    log(f"[bench] psi treecode: {1e3 * step_s:.2f} ms per step, rel-L2 {e32:.2e}, eval kernel {kms:.2f} ms")
+
+    # the FP64 streamfunction treecode (its fastest setting within 1e-5) and the FP32C direct sum
+    f64 = (64, 6, 0.4, L.LPM_F64)
+    step(*f64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        step(*f64)
+    ctx.synchronize()
+    s64 = (time.perf_counter() - t0) / 2
+    e64 = err()
+    ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, None, kernels=PSI_K, prec=L.LPM_F32C,
+                       skip=torch.arange(n, device="cuda", dtype=torch.int64), psi=PSI)
+    ds_ms = ctx.last_kernel_ms()
+    eds = err()
+    log(f"[bench] psi FP64 treecode {1e3 * s64:.1f} ms (rel-L2 {e64:.2e}); FP32C direct {ds_ms:.1f} ms "
+        f"(rel-L2 {eds:.2e})")
+    cpu = None
+    if not args.no_cpu:
+        src = (m.cx, m.cy, m.cz)
+        rate, k, dt = sample_rate(lambda ix: O.greens_direct(tuple(a[ix] for a in src), src, g, skip=ix), n, n,
+                                  args.cpu_seconds)
+        cpu = {"value": rate, "unit": "interactions/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"{k} random targets x all {n} sources, FP64 reference-order streamfunction oracle, "
+                         f"{dt:.1f} s"}
+    ctx.close()
+    pairs = n * (n - 1)
+    return {
+        "metric": f"streamfunction interactions/s at {n / 1e6:.2f}M particles, rel L2 err <=1e-5",
+        "value": pairs / step_s, "unit": "interactions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * step_s, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32c+f64", "data": "synthetic",
+        "config": {"workload": f"streamfunction of all {n} particles of a cubed-sphere level-{args.level} mesh, "
+                               "8 Gaussian vortices",
+                   "step": f"FP32C streamfunction treecode (build + moments + evaluation), leaf {args.psi_leaf}, "
+                           f"order {args.psi_order}, theta {args.psi_theta}",
+                   "rel_l2_err_vs_fp64_direct": e32, "err_sample": "8192 random targets, FP64 direct sum"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "fp64_psi_tree": {"ms_per_step": 1e3 * s64, "rel_l2_err": e64, "leaf": 64, "order": 6, "theta": 0.4,
+                          "speedup_of_step": s64 / step_s},
+        "direct_f32c_psi": {"kernel_ms": ds_ms, "rel_l2_err": eds, "interactions_per_s": pairs / (ds_ms / 1e3)},
+        "stats": st,
+    }
+
+
+def workload_bve(args) -> dict:
+    """Barotropic vorticity RK4 of a meshed particle set (lpm_bve_step_dev):
+    cubed L9, the leaf faces are the vortex particles (zeta_rel = solid body
+    alpha = 1 + the Rossby-Haurwitz wave), the vertices passive tracers, Omega
+    = 2 pi, dt = 0.01.  One step = one RK4 step of faces and vertices (4
+    velocity evaluations at 3.15M targets from 1.57M sources) by the learned
+    FP32C treecode, its order model trained on the faces' and vertices' own
+    labels at t = 0.  The plain FP64 treecode steps from the same start."""
+    import numpy as np
+    import torch
+    import lpmcxx_amd as L
+    from lpmcxx_amd import learned as LN
+    from lpmcxx_amd import tree as T
+    from oracle import oracle as O
+
+    m = L.Mesh(L.LPM_SEED_CUBED, args.level)
+    nf, nv = m.n_leaves, m.n_vertices
+    nt = nf + nv
+    omega, alpha, dt = 2 * np.pi, 1.0, 0.01
+    zrel = 2 * alpha * m.cz + L.sphere_harmonic54(m.cx, m.cy, m.cz)
+    g = zrel * m.area
+    u_ref = LN.u_ref_of(g)
+    tx, ty, tz = (np.concatenate([a, b]) for a, b in ((m.cx, m.vx), (m.cy, m.vy), (m.cz, m.vz)))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X, Y, Z = dev(tx), dev(ty), dev(tz)
+    ZA, A, G = dev(zrel + 2 * omega * m.cz), dev(m.area), dev(g)
+    SX, SY, SZ = X[:nf], Y[:nf], Z[:nf]
+    ctx = L.Context(0)
+    ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
+
+    t0 = time.perf_counter()
+    T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.leaf)
+    T.moments_dev(ctx, G, 12, 1)
+    ls = LN.labels(ctx, (tx, ty, tz), args.theta_max, u_ref, batch_stride=args.label_stride,
+                   keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
+    fr = LN.fit(ctx, ls, args.bve_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+
+    # velocity error at t = 0 on a sample of faces and vertices
+    idx = np.sort(np.random.default_rng(7).choice(nt, 4096, replace=False))
+    I = dev(idx.astype(np.int64))
+    ref = [torch.empty(len(idx), dtype=torch.float64, device="cuda") for _ in range(3)]
+    ctx.direct_sum_dev(*(dev(a[idx]) for a in (tx, ty, tz)), SX, SY, SZ, G, ref, prec=L.LPM_F64,
+                       skip=dev(np.where(idx < nf, idx, -1).astype(np.int64)))
+    U = [torch.empty_like(X) for _ in range(3)]
+
+    def err():
+        ctx.synchronize()
+        d = torch.stack([u[I] for u in U]) - torch.stack(ref)
+        return float(torch.linalg.norm(d) / torch.linalg.norm(torch.stack(ref)))
+
+    T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.leaf)
+    T.moments_dev(ctx, G, args.max_order, 1)
+    LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
+    err_l = err()
+    T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.base_leaf)
+    T.moments_dev(ctx, G, args.base_order, 1)
+    T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
+    err_b = err()
+
+    cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
+                                   theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
+    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
+                                   theta=args.base_theta, leaf_max=args.base_leaf)
+
+    def run(cfg, nsteps, warm):
+        P = [X.clone(), Y.clone(), Z.clone()]
+        W = [X.clone(), Y.clone(), Z.clone()]
+        for _ in range(warm):
+            ctx.bve_step_dev(cfg, dt, omega, nf, *W, ZA, A)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            ctx.bve_step_dev(cfg, dt, omega, nf, *P, ZA, A)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / nsteps, P
+
+    log(f"[bench] BVE: {args.steps} learned steps, {args.steps} plain FP64 steps")
+    s_l, P_l = run(cfg_l, args.steps, args.warmup)
+    kms = ctx.last_kernel_ms()
+    st = T.stats(ctx)
+    roof = flop_roof("tree_eval_lists<7, true, true>", kms, st["near"], st["far"] - st["far_high"],
+                     st["terms"] - st["terms_high"], DIRECT_FLOPS)
+    roof["launch_note"] = "last velocity evaluation (RK4 stage 4) of the last timed step"
+    s_b, P_b = run(cfg_b, args.steps, 1)
+    P0 = torch.stack([X, Y, Z])
+    traj = float(torch.linalg.norm(torch.stack(P_l) - torch.stack(P_b)) / torch.linalg.norm(torch.stack(P_b) - P0))
+    log(f"[bench] BVE: learned {1e3 * s_l:.1f} ms/step (t=0 err {err_l:.2e}), plain FP64 {1e3 * s_b:.1f} ms/step "
+        f"(t=0 err {err_b:.2e}), trajectory difference {traj:.2e}")
+    cpu = None
+    if not args.no_cpu:
+        src = (m.cx, m.cy, m.cz)
+        tgt = (tx, ty, tz)
+        rate, k, sec = sample_rate(
+            lambda ix: O.bs_direct(tuple(a[ix] for a in tgt), src, g, skip=np.where(ix < nf, ix, -1)), nf, nt,
+            args.cpu_seconds)
+        pairs_per_step = 4 * (nt * nf - nf)
+        cpu = {"value": rate / pairs_per_step, "unit": "steps/s", "cores": O.num_threads(),
+               "kind": "port", "interactions_per_s": rate,
+               "sample": f"{k} random targets of the {nt} x all {nf} sources, FP64 reference-order oracle "
+                         f"direct sum, {sec:.1f} s; steps/s = interactions/s / (4 stages x {pairs_per_step // 4} "
+                         f"pairs)"}
+    ctx.close()
+    return {
+        "metric": f"BVE RK4 steps/s at {nf / 1e6:.2f}M vortex faces + {nv / 1e6:.2f}M tracer vertices, "
+                  "rel L2 err <=1e-5",
+        "value": 1.0 / s_l, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * s_l, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32c+f64", "data": "synthetic",
+        "config": {"workload": f"barotropic vorticity RK4 of a cubed-sphere level-{args.level} mesh: {nf} faces "
+                               f"(vortex particles, solid body + Rossby-Haurwitz wave) and {nv} vertices (passive "
+                               "tracers), Omega 2 pi, dt 0.01",
+                   "step": f"learned-order FP32C treecode per stage, leaf {args.leaf}, theta_max {args.theta_max}, "
+                           f"tau {args.bve_tau}, max order {args.max_order}",
+                   "rel_l2_err_t0": err_l, "err_sample": "4096 random faces and vertices, FP64 direct sum"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "plain_fp64_tree": {"ms_per_step": 1e3 * s_b, "rel_l2_err_t0": err_b, "speedup": s_b / s_l},
+        "trajectory_rel_diff_vs_plain_fp64": traj, "label_samples": int(len(ls.ids)),
+        "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy, "stats_last_stage": st,
+    }
 
 
 def workload_mesh(args) -> dict:
@@ -292,15 +546,20 @@ def main() -> int:
     ap.add_argument("--l10-steps", type=int, default=10)
     ap.add_argument("--l10-tau", type=float, default=1.5e-7)
     ap.add_argument("--l10-label-stride", type=int, default=64)
-    ap.add_argument("--workload", choices=["learned", "mesh"], default="learned",
-                    help="learned: the velocity step (default); mesh: GPU mesh generation")
+    ap.add_argument("--workload", choices=["learned", "mesh", "psi", "bve"], default="learned",
+                    help="learned: the velocity step (default); mesh: GPU mesh generation; psi: the "
+                         "streamfunction treecode; bve: the barotropic vorticity RK4 of faces and vertices")
     ap.add_argument("--mesh-level", type=int, default=10)
+    ap.add_argument("--psi-leaf", type=int, default=256)
+    ap.add_argument("--psi-order", type=int, default=6)
+    ap.add_argument("--psi-theta", type=float, default=0.4)
+    ap.add_argument("--bve-tau", type=float, default=2e-7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
     if args.workload != "learned":
-        out = {"mesh": workload_mesh}[args.workload](args)
+        out = {"mesh": workload_mesh, "psi": workload_psi, "bve": workload_bve}[args.workload](args)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -439,14 +698,8 @@ def main() -> int:
     # the learned step that is the low-order list kernel, so the far pairs and
     # terms of the order 8..12 list kernel are left out
     def tree_roof(stats, ms, kernel):
-        far = stats["far"] - stats.get("far_high", 0)
-        terms = stats["terms"] - stats.get("terms_high", 0)
-        flops = stats["near"] * DIRECT_FLOPS + terms * 8 + far * 16
-        ach = flops / (ms / 1e3) / 1e12
-        return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
-                "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
-                "flops_per_launch": flops, "launch_ms": ms, "far_pairs": far, "near_pairs": stats["near"],
-                "far_terms": terms}
+        return flop_roof(kernel, ms, stats["near"], stats["far"] - stats.get("far_high", 0),
+                         stats["terms"] - stats.get("terms_high", 0), DIRECT_FLOPS)
 
     direct_flops = pairs * DIRECT_FLOPS
     direct_roof = {"bound": "valu", "kernel": "ds_f32c", "achieved": direct_flops / (ms32 / 1e3) / 1e12,
