@@ -32,7 +32,11 @@ struct Tree {
     // source data in tree order (device)
     DevBuf perm;      // int32 [ns]  tree position -> source index
     DevBuf spos;      // double4 [ns] (x, y, z, gamma) in tree order (gamma set by moments)
-    DevBuf keys;      // uint64 [ns] Morton keys in tree order
+    DevBuf keys;      // uint64 [ns] Morton keys, sorted on their top kFastLevels levels (or fully)
+    DevBuf kperm;     // int32 [ns]  source index in the order of `keys` (before the leaf resort)
+    // source arrays of the last build (device), so that targets which are the
+    // sources can reuse the key order instead of sorting again
+    const double *sx = nullptr, *sy = nullptr, *sz = nullptr;
     // node table (device)
     DevBuf nbeg, nend, nfirst, nkids, nlevel, nparent;  // int32 [cap]
     DevBuf nprefix;   // uint64 [cap] key prefix (top 3*level bits)

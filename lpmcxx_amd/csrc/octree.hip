@@ -85,9 +85,21 @@ __global__ void tree_root(int n, int* nbeg, int* nend, int* nlevel, int* nparent
     lvt[kLvChunks] = 0;
 }
 
+// The child intervals of node [b, e) at `level`, found by the 8 lanes of an
+// aligned lane group at once: lane j holds the first position whose child
+// code is >= j (j = 0: b) and the start of the next interval (lane 7: e), so
+// child j is [lo, hi) when it is not empty.  Every lane of the group calls
+// this (the shuffles need all eight).
+__device__ inline void child_interval(const uint64_t* keys, int b, int e, int level, bool search, int& lo, int& hi) {
+    const int j = threadIdx.x & 7;
+    lo = (search && j > 0) ? lower_code(keys, b, e, level, j) : b;
+    const int nx = __shfl_down(lo, 1, 8);
+    hi = (j == 7) ? e : nx;
+}
+
 // Children count of every node of `level` (first node and count from the
-// level table); cnt[t] = 0 for t in [count, ub) so a scan over ub entries is
-// exact.  Leaves are counted with their moment chunks.
+// level table), 8 lanes per node; cnt[t] = 0 for t in [count, ub) so a scan
+// over ub entries is exact.  Leaves are counted with their moment chunks.
 // control 0 (MAX_COORDS_PER_NODE): split iff count > param
 // control 1 (MAX_DEPTH):           split iff level < param (src/LpmOctree.cpp:145-180)
 __global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ keys, int level, int sorted_levels,
@@ -95,29 +107,32 @@ __global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ k
                                                   const int* __restrict__ nbeg, const int* __restrict__ nend,
                                                   int* __restrict__ cnt, int* __restrict__ nfirst,
                                                   int* __restrict__ nkids) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ub) return;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int j = threadIdx.x & 7;
+    if (t >= ub) return;  // whole lane groups leave together
     const int start = lvt[2 * level], count = lvt[2 * level + 1];
     if (t >= count) {
-        cnt[t] = 0;
+        if (j == 0) cnt[t] = 0;
         return;
     }
     const int i = start + t;
     const int b = nbeg[i], e = nend[i];
     const bool want = (control == 0) ? (e - b > param) : (level < param && e > b);
-    int c = 0;
+    bool search = false;
     if (want) {
-        if (level >= kMaxDepth) atomicOr(lvt + kLvTooDeep, 1);         // cannot split further
-        else if (level >= sorted_levels) atomicOr(lvt + kLvKeys, 1);   // keys sorted too coarsely
-        else {
-            int prev = b;
-            for (int code = 1; code <= 8; ++code) {
-                const int nxt = (code == 8) ? e : lower_code(keys, prev, e, level, code);
-                c += (nxt > prev);
-                prev = nxt;
-            }
+        if (level >= kMaxDepth) {
+            if (j == 0) atomicOr(lvt + kLvTooDeep, 1);         // cannot split further
+        } else if (level >= sorted_levels) {
+            if (j == 0) atomicOr(lvt + kLvKeys, 1);            // keys sorted too coarsely
+        } else {
+            search = true;
         }
     }
+    int lo, hi;
+    child_interval(keys, b, e, level, search, lo, hi);
+    const unsigned grp = unsigned(__ballot(search && hi > lo) >> (threadIdx.x & 56)) & 0xffu;
+    if (j != 0) return;
+    const int c = __popc(grp);
     cnt[t] = c;
     nfirst[i] = -1;
     nkids[i] = 0;
@@ -128,38 +143,42 @@ __global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ k
 }
 
 // Children of the nodes of `level` at first + count + off[t] (off = exclusive
-// scan of cnt), and the next level's entry of the level table.
+// scan of cnt), and the next level's entry of the level table; 8 lanes per
+// node, lane j writes child j.
 __global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ keys, int level, int* lvt,
                                                  const int* __restrict__ cnt, const int* __restrict__ off, int* nbeg,
                                                  int* nend, int* nfirst, int* nkids, int* nlevel, int* nparent,
-                                                 uint64_t* nprefix) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 uint64_t* nprefix, int ub) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int j = threadIdx.x & 7;
     const int start = lvt[2 * level], count = lvt[2 * level + 1];
-    if (t == 0) {
+    if (t == 0 && j == 0) {
         lvt[2 * level + 2] = start + count;
         lvt[2 * level + 3] = count > 0 ? off[count - 1] + cnt[count - 1] : 0;
     }
-    if (t >= count || cnt[t] <= 0) return;
+    if (t >= ub) return;  // whole lane groups leave together
+    const bool split = t < count && cnt[t] > 0;
     const int i = start + t;
-    const int b = nbeg[i], e = nend[i];
-    int k = start + count + off[t];
-    nfirst[i] = k;
-    nkids[i] = cnt[t];
-    int prev = b;
-    for (int code = 0; code < 8; ++code) {
-        const int nxt = (code == 7) ? e : lower_code(keys, prev, e, level, code + 1);
-        if (nxt > prev) {
-            nbeg[k] = prev;
-            nend[k] = nxt;
-            nlevel[k] = level + 1;
-            nparent[k] = i;
-            nprefix[k] = (nprefix[i] << 3) | uint64_t(code);
-            nfirst[k] = -1;
-            nkids[k] = 0;
-            ++k;
-        }
-        prev = nxt;
+    const int b = split ? nbeg[i] : 0, e = split ? nend[i] : 0;
+    int lo, hi;
+    child_interval(keys, b, e, level, split, lo, hi);
+    const bool mine = split && hi > lo;
+    const unsigned grp = unsigned(__ballot(mine) >> (threadIdx.x & 56)) & 0xffu;
+    if (!split) return;
+    const int first = start + count + off[t];
+    if (j == 0) {
+        nfirst[i] = first;
+        nkids[i] = cnt[t];
     }
+    if (!mine) return;
+    const int k = first + __popc(grp & ((1u << j) - 1u));
+    nbeg[k] = lo;
+    nend[k] = hi;
+    nlevel[k] = level + 1;
+    nparent[k] = i;
+    nprefix[k] = (nprefix[i] << 3) | uint64_t(j);
+    nfirst[k] = -1;
+    nkids[k] = 0;
 }
 
 // Leaf chunk list of the moment kernel: node i owns chunks n0[i] .. n0[i] +
@@ -335,7 +354,7 @@ constexpr int kLevelGroup = 4;   // levels built between two reads of the build 
 
 void tree_free(Tree* t) {
     if (!t) return;
-    for (DevBuf* b : {&t->perm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
+    for (DevBuf* b : {&t->perm, &t->kperm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
                       &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32,
                       &t->soff32, &t->nstat, &t->mpart, &t->pmom, &t->pmomg, &t->pmom32, &t->tmp0, &t->tmp1,
                       &t->tmp2, &t->tmp3, &t->sort_tmp, &t->scan_tmp, &t->lvt, &t->chunks})
@@ -350,6 +369,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     hipStream_t st = c->stream;
     t->ns = n;
     t->nn = 0;
+    t->sx = t->sy = t->sz = nullptr;  // set once the build has succeeded
     t->order = 0;
     t->psi_order = 0;
     t->leaf_max = (control == 0) ? param : 0;
@@ -359,6 +379,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
 
     LPM_CHECK(t->keys.reserve(nalloc * sizeof(uint64_t)));
     LPM_CHECK(t->perm.reserve(nalloc * sizeof(int)));
+    LPM_CHECK(t->kperm.reserve(nalloc * sizeof(int)));
     LPM_CHECK(t->spos.reserve(nalloc * sizeof(double4)));
     LPM_CHECK(t->tmp0.reserve(nalloc * sizeof(uint64_t)));   // unsorted keys / truncated keys
     LPM_CHECK(t->tmp1.reserve(nalloc * sizeof(int) + 64));   // unsorted values
@@ -408,15 +429,15 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             for (int j = 0; j < G; ++j) {
                 const int level = lk + j;
                 const int ub = int(std::max<int64_t>(1, j == 0 ? ck : bound_after(ck, j)));
-                hipLaunchKernelGGL(tree_count, dim3(blocks(ub)), dim3(256), 0, st, t->keys.as<uint64_t>(), level,
+                hipLaunchKernelGGL(tree_count, dim3(blocks(int64_t(ub) * 8)), dim3(256), 0, st, t->keys.as<uint64_t>(), level,
                                    sorted_levels, control, param, ub, lvt, t->nbeg.as<int>(), t->nend.as<int>(), cnt,
                                    t->nfirst.as<int>(), t->nkids.as<int>());
                 size_t sb = scan_bytes;
                 LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, cnt, off, ub, st));
-                hipLaunchKernelGGL(tree_emit, dim3(blocks(ub)), dim3(256), 0, st, t->keys.as<uint64_t>(), level, lvt,
-                                   cnt, off, t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(),
+                hipLaunchKernelGGL(tree_emit, dim3(blocks(int64_t(ub) * 8)), dim3(256), 0, st, t->keys.as<uint64_t>(),
+                                   level, lvt, cnt, off, t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(),
                                    t->nkids.as<int>(), t->nlevel.as<int>(), t->nparent.as<int>(),
-                                   t->nprefix.as<uint64_t>());
+                                   t->nprefix.as<uint64_t>(), ub);
             }
             LPM_HIP(c, hipGetLastError());
             LPM_HIP(c, hipMemcpyAsync(h, lvt, kLvTotal * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -455,8 +476,11 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                            t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
                            t->perm.as<int>(), t->tmp0.as<uint64_t>());
         hipLaunchKernelGGL(iota_i32, dim3(blocks(n)), dim3(256), 0, st, t->tmp1.as<int>(), n);
+        // the resorted order goes to kperm, then the two swap: perm = leaf
+        // order, kperm = the order of the sorted keys
         LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->tmp2.as<uint64_t>(), t->tmp1.as<int>(),
-                             t->perm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
+                             t->kperm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
+        std::swap(t->perm, t->kperm);
         hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
                            t->spos.as<double4>());
     }
@@ -484,6 +508,11 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     hipLaunchKernelGGL(chunk_fill, dim3(blocks(nn)), dim3(256), 0, st, int(nn), n0, nc, t->nbeg.as<int>(),
                        t->nend.as<int>(), cn, cb, ce);
     LPM_HIP(c, hipGetLastError());
+    if (n > 0) {
+        t->sx = x;
+        t->sy = y;
+        t->sz = z;
+    }
     return LPM_OK;
 }
 
@@ -526,6 +555,8 @@ static int build_common(lpm_ctx* c, const double* x, const double* y, const doub
         x = d; y = d + na; z = d + 2 * na;
     }
     LPM_CHECK(tree_build_device(c, x, y, z, n, lo, hi, control, param));
+    // staged host sources: later staged targets may reuse the same buffer
+    if (host) c->tree->sx = c->tree->sy = c->tree->sz = nullptr;
     if (n_nodes) *n_nodes = c->tree->nn;
     if (depth) *depth = c->tree->depth;
     return LPM_OK;

@@ -6,11 +6,12 @@
 //                  h = moment_sign * (y_j - centroid).  The reference uses
 //                  h = centroid - y (moment_sign = -1), which flips the sign of
 //                  every odd-order term relative to the coefficients; +1 is the
-//                  consistent expansion.  Work item = (node, chunk of 128
-//                  particles); one thread per multi-index with the chunk's
-//                  powers h^a staged in LDS (particle-major, conflict free);
-//                  chunk partials reduced per node in chunk order
-//                  (deterministic).
+//                  consistent expansion.  Leaves: work item = (leaf, chunk of
+//                  128 particles), the chunk's powers h^a staged in LDS, each
+//                  multi-index summed by 256 / nterm thread groups over parts of
+//                  the chunk, partials added in group and chunk order
+//                  (deterministic).  Internal nodes: the children's moments
+//                  translated and summed (upward pass, tree_m2m).
 // K8-K11 tree_eval one wave per 64 Morton-sorted targets, depth-first over the
 //                  node table with a wave-shared stack of (node, lane mask):
 //                  every lane evaluates the reference MAC
@@ -51,16 +52,61 @@ inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b
 // ---------------------------------------------------------------- moments
 // Moments of NC components per multi-index: NC = 3, m_k = sum_j gamma_j y_j
 // h_j^k (velocity); NC = 1, m_k = sum_j gamma_j h_j^k (streamfunction).
+
+// lex index t of order p -> (k1, k2, k3)
+__device__ inline void lex_decode(int p, int t, int& k1, int& k2, int& k3) {
+    int r = t;
+    k1 = 0;
+    k2 = 0;
+    while (r >= (p - k1) * (p - k1 + 1) / 2) { r -= (p - k1) * (p - k1 + 1) / 2; ++k1; }
+    while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
+    k3 = r;
+}
+
+// The three stored forms of a node's moments: lex FP64 (the reference's
+// order, exported and used by the reference-series traversal), graded FP64
+// (Cartesian series) and graded FP32 scaled by rho^-|k| (rho = the node's box
+// radius; FP32 far field).
+struct MomOut {
+    double* lex;
+    double* graded;
+    float* scaled;
+};
+
+// store component cc of lex term t of `node`; irho = 1 / rho
 template <int NC>
-__global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict__ chunk_node,
-                                                          const int* __restrict__ chunk_beg,
-                                                          const int* __restrict__ chunk_end,
-                                                          const double4* __restrict__ ncent,
-                                                          const double4* __restrict__ spos, int p, int nterm,
-                                                          double sign, double* __restrict__ part) {
+__device__ inline void store_moment(const MomOut& o, int64_t node, int p, int nterm, int t, int cc, double v,
+                                    double irho) {
+    int k1, k2, k3;
+    lex_decode(p, t, k1, k2, k3);
+    const int64_t base = node * nterm * NC;
+    const int64_t g = base + NC * graded_index(k1, k2, k3) + cc;
+    o.lex[base + NC * t + cc] = v;
+    o.graded[g] = v;
+    double sc = 1.0;
+    for (int d = k1 + k2 + k3; d > 0; --d) sc *= irho;
+    o.scaled[g] = float(v * sc);
+}
+
+constexpr int kMomThreads = 256;
+
+// P2M of one leaf chunk: particle powers h^a staged in LDS, then G groups of
+// nterm threads (G = 256 / nterm, at least 1) each sum the terms over a
+// contiguous part of the chunk; the group partials are added in group order.
+// direct: every leaf is one chunk, so the sum is the leaf's moments (all three
+// forms are stored); otherwise the chunk partial goes to `part`.
+template <int NC>
+__global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __restrict__ chunk_node,
+                                                                  const int* __restrict__ chunk_beg,
+                                                                  const int* __restrict__ chunk_end,
+                                                                  const double4* __restrict__ ncent,
+                                                                  const double4* __restrict__ spos, int p, int nterm,
+                                                                  double sign, double* __restrict__ part, MomOut out,
+                                                                  int direct) {
     extern __shared__ double lds[];
     double* pw = lds;                       // [kChunk][3][p] powers
     double* gy = lds + kChunk * 3 * p;      // [kChunk][NC]  gamma * y, or gamma
+    double* red = gy + kChunk * NC;         // [G][nterm][NC] group partials (G > 1)
     const int ch = blockIdx.x;
     const int node = chunk_node[ch], b = chunk_beg[ch], e = chunk_end[ch];
     const int cnt = e - b;
@@ -87,56 +133,58 @@ __global__ __launch_bounds__(256) void tree_moments_chunk(const int* __restrict_
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
-        // decode lex index t -> (k1, k2, k3)
-        int r = t, k1 = 0, k2 = 0;
-        for (;;) {
-            const int m = p - k1;                 // terms with this k1: m(m+1)/2
-            const int blk = m * (m + 1) / 2;
-            if (r < blk) break;
-            r -= blk;
-            ++k1;
-        }
-        for (;;) {
-            const int m = p - k1 - k2;
-            if (r < m) break;
-            r -= m;
-            ++k2;
-        }
-        const int k3 = r;
+    const int G = max(1, kMomThreads / nterm);
+    const double irho = 1.0 / c.w;
+    for (int it = threadIdx.x; it < G * nterm; it += blockDim.x) {
+        const int grp = it / nterm, t = it - grp * nterm;
+        int k1, k2, k3;
+        lex_decode(p, t, k1, k2, k3);
+        const int q0 = grp * cnt / G, q1 = (grp + 1) * cnt / G;
         double m[NC] = {};
-        for (int q = 0; q < cnt; ++q) {
+        for (int q = q0; q < q1; ++q) {
             const double* w = pw + q * 3 * p;
             const double P = w[k1] * w[p + k2] * w[2 * p + k3];
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) m[cc] += gy[NC * q + cc] * P;
         }
-        double* o = part + (int64_t(ch) * nterm + t) * NC;
+        if (G > 1) {
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) o[cc] = m[cc];
+            for (int cc = 0; cc < NC; ++cc) red[(grp * nterm + t) * NC + cc] = m[cc];
+        } else {
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                if (direct) store_moment<NC>(out, node, p, nterm, t, cc, m[cc], irho);
+                else part[(int64_t(ch) * nterm + t) * NC + cc] = m[cc];
+            }
+        }
+    }
+    if (G == 1) return;
+    __syncthreads();
+    for (int it = threadIdx.x; it < nterm * NC; it += blockDim.x) {
+        double v = red[it];
+        for (int grp = 1; grp < G; ++grp) v += red[grp * nterm * NC + it];
+        if (direct) store_moment<NC>(out, node, p, nterm, it / NC, it % NC, v, irho);
+        else part[int64_t(ch) * nterm * NC + it] = v;
     }
 }
 
-// graded position of the lex-order term r of order p
-__device__ inline int lex_to_graded(int p, int r) {
-    int a = 0, b = 0;
-    while (r >= (p - a) * (p - a + 1) / 2) { r -= (p - a) * (p - a + 1) / 2; ++a; }
-    while (r >= p - a - b) { r -= p - a - b; ++b; }
-    return graded_index(a, b, r);
-}
-
-__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int nterm, int ncomp,
+// chunk partials of every leaf summed in chunk order (leaves of more than one
+// chunk); internal nodes are left to the upward pass
+template <int NC>
+__global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int p, int nterm, const int* __restrict__ nkids,
                                                            const int* __restrict__ node_chunk0,
                                                            const int* __restrict__ node_nchunk,
-                                                           const double* __restrict__ part, double* __restrict__ mom) {
+                                                           const double4* __restrict__ ncent,
+                                                           const double* __restrict__ part, MomOut out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= int64_t(nn) * nterm * ncomp) return;
-    const int node = int(i / (int64_t(nterm) * ncomp));
-    const int rem = int(i - int64_t(node) * nterm * ncomp);
+    if (i >= int64_t(nn) * nterm * NC) return;
+    const int node = int(i / (int64_t(nterm) * NC));
+    if (nkids[node] != 0) return;
+    const int rem = int(i - int64_t(node) * nterm * NC);
     const int c0 = node_chunk0[node], nc = node_nchunk[node];
     double s = 0.0;
-    for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * ncomp + rem];
-    mom[i] = s;
+    for (int k = 0; k < nc; ++k) s += part[(int64_t(c0 + k) * nterm) * NC + rem];
+    store_moment<NC>(out, node, p, nterm, rem / NC, rem % NC, s, 1.0 / ncent[node].w);
 }
 
 struct BinomTab {
@@ -162,38 +210,48 @@ __device__ inline int lex_pos(int p, int a, int b, int c) {
 //   m_k(node) = sum_children sum_{j <= k} C(k, j) d^(k - j) m_j(child),
 // the same sums as the particle-by-particle definition of the leaves,
 // regrouped.  Since (h + d)^k factorizes over the axes, the translation is
-// applied as three one-dimensional binomial passes (x, then y, then z) on a
-// child's moments staged in LDS: O(p) work per term instead of O(p^3).
-// One block per node.
+// applied as three one-dimensional binomial passes (x, then y, then z) on the
+// children's moments staged in LDS: O(p) work per term instead of O(p^3).
+// One block per node; up to `gmax` children are translated at once (one
+// thread per (child, term)), then added to the node's sum in child order.
 template <int NC>
-__global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, double sign,
+__global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, double sign, int gmax,
                                                 const int* __restrict__ nfirst, const int* __restrict__ nkids,
-                                                const double4* __restrict__ ncent, double* __restrict__ mom) {
+                                                const double4* __restrict__ ncent, MomOut out) {
     extern __shared__ double lds[];
-    double* A = lds;                   // [nterm][NC]
-    double* B = lds + NC * nterm;      // [nterm][NC]
-    double* acc = lds + 2 * NC * nterm;  // [nterm][NC]
+    const int S = NC * nterm;
+    double* acc = lds;                 // [S]
+    double* A = acc + S;               // [gmax][S]
+    double* B = A + gmax * S;          // [gmax][S]
+    double* dd = B + gmax * S;         // [gmax][3] child offsets
     const int i = first + blockIdx.x;
     const int nk = nkids[i];
     if (nk == 0) return;
     const double4 c = ncent[i];
-    for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) acc[q] = 0.0;
-    for (int ch = nfirst[i]; ch < nfirst[i] + nk; ++ch) {
-        const double4 cc = ncent[ch];
-        const double d[3] = {sign * (cc.x - c.x), sign * (cc.y - c.y), sign * (cc.z - c.z)};
-        const double* mc = mom + int64_t(ch) * nterm * NC;
+    const int f = nfirst[i];
+    for (int q = threadIdx.x; q < S; q += blockDim.x) acc[q] = 0.0;
+    for (int c0 = 0; c0 < nk; c0 += gmax) {
+        const int ng = min(gmax, nk - c0);
         __syncthreads();
-        for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) A[q] = mc[q];
+        // children are consecutive nodes: their moments are one contiguous block
+        const double* mc = out.lex + int64_t(f + c0) * S;
+        for (int q = threadIdx.x; q < ng * S; q += blockDim.x) A[q] = mc[q];
+        if (threadIdx.x < 3 * ng) {
+            const int k = threadIdx.x / 3, ax = threadIdx.x - 3 * k;
+            const double4 cc = ncent[f + c0 + k];
+            dd[threadIdx.x] = sign * (ax == 0 ? cc.x - c.x : (ax == 1 ? cc.y - c.y : cc.z - c.z));
+        }
         __syncthreads();
         for (int axis = 0; axis < 3; ++axis) {
             const double* src = (axis == 1) ? B : A;
             double* dst = (axis == 1) ? A : B;
-            for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
-                int r = t, k1 = 0, k2 = 0;
-                while (r >= (p - k1) * (p - k1 + 1) / 2) { r -= (p - k1) * (p - k1 + 1) / 2; ++k1; }
-                while (r >= p - k1 - k2) { r -= p - k1 - k2; ++k2; }
-                const int k3 = r;
+            for (int it = threadIdx.x; it < ng * nterm; it += blockDim.x) {
+                const int k = it / nterm, t = it - k * nterm;
+                int k1, k2, k3;
+                lex_decode(p, t, k1, k2, k3);
                 const int kk = axis == 0 ? k1 : (axis == 1 ? k2 : k3);
+                const double d = dd[3 * k + axis];
+                const double* sk = src + k * S;
                 double sm[NC] = {};
                 double pw = 1.0;
                 for (int a = kk; a >= 0; --a) {
@@ -201,41 +259,23 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, dou
                                                                                  : lex_pos(p, k1, k2, a));
                     const double w = c_binom.v[kk][a] * pw;
 #pragma unroll
-                    for (int q = 0; q < NC; ++q) sm[q] = fma(w, src[NC * j + q], sm[q]);
-                    pw *= d[axis];
+                    for (int q = 0; q < NC; ++q) sm[q] = fma(w, sk[NC * j + q], sm[q]);
+                    pw *= d;
                 }
 #pragma unroll
-                for (int q = 0; q < NC; ++q) {
-                    if (axis < 2) dst[NC * t + q] = sm[q];
-                    else acc[NC * t + q] += sm[q];
-                }
+                for (int q = 0; q < NC; ++q) dst[k * S + NC * t + q] = sm[q];
             }
             __syncthreads();
         }
+        for (int q = threadIdx.x; q < S; q += blockDim.x) {
+            double v = acc[q];
+            for (int k = 0; k < ng; ++k) v += B[k * S + q];
+            acc[q] = v;
+        }
     }
-    double* o = mom + int64_t(i) * nterm * NC;
-    for (int q = threadIdx.x; q < NC * nterm; q += blockDim.x) o[q] = acc[q];
-}
-
-// graded FP64 and FP32 copies of the lex moments
-// Graded copies of the lex-order moments: FP64 as they are, and FP32 scaled
-// by rho^-|k| (rho = the node's box radius) for the FP32 far field.
-__global__ __launch_bounds__(256) void tree_moments_graded(int nn, int p, int nterm, int ncomp,
-                                                           const double* __restrict__ mom,
-                                                           const double4* __restrict__ ncent,
-                                                           double* __restrict__ momg, float* __restrict__ mom32) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= int64_t(nn) * nterm * ncomp) return;
-    const int node = int(i / (int64_t(nterm) * ncomp));
-    const int rem = int(i - int64_t(node) * nterm * ncomp);
-    const int gi = lex_to_graded(p, rem / ncomp);
-    const int64_t g = int64_t(node) * nterm * ncomp + ncomp * gi + rem % ncomp;
-    const double s = mom[i];
-    momg[g] = s;
-    const double irho = 1.0 / ncent[node].w;
-    double sc = 1.0;
-    for (int d = shell_of(gi); d > 0; --d) sc *= irho;
-    mom32[g] = float(s * sc);
+    __syncthreads();
+    const double irho = 1.0 / c.w;
+    for (int q = threadIdx.x; q < S; q += blockDim.x) store_moment<NC>(out, i, p, nterm, q / NC, q % NC, acc[q], irho);
 }
 
 // Norms of the degree-0, 1 and 2 moment shells (graded positions 0, 1..3,
@@ -474,8 +514,9 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
 
 }  // namespace
 
-// P2M over the build's leaf chunks, chunk reduction, upward pass level by
-// level, graded FP64 copy and FP32 copy scaled by rho^-|k|: NC components.
+// P2M over the build's leaf chunks, chunk reduction (leaves of several
+// chunks), upward pass level by level; every kernel stores the lex, graded
+// and scaled FP32 forms of the moments it makes.  NC components.
 template <int NC>
 static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, DevBuf& mom32) {
     Tree* t = c->tree;
@@ -493,23 +534,30 @@ static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, 
     LPM_CHECK(mom.reserve(size_t(nn) * nterm * NC * sizeof(double)));
     LPM_CHECK(momg.reserve(size_t(nn) * nterm * NC * sizeof(double)));
     LPM_CHECK(mom32.reserve(size_t(nn) * nterm * NC * sizeof(float)));
+    const MomOut out{mom.as<double>(), momg.as<double>(), mom32.as<float>()};
+    // every leaf one chunk: the chunk sums are the leaf moments
+    const int direct = (nch == t->nleaves) ? 1 : 0;
     if (nch > 0) {
-        const size_t lds = size_t(kChunk) * (3 * p + NC) * sizeof(double);
-        hipLaunchKernelGGL(tree_moments_chunk<NC>, dim3(unsigned(nch)), dim3(256), lds, st, d_cn, d_cb, d_ce,
-                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->mpart.as<double>());
+        const int G = std::max(1, kMomThreads / nterm);
+        const size_t lds = (size_t(kChunk) * (3 * p + NC) + (G > 1 ? size_t(G) * nterm * NC : 0)) * sizeof(double);
+        hipLaunchKernelGGL(tree_moments_chunk<NC>, dim3(unsigned(nch)), dim3(kMomThreads), lds, st, d_cn, d_cb, d_ce,
+                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->mpart.as<double>(),
+                           out, direct);
     }
-    hipLaunchKernelGGL(tree_moments_reduce, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), nterm, NC, d_n0,
-                       d_nc, t->mpart.as<double>(), mom.as<double>());
-    // upward pass, deepest internal level first
+    if (!direct)
+        hipLaunchKernelGGL(tree_moments_reduce<NC>, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), p, nterm,
+                           t->nkids.as<int>(), d_n0, d_nc, t->ncent.as<double4>(), t->mpart.as<double>(), out);
+    // upward pass, deepest internal level first; up to gmax children per
+    // block within 64 KB of LDS
+    const int S = NC * nterm;
+    const int gmax = std::max(1, std::min(8, (8192 - S - 24) / (2 * S)));
+    const size_t m2m_lds = size_t(S + 2 * gmax * S + 3 * gmax) * sizeof(double);
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(tree_m2m<NC>, dim3(unsigned(cnt)), dim3(128), size_t(3 * NC) * nterm * sizeof(double), st,
-                           int(f), p, nterm, double(sign), t->nfirst.as<int>(), t->nkids.as<int>(),
-                           t->ncent.as<double4>(), mom.as<double>());
+        hipLaunchKernelGGL(tree_m2m<NC>, dim3(unsigned(cnt)), dim3(256), m2m_lds, st, int(f), p, nterm, double(sign),
+                           gmax, t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), out);
     }
-    hipLaunchKernelGGL(tree_moments_graded, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), p, nterm, NC,
-                       mom.as<double>(), t->ncent.as<double4>(), momg.as<double>(), mom32.as<float>());
     return LPM_OK;
 }
 
@@ -549,6 +597,8 @@ int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
     return LPM_OK;
 }
 
+constexpr int kTargetLevels = 12;  // key levels that order the targets
+
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
                           int** tperm) {
     Tree* t = c->tree;
@@ -563,13 +613,23 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
     LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
     *tperm = v_out;
     if (nt == 0) return LPM_OK;
+    if (nt == t->ns && tx == t->sx && ty == t->sy && tz == t->sz) {
+        // the targets are the tree's sources: their keys are sorted already
+        LPM_HIP(c, hipMemcpyAsync(k_out, t->keys.ptr, size_t(nt) * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        LPM_HIP(c, hipMemcpyAsync(v_out, t->kperm.ptr, size_t(nt) * sizeof(int), hipMemcpyDeviceToDevice, st));
+        return LPM_OK;
+    }
     hipLaunchKernelGGL(target_keys, dim3(blocks(nt)), dim3(256), 0, st, tx, ty, tz, nt, t->root_box[0], t->root_box[1],
                        t->root_box[2], t->root_box[3], t->root_box[4], t->root_box[5], k_in, v_in);
+    // the top kTargetLevels levels order the targets (cells of about 5e-4 on
+    // the unit sphere); the batch kernels only compare neighbouring keys, so
+    // lower levels left unsorted can only split a batch, never mix two far apart
+    const int begin_bit = 3 * (kMaxDepth - kTargetLevels);
     size_t bytes = 0;
-    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0, 3 * kMaxDepth,
-                                                  st));
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, v_in, v_out, int(nt), begin_bit,
+                                                  3 * kMaxDepth, st));
     LPM_CHECK(t->sort_tmp.reserve(bytes + 256));
-    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, k_in, k_out, v_in, v_out, int(nt), 0,
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(t->sort_tmp.ptr, bytes, k_in, k_out, v_in, v_out, int(nt), begin_bit,
                                                   3 * kMaxDepth, st));
     return LPM_OK;
 }
