@@ -37,9 +37,12 @@ namespace lpm {
 namespace {
 
 constexpr int kStackB = 160;  // >= 7 * kMaxDepth + 1
-// the planning pass pops two nodes and pushes up to 16 children: at most 14
-// left behind per level
-constexpr int kStackPlan = 14 * kMaxDepth + 2 + 16;
+// The planning pass pops the top stack entries whose children fit in 16 rows
+// (at least one) and pushes up to 16 children.  Seen as segments (the
+// children pushed by one call), every segment below the top has lost an entry
+// (at most 15 left) and the segments' smallest depths increase up the stack
+// (at most kMaxDepth segments).
+constexpr int kStackPlan = 15 * kMaxDepth + 16;
 constexpr int kLabelOrder = 12;
 constexpr int kMaxLearnedOrder = 12;
 
@@ -306,6 +309,9 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
 // scatters.
 constexpr int kNodeBits = 27;
 
+// planning-stack entry of an internal node: first child | child count << kNodeBits
+__device__ inline int stack_entry(int first, int kids) { return first | (kids << kNodeBits); }
+
 // Far entries of order <= kLowOrder fill a batch's far list from the front,
 // higher orders from the back, so two evaluation kernels with different
 // register footprints each read a contiguous range.
@@ -348,29 +354,60 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         nnear = 1;
         near_pairs = unsigned(tv.nend[0] - tv.nbeg[0]);
     } else {
-        s_node[w][0] = 0;
+        s_node[w][0] = stack_entry(tv.nfirst[0], tv.nkids[0]);
         sp = 1;
     }
-    // The stack holds internal nodes to descend into.  Up to two are popped at
-    // a time and all their children classified in one model call (rows 0..7:
-    // children of the first, rows 8..15: of the second); lane l < 16 then
-    // files its child: accepted -> far list (order p), leaf -> near list,
-    // else -> the stack.
+    // The stack holds internal nodes to descend into, as their first child and
+    // child count (stack_entry), so a pop needs no node-table load.  The top entries whose
+    // children fit in the 16 rows of one model call are popped together (at
+    // least one: a node has at most 8 children, so two always fit) and their
+    // children classified at once, rows in stack order; lane l < 16 then files
+    // its child: accepted -> far list (order p), leaf -> near list, else ->
+    // the stack.
     while (sp > 0) {
-        const int na = __builtin_amdgcn_readfirstlane(s_node[w][sp - 1]);
-        const int nb2 = sp > 1 ? __builtin_amdgcn_readfirstlane(s_node[w][sp - 2]) : -1;
-        sp -= nb2 >= 0 ? 2 : 1;
-        const int nka = tv.nkids[na], fa = tv.nfirst[na];
-        const int nkb = nb2 >= 0 ? tv.nkids[nb2] : 0, fb = nb2 >= 0 ? tv.nfirst[nb2] : 0;
-        // lane l works on row r = l % 16 (child r % 8 of node r / 8); the
-        // lanes of one row hold its inputs 4g..4g+3 (g = l / 16)
+        const int ncand = min(sp, 8);
+        int kc = 0, kf = 0;
+        if (lane < ncand) {
+            const int ent = s_node[w][sp - 1 - lane];
+            kc = ent >> kNodeBits;
+            kf = ent & ((1 << kNodeBits) - 1);
+        }
+        // rows [rb[j], rb[j] + children) belong to popped entry j
+        int rb[8], rf[8];
+        int m = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int cj = __builtin_amdgcn_readlane(kc, j);
+            rf[j] = __builtin_amdgcn_readlane(kf, j);
+            const bool take = j < ncand && j == m && tot + cj <= 16;
+            rb[j] = take ? tot : (1 << 20);
+            if (take) {
+                tot += cj;
+                ++m;
+            }
+        }
+        sp -= m;
+        // lane l works on row r = l % 16; the lanes of one row hold its
+        // inputs 4g..4g+3 (g = l / 16)
         const int row = lane & 15, g = lane >> 4;
-        const int rk = row & 7;
-        const bool has = row < 8 ? rk < nka : rk < nkb;
-        const int chr = (row < 8 ? fa : fb) + rk;
+        int base = rb[0], firstc = rf[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+            if (rb[j] <= row) {
+                base = rb[j];
+                firstc = rf[j];
+            }
+        const bool has = row < tot;
+        const int chr = firstc + row - base;
         bool prop = false;
         float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+        // every node-table word of the child is loaded here, in one round
+        int ckids = 0, cfirst = 0, cb = 0, ce = 0;
         if (has) {
+            ckids = tv.nkids[chr];
+            cfirst = tv.nfirst[chr];
+            cb = tv.nbeg[chr];
+            ce = tv.nend[chr];
             const double4 cc = tv.ncent[chr];
             const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
             const double sd = sqrt(dx * dx + dy * dy + dz * dz);
@@ -378,7 +415,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
                 float fe[kIn];
-                pair_features(fe, rn, bt.r, sd, tv.nstat[chr], tv.nkids[chr] == 0, bt.nvalid, u_ref);
+                pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
 #pragma unroll
                 for (int k = 0; k < kIn; ++k) fe[k] = (fe[k] - norm.mean[k]) * norm.inv_std[k];
                 x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
@@ -394,8 +431,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
             if (prop && q > 0 && q <= max_order) p = min(q + margin, max_order);
         }
         const bool mine = lane < 16 && has;
-        const int cb = mine ? tv.nbeg[chr] : 0, ce = mine ? tv.nend[chr] : 0;
-        const bool leaf = mine && tv.nkids[chr] == 0;
+        const bool leaf = mine && ckids == 0;
         const bool is_lo = mine && p > 0 && p <= kLowOrder, is_hi = mine && p > kLowOrder;
         const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
         const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi);
@@ -409,7 +445,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
             if (is_hi) far[ls.far_cap - 1 - nhi - __popcll(m_hi & below)] = e;
         }
         if (nnear + c_near <= ls.near_cap && is_near) near[nnear + __popcll(m_near & below)] = chr;
-        if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = chr;
+        if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = stack_entry(cfirst, ckids);
         nfar += c_lo;
         nhi += c_hi;
         nnear += c_near;
