@@ -29,6 +29,13 @@ struct Model;  // order model state (mlp.hip)
 // counters of the last tree evaluation: far (target, node) pairs, near
 // (target, source) pairs, far-field series terms, order-model calls
 constexpr int kStats = 6;
+// The counters are kept in kStatSlots copies, one 128-byte line each (block b
+// adds to copy b % kStatSlots); readers sum the copies.  Tens of thousands of
+// waves flush their counts at the end of a launch, which on one line queue
+// behind each other.
+constexpr int kStatSlots = 32;
+constexpr int kStatStride = 16;  // 64-bit words per copy
+constexpr size_t kStatBytes = size_t(kStatSlots) * kStatStride * sizeof(unsigned long long);
 
 }  // namespace lpm
 
@@ -65,6 +72,9 @@ constexpr int kPinWords = 256;
 namespace lpm {
 
 int fail(lpm_ctx* c, int status, const char* fmt, ...);
+
+// the kStats evaluation counters summed over their copies (waits for the stream)
+int read_stats(lpm_ctx* c, unsigned long long* out);
 
 // bracket the main kernel of an entry point with the context's timing events
 inline void mark_begin(lpm_ctx* c) {

@@ -18,9 +18,13 @@ constexpr int kLvWords = 2 * (kMaxDepth + 2);
 constexpr int kLvOutside = kLvWords;      // a source lies outside the root box
 constexpr int kLvTooDeep = kLvWords + 1;  // a node at depth kMaxDepth must split
 constexpr int kLvKeys = kLvWords + 2;     // a node below the sorted key levels must split
-constexpr int kLvLeaves = kLvWords + 3;   // number of leaves
-constexpr int kLvChunks = kLvWords + 4;   // leaf chunks of kChunk particles
-constexpr int kLvTotal = kLvWords + 5;
+// leaves and leaf chunks of kChunk particles, each counted in kLvSpread
+// words (block b adds to word b % kLvSpread; the host sums them) so the
+// thousands of leaves of a deep level do not queue on one address
+constexpr int kLvSpread = 64;
+constexpr int kLvLeaves = kLvWords + 3;
+constexpr int kLvChunks = kLvLeaves + kLvSpread;
+constexpr int kLvTotal = kLvChunks + kLvSpread;
 
 struct Tree {
     int64_t ns = 0;       // sources

@@ -81,8 +81,7 @@ __global__ void tree_root(int n, int* nbeg, int* nend, int* nlevel, int* nparent
     lvt[1] = 1;
     lvt[kLvTooDeep] = 0;
     lvt[kLvKeys] = 0;
-    lvt[kLvLeaves] = 0;
-    lvt[kLvChunks] = 0;
+    for (int w = 0; w < 2 * kLvSpread; ++w) lvt[kLvLeaves + w] = 0;
 }
 
 // The child intervals of node [b, e) at `level`, found by the 8 lanes of an
@@ -131,14 +130,26 @@ __global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ k
     int lo, hi;
     child_interval(keys, b, e, level, search, lo, hi);
     const unsigned grp = unsigned(__ballot(search && hi > lo) >> (threadIdx.x & 56)) & 0xffu;
-    if (j != 0) return;
     const int c = __popc(grp);
-    cnt[t] = c;
-    nfirst[i] = -1;
-    nkids[i] = 0;
-    if (c == 0) {
-        atomicAdd(lvt + kLvLeaves, 1);
-        atomicAdd(lvt + kLvChunks, (e - b + kChunk - 1) / kChunk);
+    if (j == 0) {
+        cnt[t] = c;
+        nfirst[i] = -1;
+        nkids[i] = 0;
+    }
+    // leaves and their moment chunks, counted over the wave's active lanes by
+    // ballots before one atomic each (the deep levels have thousands of
+    // leaves on the same two words); a leaf of several chunks (leaf_max >
+    // kChunk) adds the chunks beyond its first itself
+    const bool is_leaf = j == 0 && c == 0;
+    const uint64_t lm = __ballot(is_leaf);
+    if (lm == 0) return;
+    const int nch = is_leaf ? (e - b + kChunk - 1) / kChunk : 0;
+    const uint64_t cm = __ballot(nch > 0);
+    const int slot = blockIdx.x % kLvSpread;
+    if (nch > 1) atomicAdd(lvt + kLvChunks + slot, nch - 1);
+    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(true))) {
+        atomicAdd(lvt + kLvLeaves + slot, __popcll(lm));
+        atomicAdd(lvt + kLvChunks + slot, __popcll(cm));
     }
 }
 
@@ -347,6 +358,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
     return LPM_OK;
 }
 
+static_assert(kLvTotal <= kPinWords, "the build table is read through the pinned words");
 constexpr int kFastLevels = 12;  // key levels sorted before the first level loop
 constexpr int kLevelGroup = 4;   // levels built between two reads of the build table
 
@@ -462,8 +474,12 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     t->level_start.push_back(nn);
     t->nn = nn;
     t->depth = depth;
-    t->nleaves = h[kLvLeaves];
-    t->nch = h[kLvChunks];
+    t->nleaves = 0;
+    t->nch = 0;
+    for (int w = 0; w < kLvSpread; ++w) {
+        t->nleaves += h[kLvLeaves + w];
+        t->nch += h[kLvChunks + w];
+    }
 
     LPM_CHECK(t->ncent.reserve(size_t(nn) * sizeof(double4)));
     LPM_CHECK(t->nbox.reserve(size_t(nn) * 6 * sizeof(double)));

@@ -507,8 +507,9 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
     for (int o = 32; o > 0; o >>= 1) npairs += __shfl_xor(npairs, o, 64);
     if (lane == 0) {
-        atomicAdd(mac_total, tot);
-        atomicAdd(mac_total + 1, npairs);
+        unsigned long long* m = mac_total + (blockIdx.x % kStatSlots) * kStatStride;
+        atomicAdd(m, tot);
+        atomicAdd(m + 1, npairs);
     }
 }
 
@@ -604,13 +605,14 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     const int64_t na = std::max<int64_t>(nt, 1);
-    LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + 64));
+    LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + kStatBytes + 128));
     uint64_t* k_in = c->ds_tgt.as<uint64_t>();
     uint64_t* k_out = k_in + na;
     int* v_in = reinterpret_cast<int*>(k_out + na);
     int* v_out = v_in + na;
-    c->stats_ptr = reinterpret_cast<unsigned long long*>(v_out + na);
-    LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
+    const uintptr_t sa = (reinterpret_cast<uintptr_t>(v_out + na) + 127) & ~uintptr_t(127);
+    c->stats_ptr = reinterpret_cast<unsigned long long*>(sa);
+    LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
     *tperm = v_out;
     if (nt == 0) return LPM_OK;
     if (nt == t->ns && tx == t->sx && ty == t->sy && tz == t->sz) {
@@ -681,6 +683,17 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     return LPM_OK;
 }
 
+int read_stats(lpm_ctx* c, unsigned long long* out) {
+    unsigned long long h[kStatSlots * kStatStride];
+    LPM_HIP(c, hipMemcpyAsync(h, c->stats_ptr, kStatBytes, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < kStats; ++k) {
+        out[k] = 0;
+        for (int s = 0; s < kStatSlots; ++s) out[k] += h[s * kStatStride + k];
+    }
+    return LPM_OK;
+}
+
 int read_batch_count(lpm_ctx* c, const int* nb_dev, int64_t* nb) {
     LPM_HIP(c, hipMemcpyAsync(c->pin, nb_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
@@ -728,10 +741,9 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
     mark_end(c);
     LPM_HIP(c, hipGetLastError());
     if (mac_out) {
-        unsigned long long h = 0;
-        LPM_HIP(c, hipMemcpyAsync(&h, d_mac, sizeof(h), hipMemcpyDeviceToHost, st));
-        LPM_HIP(c, hipStreamSynchronize(st));
-        *mac_out = int64_t(h);
+        unsigned long long h[kStats];
+        LPM_CHECK(read_stats(c, h));
+        *mac_out = int64_t(h[0]);
     }
     return LPM_OK;
 }
@@ -791,10 +803,9 @@ int lpm_tree_moments_psi(lpm_ctx* c, const double* gamma, int p) {
 int lpm_tree_last_stats(lpm_ctx* c, int64_t* far_pairs, int64_t* near_pairs) {
     if (!c) return LPM_ERR_NULL_PTR;
     if (!c->stats_ptr) return fail(c, LPM_ERR_STATE, "no tree evaluation yet");
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[kStats];
     LPM_HIP(c, hipSetDevice(c->device));
-    LPM_HIP(c, hipMemcpyAsync(h, c->stats_ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    LPM_CHECK(read_stats(c, h));
     if (far_pairs) *far_pairs = int64_t(h[0]);
     if (near_pairs) *near_pairs = int64_t(h[1]);
     return LPM_OK;

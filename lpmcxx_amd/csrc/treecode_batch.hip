@@ -237,6 +237,7 @@ __device__ inline void flush_stats(unsigned long long* stats, unsigned long long
                                    unsigned long long terms, unsigned long long calls,
                                    unsigned long long far_hi = 0, unsigned long long terms_hi = 0) {
     if ((threadIdx.x & 63) == 0) {
+        stats += (blockIdx.x % kStatSlots) * kStatStride;
         atomicAdd(stats + 0, far);
         atomicAdd(stats + 1, near);
         atomicAdd(stats + 2, terms);
@@ -852,7 +853,7 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
         LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 3 * sizeof(int), st));
-        if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStats * sizeof(unsigned long long), st));
+        if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
         hipLaunchKernelGGL(tree_plan_learned, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin, max_order, u_ref,
                            w, norm, ls, c->stats_ptr);
         auto lists = [&](auto kern) {
@@ -1101,8 +1102,7 @@ int lpm_tree_stats(lpm_ctx* c, int64_t* out) {
     if (!c || !out) return LPM_ERR_NULL_PTR;
     if (!c->stats_ptr) return fail(c, LPM_ERR_STATE, "no tree evaluation yet");
     unsigned long long h[kStats];
-    LPM_HIP(c, hipMemcpyAsync(h, c->stats_ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    LPM_CHECK(read_stats(c, h));
     for (int k = 0; k < kStats; ++k) out[k] = int64_t(h[k]);
     return LPM_OK;
 }
