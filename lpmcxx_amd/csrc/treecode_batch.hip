@@ -126,11 +126,22 @@ __device__ __forceinline__ void far_switch_scaled(std::integer_sequence<int, Ps.
     ((p == Ps + 1 ? (far_cartesian_scaled<Ps + 1>(d0, d1, d2, dd, rho, mm, S0, S1, S2), 0) : 0), ...);
 }
 
-// Far field of node n at every lane's target: t x S, added to (s0, s1, s2).
+// s += t x v
+__device__ inline void cross_add(double t0, double t1, double t2, double v0, double v1, double v2, double& s0,
+                                 double& s1, double& s2) {
+    s0 += t1 * v2 - t2 * v1;
+    s1 += t2 * v0 - t0 * v2;
+    s2 += t0 * v1 - t1 * v0;
+}
+
+// Far field of node n at every lane's target.  FP64: t x S added to (s0, s1,
+// s2).  FP32: S added to (v0, v1, v2); t x v is formed once per target at the
+// end (t x sum S = sum t x S).
 template <int P, bool F32, bool RUNTIME_P>
 __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, double t1, double t2, const double4& c,
                                          double d2, const double* __restrict__ momg,
-                                         const float* __restrict__ mom32, double& s0, double& s1, double& s2) {
+                                         const float* __restrict__ mom32, double& s0, double& s1, double& s2,
+                                         double& v0, double& v1, double& v2) {
     double S0, S1, S2;
     if constexpr (F32) {
         float A0 = 0.f, A1 = 0.f, A2 = 0.f;
@@ -140,7 +151,10 @@ __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, dou
         if constexpr (RUNTIME_P)
             far_switch_scaled(std::make_integer_sequence<int, P>{}, p, e0, e1, e2, dd, rho, mm, A0, A1, A2);
         else far_cartesian_scaled<P>(e0, e1, e2, dd, rho, mm, A0, A1, A2);
-        S0 = A0; S1 = A1; S2 = A2;
+        v0 += A0;
+        v1 += A1;
+        v2 += A2;
+        return;
     } else {
         S0 = S1 = S2 = 0.0;
         const double* mm = momg + int64_t(n) * nterm * 3;
@@ -159,7 +173,8 @@ __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, dou
 template <bool F32>
 __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, double t2, const double4& c, double d2,
                                           float eps2v, const double4* __restrict__ spos,
-                                          const float4* __restrict__ soff, double& s0, double& s1, double& s2) {
+                                          const float4* __restrict__ soff, double& s0, double& s1, double& s2,
+                                          double& v0, double& v1, double& v2) {
     if constexpr (F32) {
         const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
@@ -174,11 +189,11 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
             a1 = fmaf(wgt, dy, a1);
             a2 = fmaf(wgt, dz, a2);
         }
-        // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2)
-        const double b0 = a0, b1 = a1, b2 = a2;
-        s0 -= 2.0 * (t1 * b2 - t2 * b1);
-        s1 -= 2.0 * (t2 * b0 - t0 * b2);
-        s2 -= 2.0 * (t0 * b1 - t1 * b0);
+        // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2): -2 d
+        // gamma / den joins v, crossed with t once per target
+        v0 -= 2.0 * double(a0);
+        v1 -= 2.0 * double(a1);
+        v2 -= 2.0 * double(a2);
     } else {
         for (int j = b; j < e; ++j) {
             const double4 s = spos[j];
@@ -262,7 +277,7 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
     double t0, t1, t2;
     load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0;
     unsigned long long nfar = 0, nnear = 0;
     constexpr unsigned long long NT = unsigned(P * (P + 1) * (P + 2) / 6);
     int sp = 1;
@@ -274,20 +289,21 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
         const double s = sqrt(dx * dx + dy * dy + dz * dz);
         const double rn = tv.nrad[n];
         if (uniform(rn + bt.r < theta * s)) {
-            far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2);
+            far_node<P, F32, false>(P, n, tv.nterm, t0, t1, t2, c, d2, tv.momg, tv.mom32, s0, s1, s2, v0, v1, v2);
             ++nfar;
             continue;
         }
         const int nk = tv.nkids[n];
         if (nk == 0) {
             const int b = tv.nbeg[n], e = tv.nend[n];
-            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2);
+            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1, v2);
             nnear += unsigned(e - b);
         } else {
             const int f = tv.nfirst[n];
             for (int q = nk - 1; q >= 0; --q) s_node[w][sp++] = f + q;
         }
     }
+    cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
     if (valid) {
         ux[orig] = s0 * mult;
         uy[orig] = s1 * mult;
@@ -328,7 +344,8 @@ struct Lists {
     int* overflow;  // [0] a batch overflowed, [1] far and [2] near capacity it needs
 };
 
-__global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
+// six waves per SIMD (77 VGPRs, no spills): the traversal waits on node loads
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
                                                          int max_order, double u_ref,
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
                                                          Lists ls, unsigned long long* __restrict__ stats) {
@@ -505,22 +522,23 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     int orig;
     double t0, t1, t2;
     const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0;
     if (LOW) {
         const int* near = ls.near + bi * ls.near_cap;
         const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
         for (int j = 0; j < nnear; ++j) {
             const int n = __builtin_amdgcn_readfirstlane(near[j]);
             near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1,
-                           s2);
+                           s2, v0, v1, v2);
         }
     }
     for (int j = j0; j < j1; ++j) {
         const int e = __builtin_amdgcn_readfirstlane(far[j]);
         const int n = e & ((1 << kNodeBits) - 1);
         far_node<MAXP, F32, true>(e >> kNodeBits, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0,
-                                  s1, s2);
+                                  s1, s2, v0, v1, v2);
     }
+    cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
     if (valid) {
         if (LOW) {
             acc[pos] = s0;
