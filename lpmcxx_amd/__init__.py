@@ -276,6 +276,20 @@ class Context:
     def synchronize(self) -> None:
         self.check(self.lib.lpm_synchronize(self.h))
 
+    def after_torch(self) -> None:
+        """Make the context stream wait for the work enqueued so far on torch's
+        current stream.  The context stream does not synchronize with torch's,
+        so device tensors just written by torch (a clone, a copy, an
+        elementwise op) would otherwise be read before they are complete.
+        Every *_dev wrapper of this package calls it first; it enqueues an
+        event wait and does not block the host."""
+        import torch
+        ext = getattr(self, "_torch_ext", None)
+        if ext is None:
+            ext = torch.cuda.ExternalStream(self.stream, device=torch.device("cuda", self.device))
+            self._torch_ext = ext
+        ext.wait_stream(torch.cuda.current_stream(self.device))
+
     def direct_sum(self, targets, sources, gamma, *, kernels=LPM_K_BIOT_SAVART_SPHERE, prec=LPM_F64,
                    radius=1.0, delta=0.0, skip=None):
         """Host-array direct sum; returns (ux, uy, uz) and/or psi as float64 arrays."""
@@ -301,6 +315,7 @@ class Context:
                        prec=LPM_F64, radius=1.0, delta=0.0, skip=None, psi=None):
         """Device-tensor direct sum enqueued on the context stream (no synchronize)."""
         ux, uy, uz = out if out is not None else (None, None, None)
+        self.after_torch()
         self.check(self.lib.lpm_direct_sum_dev(self.h, kernels, prec, radius, delta, _ptr(tx), _ptr(ty), _ptr(tz),
                                                tx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(gamma), sx.numel(),
                                                _ptr(skip), _ptr(ux), _ptr(uy), _ptr(uz), _ptr(psi)))
@@ -318,6 +333,7 @@ def _rk4_steps(self, cfg, dt, omega, nsteps, x, y, z, zeta_abs, area):
 
 def _rk4_step_dev(self, cfg, dt, omega, x, y, z, zeta_abs, area):
     """One RK4 step on device tensors, in place, enqueued on the context stream."""
+    self.after_torch()
     self.check(self.lib.lpm_rk4_step_dev(self.h, C.byref(cfg), dt, omega, x.numel(), _ptr(x), _ptr(y), _ptr(z),
                                          _ptr(zeta_abs), _ptr(area)))
 
@@ -342,6 +358,7 @@ def _bve_steps(self, cfg, dt, omega, nsteps, src, zeta_abs, area, tracers=None):
 def _bve_step_dev(self, cfg, dt, omega, n_src, x, y, z, zeta_abs, area):
     """One BVE step on device tensors (first n_src entries are vortex particles,
     the rest tracers), in place, enqueued on the context stream."""
+    self.after_torch()
     self.check(self.lib.lpm_bve_step_dev(self.h, C.byref(cfg), dt, omega, n_src, x.numel() - n_src, _ptr(x), _ptr(y),
                                          _ptr(z), _ptr(zeta_abs), _ptr(area)))
 
@@ -395,6 +412,7 @@ def _mesh_generate_dev(self, seed, level, radius=1.0, leaf_vertices=False, out=N
         lv = out.get("leaf_vertices")
         if lv is not None and (lv.numel() != nl * nepf or lv.dtype != torch.int64):
             raise ValueError("out['leaf_vertices'] does not match this seed and level")
+    self.after_torch()
     self.check(self.lib.lpm_mesh_generate_dev(self.h, seed, level, radius, *(_ptr(out[k]) for k in
                                               ("vx", "vy", "vz", "cx", "cy", "cz", "area", "leaf_vertices"))))
     return out

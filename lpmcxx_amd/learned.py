@@ -169,6 +169,7 @@ def evaluate(ctx: Context, targets, theta_max: float, margin: int, max_order: in
 
 def evaluate_dev(ctx: Context, tx, ty, tz, out, theta_max: float, margin: int, max_order: int, u_ref: float,
                  delta: float = 0.0, prec: int = LPM_F32C) -> None:
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_learned_dev(ctx.h, prec, theta_max, margin, max_order, u_ref, delta, _ptr(tx),
                                                 _ptr(ty), _ptr(tz), tx.numel(), _ptr(out[0]), _ptr(out[1]),
                                                 _ptr(out[2])))

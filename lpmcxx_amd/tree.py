@@ -45,6 +45,7 @@ def build_box(ctx: Context, sources, box_min, box_max, control: int, param: int)
 
 def build_dev(ctx: Context, sx, sy, sz, max_radius: float, leaf_max: int):
     nn, depth = C.c_int64(), C.c_int()
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_build_dev(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.numel(), max_radius, leaf_max,
                                          C.byref(nn), C.byref(depth)))
     return nn.value, depth.value
@@ -56,6 +57,7 @@ def moments(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
 
 
 def moments_dev(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_moments_dev(ctx.h, _ptr(gamma), order, moment_sign))
 
 
@@ -73,6 +75,7 @@ def evaluate(ctx: Context, targets, theta: float, delta: float = 0.0, prec: int 
 def evaluate_dev(ctx: Context, tx, ty, tz, out, theta: float, delta: float = 0.0, prec: int = LPM_F64,
                  series: int = LPM_SERIES_CARTESIAN, want_mac: bool = False):
     mac = C.c_int64()
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_dev(ctx.h, prec, series, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
                                         _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
                                         C.byref(mac) if want_mac else None))
@@ -113,6 +116,7 @@ def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float
 
 def evaluate_batch_dev(ctx: Context, tx, ty, tz, out, theta: float, order: int, delta: float = 0.0,
                        prec: int = LPM_F64) -> None:
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_batch_dev(ctx.h, prec, order, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz),
                                               tx.numel(), _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
 
@@ -124,6 +128,7 @@ def moments_psi(ctx: Context, gamma, order: int) -> None:
 
 
 def moments_psi_dev(ctx: Context, gamma, order: int) -> None:
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_moments_psi_dev(ctx.h, _ptr(gamma), order))
 
 
@@ -137,6 +142,7 @@ def evaluate_psi(ctx: Context, targets, theta: float, order: int, prec: int = LP
 
 
 def evaluate_psi_dev(ctx: Context, tx, ty, tz, psi, theta: float, order: int, prec: int = LPM_F64) -> None:
+    ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_psi_dev(ctx.h, prec, order, theta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
                                             _ptr(psi)))
 

@@ -179,3 +179,40 @@ def test_point_cloud_tree(ctx, control, param):
     assert np.array_equal(G["box"][gi], R["box"][ri]) and np.array_equal(G["perm"], R["perm"])
     if control == 1:
         assert depth == param
+
+
+def test_targets_that_are_the_sources_reuse_the_key_order(ctx, mesh):
+    """Evaluating at the tree's own source arrays reuses the build's key order
+    (no target sort); the same positions in other arrays are sorted on 12 key
+    levels, which is the same stable sort: every method gives identical
+    results either way, and the same evaluation counts."""
+    import torch
+    from lpmcxx_amd import learned as LN
+    m = mesh(L.LPM_SEED_CUBED, 6)
+    g = L.random_vortices(m.cx, m.cy, m.cz, m.area, n=4, seed=3) * m.area
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    S = [dev(a) for a in (m.cx, m.cy, m.cz)]
+    C = [a.clone() for a in S]   # written on torch's stream: the wrappers order the context stream after it
+    G = dev(g)
+    T.build_dev(ctx, *S, m.max_radius, 64)
+    T.moments_dev(ctx, G, 8, 1)
+    LN.constant_model(ctx, 6)
+    u_ref = LN.u_ref_of(g)
+
+    def run(tg, method):
+        out = [torch.empty_like(S[0]) for _ in range(3)]
+        if method == "tree":
+            T.evaluate_dev(ctx, *tg, out, 0.5, prec=L.LPM_F64)
+        elif method == "batch64":
+            T.evaluate_batch_dev(ctx, *tg, out, 0.5, 8, prec=L.LPM_F64)
+        elif method == "batch32":
+            T.evaluate_batch_dev(ctx, *tg, out, 0.5, 8, prec=L.LPM_F32C)
+        else:
+            LN.evaluate_dev(ctx, *tg, out, 0.6, 0, 8, u_ref, prec=L.LPM_F32C)
+        ctx.synchronize()
+        return [a.cpu().numpy() for a in out], T.stats(ctx)
+
+    for method in ("tree", "batch64", "batch32", "learned"):
+        (a, sa), (b, sb) = run(S, method), run(C, method)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), method
+        assert sa == sb, method
