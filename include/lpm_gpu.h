@@ -241,24 +241,35 @@ int lpm_tree_eval_batch_dev(lpm_ctx* ctx, int prec, int order, double theta, dou
  * accumulation on the MFMA units; class 0 = descend, class q = accept with
  * order q.  Parameters are a flat FP32 array of LPM_MODEL_NPARAMS values:
  * W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16] (row = input).
- * Inputs are normalized by (x - mean) * inv_std (16 values each). */
+ * Inputs are clamped to the model's input range [lo, hi] and normalized by
+ * (x - mean) * inv_std (16 values each), in lpm_model_forward, the gradient
+ * and training calls and the treecode's planning pass alike. */
 #define LPM_MODEL_NPARAMS 6288
 #define LPM_MODEL_INPUTS 16
 #define LPM_MODEL_CLASSES 13
 #define LPM_LABEL_ORDERS 12
-/* He-initialized weights from `seed`, zero biases, identity normalization. */
+/* He-initialized weights from `seed`, zero biases, identity normalization,
+ * unbounded input range. */
 int lpm_model_init(lpm_ctx* ctx, uint64_t seed);
 /* Set the FP32 master parameters and the normalization (mean / inv_std may be
- * NULL: 0 / 1); resets the Adam state. */
+ * NULL: 0 / 1); resets the Adam state and the input range (unbounded). */
 int lpm_model_set_params(lpm_ctx* ctx, const float* params, const float* mean, const float* inv_std);
 int lpm_model_get_params(lpm_ctx* ctx, float* params, float* mean, float* inv_std);
+/* Input range of the model (16 values each; lo and hi both NULL: unbounded).
+ * lpm_model_train sets it to the per-input minimum and maximum of its rows,
+ * so that inputs the model never saw in training (a one-target batch of
+ * radius 0, a node of zero circulation) are answered as the nearest inputs it
+ * did see.  Errors: lo > hi or NaN (LPM_ERR_INVALID_ARG). */
+int lpm_model_set_range(lpm_ctx* ctx, const float* lo, const float* hi);
+int lpm_model_get_range(lpm_ctx* ctx, float* lo, float* hi);
 /* Gradient of the mean softmax cross entropy over n rows (x[n][16] raw
  * inputs, y[n] classes) with respect to the parameters, computed by the
  * forward and backward MFMA kernels; loss may be NULL. */
 int lpm_model_gradient(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n, float* grad, float* loss);
 /* Adam training (beta 0.9 / 0.999, eps 1e-8) for `epochs` shuffled passes in
  * mini-batches of `batch` rows; final_loss (may be NULL) is the mean loss of
- * the last epoch. */
+ * the last epoch.  Sets the input range to that of the rows (which must be
+ * finite). */
 int lpm_model_train(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n, int epochs, int batch,
                     float lr, uint64_t seed, float* final_loss);
 /* Logits [n][16] and argmax classes [n] (either may be NULL). */

@@ -138,6 +138,8 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_model_init": [_vp, C.c_uint64],
         "lpm_model_set_params": [_vp, _vp, _vp, _vp],
         "lpm_model_get_params": [_vp, _vp, _vp, _vp],
+        "lpm_model_set_range": [_vp, _vp, _vp],
+        "lpm_model_get_range": [_vp, _vp, _vp],
         "lpm_model_gradient": [_vp, _vp, _vp, C.c_int64, _vp, _vp],
         "lpm_model_train": [_vp, _vp, _vp, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint64, _vp],
         "lpm_model_forward": [_vp, _vp, C.c_int64, _vp, _vp],

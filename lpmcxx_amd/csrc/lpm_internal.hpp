@@ -139,6 +139,6 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
 
 void model_free(Model* m);
 const unsigned short* model_weights(lpm_ctx* c);  // device BF16 weights + FP32 biases, or nullptr
-const float* model_norm(lpm_ctx* c, int which);   // host feature mean (0) / inverse std (1)
+const float* model_norm(lpm_ctx* c, int which);   // host feature mean (0), inverse std (1), range lo (2), hi (3)
 
 }  // namespace lpm

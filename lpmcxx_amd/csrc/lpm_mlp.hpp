@@ -20,6 +20,8 @@ struct Model {
     bool ready = false;
     int step = 0;
     float mean[kIn] = {}, inv_std[kIn] = {};
+    // input range: raw inputs are clamped to [lo, hi] before normalization
+    float lo[kIn], hi[kIn];
     DevBuf master;   // float [kNParams]
     DevBuf adam_m, adam_v;
     DevBuf wbf;      // ushort [kBfWeights] + float biases [64 + 64 + 16] (+ pad)
@@ -27,10 +29,16 @@ struct Model {
     DevBuf data;     // training rows / labels
 };
 
-// feature normalization passed to the kernels by value: x' = (x - mean) * inv_std
+// Unbounded input range (a model without a recorded training range).
+constexpr float kRangeMax = 3.0e38f;
+
+// feature normalization passed to the kernels by value:
+// x' = (min(max(x, lo), hi) - mean) * inv_std
 struct MlpNorm {
     float mean[kIn];
     float inv_std[kIn];
+    float lo[kIn];
+    float hi[kIn];
 };
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));

@@ -322,7 +322,12 @@ const unsigned short* model_weights(lpm_ctx* c) {
     return (c->model && c->model->ready) ? c->model->wbf.as<unsigned short>() : nullptr;
 }
 const float* model_norm(lpm_ctx* c, int which) {
-    return which == 0 ? c->model->mean : c->model->inv_std;
+    const Model& M = *c->model;
+    return which == 0 ? M.mean : which == 1 ? M.inv_std : which == 2 ? M.lo : M.hi;
+}
+
+static void reset_range(Model& M) {
+    for (int i = 0; i < kIn; ++i) { M.lo[i] = -kRangeMax; M.hi[i] = kRangeMax; }
 }
 
 static int upload_params(lpm_ctx* c, Model& M, const float* params) {
@@ -365,6 +370,7 @@ int lpm_model_init(lpm_ctx* c, uint64_t seed) {
     he(kOffW2, kHid, kHid * kHid);
     he(kOffW3, kHid, kHid * kOut);
     for (int i = 0; i < kIn; ++i) { M.mean[i] = 0.f; M.inv_std[i] = 1.f; }
+    reset_range(M);
     return upload_params(c, M, p.data());
 }
 
@@ -376,7 +382,30 @@ int lpm_model_set_params(lpm_ctx* c, const float* params, const float* mean, con
         M.mean[i] = mean ? mean[i] : 0.f;
         M.inv_std[i] = inv_std ? inv_std[i] : 1.f;
     }
+    reset_range(M);
     return upload_params(c, M, params);
+}
+
+int lpm_model_set_range(lpm_ctx* c, const float* lo, const float* hi) {
+    if (!c) return LPM_ERR_NULL_PTR;
+    if (!c->model || !c->model->ready) return fail(c, LPM_ERR_STATE, "no model");
+    if (!lo != !hi) return fail(c, LPM_ERR_NULL_PTR, "lo and hi are both given or both NULL");
+    Model& M = *c->model;
+    if (!lo) {
+        reset_range(M);
+        return LPM_OK;
+    }
+    for (int i = 0; i < kIn; ++i)
+        if (!(lo[i] <= hi[i])) return fail(c, LPM_ERR_INVALID_ARG, "input range with lo > hi or NaN");
+    for (int i = 0; i < kIn; ++i) { M.lo[i] = lo[i]; M.hi[i] = hi[i]; }
+    return LPM_OK;
+}
+
+int lpm_model_get_range(lpm_ctx* c, float* lo, float* hi) {
+    if (!c || !lo || !hi) return LPM_ERR_NULL_PTR;
+    if (!c->model || !c->model->ready) return fail(c, LPM_ERR_STATE, "no model");
+    for (int i = 0; i < kIn; ++i) { lo[i] = c->model->lo[i]; hi[i] = c->model->hi[i]; }
+    return LPM_OK;
 }
 
 int lpm_model_get_params(lpm_ctx* c, float* params, float* mean, float* inv_std) {
@@ -393,12 +422,14 @@ int lpm_model_get_params(lpm_ctx* c, float* params, float* mean, float* inv_std)
     return LPM_OK;
 }
 
-// Rows x[n][16] are raw features: the model's normalization (x - mean) * inv_std
-// is applied on upload.
+// Rows x[n][16] are raw features: the model's clamp to its input range and
+// normalization (x - mean) * inv_std are applied on upload, as in the
+// treecode's planning pass.
 static int upload_rows(lpm_ctx* c, Model& M, const float* x, const int32_t* y, int64_t n) {
     std::vector<float> xn(size_t(n) * kIn);
     for (int64_t r = 0; r < n; ++r)
-        for (int k = 0; k < kIn; ++k) xn[r * kIn + k] = (x[r * kIn + k] - M.mean[k]) * M.inv_std[k];
+        for (int k = 0; k < kIn; ++k)
+            xn[r * kIn + k] = (std::min(std::max(x[r * kIn + k], M.lo[k]), M.hi[k]) - M.mean[k]) * M.inv_std[k];
     LPM_CHECK(M.data.reserve(size_t(n) * (kIn * sizeof(float) + sizeof(int32_t)) + 64));
     float* dx = M.data.as<float>();
     int* dy = reinterpret_cast<int*>(dx + size_t(n) * kIn);
@@ -442,8 +473,17 @@ int lpm_model_train(lpm_ctx* c, const float* x, const int32_t* y, int64_t n, int
     if (n < 1 || epochs < 0 || batch < 1 || !(lr > 0.f)) return fail(c, LPM_ERR_INVALID_ARG, "bad training args");
     for (int64_t r = 0; r < n; ++r)
         if (y[r] < 0 || y[r] >= kClasses) return fail(c, LPM_ERR_INVALID_ARG, "label out of range");
+    for (int64_t i = 0; i < n * kIn; ++i)
+        if (!std::isfinite(x[i])) return fail(c, LPM_ERR_INVALID_ARG, "non-finite input");
     LPM_HIP(c, hipSetDevice(c->device));
     Model& M = *c->model;
+    // the input range becomes that of the training rows
+    for (int k = 0; k < kIn; ++k) { M.lo[k] = x[k]; M.hi[k] = x[k]; }
+    for (int64_t r = 1; r < n; ++r)
+        for (int k = 0; k < kIn; ++k) {
+            M.lo[k] = std::min(M.lo[k], x[r * kIn + k]);
+            M.hi[k] = std::max(M.hi[k], x[r * kIn + k]);
+        }
     std::vector<int64_t> perm(n);
     std::iota(perm.begin(), perm.end(), 0);
     std::mt19937_64 rng(seed);

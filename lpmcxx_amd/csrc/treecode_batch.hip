@@ -435,7 +435,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
                 float fe[kIn];
                 pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
 #pragma unroll
-                for (int k = 0; k < kIn; ++k) fe[k] = (fe[k] - norm.mean[k]) * norm.inv_std[k];
+                for (int k = 0; k < kIn; ++k)
+                    fe[k] = (fminf(fmaxf(fe[k], norm.lo[k]), norm.hi[k]) - norm.mean[k]) * norm.inv_std[k];
                 x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
                 x1 = g == 0 ? fe[1] : g == 1 ? fe[5] : g == 2 ? fe[9] : fe[13];
                 x2 = g == 0 ? fe[2] : g == 1 ? fe[6] : g == 2 ? fe[10] : fe[14];
@@ -848,6 +849,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     for (int k = 0; k < kIn; ++k) {
         norm.mean[k] = model_norm(c, 0)[k];
         norm.inv_std[k] = model_norm(c, 1)[k];
+        norm.lo[k] = model_norm(c, 2)[k];
+        norm.hi[k] = model_norm(c, 3)[k];
     }
     const unsigned nb = unsigned((nbatch + 3) / 4);
     const double d2 = delta * delta, mult = -1.0 / (4.0 * kPi);

@@ -75,6 +75,19 @@ class OrderModel:
         self.ctx.check(self.ctx.lib.lpm_model_get_params(self.ctx.h, _ptr(p), _ptr(m), _ptr(s)))
         return p, m, s
 
+    def set_range(self, lo=None, hi=None) -> None:
+        """Clamp inputs to [lo, hi] (None: unbounded); training sets the range
+        of its rows."""
+        a = None if lo is None else np.ascontiguousarray(lo, np.float32)
+        b = None if hi is None else np.ascontiguousarray(hi, np.float32)
+        assert (a is None) == (b is None) and (a is None or a.size == b.size == NIN)
+        self.ctx.check(self.ctx.lib.lpm_model_set_range(self.ctx.h, _ptr(a), _ptr(b)))
+
+    def get_range(self):
+        lo, hi = np.empty(NIN, np.float32), np.empty(NIN, np.float32)
+        self.ctx.check(self.ctx.lib.lpm_model_get_range(self.ctx.h, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
     def gradient(self, x, y):
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.int32)
@@ -196,6 +209,8 @@ class FitResult:
     accuracy: float
     mean: np.ndarray
     inv_std: np.ndarray
+    lo: np.ndarray        # input range recorded by training
+    hi: np.ndarray
 
 
 def fit(ctx: Context, ls: LabelSet, tau: float, max_order: int, *, epochs: int = 20, batch: int = 4096,
@@ -209,4 +224,5 @@ def fit(ctx: Context, ls: LabelSet, tau: float, max_order: int, *, epochs: int =
     m.set_params(p, mean, inv)
     loss = m.train(ls.features, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
     _, cls = m.forward(ls.features)
-    return FitResult(len(y), np.bincount(y, minlength=NCLASSES), loss, float((cls == y).mean()), mean, inv)
+    lo, hi = m.get_range()
+    return FitResult(len(y), np.bincount(y, minlength=NCLASSES), loss, float((cls == y).mean()), mean, inv, lo, hi)

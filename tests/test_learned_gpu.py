@@ -6,7 +6,7 @@ Bars: batch treecode (FP64) equal to the oracle's batch traversal within
 gradients of the BF16 MFMA kernels within 5e-3 (relative L2) of the oracle's
 BF16-emulating backward pass; labels within 1e-5 of the oracle's per-pair
 errors; the learned traversal reducing exactly to the plain one for a
-constant model.
+constant model; model inputs clamped to the range training recorded.
 """
 import numpy as np
 import pytest
@@ -163,6 +163,41 @@ def test_model_training_learns_rule(ctx):
     assert np.mean(np.abs(cls - y) <= 1) > 0.97
 
 
+def test_model_input_range(ctx):
+    """Training records the per-input range of its rows; forward clamps inputs
+    to it before normalizing; set_params and set_range(None) unbound it."""
+    rng = np.random.default_rng(8)
+    x = np.zeros((3000, 16), np.float32)
+    x[:, :10] = rng.normal(size=(3000, 10))
+    y = np.clip(np.floor(2.0 * x[:, 0] + 6.0), 0, 12).astype(np.int32)
+    mod = LN.OrderModel(ctx)
+    mod.init(1)
+    p, _, _ = mod.get_params()
+    mean, inv = LN.normalization(x)
+    mod.set_params(p, mean, inv)
+    lo, hi = mod.get_range()
+    assert (lo == -3e38).all() and (hi == 3e38).all()
+    mod.train(x, y, epochs=1, batch=1024, seed=2)
+    lo, hi = mod.get_range()
+    assert np.array_equal(lo, x.min(0)) and np.array_equal(hi, x.max(0))
+    xo = x[:200] * 40.0                              # far outside the training range
+    zo, co = mod.forward(xo)
+    zc, cc = mod.forward(np.clip(xo, lo, hi))
+    assert np.array_equal(zo, zc) and np.array_equal(co, cc)
+    mod.set_range(None)
+    zu, _ = mod.forward(xo)
+    assert not np.array_equal(zu, zo)
+    mod.set_range(lo, hi)
+    assert np.array_equal(mod.forward(xo)[0], zo)
+    with pytest.raises(L.LpmError):
+        mod.set_range(hi, lo)
+    with pytest.raises(L.LpmError):
+        mod.train(np.full((4, 16), np.nan, np.float32), np.zeros(4, np.int32), epochs=1)
+    p2, _, _ = mod.get_params()
+    mod.set_params(p2, mean, inv)
+    assert (mod.get_range()[0] == -3e38).all()
+
+
 # ---------------------------------------------------------------- labels and learned traversal
 def _radius(R, src, i):
     """Source radius of oracle node i: exact at a leaf, max over children of
@@ -258,6 +293,38 @@ def test_learned_constant_model_all_list_kernels(ctx, mesh, order):
         assert (sl["far"], sl["near"], sl["terms"]) == (sb["far"], sb["near"], sb["terms"])
         assert sl["far_high"] == (sl["far"] if order > 7 else 0)
         assert d < (1e-13 if prec == L.LPM_F64 else 1e-6)
+
+
+def test_learned_planning_clamps_inputs_to_the_model_range(ctx, mesh):
+    """A one-target batch has radius 0, so log2(r_b / s) is -39.9, far below
+    anything in training.  A hand-made model answers order 7 when
+    log2(r_b / s) > -20 and order 1 otherwise: with the input range
+    [-10, 10] the planning pass clamps the feature and the evaluation equals
+    the constant order-7 model bit for bit; unbounded, it takes order 1."""
+    m, src, g, _ = _field(mesh, 5, 64, 8)
+    _gpu_tree(ctx, m, src, g, 64, 8)
+    u_ref = LN.u_ref_of(g)
+    parts = {name: np.zeros(shape, np.float32) for name, shape in LN.LAYOUT}
+    parts["W1"][1, 0] = 1.0      # h1 = relu(log2(r_b / s) + 20)
+    parts["b1"][0] = 20.0
+    parts["W2"][0, 0] = 1.0
+    parts["W3"][0, 7] = 1.0      # logit 7 = h1, logit 1 = 1
+    parts["b3"][1] = 1.0
+    mod = LN.OrderModel(ctx)
+    for i in (0, 1000, 5000):
+        tgt = tuple(a[i:i + 1] for a in m.targets())
+        for prec in (L.LPM_F64, L.LPM_F32C):
+            LN.constant_model(ctx, 7)
+            u7 = np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=prec))
+            LN.constant_model(ctx, 1)
+            u1 = np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=prec))
+            assert not np.array_equal(u1, u7)
+            mod.set_params(LN.pack(parts))
+            ua = np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=prec))
+            assert np.array_equal(ua, u1)
+            mod.set_range(np.full(16, -10.0), np.full(16, 10.0))
+            uc = np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 8, u_ref, prec=prec))
+            assert np.array_equal(uc, u7)
 
 
 def test_learned_errors(ctx, mesh):
