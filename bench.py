@@ -386,8 +386,17 @@ def workload_bve(args) -> dict:
     s_b, P_b = run(cfg_b, args.steps, 1)
     P0 = torch.stack([X, Y, Z])
     traj = float(torch.linalg.norm(torch.stack(P_l) - torch.stack(P_b)) / torch.linalg.norm(torch.stack(P_b) - P0))
-    log(f"[bench] BVE: learned {1e3 * s_l:.1f} ms/step (t=0 err {err_l:.2e}), plain FP64 {1e3 * s_b:.1f} ms/step "
-        f"(t=0 err {err_b:.2e}), trajectory difference {traj:.2e}")
+    # one step from the same start by each solver against the FP32C direct sum's step
+    one = {}
+    for name, cfg in (("direct", L.SolverConfig.default(method=L.LPM_METHOD_DIRECT, prec=L.LPM_F32C)),
+                      ("learned", cfg_l), ("plain", cfg_b)):
+        _, P = run(cfg, 1, 0)
+        one[name] = torch.stack(P)
+    disp_err = lambda P: float(torch.linalg.norm(P - one["direct"]) / torch.linalg.norm(one["direct"] - P0))
+    step_err_l, step_err_b = disp_err(one["learned"]), disp_err(one["plain"])
+    log(f"[bench] BVE: learned {1e3 * s_l:.1f} ms/step (t=0 err {err_l:.2e}, one-step displacement err "
+        f"{step_err_l:.2e}), plain FP64 {1e3 * s_b:.1f} ms/step (t=0 err {err_b:.2e}, one-step {step_err_b:.2e}), "
+        f"trajectory difference {traj:.2e}")
     cpu = None
     if not args.no_cpu:
         src = (m.cx, m.cy, m.cz)
@@ -413,9 +422,12 @@ def workload_bve(args) -> dict:
                                "tracers), Omega 2 pi, dt 0.01",
                    "step": f"learned-order FP32C treecode per stage, leaf {args.leaf}, theta_max {args.theta_max}, "
                            f"tau {args.bve_tau}, max order {args.max_order}",
-                   "rel_l2_err_t0": err_l, "err_sample": "4096 random faces and vertices, FP64 direct sum"},
+                   "rel_l2_err_t0": err_l, "err_sample": "4096 random faces and vertices, FP64 direct sum",
+                   "step_displacement_rel_l2_err": step_err_l,
+                   "step_err_reference": "one RK4 step of all faces and vertices by the FP32C direct sum"},
         "roofline": roof, "cpu_baseline": cpu,
-        "plain_fp64_tree": {"ms_per_step": 1e3 * s_b, "rel_l2_err_t0": err_b, "speedup": s_b / s_l},
+        "plain_fp64_tree": {"ms_per_step": 1e3 * s_b, "rel_l2_err_t0": err_b, "speedup": s_b / s_l,
+                            "step_displacement_rel_l2_err": step_err_b},
         "trajectory_rel_diff_vs_plain_fp64": traj, "label_samples": int(len(ls.ids)),
         "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy, "stats_last_stage": st,
     }
@@ -730,8 +742,22 @@ def main() -> int:
         cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
                                        theta=args.base_theta, leaf_max=args.base_leaf)
         rl, rb = rk4_rate(cfg_l), rk4_rate(cfg_b)
+
+        # one step from the same start by each solver against the FP32C direct sum's step
+        def one_step(cfg):
+            P = [X.clone(), Y.clone(), Z.clone()]
+            ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, *P, za, A)
+            ctx.synchronize()
+            return torch.stack(P)
+
+        P0 = torch.stack([X, Y, Z])
+        Pd = one_step(L.SolverConfig.default(method=L.LPM_METHOD_DIRECT, prec=L.LPM_F32C))
+        disp_err = lambda P: float(torch.linalg.norm(P - Pd) / torch.linalg.norm(Pd - P0))
+        el, eb = disp_err(one_step(cfg_l)), disp_err(one_step(cfg_b))
+        log(f"[bench] RK4 one-step displacement error vs the FP32C direct sum: learned {el:.2e}, plain FP64 {eb:.2e}")
         rk4 = {"rk4_steps_per_s": rl, "rk4_ms_per_step": 1e3 / rl, "rk4_plain_fp64_steps_per_s": rb,
-               "rk4_speedup_vs_plain_fp64": rl / rb,
+               "rk4_speedup_vs_plain_fp64": rl / rb, "rk4_step_displacement_rel_l2_err": el,
+               "rk4_plain_fp64_step_displacement_rel_l2_err": eb,
                "rk4_solver": f"learned FP32C treecode (theta_max {args.theta_max}, max order {args.max_order}, "
                              f"leaf {args.leaf}), dt 0.01, Omega 2 pi, FP64 update and renormalization"}
         log(f"[bench] RK4: learned {1e3 / rl:.1f} ms/step, plain FP64 treecode {1e3 / rb:.1f} ms/step")
