@@ -1,5 +1,6 @@
-"""RK4 time step (lpm_rk4_*): parity with the oracle's FP64 RK4 and the
-Rossby-Haurwitz phase speed on the rotating sphere.
+"""RK4 time step (lpm_rk4_*): parity with the oracle's FP64 RK4, the
+Rossby-Haurwitz phase speed on the rotating sphere, and the learned treecode's
+step against the FP32C direct sum's under the solver's default root box.
 
 For zeta_rel = 2 alpha z + 30 cos(4 lambda) z (z^2 - 1)^2 (solid body plus the
 degree-5 wavenumber-4 Rossby-Haurwitz wave of src/LpmAnalyticFunctions.h:87-120)
@@ -55,3 +56,42 @@ def test_rossby_haurwitz_phase_speed(ctx, mesh, method):
     print(f"\nmethod {method}: rel err vs propagated pattern {e:.3e}, vs frozen {e0:.3e}, vs reversed {e1:.3e}")
     assert e < 0.02 and e0 > 5 * e and e1 > 5 * e
     assert np.abs(np.sqrt(x * x + y * y + z * z) - 1).max() < 1e-14
+
+
+def test_learned_rk4_step_under_the_default_root_box(ctx, mesh):
+    """The order model is trained on the tree of the build's own root box
+    (+-(maxR + 1e-5)); the RK4 step builds its trees in the solver's default
+    box (+-1.00101), so its batches and nodes differ from the training ones.
+    One learned step must stay within 1e-5 (relative L2 of the displacement)
+    of the FP32C direct-sum step from the same start, cubed level 8."""
+    from lpmcxx_amd import learned as LN
+    from lpmcxx_amd import tree as T
+    m = mesh(L.LPM_SEED_CUBED, 8)
+    omega, alpha, dt = 2 * np.pi, 1.0, 0.01
+    zr = _rh(m.cx, m.cy, m.cz, alpha)
+    g = zr * m.area
+    u_ref = LN.u_ref_of(g)
+    ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, 1024)
+    try:
+        src = (m.cx, m.cy, m.cz)
+        T.build(ctx, src, m.max_radius, 128)
+        T.moments(ctx, g, 12, 1)
+        ls = LN.labels(ctx, src, 0.85, u_ref, batch_stride=4, keep_rate=0.5, frontier_levels=2, max_samples=1 << 21)
+        LN.fit(ctx, ls, 3e-7, 7, epochs=8)
+        za = zr + 2 * omega * m.cz
+        cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=7, theta=0.85,
+                                       leaf_max=128, u_ref=u_ref)
+        assert list(cfg_l.root_box_max) != [m.max_radius + 1e-5] * 3
+        cfg_d = L.SolverConfig.default(method=L.LPM_METHOD_DIRECT, prec=L.LPM_F32C)
+        xl = np.stack(ctx.rk4_steps(cfg_l, dt, omega, 1, *src, za, m.area))
+        xd = np.stack(ctx.rk4_steps(cfg_d, dt, omega, 1, *src, za, m.area))
+        x0 = np.stack(src)
+        d = np.linalg.norm(xl - xd, axis=0)
+        disp = np.linalg.norm(xd - x0, axis=0)
+        e = np.linalg.norm(d) / np.linalg.norm(disp)
+        print(f"\nlearned RK4 step, default root box: displacement rel L2 err {e:.2e}, max {d.max():.2e} "
+              f"(max displacement {disp.max():.2e})")
+        assert e < 1e-5
+        assert d.max() < 1e-4 * disp.max()
+    finally:
+        ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, 256)   # the context default
