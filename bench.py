@@ -45,6 +45,9 @@ FP32_NONPACKED = 78.6
 FP64_PEAK = 78.6    # TFLOP/s, MI355X FP64 vector
 HBM_PEAK = 8000.0   # GB/s, MI355X HBM3E
 DIRECT_FLOPS = 18   # velocity pair: d 3, |d|^2 5, q 2, rcp 1, w 1, acc 6
+# stereographic far pair: z - z_c 2, |dz|^2 3, rcp 1, w 2, rho w 2, w S 6, sums 3
+# = 19, of which the first complex term's 8 are counted with the terms
+STEREO_PAIR_FLOPS = 11
 PSI_NEAR_FLOPS = 12  # streamfunction pair (FP32): d 3, |d|^2 5, log 1, gamma * log 2, strength sum 1
 METRIC = "Biot-Savart interactions/s + RK4 steps/s at 1.57M particles, rel L2 err ≤1e-5"
 
@@ -104,12 +107,14 @@ def rk4_level10(ctx, args):
     ZA, A = dev(zrel + 2 * omega * m.cz), dev(m.area)
 
     ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
+    set_stereo(ctx, args, False)
     t0 = time.perf_counter()
     T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, 12, 1)
     ls = LN.labels(ctx, (m.cx, m.cy, m.cz), args.theta_max, u_ref, batch_stride=args.l10_label_stride,
                    keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
-    fr = LN.fit(ctx, ls, args.l10_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    fr = LN.fit(ctx, ls, args.l10_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3,
+                stereo_extra=stereo_extra(args))
     t_fit = time.perf_counter() - t0
     log(f"[bench] level-{args.l10_level} labels {len(ls.ids)}, labels + training {t_fit:.1f} s")
 
@@ -126,10 +131,12 @@ def rk4_level10(ctx, args):
         d = torch.stack([u[I] for u in U]) - torch.stack(ref)
         return float(torch.linalg.norm(d) / torch.linalg.norm(torch.stack(ref)))
 
+    set_stereo(ctx, args, True)
     T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, args.max_order, 1)
     LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
     err_l = sample_err()
+    set_stereo(ctx, args, False)
     T.build_dev(ctx, X, Y, Z, m.max_radius, args.base_leaf)
     T.moments_dev(ctx, G, args.base_order, 1)
     T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
@@ -141,6 +148,7 @@ def rk4_level10(ctx, args):
                                    theta=args.base_theta, leaf_max=args.base_leaf)
 
     def run(cfg):
+        set_stereo(ctx, args, cfg.method == L.LPM_METHOD_TREE_LEARNED)
         warm = [X.clone(), Y.clone(), Z.clone()]
         ctx.rk4_step_dev(cfg, dt, omega, *warm, ZA, A)
         P = [X.clone(), Y.clone(), Z.clone()]
@@ -169,14 +177,42 @@ def rk4_level10(ctx, args):
 
 
 def flop_roof(kernel: str, ms: float, near: int, far: int, terms: int, near_flops: int, far_flops: int = 16,
-              term_flops: int = 8) -> dict:
+              term_flops: int = 8, stereo_pairs: int = 0, stereo_terms: int = 0) -> dict:
     """FP32 VALU roofline of one treecode evaluation kernel from its pair and
-    term counts and its launch time (HIP events on the library stream)."""
-    flops = near * near_flops + far * far_flops + terms * term_flops
+    term counts and its launch time (HIP events on the library stream).
+    Stereographic far pairs: STEREO_PAIR_FLOPS each plus 8 per complex term."""
+    flops = (near * near_flops + far * far_flops + terms * term_flops + stereo_pairs * STEREO_PAIR_FLOPS
+             + stereo_terms * 8)
     ach = flops / (ms / 1e3) / 1e12
-    return {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
-            "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
-            "flops_per_launch": flops, "launch_ms": ms, "far_pairs": far, "near_pairs": near, "far_terms": terms}
+    out = {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
+           "frac": ach / FP32_PEAK, "frac_nonpacked": ach / FP32_NONPACKED, "traffic": None,
+           "flops_per_launch": flops, "launch_ms": ms, "far_pairs": far, "near_pairs": near, "far_terms": terms}
+    if stereo_pairs:
+        out.update(stereo_far_pairs=stereo_pairs, stereo_terms=stereo_terms)
+    return out
+
+
+def lists_roof(kernel: str, ms: float, st: dict) -> dict:
+    """Roofline of the low-order list kernel of a learned evaluation: near
+    pairs, the Cartesian far pairs and terms of order <= 7, the stereographic
+    far pairs and terms."""
+    return flop_roof(kernel, ms, st["near"], st["far"] - st["far_high"] - st["far_stereo"],
+                     st["terms"] - st["terms_high"], DIRECT_FLOPS, stereo_pairs=st["far_stereo"],
+                     stereo_terms=st["terms_stereo"])
+
+
+def set_stereo(ctx, args, on: bool) -> None:
+    """The stereographic far field of the learned evaluation (LPM_OPT_STEREO);
+    off for the other solvers, whose moments would otherwise include the
+    stereographic ones."""
+    import lpmcxx_amd as L
+    ctx.set_option(L.LPM_OPT_STEREO, 1 if (on and not args.no_stereo) else 0)
+    ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, args.stereo_extra)
+    ctx.set_option(L.LPM_OPT_STEREO_RATIO, args.stereo_ratio)
+
+
+def stereo_extra(args):
+    return None if args.no_stereo else args.stereo_extra
 
 
 def sample_rate(fn, n_src: int, n_tgt: int, target_seconds: float, seed: int = 5):
@@ -326,13 +362,15 @@ def workload_bve(args) -> dict:
     SX, SY, SZ = X[:nf], Y[:nf], Z[:nf]
     ctx = L.Context(0)
     ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
+    set_stereo(ctx, args, False)
 
     t0 = time.perf_counter()
     T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, 12, 1)
     ls = LN.labels(ctx, (tx, ty, tz), args.theta_max, u_ref, batch_stride=args.label_stride,
                    keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
-    fr = LN.fit(ctx, ls, args.bve_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    fr = LN.fit(ctx, ls, args.bve_tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3,
+                stereo_extra=stereo_extra(args))
     t_fit = time.perf_counter() - t0
 
     # velocity error at t = 0 on a sample of faces and vertices
@@ -348,10 +386,12 @@ def workload_bve(args) -> dict:
         d = torch.stack([u[I] for u in U]) - torch.stack(ref)
         return float(torch.linalg.norm(d) / torch.linalg.norm(torch.stack(ref)))
 
+    set_stereo(ctx, args, True)
     T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, args.max_order, 1)
     LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
     err_l = err()
+    set_stereo(ctx, args, False)
     T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.base_leaf)
     T.moments_dev(ctx, G, args.base_order, 1)
     T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
@@ -363,6 +403,7 @@ def workload_bve(args) -> dict:
                                    theta=args.base_theta, leaf_max=args.base_leaf)
 
     def run(cfg, nsteps, warm):
+        set_stereo(ctx, args, cfg.method == L.LPM_METHOD_TREE_LEARNED)
         P = [X.clone(), Y.clone(), Z.clone()]
         W = [X.clone(), Y.clone(), Z.clone()]
         for _ in range(warm):
@@ -380,8 +421,7 @@ def workload_bve(args) -> dict:
     s_l, P_l = run(cfg_l, args.steps, args.warmup)
     kms = ctx.last_kernel_ms()
     st = T.stats(ctx)
-    roof = flop_roof("tree_eval_lists<7, true, true>", kms, st["near"], st["far"] - st["far_high"],
-                     st["terms"] - st["terms_high"], DIRECT_FLOPS)
+    roof = lists_roof("tree_eval_lists<7, true, true>", kms, st)
     roof["launch_note"] = "last velocity evaluation (RK4 stage 4) of the last timed step"
     s_b, P_b = run(cfg_b, args.steps, 1)
     P0 = torch.stack([X, Y, Z])
@@ -533,7 +573,13 @@ def main() -> int:
     # learned treecode
     ap.add_argument("--leaf", type=int, default=128)
     ap.add_argument("--theta-max", type=float, default=0.85)
-    ap.add_argument("--tau", type=float, default=3e-7)
+    ap.add_argument("--tau", type=float, default=None,
+                    help="per-pair error bound of the labels (default 7e-7 with the stereographic far field, "
+                         "3e-7 without)")
+    ap.add_argument("--no-stereo", action="store_true",
+                    help="Cartesian far field only (no LPM_OPT_STEREO in the learned evaluation)")
+    ap.add_argument("--stereo-extra", type=int, default=1, help="LPM_OPT_STEREO_EXTRA_ORDER")
+    ap.add_argument("--stereo-ratio", type=float, default=1.5, help="LPM_OPT_STEREO_RATIO")
     ap.add_argument("--max-order", type=int, default=7)
     ap.add_argument("--margin", type=int, default=0)
     ap.add_argument("--learned-cell", type=int, default=1024, help="targets per batch cell (learned treecode)")
@@ -567,6 +613,8 @@ def main() -> int:
     ap.add_argument("--psi-theta", type=float, default=0.4)
     ap.add_argument("--bve-tau", type=float, default=2e-7)
     args = ap.parse_args()
+    if args.tau is None:
+        args.tau = 3e-7 if args.no_stereo else 7e-7
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
@@ -660,6 +708,7 @@ def main() -> int:
 
     # --- learned order model: labels from the FP64 treecode, training on the GPU
     ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, args.learned_cell)
+    set_stereo(ctx, args, False)
     t0 = time.perf_counter()
     T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
     T.moments_dev(ctx, G, 12, 1)
@@ -667,8 +716,10 @@ def main() -> int:
                    keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
     t_label = time.perf_counter() - t0
     t0 = time.perf_counter()
-    fr = LN.fit(ctx, ls, args.tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3)
+    fr = LN.fit(ctx, ls, args.tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3,
+                stereo_extra=stereo_extra(args))
     t_train = time.perf_counter() - t0
+    set_stereo(ctx, args, True)
     log(f"[bench] labels {len(ls.ids)} in {t_label:.1f} s, training {t_train:.1f} s, loss {fr.final_loss:.3f}, "
         f"accuracy {fr.accuracy:.3f}")
 
@@ -715,8 +766,7 @@ def main() -> int:
     # the learned step that is the low-order list kernel, so the far pairs and
     # terms of the order 8..12 list kernel are left out
     def tree_roof(stats, ms, kernel):
-        return flop_roof(kernel, ms, stats["near"], stats["far"] - stats.get("far_high", 0),
-                         stats["terms"] - stats.get("terms_high", 0), DIRECT_FLOPS)
+        return lists_roof(kernel, ms, stats)
 
     direct_flops = pairs * DIRECT_FLOPS
     direct_roof = {"bound": "valu", "kernel": "ds_f32c", "achieved": direct_flops / (ms32 / 1e3) / 1e12,
@@ -737,6 +787,7 @@ def main() -> int:
         A = dev(m.area)
 
         def rk4_rate(cfg, reps=2):
+            set_stereo(ctx, args, cfg.method == L.LPM_METHOD_TREE_LEARNED)
             Xs, Ys, Zs = X.clone(), Y.clone(), Z.clone()
             ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A)
             return 1.0 / wall(lambda: ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, Xs, Ys, Zs, za, A), reps)
@@ -750,6 +801,7 @@ def main() -> int:
 
         # one step from the same start by each solver against the FP32C direct sum's step
         def one_step(cfg):
+            set_stereo(ctx, args, cfg.method == L.LPM_METHOD_TREE_LEARNED)
             P = [X.clone(), Y.clone(), Z.clone()]
             ctx.rk4_step_dev(cfg, 0.01, 2 * np.pi, *P, za, A)
             ctx.synchronize()
@@ -784,7 +836,10 @@ def main() -> int:
                                "8 Gaussian vortices", "particles": n, "pairs_per_step": pairs,
                    "step": {"learned": f"learned-order FP32C treecode (build + moments + evaluation), leaf "
                                        f"{args.leaf}, theta_max {args.theta_max}, tau {args.tau}, max order "
-                                       f"{args.max_order}, margin {args.margin}",
+                                       f"{args.max_order}, margin {args.margin}, far field: "
+                                       + ("Cartesian" if args.no_stereo else
+                                          f"stereographic where the plane ratio <= {args.stereo_ratio} x the "
+                                          f"sphere ratio (order p + {args.stereo_extra}), else Cartesian"),
                             "batch": f"fixed-order FP32C batch treecode p={args.batch_order} theta="
                                      f"{args.batch_theta} leaf {args.batch_leaf}",
                             "direct_f32c": "FP32C direct sum"}[step_name],

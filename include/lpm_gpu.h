@@ -164,6 +164,20 @@ int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
  * treecodes are cut at the octree cells of the shallowest level with at least
  * n_targets / value occupied cells (default 256). */
 #define LPM_OPT_BATCH_CELL_TARGETS 1
+/* LPM_OPT_STEREO (0 or 1, default 0): lpm_tree_moments* also makes the
+ * stereographic moments of every node, and the FP32C learned evaluation with
+ * delta = 0 evaluates a far pair that the model accepts at order p as the
+ * plane Cauchy series of the sphere kernel in stereographic coordinates, of
+ * order p + LPM_OPT_STEREO_EXTRA_ORDER (at most 15; default extra 1), when
+ * the pair's plane ratio (rho_n + r_b) / |z_b - z_c| is at most
+ * LPM_OPT_STEREO_RATIO (default 1.0) times its ratio (r_n + r_b) / s on the
+ * sphere.  Every target batch uses the projection from the pole opposite its
+ * hemisphere.  P + 1 complex terms replace the (p + 1)(p + 2)(p + 3)/6
+ * Cartesian terms of the pair.  Options take effect at the next
+ * lpm_tree_moments* call (the stereographic moments) and evaluation. */
+#define LPM_OPT_STEREO 2
+#define LPM_OPT_STEREO_EXTRA_ORDER 3
+#define LPM_OPT_STEREO_RATIO 4
 int lpm_set_option(lpm_ctx* ctx, int option, double value);
 
 /* Duration of the last pair or evaluation kernel, measured with HIP events
@@ -182,8 +196,10 @@ int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
  * pairs, out[2] far-field series terms summed over the far pairs
  * (p(p+1)(p+2)/6 each), out[3] order-model evaluations, and for
  * lpm_tree_eval_learned* the part of out[0] (out[4]) and of out[2] (out[5])
- * evaluated by the order 8..12 list kernel (zero otherwise).  out holds
- * LPM_TREE_NSTATS values.  Synchronizes.
+ * evaluated by the order 8..12 list kernel (zero otherwise), the part of
+ * out[0] evaluated as stereographic series (out[6]) and their complex terms
+ * (out[7], P + 1 each, not in out[2]).  out holds LPM_TREE_NSTATS values.
+ * Synchronizes.
  * No counterpart in the reference (its macCounter is out[0] of lpm_tree_eval). */
 #define LPM_STAT_FAR 0
 #define LPM_STAT_NEAR 1
@@ -191,7 +207,9 @@ int lpm_last_kernel_ms(lpm_ctx* ctx, float* ms);
 #define LPM_STAT_MODEL 3
 #define LPM_STAT_FAR_HIGH 4
 #define LPM_STAT_TERMS_HIGH 5
-#define LPM_TREE_NSTATS 6
+#define LPM_STAT_FAR_STEREO 6
+#define LPM_STAT_TERMS_STEREO 7
+#define LPM_TREE_NSTATS 8
 int lpm_tree_stats(lpm_ctx* ctx, int64_t* out);
 
 /* ------------------------------------------------------------------ streamfunction treecode

@@ -95,6 +95,7 @@ class SolverConfig(C.Structure):
 
 
 LPM_OPT_BATCH_CELL_TARGETS = 1
+LPM_OPT_STEREO, LPM_OPT_STEREO_EXTRA_ORDER, LPM_OPT_STEREO_RATIO = 2, 3, 4
 LPM_METHOD_DIRECT = 0
 LPM_METHOD_TREE = 1
 LPM_METHOD_TREE_BATCH = 2

@@ -1,6 +1,7 @@
 // C ABI of liblpm_gpu: contexts, status codes, host-pointer wrappers.
 #include "lpm_internal.hpp"
 
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -91,6 +92,19 @@ int lpm_set_option(lpm_ctx* c, int option, double value) {
         case LPM_OPT_BATCH_CELL_TARGETS:
             if (!(value >= 1.0 && value <= 1e9)) return fail(c, LPM_ERR_INVALID_ARG, "batch cell targets out of range");
             c->batch_cell_targets = int(value);
+            return LPM_OK;
+        case LPM_OPT_STEREO:
+            if (!(value == 0.0 || value == 1.0)) return fail(c, LPM_ERR_INVALID_ARG, "LPM_OPT_STEREO is 0 or 1");
+            c->stereo = int(value);
+            return LPM_OK;
+        case LPM_OPT_STEREO_EXTRA_ORDER:
+            if (!(value >= 0.0 && value <= 14.0 && value == std::floor(value)))
+                return fail(c, LPM_ERR_INVALID_ARG, "stereographic extra order must be an integer in 0..14");
+            c->stereo_extra = int(value);
+            return LPM_OK;
+        case LPM_OPT_STEREO_RATIO:
+            if (!(value > 0.0 && value <= 4.0)) return fail(c, LPM_ERR_INVALID_ARG, "stereographic ratio out of (0, 4]");
+            c->stereo_ratio = value;
             return LPM_OK;
         default: return fail(c, LPM_ERR_INVALID_ARG, "unknown option %d", option);
     }

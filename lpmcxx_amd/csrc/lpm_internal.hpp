@@ -27,8 +27,10 @@ struct Tree;   // octree state (octree.hip)
 struct Model;  // order model state (mlp.hip)
 
 // counters of the last tree evaluation: far (target, node) pairs, near
-// (target, source) pairs, far-field series terms, order-model calls
-constexpr int kStats = 6;
+// (target, source) pairs, far-field series terms, order-model calls, the far
+// pairs and terms of the order 8..12 list kernel, the stereographic far pairs
+// and their complex terms
+constexpr int kStats = 8;
 // The counters are kept in kStatSlots copies, one 128-byte line each (block b
 // adds to copy b % kStatSlots); readers sum the copies.  Tens of thousands of
 // waves flush their counts at the end of a launch, which on one line queue
@@ -55,6 +57,9 @@ struct lpm_ctx {
     lpm::DevBuf mesh_work;
     int far_cap = 1024, near_cap = 512;
     int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
+    int stereo = 0;                // LPM_OPT_STEREO
+    int stereo_extra = 1;          // LPM_OPT_STEREO_EXTRA_ORDER
+    double stereo_ratio = 1.0;     // LPM_OPT_STEREO_RATIO
     lpm::Tree* tree = nullptr;
     lpm::Model* model = nullptr;
     unsigned long long* stats_ptr = nullptr;  // kStats counters of the last tree evaluation (device)
@@ -112,6 +117,8 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
 int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign);
 // streamfunction moments mu_k = sum_j gamma_j (y_j - c)^k of every node (one component)
 int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p);
+// stereographic node centres, radii and moments of both projections (stereo.hip)
+int tree_stereo_moments(lpm_ctx* c);
 int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);

@@ -64,6 +64,12 @@ struct Tree {
     // streamfunction moments mu_k = sum_j gamma_j h_j^k (one component): lex, graded, FP32 graded scaled
     int psi_order = 0;    // p of the streamfunction moments, 0 = none
     DevBuf pmom, pmomg, pmom32;
+    // stereographic node data (lpm_stereo.hpp), [2 projections][nn]; made
+    // with the moments when LPM_OPT_STEREO is set
+    int stereo_terms = 0;  // terms per node and projection, 0 = none
+    DevBuf sgeo;          // double4: centre (re, im), source radius about it (+inf: unusable), unused
+    DevBuf smom;          // double2 [.][kStereoTerms]: a_k
+    DevBuf smom32;        // float2  [.][kStereoTerms]: a_k / rho^k
     // scratch
     DevBuf mpart;         // double [nch][nterms][3] leaf-chunk moment partials
     DevBuf tmp0, tmp1, tmp2, tmp3, sort_tmp, scan_tmp;

@@ -384,6 +384,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     t->sx = t->sy = t->sz = nullptr;  // set once the build has succeeded
     t->order = 0;
     t->psi_order = 0;
+    t->stereo_terms = 0;
     t->leaf_max = (control == 0) ? param : 0;
     for (int d = 0; d < 3; ++d) { t->root_box[d] = blo[d]; t->root_box[3 + d] = bhi[d]; }
     const int leaf_max = (control == 0) ? param : 1;

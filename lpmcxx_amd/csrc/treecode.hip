@@ -584,6 +584,8 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(t->nn)), dim3(256), 0, c->stream, int(t->nn), p, nterm,
                        t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
+    t->stereo_terms = 0;
+    if (c->stereo) LPM_CHECK(tree_stereo_moments(c));
     t->order = p;
     t->moment_sign = sign;
     return LPM_OK;

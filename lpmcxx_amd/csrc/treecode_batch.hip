@@ -27,6 +27,7 @@
 // per-node results are added to FP64 per-target sums.
 #include "lpm_mlp.hpp"
 #include "lpm_series.hpp"
+#include "lpm_stereo.hpp"
 #include "lpm_tree.hpp"
 
 #include <algorithm>
@@ -250,7 +251,8 @@ __device__ inline int64_t load_target(const Targets& tg, int64_t b, bool& valid,
 
 __device__ inline void flush_stats(unsigned long long* stats, unsigned long long far, unsigned long long near,
                                    unsigned long long terms, unsigned long long calls,
-                                   unsigned long long far_hi = 0, unsigned long long terms_hi = 0) {
+                                   unsigned long long far_hi = 0, unsigned long long terms_hi = 0,
+                                   unsigned long long far_st = 0, unsigned long long terms_st = 0) {
     if ((threadIdx.x & 63) == 0) {
         stats += (blockIdx.x % kStatSlots) * kStatStride;
         atomicAdd(stats + 0, far);
@@ -259,7 +261,89 @@ __device__ inline void flush_stats(unsigned long long* stats, unsigned long long
         atomicAdd(stats + 3, calls);
         if (far_hi) atomicAdd(stats + 4, far_hi);
         if (terms_hi) atomicAdd(stats + 5, terms_hi);
+        if (far_st) atomicAdd(stats + 6, far_st);
+        if (terms_st) atomicAdd(stats + 7, terms_st);
     }
+}
+
+// ---------------------------------------------------------------- stereographic far field
+// Node data of both projections and the switch of the learned FP32C
+// evaluation (lpm_stereo.hpp).
+struct StereoView {
+    const double4* sgeo;   // [2][nn] centre (re, im), rho
+    const float2* smom32;  // [2][nn][kStereoTerms] a_k / rho^k
+    int64_t nn;
+    int on;                // evaluate accepted pairs as plane series where they qualify
+    int extra;             // order P = p + extra
+    int maxp;              // highest P the moments allow
+    double ratio;          // plane ratio must be <= ratio x the sphere ratio
+};
+
+// The batch's projection and plane centre / radius; ok is false when a target
+// lies at the projection's pole (the batch then keeps the Cartesian series).
+struct StereoBatch {
+    int proj;
+    bool ok;
+    double zr, zi;    // this lane's target
+    double cr, ci, r; // batch centre and radius in the plane
+};
+
+// a wave-uniform double held in scalar registers
+__device__ inline double uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffffll)), hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+__device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t0, double t1, double t2) {
+    StereoBatch s;
+    s.proj = bt.cz >= 0.0 ? 0 : 1;
+    const bool good = stereo_point(s.proj, t0, t1, t2, s.zr, s.zi) || !valid;
+    s.ok = __ballot(!good) == 0ull;
+    const double inv = 1.0 / double(bt.nvalid);
+    s.cr = uniform_d(wsum(valid ? s.zr : 0.0) * inv);
+    s.ci = uniform_d(wsum(valid ? s.zi : 0.0) * inv);
+    const double er = s.zr - s.cr, ei = s.zi - s.ci;
+    s.r = uniform_d(sqrt(wmax(valid ? er * er + ei * ei : 0.0)));
+    return s;
+}
+
+// Cauchy series of node entry gi (projection-major index) at the lane's
+// target: S += w sum_k (a_k / rho^k) (rho w)^k, w = 1 / (z - z_c), by Horner
+// in FP32; G += the node's total strength.
+__device__ __forceinline__ void stereo_node(int P, int64_t gi, const StereoView& sv, double zr, double zi, double& Sr,
+                                            double& Si, double& G) {
+    const double4 g = sv.sgeo[gi];
+    const float dr = float(zr - g.x), di = float(zi - g.y);
+    const float ir2 = __builtin_amdgcn_rcpf(fmaf(dr, dr, di * di));
+    const float wr = dr * ir2, wi = -di * ir2;
+    const float rho = float(g.z);
+    const float xr = rho * wr, xi = rho * wi;
+    const float2* a = sv.smom32 + gi * kStereoTerms;
+    float sr = a[P].x, si = a[P].y;
+    for (int k = P - 1; k >= 0; --k) {
+        const float2 ak = a[k];
+        const float tr = fmaf(sr, xr, fmaf(-si, xi, ak.x));
+        si = fmaf(sr, xi, fmaf(si, xr, ak.y));
+        sr = tr;
+    }
+    Sr += double(fmaf(sr, wr, -si * wi));
+    Si += double(fmaf(sr, wi, si * wr));
+    G += double(a[0].x);
+}
+
+// The unscaled velocity sum sum g (t x y) / (1 - t.y) of the plane series
+// (S, G) at the target z = zr + i zi, added to (s0, s1, s2).
+__device__ inline void stereo_add(int proj, double zr, double zi, double Sr, double Si, double G, double& s0,
+                                  double& s1, double& s2) {
+    const double q = 1.0 + zr * zr + zi * zi;
+    const double wr = Sr - G * zr / q, wi = Si + G * zi / q;  // S - G conj(z) / (1 + |z|^2)
+    const double u0 = -(wi * (1.0 + zi * zi - zr * zr) - 2.0 * wr * zr * zi);
+    const double u1 = -(wr * (1.0 + zr * zr - zi * zi) - 2.0 * wi * zr * zi);
+    const double u2 = 2.0 * (wi * zr + wr * zi);
+    s0 += u0;
+    s1 += proj == 0 ? u1 : -u1;
+    s2 += proj == 0 ? u2 : -u2;
 }
 
 // ---------------------------------------------------------------- plain batch treecode
@@ -344,11 +428,15 @@ struct Lists {
     int* overflow;  // [0] a batch overflowed, [1] far and [2] near capacity it needs
 };
 
-// six waves per SIMD (77 VGPRs, no spills): the traversal waits on node loads
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
+// six waves per SIMD (77 VGPRs, no spills): the traversal waits on node
+// loads.  STEREO: accepted pairs may become stereographic entries (five
+// waves: the batch's plane data would spill at six).
+template <bool STEREO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEREO ? 5 : 6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
                                                          int max_order, double u_ref,
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
-                                                         Lists ls, unsigned long long* __restrict__ stats) {
+                                                         Lists ls, StereoView sv,
+                                                         unsigned long long* __restrict__ stats) {
     __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackPlan];
     mlp_stage_weights(wts, s_model);
@@ -361,10 +449,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     double t0, t1, t2;
     load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
+    StereoBatch sb{};
+    if (STEREO) sb = stereo_batch(bt, valid, t0, t1, t2);
+    const bool st_on = STEREO && sb.ok;
     int* far = ls.far + bi * ls.far_cap;
     int* near = ls.near + bi * ls.near_cap;
     int nfar = 0, nhi = 0, nnear = 0;
-    unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0;
+    unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0, nst = 0, nterms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int sp = 0;
     if (tv.nkids[0] == 0) {  // the root is a leaf
@@ -418,6 +509,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         const bool has = row < tot;
         const int chr = firstc + row - base;
         bool prop = false;
+        double r3 = 0.0;  // the pair's ratio (r_n + r_b) / s on the sphere
         float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
         // every node-table word of the child is loaded here, in one round
         int ckids = 0, cfirst = 0, cb = 0, ce = 0;
@@ -432,6 +524,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             const double rn = tv.nrad[chr];
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
+                r3 = (rn + bt.r) / sd;
                 float fe[kIn];
                 pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
 #pragma unroll
@@ -451,7 +544,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         }
         const bool mine = lane < 16 && has;
         const bool leaf = mine && ckids == 0;
-        const bool is_lo = mine && p > 0 && p <= kLowOrder, is_hi = mine && p > kLowOrder;
+        // an accepted pair whose plane ratio is no worse (by sv.ratio) than its
+        // sphere ratio becomes a stereographic entry of order p + extra
+        bool is_st = false;
+        int pz = 0;
+        if (st_on && mine && p > 0) {
+            const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + chr];
+            const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
+            const double dist = sqrt(er * er + ei * ei);
+            const double rz = gz.z + sb.r;
+            is_st = rz < sv.ratio * r3 * dist && rz < 0.95 * dist;
+            pz = min(p + sv.extra, sv.maxp);
+        }
+        const bool is_lo = mine && p > 0 && (p <= kLowOrder || is_st), is_hi = mine && p > kLowOrder && !is_st;
         const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
         const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi);
         const uint64_t m_near = __ballot(is_near), m_desc = __ballot(is_desc);
@@ -459,7 +564,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         // entries are written only while the batch's lists fit (else the
         // batch is reported as overflowing at the end)
         if (nfar + c_lo + nhi + c_hi <= ls.far_cap) {
-            const int e = chr | (p << kNodeBits);
+            const int e = is_st ? int(unsigned(chr) | (unsigned(pz) << kNodeBits) | 0x80000000u) : chr | (p << kNodeBits);
             if (is_lo) far[nfar + __popcll(m_lo & below)] = e;
             if (is_hi) far[ls.far_cap - 1 - nhi - __popcll(m_hi & below)] = e;
         }
@@ -471,18 +576,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         sp += c_desc;
         // terms of the accepted children and sources of the near leaves
         // (lanes 0..15: an xor butterfly over 16 lanes sums them)
-        unsigned t_all = (is_lo || is_hi) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
+        unsigned t_all = ((is_lo || is_hi) && !is_st) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
         unsigned t_hi = is_hi ? t_all : 0u;
         unsigned np = is_near ? unsigned(ce - cb) : 0u;
+        unsigned t_st = is_st ? unsigned(pz + 1) : 0u;
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) {
             t_all += __shfl_xor(t_all, o, 64);
             t_hi += __shfl_xor(t_hi, o, 64);
             np += __shfl_xor(np, o, 64);
+            t_st += __shfl_xor(t_st, o, 64);
         }
         nterms += __builtin_amdgcn_readfirstlane(t_all);
         nterms_hi += __builtin_amdgcn_readfirstlane(t_hi);
         near_pairs += __builtin_amdgcn_readfirstlane(np);
+        nterms_st += __builtin_amdgcn_readfirstlane(t_st);
+        nst += __popcll(__ballot(is_st));
         wave_sync();
     }
     if (lane == 0) {
@@ -500,16 +609,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     }
     const unsigned long long nv = unsigned(bt.nvalid);
     flush_stats(stats, (unsigned long long)(nfar + nhi) * nv, near_pairs * nv, nterms * nv, ncalls,
-                (unsigned long long)nhi * nv, nterms_hi * nv);
+                (unsigned long long)nhi * nv, nterms_hi * nv, nst * nv, nterms_st * nv);
 }
 
 // The batch's interaction list.  LOW: near leaves and the far entries of
 // order <= kLowOrder, written to acc[3][nt] (sorted positions, FP64); else
 // the far entries of order kLowOrder + 1 .. MAXP, added to acc.  The order is
 // wave-uniform per entry.
-template <int MAXP, bool LOW, bool F32>
-__global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, double d2, float eps2v,
-                                                       double* __restrict__ acc) {
+template <int MAXP, bool LOW, bool F32, bool STEREO = false>
+__global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, StereoView sv, double d2,
+                                                       float eps2v, double* __restrict__ acc) {
     const int64_t bi = wave_index();
     if (!is_batch(tg, bi)) return;
     const int* far = ls.far + bi * ls.far_cap;
@@ -524,6 +633,10 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     double t0, t1, t2;
     const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    // stereographic entries (low list only): Cauchy sums in the batch's projection
+    StereoBatch sb{};
+    double Sr = 0.0, Si = 0.0, Gs = 0.0;
+    if (STEREO) sb = stereo_batch(batch_of(valid, t0, t1, t2), valid, t0, t1, t2);
     if (LOW) {
         const int* near = ls.near + bi * ls.near_cap;
         const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
@@ -534,12 +647,18 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
         }
     }
     for (int j = j0; j < j1; ++j) {
-        const int e = __builtin_amdgcn_readfirstlane(far[j]);
-        const int n = e & ((1 << kNodeBits) - 1);
-        far_node<MAXP, F32, true>(e >> kNodeBits, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0,
-                                  s1, s2, v0, v1, v2);
+        const unsigned e = unsigned(__builtin_amdgcn_readfirstlane(far[j]));
+        const int n = int(e & ((1u << kNodeBits) - 1u));
+        const int p = int((e >> kNodeBits) & 15u);
+        if (STEREO && (e >> 31)) {
+            stereo_node(p, int64_t(sb.proj) * sv.nn + n, sv, sb.zr, sb.zi, Sr, Si, Gs);
+            continue;
+        }
+        far_node<MAXP, F32, true>(p, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0, s1, s2, v0, v1,
+                                  v2);
     }
     cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
+    if (STEREO) stereo_add(sb.proj, sb.zr, sb.zi, Sr, Si, Gs, s0, s1, s2);
     if (valid) {
         if (LOW) {
             acc[pos] = s0;
@@ -857,6 +976,14 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     const float eps2v = float(2.0 * d2);
     const unsigned short* w = model_weights(c);
     const bool f32 = prec == LPM_F32C;
+    // stereographic far entries: FP32C with delta = 0 and the option set
+    StereoView sv{t->sgeo.as<double4>(), t->smom32.as<float2>(), t->nn, 0, c->stereo_extra, t->stereo_terms - 1,
+                  c->stereo_ratio};
+    if (c->stereo && f32 && d2 == 0.0) {
+        if (t->stereo_terms == 0)
+            return fail(c, LPM_ERR_STATE, "no stereographic moments (lpm_tree_moments after setting LPM_OPT_STEREO)");
+        sv.on = 1;
+    }
     // plan, evaluate, and check once at the end that every batch's lists fit;
     // if one did not, grow the lists to what it needed and do it again
     for (int attempt = 0;; ++attempt) {
@@ -875,14 +1002,19 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         ls.near_cap = c->near_cap;
         LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 3 * sizeof(int), st));
         if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
-        hipLaunchKernelGGL(tree_plan_learned, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin, max_order, u_ref,
-                           w, norm, ls, c->stats_ptr);
+        if (sv.on)
+            hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
+                               max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
+        else
+            hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
+                               max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
         auto lists = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, d2, eps2v, acc);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
         };
         // the timed launch is the low-order list kernel, the one that dominates
         mark_begin(c);
-        if (f32) lists(tree_eval_lists<kLowOrder, true, true>);
+        if (sv.on) lists(tree_eval_lists<kLowOrder, true, true, true>);
+        else if (f32) lists(tree_eval_lists<kLowOrder, true, true>);
         else lists(tree_eval_lists<kLowOrder, true, false>);
         mark_end(c);
         if (max_order > kLowOrder) {

@@ -144,18 +144,29 @@ def labels(ctx: Context, targets, theta_max: float, u_ref: float, delta: float =
 # (vector instructions per lane: ~9 for a term, ~11 for a pair)
 FAR_TERM_COST = 9.0
 NEAR_PAIR_COST = 11.0
+# stereographic entries (LPM_OPT_STEREO): ~25 instructions per entry and ~4
+# per complex term; ~13 per near pair in that kernel
+STEREO_ENTRY_COST = 25.0
+STEREO_TERM_COST = 4.0
+STEREO_NEAR_PAIR_COST = 13.0
 
 
-def order_labels(ls: LabelSet, tau: float, max_order: int) -> np.ndarray:
+def order_labels(ls: LabelSet, tau: float, max_order: int, stereo_extra: int | None = None) -> np.ndarray:
     """Class of every sample: the lowest order p <= max_order whose error is at
     most tau, or 0 (descend) when there is none or, at a leaf, when the
-    series would cost more than summing the leaf directly."""
+    series would cost more than summing the leaf directly.  With stereo_extra
+    (the LPM_OPT_STEREO_EXTRA_ORDER of the evaluation) a far leaf is priced as
+    a stereographic entry of order p + stereo_extra."""
     ok = ls.errors[:, :max_order] <= tau
     has = ok.any(axis=1)
     p = np.where(has, ok.argmax(axis=1) + 1, 0)
     leaf = ls.features[:, 9] > 0.5
     npart = np.exp2(ls.features[:, 6])
-    dearer = leaf & (FAR_TERM_COST * nterms(p) >= NEAR_PAIR_COST * npart)
+    if stereo_extra is None:
+        dearer = leaf & (FAR_TERM_COST * nterms(p) >= NEAR_PAIR_COST * npart)
+    else:
+        far = STEREO_ENTRY_COST + STEREO_TERM_COST * (p + stereo_extra + 1)
+        dearer = leaf & (far >= STEREO_NEAR_PAIR_COST * npart)
     return np.where(dearer, 0, p).astype(np.int32)
 
 
@@ -214,9 +225,10 @@ class FitResult:
 
 
 def fit(ctx: Context, ls: LabelSet, tau: float, max_order: int, *, epochs: int = 20, batch: int = 4096,
-        lr: float = 2e-3, seed: int = 1) -> FitResult:
-    """Train the context's order model on a label set for the bound tau."""
-    y = order_labels(ls, tau, max_order)
+        lr: float = 2e-3, seed: int = 1, stereo_extra: int | None = None) -> FitResult:
+    """Train the context's order model on a label set for the bound tau
+    (stereo_extra: leaves priced for the stereographic far field)."""
+    y = order_labels(ls, tau, max_order, stereo_extra)
     mean, inv = normalization(ls.features)
     m = OrderModel(ctx)
     m.init(seed)

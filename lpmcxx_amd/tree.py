@@ -90,17 +90,20 @@ def last_stats(ctx: Context):
 
 
 LPM_STAT_FAR, LPM_STAT_NEAR, LPM_STAT_TERMS, LPM_STAT_MODEL = 0, 1, 2, 3
-LPM_STAT_FAR_HIGH, LPM_STAT_TERMS_HIGH, LPM_TREE_NSTATS = 4, 5, 6
+LPM_STAT_FAR_HIGH, LPM_STAT_TERMS_HIGH, LPM_STAT_FAR_STEREO, LPM_STAT_TERMS_STEREO = 4, 5, 6, 7
+LPM_TREE_NSTATS = 8
 
 
 def stats(ctx: Context) -> dict:
     """Counters of the last evaluation: far pairs, near pairs, far-field series
     terms, order-model calls, and (learned evaluation) the far pairs and terms
-    handled by the order 8..12 list kernel."""
+    handled by the order 8..12 list kernel and the far pairs evaluated as
+    stereographic series with their complex terms."""
     out = np.zeros(LPM_TREE_NSTATS, np.int64)
     ctx.check(ctx.lib.lpm_tree_stats(ctx.h, _ptr(out)))
     return {"far": int(out[0]), "near": int(out[1]), "terms": int(out[2]), "model": int(out[3]),
-            "far_high": int(out[4]), "terms_high": int(out[5])}
+            "far_high": int(out[4]), "terms_high": int(out[5]), "far_stereo": int(out[6]),
+            "terms_stereo": int(out[7])}
 
 
 def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float = 0.0, prec: int = LPM_F64):

@@ -327,6 +327,76 @@ def test_learned_planning_clamps_inputs_to_the_model_range(ctx, mesh):
             assert np.array_equal(uc, u7)
 
 
+# ---------------------------------------------------------------- stereographic far field
+def _stereo(ctx, on, extra=1, ratio=1.0):
+    ctx.set_option(L.LPM_OPT_STEREO, 1 if on else 0)
+    ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, extra)
+    ctx.set_option(L.LPM_OPT_STEREO_RATIO, ratio)
+
+
+def test_stereo_far_field_against_the_cartesian_series(ctx, mesh):
+    """A constant model accepts every pair passing theta_max = 0.4 at order p.
+    The pairs whose plane ratio is at most their sphere ratio become
+    stereographic entries of order p + extra; over both projections (targets
+    on the whole sphere: vertices and faces of cubed level 5) the result is at
+    least as accurate as the Cartesian series of order p against the direct
+    sum, and two more orders are no less accurate."""
+    m, src, g, tgt = _field(mesh, 5, 64, 8)
+    ud = np.stack(O.bs_direct(tgt, src, g))
+    u_ref = LN.u_ref_of(g)
+    err = {}
+    try:
+        for on, extra, cls in ((False, 0, 6), (False, 0, 8), (True, 0, 6), (True, 2, 6), (True, 0, 8)):
+            _stereo(ctx, on, extra, 1.0)
+            _gpu_tree(ctx, m, src, g, 64, 8)
+            LN.constant_model(ctx, cls)
+            u = np.stack(LN.evaluate(ctx, tgt, 0.4, 0, 8, u_ref, prec=L.LPM_F32C))
+            st = T.stats(ctx)
+            err[on, extra, cls] = O.rel_l2(u, ud)
+            if on:
+                assert st["far_stereo"] > 0.3 * st["far"]
+                assert st["terms_stereo"] == st["far_stereo"] * (cls + extra + 1)
+                assert st["terms"] == (st["far"] - st["far_stereo"]) * cls * (cls + 1) * (cls + 2) // 6
+            else:
+                assert st["far_stereo"] == 0
+        print("\n" + ", ".join(f"stereo {k[0]} extra {k[1]} order {k[2]}: {v:.2e}" for k, v in err.items()))
+        assert err[True, 0, 6] <= 1.05 * err[False, 0, 6]
+        assert err[True, 0, 8] <= 1.05 * err[False, 0, 8]
+        assert err[True, 2, 6] <= err[True, 0, 6]
+    finally:
+        _stereo(ctx, False)
+
+
+def test_stereo_is_off_for_fp64_and_smoothing(ctx, mesh):
+    """The stereographic form holds for delta = 0 on the unit sphere: with
+    delta > 0, and in FP64, the option changes nothing."""
+    m, src, g, tgt = _field(mesh, 4, 32, 6)
+    u_ref = LN.u_ref_of(g)
+    try:
+        out = {}
+        for on in (False, True):
+            _stereo(ctx, on, 2)
+            _gpu_tree(ctx, m, src, g, 32, 6)
+            LN.constant_model(ctx, 5)
+            out[on] = [np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 6, u_ref, delta=0.01, prec=L.LPM_F32C)),
+                       np.stack(LN.evaluate(ctx, tgt, 0.6, 0, 6, u_ref, prec=L.LPM_F64))]
+            assert T.stats(ctx)["far_stereo"] == 0
+        for a, b in zip(out[False], out[True]):
+            assert np.array_equal(a, b)
+        # moments made before the option was set: an error, not a silent fallback
+        _stereo(ctx, False)
+        _gpu_tree(ctx, m, src, g, 32, 6)
+        _stereo(ctx, True)
+        with pytest.raises(L.LpmError):
+            LN.evaluate(ctx, tgt, 0.6, 0, 6, u_ref, prec=L.LPM_F32C)
+        with pytest.raises(L.LpmError):
+            ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, 15)
+        with pytest.raises(L.LpmError):
+            ctx.set_option(L.LPM_OPT_STEREO, 2)
+    finally:
+        _stereo(ctx, False)
+
+
 def test_learned_errors(ctx, mesh):
     m, src, g, tgt = _field(mesh, 3, 32, 4)
     _gpu_tree(ctx, m, src, g, 32, 4)
