@@ -58,12 +58,14 @@ def test_rossby_haurwitz_phase_speed(ctx, mesh, method):
     assert np.abs(np.sqrt(x * x + y * y + z * z) - 1).max() < 1e-14
 
 
-def test_learned_rk4_step_under_the_default_root_box(ctx, mesh):
+@pytest.mark.parametrize("stereo", [False, True])
+def test_learned_rk4_step_under_the_default_root_box(ctx, mesh, stereo):
     """The order model is trained on the tree of the build's own root box
     (+-(maxR + 1e-5)); the RK4 step builds its trees in the solver's default
     box (+-1.00101), so its batches and nodes differ from the training ones.
     One learned step must stay within 1e-5 (relative L2 of the displacement)
-    of the FP32C direct-sum step from the same start, cubed level 8."""
+    of the FP32C direct-sum step from the same start, cubed level 8, with the
+    Cartesian and with the stereographic far field."""
     from lpmcxx_amd import learned as LN
     from lpmcxx_amd import tree as T
     m = mesh(L.LPM_SEED_CUBED, 8)
@@ -77,7 +79,10 @@ def test_learned_rk4_step_under_the_default_root_box(ctx, mesh):
         T.build(ctx, src, m.max_radius, 128)
         T.moments(ctx, g, 12, 1)
         ls = LN.labels(ctx, src, 0.85, u_ref, batch_stride=4, keep_rate=0.5, frontier_levels=2, max_samples=1 << 21)
-        LN.fit(ctx, ls, 3e-7, 7, epochs=8)
+        LN.fit(ctx, ls, 7e-7 if stereo else 3e-7, 7, epochs=8, stereo_extra=1 if stereo else None)
+        ctx.set_option(L.LPM_OPT_STEREO, 1 if stereo else 0)
+        ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, 1)
+        ctx.set_option(L.LPM_OPT_STEREO_RATIO, 1.5)
         za = zr + 2 * omega * m.cz
         cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=7, theta=0.85,
                                        leaf_max=128, u_ref=u_ref)
@@ -89,9 +94,10 @@ def test_learned_rk4_step_under_the_default_root_box(ctx, mesh):
         d = np.linalg.norm(xl - xd, axis=0)
         disp = np.linalg.norm(xd - x0, axis=0)
         e = np.linalg.norm(d) / np.linalg.norm(disp)
-        print(f"\nlearned RK4 step, default root box: displacement rel L2 err {e:.2e}, max {d.max():.2e} "
-              f"(max displacement {disp.max():.2e})")
+        print(f"\nlearned RK4 step (stereographic {stereo}), default root box: displacement rel L2 err {e:.2e}, "
+              f"max {d.max():.2e} (max displacement {disp.max():.2e})")
         assert e < 1e-5
         assert d.max() < 1e-4 * disp.max()
     finally:
-        ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, 256)   # the context default
+        ctx.set_option(L.LPM_OPT_BATCH_CELL_TARGETS, 256)   # the context defaults
+        ctx.set_option(L.LPM_OPT_STEREO, 0)
