@@ -215,6 +215,11 @@ def stereo_extra(args):
     return None if args.no_stereo else args.stereo_extra
 
 
+def lists_kernel(args) -> str:
+    """The low-order list kernel of the learned evaluation."""
+    return "tree_eval_lists<7, true, true>" if args.no_stereo else "tree_eval_lists<7, true, true, true>"
+
+
 def sample_rate(fn, n_src: int, n_tgt: int, target_seconds: float, seed: int = 5):
     """Time the oracle fn(idx) on a growing random sample of targets until it
     takes target_seconds; returns (pairs/s, sample size, seconds)."""
@@ -421,7 +426,7 @@ def workload_bve(args) -> dict:
     s_l, P_l = run(cfg_l, args.steps, args.warmup)
     kms = ctx.last_kernel_ms()
     st = T.stats(ctx)
-    roof = lists_roof("tree_eval_lists<7, true, true>", kms, st)
+    roof = lists_roof(lists_kernel(args), kms, st)
     roof["launch_note"] = "last velocity evaluation (RK4 stage 4) of the last timed step"
     s_b, P_b = run(cfg_b, args.steps, 1)
     P0 = torch.stack([X, Y, Z])
@@ -554,7 +559,8 @@ def measured_traffic(kernel: str, args) -> tuple:
     """HBM bytes per launch of `kernel` from the committed counter summary of
     the default level-9 learned step (FETCH_SIZE / WRITE_SIZE passes scaled by
     a 256 MiB copy); None for other workloads."""
-    defaults = (args.level, args.leaf, args.theta_max, args.max_order, args.learned_cell) == (9, 128, 0.85, 7, 1024)
+    defaults = ((args.level, args.leaf, args.theta_max, args.max_order, args.learned_cell, args.tau, args.no_stereo,
+                 args.stereo_extra, args.stereo_ratio) == (9, 128, 0.85, 7, 1024, 7e-7, False, 1, 1.5))
     f = ROOT / "profiles" / "tree_eval_lists_l9_traffic.json"
     if not defaults or not f.exists():
         return None, None
@@ -778,7 +784,7 @@ def main() -> int:
                    "launch_ms": ms32}
     direct_roof["frac"] = direct_roof["achieved"] / FP32_PEAK
     direct_roof["frac_nonpacked"] = direct_roof["achieved"] / FP32_NONPACKED
-    roof = {"learned": tree_roof(st, learned_eval_ms, "tree_eval_lists<7, true, true>"),
+    roof = {"learned": tree_roof(st, learned_eval_ms, lists_kernel(args)),
             "batch": tree_roof(batch_stats, batch_eval_ms, "tree_eval_batch"),
             "direct_f32c": direct_roof}[step_name]
     if step_name == "learned":

@@ -297,7 +297,7 @@ __device__ inline double uniform_d(double v) {
 
 __device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t0, double t1, double t2) {
     StereoBatch s;
-    s.proj = bt.cz >= 0.0 ? 0 : 1;
+    s.proj = __builtin_amdgcn_readfirstlane(bt.cz >= 0.0 ? 0 : 1);
     const bool good = stereo_point(s.proj, t0, t1, t2, s.zr, s.zi) || !valid;
     s.ok = __ballot(!good) == 0ull;
     const double inv = 1.0 / double(bt.nvalid);
@@ -319,17 +319,30 @@ __device__ __forceinline__ void stereo_node(int P, int64_t gi, const StereoView&
     const float wr = dr * ir2, wi = -di * ir2;
     const float rho = float(g.z);
     const float xr = rho * wr, xi = rho * wi;
+    // the node's 16 terms (one 128-byte line, scalar loads: gi is wave-uniform)
+    // are loaded together; the switch enters the unrolled Horner sum at term P
     const float2* a = sv.smom32 + gi * kStereoTerms;
-    float sr = a[P].x, si = a[P].y;
-    for (int k = P - 1; k >= 0; --k) {
-        const float2 ak = a[k];
-        const float tr = fmaf(sr, xr, fmaf(-si, xi, ak.x));
-        si = fmaf(sr, xi, fmaf(si, xr, ak.y));
-        sr = tr;
+    float2 am[kStereoTerms];
+#pragma unroll
+    for (int k = 0; k < kStereoTerms; ++k) am[k] = a[k];
+    float sr = 0.f, si = 0.f;
+#define LPM_HORNER(K)                                          \
+    case K: {                                                  \
+        const float tr = fmaf(sr, xr, fmaf(-si, xi, am[K].x)); \
+        si = fmaf(sr, xi, fmaf(si, xr, am[K].y));              \
+        sr = tr;                                               \
+    }                                                          \
+        [[fallthrough]];
+    switch (P) {
+        LPM_HORNER(15) LPM_HORNER(14) LPM_HORNER(13) LPM_HORNER(12) LPM_HORNER(11) LPM_HORNER(10)
+        LPM_HORNER(9) LPM_HORNER(8) LPM_HORNER(7) LPM_HORNER(6) LPM_HORNER(5) LPM_HORNER(4)
+        LPM_HORNER(3) LPM_HORNER(2) LPM_HORNER(1) LPM_HORNER(0)
+        default: break;
     }
+#undef LPM_HORNER
     Sr += double(fmaf(sr, wr, -si * wi));
     Si += double(fmaf(sr, wi, si * wr));
-    G += double(a[0].x);
+    G += double(am[0].x);
 }
 
 // The unscaled velocity sum sum g (t x y) / (1 - t.y) of the plane series
