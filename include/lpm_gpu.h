@@ -164,11 +164,13 @@ int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
  * treecodes are cut at the octree cells of the shallowest level with at least
  * n_targets / value occupied cells (default 256). */
 #define LPM_OPT_BATCH_CELL_TARGETS 1
-/* LPM_OPT_STEREO (0 or 1, default 0): lpm_tree_moments* also makes the
- * stereographic moments of every node, and the FP32C learned evaluation with
+/* LPM_OPT_STEREO (0 or 1, default 0): lpm_tree_moments* of order p also makes
+ * the stereographic moments of every node (orders up to p +
+ * LPM_OPT_STEREO_EXTRA_ORDER, at most 15), and the FP32C learned evaluation with
  * delta = 0 evaluates a far pair that the model accepts at order p as the
  * plane Cauchy series of the sphere kernel in stereographic coordinates, of
- * order p + LPM_OPT_STEREO_EXTRA_ORDER (at most 15; default extra 1), when
+ * order p + LPM_OPT_STEREO_EXTRA_ORDER (default extra 1, capped by the
+ * moments made), when
  * the pair's plane ratio (rho_n + r_b) / |z_b - z_c| is at most
  * LPM_OPT_STEREO_RATIO (default 1.0) times its ratio (r_n + r_b) / s on the
  * sphere.  Every target batch uses the projection from the pole opposite its

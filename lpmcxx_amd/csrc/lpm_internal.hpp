@@ -117,8 +117,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
 int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign);
 // streamfunction moments mu_k = sum_j gamma_j (y_j - c)^k of every node (one component)
 int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p);
-// stereographic node centres, radii and moments of both projections (stereo.hip)
-int tree_stereo_moments(lpm_ctx* c);
+// stereographic node centres, radii and moments of both projections for
+// Cartesian orders up to p (stereo.hip)
+int tree_stereo_moments(lpm_ctx* c, int p);
 int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);

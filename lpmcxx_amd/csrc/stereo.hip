@@ -10,6 +10,7 @@
 #include "lpm_stereo.hpp"
 #include "lpm_tree.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace lpm {
@@ -41,8 +42,8 @@ __device__ inline void store_node(int64_t i, bool ok, double cr, double ci, doub
     }
 }
 
-// Leaves; thread = (projection, node).
-__global__ __launch_bounds__(256) void stereo_leaf(int nn, const int* __restrict__ nkids, const int* __restrict__ nbeg,
+// Leaves; thread = (projection, node); terms k < T (the others stay 0).
+__global__ __launch_bounds__(256) void stereo_leaf(int nn, int T, const int* __restrict__ nkids, const int* __restrict__ nbeg,
                                                    const int* __restrict__ nend, const double4* __restrict__ ncent,
                                                    const double4* __restrict__ spos, double4* __restrict__ sgeo,
                                                    double2* __restrict__ smom, float2* __restrict__ smom32) {
@@ -66,11 +67,13 @@ __global__ __launch_bounds__(256) void stereo_leaf(int nn, const int* __restrict
         double pr = s.w, pim = 0.0;  // g (z - z_c)^k
 #pragma unroll
         for (int k = 0; k < kStereoTerms; ++k) {
-            a[k].x += pr;
-            a[k].y += pim;
-            const double t = pr * dr - pim * di;
-            pim = pr * di + pim * dr;
-            pr = t;
+            if (k < T) {
+                a[k].x += pr;
+                a[k].y += pim;
+                const double t = pr * dr - pim * di;
+                pim = pr * di + pim * dr;
+                pr = t;
+            }
         }
     }
     store_node(i, ok, cr, ci, sqrt(rho2), a, sgeo, smom, smom32);
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(256) void stereo_leaf(int nn, const int* __restrict
 // Internal nodes of one level (first node f, cnt nodes); thread = (projection, node).
 // With d = z_c' - z_c, a_k = sum_l C(k, l) a'_l d^(k-l): the in-place
 // transform b_k += d b_(k-1) for k = K-1 .. j, j = 1 .. K-1.
-__global__ __launch_bounds__(256) void stereo_m2m(int f, int cnt, int nn, const int* __restrict__ nfirst,
+__global__ __launch_bounds__(256) void stereo_m2m(int f, int cnt, int nn, int T, const int* __restrict__ nfirst,
                                                   const int* __restrict__ nkids, const double4* __restrict__ ncent,
                                                   double4* __restrict__ sgeo, double2* __restrict__ smom,
                                                   float2* __restrict__ smom32) {
@@ -108,8 +111,10 @@ __global__ __launch_bounds__(256) void stereo_m2m(int f, int cnt, int nn, const 
         for (int j = 1; j < kStereoTerms; ++j)
 #pragma unroll
             for (int k = kStereoTerms - 1; k >= j; --k) {
-                b[k].x += dr * b[k - 1].x - di * b[k - 1].y;
-                b[k].y += dr * b[k - 1].y + di * b[k - 1].x;
+                if (k < T) {
+                    b[k].x += dr * b[k - 1].x - di * b[k - 1].y;
+                    b[k].y += dr * b[k - 1].y + di * b[k - 1].x;
+                }
             }
 #pragma unroll
         for (int k = 0; k < kStereoTerms; ++k) {
@@ -122,8 +127,11 @@ __global__ __launch_bounds__(256) void stereo_m2m(int f, int cnt, int nn, const 
 
 }  // namespace
 
-int tree_stereo_moments(lpm_ctx* c) {
+int tree_stereo_moments(lpm_ctx* c, int p) {
     Tree* t = c->tree;
+    // an accepted pair of Cartesian order q <= p becomes a series of order
+    // q + extra: terms up to p + extra are needed
+    const int T = std::min(kStereoTerms, p + c->stereo_extra + 1);
     hipStream_t st = c->stream;
     const int64_t nn = t->nn;
     LPM_CHECK(t->sgeo.reserve(size_t(2 * nn) * sizeof(double4)));
@@ -132,16 +140,16 @@ int tree_stereo_moments(lpm_ctx* c) {
     double4* g = t->sgeo.as<double4>();
     double2* m = t->smom.as<double2>();
     float2* m32 = t->smom32.as<float2>();
-    hipLaunchKernelGGL(stereo_leaf, dim3(blocks(2 * nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+    hipLaunchKernelGGL(stereo_leaf, dim3(blocks(2 * nn)), dim3(256), 0, st, int(nn), T, t->nkids.as<int>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(), g, m, m32);
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(stereo_m2m, dim3(blocks(2 * cnt)), dim3(256), 0, st, int(f), int(cnt), int(nn),
+        hipLaunchKernelGGL(stereo_m2m, dim3(blocks(2 * cnt)), dim3(256), 0, st, int(f), int(cnt), int(nn), T,
                            t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), g, m, m32);
     }
     LPM_HIP(c, hipGetLastError());
-    t->stereo_terms = kStereoTerms;
+    t->stereo_terms = T;
     return LPM_OK;
 }
 

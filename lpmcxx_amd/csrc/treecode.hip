@@ -585,7 +585,7 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
                        t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
     t->stereo_terms = 0;
-    if (c->stereo) LPM_CHECK(tree_stereo_moments(c));
+    if (c->stereo) LPM_CHECK(tree_stereo_moments(c, p));
     t->order = p;
     t->moment_sign = sign;
     return LPM_OK;

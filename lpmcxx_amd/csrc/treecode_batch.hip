@@ -441,11 +441,11 @@ struct Lists {
     int* overflow;  // [0] a batch overflowed, [1] far and [2] near capacity it needs
 };
 
-// six waves per SIMD (77 VGPRs, no spills): the traversal waits on node
-// loads.  STEREO: accepted pairs may become stereographic entries (five
-// waves: the batch's plane data would spill at six).
+// six waves per SIMD (76 VGPRs, 78 with STEREO, no spills): the traversal
+// waits on node loads.  STEREO: accepted pairs may become stereographic
+// entries (the batch's plane centre and radius are kept in scalar registers).
 template <bool STEREO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEREO ? 5 : 6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
                                                          int max_order, double u_ref,
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
                                                          Lists ls, StereoView sv,

@@ -79,7 +79,9 @@ def test_learned_rk4_step_under_the_default_root_box(ctx, mesh, stereo):
         T.build(ctx, src, m.max_radius, 128)
         T.moments(ctx, g, 12, 1)
         ls = LN.labels(ctx, src, 0.85, u_ref, batch_stride=4, keep_rate=0.5, frontier_levels=2, max_samples=1 << 21)
-        LN.fit(ctx, ls, 7e-7 if stereo else 3e-7, 7, epochs=8, stereo_extra=1 if stereo else None)
+        # per-pair bounds with margin below 1e-5 at this level (the stereographic
+        # series is more accurate at equal order, so its bound is larger)
+        LN.fit(ctx, ls, 5e-7 if stereo else 2e-7, 7, epochs=8, stereo_extra=1 if stereo else None)
         ctx.set_option(L.LPM_OPT_STEREO, 1 if stereo else 0)
         ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, 1)
         ctx.set_option(L.LPM_OPT_STEREO_RATIO, 1.5)
