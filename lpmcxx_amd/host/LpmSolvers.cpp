@@ -462,6 +462,14 @@ void TreeSum::ensureMoments(int order) {
         _uref += std::abs(g.back());
     }
     _uref /= 4.0 * PI;
+    const bool stereo = _learned && _trained && _model.stereoExtra >= 0;
+    _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO, stereo ? 1.0 : 0.0), "TreeSum::computeMeshMoments");
+    if (stereo) {
+        _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO_EXTRA_ORDER, double(_model.stereoExtra)),
+                    "TreeSum::computeMeshMoments");
+        _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO_RATIO, _model.stereoRatio),
+                    "TreeSum::computeMeshMoments");
+    }
     _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), order, _momentSign), "TreeSum::computeMeshMoments");
     _haveMoments = true;
     _momentOrder = order;
@@ -472,6 +480,8 @@ float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
     if (!pm) throw std::logic_error("TreeSum::trainOrderModel: mesh expired");
     if (cfg.maxOrder < 1 || cfg.maxOrder > 12 || !(cfg.tau > 0.0))
         throw std::invalid_argument("TreeSum::trainOrderModel: maxOrder in 1..12 and tau > 0");
+    if (cfg.stereoExtra < -1 || cfg.stereoExtra > 14 || !(cfg.stereoRatio > 0.0 && cfg.stereoRatio <= 4.0))
+        throw std::invalid_argument("TreeSum::trainOrderModel: stereoExtra in -1..14, stereoRatio in (0, 4]");
     if (_momentSign != 1) setMomentSign(1);
     ensureMoments(12);
     std::vector<scalar_type> tx, ty, tz;
@@ -488,7 +498,9 @@ float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
                 "TreeSum::trainOrderModel");
     if (ns == 0) throw std::runtime_error("TreeSum::trainOrderModel: no candidate pairs (mesh too small?)");
     // class = lowest order meeting tau; 0 = descend, also at a leaf whose
-    // direct sum is cheaper than the series (9 instructions per term, 11 per pair)
+    // direct sum is cheaper than the series (Cartesian: 9 instructions per
+    // term, 11 per pair; stereographic: 25 per entry + 4 per complex term, 13
+    // per pair)
     std::vector<int32_t> y(ns);
     for (int64_t i = 0; i < ns; ++i) {
         int p = 0;
@@ -496,7 +508,10 @@ float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
             if (err[size_t(i) * LPM_LABEL_ORDERS + q - 1] <= cfg.tau) { p = q; break; }
         const float* f = &feat[size_t(i) * LPM_MODEL_INPUTS];
         const double npart = std::exp2(double(f[6]));
-        if (p > 0 && f[9] > 0.5f && 9.0 * (p * (p + 1) * (p + 2) / 6) >= 11.0 * npart) p = 0;
+        const double far = cfg.stereoExtra >= 0 ? 25.0 + 4.0 * (p + cfg.stereoExtra + 1)
+                                                : 9.0 * (p * (p + 1) * (p + 2) / 6);
+        const double near = (cfg.stereoExtra >= 0 ? 13.0 : 11.0) * npart;
+        if (p > 0 && f[9] > 0.5f && far >= near) p = 0;
         y[i] = p;
     }
     float mean[LPM_MODEL_INPUTS] = {}, inv[LPM_MODEL_INPUTS];
@@ -522,6 +537,7 @@ float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
     _model = cfg;
     _trained = true;
     _learned = true;
+    _haveMoments = false;  // meshSolve makes the moments of its order (and the stereographic ones)
     return loss;
 }
 

@@ -307,6 +307,11 @@ struct OrderModelConfig {
     int labelStride = 16;         // every labelStride-th target batch is labelled
     scalar_type keepRate = 0.3;   // fraction of its candidate pairs kept
     uint64_t seed = 1;
+    // stereographic far field (LPM_OPT_STEREO) in meshSolve: -1 = off, else the
+    // extra order of the plane series (the bench uses 1 with tau 7e-7 and
+    // stereoRatio 1.5); leaves are then priced as stereographic entries
+    int stereoExtra = -1;
+    scalar_type stereoRatio = 1.5;
 };
 
 class TreeSum : public Tree {

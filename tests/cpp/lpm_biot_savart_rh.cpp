@@ -110,10 +110,20 @@ int main() {
         int macL = 0;
         lt.meshSolve(mpiVerts, mpiFaces, 0.0, macL);
         const double lerr = maxRelErr(*pm->getFaceFieldPtr("learnedVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
+        // the same with the stereographic far field
+        TreeSum st(8, 64, pm, kernel, "stereoVelocity", "vorticity");
+        cfg.stereoExtra = 1;
+        st.trainOrderModel(cfg);
+        st.setPrecision(LPM_F32C);
+        int macS = 0;
+        st.meshSolve(mpiVerts, mpiFaces, 0.0, macS);
+        const double serr = maxRelErr(*pm->getFaceFieldPtr("stereoVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
         std::printf("cubed level 6: batch tree vs direct %.3e (far pairs %d), learned vs direct %.3e (far pairs %d, "
-                    "training loss %.3f)\n", berr, macB, lerr, macL, loss);
+                    "training loss %.3f), learned FP32C with the stereographic far field %.3e (far pairs %d)\n",
+                    berr, macB, lerr, macL, loss, serr, macS);
         CHECK(berr < 1e-4 && macB > 0);
         CHECK(lerr < 1e-3 && macL > 0);
+        CHECK(serr < 1e-3 && macS > 0);
         CHECK(loss < 1.5f);
     }
     // a kernel the GPU cannot evaluate is the reference's "invalid kernel"
