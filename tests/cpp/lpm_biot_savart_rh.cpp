@@ -86,7 +86,7 @@ int main() {
         // batch acceptance and the learned order model through the same API
         CubedSphereSeed cseed;
         auto pm = std::make_shared<MeshedParticles>(cseed, 6, true, 1.0, 0);
-        for (const char* n : {"velocity", "batchVelocity", "learnedVelocity"}) {
+        for (const char* n : {"velocity", "batchVelocity", "learnedVelocity", "stereoVelocity"}) {
             pm->createVertexField(n, "length/time", 3);
             pm->createFaceField(n, "length/time", 3);
         }
