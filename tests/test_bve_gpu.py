@@ -80,6 +80,32 @@ def test_bve_tree_follows_direct(ctx, mesh):
             assert np.abs(np.stack(a) - np.stack(b)).max() < 1e-7
 
 
+@pytest.mark.parametrize("stereo", [False, True])
+def test_bve_learned_follows_direct(ctx, mesh, stereo):
+    """The learned FP32C treecode (a constant model: every pair passing
+    theta_max = 0.4 at order 7), with the Cartesian and with the
+    stereographic far field, follows the direct-sum BVE over 5 steps."""
+    from lpmcxx_amd import learned as LN
+    m = mesh(L.LPM_SEED_CUBED, 5)
+    src, verts = (m.cx, m.cy, m.cz), (m.vx, m.vy, m.vz)
+    za = L.gaussian_vortex(m.cx, m.cy, m.cz, m.area) + 4 * np.pi * m.cz
+    d = ctx.bve_steps(L.SolverConfig.default(method=L.LPM_METHOD_DIRECT), 0.01, 2 * np.pi, 5, src, za, m.area, verts)
+    u_ref = LN.u_ref_of((za - 4 * np.pi * m.cz) * m.area)
+    try:
+        ctx.set_option(L.LPM_OPT_STEREO, 1 if stereo else 0)
+        LN.constant_model(ctx, 7)
+        cfg = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=7, theta=0.4,
+                                     leaf_max=64, u_ref=u_ref)
+        r = ctx.bve_steps(cfg, 0.01, 2 * np.pi, 5, src, za, m.area, verts)
+        from lpmcxx_amd import tree as T
+        assert (T.stats(ctx)["far_stereo"] > 0) == stereo
+        err = max(np.abs(np.stack(a) - np.stack(b)).max() for a, b in zip(r, d))
+        print(f"\nlearned BVE (stereographic {stereo}) vs direct after 5 steps: max position difference {err:.2e}")
+        assert err < 3e-7  # order 7 at theta 0.4: ~1e-7 in both forms
+    finally:
+        ctx.set_option(L.LPM_OPT_STEREO, 0)
+
+
 def test_bve_edge_cases(ctx, mesh):
     m = mesh(L.LPM_SEED_ICOS_TRI, 2)
     cfg = L.SolverConfig.default()
