@@ -560,7 +560,7 @@ def measured_traffic(kernel: str, args) -> tuple:
     the default level-9 learned step (FETCH_SIZE / WRITE_SIZE passes scaled by
     a 256 MiB copy); None for other workloads."""
     defaults = ((args.level, args.leaf, args.theta_max, args.max_order, args.learned_cell, args.tau, args.no_stereo,
-                 args.stereo_extra, args.stereo_ratio) == (9, 128, 0.85, 7, 1024, 7e-7, False, 1, 1.5))
+                 args.stereo_extra, args.stereo_ratio) == (9, 128, 0.92, 7, 1024, 7e-7, False, 1, 2.0))
     f = ROOT / "profiles" / "tree_eval_lists_l9_traffic.json"
     if not defaults or not f.exists():
         return None, None
@@ -578,14 +578,14 @@ def main() -> int:
     ap.add_argument("--level", type=int, default=9)
     # learned treecode
     ap.add_argument("--leaf", type=int, default=128)
-    ap.add_argument("--theta-max", type=float, default=0.85)
+    ap.add_argument("--theta-max", type=float, default=None, help="default 0.92 (stereographic), 0.85 (Cartesian)")
     ap.add_argument("--tau", type=float, default=None,
                     help="per-pair error bound of the labels (default 7e-7 with the stereographic far field, "
                          "3e-7 without)")
     ap.add_argument("--no-stereo", action="store_true",
                     help="Cartesian far field only (no LPM_OPT_STEREO in the learned evaluation)")
     ap.add_argument("--stereo-extra", type=int, default=1, help="LPM_OPT_STEREO_EXTRA_ORDER")
-    ap.add_argument("--stereo-ratio", type=float, default=1.5, help="LPM_OPT_STEREO_RATIO")
+    ap.add_argument("--stereo-ratio", type=float, default=2.0, help="LPM_OPT_STEREO_RATIO")
     ap.add_argument("--max-order", type=int, default=7)
     ap.add_argument("--margin", type=int, default=0)
     ap.add_argument("--learned-cell", type=int, default=1024, help="targets per batch cell (learned treecode)")
@@ -619,6 +619,8 @@ def main() -> int:
     ap.add_argument("--psi-theta", type=float, default=0.4)
     ap.add_argument("--bve-tau", type=float, default=None, help="default 4e-7 (stereographic), 2e-7 (Cartesian)")
     args = ap.parse_args()
+    if args.theta_max is None:
+        args.theta_max = 0.85 if args.no_stereo else 0.92
     if args.tau is None:
         args.tau = 3e-7 if args.no_stereo else 7e-7
     if args.l10_tau is None:

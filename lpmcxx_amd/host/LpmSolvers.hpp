@@ -308,10 +308,11 @@ struct OrderModelConfig {
     scalar_type keepRate = 0.3;   // fraction of its candidate pairs kept
     uint64_t seed = 1;
     // stereographic far field (LPM_OPT_STEREO) in meshSolve: -1 = off, else the
-    // extra order of the plane series (the bench uses 1 with tau 7e-7 and
-    // stereoRatio 1.5); leaves are then priced as stereographic entries
+    // extra order of the plane series (the bench uses 1 with tau 7e-7,
+    // thetaMax 0.92 and stereoRatio 2.0); leaves are then priced as
+    // stereographic entries
     int stereoExtra = -1;
-    scalar_type stereoRatio = 1.5;
+    scalar_type stereoRatio = 2.0;
 };
 
 class TreeSum : public Tree {
