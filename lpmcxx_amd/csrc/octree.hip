@@ -259,6 +259,27 @@ __global__ __launch_bounds__(256) void tree_leaf_keys(int nn, const int* __restr
     for (int p = nbeg[i] + threadIdx.x; p < nend[i]; p += blockDim.x) tkeys[perm[p]] = keys[p] & mask;
 }
 
+// Leaf-local ascending source order without a sort: block per leaf (at most
+// kRankMax sources), each source's position is the number of the leaf's
+// sources with a smaller index.
+constexpr int kRankMax = 1024;
+__global__ __launch_bounds__(256) void tree_leaf_rank(int nn, const int* __restrict__ nkids,
+                                                      const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                      const int* __restrict__ perm_in, int* __restrict__ perm_out) {
+    __shared__ int v[kRankMax];
+    const int i = blockIdx.x;
+    if (i >= nn || nkids[i] != 0) return;
+    const int b = nbeg[i], m = nend[i] - b;
+    for (int t = threadIdx.x; t < m; t += blockDim.x) v[t] = perm_in[b + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        const int x = v[t];
+        int r = 0;
+        for (int k = 0; k < m; ++k) r += v[k] < x ? 1 : 0;
+        perm_out[b + r] = x;
+    }
+}
+
 __global__ __launch_bounds__(256) void iota_i32(int* v, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) v[i] = int(i);
@@ -488,15 +509,21 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                        bhi[2], t->nlevel.as<int>(), t->nprefix.as<uint64_t>(), t->ncent.as<double4>(),
                        t->nbox.as<double>());
     if (n > 0) {
-        // leaf-local ascending source order: keys truncated to the leaf depth, sorted from source order
-        hipLaunchKernelGGL(tree_leaf_keys, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
-                           t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
-                           t->perm.as<int>(), t->tmp0.as<uint64_t>());
-        hipLaunchKernelGGL(iota_i32, dim3(blocks(n)), dim3(256), 0, st, t->tmp1.as<int>(), n);
-        // the resorted order goes to kperm, then the two swap: perm = leaf
-        // order, kperm = the order of the sorted keys
-        LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->tmp2.as<uint64_t>(), t->tmp1.as<int>(),
-                             t->kperm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
+        // leaf-local ascending source order goes to kperm, then the two swap:
+        // perm = leaf order, kperm = the order of the sorted keys.  Leaves of
+        // at most kRankMax sources (max-per-node control): ranks within the
+        // leaf; else keys truncated to the leaf depth, sorted from source order.
+        if (control == 0 && leaf_max <= kRankMax) {
+            hipLaunchKernelGGL(tree_leaf_rank, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+                               t->nbeg.as<int>(), t->nend.as<int>(), t->perm.as<int>(), t->kperm.as<int>());
+        } else {
+            hipLaunchKernelGGL(tree_leaf_keys, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+                               t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
+                               t->perm.as<int>(), t->tmp0.as<uint64_t>());
+            hipLaunchKernelGGL(iota_i32, dim3(blocks(n)), dim3(256), 0, st, t->tmp1.as<int>(), n);
+            LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->tmp2.as<uint64_t>(), t->tmp1.as<int>(),
+                                 t->kperm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
+        }
         std::swap(t->perm, t->kperm);
         hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
                            t->spos.as<double4>());
