@@ -380,7 +380,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
 }
 
 static_assert(kLvTotal <= kPinWords, "the build table is read through the pinned words");
-constexpr int kFastLevels = 12;  // key levels sorted before the first level loop
+constexpr int kFastLevels = 10;  // key levels sorted before the first level loop (30 bits: 4 radix passes)
 constexpr int kLevelGroup = 4;   // levels built between two reads of the build table
 
 }  // namespace
