@@ -8,7 +8,9 @@ resident in HBM before timing starts.
 
 One step = one velocity evaluation of all particles by the learned-order
 FP32C treecode: tree build + moments + model-planned evaluation, the work an
-RK4 stage repeats.  value = N (N - 1) / step time (effective interactions/s),
+RK4 stage repeats.  Accepted pairs whose plane geometry qualifies are
+evaluated as stereographic Cauchy series (LPM_OPT_STEREO; --no-stereo keeps
+the Cartesian series throughout).  value = N (N - 1) / step time (effective interactions/s),
 valid only if the step's relative L2 error against the FP64 direct sum of
 this run is <= 1e-5; otherwise the fixed-order FP32C batch treecode is the
 step (the fallback), and if that misses too, the FP32C direct sum.
