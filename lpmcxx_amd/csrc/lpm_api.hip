@@ -81,6 +81,12 @@ void lpm_destroy(lpm_ctx* c) {
     if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->aux) {
+        (void)hipStreamSynchronize(c->aux);
+        (void)hipStreamDestroy(c->aux);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

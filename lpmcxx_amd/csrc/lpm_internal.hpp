@@ -68,6 +68,10 @@ struct lpm_ctx {
     bool ev_valid = false;
     // pinned host words for small device-to-host reads (build table, flags)
     int* pin = nullptr;
+    // second stream (default priority) and its fork / join events: the
+    // stereographic moments run beside the Cartesian ones (made on first use)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace lpm {
@@ -118,8 +122,8 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign);
 // streamfunction moments mu_k = sum_j gamma_j (y_j - c)^k of every node (one component)
 int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p);
 // stereographic node centres, radii and moments of both projections for
-// Cartesian orders up to p (stereo.hip)
-int tree_stereo_moments(lpm_ctx* c, int p);
+// Cartesian orders up to p (stereo.hip), launched on stream st
+int tree_stereo_moments(lpm_ctx* c, int p, hipStream_t st);
 int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delta, const double* tx,
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);

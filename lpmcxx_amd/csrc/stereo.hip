@@ -127,12 +127,11 @@ __global__ __launch_bounds__(256) void stereo_m2m(int f, int cnt, int nn, int T,
 
 }  // namespace
 
-int tree_stereo_moments(lpm_ctx* c, int p) {
+int tree_stereo_moments(lpm_ctx* c, int p, hipStream_t st) {
     Tree* t = c->tree;
     // an accepted pair of Cartesian order q <= p becomes a series of order
     // q + extra: terms up to p + extra are needed
     const int T = std::min(kStereoTerms, p + c->stereo_extra + 1);
-    hipStream_t st = c->stream;
     const int64_t nn = t->nn;
     LPM_CHECK(t->sgeo.reserve(size_t(2 * nn) * sizeof(double4)));
     LPM_CHECK(t->smom.reserve(size_t(2 * nn) * kStereoTerms * sizeof(double2)));

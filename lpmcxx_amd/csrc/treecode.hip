@@ -580,12 +580,24 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     const int nterm = nterms(p);
     LPM_CHECK(t->nstat.reserve(size_t(t->nn) * sizeof(float4)));
     LPM_CHECK(set_strengths(c, gamma));
+    t->stereo_terms = 0;
+    if (c->stereo) {
+        // the stereographic moments on the second stream, beside the Cartesian ones
+        if (!c->aux) {
+            LPM_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+            LPM_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            LPM_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        }
+        LPM_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+        LPM_HIP(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+        LPM_CHECK(tree_stereo_moments(c, p, c->aux));
+        LPM_HIP(c, hipEventRecord(c->ev_join, c->aux));
+    }
     LPM_CHECK(moments_pass<3>(c, p, sign, t->mom, t->momg, t->mom32));
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(t->nn)), dim3(256), 0, c->stream, int(t->nn), p, nterm,
                        t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
-    t->stereo_terms = 0;
-    if (c->stereo) LPM_CHECK(tree_stereo_moments(c, p));
+    if (c->stereo) LPM_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     t->order = p;
     t->moment_sign = sign;
     return LPM_OK;
