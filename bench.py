@@ -610,7 +610,7 @@ def main() -> int:
     ap.add_argument("--no-rk4-l10", action="store_true", help="skip the level-10 RK4 comparison")
     ap.add_argument("--l10-level", type=int, default=10)
     ap.add_argument("--l10-steps", type=int, default=10)
-    ap.add_argument("--l10-tau", type=float, default=None, help="default 3.5e-7 (stereographic), 1.5e-7 (Cartesian)")
+    ap.add_argument("--l10-tau", type=float, default=None, help="default 5e-7 (stereographic), 1.5e-7 (Cartesian)")
     ap.add_argument("--l10-label-stride", type=int, default=64)
     ap.add_argument("--workload", choices=["learned", "mesh", "psi", "bve"], default="learned",
                     help="learned: the velocity step (default); mesh: GPU mesh generation; psi: the "
@@ -619,16 +619,16 @@ def main() -> int:
     ap.add_argument("--psi-leaf", type=int, default=256)
     ap.add_argument("--psi-order", type=int, default=6)
     ap.add_argument("--psi-theta", type=float, default=0.4)
-    ap.add_argument("--bve-tau", type=float, default=None, help="default 4e-7 (stereographic), 2e-7 (Cartesian)")
+    ap.add_argument("--bve-tau", type=float, default=None, help="default 6e-7 (stereographic), 2e-7 (Cartesian)")
     args = ap.parse_args()
     if args.theta_max is None:
         args.theta_max = 0.85 if args.no_stereo else 0.92
     if args.tau is None:
         args.tau = 3e-7 if args.no_stereo else 7e-7
     if args.l10_tau is None:
-        args.l10_tau = 1.5e-7 if args.no_stereo else 3.5e-7
+        args.l10_tau = 1.5e-7 if args.no_stereo else 5e-7
     if args.bve_tau is None:
-        args.bve_tau = 2e-7 if args.no_stereo else 4e-7
+        args.bve_tau = 2e-7 if args.no_stereo else 6e-7
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
