@@ -16,7 +16,7 @@ this run is <= 1e-5; otherwise the fixed-order FP32C batch treecode is the
 step (the fallback), and if that misses too, the FP32C direct sum.
 
 Measured in the same run:
-  * FP64 direct sum (the error judge) and FP32C direct sum, with the roofline
+  * FP64 direct sum (the error reference) and FP32C direct sum, with the roofline
     of their pair kernels;
   * the plain FP64 GPU treecode (reference acceptance per target, Cartesian
     series, fixed p and theta) and the fixed-order FP32C batch treecode;
@@ -242,7 +242,7 @@ def workload_psi(args) -> dict:
     """Streamfunction treecode (lpm_tree_*_psi) on the particles of the
     default workload (cubed L9, 8 Gaussian vortices): one step = tree build +
     streamfunction moments + FP32C batch evaluation of psi at every particle.
-    Judged against the FP64 direct streamfunction on 8,192 random targets
+    Errors are measured against the FP64 direct streamfunction on 8,192 random targets
     (self pair excluded); the FP64 streamfunction treecode and the FP32C
     direct sum are timed beside it."""
     import numpy as np

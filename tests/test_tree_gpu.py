@@ -124,7 +124,7 @@ def test_theta_zero_is_direct_sum(ctx, mesh):
 
 def test_cartesian_accuracy_and_f32c(ctx, mesh):
     """Cubed-sphere level 6 (24,576 faces), leaf 128: error against the FP64
-    direct sum at theta 0.7/p 8 (the survey's config 3 settings) and at
+    direct sum at theta 0.7/p 8 (the reference test's cubed level 8 settings) and at
     theta 0.5/p 10; the FP32C near field stays within 1e-5 of FP64."""
     m, src, g, tgt = _case(mesh, L.LPM_SEED_CUBED, 6)
     ref = O.bs_direct(tgt, src, g)
