@@ -9,10 +9,21 @@
  * Entry points without the _dev suffix take HOST pointers, copy to device,
  * run, copy back and synchronize.  The _dev variants take DEVICE pointers,
  * enqueue on the context's stream and return without synchronizing, with
- * two exceptions: lpm_tree_build_dev waits for the device to size the node
- * table (once per 4 tree levels; it returns the node count and depth), and
+ * these exceptions: lpm_tree_build_dev waits for the device to size the node
+ * table (once per 4 tree levels; it returns the node count and depth),
  * lpm_tree_eval_learned_dev waits once at its end to check that every
- * batch's interaction lists fit (they are grown and planned again if not).
+ * batch's interaction lists fit (they are grown and planned again if not),
+ * and the sphere check below waits once for the points it reads.
+ *
+ * Sphere precondition.  LPM_F32C forms R^2 - x.y as |x - y|^2 / 2, and the
+ * Cartesian, stereographic and logarithmic series expand kernels of
+ * |x - y|^2; both are equal to the reference's R^2 - x.y only when |x| = R
+ * for every target and source.  The entry points using these forms (the
+ * direct sum with LPM_F32C; every treecode form but LPM_F64 with
+ * LPM_SERIES_REFERENCE, where R = 1; the RK4 / BVE steps with such a
+ * solver) check that | |x|^2 - R^2 | <= 1e-12 R^2 for every point and
+ * return LPM_ERR_INVALID_ARG otherwise.  Targets that are the tree's own
+ * source arrays are not read again (the build records its sources' range).
  *
  * A context owns one HIP stream, one growable device workspace and at most
  * one source octree.  Use one context per host thread.
@@ -78,7 +89,8 @@ void* lpm_stream(lpm_ctx* ctx);
  *   potential psi_i = -1/(4 pi R) sum_{j != skip_i} gamma_j log(R^2 - x_i.y_j)
  *             (src/LpmScalarKernel.h:35-38).
  * kernels  : LPM_K_BIOT_SAVART_SPHERE and/or LPM_K_GREENS_SPHERE (bit mask).
- * prec     : LPM_F64 or LPM_F32C.
+ * prec     : LPM_F64 or LPM_F32C (LPM_F32C: every point on |x| = radius,
+ *             see the sphere precondition above).
  * gamma_j  : source strength (vorticity x area in the mesh solve).
  * tgt_self_src : per-target source index excluded from the potential sum
  *             (the face-target self pair of src/LpmDirectSum.cpp:52-75), -1 for

@@ -195,6 +195,58 @@ def _ptr(a) -> int | None:
     raise TypeError(f"unsupported buffer type {type(a)}")
 
 
+def _as_host(arrays: dict, n: int | None = None, dtype=np.float64) -> list:
+    """C-contiguous host copies (numpy, dtype) of named arrays, all of length n
+    (or of the first array's length); ValueError before any native call
+    otherwise, so the copies to the device never over-read a short array."""
+    out = []
+    for name, a in arrays.items():
+        if a is None:
+            out.append(None)
+            continue
+        if hasattr(a, "device") and getattr(a.device, "type", "cpu") != "cpu":
+            raise ValueError(f"{name}: host array expected, got a tensor on {a.device}")
+        v = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+        if n is None:
+            n = v.size
+        if v.size != n:
+            raise ValueError(f"{name} has {v.size} entries, expected {n}")
+        out.append(v)
+    return out
+
+
+def _numel(t, name: str) -> int:
+    """Entry count of a torch tensor (TypeError for anything else)."""
+    if not hasattr(t, "numel") or not hasattr(t, "data_ptr"):
+        raise TypeError(f"{name}: torch tensor expected, got {type(t).__name__}")
+    return t.numel()
+
+
+def _dev(ctx, tensors: dict, n: int, dtype: str = "float64", optional: tuple = ()) -> None:
+    """Checks device tensors before a *_dev call: each a contiguous torch tensor
+    of the given dtype with n entries on the context's GPU (names listed in
+    `optional` may be None).  Raises ValueError / TypeError otherwise: the
+    kernels take n from one argument and would read or write the others out
+    of bounds."""
+    import torch
+    want = {"float64": torch.float64, "int64": torch.int64}[dtype]
+    for name, t in tensors.items():
+        if t is None:
+            if name in optional:
+                continue
+            raise ValueError(f"{name} is required")
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name}: torch tensor expected, got {type(t).__name__}")
+        if t.dtype != want:
+            raise ValueError(f"{name}: dtype {t.dtype}, expected {want}")
+        if t.device.type != "cuda" or (t.device.index if t.device.index is not None else 0) != ctx.device:
+            raise ValueError(f"{name}: on {t.device}, expected cuda:{ctx.device}")
+        if t.numel() != n:
+            raise ValueError(f"{name} has {t.numel()} entries, expected {n}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+
+
 # ---------------------------------------------------------------------- mesh
 class Mesh:
     """Vertices and leaf faces of a refined spherical mesh (reference index order)."""
@@ -296,11 +348,10 @@ class Context:
     def direct_sum(self, targets, sources, gamma, *, kernels=LPM_K_BIOT_SAVART_SPHERE, prec=LPM_F64,
                    radius=1.0, delta=0.0, skip=None):
         """Host-array direct sum; returns (ux, uy, uz) and/or psi as float64 arrays."""
-        tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
-        sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
+        sx, sy, sz, g = _as_host({"sx": sources[0], "sy": sources[1], "sz": sources[2], "gamma": gamma})
         nt, ns = tx.size, sx.size
-        sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.int64)
+        (sk,) = _as_host({"skip": skip}, nt, np.int64)
         vel = bool(kernels & LPM_K_BIOT_SAVART_SPHERE)
         pot = bool(kernels & LPM_K_GREENS_SPHERE)
         ux = np.zeros(nt) if vel else None
@@ -318,6 +369,13 @@ class Context:
                        prec=LPM_F64, radius=1.0, delta=0.0, skip=None, psi=None):
         """Device-tensor direct sum enqueued on the context stream (no synchronize)."""
         ux, uy, uz = out if out is not None else (None, None, None)
+        nt, ns = _numel(tx, "tx"), _numel(sx, "sx")
+        vel, pot = bool(kernels & LPM_K_BIOT_SAVART_SPHERE), bool(kernels & LPM_K_GREENS_SPHERE)
+        _dev(self, {"tx": tx, "ty": ty, "tz": tz}, nt)
+        _dev(self, {"sx": sx, "sy": sy, "sz": sz, "gamma": gamma}, ns)
+        _dev(self, {"ux": ux, "uy": uy, "uz": uz}, nt, optional=() if vel else ("ux", "uy", "uz"))
+        _dev(self, {"psi": psi}, nt, optional=() if pot else ("psi",))
+        _dev(self, {"skip": skip}, nt, "int64", optional=("skip",))
         self.after_torch()
         self.check(self.lib.lpm_direct_sum_dev(self.h, kernels, prec, radius, delta, _ptr(tx), _ptr(ty), _ptr(tz),
                                                tx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(gamma), sx.numel(),
@@ -326,9 +384,10 @@ class Context:
 
 def _rk4_steps(self, cfg, dt, omega, nsteps, x, y, z, zeta_abs, area):
     """nsteps RK4 steps on host arrays; returns new (x, y, z) copies."""
-    x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in (x, y, z))
-    za = np.ascontiguousarray(zeta_abs, dtype=np.float64)
-    ar = np.ascontiguousarray(area, dtype=np.float64)
+    x, y, z, za, ar = _as_host({"x": x, "y": y, "z": z, "zeta_abs": zeta_abs, "area": area})
+    x, y, z = x.copy(), y.copy(), z.copy()
+    if cfg.method != LPM_METHOD_DIRECT:
+        self.tree_sources = None
     self.check(self.lib.lpm_rk4_steps(self.h, C.byref(cfg), dt, omega, nsteps, x.size, _ptr(x), _ptr(y), _ptr(z),
                                       _ptr(za), _ptr(ar)))
     return x, y, z
@@ -336,6 +395,9 @@ def _rk4_steps(self, cfg, dt, omega, nsteps, x, y, z, zeta_abs, area):
 
 def _rk4_step_dev(self, cfg, dt, omega, x, y, z, zeta_abs, area):
     """One RK4 step on device tensors, in place, enqueued on the context stream."""
+    _dev(self, {"x": x, "y": y, "z": z, "zeta_abs": zeta_abs, "area": area}, _numel(x, "x"))
+    if cfg.method != LPM_METHOD_DIRECT:
+        self.tree_sources = None  # the step rebuilds the context's tree
     self.after_torch()
     self.check(self.lib.lpm_rk4_step_dev(self.h, C.byref(cfg), dt, omega, x.numel(), _ptr(x), _ptr(y), _ptr(z),
                                          _ptr(zeta_abs), _ptr(area)))
@@ -345,14 +407,20 @@ def _bve_steps(self, cfg, dt, omega, nsteps, src, zeta_abs, area, tracers=None):
     """nsteps barotropic-vorticity RK4 steps of vortex particles src = (x, y, z)
     (with zeta_abs, area) and passive tracers = (x, y, z) on host arrays;
     returns the new (src, tracers) positions."""
-    nt = 0 if tracers is None else len(tracers[0])
-    ns = len(src[0])
-    if nt:
-        x, y, z = (np.concatenate([np.asarray(a, np.float64), np.asarray(b, np.float64)]) for a, b in zip(src, tracers))
+    sx, sy, sz, za, ar = _as_host({"src x": src[0], "src y": src[1], "src z": src[2], "zeta_abs": zeta_abs,
+                                "area": area})
+    ns = sx.size
+    if tracers is not None:
+        qx, qy, qz = _as_host({"tracer x": tracers[0], "tracer y": tracers[1], "tracer z": tracers[2]})
+        nt = qx.size
     else:
-        x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in src)
-    za = np.ascontiguousarray(zeta_abs, dtype=np.float64)
-    ar = np.ascontiguousarray(area, dtype=np.float64)
+        nt = 0
+    if nt:
+        x, y, z = np.concatenate([sx, qx]), np.concatenate([sy, qy]), np.concatenate([sz, qz])
+    else:
+        x, y, z = sx.copy(), sy.copy(), sz.copy()
+    if cfg.method != LPM_METHOD_DIRECT:
+        self.tree_sources = None
     self.check(self.lib.lpm_bve_steps(self.h, C.byref(cfg), dt, omega, nsteps, ns, nt, _ptr(x), _ptr(y), _ptr(z),
                                       _ptr(za), _ptr(ar)))
     return (x[:ns], y[:ns], z[:ns]), ((x[ns:], y[ns:], z[ns:]) if nt else None)
@@ -361,6 +429,13 @@ def _bve_steps(self, cfg, dt, omega, nsteps, src, zeta_abs, area, tracers=None):
 def _bve_step_dev(self, cfg, dt, omega, n_src, x, y, z, zeta_abs, area):
     """One BVE step on device tensors (first n_src entries are vortex particles,
     the rest tracers), in place, enqueued on the context stream."""
+    n = _numel(x, "x")
+    if not 0 <= n_src <= n:
+        raise ValueError(f"n_src = {n_src} outside [0, {n}]")
+    _dev(self, {"x": x, "y": y, "z": z}, n)
+    _dev(self, {"zeta_abs": zeta_abs, "area": area}, n_src)
+    if cfg.method != LPM_METHOD_DIRECT:
+        self.tree_sources = None
     self.after_torch()
     self.check(self.lib.lpm_bve_step_dev(self.h, C.byref(cfg), dt, omega, n_src, x.numel() - n_src, _ptr(x), _ptr(y),
                                          _ptr(z), _ptr(zeta_abs), _ptr(area)))

@@ -39,6 +39,45 @@ int hip_fail(lpm_ctx* c, hipError_t e, const char* what) {
     return fail(c, LPM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+namespace {
+__global__ __launch_bounds__(256) void r2_range_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, int64_t n,
+                                                       unsigned long long* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool v = i < n;
+    const double r2 = v ? x[i] * x[i] + y[i] * y[i] + z[i] * z[i] : 0.0;
+    r2_block_range(r2, v, out);
+}
+}  // namespace
+
+int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double* r2min,
+             double* r2max) {
+    LPM_CHECK(c->r2w.reserve(2 * sizeof(unsigned long long)));
+    unsigned long long* w = c->r2w.as<unsigned long long>();
+    LPM_HIP(c, hipMemsetAsync(w, 0, 2 * sizeof(unsigned long long), c->stream));
+    if (n > 0)
+        hipLaunchKernelGGL(r2_range_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream, x, y, z, n, w);
+    LPM_HIP(c, hipGetLastError());
+    unsigned long long* h = reinterpret_cast<unsigned long long*>(c->pin);
+    LPM_HIP(c, hipMemcpyAsync(h, w, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    *r2min = r2_from_words(h, 0);
+    *r2max = r2_from_words(h, 1);
+    return LPM_OK;
+}
+
+int check_on_sphere(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
+                    const char* what) {
+    if (n <= 0) return LPM_OK;
+    double lo = 0.0, hi = 0.0;
+    LPM_CHECK(r2_range(c, x, y, z, n, &lo, &hi));
+    if (!r2_on_sphere(lo, hi, R))
+        return fail(c, LPM_ERR_INVALID_ARG,
+                    "%s: this form needs |x| = R = %.17g (|x|^2 in [%.17g, %.17g]); use LPM_F64 with the reference "
+                    "series for points off the sphere", what, R, lo, hi);
+    return LPM_OK;
+}
+
 }  // namespace lpm
 
 using namespace lpm;
@@ -170,12 +209,22 @@ static int check_direct_args(lpm_ctx* c, int kernels, int prec, double radius, d
     return LPM_OK;
 }
 
+// LPM_F32C: targets and sources on |x| = radius
+static int check_direct_sphere(lpm_ctx* c, int prec, double radius, const double* tx, const double* ty,
+                               const double* tz, int64_t nt, const double* sx, const double* sy, const double* sz,
+                               int64_t ns) {
+    if (prec != LPM_F32C || nt == 0) return LPM_OK;
+    LPM_CHECK(check_on_sphere(c, tx, ty, tz, nt, radius, "lpm_direct_sum (LPM_F32C) targets"));
+    return check_on_sphere(c, sx, sy, sz, ns, radius, "lpm_direct_sum (LPM_F32C) sources");
+}
+
 int lpm_direct_sum_dev(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
                        const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
                        const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy,
                        double* uz, double* psi) {
     LPM_CHECK(check_direct_args(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, ux, uy, uz, psi));
     LPM_HIP(c, hipSetDevice(c->device));
+    LPM_CHECK(check_direct_sphere(c, prec, radius, tx, ty, tz, nt, sx, sy, sz, ns));
     return direct_sum_device(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz,
                              psi);
 }
@@ -210,6 +259,7 @@ int lpm_direct_sum(lpm_ctx* c, int kernels, int prec, double radius, double delt
         LPM_HIP(c, h2d(dg, gamma, ns * sizeof(double)));
     }
     if (skip) LPM_HIP(c, hipMemcpyAsync(dskip, skip, nt * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    LPM_CHECK(check_direct_sphere(c, prec, radius, dtx, dty, dtz, nt, dsx, dsy, dsz, ns));
     LPM_CHECK(direct_sum_device(c, kernels, prec, radius, delta, dtx, dty, dtz, nt, dsx, dsy, dsz, dg, ns, dskip,
                                 vel ? o : nullptr, vel ? o + nt : nullptr, vel ? o + 2 * nt : nullptr,
                                 pot ? o + 3 * nt : nullptr));

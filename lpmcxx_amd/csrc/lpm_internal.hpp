@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -55,6 +57,8 @@ struct lpm_ctx {
     lpm::DevBuf tb_work, tl_work;
     // level tables of the GPU mesh generator
     lpm::DevBuf mesh_work;
+    // two words of the sphere check (r2_range)
+    lpm::DevBuf r2w;
     int far_cap = 1024, near_cap = 512;
     int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
     int stereo = 0;                // LPM_OPT_STEREO
@@ -128,13 +132,15 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);
 // Morton order of the targets in the tree's root box (tperm: sorted position
-// -> target index, device); also clears the evaluation counters.
+// -> target index, device); also clears the evaluation counters.  *levels
+// (nullable) receives the number of top key levels by which the sorted keys
+// are ordered: below them, neighbouring keys keep the input order.
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
-                          int** tperm);
+                          int** tperm, int* levels = nullptr);
 // Cell-aligned target batches: the Morton-sorted targets split at the
 // boundaries of the octree cells of level L (the shallowest level with at
-// least nt / batch_cell_targets occupied cells) and cut into runs of at most
-// 64.  Enqueued without waiting: bstart[b] = first sorted position of batch
+// least nt / batch_cell_targets occupied cells, and at most the number of
+// key levels the sort ordered) and cut into runs of at most 64.  Enqueued without waiting: bstart[b] = first sorted position of batch
 // b, bstart[nb] = nt; the count nb stays in device memory (*nb_dev) and
 // *nb_bound >= nb bounds it for launch sizes.
 int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
@@ -148,6 +154,63 @@ int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double
 int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin, int max_order, double u_ref,
                              double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
                              double* ux, double* uy, double* uz);
+
+// ---------------------------------------------------------------- sphere precondition
+// The |x - y|^2 / 2 forms of R^2 - x.y (LPM_F32C, and the Cartesian,
+// stereographic and logarithmic series) hold only for points on the sphere
+// |x| = R.  Public entry points using them check every point: |x|^2 within
+// kSphereTol R^2 of R^2 (positions normalized in FP64 are within a few 1e-16).
+constexpr double kSphereTol = 1e-12;
+inline bool r2_on_sphere(double r2min, double r2max, double R) {
+    const double R2 = R * R;
+    return std::fabs(r2min - R2) <= kSphereTol * R2 && std::fabs(r2max - R2) <= kSphereTol * R2;
+}
+// Block-wide range of r2 (every thread of the block calls it; `valid` false
+// for threads without a point) folded into out[0] = max of ~bits(min r2) and
+// out[1] = max of bits(max r2): r2 >= 0, so its bits order like its value
+// (NaN orders above +inf and fails the check).  Both words start at 0.
+__device__ inline void r2_block_range(double r2, bool valid, unsigned long long* out) {
+    __shared__ unsigned long long s_lo[16], s_hi[16];
+    unsigned long long lo = valid ? ~static_cast<unsigned long long>(__double_as_longlong(r2)) : 0ull;
+    unsigned long long hi = valid ? static_cast<unsigned long long>(__double_as_longlong(r2)) : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a > lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_lo[w] = lo; s_hi[w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < int(blockDim.x >> 6); ++k) {
+            lo = s_lo[k] > lo ? s_lo[k] : lo;
+            hi = s_hi[k] > hi ? s_hi[k] : hi;
+        }
+        // most blocks change neither word: skip their atomics
+        if (lo > out[0]) atomicMax(&out[0], lo);
+        if (hi > out[1]) atomicMax(&out[1], hi);
+    }
+}
+inline double r2_from_words(const unsigned long long* w, int which) {
+    const unsigned long long b = which == 0 ? ~w[0] : w[1];
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+// min and max of x^2 + y^2 + z^2 over n device points (waits for the stream)
+int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double* r2min,
+             double* r2max);
+// LPM_ERR_INVALID_ARG unless every one of the n device points lies on |x| = R
+int check_on_sphere(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
+                    const char* what);
+
+// The sphere precondition of the treecode forms that need it (every form but
+// LPM_F64 with LPM_SERIES_REFERENCE): the tree's sources (range recorded by
+// the build) and the targets on the unit sphere.  Targets that are the tree's
+// sources are not read again; others are (one wait for the stream).
+int check_tree_sphere(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                      const char* what);
 
 void model_free(Model* m);
 const unsigned short* model_weights(lpm_ctx* c);  // device BF16 weights + FP32 biases, or nullptr

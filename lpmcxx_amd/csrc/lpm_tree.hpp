@@ -24,7 +24,10 @@ constexpr int kLvKeys = kLvWords + 2;     // a node below the sorted key levels 
 constexpr int kLvSpread = 64;
 constexpr int kLvLeaves = kLvWords + 3;
 constexpr int kLvChunks = kLvLeaves + kLvSpread;
-constexpr int kLvTotal = kLvChunks + kLvSpread;
+// two 64-bit words (8-byte aligned): the range of |x|^2 over the sources
+// (r2_block_range), for the sphere precondition of the series
+constexpr int kLvR2 = (kLvChunks + kLvSpread + 1) & ~1;
+constexpr int kLvTotal = kLvR2 + 4;
 
 struct Tree {
     int64_t ns = 0;       // sources
@@ -32,12 +35,14 @@ struct Tree {
     int64_t nleaves = 0;
     int depth = 0;
     int leaf_max = 0;
+    double r2min = 0.0, r2max = 0.0;  // range of |x|^2 over the sources
     double root_box[6] = {0, 0, 0, 0, 0, 0};  // xmin ymin zmin xmax ymax zmax
     // source data in tree order (device)
     DevBuf perm;      // int32 [ns]  tree position -> source index
     DevBuf spos;      // double4 [ns] (x, y, z, gamma) in tree order (gamma set by moments)
     DevBuf keys;      // uint64 [ns] Morton keys, sorted on their top kFastLevels levels (or fully)
     DevBuf kperm;     // int32 [ns]  source index in the order of `keys` (before the leaf resort)
+    int sorted_levels = 0;  // top key levels by which `keys` is sorted (kFastLevels or kMaxDepth)
     // source arrays of the last build (device), so that targets which are the
     // sources can reuse the key order instead of sorting again
     const double *sx = nullptr, *sy = nullptr, *sz = nullptr;

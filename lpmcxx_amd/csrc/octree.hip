@@ -38,8 +38,11 @@ __global__ __launch_bounds__(256) void tree_keys(const double* __restrict__ x, c
                                                  int* __restrict__ lvt) {
     int* outside = lvt + kLvOutside;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
+    const bool valid = i < n;
+    const double px = valid ? x[i] : 0.0, py = valid ? y[i] : 0.0, pz = valid ? z[i] : 0.0;
+    r2_block_range(px * px + py * py + pz * pz, valid,
+                   reinterpret_cast<unsigned long long*>(lvt + kLvR2));
+    if (!valid) return;
     if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
     double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
     uint64_t key = 0;
@@ -496,6 +499,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     t->level_start.push_back(nn);
     t->nn = nn;
     t->depth = depth;
+    t->sorted_levels = sorted_levels;
+    t->r2min = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 0);
+    t->r2max = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 1);
     t->nleaves = 0;
     t->nch = 0;
     for (int w = 0; w < kLvSpread; ++w) {

@@ -142,6 +142,14 @@ static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_
     return LPM_OK;
 }
 
+// the solver forms that need every particle on the unit sphere (every form
+// but the LPM_F64 direct sum and the LPM_F64 reference-series treecode)
+static bool needs_sphere(const lpm_solver_config* cfg) {
+    if (cfg->prec == LPM_F32C) return true;
+    if (cfg->method == LPM_METHOD_DIRECT) return false;
+    return cfg->method != LPM_METHOD_TREE || cfg->series == LPM_SERIES_CARTESIAN;
+}
+
 static int steps_host(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double omega, int nsteps, int64_t ns,
                       int64_t nt, double* x, double* y, double* z, const double* zeta_abs, const double* area) {
     if (nsteps < 0) return fail(c, LPM_ERR_INVALID_ARG, "nsteps must be >= 0");
@@ -160,6 +168,7 @@ static int steps_host(lpm_ctx* c, const lpm_solver_config* cfg, double dt, doubl
         LPM_HIP(c, hipMemcpyAsync(dz, zeta_abs, bs, hipMemcpyHostToDevice, st));
         LPM_HIP(c, hipMemcpyAsync(da, area, bs, hipMemcpyHostToDevice, st));
     }
+    if (needs_sphere(cfg)) LPM_CHECK(check_on_sphere(c, d, d + n, d + 2 * n, n, 1.0, "RK4 particles"));
     for (int s = 0; s < nsteps; ++s) LPM_CHECK(rk4_step_device(c, *cfg, dt, omega, ns, nt, d, d + n, d + 2 * n, dz, da));
     LPM_HIP(c, hipMemcpyAsync(x, d, b, hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipMemcpyAsync(y, d + n, b, hipMemcpyDeviceToHost, st));
@@ -189,6 +198,7 @@ int lpm_rk4_step_dev(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double
                      double* y, double* z, const double* zeta_abs, const double* area) {
     LPM_CHECK(check_rk4(c, cfg, dt, n, 0, x, y, z, zeta_abs, area));
     LPM_HIP(c, hipSetDevice(c->device));
+    if (needs_sphere(cfg)) LPM_CHECK(check_on_sphere(c, x, y, z, n, 1.0, "lpm_rk4_step_dev particles"));
     return rk4_step_device(c, *cfg, dt, omega, n, 0, x, y, z, zeta_abs, area);
 }
 
@@ -202,6 +212,8 @@ int lpm_bve_step_dev(lpm_ctx* c, const lpm_solver_config* cfg, double dt, double
                      int64_t n_tracer, double* x, double* y, double* z, const double* zeta_abs, const double* area) {
     LPM_CHECK(check_rk4(c, cfg, dt, n_src, n_tracer, x, y, z, zeta_abs, area));
     LPM_HIP(c, hipSetDevice(c->device));
+    if (needs_sphere(cfg))
+        LPM_CHECK(check_on_sphere(c, x, y, z, n_src + n_tracer, 1.0, "lpm_bve_step_dev particles"));
     return rk4_step_device(c, *cfg, dt, omega, n_src, n_tracer, x, y, z, zeta_abs, area);
 }
 

@@ -371,14 +371,19 @@ __global__ __launch_bounds__(256) void batch_flags(const int* __restrict__ cs, i
 }
 
 // The batching level L: the shallowest level with at least `want` occupied
-// cells (hist[l] = adjacent sorted keys whose common prefix is l levels).
-// One thread.
-__global__ void batch_level(const unsigned* __restrict__ hist, int64_t want, int* __restrict__ level) {
-    int L = kMaxDepth;
+// cells (hist[l] = adjacent sorted keys whose common prefix is l levels), and
+// at most max_level, the key levels by which the keys are sorted.  Up to
+// max_level every cell is one run of the sorted keys, so the cells of level L
+// number fewer than 8 want (fewer than want when max_level caps L).  Below
+// it, neighbouring keys are in input order and a cell would fall apart into
+// runs of single targets.  One thread.
+__global__ void batch_level(const unsigned* __restrict__ hist, int64_t want, int max_level,
+                            int* __restrict__ level) {
+    int L = max_level;
     int64_t cells = 1;
-    for (int l = 0; l <= kMaxDepth; ++l) {
+    for (int l = 0; l < max_level; ++l) {
         if (cells >= want) { L = l; break; }
-        if (l < kMaxDepth) cells += hist[l];
+        cells += hist[l];
     }
     *level = L;
 }
@@ -615,7 +620,7 @@ int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
 constexpr int kTargetLevels = 12;  // key levels that order the targets
 
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
-                          int** tperm) {
+                          int** tperm, int* levels) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     const int64_t na = std::max<int64_t>(nt, 1);
@@ -628,9 +633,11 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
     c->stats_ptr = reinterpret_cast<unsigned long long*>(sa);
     LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
     *tperm = v_out;
+    if (levels) *levels = kTargetLevels;
     if (nt == 0) return LPM_OK;
     if (nt == t->ns && tx == t->sx && ty == t->sy && tz == t->sz) {
         // the targets are the tree's sources: their keys are sorted already
+        if (levels) *levels = t->sorted_levels;
         LPM_HIP(c, hipMemcpyAsync(k_out, t->keys.ptr, size_t(nt) * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
         LPM_HIP(c, hipMemcpyAsync(v_out, t->kperm.ptr, size_t(nt) * sizeof(int), hipMemcpyDeviceToDevice, st));
         return LPM_OK;
@@ -652,7 +659,8 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
 
 int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
                         int** bstart, int64_t* nb_bound, const int** nb_dev) {
-    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm));
+    int sorted_levels = 0;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm, &sorted_levels));
     *nb_bound = 0;
     if (nt == 0) return LPM_OK;
     hipStream_t st = c->stream;
@@ -679,7 +687,7 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
     LPM_HIP(c, hipMemsetAsync(hist, 0, 128, st));
     hipLaunchKernelGGL(batch_lcp_hist, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, hist);
-    hipLaunchKernelGGL(batch_level, dim3(1), dim3(1), 0, st, hist, want, level);
+    hipLaunchKernelGGL(batch_level, dim3(1), dim3(1), 0, st, hist, want, sorted_levels, level);
     hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, level, cs);
     size_t b1 = tmp_bytes;
     LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(tmp, b1, cs, cs, hipcub::Max(), int(nt), st));
@@ -690,10 +698,23 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     hipLaunchKernelGGL(batch_close, dim3(1), dim3(1), 0, st, bs, nsel, int(nt));
     LPM_HIP(c, hipGetLastError());
     // every level-L cell of s targets makes ceil(s / 64) batches; the level
-    // below L has fewer than `want` cells, so L has fewer than 8 want
+    // above L has fewer than `want` cells, so L has fewer than 8 want (L is
+    // never below the sorted key levels, where that would not hold)
     *bstart = bs;
     *nb_bound = std::min<int64_t>(nt, (nt + kWave - 1) / kWave + 8 * want);
     *nb_dev = nsel;
+    return LPM_OK;
+}
+
+int check_tree_sphere(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                      const char* what) {
+    const Tree* t = c->tree;
+    if (t->ns > 0 && !r2_on_sphere(t->r2min, t->r2max, 1.0))
+        return fail(c, LPM_ERR_INVALID_ARG,
+                    "%s: this form needs the tree's sources on the unit sphere (|x|^2 in [%.17g, %.17g]); use "
+                    "LPM_F64 with LPM_SERIES_REFERENCE for other points", what, t->r2min, t->r2max);
+    if (nt > 0 && !(nt == t->ns && tx == t->sx && ty == t->sy && tz == t->sz))
+        return check_on_sphere(c, tx, ty, tz, nt, 1.0, what);
     return LPM_OK;
 }
 
@@ -858,6 +879,8 @@ int lpm_tree_eval_dev(lpm_ctx* c, int prec, int series, double theta, double del
                       int64_t* mac_count) {
     LPM_CHECK(check_eval(c, prec, series, theta, delta, tx, ty, tz, nt, ux, uy, uz));
     LPM_HIP(c, hipSetDevice(c->device));
+    if (prec == LPM_F32C || series == LPM_SERIES_CARTESIAN)
+        LPM_CHECK(check_tree_sphere(c, tx, ty, tz, nt, "lpm_tree_eval"));
     return tree_eval_device(c, prec, series, theta, delta, tx, ty, tz, nt, ux, uy, uz, mac_count, 1.0);
 }
 
@@ -872,6 +895,8 @@ int lpm_tree_eval(lpm_ctx* c, int prec, int series, double theta, double delta, 
     LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    if (prec == LPM_F32C || series == LPM_SERIES_CARTESIAN)
+        LPM_CHECK(check_tree_sphere(c, d, d + nt, d + 2 * nt, nt, "lpm_tree_eval"));
     int64_t mac = 0;
     LPM_CHECK(tree_eval_device(c, prec, series, theta, delta, d, d + nt, d + 2 * nt, nt, d + 3 * nt, d + 4 * nt,
                                d + 5 * nt, &mac, 1.0));

@@ -1153,6 +1153,7 @@ int lpm_tree_eval_batch_dev(lpm_ctx* c, int prec, int order, double theta, doubl
                             const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
     LPM_CHECK(check_batch(c, prec, order, theta, delta, tx, ty, tz, nt, ux, uy, uz));
     LPM_HIP(c, hipSetDevice(c->device));
+    LPM_CHECK(check_tree_sphere(c, tx, ty, tz, nt, "lpm_tree_eval_batch"));
     return tree_eval_batch_device(c, prec, order, theta, delta, tx, ty, tz, nt, ux, uy, uz);
 }
 
@@ -1163,6 +1164,7 @@ int lpm_tree_eval_batch(lpm_ctx* c, int prec, int order, double theta, double de
     if (nt == 0) return tree_eval_batch_device(c, prec, order, theta, delta, tx, ty, tz, 0, ux, uy, uz);
     return with_staged(c, tx, ty, tz, nt, ux, uy, uz,
                        [&](const double* x, const double* y, const double* z, double* u, double* v, double* w) {
+                           LPM_CHECK(check_tree_sphere(c, x, y, z, nt, "lpm_tree_eval_batch"));
                            return tree_eval_batch_device(c, prec, order, theta, delta, x, y, z, nt, u, v, w);
                        });
 }
@@ -1186,6 +1188,7 @@ int lpm_tree_eval_psi_dev(lpm_ctx* c, int prec, int order, double theta, const d
                           const double* tz, int64_t nt, double* psi) {
     LPM_CHECK(check_psi(c, prec, order, theta, tx, ty, tz, nt, psi));
     LPM_HIP(c, hipSetDevice(c->device));
+    LPM_CHECK(check_tree_sphere(c, tx, ty, tz, nt, "lpm_tree_eval_psi"));
     return tree_eval_psi_device(c, prec, order, theta, tx, ty, tz, nt, psi);
 }
 
@@ -1200,6 +1203,7 @@ int lpm_tree_eval_psi(lpm_ctx* c, int prec, int order, double theta, const doubl
     LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_CHECK(check_tree_sphere(c, d, d + nt, d + 2 * nt, nt, "lpm_tree_eval_psi"));
     LPM_CHECK(tree_eval_psi_device(c, prec, order, theta, d, d + nt, d + 2 * nt, nt, d + 3 * nt));
     LPM_HIP(c, hipMemcpyAsync(psi, d + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipStreamSynchronize(st));
@@ -1211,6 +1215,7 @@ int lpm_tree_eval_learned_dev(lpm_ctx* c, int prec, double theta_max, int margin
                               double* ux, double* uy, double* uz) {
     LPM_CHECK(check_learned(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, nt, ux, uy, uz));
     LPM_HIP(c, hipSetDevice(c->device));
+    LPM_CHECK(check_tree_sphere(c, tx, ty, tz, nt, "lpm_tree_eval_learned"));
     return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, nt, ux, uy, uz);
 }
 
@@ -1223,6 +1228,7 @@ int lpm_tree_eval_learned(lpm_ctx* c, int prec, double theta_max, int margin, in
         return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, tx, ty, tz, 0, ux, uy, uz);
     return with_staged(c, tx, ty, tz, nt, ux, uy, uz,
                        [&](const double* x, const double* y, const double* z, double* u, double* v, double* w) {
+                           LPM_CHECK(check_tree_sphere(c, x, y, z, nt, "lpm_tree_eval_learned"));
                            return tree_eval_learned_device(c, prec, theta_max, margin, max_order, u_ref, delta, x, y,
                                                            z, nt, u, v, w);
                        });
@@ -1250,6 +1256,7 @@ int lpm_tree_labels(lpm_ctx* c, double theta_max, double delta, double u_ref, in
     LPM_HIP(c, hipMemcpyAsync(d, tx, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + nt, ty, nt * sizeof(double), hipMemcpyHostToDevice, st));
     LPM_HIP(c, hipMemcpyAsync(d + 2 * nt, tz, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    LPM_CHECK(check_tree_sphere(c, d, d + nt, d + 2 * nt, nt, "lpm_tree_labels"));
     int64_t ns = 0;
     LPM_CHECK(tree_labels_device(c, theta_max, delta, u_ref, batch_stride, keep_rate, frontier_levels,
                                  uint32_t(seed ^ (seed >> 32)),

@@ -259,8 +259,19 @@ void Particles::initializeFieldWithFunction(const std::string& name, const Analy
     }
 }
 
-// src/LpmDirectSum.cpp:175-254.  The singular branch there stops at
-// endIndex(i) instead of endIndex(myRank); the block here is the rank's.
+namespace {
+// The items of every rank's block, [startIndex(0), endIndex(P - 1)] = all of
+// them.  One process runs the solve for all blocks (one GPU holds every
+// block), so after it each rank's Field already holds what the reference's
+// meshBroadcast would have gathered, and the broadcast has nothing to do.
+struct AllBlocks {
+    index_type first, last;
+    explicit AllBlocks(const MPIReplicatedData& m) : first(m.startIndex(0)), last(m.endIndex(m.getSize() - 1)) {}
+};
+}  // namespace
+
+// src/LpmDirectSum.cpp:175-254 over every rank's block.  The singular branch
+// there stops at endIndex(i) instead of endIndex(myRank); not reproduced.
 void DirectSum::meshfreeSolve(const MPIReplicatedData& mp) const {
     auto crd = _srcLocs.lock();
     auto src = _srcVals.lock();
@@ -274,8 +285,8 @@ void DirectSum::meshfreeSolve(const MPIReplicatedData& mp) const {
     const int64_t ns = int64_t(crd->n());
     std::vector<scalar_type> g(ns);
     for (int64_t j = 0; j < ns; ++j) g[j] = src->getScalar(j) * (w ? w->getScalar(j) : 1.0);
-    const int r = mp.getRank();
-    const index_type b = mp.startIndex(r), e = mp.endIndex(r);
+    const AllBlocks blk(mp);
+    const index_type b = blk.first, e = blk.last;
     const int64_t nt = std::max<int64_t>(0, int64_t(e - b + 1));
     std::vector<int64_t> skip(nt);
     for (int64_t i = 0; i < nt; ++i) skip[i] = gk->isSingular() ? int64_t(b + i) : -1;
@@ -290,7 +301,7 @@ void DirectSum::meshfreeSolve(const MPIReplicatedData& mp) const {
 
 namespace {
 // Leaf-face sources (ascending face index) with strength zeta * area and the
-// targets of one rank's blocks: vertices [vs, ve] then leaf faces [fs, fe].
+// targets of every rank's blocks: all vertices, then all leaf faces.
 struct MeshProblem {
     std::vector<scalar_type> sx, sy, sz, g, tx, ty, tz;
     std::vector<int64_t> skip;
@@ -308,14 +319,14 @@ MeshProblem materialize(const MeshedParticles& pm, const Field& src, const MPIRe
         p.sx.push_back(c.x); p.sy.push_back(c.y); p.sz.push_back(c.z);
         p.g.push_back(src.getScalar(j) * pm.getFaceArea(j));
     }
-    const int r = mv.getRank();
-    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+    const AllBlocks bv(mv), bf(mf);
+    for (index_type i = bv.first; i <= bv.last; ++i) {
         const XyzVector v = pm.getVertex(i);
         p.tx.push_back(v.x); p.ty.push_back(v.y); p.tz.push_back(v.z);
         p.skip.push_back(-1);
         p.vtx.push_back(i);
     }
-    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+    for (index_type i = bf.first; i <= bf.last; ++i) {
         if (pm.faceIsDivided(i)) continue;
         p.tx.push_back(p.sx[leafOf[i]]); p.ty.push_back(p.sy[leafOf[i]]); p.tz.push_back(p.sz[leafOf[i]]);
         p.skip.push_back(leafOf[i]);
@@ -422,13 +433,13 @@ void TreeSum::scalarSolve(const MPIReplicatedData& mv, const MPIReplicatedData& 
     _haveMoments = false;  // the near-field strengths were reset
     std::vector<scalar_type> tx, ty, tz;
     std::vector<index_type> vtx, face;
-    const int r = mv.getRank();
-    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+    const AllBlocks bv(mv), bf(mf);
+    for (index_type i = bv.first; i <= bv.last; ++i) {
         const XyzVector v = pm->getVertex(i);
         tx.push_back(v.x); ty.push_back(v.y); tz.push_back(v.z);
         vtx.push_back(i);
     }
-    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+    for (index_type i = bf.first; i <= bf.last; ++i) {
         if (pm->faceIsDivided(i)) continue;
         const XyzVector c = pm->getFaceCentroid(i);
         tx.push_back(c.x); ty.push_back(c.y); tz.push_back(c.z);
@@ -553,13 +564,13 @@ void TreeSum::meshSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf
     ensureMoments(_learned ? std::max(_model.maxOrder, 1) : _maxSeriesOrder);
     std::vector<scalar_type> tx, ty, tz;
     std::vector<index_type> vtx, face;
-    const int r = mv.getRank();
-    for (index_type i = mv.startIndex(r); i <= mv.endIndex(r); ++i) {
+    const AllBlocks bv(mv), bf(mf);
+    for (index_type i = bv.first; i <= bv.last; ++i) {
         const XyzVector v = pm->getVertex(i);
         tx.push_back(v.x); ty.push_back(v.y); tz.push_back(v.z);
         vtx.push_back(i);
     }
-    for (index_type i = mf.startIndex(r); i <= mf.endIndex(r); ++i) {
+    for (index_type i = bf.first; i <= bf.last; ++i) {
         if (pm->faceIsDivided(i)) continue;
         const XyzVector c = pm->getFaceCentroid(i);
         tx.push_back(c.x); ty.push_back(c.y); tz.push_back(c.z);

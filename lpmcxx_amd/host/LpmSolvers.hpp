@@ -269,12 +269,14 @@ class DirectSum {
 
     // Meshfree direct sum over point particles (src/LpmDirectSum.h:22-23):
     // target field = sum_j kernel(x_i, x_j) src_j [weight_j], j != i for a
-    // singular kernel, over the rank block [startIndex, endIndex] of the
-    // particles.  The GPU evaluates SphereGreensFn.
+    // singular kernel.  The GPU evaluates SphereGreensFn.
     DirectSum(std::shared_ptr<Particles> particles, const std::shared_ptr<ScalarKernel> kernel,
               const std::string tgtFieldName, const std::string srcFieldName,
               const std::string srcWeightName = std::string());
 
+    // The solves fill every rank's block [startIndex(r), endIndex(r)], r = 0 ..
+    // nProcs - 1, in this process (one GPU holds all of them), so the Fields
+    // hold after the solve what the reference's solve + meshBroadcast leave.
     void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces) const;
     void meshfreeSolve(const MPIReplicatedData& mpiParticles) const;
     // Every rank's block is computed in this process: nothing to exchange.
@@ -333,8 +335,12 @@ class TreeSum : public Tree {
     void initScalarKernel(ScalarKernel* kernel) { sKern = kernel; vKern = nullptr; }
 
     void computeMeshMoments();
+    // Fills every rank's block in this process (as DirectSum::meshSolve);
+    // macCounter is then the sum over all blocks, the reference's MPI_Reduce
+    // total.
     void meshSolve(const MPIReplicatedData& mpiVerts, const MPIReplicatedData& mpiFaces, const scalar_type mp_tol,
                    int& macCounter);
+    // Every rank's block is computed in this process: nothing to exchange.
     void meshBroadcast(const MPIReplicatedData&, const MPIReplicatedData&) const {}
 
     // Defaults reproduce the reference: moment sign -1 (h = centroid - y),

@@ -22,7 +22,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import LPM_F32C, LPM_F64, Context, _ptr
+from . import LPM_F32C, LPM_F64, Context, _as_host, _dev, _numel, _ptr
 
 NPARAMS = 6288
 NIN = 16
@@ -53,6 +53,19 @@ def nterms(p) -> np.ndarray:
     return p * (p + 1) * (p + 2) // 6
 
 
+def _rows(x, y):
+    """Model rows as float32 [n][16] and labels as int32 [n]: ValueError on
+    other shapes (the kernels read 16 inputs per row and one label per row)."""
+    x = np.ascontiguousarray(x, np.float32)
+    if x.ndim != 2 or x.shape[1] != NIN:
+        raise ValueError(f"model inputs must be [n][{NIN}], got {x.shape}")
+    if y is not None:
+        y = np.ascontiguousarray(y, np.int32).reshape(-1)
+        if y.size != x.shape[0]:
+            raise ValueError(f"{y.size} labels for {x.shape[0]} rows")
+    return x, y
+
+
 class OrderModel:
     """The order model of a context (BF16 MFMA kernels, FP32 master weights)."""
 
@@ -63,10 +76,8 @@ class OrderModel:
         self.ctx.check(self.ctx.lib.lpm_model_init(self.ctx.h, seed))
 
     def set_params(self, params, mean=None, inv_std=None) -> None:
-        p = np.ascontiguousarray(params, np.float32)
-        assert p.size == NPARAMS
-        m = None if mean is None else np.ascontiguousarray(mean, np.float32)
-        s = None if inv_std is None else np.ascontiguousarray(inv_std, np.float32)
+        (p,) = _as_host({"params": params}, NPARAMS, np.float32)
+        m, s = _as_host({"mean": mean, "inv_std": inv_std}, NIN, np.float32)
         self.ctx.check(self.ctx.lib.lpm_model_set_params(self.ctx.h, _ptr(p), _ptr(m), _ptr(s)))
 
     def get_params(self):
@@ -78,9 +89,9 @@ class OrderModel:
     def set_range(self, lo=None, hi=None) -> None:
         """Clamp inputs to [lo, hi] (None: unbounded); training sets the range
         of its rows."""
-        a = None if lo is None else np.ascontiguousarray(lo, np.float32)
-        b = None if hi is None else np.ascontiguousarray(hi, np.float32)
-        assert (a is None) == (b is None) and (a is None or a.size == b.size == NIN)
+        if (lo is None) != (hi is None):
+            raise ValueError("give both lo and hi, or neither")
+        a, b = _as_host({"lo": lo, "hi": hi}, NIN, np.float32)
         self.ctx.check(self.ctx.lib.lpm_model_set_range(self.ctx.h, _ptr(a), _ptr(b)))
 
     def get_range(self):
@@ -89,8 +100,7 @@ class OrderModel:
         return lo, hi
 
     def gradient(self, x, y):
-        x = np.ascontiguousarray(x, np.float32)
-        y = np.ascontiguousarray(y, np.int32)
+        x, y = _rows(x, y)
         g = np.empty(NPARAMS, np.float32)
         loss = C.c_float()
         self.ctx.check(self.ctx.lib.lpm_model_gradient(self.ctx.h, _ptr(x), _ptr(y), len(y), _ptr(g),
@@ -98,15 +108,14 @@ class OrderModel:
         return g, loss.value
 
     def train(self, x, y, epochs: int = 10, batch: int = 4096, lr: float = 2e-3, seed: int = 1) -> float:
-        x = np.ascontiguousarray(x, np.float32)
-        y = np.ascontiguousarray(y, np.int32)
+        x, y = _rows(x, y)
         loss = C.c_float()
         self.ctx.check(self.ctx.lib.lpm_model_train(self.ctx.h, _ptr(x), _ptr(y), len(y), epochs, batch,
                                                     C.c_float(lr), seed, C.byref(loss)))
         return loss.value
 
     def forward(self, x):
-        x = np.ascontiguousarray(x, np.float32)
+        x, _ = _rows(x, None)
         n = x.shape[0]
         logits = np.empty((n, NIN), np.float32)
         cls = np.empty(n, np.int32)
@@ -127,7 +136,7 @@ def labels(ctx: Context, targets, theta_max: float, u_ref: float, delta: float =
     needs moments of order >= 12 with moment_sign +1.  frontier_levels limits
     the candidates to the first levels at and below the first node of each
     branch that passes theta_max: the pairs the learned traversal meets."""
-    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
     f = np.empty((max_samples, NIN), np.float32)
     e = np.empty((max_samples, NORDERS), np.float32)
     ids = np.empty((max_samples, 2), np.int32)
@@ -183,7 +192,7 @@ def normalization(features: np.ndarray):
 
 def evaluate(ctx: Context, targets, theta_max: float, margin: int, max_order: int, u_ref: float,
              delta: float = 0.0, prec: int = LPM_F32C):
-    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
     n = tx.size
     ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
     ctx.check(ctx.lib.lpm_tree_eval_learned(ctx.h, prec, theta_max, margin, max_order, u_ref, delta, _ptr(tx),
@@ -193,6 +202,7 @@ def evaluate(ctx: Context, targets, theta_max: float, margin: int, max_order: in
 
 def evaluate_dev(ctx: Context, tx, ty, tz, out, theta_max: float, margin: int, max_order: int, u_ref: float,
                  delta: float = 0.0, prec: int = LPM_F32C) -> None:
+    _dev(ctx, {"tx": tx, "ty": ty, "tz": tz, "ux": out[0], "uy": out[1], "uz": out[2]}, _numel(tx, "tx"))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_learned_dev(ctx.h, prec, theta_max, margin, max_order, u_ref, delta, _ptr(tx),
                                                 _ptr(ty), _ptr(tz), tx.numel(), _ptr(out[0]), _ptr(out[1]),

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import LPM_F64, LPM_SERIES_CARTESIAN, Context, _ptr
+from . import LPM_F64, LPM_SERIES_CARTESIAN, Context, _as_host, _dev, _numel, _ptr
 
 
 @dataclass
@@ -24,46 +24,58 @@ class TreeResult:
     depth: int
 
 
+def _sources(ctx: Context) -> int:
+    """Source count of the context's tree (set by the build wrappers)."""
+    ns = getattr(ctx, "tree_sources", None)
+    if ns is None:
+        raise ValueError("no tree: build it first")
+    return ns
+
+
 def build(ctx: Context, sources, max_radius: float, leaf_max: int):
-    sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
+    sx, sy, sz = _as_host({"sx": sources[0], "sy": sources[1], "sz": sources[2]})
     nn, depth = C.c_int64(), C.c_int()
     ctx.check(ctx.lib.lpm_tree_build(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.size, max_radius, leaf_max,
                                      C.byref(nn), C.byref(depth)))
+    ctx.tree_sources = sx.size
     return nn.value, depth.value
 
 
 def build_box(ctx: Context, sources, box_min, box_max, control: int, param: int):
     """General root box and depth control (control 0: max per node, 1: max depth)."""
-    sx, sy, sz = (np.ascontiguousarray(a, dtype=np.float64) for a in sources)
-    lo = np.ascontiguousarray(box_min, dtype=np.float64)
-    hi = np.ascontiguousarray(box_max, dtype=np.float64)
+    sx, sy, sz = _as_host({"sx": sources[0], "sy": sources[1], "sz": sources[2]})
+    lo, hi = _as_host({"box_min": box_min, "box_max": box_max}, 3)
     nn, depth = C.c_int64(), C.c_int()
     ctx.check(ctx.lib.lpm_tree_build_box(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.size, _ptr(lo), _ptr(hi), control,
                                          param, C.byref(nn), C.byref(depth)))
+    ctx.tree_sources = sx.size
     return nn.value, depth.value
 
 
 def build_dev(ctx: Context, sx, sy, sz, max_radius: float, leaf_max: int):
     nn, depth = C.c_int64(), C.c_int()
+    _dev(ctx, {"sx": sx, "sy": sy, "sz": sz}, _numel(sx, "sx"))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_build_dev(ctx.h, _ptr(sx), _ptr(sy), _ptr(sz), sx.numel(), max_radius, leaf_max,
                                          C.byref(nn), C.byref(depth)))
+    ctx.tree_sources = sx.numel()
     return nn.value, depth.value
 
 
 def moments(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
-    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    (g,) = _as_host({"gamma": gamma}, _sources(ctx))
     ctx.check(ctx.lib.lpm_tree_moments(ctx.h, _ptr(g), order, moment_sign))
 
 
 def moments_dev(ctx: Context, gamma, order: int, moment_sign: int = 1) -> None:
+    _dev(ctx, {"gamma": gamma}, _sources(ctx))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_moments_dev(ctx.h, _ptr(gamma), order, moment_sign))
 
 
 def evaluate(ctx: Context, targets, theta: float, delta: float = 0.0, prec: int = LPM_F64,
              series: int = LPM_SERIES_CARTESIAN):
-    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
     n = tx.size
     ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
     mac = C.c_int64()
@@ -75,6 +87,7 @@ def evaluate(ctx: Context, targets, theta: float, delta: float = 0.0, prec: int 
 def evaluate_dev(ctx: Context, tx, ty, tz, out, theta: float, delta: float = 0.0, prec: int = LPM_F64,
                  series: int = LPM_SERIES_CARTESIAN, want_mac: bool = False):
     mac = C.c_int64()
+    _dev(ctx, {"tx": tx, "ty": ty, "tz": tz, "ux": out[0], "uy": out[1], "uz": out[2]}, _numel(tx, "tx"))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_dev(ctx.h, prec, series, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
                                         _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
@@ -109,7 +122,7 @@ def stats(ctx: Context) -> dict:
 def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float = 0.0, prec: int = LPM_F64):
     """Batch-acceptance treecode (lpm_tree_eval_batch): one decision per 64
     Morton-sorted targets, Cartesian series of the given order."""
-    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
     n = tx.size
     ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
     ctx.check(ctx.lib.lpm_tree_eval_batch(ctx.h, prec, order, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz), n,
@@ -119,6 +132,7 @@ def evaluate_batch(ctx: Context, targets, theta: float, order: int, delta: float
 
 def evaluate_batch_dev(ctx: Context, tx, ty, tz, out, theta: float, order: int, delta: float = 0.0,
                        prec: int = LPM_F64) -> None:
+    _dev(ctx, {"tx": tx, "ty": ty, "tz": tz, "ux": out[0], "uy": out[1], "uz": out[2]}, _numel(tx, "tx"))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_batch_dev(ctx.h, prec, order, theta, delta, _ptr(tx), _ptr(ty), _ptr(tz),
                                               tx.numel(), _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
@@ -126,11 +140,12 @@ def evaluate_batch_dev(ctx: Context, tx, ty, tz, out, theta: float, order: int, 
 
 def moments_psi(ctx: Context, gamma, order: int) -> None:
     """Streamfunction moments sum_j gamma_j (y_j - c)^k of the current tree."""
-    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    (g,) = _as_host({"gamma": gamma}, _sources(ctx))
     ctx.check(ctx.lib.lpm_tree_moments_psi(ctx.h, _ptr(g), order))
 
 
 def moments_psi_dev(ctx: Context, gamma, order: int) -> None:
+    _dev(ctx, {"gamma": gamma}, _sources(ctx))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_moments_psi_dev(ctx.h, _ptr(gamma), order))
 
@@ -138,13 +153,14 @@ def moments_psi_dev(ctx: Context, gamma, order: int) -> None:
 def evaluate_psi(ctx: Context, targets, theta: float, order: int, prec: int = LPM_F64) -> np.ndarray:
     """Streamfunction treecode (lpm_tree_eval_psi), batch acceptance, log
     series of the given order."""
-    tx, ty, tz = (np.ascontiguousarray(a, dtype=np.float64) for a in targets)
+    tx, ty, tz = _as_host({"tx": targets[0], "ty": targets[1], "tz": targets[2]})
     psi = np.zeros(tx.size)
     ctx.check(ctx.lib.lpm_tree_eval_psi(ctx.h, prec, order, theta, _ptr(tx), _ptr(ty), _ptr(tz), tx.size, _ptr(psi)))
     return psi
 
 
 def evaluate_psi_dev(ctx: Context, tx, ty, tz, psi, theta: float, order: int, prec: int = LPM_F64) -> None:
+    _dev(ctx, {"tx": tx, "ty": ty, "tz": tz, "psi": psi}, _numel(tx, "tx"))
     ctx.after_torch()
     ctx.check(ctx.lib.lpm_tree_eval_psi_dev(ctx.h, prec, order, theta, _ptr(tx), _ptr(ty), _ptr(tz), tx.numel(),
                                             _ptr(psi)))

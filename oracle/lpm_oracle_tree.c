@@ -517,9 +517,9 @@ void oracle_node_far(double cx, double cy, double cz, double tx, double ty, doub
 
 /* ---------------------------------------------------------------- batch acceptance
  * Batch-MAC treecode (no counterpart in the reference; the specification of
- * lpm_tree_eval_batch in include/lpm_gpu.h).  Targets are ordered by 63-bit
- * Morton keys of 21 bisections of the root box (clamped into it), ties by
- * index; batches are runs of at most 64 sorted targets inside one octree
+ * lpm_tree_eval_batch in include/lpm_gpu.h).  Targets are ordered by the
+ * top levels of 63-bit Morton keys of 21 bisections of the root box (clamped
+ * into it), ties by index; batches are runs of at most 64 sorted targets inside one octree
  * cell (oracle_target_batches); node n is accepted when r_n + r_b <
  * theta |c_b - c_n| with r_n the source radius of node_radius.  Batch
  * centroid and radius are
@@ -530,9 +530,21 @@ typedef struct {
     int64_t idx;
 } keyed;
 
+/* The targets are ordered by the top g_target_levels key levels (ties in
+ * index order: a stable sort), and the batch level is at most that: the
+ * GPU sorts general targets on 12 levels and reuses the tree build's key
+ * order (10 levels unless the tree is deeper) for targets that are the
+ * sources.  Below the sorted levels neighbouring keys keep the input order,
+ * so cells there would not be contiguous runs. */
+static int g_target_levels = 12;
+
+void oracle_set_target_levels(int levels) { g_target_levels = levels < 1 ? 1 : (levels > 21 ? 21 : levels); }
+
 static int cmp_keyed(const void* a, const void* b) {
     const keyed *x = (const keyed*)a, *y = (const keyed*)b;
-    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    const int sh = 3 * (21 - g_target_levels);
+    const uint64_t kx = x->key >> sh, ky = y->key >> sh;
+    if (kx != ky) return kx < ky ? -1 : 1;
     return (x->idx > y->idx) - (x->idx < y->idx);
 }
 
@@ -580,8 +592,9 @@ static int key_lcp(uint64_t a, uint64_t b) {
 }
 
 /* Target batches: the sorted targets split where the level-L cell changes,
- * L the shallowest level with at least max(1, n / 256) occupied cells, and
- * cut into runs of at most 64.  Fills order and bstart; returns the count. */
+ * L the shallowest level with at least max(1, n / 256) occupied cells and at
+ * most g_target_levels, and cut into runs of at most 64.  Fills order and
+ * bstart; returns the count. */
 int64_t oracle_target_batches(const double* tx, const double* ty, const double* tz, int64_t n, int64_t* order,
                               int64_t* bstart) {
     if (n == 0) return 0;
@@ -590,11 +603,11 @@ int64_t oracle_target_batches(const double* tx, const double* ty, const double* 
     int64_t hist[22] = {0};
     for (int64_t i = 1; i < n; ++i) hist[key_lcp(keys[i - 1], keys[i])]++;
     const int64_t want = n / 256 > 1 ? n / 256 : 1;
-    int L = 21;
+    int L = g_target_levels;
     int64_t cells = 1;
-    for (int l = 0; l <= 21; ++l) {
+    for (int l = 0; l < g_target_levels; ++l) {
         if (cells >= want) { L = l; break; }
-        if (l < 21) cells += hist[l];
+        cells += hist[l];
     }
     int64_t nb = 0, cs = 0;
     for (int64_t i = 0; i < n; ++i) {

@@ -58,6 +58,8 @@ def _declare_tree(L: C.CDLL) -> None:
         L.oracle_coeffs_series.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_int, _vp]
         L.oracle_coeffs_series.restype = C.c_int
+        L.oracle_set_target_levels.argtypes = [C.c_int]
+        L.oracle_set_target_levels.restype = None
         L.oracle_target_order.argtypes = [_vp, _vp, _vp, C.c_int64, _vp]
         L.oracle_target_order.restype = None
         L.oracle_target_batches.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp]
@@ -211,8 +213,16 @@ def node_far(c, t, sources, gamma, p, moment_sign, series, delta=0.0):
     return out
 
 
+def set_target_levels(levels: int) -> None:
+    """Key levels that order the targets of the batch functions (12: general
+    targets; the tree build's sorted levels, 10 unless the tree is deeper,
+    for targets that are the tree's own sources).  Module state."""
+    lib().oracle_set_target_levels(levels)
+
+
 def target_order(targets):
-    """Morton order of the targets in the last built oracle tree's root box."""
+    """Morton order (top sorted key levels, ties by index) of the targets in
+    the last built oracle tree's root box."""
     tx, ty, tz = (_c(a) for a in targets)
     out = np.empty(tx.size, np.int64)
     lib().oracle_target_order(tx.ctypes.data, ty.ctypes.data, tz.ctypes.data, tx.size, out.ctypes.data)

@@ -4,6 +4,7 @@
 // exact velocity rossbyHaurwitz54Velocity).  Exit status = failures.
 #include "LpmSolvers.hpp"
 
+#include <algorithm>
 #include <cstdio>
 
 using namespace Lpm;
@@ -125,6 +126,59 @@ int main() {
         CHECK(lerr < 1e-3 && macL > 0);
         CHECK(serr < 1e-3 && macS > 0);
         CHECK(loss < 1.5f);
+    }
+    {
+        // several ranks in one process: every rank's solve fills every block,
+        // so each rank's Fields equal the one-rank result (the reference's
+        // solve + meshBroadcast), and macCounter is the total over all blocks
+        auto pm = std::make_shared<MeshedParticles>(seed, 4, true, 1.0, 0);
+        for (const char* n : {"u1", "t1", "uP", "tP"}) {
+            pm->createVertexField(n, "length/time", 3);
+            pm->createFaceField(n, "length/time", 3);
+        }
+        pm->createVertexField("vorticity", "1/time", 1);
+        pm->createFaceField("vorticity", "1/time", 1);
+        pm->initializeVertexFieldWithFunction("vorticity", &y54);
+        pm->initializeFaceFieldWithFunction("vorticity", &y54);
+        MPIReplicatedData v1(pm->nVertices(), 0, 1), f1(pm->nFaces(), 0, 1);
+        DirectSum(pm, kernel, "u1", "vorticity").meshSolve(v1, f1);
+        TreeSum t1(6, 64, pm, kernel, "t1", "vorticity");
+        t1.computeMeshMoments();
+        int mac1 = 0;
+        t1.meshSolve(v1, f1, 0.5, mac1);
+        int same = 0, ranks = 0;
+        for (int P : {2, 3}) {
+            for (int r = 0; r < P; ++r) {
+                MPIReplicatedData vP(pm->nVertices(), r, P), fP(pm->nFaces(), r, P);
+                for (const char* f : {"uP", "tP"}) {
+                    pm->getVertexFieldPtr(f)->initializeToConstant(0.0);
+                    pm->getFaceFieldPtr(f)->initializeToConstant(0.0);
+                }
+                DirectSum ds(pm, kernel, "uP", "vorticity");
+                ds.meshSolve(vP, fP);
+                ds.meshBroadcast(vP, fP);
+                TreeSum tP(6, 64, pm, kernel, "tP", "vorticity");
+                tP.computeMeshMoments();
+                int macP = 0;
+                tP.meshSolve(vP, fP, 0.5, macP);
+                tP.meshBroadcast(vP, fP);
+                bool eq = macP == mac1;
+                for (const char* a : {"u1", "t1"}) {
+                    const char* b = a[0] == 'u' ? "uP" : "tP";
+                    for (int c = 0; c < 3; ++c) {
+                        const Field &va = *pm->getVertexFieldPtr(a), &vb = *pm->getVertexFieldPtr(b);
+                        const Field &fa = *pm->getFaceFieldPtr(a), &fb = *pm->getFaceFieldPtr(b);
+                        eq = eq && std::equal(va.getPtrToData(c), va.getPtrToData(c) + va.n(), vb.getPtrToData(c));
+                        eq = eq && std::equal(fa.getPtrToData(c), fa.getPtrToData(c) + fa.n(), fb.getPtrToData(c));
+                    }
+                }
+                same += eq;
+                ++ranks;
+            }
+        }
+        std::printf("icos level 4, 2 and 3 ranks in one process: %d of %d rank solves equal the one-rank solve "
+                    "(mac %d)\n", same, ranks, mac1);
+        CHECK(same == ranks && mac1 > 0);
     }
     // a kernel the GPU cannot evaluate is the reference's "invalid kernel"
     struct Other : VectorKernel {

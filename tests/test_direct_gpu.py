@@ -46,22 +46,29 @@ def test_full_mesh_parity(ctx, mesh, seed, level):
 
 
 def test_cubed_level7_sampled_targets(ctx, mesh):
-    """Full GPU sum at 98,306 + 98,304 targets; oracle on 2,048 sampled targets."""
+    """Full GPU sum at 98,306 + 98,304 targets, velocity and streamfunction;
+    oracle on 2,048 sampled targets."""
     m = mesh(L.LPM_SEED_CUBED, 7)
     tgt, src, g, skip = _setup(m)
-    u64 = ctx.direct_sum(tgt, src, g, prec=L.LPM_F64)
-    u32 = ctx.direct_sum(tgt, src, g, prec=L.LPM_F32C)
+    u64, p64 = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F64, skip=skip)
+    u32, p32 = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip)
     idx = np.sort(np.random.default_rng(11).choice(len(tgt[0]), 2048, replace=False))
     sub = tuple(t[idx] for t in tgt)
     u_ref = O.bs_direct(sub, src, g, skip=skip[idx])
+    p_ref = O.greens_direct(sub, src, g, skip=skip[idx])
     u64s = tuple(u[idx] for u in u64)
     u32s = tuple(u[idx] for u in u32)
     assert O.rel_l2(u64s, u_ref) <= 1e-12 and O.rel_max(u64s, u_ref) <= 1e-12
-    e = O.rel_l2(u32s, u_ref)
-    print(f"\ncubed L7 f32c rel-L2 {e:.3e} max {O.rel_max(u32s, u_ref):.3e}")
-    assert e <= 1e-5
-    # the analytic Rossby-Haurwitz velocity is approached (first-order method)
+    assert O.rel_l2(p64[idx], p_ref) <= 1e-12 and O.rel_max(p64[idx], p_ref) <= 1e-12
+    e, ep = O.rel_l2(u32s, u_ref), O.rel_l2(p32[idx], p_ref)
+    print(f"\ncubed L7 f32c rel-L2 velocity {e:.3e} (max {O.rel_max(u32s, u_ref):.3e}), "
+          f"streamfunction {ep:.3e} (max {O.rel_max(p32[idx], p_ref):.3e})")
+    assert e <= 1e-5 and ep <= 1e-5
+    # the analytic Rossby-Haurwitz velocity is approached (first-order method),
+    # and psi = zeta / 30 for zeta = Y_5^4 (tests/LpmPoissonSolverTests.cpp:290-301)
     assert O.rel_l2(u64, L.rh54_velocity(*tgt)) < 2e-3
+    zeta_t = L.sphere_harmonic54(*tgt)
+    assert np.linalg.norm(p64 - zeta_t / 30.0) / np.linalg.norm(zeta_t / 30.0) < 5e-3  # 3.1e-2 at L5, 9.0e-3 at L6
 
 
 @pytest.mark.parametrize("nt,ns", [(1, 1), (1, 257), (300, 1), (65, 513), (1000, 255), (4097, 3001)])

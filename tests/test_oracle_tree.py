@@ -203,7 +203,13 @@ def test_target_order_is_stable_morton(mesh):
     order = O.target_order(tuple(P.T.copy()))
     lo, hi = R["box"][0, :3], R["box"][0, 3:]
     key = _morton_keys(P, lo, hi)
-    assert np.array_equal(order, np.argsort(key, kind="stable"))
+    # ordered on the top 12 of the 21 key levels, ties by index
+    assert np.array_equal(order, np.argsort(key >> np.uint64(27), kind="stable"))
+    O.set_target_levels(21)
+    try:
+        assert np.array_equal(O.target_order(tuple(P.T.copy())), np.argsort(key, kind="stable"))
+    finally:
+        O.set_target_levels(12)
 
 
 def test_target_batches_are_cell_aligned(mesh):
@@ -220,7 +226,7 @@ def test_target_batches_are_cell_aligned(mesh):
     # the level of the split: shallowest with >= n/256 occupied cells
     lcp = np.array([21 if a == b else (62 - (int(a ^ b).bit_length() - 1)) // 3 for a, b in zip(key[:-1], key[1:])])
     cells = [1 + np.sum(lcp < l) for l in range(22)]
-    lev = next(l for l in range(22) if cells[l] >= max(1, n // 256))
+    lev = next((l for l in range(12) if cells[l] >= max(1, n // 256)), 12)
     cell_start = np.maximum.accumulate(np.where(np.append(True, lcp < lev), np.arange(n), 0))
     expect = np.flatnonzero((np.arange(n) - cell_start) % 64 == 0)
     assert np.array_equal(bstart, expect)
@@ -280,3 +286,137 @@ def test_pair_errors_match_single_box_series(mesh):
             ref[q - 1] = max(ref[q - 1], np.linalg.norm(far - ex) / (4 * np.pi * u_ref))
     assert np.allclose(err, ref, rtol=1e-9, atol=1e-16)
     assert err[-1] < 1e-3 * err[0]
+
+
+def _node_table(R):
+    """Parent-before-child node order and each node's child list from an exported oracle tree."""
+    nn = len(R["level"])
+    order = np.argsort(R["level"], kind="stable")
+    kids = [list(range(int(R["first_kid"][i]), int(R["first_kid"][i] + R["n_kids"][i]))) for i in range(nn)]
+    return order, kids
+
+
+@pytest.mark.parametrize("level", [4, 5])
+@pytest.mark.parametrize("theta", [0.33, 0.5, 0.7])
+def test_mac_count_recount(mesh, level, theta):
+    """Per-target macCounter of the reference traversal (src/LpmTreeSum.cpp:106-137,
+    MAC :22-24) recounted in numpy from the oracle's node table: a node is
+    counted for a target when maxRadius / distance(centroid, target) < theta
+    and no ancestor was; distance in the reference's operation order."""
+    m = mesh(L.LPM_SEED_CUBED, level)
+    src = (m.cx, m.cy, m.cz)
+    R = O.tree_build(src, m.max_radius, 32)
+    tgt = m.targets()
+    g = np.ones(len(src[0]))
+    mom = O.tree_moments(src, g, 1, 1)
+    _, total, per_target = O.tree_eval(src, g, mom, 1, theta, tgt, series=1)
+    order, kids = _node_table(R)
+    tx, ty, tz = tgt
+    reached = {0: np.ones(tx.size, bool)}
+    count = np.zeros(tx.size, np.int64)
+    for i in order:
+        r = reached.pop(i)
+        c = R["cent"][i]
+        d = np.sqrt((tx - c[0]) * (tx - c[0]) + (ty - c[1]) * (ty - c[1]) + (tz - c[2]) * (tz - c[2]))
+        acc = r & (c[3] / d < theta)
+        count += acc
+        for k in kids[i]:
+            reached[k] = r & ~acc
+    assert np.array_equal(count, per_target)
+    assert total == count.sum() > 0
+
+
+def _butterfly_sum(v):
+    """Sum over the last axis (64 lanes) in the order of a wave's xor butterfly."""
+    v = v.copy()
+    lanes = np.arange(64)
+    for o in (32, 16, 8, 4, 2, 1):
+        v = v + v[..., lanes ^ o]
+    return v[..., 0]
+
+
+@pytest.mark.parametrize("level", [4, 5])
+@pytest.mark.parametrize("theta", [0.33, 0.5, 0.7])
+def test_batch_pair_counts_recount(mesh, level, theta):
+    """Far (batch, node) and near (target, source) pair counts of the batch
+    traversal recounted in numpy: batch centroid by the butterfly sum times
+    1/nv, radius the largest distance; node source radius exact at leaves
+    and max(|c_k - c| + r_k) above; accept when r_n + r_b < theta |c_b - c_n|."""
+    m = mesh(L.LPM_SEED_CUBED, level)
+    src = (m.cx, m.cy, m.cz)
+    R = O.tree_build(src, m.max_radius, 32)
+    tgt = m.targets()
+    g = np.ones(len(src[0]))
+    _, far, near = O.tree_eval_batch(src, g, O.tree_moments(src, g, 1, 1), 1, theta, tgt)
+    order, bstart = O.target_batches(tgt)
+    n = len(tgt[0])
+    bend = np.append(bstart[1:], n)
+    nb = len(bstart)
+    P = np.stack(tgt, 1)[order]
+    T = np.zeros((nb, 64, 3))
+    nv = bend - bstart
+    for b in range(nb):
+        T[b, :nv[b]] = P[bstart[b]:bend[b]]
+    cb = np.stack([_butterfly_sum(T[:, :, d]) * (1.0 / nv) for d in range(3)], 1)
+    lane_ok = np.arange(64)[None, :] < nv[:, None]
+    d2 = ((T[:, :, 0] - cb[:, None, 0]) * (T[:, :, 0] - cb[:, None, 0]) +
+          (T[:, :, 1] - cb[:, None, 1]) * (T[:, :, 1] - cb[:, None, 1]) +
+          (T[:, :, 2] - cb[:, None, 2]) * (T[:, :, 2] - cb[:, None, 2]))
+    rb = np.sqrt(np.where(lane_ok, d2, 0.0).max(1))
+    nodes, kids = _node_table(R)
+    S = np.stack(src, 1)
+    nrad = np.zeros(len(R["level"]))
+    for i in nodes[::-1]:                       # children before parents
+        c = R["cent"][i, :3]
+        if not kids[i]:
+            q = S[R["perm"][R["begin"][i]:R["end"][i]]]
+            nrad[i] = np.sqrt(((q[:, 0] - c[0]) * (q[:, 0] - c[0]) + (q[:, 1] - c[1]) * (q[:, 1] - c[1]) +
+                               (q[:, 2] - c[2]) * (q[:, 2] - c[2])).max())
+        else:
+            nrad[i] = max(np.sqrt((R["cent"][k, 0] - c[0]) ** 2 + (R["cent"][k, 1] - c[1]) ** 2 +
+                                  (R["cent"][k, 2] - c[2]) ** 2) + nrad[k] for k in kids[i])
+    reached = {0: np.ones(nb, bool)}
+    far_n = near_n = 0
+    for i in nodes:
+        r = reached.pop(i)
+        c = R["cent"][i]
+        s = np.sqrt((cb[:, 0] - c[0]) * (cb[:, 0] - c[0]) + (cb[:, 1] - c[1]) * (cb[:, 1] - c[1]) +
+                    (cb[:, 2] - c[2]) * (cb[:, 2] - c[2]))
+        acc = r & (nrad[i] + rb < theta * s)
+        far_n += int(nv[acc].sum())
+        rest = r & ~acc
+        if not kids[i]:
+            near_n += int(nv[rest].sum()) * int(R["end"][i] - R["begin"][i])
+        for k in kids[i]:
+            reached[k] = rest
+    assert (far_n, near_n) == (far, near)
+    assert far > 0 and near > 0
+
+
+def test_target_batches_clustered_random_order(mesh):
+    """Targets clustered in a 0.005-rad cap, in random order: the batch level
+    stops at the 12 sorted key levels, every batch stays inside one level-L
+    cell, and the batch count stays within n/64 + 8 max(1, n/256) (the launch
+    bound of the GPU batch kernels)."""
+    m = mesh(L.LPM_SEED_CUBED, 4)
+    R = O.tree_build((m.cx, m.cy, m.cz), m.max_radius, 32)
+    rng = np.random.default_rng(4)
+    n = 20000
+    th = 0.005 * np.sqrt(rng.uniform(size=n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    P = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    tgt = tuple(P.T.copy())
+    order, bstart = O.target_batches(tgt)
+    want = max(1, n // 256)
+    assert len(bstart) <= n // 64 + 1 + 8 * want
+    key = _morton_keys(P[order], R["box"][0, :3], R["box"][0, 3:])
+    top = key >> np.uint64(27)
+    assert np.all(top[1:] >= top[:-1])        # sorted on the top 12 levels
+    sizes = np.diff(np.append(bstart, n))
+    assert np.all(sizes >= 1) and np.all(sizes <= 64)
+    lcp = np.array([12 if a == b else (35 - (int(a ^ b).bit_length() - 1)) // 3 for a, b in zip(top[:-1], top[1:])])
+    cells = [1 + np.sum(lcp < l) for l in range(13)]
+    lev = next((l for l in range(12) if cells[l] >= want), 12)
+    shift = np.uint64(3 * (12 - lev))
+    for b0, sz in zip(bstart, sizes):
+        assert len(set((top[b0:b0 + sz] >> shift).tolist())) == 1
