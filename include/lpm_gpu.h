@@ -110,6 +110,27 @@ int lpm_direct_sum_dev(lpm_ctx* ctx, int kernels, int prec, double radius, doubl
                        const double* sx, const double* sy, const double* sz, const double* gamma, int64_t ns,
                        const int64_t* tgt_self_src,
                        double* ux, double* uy, double* uz, double* psi);
+/* lpm_direct_sum_dev without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as lpm_direct_sum_dev; its workspace (the
+ * prepared sources and the per-split partials) lives in one static device
+ * array of 32 MiB per GPU (made when the library is loaded on that GPU, held
+ * until the process ends), so it calls no hipMalloc / hipFree.  Fits when
+ * 256 + ns x 48 (LPM_F32C; 32 for LPM_F64, rounded up to 256 sources and 256
+ * bytes) + nt x 32 bytes <= 32 MiB, e.g. 98,304 sources with up to 900,000
+ * targets; larger problems return LPM_ERR_INVALID_ARG.  When the partials of
+ * every split do not fit, fewer splits are used (the sum over splits then
+ * runs in another order: results agree with lpm_direct_sum_dev to rounding).
+ * With LPM_F32C it waits once for the stream to check the sphere
+ * precondition; with LPM_F64 it does not wait.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (several contexts or
+ * threads, several streams) race and must be serialized by the caller. */
+int lpm_direct_sum_dev_noalloc(lpm_ctx* ctx, int kernels, int prec, double radius, double delta,
+                               const double* tx, const double* ty, const double* tz, int64_t nt,
+                               const double* sx, const double* sy, const double* sz, const double* gamma,
+                               int64_t ns, const int64_t* tgt_self_src,
+                               double* ux, double* uy, double* uz, double* psi);
 
 /* ------------------------------------------------------------------ treecode
  * Source octree (TreeSum mesh constructor, src/LpmTreeSum.cpp:221-238 and

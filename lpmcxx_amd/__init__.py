@@ -73,6 +73,8 @@ def gpu_lib() -> C.CDLL:
                    _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]
         lib.lpm_direct_sum.argtypes = ds_args
         lib.lpm_direct_sum_dev.argtypes = ds_args
+        if hasattr(lib, "lpm_direct_sum_dev_noalloc"):
+            lib.lpm_direct_sum_dev_noalloc.argtypes = ds_args
         _declare_extra(lib)
         _gpu = lib
     return _gpu

@@ -39,7 +39,9 @@
 // make every tile near: results stay accurate, at FP64 speed.
 #include "lpm_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace lpm {
 namespace {
@@ -383,46 +385,64 @@ void launch_f64(dim3 g, hipStream_t st, const double* tx, const double* ty, cons
 constexpr int kTplF32 = 4;
 constexpr int kTplF64 = 2;
 
-}  // namespace
+__global__ __launch_bounds__(256) void fill_zero(double* __restrict__ a, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
 
-int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
-                      const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
-                      const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy,
-                      double* uz, double* psi) {
-    const bool vel = (kernels & LPM_K_BIOT_SAVART_SPHERE) != 0;
-    const bool pot = (kernels & LPM_K_GREENS_SPHERE) != 0;
-    if (nt == 0) return LPM_OK;
-    hipStream_t st = c->stream;
-    if (ns == 0) {  // empty source set: zero outputs
-        if (vel) {
-            LPM_HIP(c, hipMemsetAsync(ux, 0, nt * sizeof(double), st));
-            LPM_HIP(c, hipMemsetAsync(uy, 0, nt * sizeof(double), st));
-            LPM_HIP(c, hipMemsetAsync(uz, 0, nt * sizeof(double), st));
-        }
-        if (pot) LPM_HIP(c, hipMemsetAsync(psi, 0, nt * sizeof(double), st));
-        return LPM_OK;
-    }
-    const int ntiles = int((ns + kTile - 1) / kTile);
-    const int64_t nsp = int64_t(ntiles) * kTile;
+// Launch shape and workspace of one direct sum.
+struct DsPlan {
+    int ntiles = 0, tps = 0, nsplit = 0;
+    int64_t nsp = 0, gx = 0;
+    size_t src_bytes = 0;   // prepared sources: FP64 double4 [nsp] (+ FP32C float4 [nsp] and tile centres)
+    size_t part_bytes = 0;  // FP64 partials [nsplit][4][nt]
+};
+
+// max_part_bytes > 0 lowers the split count until the partials fit (at
+// least one split); 0 leaves it at the count that fills the chip.
+DsPlan ds_plan(int prec, int64_t nt, int64_t ns, size_t max_part_bytes) {
+    DsPlan P;
+    P.ntiles = int((ns + kTile - 1) / kTile);
+    P.nsp = int64_t(P.ntiles) * kTile;
+    if (P.ntiles == 0 || nt == 0) return P;  // no sources or targets: no launch, no workspace
     const int tpl = (prec == LPM_F32C) ? kTplF32 : kTplF64;
     const int64_t per_block = int64_t(kWavesPerBlock) * 64 * tpl;
-    const int64_t gx = (nt + per_block - 1) / per_block;
+    P.gx = (nt + per_block - 1) / per_block;
     // enough (group, split) tasks for 8 blocks per CU on 256 CUs
-    int nsplit = int(std::min<int64_t>(ntiles, std::max<int64_t>(1, (2048 + gx - 1) / gx)));
-    const int tps = (ntiles + nsplit - 1) / nsplit;
-    nsplit = (ntiles + tps - 1) / tps;
+    int nsplit = int(std::min<int64_t>(P.ntiles, std::max<int64_t>(1, (2048 + P.gx - 1) / P.gx)));
+    if (max_part_bytes > 0)
+        nsplit = int(std::max<int64_t>(1, std::min<int64_t>(nsplit, int64_t(max_part_bytes / (32 * size_t(nt))))));
+    P.tps = (P.ntiles + nsplit - 1) / nsplit;
+    P.nsplit = (P.ntiles + P.tps - 1) / P.tps;
+    P.src_bytes = prec == LPM_F32C
+                      ? size_t(P.nsp) * (sizeof(float4) + sizeof(double4)) + size_t(P.ntiles) * 4 * sizeof(double)
+                      : size_t(P.nsp) * sizeof(double4);
+    P.part_bytes = size_t(P.nsplit) * 4 * size_t(nt) * sizeof(double);
+    return P;
+}
+
+// The kernels of one direct sum on prepared workspace: src (src_bytes),
+// part (part_bytes) and gtot (one double).  nt > 0.
+int ds_launch(lpm_ctx* c, const DsPlan& P, int kernels, int prec, double radius, double delta, const double* tx,
+              const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy, const double* sz,
+              const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy, double* uz, double* psi,
+              void* srcbuf, double* part, double* gtot) {
+    const bool vel = (kernels & LPM_K_BIOT_SAVART_SPHERE) != 0;
+    const bool pot = (kernels & LPM_K_GREENS_SPHERE) != 0;
+    hipStream_t st = c->stream;
+    if (ns == 0) {  // empty source set: zero outputs
+        for (double* o : {vel ? ux : nullptr, vel ? uy : nullptr, vel ? uz : nullptr, pot ? psi : nullptr})
+            if (o) hipLaunchKernelGGL(fill_zero, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, o, nt);
+        LPM_HIP(c, hipGetLastError());
+        return LPM_OK;
+    }
     const int64_t ntp = nt;
-
-    LPM_CHECK(c->ds_part.reserve(size_t(nsplit) * 4 * ntp * sizeof(double)));
-    LPM_CHECK(c->ds_misc.reserve(64));
-    double* part = c->ds_part.as<double>();
-    double* gtot = c->ds_misc.as<double>();
-
     const double pi = 3.1415926535897932384626433832795027975;
+    const dim3 g{unsigned(P.gx), unsigned(P.nsplit), 1u};
+    const int ntiles = P.ntiles, tps = P.tps;
+    const int64_t nsp = P.nsp;
     if (prec == LPM_F32C) {
-        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * (sizeof(float4) + sizeof(double4)) +
-                                    size_t(ntiles) * 4 * sizeof(double)));
-        double4* src64 = c->ds_src.as<double4>();
+        double4* src64 = static_cast<double4*>(srcbuf);
         float4* src = reinterpret_cast<float4*>(src64 + nsp);
         double* scen = reinterpret_cast<double*>(src + nsp);
         hipLaunchKernelGGL(ds_prep_src_f32c, dim3(ntiles), dim3(256), 0, st, sx, sy, sz, gamma, ns, src, scen);
@@ -431,23 +451,20 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         const double d2x2 = 2.0 * delta * delta;
         if (pot) hipLaunchKernelGGL(ds_gamma_total, dim3(1), dim3(1024), 0, st, gamma, ns, gtot);
         const float eps2v = float(2.0 * delta * delta);
-        const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
         mark_begin(c);
         if (vel && pot) launch_f32c<kTplF32, true, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         else if (vel) launch_f32c<kTplF32, true, false>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         mark_end(c);
         LPM_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp, nt,
-                           tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),
+        hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit, ntp,
+                           nt, tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),
                            -std::log(2.0) / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
     } else {
-        LPM_CHECK(c->ds_src.reserve(size_t(nsp) * sizeof(double4)));
-        double4* src = c->ds_src.as<double4>();
+        double4* src = static_cast<double4*>(srcbuf);
         hipLaunchKernelGGL(ds_prep_src_f64, dim3(unsigned((nsp + 255) / 256)), dim3(256), 0, st, sx, sy, sz, gamma,
                            ns, nsp, src);
         const int64_t per = int64_t(tps) * kTile;
-        const dim3 g{unsigned(gx), unsigned(nsplit), 1u};
         const double R2 = radius * radius, d2 = delta * delta;
         mark_begin(c);
         if (vel && pot) launch_f64<kTplF64, true, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
@@ -455,12 +472,80 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         else launch_f64<kTplF64, false, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
         mark_end(c);
         LPM_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(ds_reduce<false>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, nsplit, ntp,
-                           nt, tx, ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius),
+        hipLaunchKernelGGL(ds_reduce<false>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit,
+                           ntp, nt, tx, ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius),
                            -1.0 / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
     }
     LPM_HIP(c, hipGetLastError());
     return LPM_OK;
+}
+
+}  // namespace
+
+int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                      const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                      const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux, double* uy,
+                      double* uz, double* psi) {
+    if (nt == 0) return LPM_OK;
+    const DsPlan P = ds_plan(prec, nt, ns, 0);
+    if (ns > 0) {
+        LPM_CHECK(c->ds_part.reserve(P.part_bytes));
+        LPM_CHECK(c->ds_misc.reserve(64));
+        LPM_CHECK(c->ds_src.reserve(P.src_bytes));
+    }
+    return ds_launch(c, P, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz, psi,
+                     c->ds_src.ptr, c->ds_part.as<double>(), c->ds_misc.as<double>());
+}
+
+// ---------------------------------------------------------------- static scratch
+// Workspace of lpm_direct_sum_dev_noalloc: one array per GPU, made when the
+// library's code object is loaded on it and held until the process ends.
+// Layout: [0, 256) the total strength and the two words of the sphere check,
+// then the prepared sources, then the partials (both 256-byte aligned).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each device
+}  // namespace
+
+int scratch_init(lpm_ctx* c) {
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, LPM_ERR_INVALID_ARG, "device %d beyond %d", c->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void* p = nullptr;
+        LPM_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the device is current (lpm_create)
+        g_scratch_ptr[c->device] = p;
+    }
+    c->scratch = g_scratch_ptr[c->device];
+    return LPM_OK;
+}
+
+int direct_sum_device_noalloc(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                              const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                              const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux,
+                              double* uy, double* uz, double* psi) {
+    if (!c->scratch) return fail(c, LPM_ERR_STATE, "no static scratch address for device %d", c->device);
+    if (nt == 0) return LPM_OK;
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    double* gtot = reinterpret_cast<double*>(base);
+    unsigned long long* r2w = reinterpret_cast<unsigned long long*>(base + 64);
+    const size_t head = 256;
+    const DsPlan P0 = ds_plan(prec, nt, ns, 0);
+    const size_t src_bytes = (P0.src_bytes + 255) & ~size_t(255);
+    const size_t need_min = head + src_bytes + 32 * size_t(nt);  // one split
+    if (ns > 0 && need_min > kScratchBytes)
+        return fail(c, LPM_ERR_INVALID_ARG,
+                    "lpm_direct_sum_dev_noalloc: %lld targets x %lld sources need %zu bytes of workspace; the static "
+                    "scratch holds %zu (use lpm_direct_sum_dev)", (long long)nt, (long long)ns, need_min,
+                    size_t(kScratchBytes));
+    if (prec == LPM_F32C) {
+        LPM_CHECK(check_on_sphere_words(c, tx, ty, tz, nt, radius, r2w, "lpm_direct_sum_dev_noalloc (LPM_F32C) targets"));
+        LPM_CHECK(check_on_sphere_words(c, sx, sy, sz, ns, radius, r2w, "lpm_direct_sum_dev_noalloc (LPM_F32C) sources"));
+    }
+    const DsPlan P = ds_plan(prec, nt, ns, kScratchBytes - head - src_bytes);
+    return ds_launch(c, P, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz, psi,
+                     base + head, reinterpret_cast<double*>(base + head + src_bytes), gtot);
 }
 
 }  // namespace lpm

@@ -40,6 +40,8 @@ int hip_fail(lpm_ctx* c, hipError_t e, const char* what) {
 }
 
 namespace {
+__global__ void clear_words(unsigned long long* w) { w[threadIdx.x] = 0ull; }
+
 __global__ __launch_bounds__(256) void r2_range_kernel(const double* __restrict__ x, const double* __restrict__ y,
                                                        const double* __restrict__ z, int64_t n,
                                                        unsigned long long* __restrict__ out) {
@@ -50,11 +52,9 @@ __global__ __launch_bounds__(256) void r2_range_kernel(const double* __restrict_
 }
 }  // namespace
 
-int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double* r2min,
-             double* r2max) {
-    LPM_CHECK(c->r2w.reserve(2 * sizeof(unsigned long long)));
-    unsigned long long* w = c->r2w.as<unsigned long long>();
-    LPM_HIP(c, hipMemsetAsync(w, 0, 2 * sizeof(unsigned long long), c->stream));
+int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, unsigned long long* w,
+             double* r2min, double* r2max) {
+    hipLaunchKernelGGL(clear_words, dim3(1), dim3(2), 0, c->stream, w);
     if (n > 0)
         hipLaunchKernelGGL(r2_range_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream, x, y, z, n, w);
     LPM_HIP(c, hipGetLastError());
@@ -69,8 +69,15 @@ int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int6
 int check_on_sphere(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
                     const char* what) {
     if (n <= 0) return LPM_OK;
+    LPM_CHECK(c->r2w.reserve(2 * sizeof(unsigned long long)));
+    return check_on_sphere_words(c, x, y, z, n, R, c->r2w.as<unsigned long long>(), what);
+}
+
+int check_on_sphere_words(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
+                          unsigned long long* w, const char* what) {
+    if (n <= 0) return LPM_OK;
     double lo = 0.0, hi = 0.0;
-    LPM_CHECK(r2_range(c, x, y, z, n, &lo, &hi));
+    LPM_CHECK(r2_range(c, x, y, z, n, w, &lo, &hi));
     if (!r2_on_sphere(lo, hi, R))
         return fail(c, LPM_ERR_INVALID_ARG,
                     "%s: this form needs |x| = R = %.17g (|x|^2 in [%.17g, %.17g]); use LPM_F64 with the reference "
@@ -101,6 +108,10 @@ int lpm_create(lpm_ctx** out, int device) {
         if (c->ev_end) (void)hipEventDestroy(c->ev_end);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
+        return LPM_ERR_HIP;
+    }
+    if (scratch_init(c) != LPM_OK) {
+        lpm_destroy(c);
         return LPM_ERR_HIP;
     }
     *out = c;
@@ -227,6 +238,16 @@ int lpm_direct_sum_dev(lpm_ctx* c, int kernels, int prec, double radius, double 
     LPM_CHECK(check_direct_sphere(c, prec, radius, tx, ty, tz, nt, sx, sy, sz, ns));
     return direct_sum_device(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz,
                              psi);
+}
+
+int lpm_direct_sum_dev_noalloc(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                               const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                               const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux,
+                               double* uy, double* uz, double* psi) {
+    LPM_CHECK(check_direct_args(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, ux, uy, uz, psi));
+    LPM_HIP(c, hipSetDevice(c->device));
+    return direct_sum_device_noalloc(c, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux,
+                                     uy, uz, psi);
 }
 
 int lpm_direct_sum(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx, const double* ty,

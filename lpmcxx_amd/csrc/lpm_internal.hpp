@@ -59,6 +59,8 @@ struct lpm_ctx {
     lpm::DevBuf mesh_work;
     // two words of the sphere check (r2_range)
     lpm::DevBuf r2w;
+    // this device's address of the static scratch of lpm_direct_sum_dev_noalloc
+    void* scratch = nullptr;
     int far_cap = 1024, near_cap = 512;
     int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
     int stereo = 0;                // LPM_OPT_STEREO
@@ -112,6 +114,20 @@ int hip_fail(lpm_ctx* c, hipError_t e, const char* what);
         int _s = (x);                \
         if (_s != LPM_OK) return _s; \
     } while (0)
+
+// Static device workspace of lpm_direct_sum_dev_noalloc (direct_sum.hip):
+// one array of kScratchBytes per GPU, its address looked up once per device
+// by scratch_init (called from lpm_create with the device current).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+int scratch_init(lpm_ctx* c);
+// the direct sum on the static scratch: no allocation (LPM_ERR_INVALID_ARG
+// when the workspace of nt x ns does not fit); LPM_F32C waits once for the
+// sphere check
+int direct_sum_device_noalloc(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
+                              const double* ty, const double* tz, int64_t nt, const double* sx, const double* sy,
+                              const double* sz, const double* gamma, int64_t ns, const int64_t* skip, double* ux,
+                              double* uy, double* uz, double* psi);
 
 // direct sum (direct_sum.hip); device pointers, enqueued on ctx->stream
 int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
@@ -198,12 +214,17 @@ inline double r2_from_words(const unsigned long long* w, int which) {
     std::memcpy(&d, &b, sizeof d);
     return d;
 }
-// min and max of x^2 + y^2 + z^2 over n device points (waits for the stream)
-int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double* r2min,
-             double* r2max);
+// min and max of x^2 + y^2 + z^2 over n device points, through the two
+// device words w (cleared by a kernel, read back through the pinned words;
+// waits for the stream)
+int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n,
+             unsigned long long* w, double* r2min, double* r2max);
 // LPM_ERR_INVALID_ARG unless every one of the n device points lies on |x| = R
+// (the check's words in the context's buffer, or in w)
 int check_on_sphere(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
                     const char* what);
+int check_on_sphere_words(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, double R,
+                          unsigned long long* w, const char* what);
 
 // The sphere precondition of the treecode forms that need it (every form but
 // LPM_F64 with LPM_SERIES_REFERENCE): the tree's sources (range recorded by
