@@ -132,6 +132,31 @@ int lpm_direct_sum_dev_noalloc(lpm_ctx* ctx, int kernels, int prec, double radiu
                                int64_t ns, const int64_t* tgt_self_src,
                                double* ux, double* uy, double* uz, double* psi);
 
+/* ------------------------------------------------------------------ Taylor series
+ * TaylorSeries3d / SphereGreensSeries (src/LpmTaylorSeries3d.h:20-63).  The
+ * series keeps the multi-indices |k| <= max_p (0 <= max_p <= 20), in the lex
+ * order of MultiIndex (src/LpmMultiIndex.h:34-43, the order of the
+ * reference's std::map), nterms = (max_p+1)(max_p+2)(max_p+3)/6.  Host
+ * pointers; both wait for the device.
+ *
+ * lpm_series_coeffs: SphereGreensSeries::computeCoeffs
+ * (src/LpmTaylorSeries3d.cpp:63-117) for n (target t_i, expansion centre c_i)
+ * pairs: coeffs[i][k] of 1/(1 - t.(c + h)) in h, with the reference's loop
+ * order and FP64 operations (its bit pattern).
+ * Replaces: SphereGreensSeries::computeCoeffs(tgtVec, srcVec, param). */
+int lpm_series_coeffs(lpm_ctx* ctx, int max_p, int64_t n, const double* tx, const double* ty, const double* tz,
+                      const double* cx, const double* cy, const double* cz, double* coeffs);
+/* lpm_series_moments: TaylorSeries3d::computeMoments
+ * (src/LpmTaylorSeries3d.cpp:119-154): moments[k] = sum_i (y_i - c)^k s_i over
+ * the n points in index order, the power pow(h_x, k1) pow(h_y, k2) pow(h_z, k3)
+ * (MultiIndex::vectorPower, src/LpmMultiIndex.h:25-27).  w == NULL: strength
+ * s_i; weight_order 0: power x (s_i w_i) (values x weights); weight_order 1:
+ * (power x s_i) x w_i (face values x face areas).
+ * Replaces: TaylorSeries3d::computeMoments (its three overloads). */
+int lpm_series_moments(lpm_ctx* ctx, int max_p, int64_t n, const double* x, const double* y, const double* z,
+                       const double* s, const double* w, int weight_order, double cx, double cy, double cz,
+                       double* moments);
+
 /* ------------------------------------------------------------------ treecode
  * Source octree (TreeSum mesh constructor, src/LpmTreeSum.cpp:221-238 and
  * generateTreeFromMesh :266-290): root [-max_radius-1e-5, max_radius+1e-5)^3,

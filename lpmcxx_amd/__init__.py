@@ -152,12 +152,16 @@ def _declare_extra(lib: C.CDLL) -> None:
         "lpm_bve_step_dev": [_vp, _vp, C.c_double, C.c_double, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp],
         "lpm_bve_steps": [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp,
                           _vp],
+        "lpm_series_coeffs": [_vp, C.c_int, C.c_int64] + [_vp] * 7,
+        "lpm_series_moments": [_vp, C.c_int, C.c_int64] + [_vp] * 5 + [C.c_int, C.c_double, C.c_double,
+                                                                      C.c_double, _vp],
+        "lpm_direct_sum_dev_noalloc": None,
         "lpm_mesh_counts": [C.c_int, C.c_int, _i64p, _i64p, _i64p],
         "lpm_mesh_generate": [_vp, C.c_int, C.c_int, C.c_double] + [_vp] * 8,
         "lpm_mesh_generate_dev": [_vp, C.c_int, C.c_int, C.c_double] + [_vp] * 8,
     }
     for name, args in sigs.items():
-        if hasattr(lib, name):
+        if args is not None and hasattr(lib, name):
             getattr(lib, name).argtypes = args
     if hasattr(lib, "lpm_solver_config_default"):
         lib.lpm_solver_config_default.restype = None

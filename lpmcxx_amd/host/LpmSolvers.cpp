@@ -121,8 +121,13 @@ GpuContext& GpuContext::threadDefault() {
 
 // ---------------------------------------------------------------- Tree
 Tree::Tree(const std::shared_ptr<Coords> crds, const scalar_type, const int) : _crds(crds) {
-    _gpu = &GpuContext::threadDefault();
+    ownContext();
     _nnodes = 1;
+}
+
+void Tree::ownContext() {
+    if (!_ctx) _ctx = std::make_shared<GpuContext>(0);
+    _gpu = _ctx.get();
 }
 
 void Tree::buildTree(const TREE_DEPTH_CONTROL type, const index_type param, const bool do_shrink) {
@@ -144,12 +149,20 @@ void Tree::buildTree(const TREE_DEPTH_CONTROL type, const index_type param, cons
         inside.push_back(i);
         x.push_back(c->x[i]); y.push_back(c->y[i]); z.push_back(c->z[i]);
     }
+    int control = type == MAX_COORDS_PER_NODE ? LPM_TREE_MAX_PER_NODE : LPM_TREE_MAX_DEPTH_CONTROL;
+    int cparam = int(param);
+    if (type == MAX_COORDS_PER_NODE && c->n() > param && index_type(inside.size()) <= param) {
+        // the root's list counts the points outside it too, so the root splits
+        // once while its children (all of them inside) stay leaves
+        control = LPM_TREE_MAX_DEPTH_CONTROL;
+        cparam = 1;
+    }
     int64_t nn = 0;
     int depth = 0;
-    _gpu->check(lpm_tree_build_box(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), lo, hi,
-                                   type == MAX_COORDS_PER_NODE ? LPM_TREE_MAX_PER_NODE : LPM_TREE_MAX_DEPTH_CONTROL,
-                                   int(param), &nn, &depth),
+    _gpu->check(lpm_tree_build_box(_gpu->get(), x.data(), y.data(), z.data(), int64_t(x.size()), lo, hi, control,
+                                   cparam, &nn, &depth),
                 "Tree::buildTree");
+    _gpuSources = inside;
     _nnodes = nn;
     _depth = depth;
     importFromGpu(index_type(x.size()));
@@ -173,7 +186,9 @@ void Tree::importFromGpu(index_type ns) {
                                 nk.data(), range.data(), perm.data()),
                 "Tree::importFromGpu");
     // depth-first ids, children in octant order
-    std::vector<index_type> order, dfs_id(nn);
+    std::vector<index_type>& order = _gpuNode;
+    order.clear();
+    std::vector<index_type> dfs_id(nn);
     std::vector<index_type> stack = {0};
     while (!stack.empty()) {
         const index_type i = stack.back();
@@ -397,8 +412,48 @@ TreeSum::TreeSum(const int max_series_order, const int max_particles_per_box, st
     buildFromMesh(*pmesh, max_particles_per_box);
 }
 
+TreeSum::TreeSum(const std::shared_ptr<Coords> crds, const int max_series_order, const int max_particles_per_box,
+                 const scalar_type smoother, const bool do_shrink)
+    : Tree(crds), _maxSeriesOrder(max_series_order), _maxParticlesPerNode(max_particles_per_box), _smooth(smoother) {
+    if (max_series_order < 1 || max_series_order > 12)
+        throw std::invalid_argument("TreeSum: max_series_order must be in [1, 12]");
+    buildTree(MAX_COORDS_PER_NODE, max_particles_per_box, do_shrink);
+}
+
+void TreeSum::computeMoments(const std::shared_ptr<Field> strength) {
+    auto c = _crds.lock();
+    if (!c) throw std::logic_error("TreeSum::computeMoments: coordinates expired (point-cloud tree only)");
+    if (!strength || strength->n() < c->n()) throw std::invalid_argument("TreeSum::computeMoments: strength field");
+    std::vector<scalar_type> g;
+    g.reserve(_gpuSources.size());
+    for (index_type i : _gpuSources) g.push_back(strength->getScalar(i));
+    _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), _maxSeriesOrder, _momentSign), "TreeSum::computeMoments");
+    _haveMoments = true;
+    _momentOrder = _maxSeriesOrder;
+    _momentsImported = false;
+}
+
+std::map<MultiIndex, std::vector<scalar_type>> TreeSum::nodeMoments(index_type i) const {
+    if (!_haveMoments) throw std::logic_error("TreeSum::nodeMoments: no moments (computeMoments first)");
+    if (i < 0 || i >= _nnodes) throw std::out_of_range("TreeSum::nodeMoments: node id");
+    const int p = _momentOrder, nt = p * (p + 1) * (p + 2) / 6;
+    auto* self = const_cast<TreeSum*>(this);
+    if (!_momentsImported) {
+        self->_moments.assign(size_t(_nnodes) * nt * 3, 0.0);
+        _gpu->check(lpm_tree_export_moments(_gpu->get(), self->_moments.data()), "TreeSum::nodeMoments");
+        self->_momentsImported = true;
+    }
+    std::map<MultiIndex, std::vector<scalar_type>> out;
+    const scalar_type* m = _moments.data() + size_t(_gpuNode[i]) * nt * 3;
+    int t = 0;
+    for (int a = 0; a < p; ++a)
+        for (int b = 0; a + b < p; ++b)
+            for (int c = 0; a + b + c < p; ++c, ++t) out[MultiIndex(a, b, c)] = {m[3 * t], m[3 * t + 1], m[3 * t + 2]};
+    return out;
+}
+
 void TreeSum::buildFromMesh(const MeshedParticles& pm, int leafMax) {
-    _gpu = &GpuContext::threadDefault();
+    ownContext();
     std::vector<scalar_type> x, y, z;
     for (index_type i = 0; i < pm.nFaces(); ++i) {
         if (pm.faceIsDivided(i)) continue;
@@ -484,6 +539,7 @@ void TreeSum::ensureMoments(int order) {
     _gpu->check(lpm_tree_moments(_gpu->get(), g.data(), order, _momentSign), "TreeSum::computeMeshMoments");
     _haveMoments = true;
     _momentOrder = order;
+    _momentsImported = false;
 }
 
 float TreeSum::trainOrderModel(const OrderModelConfig& cfg) {
@@ -628,52 +684,64 @@ std::string TaylorSeries3d::infoString() const {
     return ss.str();
 }
 
-// The loop order of src/LpmTaylorSeries3d.cpp:63-117.
+// Coefficients on the GPU (lpm_series_coeffs): the loop order and FP64
+// operations of src/LpmTaylorSeries3d.cpp:63-117, so the map holds the
+// reference's values.  The map iterates in lex order, the ABI's order.
 void SphereGreensSeries::computeCoeffs(const XyzVector& t, const XyzVector& s, const scalar_type) {
-    const scalar_type D = 1.0 - t.dotProduct(s);
-    auto A = [&](int a, int b, int c) -> scalar_type& { return coeffs.at(MultiIndex(a, b, c)); };
-    A(0, 0, 0) = 1.0 / D;
-    for (int k1 = 1; k1 <= maxP; ++k1) A(k1, 0, 0) = t.x * A(k1 - 1, 0, 0) / D;
-    for (int k2 = 1; k2 <= maxP; ++k2) A(0, k2, 0) = t.y * A(0, k2 - 1, 0) / D;
-    for (int k1 = 1; k1 <= maxP; ++k1)
-        for (int k2 = 1; k2 <= maxP - k1; ++k2)
-            A(k1, k2, 0) = t.y * A(k1, k2 - 1, 0) * (scalar_type(k1 + k2) / k2) / D;
-    for (int k3 = 1; k3 <= maxP; ++k3) {
-        A(0, 0, k3) = t.z * A(0, 0, k3 - 1) / D;
-        for (int k1 = 1; k1 <= maxP - k3; ++k1) A(k1, 0, k3) = t.x * A(k1 - 1, 0, k3) * (scalar_type(k1 + k3) / k1) / D;
-        for (int k2 = 1; k2 <= maxP - k3; ++k2) A(0, k2, k3) = t.y * A(0, k2 - 1, k3) * (scalar_type(k2 + k3) / k2) / D;
-        for (int k1 = 1; k1 <= maxP - k3; ++k1)
-            for (int k2 = 1; k2 <= maxP - k3 - k1; ++k2)
-                A(k1, k2, k3) = t.y * A(k1, k2 - 1, k3) * (scalar_type(k1 + k2 + k3) / k2) / D;
-    }
+    std::vector<scalar_type> out(coeffs.size());
+    GpuContext& gpu = GpuContext::threadDefault();
+    gpu.check(lpm_series_coeffs(gpu.get(), maxP, 1, &t.x, &t.y, &t.z, &s.x, &s.y, &s.z, out.data()),
+              "SphereGreensSeries::computeCoeffs");
+    size_t i = 0;
+    for (auto& e : coeffs) e.second = out[i++];
 }
 
-// m_k = sum_i (y_i - c)^k * strength_i (src/LpmTaylorSeries3d.cpp:119-154)
+// m_k = sum_i (y_i - c)^k * strength_i on the GPU (lpm_series_moments,
+// src/LpmTaylorSeries3d.cpp:119-154), points in the order of inds.
+void TaylorSeries3d::momentsOnGpu(const std::vector<scalar_type>& x, const std::vector<scalar_type>& y,
+                                  const std::vector<scalar_type>& z, const std::vector<scalar_type>& s,
+                                  const std::vector<scalar_type>* w, int weightOrder, const XyzVector& cntd) {
+    std::vector<scalar_type> out(moments.size());
+    GpuContext& gpu = GpuContext::threadDefault();
+    gpu.check(lpm_series_moments(gpu.get(), maxP, int64_t(x.size()), x.data(), y.data(), z.data(), s.data(),
+                                 w ? w->data() : nullptr, weightOrder, cntd.x, cntd.y, cntd.z, out.data()),
+              "TaylorSeries3d::computeMoments");
+    size_t i = 0;
+    for (auto& e : moments) e.second = out[i++];
+}
+
 void TaylorSeries3d::computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& inds,
                                     const XyzVector& cntd, const std::shared_ptr<Field> str) {
-    for (auto& e : moments) {
-        e.second = 0.0;
-        for (index_type i : inds) e.second += e.first.vectorPower(crds->getVec(i) - cntd) * str->getScalar(i);
+    std::vector<scalar_type> x, y, z, s;
+    for (index_type i : inds) {
+        x.push_back(crds->x[i]); y.push_back(crds->y[i]); z.push_back(crds->z[i]);
+        s.push_back(str->getScalar(i));
     }
+    momentsOnGpu(x, y, z, s, nullptr, 0, cntd);
 }
 
 void TaylorSeries3d::computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& inds,
                                     const XyzVector& cntd, const std::shared_ptr<Field> vals,
                                     const std::shared_ptr<Field> wts) {
-    for (auto& e : moments) {
-        e.second = 0.0;
-        for (index_type i : inds)
-            e.second += e.first.vectorPower(crds->getVec(i) - cntd) * (vals->getScalar(i) * wts->getScalar(i));
+    std::vector<scalar_type> x, y, z, s, w;
+    for (index_type i : inds) {
+        x.push_back(crds->x[i]); y.push_back(crds->y[i]); z.push_back(crds->z[i]);
+        s.push_back(vals->getScalar(i));
+        w.push_back(wts->getScalar(i));
     }
+    momentsOnGpu(x, y, z, s, &w, 0, cntd);
 }
 
 void TaylorSeries3d::computeMoments(const std::shared_ptr<PolyMesh2d> faces, const std::vector<index_type>& inds,
                                     const XyzVector& cntd, const std::shared_ptr<Field> vals) {
-    for (auto& e : moments) {
-        e.second = 0.0;
-        for (index_type i : inds)
-            e.second += e.first.vectorPower(faces->faceCentroid(i) - cntd) * vals->getScalar(i) * faces->faceArea(i);
+    std::vector<scalar_type> x, y, z, s, a;
+    for (index_type i : inds) {
+        const XyzVector c = faces->faceCentroid(i);
+        x.push_back(c.x); y.push_back(c.y); z.push_back(c.z);
+        s.push_back(vals->getScalar(i));
+        a.push_back(faces->faceArea(i));
     }
+    momentsOnGpu(x, y, z, s, &a, 1, cntd);
 }
 
 }  // namespace Lpm

@@ -251,11 +251,17 @@ class Tree {
     Tree() {}
     // Reads the GPU node table of the context into _nodes (depth-first ids).
     void importFromGpu(index_type nSources);
+    // A GPU context of this tree's own: the device tree, its moments and the
+    // order model stay with the object however many trees a thread holds.
+    void ownContext();
 
     int _depth = 0;
     index_type _nnodes = 0;
     std::weak_ptr<Coords> _crds;
     std::vector<Node> _nodes;
+    std::vector<index_type> _gpuNode;     // depth-first id -> node id of the device tree
+    std::vector<index_type> _gpuSources;  // device source j -> coordinate index (points inside the root box)
+    std::shared_ptr<GpuContext> _ctx;
     GpuContext* _gpu = nullptr;
 };
 
@@ -319,6 +325,13 @@ struct OrderModelConfig {
 
 class TreeSum : public Tree {
   public:
+    // Point-cloud tree (src/LpmTreeSum.h:49-50, src/LpmTreeSum.cpp:212-219,
+    // generateTree :240-263): the padded root box of Tree(crds), a node splits
+    // iff it holds more than max_particles_per_box points; smoother is the
+    // far field's delta.  do_shrink = true is rejected with
+    // std::invalid_argument, as by Tree::buildTree.
+    TreeSum(const std::shared_ptr<Coords> crds, const int max_series_order, const int max_particles_per_box,
+            const scalar_type smoother, const bool do_shrink = false);
     TreeSum(const int max_series_order, const int max_particles_per_box, std::shared_ptr<MeshedParticles> pmesh,
             const std::shared_ptr<VectorKernel> kernel, const std::string tgtFieldName,
             const std::string srcFieldName);
@@ -335,6 +348,15 @@ class TreeSum : public Tree {
     void initScalarKernel(ScalarKernel* kernel) { sKern = kernel; vKern = nullptr; }
 
     void computeMeshMoments();
+    // Point-cloud moments (src/LpmTreeSum.cpp:173-180, SumNode::calc_moments
+    // :139-147): for every node, the sum over the points in its box of
+    // s_j y_j h_j^k, |k| < max_series_order, h = c - y (the reference's sign,
+    // the default) or y - c after setMomentSign(+1); strength holds one value
+    // per coordinate.  Points outside the root box reach no node.
+    void computeMoments(const std::shared_ptr<Field> strength);
+    // SumNode::moments of node i (depth-first id) after computeMoments or
+    // computeMeshMoments: multi-index -> (x, y, z) components, lex order.
+    std::map<MultiIndex, std::vector<scalar_type>> nodeMoments(index_type i) const;
     // Fills every rank's block in this process (as DirectSum::meshSolve);
     // macCounter is then the sum over all blocks, the reference's MPI_Reduce
     // total.
@@ -381,11 +403,13 @@ class TreeSum : public Tree {
     VectorKernel* vKern = nullptr;
     ScalarKernel* sKern = nullptr;
     std::vector<index_type> _leafFaces;  // source j -> face index
+    std::vector<scalar_type> _moments;   // device-tree moments [node][term][3] (lex), made by nodeMoments
+    bool _momentsImported = false;
     void buildFromMesh(const MeshedParticles& pm, int leafMax);
     void scalarSolve(const MPIReplicatedData& mv, const MPIReplicatedData& mf, scalar_type mp_tol, int& macCounter);
 };
 
-// ---------------------------------------------------------------- Taylor series (host, FP64)
+// ---------------------------------------------------------------- Taylor series (coefficients and moments on the GPU)
 class TaylorSeries3d {
   public:
     virtual ~TaylorSeries3d() {}
@@ -394,6 +418,7 @@ class TaylorSeries3d {
     scalar_type sum() const;
     std::string infoString() const;
     virtual void computeCoeffs(const XyzVector& tgtVec, const XyzVector& srcVec, const scalar_type param = 0.0) = 0;
+    virtual std::unique_ptr<TaylorSeries3d> createEmpty() const = 0;
     void computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& crdInds,
                         const XyzVector& cntd, const std::shared_ptr<Field> srcStr);
     void computeMoments(const std::shared_ptr<Coords> crds, const std::vector<index_type>& crdInds,
@@ -405,6 +430,10 @@ class TaylorSeries3d {
 
   protected:
     explicit TaylorSeries3d(const int maxSeriesOrder);
+    // lpm_series_moments on the gathered points; weightOrder as in lpm_gpu.h
+    void momentsOnGpu(const std::vector<scalar_type>& x, const std::vector<scalar_type>& y,
+                      const std::vector<scalar_type>& z, const std::vector<scalar_type>& s,
+                      const std::vector<scalar_type>* w, int weightOrder, const XyzVector& cntd);
     std::map<MultiIndex, scalar_type> coeffs, moments;
     int maxP;
 };
@@ -413,7 +442,11 @@ class TaylorSeries3d {
 class SphereGreensSeries : public TaylorSeries3d {
   public:
     explicit SphereGreensSeries(const int maxSeriesOrder) : TaylorSeries3d(maxSeriesOrder) {}
+    // on the GPU (lpm_series_coeffs), the reference's values
     void computeCoeffs(const XyzVector& tgtVec, const XyzVector& srcVec, const scalar_type param = 0.0) override;
+    std::unique_ptr<TaylorSeries3d> createEmpty() const override {
+        return std::unique_ptr<TaylorSeries3d>(new SphereGreensSeries(maxP));
+    }
 };
 
 }  // namespace Lpm

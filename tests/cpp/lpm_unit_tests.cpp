@@ -1,8 +1,9 @@
 // Unit checks of the host Lpm API that need no GPU: multi-indices, boxes,
-// replicated-data partition, kernels, Taylor series, mesh fields.
-// Mirrors tests/LpmMultiIndexUnitTests.cpp, tests/LpmBox3dUnitTests.cpp,
-// tests/LpmMPIReplicatedDataUnitTests.cpp and tests/LpmTaylorSeries3dUnitTests.cpp
-// of the reference, with assertions.  Exit status = number of failures.
+// replicated-data partition, kernels, mesh fields, VTK / CSV output.
+// Mirrors tests/LpmMultiIndexUnitTests.cpp, tests/LpmBox3dUnitTests.cpp and
+// tests/LpmMPIReplicatedDataUnitTests.cpp of the reference, with assertions
+// (the Taylor series, whose coefficients and moments run on the GPU, are in
+// lpm_series_tree.cpp).  Exit status = number of failures.
 #include "LpmSolvers.hpp"
 #include "LpmVtk.hpp"
 
@@ -61,28 +62,6 @@ int main(int argc, char** argv) {
     CHECK(relerr(bs.sumMultiplier(), -1.0 / (4.0 * PI)) < 1e-15);
     const SphereGreensFn gf(1.0);
     CHECK(relerr(gf.evaluate(np, src), -std::log(1 - std::cos(th)) / (4.0 * PI)) < 1e-14);
-
-    // SphereGreensSeries coefficients = |k|!/k! t^k / D^(|k|+1)
-    SphereGreensSeries ser(6);
-    const XyzVector t(0.3, -0.5, 0.81), s(0.1, 0.2, -0.3);
-    ser.computeCoeffs(t, s);
-    const double D = 1.0 - t.dotProduct(s);
-    for (int a = 0; a <= 6; ++a)
-        for (int b = 0; a + b <= 6; ++b)
-            for (int c = 0; a + b + c <= 6; ++c) {
-                const MultiIndex m(a, b, c);
-                const double exact = double(factorial(a + b + c)) / m.factorial() * std::pow(t.x, a) *
-                                     std::pow(t.y, b) * std::pow(t.z, c) / std::pow(D, a + b + c + 1);
-                CHECK(relerr(ser.getCoeff(m), exact) < 1e-13);
-            }
-    // the series reproduces 1/(1 - t.y) for y = s + h with small h
-    auto crds = std::make_shared<Coords>();
-    crds->insert(s + XyzVector(0.01, -0.02, 0.015));
-    auto str = std::make_shared<Field>(1, 1, "str");
-    str->replace(0, 1.0);
-    ser.computeMoments(crds, {0}, s, str);
-    const double direct = 1.0 / (1.0 - t.dotProduct(crds->getVec(0)));
-    CHECK(relerr(ser.sum(), direct) < 1e-10);
 
     // mesh fields
     CubedSphereSeed seed;

@@ -61,6 +61,12 @@ def test_unit_api(tmp_path):
 
 
 @pytest.mark.gpu
+def test_series_and_point_cloud_tree():
+    r = _run("lpm_series_tree")
+    assert r.returncode == 0 and "0 failure(s)" in r.stdout
+
+
+@pytest.mark.gpu
 def test_biot_savart_rh_wave():
     r = _run("lpm_biot_savart_rh")
     assert r.returncode == 0 and "0 failure(s)" in r.stdout

@@ -147,3 +147,74 @@ def test_gpu_generator_counts_match_host_builder(mesh):
     for seed, level in ((2, 3), (1, 13), (0, -1)):
         with pytest.raises(L.LpmError):
             L.mesh_counts(seed, level)
+
+
+def _seed_fixture(name):
+    import json
+    from pathlib import Path
+    d = json.loads((Path(__file__).resolve().parent / "golden" / "mesh_seeds.json").read_text())[name]
+    xyz = [np.array([float(v) for v in r]) for r in d["xyz"]]
+    # the root faces' vertex lists by the reference's rule (src/LpmFaces.cpp:98-121): the
+    # start vertex of each edge in the face's edge list, orig when the face is the edge's left face
+    verts = []
+    for f, el in enumerate(d["face_edges"]):
+        vl = []
+        for e in el:
+            o, de, left, right = d["edges"][e]
+            vl.append(o if left == f else de)
+        verts.append(vl)
+    assert verts == d["face_verts"]
+    return xyz, verts
+
+
+def _divide_level(xyz, faces):
+    """One uniform refinement in the reference's order (src/LpmPolyMesh2d.cpp:58-65:
+    leaves in index order; src/LpmQuadFaces.cpp:14-103, src/LpmTriFaces.cpp:13-95:
+    each face's edges in order, an undivided edge gets the next vertex (its
+    sphere midpoint, src/LpmEdges.cpp:99-120), then a quad's centroid; the
+    children placed as the reference's newFaceVertInds)."""
+    mid = {}
+    out = []
+    for F in faces:
+        n = len(F)
+        m = []
+        for j in range(n):
+            a, b = F[j], F[(j + 1) % n]
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = 0.5 * (xyz[a] + xyz[b])
+                mid[key] = len(xyz)
+                xyz.append(p / np.linalg.norm(p))
+            m.append(mid[key])
+        if n == 4:
+            c = sum(xyz[v] for v in F) / 4.0
+            ci = len(xyz)
+            xyz.append(c / np.linalg.norm(c))
+            for j in range(4):
+                ch = [0] * 4
+                ch[j], ch[(j + 1) % 4], ch[(j + 2) % 4], ch[(j + 3) % 4] = F[j], m[j], ci, m[(j - 1) % 4]
+                out.append(ch)
+        else:
+            out += [[F[0], m[0], m[2]], [m[0], F[1], m[1]], [m[2], m[1], F[2]], [m[1], m[2], m[0]]]
+    return out
+
+
+@pytest.mark.parametrize("seed,name", [(L.LPM_SEED_CUBED, "cubed"), (L.LPM_SEED_ICOS_TRI, "icos")])
+def test_index_order_levels_1_2(seed, name):
+    """Vertex and leaf-face index order of the mesh builder against the order
+    derived from the seed files (tests/golden/mesh_seeds.json) and the
+    reference's division rules, levels 0-2: every vertex and every leaf
+    centroid at its index."""
+    xyz, faces = _seed_fixture(name)
+    for level in range(3):
+        if level:
+            faces = _divide_level(xyz, faces)
+        m = L.Mesh(seed, level)
+        V = np.stack(xyz, 1)
+        assert m.n_vertices == V.shape[1] and m.n_leaves == len(faces)
+        assert np.abs(np.stack([m.vx, m.vy, m.vz]) - V).max() < 1e-15
+        C = np.stack([sum(V[:, v] for v in F) / len(F) for F in faces], 1)
+        C /= np.linalg.norm(C, axis=0)
+        assert np.abs(np.stack([m.cx, m.cy, m.cz]) - C).max() < 1e-15
+        # the leaves are the last faces, children of one parent consecutive
+        assert np.array_equal(m.face_index, np.arange(m.n_faces - m.n_leaves, m.n_faces))
