@@ -1,6 +1,7 @@
 // C ABI of liblpm_gpu: contexts, status codes, host-pointer wrappers.
 #include "lpm_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -42,13 +43,25 @@ int hip_fail(lpm_ctx* c, hipError_t e, const char* what) {
 namespace {
 __global__ void clear_words(unsigned long long* w) { w[threadIdx.x] = 0ull; }
 
+// grid-stride: each thread folds its points' range first, so only the few
+// hundred blocks of the grid reach the two words
 __global__ __launch_bounds__(256) void r2_range_kernel(const double* __restrict__ x, const double* __restrict__ y,
                                                        const double* __restrict__ z, int64_t n,
                                                        unsigned long long* __restrict__ out) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool v = i < n;
-    const double r2 = v ? x[i] * x[i] + y[i] * y[i] + z[i] * z[i] : 0.0;
-    r2_block_range(r2, v, out);
+    double lo = 0.0, hi = 0.0;
+    bool any = false;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double r2 = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
+        // NaN must win both: compare on the bit patterns the words use
+        const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(r2));
+        if (!any || b < static_cast<unsigned long long>(__double_as_longlong(lo))) lo = r2;
+        if (!any || b > static_cast<unsigned long long>(__double_as_longlong(hi))) hi = r2;
+        any = true;
+    }
+    // the low word keeps ~bits(min), the high word bits(max): fold both ends
+    r2_block_range(lo, any, out);
+    __syncthreads();  // the second call reuses the block's reduction words
+    r2_block_range(hi, any, out);
 }
 }  // namespace
 
@@ -56,7 +69,8 @@ int r2_range(lpm_ctx* c, const double* x, const double* y, const double* z, int6
              double* r2min, double* r2max) {
     hipLaunchKernelGGL(clear_words, dim3(1), dim3(2), 0, c->stream, w);
     if (n > 0)
-        hipLaunchKernelGGL(r2_range_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream, x, y, z, n, w);
+        hipLaunchKernelGGL(r2_range_kernel, dim3(unsigned(std::min<int64_t>((n + 255) / 256, 512))), dim3(256), 0,
+                           c->stream, x, y, z, n, w);
     LPM_HIP(c, hipGetLastError());
     unsigned long long* h = reinterpret_cast<unsigned long long*>(c->pin);
     LPM_HIP(c, hipMemcpyAsync(h, w, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
@@ -126,7 +140,7 @@ void lpm_destroy(lpm_ctx* c) {
     model_free(c->model);
     c->tree = nullptr;
     for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work,
-                      &c->tb_work, &c->tl_work, &c->mesh_work})
+                      &c->tb_work, &c->tl_work, &c->mesh_work, &c->lab_work, &c->r2w})
         b->release();
     if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);

@@ -57,11 +57,13 @@ struct lpm_ctx {
     lpm::DevBuf tb_work, tl_work;
     // level tables of the GPU mesh generator
     lpm::DevBuf mesh_work;
+    // ordering of the label rows (lpm_tree_labels)
+    lpm::DevBuf lab_work;
     // two words of the sphere check (r2_range)
     lpm::DevBuf r2w;
     // this device's address of the static scratch of lpm_direct_sum_dev_noalloc
     void* scratch = nullptr;
-    int far_cap = 1024, near_cap = 512;
+    int far_cap = 1024, near_cap = 512, st_cap = 1024;
     int batch_cell_targets = 256;  // LPM_OPT_BATCH_CELL_TARGETS
     int stereo = 0;                // LPM_OPT_STEREO
     int stereo_extra = 1;          // LPM_OPT_STEREO_EXTRA_ORDER
@@ -81,7 +83,7 @@ struct lpm_ctx {
 };
 
 namespace lpm {
-constexpr int kPinWords = 256;
+constexpr int kPinWords = 512;
 }
 
 namespace lpm {
@@ -184,7 +186,9 @@ inline bool r2_on_sphere(double r2min, double r2max, double R) {
 // Block-wide range of r2 (every thread of the block calls it; `valid` false
 // for threads without a point) folded into out[0] = max of ~bits(min r2) and
 // out[1] = max of bits(max r2): r2 >= 0, so its bits order like its value
-// (NaN orders above +inf and fails the check).  Both words start at 0.
+// (NaN orders above +inf and fails the check).  Both words start at 0.  The
+// caller may spread the blocks over several word pairs (out + 2 (block %
+// slots)) so that thousands of blocks do not queue on one address.
 __device__ inline void r2_block_range(double r2, bool valid, unsigned long long* out) {
     __shared__ unsigned long long s_lo[16], s_hi[16];
     unsigned long long lo = valid ? ~static_cast<unsigned long long>(__double_as_longlong(r2)) : 0ull;
@@ -208,8 +212,13 @@ __device__ inline void r2_block_range(double r2, bool valid, unsigned long long*
         if (hi > out[1]) atomicMax(&out[1], hi);
     }
 }
-inline double r2_from_words(const unsigned long long* w, int which) {
-    const unsigned long long b = which == 0 ? ~w[0] : w[1];
+inline double r2_from_words(const unsigned long long* w, int which, int slots = 1) {
+    unsigned long long lo = 0, hi = 0;
+    for (int k = 0; k < slots; ++k) {
+        lo = w[2 * k] > lo ? w[2 * k] : lo;
+        hi = w[2 * k + 1] > hi ? w[2 * k + 1] : hi;
+    }
+    const unsigned long long b = which == 0 ? ~lo : hi;
     double d;
     std::memcpy(&d, &b, sizeof d);
     return d;

@@ -24,10 +24,12 @@ constexpr int kLvKeys = kLvWords + 2;     // a node below the sorted key levels 
 constexpr int kLvSpread = 64;
 constexpr int kLvLeaves = kLvWords + 3;
 constexpr int kLvChunks = kLvLeaves + kLvSpread;
-// two 64-bit words (8-byte aligned): the range of |x|^2 over the sources
-// (r2_block_range), for the sphere precondition of the series
+// kR2Slots pairs of 64-bit words (8-byte aligned): the range of |x|^2 over
+// the sources (r2_block_range, block b in pair b % kR2Slots), for the sphere
+// precondition of the series
+constexpr int kR2Slots = 64;
 constexpr int kLvR2 = (kLvChunks + kLvSpread + 1) & ~1;
-constexpr int kLvTotal = kLvR2 + 4;
+constexpr int kLvTotal = kLvR2 + 4 * kR2Slots;
 
 struct Tree {
     int64_t ns = 0;       // sources

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void tree_keys(const double* __restrict__ x, c
     const bool valid = i < n;
     const double px = valid ? x[i] : 0.0, py = valid ? y[i] : 0.0, pz = valid ? z[i] : 0.0;
     r2_block_range(px * px + py * py + pz * pz, valid,
-                   reinterpret_cast<unsigned long long*>(lvt + kLvR2));
+                   reinterpret_cast<unsigned long long*>(lvt + kLvR2) + 2 * (blockIdx.x % kR2Slots));
     if (!valid) return;
     if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
     double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
@@ -500,8 +500,8 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     t->nn = nn;
     t->depth = depth;
     t->sorted_levels = sorted_levels;
-    t->r2min = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 0);
-    t->r2max = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 1);
+    t->r2min = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 0, kR2Slots);
+    t->r2max = r2_from_words(reinterpret_cast<const unsigned long long*>(h + kLvR2), 1, kR2Slots);
     t->nleaves = 0;
     t->nch = 0;
     for (int w = 0; w < kLvSpread; ++w) {

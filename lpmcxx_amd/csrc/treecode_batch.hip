@@ -30,6 +30,8 @@
 #include "lpm_stereo.hpp"
 #include "lpm_tree.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <utility>
@@ -179,9 +181,7 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
     if constexpr (F32) {
         const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll 4
-        for (int j = b; j < e; ++j) {
-            const float4 o = soff[j];
+        auto pair = [&](const float4& o) {
             const float dx = q0 - o.x, dy = q1 - o.y, dz = q2 - o.z;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
             // the reference's |den| <= 1e-13 rule: |d|^2 + 2 delta^2 <= 2e-13 contributes 0
@@ -189,7 +189,9 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
             a0 = fmaf(wgt, dx, a0);
             a1 = fmaf(wgt, dy, a1);
             a2 = fmaf(wgt, dz, a2);
-        }
+        };
+#pragma unroll 4
+        for (int j = b; j < e; ++j) pair(soff[j]);
         // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2): -2 d
         // gamma / den joins v, crossed with t once per target
         v0 -= 2.0 * double(a0);
@@ -308,41 +310,69 @@ __device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t
     return s;
 }
 
-// Cauchy series of node entry gi (projection-major index) at the lane's
-// target: S += w sum_k (a_k / rho^k) (rho w)^k, w = 1 / (z - z_c), by Horner
-// in FP32; G += the node's total strength.
-__device__ __forceinline__ void stereo_node(int P, int64_t gi, const StereoView& sv, double zr, double zi, double& Sr,
-                                            double& Si, double& G) {
+// Node data of one stereographic entry: plane centre and radius (FP64) and
+// the first kStPre terms a_k / rho^k.  gi is wave-uniform, so these are
+// scalar loads; the list kernel requests the next entry's data before it
+// sums the current one.
+// Terms of a stereographic entry fetched ahead of its sum (orders P < kStPre;
+// higher orders load the rest when they are reached).
+constexpr int kStPre = 10;
+static_assert(kStPre == 10 && kStPre <= kStereoTerms, "stereo_sum unrolls terms 9 .. 0");
+
+struct StNode {
+    double cr, ci, rho;
+    float2 a[kStPre];
+};
+
+__device__ __forceinline__ StNode stereo_fetch(const StereoView& sv, int64_t gi) {
+    StNode s;
     const double4 g = sv.sgeo[gi];
-    const float dr = float(zr - g.x), di = float(zi - g.y);
+    s.cr = g.x;
+    s.ci = g.y;
+    s.rho = g.z;
+    const float2* a = sv.smom32 + gi * kStereoTerms;
+#pragma unroll
+    for (int k = 0; k < kStPre; ++k) s.a[k] = a[k];
+    return s;
+}
+
+// Cauchy series of node entry gi at the lane's target from its fetched data:
+// S += w sum_k (a_k / rho^k) (rho w)^k, w = 1 / (z - z_c), by Horner in FP32
+// entering the unrolled sum at term P; G += the node's total strength.
+// Orders P >= kStPre read their remaining terms here.
+__device__ __forceinline__ void stereo_sum(const StNode& nd, int P, int64_t gi, const StereoView& sv, double zr,
+                                           double zi, float& Sr, float& Si, float& G) {
+    const float dr = float(zr - nd.cr), di = float(zi - nd.ci);
     const float ir2 = __builtin_amdgcn_rcpf(fmaf(dr, dr, di * di));
     const float wr = dr * ir2, wi = -di * ir2;
-    const float rho = float(g.z);
+    const float rho = float(nd.rho);
     const float xr = rho * wr, xi = rho * wi;
-    // the node's 16 terms (one 128-byte line, scalar loads: gi is wave-uniform)
-    // are loaded together; the switch enters the unrolled Horner sum at term P
-    const float2* a = sv.smom32 + gi * kStereoTerms;
-    float2 am[kStereoTerms];
-#pragma unroll
-    for (int k = 0; k < kStereoTerms; ++k) am[k] = a[k];
     float sr = 0.f, si = 0.f;
-#define LPM_HORNER(K)                                          \
-    case K: {                                                  \
-        const float tr = fmaf(sr, xr, fmaf(-si, xi, am[K].x)); \
-        si = fmaf(sr, xi, fmaf(si, xr, am[K].y));              \
-        sr = tr;                                               \
-    }                                                          \
+    if (P >= kStPre) {
+        const float2* a = sv.smom32 + gi * kStereoTerms;
+        for (int k = P; k >= kStPre; --k) {
+            const float2 ak = a[k];
+            const float tr = fmaf(sr, xr, fmaf(-si, xi, ak.x));
+            si = fmaf(sr, xi, fmaf(si, xr, ak.y));
+            sr = tr;
+        }
+    }
+#define LPM_HORNER(K)                                                \
+    case K: {                                                        \
+        const float tr = fmaf(sr, xr, fmaf(-si, xi, nd.a[K].x));     \
+        si = fmaf(sr, xi, fmaf(si, xr, nd.a[K].y));                  \
+        sr = tr;                                                     \
+    }                                                                \
         [[fallthrough]];
-    switch (P) {
-        LPM_HORNER(15) LPM_HORNER(14) LPM_HORNER(13) LPM_HORNER(12) LPM_HORNER(11) LPM_HORNER(10)
+    switch (P < kStPre ? P : kStPre - 1) {
         LPM_HORNER(9) LPM_HORNER(8) LPM_HORNER(7) LPM_HORNER(6) LPM_HORNER(5) LPM_HORNER(4)
         LPM_HORNER(3) LPM_HORNER(2) LPM_HORNER(1) LPM_HORNER(0)
         default: break;
     }
 #undef LPM_HORNER
-    Sr += double(fmaf(sr, wr, -si * wi));
-    Si += double(fmaf(sr, wi, si * wr));
-    G += double(am[0].x);
+    Sr += fmaf(sr, wr, -si * wi);
+    Si += fmaf(sr, wi, si * wr);
+    G += nd.a[0].x;
 }
 
 // The unscaled velocity sum sum g (t x y) / (1 - t.y) of the plane series
@@ -434,12 +464,21 @@ constexpr int kLowOrder = 7;
 struct Lists {
     int* far;
     int* near;
+    int* st;          // stereographic entries (node | P << kNodeBits)
     int* far_cnt;     // low-order far entries (front)
     int* far_hi_cnt;  // high-order far entries (back)
     int* near_cnt;
-    int far_cap, near_cap;
-    int* overflow;  // [0] a batch overflowed, [1] far and [2] near capacity it needs
+    int* st_cnt;
+    int far_cap, near_cap, st_cap;
+    int* overflow;  // [0] a batch overflowed, [1] far, [2] near and [3] stereographic capacity it needs
+    unsigned* cost;    // estimated vector instructions of each batch's lists (planning pass)
+    const int* order;  // batches by decreasing cost: wave w of the list kernels takes batch order[w]
 };
+
+// Vector instructions per near pair, Cartesian term, Cartesian entry,
+// stereographic entry and complex term: the list kernels' costs, used to run
+// the dearest batches first so that the cheap ones fill the end of the launch.
+constexpr unsigned kCostNear = 14, kCostTerm = 5, kCostFar = 35, kCostSt = 24, kCostStTerm = 4;
 
 // six waves per SIMD (76 VGPRs, 78 with STEREO, no spills): the traversal
 // waits on node loads.  STEREO: accepted pairs may become stereographic
@@ -452,7 +491,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
                                                          unsigned long long* __restrict__ stats) {
     __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackPlan];
+    // the input normalization in LDS (mean, inv_std, lo, hi; 4 inputs per
+    // float4): a lane reads the four of its input group per model call
+    // instead of the kernel holding 64 values in scalar registers
+    __shared__ float4 s_norm[4][kIn / 4];
     mlp_stage_weights(wts, s_model);
+    if (threadIdx.x < 4 * kIn) {
+        const int q = threadIdx.x / kIn, k = threadIdx.x % kIn;
+        const float* src = q == 0 ? norm.mean : q == 1 ? norm.inv_std : q == 2 ? norm.lo : norm.hi;
+        reinterpret_cast<float*>(&s_norm[q][0])[k] = src[k];
+    }
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t bi = wave_index();
@@ -467,7 +515,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     const bool st_on = STEREO && sb.ok;
     int* far = ls.far + bi * ls.far_cap;
     int* near = ls.near + bi * ls.near_cap;
-    int nfar = 0, nhi = 0, nnear = 0;
+    int* stl = ls.st + bi * ls.st_cap;
+    int nfar = 0, nhi = 0, nnear = 0, nstl = 0;
     unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0, nst = 0, nterms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int sp = 0;
@@ -540,13 +589,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
                 r3 = (rn + bt.r) / sd;
                 float fe[kIn];
                 pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
-#pragma unroll
-                for (int k = 0; k < kIn; ++k)
-                    fe[k] = (fminf(fmaxf(fe[k], norm.lo[k]), norm.hi[k]) - norm.mean[k]) * norm.inv_std[k];
                 x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
                 x1 = g == 0 ? fe[1] : g == 1 ? fe[5] : g == 2 ? fe[9] : fe[13];
                 x2 = g == 0 ? fe[2] : g == 1 ? fe[6] : g == 2 ? fe[10] : fe[14];
                 x3 = g == 0 ? fe[3] : g == 1 ? fe[7] : g == 2 ? fe[11] : fe[15];
+                const float4 mu = s_norm[0][g], iv = s_norm[1][g], lo = s_norm[2][g], hi = s_norm[3][g];
+                x0 = (fminf(fmaxf(x0, lo.x), hi.x) - mu.x) * iv.x;
+                x1 = (fminf(fmaxf(x1, lo.y), hi.y) - mu.y) * iv.y;
+                x2 = (fminf(fmaxf(x2, lo.z), hi.z) - mu.z) * iv.z;
+                x3 = (fminf(fmaxf(x3, lo.w), hi.w) - mu.w) * iv.w;
             }
         }
         int p = 0;
@@ -569,22 +620,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             is_st = rz < sv.ratio * r3 * dist && rz < 0.95 * dist;
             pz = min(p + sv.extra, sv.maxp);
         }
-        const bool is_lo = mine && p > 0 && (p <= kLowOrder || is_st), is_hi = mine && p > kLowOrder && !is_st;
+        const bool is_lo = mine && p > 0 && p <= kLowOrder && !is_st, is_hi = mine && p > kLowOrder && !is_st;
         const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
-        const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi);
+        const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi), m_st = __ballot(is_st);
         const uint64_t m_near = __ballot(is_near), m_desc = __ballot(is_desc);
         const int c_lo = __popcll(m_lo), c_hi = __popcll(m_hi), c_near = __popcll(m_near), c_desc = __popcll(m_desc);
+        const int c_st = __popcll(m_st);
         // entries are written only while the batch's lists fit (else the
         // batch is reported as overflowing at the end)
         if (nfar + c_lo + nhi + c_hi <= ls.far_cap) {
-            const int e = is_st ? int(unsigned(chr) | (unsigned(pz) << kNodeBits) | 0x80000000u) : chr | (p << kNodeBits);
+            const int e = chr | (p << kNodeBits);
             if (is_lo) far[nfar + __popcll(m_lo & below)] = e;
             if (is_hi) far[ls.far_cap - 1 - nhi - __popcll(m_hi & below)] = e;
         }
-        if (nnear + c_near <= ls.near_cap && is_near) near[nnear + __popcll(m_near & below)] = chr;
+        if (nstl + c_st <= ls.st_cap && is_st) stl[nstl + __popcll(m_st & below)] = chr | (pz << kNodeBits);
+        if (nnear + c_near <= ls.near_cap && is_near)
+            near[nnear + __popcll(m_near & below)] = chr;
         if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = stack_entry(cfirst, ckids);
         nfar += c_lo;
         nhi += c_hi;
+        nstl += c_st;
         nnear += c_near;
         sp += c_desc;
         // terms of the accepted children and sources of the near leaves
@@ -610,18 +665,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     if (lane == 0) {
         // a batch whose lists do not fit is evaluated as empty and flagged with
         // the capacities it needs; the caller plans again with larger lists
-        const bool ovf = nfar + nhi > ls.far_cap || nnear > ls.near_cap;
+        const bool ovf = nfar + nhi > ls.far_cap || nnear > ls.near_cap || nstl > ls.st_cap;
         ls.far_cnt[bi] = ovf ? 0 : nfar;
         ls.far_hi_cnt[bi] = ovf ? 0 : nhi;
         ls.near_cnt[bi] = ovf ? 0 : nnear;
+        ls.st_cnt[bi] = ovf ? 0 : nstl;
+        const unsigned long long cost = kCostNear * near_pairs + kCostTerm * nterms +
+                                        kCostFar * unsigned(nfar + nhi) + kCostSt * nst + kCostStTerm * nterms_st;
+        ls.cost[bi] = ovf ? 0u : unsigned(cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull);
         if (ovf) {
             atomicOr(ls.overflow, 1);
             atomicMax(ls.overflow + 1, nfar + nhi);
             atomicMax(ls.overflow + 2, nnear);
+            atomicMax(ls.overflow + 3, nstl);
         }
     }
     const unsigned long long nv = unsigned(bt.nvalid);
-    flush_stats(stats, (unsigned long long)(nfar + nhi) * nv, near_pairs * nv, nterms * nv, ncalls,
+    flush_stats(stats, (unsigned long long)(nfar + nhi + nstl) * nv, near_pairs * nv, nterms * nv, ncalls,
                 (unsigned long long)nhi * nv, nterms_hi * nv, nst * nv, nterms_st * nv);
 }
 
@@ -632,7 +692,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
 template <int MAXP, bool LOW, bool F32, bool STEREO = false>
 __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, StereoView sv, double d2,
                                                        float eps2v, double* __restrict__ acc) {
-    const int64_t bi = wave_index();
+    const int64_t wi = wave_index();
+    if (wi >= tg.nb) return;
+    const int64_t bi = __builtin_amdgcn_readfirstlane(ls.order[wi]);
     if (!is_batch(tg, bi)) return;
     const int* far = ls.far + bi * ls.far_cap;
     int j0 = 0, j1 = __builtin_amdgcn_readfirstlane(ls.far_cnt[bi]);
@@ -646,32 +708,51 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     double t0, t1, t2;
     const int64_t pos = load_target(tg, bi, valid, orig, t0, t1, t2);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0;
-    // stereographic entries (low list only): Cauchy sums in the batch's projection
-    StereoBatch sb{};
-    double Sr = 0.0, Si = 0.0, Gs = 0.0;
-    if (STEREO) sb = stereo_batch(batch_of(valid, t0, t1, t2), valid, t0, t1, t2);
     if (LOW) {
         const int* near = ls.near + bi * ls.near_cap;
         const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
         for (int j = 0; j < nnear; ++j) {
             const int n = __builtin_amdgcn_readfirstlane(near[j]);
-            near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1,
-                           s2, v0, v1, v2);
+            near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
+                           v0, v1, v2);
         }
     }
     for (int j = j0; j < j1; ++j) {
         const unsigned e = unsigned(__builtin_amdgcn_readfirstlane(far[j]));
         const int n = int(e & ((1u << kNodeBits) - 1u));
         const int p = int((e >> kNodeBits) & 15u);
-        if (STEREO && (e >> 31)) {
-            stereo_node(p, int64_t(sb.proj) * sv.nn + n, sv, sb.zr, sb.zi, Sr, Si, Gs);
-            continue;
-        }
         far_node<MAXP, F32, true>(p, n, tv.nterm, t0, t1, t2, tv.ncent[n], d2, tv.momg, tv.mom32, s0, s1, s2, v0, v1,
                                   v2);
     }
     cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
-    if (STEREO) stereo_add(sb.proj, sb.zr, sb.zi, Sr, Si, Gs, s0, s1, s2);
+    if (STEREO && LOW) {
+        // stereographic entries in the batch's projection, software-pipelined:
+        // entry j + 1's node data and entry j + 2's list word are requested
+        // before entry j is summed
+        const int* stl = ls.st + bi * ls.st_cap;
+        const int nst = __builtin_amdgcn_readfirstlane(ls.st_cnt[bi]);
+        if (nst > 0) {
+            const StereoBatch sb = stereo_batch(batch_of(valid, t0, t1, t2), valid, t0, t1, t2);
+            // FP32 sums over runs of kStRun entries, each run then added in FP64
+            constexpr int kStRun = 16;
+            double Sr = 0.0, Si = 0.0, Gs = 0.0;
+            float sr = 0.f, si = 0.f, gs = 0.f;
+            const int64_t base = int64_t(sb.proj) * sv.nn;
+            const unsigned mask = (1u << kNodeBits) - 1u;
+            for (int j = 0; j < nst; ++j) {
+                const unsigned e = unsigned(__builtin_amdgcn_readfirstlane(stl[j]));
+                const StNode cur = stereo_fetch(sv, base + (e & mask));
+                stereo_sum(cur, int(e >> kNodeBits), base + (e & mask), sv, sb.zr, sb.zi, sr, si, gs);
+                if ((j % kStRun) == kStRun - 1 || j + 1 == nst) {
+                    Sr += double(sr);
+                    Si += double(si);
+                    Gs += double(gs);
+                    sr = si = gs = 0.f;
+                }
+            }
+            stereo_add(sb.proj, sb.zr, sb.zi, Sr, Si, Gs, s0, s1, s2);
+        }
+    }
     if (valid) {
         if (LOW) {
             acc[pos] = s0;
@@ -683,6 +764,15 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
             acc[2 * tg.n + pos] += s2;
         }
     }
+}
+
+// cost 0 and the identity order for every wave of the launch (the planning
+// pass sets the costs of the real batches)
+__global__ __launch_bounds__(256) void list_order_init(int64_t n, unsigned* __restrict__ cost, int* __restrict__ idx) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cost[i] = 0u;
+    idx[i] = int(i);
 }
 
 __global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ perm, int64_t n,
@@ -1000,27 +1090,52 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     // plan, evaluate, and check once at the end that every batch's lists fit;
     // if one did not, grow the lists to what it needed and do it again
     for (int attempt = 0;; ++attempt) {
-        const size_t bytes = size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + 3) * sizeof(int) + 64 +
-                             size_t(3 * nt) * sizeof(double);
+        const int st_cap = sv.on ? c->st_cap : 0;
+        size_t sort_bytes = 0;
+        {
+            unsigned* k0 = nullptr;
+            int* v0 = nullptr;
+            LPM_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, k0, k0, v0, v0, int(nbatch),
+                                                                    0, 32, st));
+        }
+        const size_t bytes =
+            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + 4) * sizeof(int) + 64 +
+            size_t(3 * nt) * sizeof(double) + sort_bytes + 256;
         LPM_CHECK(c->tl_work.reserve(bytes));
         double* acc = c->tl_work.as<double>();
         Lists ls;
         ls.far = reinterpret_cast<int*>(acc + 3 * nt);
         ls.near = ls.far + size_t(nbatch) * c->far_cap;
-        ls.far_cnt = ls.near + size_t(nbatch) * c->near_cap;
+        ls.st = ls.near + size_t(nbatch) * c->near_cap;
+        ls.far_cnt = ls.st + size_t(nbatch) * st_cap;
         ls.far_hi_cnt = ls.far_cnt + nbatch;
         ls.near_cnt = ls.far_hi_cnt + nbatch;
-        ls.overflow = ls.near_cnt + nbatch;
+        ls.st_cnt = ls.near_cnt + nbatch;
+        ls.overflow = ls.st_cnt + nbatch;
+        unsigned* cost_out = reinterpret_cast<unsigned*>(ls.overflow + 16);
+        ls.cost = cost_out + nbatch;
+        int* order_in = reinterpret_cast<int*>(ls.cost + nbatch);
+        int* order = order_in + nbatch;
+        void* sort_tmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(order + nbatch) + 255) & ~uintptr_t(255));
+        ls.order = order;
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
-        LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 3 * sizeof(int), st));
+        ls.st_cap = st_cap;
+        LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 4 * sizeof(int), st));
         if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
+        hipLaunchKernelGGL(list_order_init, dim3(unsigned((nbatch + 255) / 256)), dim3(256), 0, st, nbatch, ls.cost,
+                           order_in);
         if (sv.on)
             hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
         else
             hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
+        {
+            size_t sb = sort_bytes;
+            LPM_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sb, ls.cost, cost_out, order_in, order,
+                                                                    int(nbatch), 0, 32, st));
+        }
         auto lists = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
         };
@@ -1042,14 +1157,38 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult,
                            ux, uy, uz);
         LPM_HIP(c, hipGetLastError());
-        LPM_HIP(c, hipMemcpyAsync(c->pin, ls.overflow, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+        LPM_HIP(c, hipMemcpyAsync(c->pin, ls.overflow, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         LPM_HIP(c, hipStreamSynchronize(st));
         if (!c->pin[0]) break;
         if (attempt > 0) return fail(c, LPM_ERR_KERNEL, "interaction lists overflow after growing");
         c->far_cap = std::max(c->far_cap, c->pin[1]);
         c->near_cap = std::max(c->near_cap, c->pin[2]);
+        c->st_cap = std::max(c->st_cap, c->pin[3]);
     }
     return LPM_OK;
+}
+
+// Sort keys of the label rows: (batch, node) as one 64-bit key per row.
+__global__ __launch_bounds__(256) void label_keys(const int* __restrict__ ids, int64_t n, uint64_t* __restrict__ key,
+                                                  int* __restrict__ val) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (uint64_t(unsigned(ids[2 * i])) << 32) | unsigned(ids[2 * i + 1]);
+    val[i] = int(i);
+}
+
+// Row i of the sorted set = row src[i] of the unsorted one.
+__global__ __launch_bounds__(256) void label_gather(const int* __restrict__ src, int64_t n,
+                                                    const float* __restrict__ f_in, const float* __restrict__ e_in,
+                                                    const int* __restrict__ id_in, float* __restrict__ f_out,
+                                                    float* __restrict__ e_out, int* __restrict__ id_out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = src[i];
+    for (int k = 0; k < kIn; ++k) f_out[i * kIn + k] = f_in[r * kIn + k];
+    for (int k = 0; k < kLabelOrder; ++k) e_out[i * kLabelOrder + k] = e_in[r * kLabelOrder + k];
+    id_out[2 * i] = id_in[2 * r];
+    id_out[2 * i + 1] = id_in[2 * r + 1];
 }
 
 static int tree_labels_device(lpm_ctx* c, double theta_max, double delta, double u_ref, int64_t bstride,
@@ -1077,7 +1216,35 @@ static int tree_labels_device(lpm_ctx* c, double theta_max, double delta, double
     unsigned long long h = 0;
     LPM_HIP(c, hipMemcpyAsync(&h, counter, sizeof(h), hipMemcpyDeviceToHost, st));
     LPM_HIP(c, hipStreamSynchronize(st));
-    *n_out = std::min<int64_t>(int64_t(h), max_samples);
+    const int64_t n = std::min<int64_t>(int64_t(h), max_samples);
+    *n_out = n;
+    if (n < 2) return LPM_OK;
+    // The rows arrive in the order of an atomic counter, which varies from
+    // run to run; sorted by (batch, node) they are the same set in the same
+    // order every time, so training (which shuffles with a fixed seed) is
+    // reproducible.  Every (batch, node) pair is labelled at most once.
+    size_t sort_bytes = 0;
+    uint64_t* k0 = nullptr;
+    int* v0 = nullptr;
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k0, v0, v0, int(n), 0, 64, st));
+    const size_t rows = size_t(n) * (kIn + kLabelOrder + 2) * 4;
+    LPM_CHECK(c->lab_work.reserve(size_t(n) * 24 + rows + sort_bytes + 1024));
+    uint64_t* kin = c->lab_work.as<uint64_t>();
+    uint64_t* kout = kin + n;
+    int* vin = reinterpret_cast<int*>(kout + n);
+    int* vout = vin + n;
+    float* f2 = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(vout + n) + 255) & ~uintptr_t(255));
+    float* e2 = f2 + size_t(n) * kIn;
+    int* i2 = reinterpret_cast<int*>(e2 + size_t(n) * kLabelOrder);
+    void* tmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(i2 + 2 * n) + 255) & ~uintptr_t(255));
+    const unsigned g = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL(label_keys, dim3(g), dim3(256), 0, st, ids, n, kin, vin);
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, kin, kout, vin, vout, int(n), 0, 64, st));
+    hipLaunchKernelGGL(label_gather, dim3(g), dim3(256), 0, st, vout, n, feat, err, ids, f2, e2, i2);
+    LPM_HIP(c, hipGetLastError());
+    LPM_HIP(c, hipMemcpyAsync(feat, f2, size_t(n) * kIn * 4, hipMemcpyDeviceToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(err, e2, size_t(n) * kLabelOrder * 4, hipMemcpyDeviceToDevice, st));
+    LPM_HIP(c, hipMemcpyAsync(ids, i2, size_t(n) * 2 * 4, hipMemcpyDeviceToDevice, st));
     return LPM_OK;
 }
 
