@@ -16,11 +16,18 @@ this run is <= 1e-5; otherwise the fixed-order FP32C batch treecode is the
 step (the fallback), and if that misses too, the FP32C direct sum.
 
 Measured in the same run:
-  * FP64 direct sum (the error reference) and FP32C direct sum, with the roofline
-    of their pair kernels;
+  * FP64 direct sums (the error reference: the reference's pair formula with
+    IEEE division; and the fused LPM_F64F form) and the FP32C direct sum, with
+    the roofline of their pair kernels;
   * the plain FP64 GPU treecode (reference acceptance per target, Cartesian
-    series, fixed p and theta) and the fixed-order FP32C batch treecode;
-  * training of the order model on GPU-generated labels (time reported);
+    series, fixed p and theta, fused FP64 near field; the fastest such
+    setting within 1e-5) with its FP64 roofline, and the fixed-order FP32C
+    batch treecode;
+  * training of the order model on GPU-generated labels (time reported, and
+    the number of evaluations after which it has paid for itself);
+  * the model trained on this field evaluated on held-out fields (another
+    vortex seed, the Rossby-Haurwitz field) against models trained on those;
+  * the pairs actually evaluated per second beside the effective rate;
   * RK4 steps/s with the learned treecode and with the plain FP64 treecode;
   * the roofline of the step's evaluation kernels from HIP events recorded by
     the library on its stream (lpm_last_kernel_ms);
@@ -139,15 +146,15 @@ def rk4_level10(ctx, args):
     LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
     err_l = sample_err()
     set_stereo(ctx, args, False)
-    T.build_dev(ctx, X, Y, Z, m.max_radius, args.base_leaf)
-    T.moments_dev(ctx, G, args.base_order, 1)
-    T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
+    T.build_dev(ctx, X, Y, Z, m.max_radius, args.l10_base_leaf)
+    T.moments_dev(ctx, G, args.l10_base_order, 1)
+    T.evaluate_dev(ctx, X, Y, Z, U, args.l10_base_theta, prec=L.LPM_F64F, series=L.LPM_SERIES_CARTESIAN)
     err_b = sample_err()
 
     cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
                                    theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
-    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
-                                   theta=args.base_theta, leaf_max=args.base_leaf)
+    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64F, order=args.l10_base_order,
+                                   theta=args.l10_base_theta, leaf_max=args.l10_base_leaf)
 
     def run(cfg):
         set_stereo(ctx, args, cfg.method == L.LPM_METHOD_TREE_LEARNED)
@@ -176,6 +183,109 @@ def rk4_level10(ctx, args):
             "err_sample": "4096 random targets vs the FP64 direct sum over all sources",
             "trajectory_rel_diff": traj, "tau": args.l10_tau, "label_samples": int(len(ls.ids)),
             "labels_and_training_s": t_fit, "train_accuracy": fr.accuracy}
+
+
+def generalization(ctx, args, m, X, Y, Z, steps: int) -> dict:
+    """The order model trained on the seed-1234 vortex field, evaluated without
+    retraining on held-out fields of the same mesh: 8 Gaussian vortices of
+    seed 4321, and the solid body + Rossby-Haurwitz field 2 z + 30 cos(4
+    lambda) z (z^2 - 1)^2.  Each is then evaluated again with a model
+    trained on its own labels.  Reported per field and model: ms per step,
+    rel-L2 error on 4096 random targets against the FP64 direct sum, near
+    and far pairs; and the labels + training time of the own-field model."""
+    import numpy as np
+    import torch
+    import lpmcxx_amd as L
+    from lpmcxx_amd import learned as LN
+    from lpmcxx_amd import tree as T
+
+    om = LN.OrderModel(ctx)
+    trained = om.get_params()
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    n = m.n_leaves
+    idx = np.sort(np.random.default_rng(17).choice(n, 4096, replace=False))
+    I = dev(idx.astype(np.int64))
+    S = [dev(a[idx]) for a in (m.cx, m.cy, m.cz)]
+    U = [torch.empty_like(X) for _ in range(3)]
+    fields = {"vortices_seed_4321": L.random_vortices(m.cx, m.cy, m.cz, m.area, n=8, seed=4321),
+              "rossby_haurwitz": 2.0 * m.cz + L.sphere_harmonic54(m.cx, m.cy, m.cz)}
+    out = {}
+    for name, zeta in fields.items():
+        g = zeta * m.area
+        G = dev(g)
+        u_ref = LN.u_ref_of(g)
+        ref = [torch.empty_like(S[0]) for _ in range(3)]
+        ctx.direct_sum_dev(*S, X, Y, Z, G, ref, prec=L.LPM_F64)
+        ctx.synchronize()  # the library's stream, not torch's
+        R = torch.stack(ref)
+
+        def step():
+            T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+            T.moments_dev(ctx, G, args.max_order, 1)
+            LN.evaluate_dev(ctx, X, Y, Z, U, args.theta_max, args.margin, args.max_order, u_ref, prec=L.LPM_F32C)
+
+        def measure():
+            set_stereo(ctx, args, True)
+            step()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+            err = float(torch.linalg.norm(torch.stack([u[I] for u in U]) - R) / torch.linalg.norm(R))
+            st = T.stats(ctx)
+            return {"ms_per_step": 1e3 * dt, "rel_l2_err": err, "near_pairs": st["near"], "far_pairs": st["far"]}
+
+        om.set_params(*trained)
+        res = {"seed_1234_model": measure()}
+        set_stereo(ctx, args, False)
+        t0 = time.perf_counter()
+        T.build_dev(ctx, X, Y, Z, m.max_radius, args.leaf)
+        T.moments_dev(ctx, G, 12, 1)
+        ls = LN.labels(ctx, (m.cx, m.cy, m.cz), args.theta_max, u_ref, batch_stride=args.label_stride,
+                       keep_rate=args.label_keep, frontier_levels=2, max_samples=3 << 20)
+        LN.fit(ctx, ls, args.tau, args.max_order, epochs=args.epochs, batch=4096, lr=2e-3,
+               stereo_extra=stereo_extra(args))
+        t_fit = time.perf_counter() - t0
+        res["own_field_model"] = measure()
+        res["own_field_model"]["labels_and_training_s"] = t_fit
+        a, b = res["seed_1234_model"], res["own_field_model"]
+        log(f"[bench] generalization {name}: seed-1234 model {a['ms_per_step']:.2f} ms err {a['rel_l2_err']:.2e} "
+            f"(near {a['near_pairs']}, far {a['far_pairs']}); own model {b['ms_per_step']:.2f} ms err "
+            f"{b['rel_l2_err']:.2e} (near {b['near_pairs']}, far {b['far_pairs']}), trained in {t_fit:.2f} s")
+        out[name] = res
+    om.set_params(*trained)
+    out["err_sample"] = "4096 random targets vs the FP64 direct sum over all sources"
+    return out
+
+
+def cart_far_flops(p: int) -> int:
+    """FP64 flops of one accepted (target, node) pair of the per-target
+    treecode at order p: R2 (6), its reciprocal (1), b_0 and the first three
+    products (4), 2 d (3), then per term k != 0 the recurrence (an FMA per
+    non-zero k_i, the first a multiply; a subtraction per k_i >= 2; the scaling
+    by 1/R2) and three FMAs against the moments, and t x S (9)."""
+    tot = 6 + 1 + 4 + 3 + 9
+    for n in range(1, p):
+        for k1 in range(n, -1, -1):
+            for k2 in range(n - k1, -1, -1):
+                k = (k1, k2, n - k1 - k2)
+                n1, n2 = sum(1 for a in k if a >= 1), sum(1 for a in k if a >= 2)
+                tot += 2 * n1 - 1 + n2 + 1 + 6
+    return tot
+
+
+def fp64_tree_roof(st: dict, ms: float, p: int) -> dict:
+    """FP64 roofline of tree_eval<p> with the fused near field: DIRECT_FLOPS
+    per near pair, cart_far_flops(p) per accepted node; the MAC tests of the
+    descent are overhead and not counted."""
+    flops = st["near"] * DIRECT_FLOPS + st["far"] * cart_far_flops(p)
+    ach = flops / (ms / 1e3) / 1e12
+    return {"bound": "valu", "kernel": f"tree_eval<{p}, fused FP64 near field>", "achieved": ach,
+            "peak": FP64_PEAK, "unit": "TFLOP/s", "frac": ach / FP64_PEAK, "flops_per_launch": flops,
+            "launch_ms": ms, "near_pairs": st["near"], "far_pairs": st["far"],
+            "far_flops_per_pair": cart_far_flops(p), "near_flops_per_pair": DIRECT_FLOPS}
 
 
 def flop_roof(kernel: str, ms: float, near: int, far: int, terms: int, near_flops: int, far_flops: int = 16,
@@ -401,12 +511,12 @@ def workload_bve(args) -> dict:
     set_stereo(ctx, args, False)
     T.build_dev(ctx, SX, SY, SZ, m.max_radius, args.base_leaf)
     T.moments_dev(ctx, G, args.base_order, 1)
-    T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
+    T.evaluate_dev(ctx, X, Y, Z, U, args.base_theta, prec=L.LPM_F64F, series=L.LPM_SERIES_CARTESIAN)
     err_b = err()
 
     cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
                                    theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
-    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
+    cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64F, order=args.base_order,
                                    theta=args.base_theta, leaf_max=args.base_leaf)
 
     def run(cfg, nsteps, warm):
@@ -597,7 +707,7 @@ def main() -> int:
     # plain FP64 treecode (reference acceptance) and fixed-order batch treecode
     ap.add_argument("--base-order", type=int, default=6)
     ap.add_argument("--base-theta", type=float, default=0.35)
-    ap.add_argument("--base-leaf", type=int, default=128)
+    ap.add_argument("--base-leaf", type=int, default=256)
     ap.add_argument("--batch-order", type=int, default=7)
     ap.add_argument("--batch-theta", type=float, default=0.4)
     ap.add_argument("--batch-leaf", type=int, default=256)
@@ -605,6 +715,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
+    ap.add_argument("--no-generalization", action="store_true", help="skip the held-out field evaluation")
     ap.add_argument("--no-direct64", action="store_true", help="judge errors against the FP32C direct sum")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--no-rk4-l10", action="store_true", help="skip the level-10 RK4 comparison")
@@ -612,6 +723,11 @@ def main() -> int:
     ap.add_argument("--l10-steps", type=int, default=10)
     ap.add_argument("--l10-tau", type=float, default=None, help="default 5e-7 (stereographic), 1.5e-7 (Cartesian)")
     ap.add_argument("--l10-label-stride", type=int, default=64)
+    # the fastest plain FP64 setting within 1e-5 at level 10 on the Rossby-Haurwitz field (p 6, theta 0.35
+    # gives 1.00e-5 there)
+    ap.add_argument("--l10-base-order", type=int, default=7, help="plain FP64 treecode order at level 10")
+    ap.add_argument("--l10-base-theta", type=float, default=0.4)
+    ap.add_argument("--l10-base-leaf", type=int, default=256)
     ap.add_argument("--workload", choices=["learned", "mesh", "psi", "bve"], default="learned",
                     help="learned: the velocity step (default); mesh: GPU mesh generation; psi: the "
                          "streamfunction treecode; bve: the barotropic vorticity RK4 of faces and vertices")
@@ -683,28 +799,43 @@ def main() -> int:
         k32.append(ctx.last_kernel_ms())
     ms32 = float(np.mean(k32))
     if args.no_direct64:
-        U64, ms64 = U32, None
+        U64, ms64, ms64f, err64f = U32, None, None, None
     else:
-        log("[bench] FP64 direct sum")
+        log("[bench] FP64 direct sums (IEEE-division form, fused form)")
         U64 = [torch.empty_like(X) for _ in range(3)]
         ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, U64, prec=L.LPM_F64)
         ms64 = ctx.last_kernel_ms()
+        U64F = [torch.empty_like(X) for _ in range(3)]
+        ctx.direct_sum_dev(X, Y, Z, X, Y, Z, G, U64F, prec=L.LPM_F64F)
+        ms64f = ctx.last_kernel_ms()
+        err64f = rel_l2(U64F, U64)
+        del U64F
     err32 = rel_l2(U32, U64)
-    log(f"[bench] direct: f64 {ms64} ms, f32c {ms32:.1f} ms (pair kernels), f32c rel-L2 {err32:.2e}")
+    log(f"[bench] direct: f64 {ms64} ms, f64f {ms64f} ms (rel-L2 vs f64 {err64f}), f32c {ms32:.1f} ms (pair "
+        f"kernels), f32c rel-L2 {err32:.2e}")
 
     UT = [torch.empty_like(X) for _ in range(3)]
 
-    # --- plain FP64 GPU treecode: reference acceptance per target, Cartesian series
-    def base_step():
+    # --- plain FP64 GPU treecode: reference acceptance per target, Cartesian
+    # series, fused FP64 near field (the faster FP64 form); the IEEE-division
+    # near field timed beside it
+    def base_step(prec=L.LPM_F64F):
         T.build_dev(ctx, X, Y, Z, m.max_radius, args.base_leaf)
         T.moments_dev(ctx, G, args.base_order, 1)
-        T.evaluate_dev(ctx, X, Y, Z, UT, args.base_theta, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
+        T.evaluate_dev(ctx, X, Y, Z, UT, args.base_theta, prec=prec, series=L.LPM_SERIES_CARTESIAN)
 
     base_step()
     base_s = wall(base_step, 3)
     base_eval_ms = ctx.last_kernel_ms()
     base_err = rel_l2(UT, U64)
-    log(f"[bench] plain FP64 treecode: {1e3 * base_s:.1f} ms/step (eval {base_eval_ms:.1f}), rel-L2 {base_err:.2e}")
+    base_st = T.stats(ctx)
+    base_roof = fp64_tree_roof(base_st, base_eval_ms, args.base_order)
+    base_step(L.LPM_F64)
+    base_ieee_s = wall(lambda: base_step(L.LPM_F64), 2)
+    base_ieee_eval_ms = ctx.last_kernel_ms()
+    log(f"[bench] plain FP64 treecode: {1e3 * base_s:.1f} ms/step (eval {base_eval_ms:.1f}, "
+        f"{base_roof['frac']:.3f} of the FP64 peak), rel-L2 {base_err:.2e}; IEEE-division near field "
+        f"{1e3 * base_ieee_s:.1f} ms/step (eval {base_ieee_eval_ms:.1f})")
 
     # --- fixed-order FP32C batch treecode (the fallback)
     def batch_step():
@@ -768,6 +899,8 @@ def main() -> int:
         f"{learned_eval_ms:.2f} ms), "
         f"rel-L2 {learned_err:.2e}; far {st['far']} near {st['near']} terms {st['terms']} model {st['model']}")
 
+    gen = None if args.no_generalization else generalization(ctx, args, m, X, Y, Z, args.steps)
+
     if learned_err <= 1e-5:
         step_name, step_s, step_err = "learned", learned_s, learned_err
     elif batch_err <= 1e-5:
@@ -775,6 +908,10 @@ def main() -> int:
     else:
         step_name, step_s, step_err = "direct_f32c", ms32 / 1e3, err32
     value = pairs / step_s
+
+    def payoff(cost_s, other_s, own_s):
+        """evaluations after which labels + training cost less than the time saved"""
+        return None if other_s <= own_s else cost_s / (other_s - own_s)
 
     # roofline of the step's dominant evaluation kernel (FP32 VALU work); for
     # the learned step that is the low-order list kernel, so the far pairs and
@@ -809,7 +946,7 @@ def main() -> int:
         log("[bench] RK4 steps")
         cfg_l = L.SolverConfig.default(method=L.LPM_METHOD_TREE_LEARNED, prec=L.LPM_F32C, order=args.max_order,
                                        theta=args.theta_max, leaf_max=args.leaf, margin=args.margin, u_ref=u_ref)
-        cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64, order=args.base_order,
+        cfg_b = L.SolverConfig.default(method=L.LPM_METHOD_TREE, prec=L.LPM_F64F, order=args.base_order,
                                        theta=args.base_theta, leaf_max=args.base_leaf)
         rl, rb = rk4_rate(cfg_l), rk4_rate(cfg_b)
 
@@ -860,21 +997,33 @@ def main() -> int:
                    "rel_l2_err_vs_fp64_direct": step_err},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "pairs_evaluated_per_step": st["near"] + st["far"],
+        "pairs_evaluated_per_s": (st["near"] + st["far"]) / learned_s,
         "learned": {"ms_per_step": 1e3 * learned_s, "low_list_kernel_ms": learned_eval_ms, "rel_l2_err": learned_err,
                     "speedup_vs_plain_fp64_tree": base_s / learned_s, "label_samples": int(len(ls.ids)),
+                    "pairs_evaluated_per_s": (st["near"] + st["far"]) / learned_s,
+                    "effective_interactions_per_s": pairs / learned_s,
+                    "labels_and_training_s": t_label + t_train,
+                    "payoff_evaluations_vs_batch_f32c_tree": payoff(t_label + t_train, batch_s, learned_s),
+                    "payoff_evaluations_vs_plain_fp64_tree": payoff(t_label + t_train, base_s, learned_s),
+                    "generalization": gen,
                     "label_s": t_label, "train_s": t_train, "train_loss": fr.final_loss,
                     "train_accuracy": fr.accuracy, "stats": st, "nodes": nn, "depth": depth, "leaves": nleaves},
-        "plain_fp64_tree": {"ms_per_step": 1e3 * base_s, "eval_ms": base_eval_ms, "rel_l2_err": base_err,
+        "plain_fp64_tree": {"form": "LPM_F64F (fused FP64 near field), Cartesian series",
+                            "ms_per_step": 1e3 * base_s, "eval_ms": base_eval_ms, "rel_l2_err": base_err,
                             "order": args.base_order, "theta": args.base_theta, "leaf": args.base_leaf,
-                            "effective_interactions_per_s": pairs / base_s},
+                            "effective_interactions_per_s": pairs / base_s, "roofline": base_roof,
+                            "ieee_form_ms_per_step": 1e3 * base_ieee_s, "ieee_form_eval_ms": base_ieee_eval_ms},
         "batch_f32c_tree": {"ms_per_step": 1e3 * batch_s, "eval_ms": batch_eval_ms, "rel_l2_err": batch_err,
                             "order": args.batch_order, "theta": args.batch_theta, "leaf": args.batch_leaf,
                             "speedup_vs_plain_fp64_tree": base_s / batch_s, "stats": batch_stats},
         "direct_f32c": {"interactions_per_s": pairs / (ms32 / 1e3), "kernel_ms": ms32, "rel_l2_err": err32,
                         "roofline": direct_roof},
         "direct_f64": None if ms64 is None else
-        {"interactions_per_s": pairs / (ms64 / 1e3), "kernel_ms": ms64,
-         "frac_of_fp64_peak": direct_flops / (ms64 / 1e3) / (FP64_PEAK * 1e12)},
+        {"kernel": "ds_f64 (LPM_F64F: fused FP64)", "interactions_per_s": pairs / (ms64f / 1e3),
+         "kernel_ms": ms64f, "frac_of_fp64_peak": direct_flops / (ms64f / 1e3) / (FP64_PEAK * 1e12),
+         "rel_l2_vs_ieee_form": err64f, "ieee_form_kernel_ms": ms64,
+         "ieee_form_frac_of_fp64_peak": direct_flops / (ms64 / 1e3) / (FP64_PEAK * 1e12)},
     }
     if rk4:
         out.update(rk4)

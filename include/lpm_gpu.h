@@ -58,6 +58,13 @@ enum lpm_status {
 #define LPM_F64 0  /* FP64, the reference's per-pair formula */
 #define LPM_F32C 1 /* FP32 pair math on tile-local offsets, |x-y|^2/2 denominator,
                       compensated FP32 tile sums, FP64 final reduction */
+#define LPM_F64F 2 /* FP64, the reference's pair regrouped for the FP64 units:
+                      x x sum_j w_j (y_j - x) with the reference's denominator and
+                      zero rule, refined hardware reciprocal (about an ulp), fused
+                      multiply-adds; agrees with LPM_F64 to rounding.  Accepted by
+                      the direct sum (velocity; the potential is as LPM_F64), by
+                      lpm_tree_eval (near field; the far field is as LPM_F64) and
+                      by the RK4 / BVE steps with those methods */
 
 /* far-field series of the treecode */
 #define LPM_SERIES_REFERENCE 0 /* SumNode::calc_coeffs: Taylor series of 1/(1 + delta^2 - t.y) about the box
@@ -89,7 +96,7 @@ void* lpm_stream(lpm_ctx* ctx);
  *   potential psi_i = -1/(4 pi R) sum_{j != skip_i} gamma_j log(R^2 - x_i.y_j)
  *             (src/LpmScalarKernel.h:35-38).
  * kernels  : LPM_K_BIOT_SAVART_SPHERE and/or LPM_K_GREENS_SPHERE (bit mask).
- * prec     : LPM_F64 or LPM_F32C (LPM_F32C: every point on |x| = radius,
+ * prec     : LPM_F64, LPM_F64F or LPM_F32C (LPM_F32C: every point on |x| = radius,
  *             see the sphere precondition above).
  * gamma_j  : source strength (vorticity x area in the mesh solve).
  * tgt_self_src : per-target source index excluded from the potential sum
@@ -117,7 +124,7 @@ int lpm_direct_sum_dev(lpm_ctx* ctx, int kernels, int prec, double radius, doubl
  * prepared sources and the per-split partials) lives in one static device
  * array of 32 MiB per GPU (made when the library is loaded on that GPU, held
  * until the process ends), so it calls no hipMalloc / hipFree.  Fits when
- * 256 + ns x 48 (LPM_F32C; 32 for LPM_F64, rounded up to 256 sources and 256
+ * 256 + ns x 48 (LPM_F32C; 32 for LPM_F64[F], rounded up to 256 sources and 256
  * bytes) + nt x 32 bytes <= 32 MiB, e.g. 98,304 sources with up to 900,000
  * targets; larger problems return LPM_ERR_INVALID_ARG.  When the partials of
  * every split do not fit, fewer splits are used (the sum over splits then
@@ -191,6 +198,8 @@ int lpm_tree_moments_dev(lpm_ctx* ctx, const double* gamma, int order, int momen
  * kernel at unaccepted leaves; result scaled by -1/(4 pi).  mac_count (may be
  * NULL) receives the number of accepted (target, node) pairs, the reference's
  * macCounter.  LPM_SERIES_CARTESIAN requires moments with moment_sign = +1.
+ * prec: LPM_F64 (reference pair formula at the leaves), LPM_F64F (its fused
+ * form) or LPM_F32C (FP32 leaf sums); the far field is FP64 in all three.
  * Replaces: TreeSum::meshSolve(verts, faces, mp_tol, macCounter) src/LpmTreeSum.h:67. */
 int lpm_tree_eval(lpm_ctx* ctx, int prec, int series, double theta, double delta,
                   const double* tx, const double* ty, const double* tz, int64_t nt,
@@ -388,7 +397,7 @@ int lpm_tree_eval_learned_dev(lpm_ctx* ctx, int prec, double theta_max, int marg
 #define LPM_METHOD_TREE_LEARNED 3  /* lpm_tree_eval_learned: theta = theta_max, order = max_order */
 typedef struct lpm_solver_config {
     int method;              /* LPM_METHOD_DIRECT or LPM_METHOD_TREE */
-    int prec;                /* LPM_F64 or LPM_F32C */
+    int prec;                /* LPM_F64 or LPM_F32C; LPM_F64F with LPM_METHOD_DIRECT / LPM_METHOD_TREE */
     int series;              /* tree: LPM_SERIES_CARTESIAN (moments use sign +1) or LPM_SERIES_REFERENCE */
     int order;               /* tree: series order p, |k| < p, in [1, 12] */
     int leaf_max;            /* tree: sources per leaf */

@@ -25,6 +25,7 @@ LPM_K_BIOT_SAVART_SPHERE = 1
 LPM_K_GREENS_SPHERE = 2
 LPM_F64 = 0
 LPM_F32C = 1
+LPM_F64F = 2
 LPM_SEED_ICOS_TRI = 0
 LPM_SERIES_REFERENCE = 0
 LPM_SERIES_CARTESIAN = 1

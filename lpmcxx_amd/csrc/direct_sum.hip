@@ -22,6 +22,12 @@
 // The library is compiled with -ffp-contract=off so each pair value is
 // bit-identical to the reference's; only the summation order differs.
 //
+// FP64F form (LPM_F64F): the FP64 pair regrouped for the FP64 units --
+// u_i = -1/(4 pi R) x_i x sum_j w_j (y_j - x_i), w_j = gamma_j / den_j with
+// the reference's denominator and zero rule, a refined hardware
+// reciprocal and FMA accumulation (21 VALU instructions per pair against
+// about 45 for the IEEE-division form).  It agrees with LPM_F64 to rounding.
+//
 // FP32C form (LPM_F32C): R^2 - x.y is formed as |d|^2/2 with d = x - y,
 // u = 1/(4 pi R) x * sum_j [2 gamma_j / (|d|^2 + 2 delta^2)] d_j.
 // Positions enter as FP32 offsets from FP64 centers (per wave of targets and
@@ -266,7 +272,15 @@ __global__ __launch_bounds__(256) void ds_f32c(const double* __restrict__ tx, co
 }
 
 // ---------------------------------------------------------------- FP64 pair kernel
-template <int TPL, bool VEL, bool PSI>
+// FAST (LPM_F64F): the same pair value regrouped.  x_i x y_j = x_i x (y_j - x_i),
+// so sum_j w_j (x_i x y_j) = x_i x sum_j w_j e_j with e_j = y_j - x_i
+// (exactly zero for a coincident pair; e_j is exact when y_j is within a
+// factor 2 of x_i, so close pairs add no cancellation);
+// the cross product is applied once per target by ds_reduce.  The
+// denominator is formed as in the reference (same dot product order, same
+// zero rule); the weight gamma / den comes from v_rcp_f64 refined to about
+// an ulp (weight_f64f); the accumulation uses FMA.
+template <int TPL, bool VEL, bool PSI, bool FAST>
 __global__ __launch_bounds__(256) void ds_f64(const double* __restrict__ tx, const double* __restrict__ ty,
                                               const double* __restrict__ tz, int64_t nt,
                                               const double4* __restrict__ src, int64_t nsp, int64_t per_split,
@@ -296,7 +310,15 @@ __global__ __launch_bounds__(256) void ds_f64(const double* __restrict__ tx, con
         const double4 s = src[j];
 #pragma unroll
         for (int k = 0; k < TPL; ++k) {
-            if (VEL) {
+            if (VEL && FAST) {
+                const double ex = s.x - px[k], ey = s.y - py[k], ez = s.z - pz[k];
+                const double dot = px[k] * s.x + py[k] * s.y + pz[k] * s.z;
+                const double den = R2 - dot + d2;
+                const double w = fabs(den) > 1.0e-13 ? weight_f64f(s.w, den) : 0.0;
+                ax[k] = __builtin_fma(w, ex, ax[k]);
+                ay[k] = __builtin_fma(w, ey, ay[k]);
+                az[k] = __builtin_fma(w, ez, az[k]);
+            } else if (VEL) {
                 const double cx = py[k] * s.z - pz[k] * s.y;
                 const double cy = pz[k] * s.x - px[k] * s.z;
                 const double cz = px[k] * s.y - py[k] * s.x;
@@ -328,7 +350,8 @@ __global__ __launch_bounds__(256) void ds_f64(const double* __restrict__ tx, con
 // ---------------------------------------------------------------- reduction over splits
 // f32c: u = vel_scale * (x * S), psi = psi_scale * (A - (Gtot - gamma[skip]))
 // f64 : u = vel_scale * S,       psi = psi_scale * A
-template <bool F32C>
+// f64f: u = vel_scale * (x * S), psi = psi_scale * A
+template <bool CROSS, bool F32C>
 __global__ __launch_bounds__(256) void ds_reduce(const double* __restrict__ part, int nsplit, int64_t ntp, int64_t nt,
                                                  const double* __restrict__ tx, const double* __restrict__ ty,
                                                  const double* __restrict__ tz, int vel, int psi, double vel_scale,
@@ -344,7 +367,7 @@ __global__ __launch_bounds__(256) void ds_reduce(const double* __restrict__ part
         if (psi) sp += p[3 * ntp + i];
     }
     if (vel) {
-        if (F32C) {
+        if (CROSS) {
             const double x = tx[i], y = ty[i], z = tz[i];
             ux[i] = vel_scale * (y * sz - z * sy);
             uy[i] = vel_scale * (z * sx - x * sz);
@@ -375,11 +398,18 @@ void launch_f32c(dim3 g, hipStream_t st, const double* tx, const double* ty, con
 }
 
 template <int TPL, bool VEL, bool PSI>
-void launch_f64(dim3 g, hipStream_t st, const double* tx, const double* ty, const double* tz, int64_t nt,
+void launch_f64(dim3 g, hipStream_t st, bool fast, const double* tx, const double* ty, const double* tz, int64_t nt,
                 const double4* src, int64_t nsp, int64_t per, double R2, double d2, const int64_t* skip, double* part,
                 int64_t ntp) {
-    hipLaunchKernelGGL((ds_f64<TPL, VEL, PSI>), g, dim3(256), 0, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip,
-                       part, ntp);
+    if constexpr (VEL) {  // the fused form changes the velocity pair only
+        if (fast) {
+            hipLaunchKernelGGL((ds_f64<TPL, VEL, PSI, true>), g, dim3(256), 0, st, tx, ty, tz, nt, src, nsp, per, R2,
+                               d2, skip, part, ntp);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((ds_f64<TPL, VEL, PSI, false>), g, dim3(256), 0, st, tx, ty, tz, nt, src, nsp, per, R2, d2,
+                       skip, part, ntp);
 }
 
 constexpr int kTplF32 = 4;
@@ -457,7 +487,7 @@ int ds_launch(lpm_ctx* c, const DsPlan& P, int kernels, int prec, double radius,
         else launch_f32c<kTplF32, false, true>(g, st, tx, ty, tz, nt, src, scen, src64, d2x2, ntiles, tps, eps2v, skip, part, ntp);
         mark_end(c);
         LPM_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(ds_reduce<true>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit, ntp,
+        hipLaunchKernelGGL((ds_reduce<true, true>), dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit, ntp,
                            nt, tx, ty, tz, int(vel), int(pot), 2.0 / (4.0 * pi * radius),
                            -std::log(2.0) / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
     } else {
@@ -466,15 +496,19 @@ int ds_launch(lpm_ctx* c, const DsPlan& P, int kernels, int prec, double radius,
                            ns, nsp, src);
         const int64_t per = int64_t(tps) * kTile;
         const double R2 = radius * radius, d2 = delta * delta;
+        const bool fast = prec == LPM_F64F;
         mark_begin(c);
-        if (vel && pot) launch_f64<kTplF64, true, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
-        else if (vel) launch_f64<kTplF64, true, false>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
-        else launch_f64<kTplF64, false, true>(g, st, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        if (vel && pot)
+            launch_f64<kTplF64, true, true>(g, st, fast, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        else if (vel)
+            launch_f64<kTplF64, true, false>(g, st, fast, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
+        else launch_f64<kTplF64, false, true>(g, st, fast, tx, ty, tz, nt, src, nsp, per, R2, d2, skip, part, ntp);
         mark_end(c);
         LPM_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(ds_reduce<false>, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit,
-                           ntp, nt, tx, ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius),
-                           -1.0 / (4.0 * pi * radius), gtot, gamma, skip, ux, uy, uz, psi);
+        auto reduce = fast ? ds_reduce<true, false> : ds_reduce<false, false>;
+        hipLaunchKernelGGL(reduce, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, part, P.nsplit, ntp, nt, tx,
+                           ty, tz, int(vel), int(pot), -1.0 / (4.0 * pi * radius), -1.0 / (4.0 * pi * radius), gtot,
+                           gamma, skip, ux, uy, uz, psi);
     }
     LPM_HIP(c, hipGetLastError());
     return LPM_OK;

@@ -221,7 +221,8 @@ static int check_direct_args(lpm_ctx* c, int kernels, int prec, double radius, d
     c->err.clear();
     if (kernels <= 0 || (kernels & ~(LPM_K_BIOT_SAVART_SPHERE | LPM_K_GREENS_SPHERE)))
         return fail(c, LPM_ERR_KERNEL, "invalid kernel mask %d", kernels);
-    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
+    if (prec != LPM_F64 && prec != LPM_F32C && prec != LPM_F64F)
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid precision %d", prec);
     if (nt < 0 || ns < 0 || ns > (int64_t(1) << 31) - 4096 || nt > (int64_t(1) << 40))
         return fail(c, LPM_ERR_INVALID_ARG, "invalid sizes nt=%lld ns=%lld", (long long)nt, (long long)ns);
     if (!(radius > 0.0) || !(delta >= 0.0) || !std::isfinite(radius) || !std::isfinite(delta))

@@ -173,6 +173,19 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
                              double delta, const double* tx, const double* ty, const double* tz, int64_t nt,
                              double* ux, double* uy, double* uz);
 
+// ---------------------------------------------------------------- fused FP64 weight
+// gamma / den for the LPM_F64F pair: r0 = v_rcp_f64(den) (relative error
+// below 2^-24), e = 1 - den r0 by one FMA, and 1 / den = r0 / (1 - e) =
+// r0 (1 + e + e^2 + O(e^3)), so gamma / den = w0 + w0 (e + e^2) with
+// w0 = gamma r0: within about an ulp of the IEEE quotient (e^3 < 1e-22).
+// den == 0 gives NaN; the callers' zero rule (|den| <= 1e-13) discards it.
+__device__ __forceinline__ double weight_f64f(double gamma, double den) {
+    const double r0 = __builtin_amdgcn_rcp(den);
+    const double w0 = gamma * r0;
+    const double e = __builtin_fma(-den, r0, 1.0);
+    return __builtin_fma(w0, __builtin_fma(e, e, e), w0);
+}
+
 // ---------------------------------------------------------------- sphere precondition
 // The |x - y|^2 / 2 forms of R^2 - x.y (LPM_F32C, and the Cartesian,
 // stereographic and logarithmic series) hold only for points on the sphere

@@ -124,7 +124,10 @@ static int check_rk4(lpm_ctx* c, const lpm_solver_config* cfg, double dt, int64_
     if (!std::isfinite(dt)) return fail(c, LPM_ERR_INVALID_ARG, "invalid dt");
     if (cfg->method < LPM_METHOD_DIRECT || cfg->method > LPM_METHOD_TREE_LEARNED)
         return fail(c, LPM_ERR_INVALID_ARG, "invalid velocity method");
-    if (cfg->prec != LPM_F64 && cfg->prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
+    if (cfg->prec != LPM_F64 && cfg->prec != LPM_F32C && cfg->prec != LPM_F64F)
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
+    if (cfg->prec == LPM_F64F && cfg->method != LPM_METHOD_DIRECT && cfg->method != LPM_METHOD_TREE)
+        return fail(c, LPM_ERR_INVALID_ARG, "LPM_F64F serves the direct sum and the per-target treecode only");
     if (cfg->method == LPM_METHOD_TREE_LEARNED) {
         if (!model_weights(c)) return fail(c, LPM_ERR_STATE, "no order model (lpm_model_set_params)");
         if (!(cfg->u_ref > 0.0) || cfg->margin < 0)

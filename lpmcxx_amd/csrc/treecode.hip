@@ -30,7 +30,11 @@
 //                  then t x S.  FP64 throughout.
 //                  Near field: LPM_F64 uses the reference's pair formula;
 //                  LPM_F32C uses FP32 offsets from the leaf centroid,
-//                  |d|^2/2 denominators and an FP32 per-leaf sum.
+//                  |d|^2/2 denominators and an FP32 per-leaf sum; LPM_F64F
+//                  keeps the reference's denominator and zero rule, takes
+//                  gamma / den from v_rcp_f64 refined to about an ulp and
+//                  sums w_j (y_j - t) with FMA, applying t x once per
+//                  target (t x y_j = t x (y_j - t)).
 #include "lpm_tree.hpp"
 #include "lpm_mlp.hpp"
 #include "lpm_series.hpp"
@@ -398,7 +402,12 @@ __device__ inline uint64_t uniform64(uint64_t v) {
     return (uint64_t(hi) << 32) | lo;
 }
 
-template <int P, bool F32NEAR>
+// near-field forms of tree_eval
+constexpr int kNearF64 = 0;   // the reference's pair formula, IEEE division
+constexpr int kNearF32 = 1;   // FP32 offsets from the leaf centroid, |d|^2/2 denominators
+constexpr int kNearF64F = 2;  // the reference's denominator, refined hardware reciprocal, t x sum w (y - t)
+
+template <int P, int NEAR>
 __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, const double* __restrict__ ty,
                                                  const double* __restrict__ tz, const int* __restrict__ tperm,
                                                  int64_t nt, const double4* __restrict__ ncent,
@@ -419,6 +428,7 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
     const int orig = valid ? tperm[it] : 0;
     const double t0 = valid ? tx[orig] : 0.0, t1 = valid ? ty[orig] : 0.0, t2 = valid ? tz[orig] : 0.0;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;  // kNearF64F: sum_j w_j (y_j - t) over the near sources
     unsigned mac = 0;
     unsigned long long npairs = 0;
     const uint64_t vmask = __ballot(valid);
@@ -458,7 +468,7 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
             const bool mine = (rest >> lane) & 1ull;
             const int b = nbeg[n], e = nend[n];
             npairs += mine ? (unsigned long long)(e - b) : 0ull;
-            if (F32NEAR) {
+            if (NEAR == kNearF32) {
                 const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
                 float a0 = 0.f, a1 = 0.f, a2 = 0.f;
                 if (mine) {
@@ -477,6 +487,18 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
                     s0 -= 2.0 * (t1 * b2 - t2 * b1);
                     s1 -= 2.0 * (t2 * b0 - t0 * b2);
                     s2 -= 2.0 * (t0 * b1 - t1 * b0);
+                }
+            } else if (NEAR == kNearF64F) {
+                if (mine) {
+                    for (int j = b; j < e; ++j) {
+                        const double4 s = spos[j];
+                        const double dot = t0 * s.x + t1 * s.y + t2 * s.z;
+                        const double den = R2 - dot + d2;
+                        const double wgt = fabs(den) > 1.0e-13 ? weight_f64f(s.w, den) : 0.0;
+                        e0 = __builtin_fma(wgt, s.x - t0, e0);
+                        e1 = __builtin_fma(wgt, s.y - t1, e1);
+                        e2 = __builtin_fma(wgt, s.z - t2, e2);
+                    }
                 }
             } else if (mine) {
                 for (int j = b; j < e; ++j) {
@@ -501,6 +523,11 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
                 ++sp;
             }
         }
+    }
+    if (NEAR == kNearF64F) {
+        s0 += t1 * e2 - t2 * e1;
+        s1 += t2 * e0 - t0 * e2;
+        s2 += t0 * e1 - t1 * e0;
     }
     if (valid) {
         ux[orig] = s0 * mult;
@@ -758,14 +785,16 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
     };
     mark_begin(c);
     if (series == LPM_SERIES_REFERENCE) {
-        if (prec == LPM_F32C) args(tree_eval<0, true>);
-        else args(tree_eval<0, false>);
+        if (prec == LPM_F32C) args(tree_eval<0, kNearF32>);
+        else if (prec == LPM_F64F) args(tree_eval<0, kNearF64F>);
+        else args(tree_eval<0, kNearF64>);
     } else {
         switch (p) {
-#define LPM_CASE(PP)                                      \
-    case PP:                                              \
-        if (prec == LPM_F32C) args(tree_eval<PP, true>);  \
-        else args(tree_eval<PP, false>);                  \
+#define LPM_CASE(PP)                                              \
+    case PP:                                                      \
+        if (prec == LPM_F32C) args(tree_eval<PP, kNearF32>);      \
+        else if (prec == LPM_F64F) args(tree_eval<PP, kNearF64F>); \
+        else args(tree_eval<PP, kNearF64>);                       \
         break;
             LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
             LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
@@ -863,7 +892,8 @@ static int check_eval(lpm_ctx* c, int prec, int series, double theta, double del
     c->err.clear();
     if (!c->tree) return fail(c, LPM_ERR_STATE, "lpm_tree_eval before lpm_tree_build");
     if (c->tree->order == 0) return fail(c, LPM_ERR_STATE, "lpm_tree_eval before lpm_tree_moments");
-    if (prec != LPM_F64 && prec != LPM_F32C) return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
+    if (prec != LPM_F64 && prec != LPM_F32C && prec != LPM_F64F)
+        return fail(c, LPM_ERR_INVALID_ARG, "invalid precision");
     if (series != LPM_SERIES_REFERENCE && series != LPM_SERIES_CARTESIAN)
         return fail(c, LPM_ERR_INVALID_ARG, "invalid series %d", series);
     if (series == LPM_SERIES_CARTESIAN && c->tree->moment_sign != 1)

@@ -74,6 +74,7 @@ def test_config3_treecode_l8(ctx):
     T.moments(ctx, g, 8, 1)
     u, mac = T.evaluate(ctx, sub, 0.7, prec=L.LPM_F64, series=L.LPM_SERIES_CARTESIAN)
     u32, mac32 = T.evaluate(ctx, sub, 0.7, prec=L.LPM_F32C, series=L.LPM_SERIES_CARTESIAN)
+    uf, macf = T.evaluate(ctx, sub, 0.7, prec=L.LPM_F64F, series=L.LPM_SERIES_CARTESIAN)
     R = O.tree_build(src, m.max_radius, 128)
     mom = O.tree_moments(src, g, 8, 1)
     ur, macr, mac_per = O.tree_eval(src, g, mom, 8, 0.7, sub, series=1)
@@ -82,8 +83,9 @@ def test_config3_treecode_l8(ctx):
     print(f"\ncubed L8 (nodes {len(R['level'])}): FP64 tree vs oracle tree rel-L2 {e64:.2e} max "
           f"{O.rel_max(u, ur):.2e}, mac {mac} (oracle {macr}); FP32 moments/near vs FP64 tree {e32:.2e}; "
           f"tree (theta 0.7, p 8) vs direct sum {et:.2e}; {time.perf_counter() - t0:.1f} s")
-    assert mac == macr == int(mac_per.sum()) and mac32 == mac
+    assert mac == macr == int(mac_per.sum()) and mac32 == mac == macf
     assert e64 <= 1e-12 and O.rel_max(u, ur) <= 1e-12
+    assert O.rel_l2(uf, ur) <= 1e-12 and O.rel_max(uf, ur) <= 1e-12
     assert e32 <= 1e-5
     assert et < 1e-3
 

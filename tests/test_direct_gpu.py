@@ -1,7 +1,7 @@
 """GPU parity of the direct sum (liblpm_gpu, include/lpm_gpu.h) against the oracle.
 
-Bars: FP64 relative L2 and max error <= 1e-12 against the oracle's
-reference-order FP64 sum; FP32C relative L2 <= 1e-5 (velocity) with the
+Bars: FP64 (LPM_F64 and the fused LPM_F64F) relative L2 and max error
+<= 1e-12 against the oracle's reference-order FP64 sum; FP32C relative L2 <= 1e-5 (velocity) with the
 measured error printed.  Inputs are the mesh particle sets of the workloads
 (icosahedral level 4, cubed-sphere levels 5 and 7) plus ragged, empty,
 smoothed and scaled cases.
@@ -31,9 +31,10 @@ def test_full_mesh_parity(ctx, mesh, seed, level):
     tgt, src, g, skip = _setup(m)
     u_ref = O.bs_direct(tgt, src, g, skip=skip)
     p_ref = O.greens_direct(tgt, src, g, skip=skip)
-    (u64, p64) = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F64, skip=skip)
-    assert O.rel_l2(u64, u_ref) <= 1e-12 and O.rel_max(u64, u_ref) <= 1e-12
-    assert O.rel_l2(p64, p_ref) <= 1e-12 and O.rel_max(p64, p_ref) <= 1e-12
+    for prec in (L.LPM_F64, L.LPM_F64F):
+        (u64, p64) = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=prec, skip=skip)
+        assert O.rel_l2(u64, u_ref) <= 1e-12 and O.rel_max(u64, u_ref) <= 1e-12, prec
+        assert O.rel_l2(p64, p_ref) <= 1e-12 and O.rel_max(p64, p_ref) <= 1e-12, prec
     (u32, p32) = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip)
     e_u, e_p = O.rel_l2(u32, u_ref), O.rel_l2(p32, p_ref)
     print(f"\nseed={seed} level={level} f32c rel-L2 velocity {e_u:.3e} potential {e_p:.3e} "
@@ -51,6 +52,7 @@ def test_cubed_level7_sampled_targets(ctx, mesh):
     m = mesh(L.LPM_SEED_CUBED, 7)
     tgt, src, g, skip = _setup(m)
     u64, p64 = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F64, skip=skip)
+    uf = ctx.direct_sum(tgt, src, g, kernels=VEL, prec=L.LPM_F64F)
     u32, p32 = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip)
     idx = np.sort(np.random.default_rng(11).choice(len(tgt[0]), 2048, replace=False))
     sub = tuple(t[idx] for t in tgt)
@@ -59,6 +61,11 @@ def test_cubed_level7_sampled_targets(ctx, mesh):
     u64s = tuple(u[idx] for u in u64)
     u32s = tuple(u[idx] for u in u32)
     assert O.rel_l2(u64s, u_ref) <= 1e-12 and O.rel_max(u64s, u_ref) <= 1e-12
+    ufs = tuple(u[idx] for u in uf)
+    print(f"\ncubed L7 f64f rel-L2 {O.rel_l2(ufs, u_ref):.3e} max {O.rel_max(ufs, u_ref):.3e}; whole mesh "
+          f"f64f vs f64 rel-L2 {O.rel_l2(uf, u64):.3e}")
+    assert O.rel_l2(ufs, u_ref) <= 1e-12 and O.rel_max(ufs, u_ref) <= 1e-12
+    assert O.rel_l2(uf, u64) <= 1e-13
     assert O.rel_l2(p64[idx], p_ref) <= 1e-12 and O.rel_max(p64[idx], p_ref) <= 1e-12
     e, ep = O.rel_l2(u32s, u_ref), O.rel_l2(p32[idx], p_ref)
     print(f"\ncubed L7 f32c rel-L2 velocity {e:.3e} (max {O.rel_max(u32s, u_ref):.3e}), "
@@ -79,8 +86,9 @@ def test_ragged_sizes(ctx, nt, ns):
     g = rng.normal(size=ns)
     ref = O.bs_direct(tuple(t), tuple(s), g)
     pref = O.greens_direct(tuple(t), tuple(s), g)
-    u, p = ctx.direct_sum(tuple(t), tuple(s), g, kernels=VEL | PSI, prec=L.LPM_F64)
-    assert O.rel_l2(u, ref) <= 1e-12 and O.rel_l2(p, pref) <= 1e-12
+    for prec in (L.LPM_F64, L.LPM_F64F):
+        u, p = ctx.direct_sum(tuple(t), tuple(s), g, kernels=VEL | PSI, prec=prec)
+        assert O.rel_l2(u, ref) <= 1e-12 and O.rel_l2(p, pref) <= 1e-12, prec
     u, p = ctx.direct_sum(tuple(t), tuple(s), g, kernels=VEL | PSI, prec=L.LPM_F32C)
     assert O.rel_l2(u, ref) <= 1e-5 and O.rel_l2(p, pref) <= 1e-5
 
@@ -92,10 +100,30 @@ def test_smoothing_and_radius(ctx, mesh):
         t = tuple(R * a for a in tgt)
         s = tuple(R * a for a in src)
         ref = O.bs_direct(t, s, g, radius=R, delta=delta)
-        u = ctx.direct_sum(t, s, g, prec=L.LPM_F64, radius=R, delta=delta)
-        assert O.rel_l2(u, ref) <= 1e-12, (R, delta)
+        for prec in (L.LPM_F64, L.LPM_F64F):
+            u = ctx.direct_sum(t, s, g, prec=prec, radius=R, delta=delta)
+            assert O.rel_l2(u, ref) <= 1e-12, (R, delta, prec)
         u = ctx.direct_sum(t, s, g, prec=L.LPM_F32C, radius=R, delta=delta)
         assert O.rel_l2(u, ref) <= 1e-5, (R, delta)
+
+
+def test_f64f_off_sphere_and_smoothed(ctx):
+    """LPM_F64F keeps the reference's general formula: points off the sphere
+    (|x| in [0.3, 0.9], so R^2 - x.y >= 0.19) and a smoothing length, against
+    the oracle; and the zero rule with duplicated sources and delta = 0."""
+    rng = np.random.default_rng(21)
+    t = rng.normal(size=(3, 700)); t *= rng.uniform(0.3, 0.9, 700) / np.linalg.norm(t, axis=0)
+    s = rng.normal(size=(3, 1100)); s *= rng.uniform(0.3, 0.9, 1100) / np.linalg.norm(s, axis=0)
+    g = rng.normal(size=1100)
+    for delta in (0.0, 0.1):
+        ref = O.bs_direct(tuple(t), tuple(s), g, delta=delta)
+        u = ctx.direct_sum(tuple(t), tuple(s), g, prec=L.LPM_F64F, delta=delta)
+        assert O.rel_l2(u, ref) <= 1e-12 and O.rel_max(u, ref) <= 1e-12, delta
+    sd = np.concatenate([s, s[:, :50]], axis=1)
+    gd = np.concatenate([g, g[:50]])
+    ref = O.bs_direct(tuple(sd), tuple(sd), gd)
+    u = ctx.direct_sum(tuple(sd), tuple(sd), gd, prec=L.LPM_F64F)
+    assert all(np.isfinite(a).all() for a in u) and O.rel_l2(u, ref) <= 1e-12
 
 
 def test_empty_inputs(ctx):
@@ -113,7 +141,7 @@ def test_coincident_sources_give_zero_velocity(ctx):
     s = rng.normal(size=(3, 100)); s /= np.linalg.norm(s, axis=0)
     g = rng.normal(size=100)
     ref = O.bs_direct(tuple(s), tuple(s), g)
-    for prec, tol in [(L.LPM_F64, 1e-12), (L.LPM_F32C, 1e-5)]:
+    for prec, tol in [(L.LPM_F64, 1e-12), (L.LPM_F64F, 1e-12), (L.LPM_F32C, 1e-5)]:
         u = ctx.direct_sum(tuple(s), tuple(s), g, prec=prec)
         assert all(np.isfinite(a).all() for a in u)
         assert O.rel_l2(u, ref) <= tol
@@ -122,7 +150,7 @@ def test_coincident_sources_give_zero_velocity(ctx):
 def test_deterministic(ctx, mesh):
     m = mesh(L.LPM_SEED_CUBED, 4)
     tgt, src, g, skip = _setup(m)
-    for prec in (L.LPM_F64, L.LPM_F32C):
+    for prec in (L.LPM_F64, L.LPM_F64F, L.LPM_F32C):
         a = ctx.direct_sum(tgt, src, g, prec=prec, kernels=VEL | PSI, skip=skip)
         b = ctx.direct_sum(tgt, src, g, prec=prec, kernels=VEL | PSI, skip=skip)
         assert all(np.array_equal(x, y) for x, y in zip(a[0], b[0])) and np.array_equal(a[1], b[1])

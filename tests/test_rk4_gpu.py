@@ -30,6 +30,14 @@ def test_rk4_matches_oracle(ctx, mesh):
     r = O.rk4_steps(m.cx, m.cy, m.cz, za, m.area, dt, omega, 2)
     d = np.sqrt(sum((a - b) ** 2 for a, b in zip(g, r)))
     assert d.max() < 1e-12
+    # the fused FP64 direct sum (LPM_F64F) to the same bar
+    cfg = L.SolverConfig.default(method=L.LPM_METHOD_DIRECT, prec=L.LPM_F64F)
+    gf = ctx.rk4_steps(cfg, dt, omega, 2, m.cx, m.cy, m.cz, za, m.area)
+    assert np.sqrt(sum((a - b) ** 2 for a, b in zip(gf, r))).max() < 1e-12
+    # LPM_F64F serves the direct sum and the per-target treecode only
+    with pytest.raises(L.LpmError):
+        ctx.rk4_steps(L.SolverConfig.default(method=L.LPM_METHOD_TREE_BATCH, prec=L.LPM_F64F), dt, omega, 1,
+                      m.cx, m.cy, m.cz, za, m.area)
     assert np.abs(np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2) - 1).max() < 1e-15
     moved = np.sqrt((g[0] - m.cx) ** 2 + (g[1] - m.cy) ** 2 + (g[2] - m.cz) ** 2)
     assert moved.max() > 1e-3          # particles did move

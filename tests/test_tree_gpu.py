@@ -93,11 +93,14 @@ def test_eval_parity(ctx, mesh, series, theta, p, sign):
     T.build(ctx, src, m.max_radius, 32)
     T.moments(ctx, g, p, sign)
     u, mac = T.evaluate(ctx, tgt, theta, series=series)
+    uf, macf = T.evaluate(ctx, tgt, theta, series=series, prec=L.LPM_F64F)
     O.tree_build(src, m.max_radius, 32)
     mom = O.tree_moments(src, g, p, sign)
     ur, macr, _ = O.tree_eval(src, g, mom, p, theta, tgt, series=series)
-    assert mac == macr
+    assert mac == macr == macf
     assert O.rel_l2(u, ur) < 1e-12 and O.rel_max(u, ur) < 1e-12
+    # the fused FP64 near field (LPM_F64F) to the same bar
+    assert O.rel_l2(uf, ur) < 1e-12 and O.rel_max(uf, ur) < 1e-12
 
 
 def test_mac_count_with_divergent_reference_series(ctx, mesh):
@@ -120,6 +123,8 @@ def test_theta_zero_is_direct_sum(ctx, mesh):
     u, mac = T.evaluate(ctx, tgt, 0.0)
     ref = O.bs_direct(tgt, src, g)
     assert mac == 0 and O.rel_l2(u, ref) < 1e-13
+    uf, mac = T.evaluate(ctx, tgt, 0.0, prec=L.LPM_F64F)
+    assert mac == 0 and O.rel_l2(uf, ref) < 1e-13 and O.rel_max(uf, ref) < 1e-12
 
 
 def test_cartesian_accuracy_and_f32c(ctx, mesh):
