@@ -359,7 +359,10 @@ int lpm_model_gradient(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n
  * finite). */
 int lpm_model_train(lpm_ctx* ctx, const float* x, const int32_t* y, int64_t n, int epochs, int batch,
                     float lr, uint64_t seed, float* final_loss);
-/* Logits [n][16] and argmax classes [n] (either may be NULL). */
+/* Logits [n][16] (the training forward) and classes [n] (the inference of
+ * the planning pass, which sums the two K = 64 layers in 32-wide blocks:
+ * it may differ from the argmax of the logits where two classes are within
+ * FP32 rounding); either may be NULL. */
 int lpm_model_forward(lpm_ctx* ctx, const float* x, int64_t n, float* logits, int32_t* cls);
 /* Training data from the FP64 treecode: for every batch_stride-th target
  * batch the wave walks the tree; a node passing maxRadius_n + r_b <

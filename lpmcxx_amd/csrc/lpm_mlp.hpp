@@ -1,8 +1,10 @@
 // Order model: a 16 -> 64 -> 64 -> 16 ReLU perceptron (10 features, 13
 // classes) that picks the Taylor order p (class p, 1..12) or rejects the
 // multipole acceptance (class 0) for a (wave of targets, cluster) pair.
-// BF16 weights and activations on the MFMA units (v_mfma_f32_16x16x16_bf16),
-// FP32 accumulation, FP32 biases and master weights.
+// BF16 weights and activations on the MFMA units (v_mfma_f32_16x16x16_bf16
+// for training and the first layer, v_mfma_f32_16x16x32_bf16 for the K = 64
+// layers of the inference in the planning pass), FP32 accumulation, FP32
+// biases and master weights.
 #pragma once
 
 #include "lpm_internal.hpp"
@@ -65,30 +67,53 @@ __device__ inline void wave_sync() {
 // A[m][k]: lane holds A[c][4g + i]; B[k][n]: lane holds B[4g + i][c];
 // D[m][n]: lane holds D[4g + i][c], i = 0..3.
 
-// Weights of the model as MFMA B fragments in LDS: 24 tiles x 64 lanes x 4
-// BF16 (W1: tiles 0..3 = n-tiles, W2: 4 + 4 kt + nt, W3: 20 + kt), then the
-// FP32 biases b1, b2, b3.
-constexpr int kFragTiles = 24;
-constexpr int kFragBytes = kFragTiles * 64 * 8;
+// Weights of the model as MFMA A fragments in LDS (the layers are computed
+// transposed, see mlp_classify_regs): W1 as four 16x16x16 tiles (64 lanes x
+// 4 BF16 each), W2 as eight 16x16x32 tiles (tile 2 nt + kb: output tile nt,
+// K block kb; 64 lanes x 8 BF16), W3 as two 16x16x32 tiles (K block kb),
+// then the FP32 biases b1, b2, b3.
+// A 16x16x32 K block kb holds the 32 features 32 kb .. 32 kb + 31 in the
+// order in which the previous layer's 16x16 results hand them over: lane
+// group g's eight values are features 32 kb + 4 g + i and 32 kb + 16 + 4 g
+// + i (i = 0..3), so its B operand is two result fragments side by side.
+constexpr int kFragW1 = 4 * 64 * 8, kFragW2 = 8 * 64 * 16, kFragW3 = 2 * 64 * 16;
+constexpr int kFragBytes = kFragW1 + kFragW2 + kFragW3;
 constexpr int kModelLds = kFragBytes + (2 * kHid + kOut) * 4;
+
+// feature of K block kb held by lane group g as its element j (16x16x32 operands)
+__host__ __device__ constexpr int kblock_feature(int kb, int g, int j) {
+    return 32 * kb + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
 
 // Fill the LDS model image from the device copy (BF16 weights + FP32
 // biases); all threads of the block take part, the caller synchronizes.
 __device__ inline void mlp_stage_weights(const unsigned short* __restrict__ w, unsigned char* lds) {
-    uint2* frag = reinterpret_cast<uint2*>(lds);
+    uint2* f1 = reinterpret_cast<uint2*>(lds);
+    uint4* f2 = reinterpret_cast<uint4*>(lds + kFragW1);
+    uint4* f3 = reinterpret_cast<uint4*>(lds + kFragW1 + kFragW2);
     const unsigned short* W1 = w;
     const unsigned short* W2 = w + kIn * kHid;
     const unsigned short* W3 = W2 + kHid * kHid;
-    for (int f = threadIdx.x; f < kFragTiles * 64; f += blockDim.x) {
-        const int tile = f >> 6, l = f & 63, g = l >> 4, c = l & 15;
-        unsigned short e[4];
-        for (int i = 0; i < 4; ++i) {
-            const int k = 4 * g + i;
-            if (tile < 4) e[i] = W1[k * kHid + tile * 16 + c];
-            else if (tile < 20) e[i] = W2[(((tile - 4) >> 2) * 16 + k) * kHid + ((tile - 4) & 3) * 16 + c];
-            else e[i] = W3[((tile - 20) * 16 + k) * kOut + c];
+    for (int f = threadIdx.x; f < 14 * 64; f += blockDim.x) {
+        const int l = f & 63, g = l >> 4, c = l & 15;
+        unsigned short e[8];
+        if (f < 4 * 64) {
+            const int nt = f >> 6;
+            for (int i = 0; i < 4; ++i) e[i] = W1[(4 * g + i) * kHid + nt * 16 + c];
+            f1[f] = make_uint2(unsigned(e[0]) | (unsigned(e[1]) << 16), unsigned(e[2]) | (unsigned(e[3]) << 16));
+            continue;
         }
-        frag[f] = make_uint2(unsigned(e[0]) | (unsigned(e[1]) << 16), unsigned(e[2]) | (unsigned(e[3]) << 16));
+        const bool l2 = f < 12 * 64;
+        const int tile = l2 ? (f >> 6) - 4 : (f >> 6) - 12;
+        const int nt = l2 ? tile >> 1 : 0, kb = l2 ? tile & 1 : tile;
+        for (int j = 0; j < 8; ++j) {
+            const int k = kblock_feature(kb, g, j);
+            e[j] = l2 ? W2[k * kHid + nt * 16 + c] : W3[k * kOut + c];
+        }
+        const uint4 v = make_uint4(unsigned(e[0]) | (unsigned(e[1]) << 16), unsigned(e[2]) | (unsigned(e[3]) << 16),
+                                   unsigned(e[4]) | (unsigned(e[5]) << 16), unsigned(e[6]) | (unsigned(e[7]) << 16));
+        if (l2) f2[f - 4 * 64] = v;
+        else f3[f - 12 * 64] = v;
     }
     const float* fb = reinterpret_cast<const float*>(w + kBfWeights);
     float* lb = reinterpret_cast<float*>(lds + kFragBytes);
@@ -120,15 +145,32 @@ __device__ inline bf16x4 cvt_bf16x4(float a, float b, float c, float d) {
     return r;
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline f32x4 mfma32(uint4 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), b, c, 0, 0, 0);
+}
+
+// the B operand of K block kb: result fragments 2 kb and 2 kb + 1 side by side
+__device__ inline bf16x8 join_bf16(bf16x4 lo, bf16x4 hi) {
+    const s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
 // Register-only inference for up to 16 rows.  Every layer is computed
 // transposed, H_out^T = W^T H_in^T, so that the MFMA result layout (lane
 // (g, c) holds row c, columns 4g..4g+3) is directly the operand layout of
-// the next layer and no LDS transposes are needed.  x: this lane's four
-// normalized inputs, row (lane % 16), columns 4g..4g+3 (g = lane / 16), as
-// BF16.  Returns the argmax class of row (lane % 16) in every lane.
+// the next layer and no LDS transposes are needed.  Layer 1 (K = 16) runs on
+// v_mfma_f32_16x16x16_bf16 (4 instructions), layers 2 and 3 (K = 64) on
+// gfx950's v_mfma_f32_16x16x32_bf16 (8 + 2 instructions).  x: this lane's
+// four normalized inputs, row (lane % 16), columns 4g..4g+3 (g = lane / 16),
+// as BF16.  Returns the argmax class of row (lane % 16) in every lane.
 __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
-    const uint2* frag = reinterpret_cast<const uint2*>(model);
+    const uint2* f1 = reinterpret_cast<const uint2*>(model);
+    const uint4* f2 = reinterpret_cast<const uint4*>(model + kFragW1);
+    const uint4* f3 = reinterpret_cast<const uint4*>(model + kFragW1 + kFragW2);
     const float* b1 = reinterpret_cast<const float*>(model + kFragBytes);
     const float* b2 = b1 + kHid;
     const float* b3 = b2 + kHid;
@@ -136,23 +178,24 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(as_bf16x4(frag[nt * 64 + lane]), x, acc);
+        acc = mfma16(as_bf16x4(f1[nt * 64 + lane]), x, acc);
         const float4 b = *reinterpret_cast<const float4*>(b1 + nt * 16 + 4 * g);
         h1[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
                             fmaxf(acc[3] + b.w, 0.f));
     }
+    const bf16x8 k1[2] = {join_bf16(h1[0], h1[1]), join_bf16(h1[2], h1[3])};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(as_bf16x4(frag[(4 + 4 * kt + nt) * 64 + lane]), h1[kt], acc);
+        for (int kb = 0; kb < 2; ++kb) acc = mfma32(f2[(2 * nt + kb) * 64 + lane], k1[kb], acc);
         const float4 b = *reinterpret_cast<const float4*>(b2 + nt * 16 + 4 * g);
         h2[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
                             fmaxf(acc[3] + b.w, 0.f));
     }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma16(as_bf16x4(frag[(20 + kt) * 64 + lane]), h2[kt], acc);
+    acc = mfma32(f3[lane], join_bf16(h2[0], h2[1]), acc);
+    acc = mfma32(f3[64 + lane], join_bf16(h2[2], h2[3]), acc);
     const float4 b = *reinterpret_cast<const float4*>(b3 + 4 * g);
     const float z[4] = {acc[0] + b.x, acc[1] + b.y, acc[2] + b.z, acc[3] + b.w};
     float v = -3.0e38f;

@@ -274,6 +274,24 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
     else fb[2 * kHid + (p - kOffB3)] = w;
 }
 
+// Classes of n (normalized) rows by mlp_classify_regs, the inference of the
+// planning pass: 16 rows per wave, lane (g, r) holding inputs 4g..4g+3 of
+// row r.
+__global__ __launch_bounds__(64) void mlp_classify_kernel(const float* __restrict__ X, int64_t n,
+                                                          const unsigned short* __restrict__ w,
+                                                          int* __restrict__ cls) {
+    __shared__ __align__(16) unsigned char s_model[kModelLds];
+    mlp_stage_weights(w, s_model);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t row = int64_t(blockIdx.x) * 16 + r;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (row < n)
+        for (int i = 0; i < 4; ++i) x[i] = X[row * kIn + 4 * g + i];
+    const int q = mlp_classify_regs(s_model, cvt_bf16x4(x[0], x[1], x[2], x[3]));
+    if (lane < 16 && row < n) cls[row] = q;
+}
+
 __global__ __launch_bounds__(64) void mlp_forward_kernel(const float* __restrict__ X, int64_t n,
                                                          const unsigned short* __restrict__ w,
                                                          float* __restrict__ logits, int* __restrict__ cls) {
@@ -538,7 +556,10 @@ int lpm_model_forward(lpm_ctx* c, const float* x, int64_t n, float* logits, int3
     float* dl = c->h_out.as<float>();
     int* dc = reinterpret_cast<int*>(dl + size_t(n) * kOut);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(unsigned((n + kRows - 1) / kRows)), dim3(64), kLds, c->stream,
-                       M.data.as<float>(), n, M.wbf.as<unsigned short>(), dl, dc);
+                       M.data.as<float>(), n, M.wbf.as<unsigned short>(), dl, nullptr);
+    // the classes come from the planning pass's own inference
+    hipLaunchKernelGGL(mlp_classify_kernel, dim3(unsigned((n + 15) / 16)), dim3(64), 0, c->stream,
+                       M.data.as<float>(), n, M.wbf.as<unsigned short>(), dc);
     if (logits)
         LPM_HIP(c, hipMemcpyAsync(logits, dl, size_t(n) * kOut * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (cls) LPM_HIP(c, hipMemcpyAsync(cls, dc, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -100,78 +100,109 @@ __device__ inline void child_interval(const uint64_t* keys, int b, int e, int le
 }
 
 // Children count of every node of `level` (first node and count from the
-// level table), 8 lanes per node; cnt[t] = 0 for t in [count, ub) so a scan
-// over ub entries is exact.  Leaves are counted with their moment chunks.
+// level table), 8 lanes per node, 32 nodes per block: cnt[t] for t < count
+// and blk[b] = the children of block b's nodes (so the emit pass finds its
+// offsets without a separate scan).  Leaves are counted with their moment
+// chunks.
 // control 0 (MAX_COORDS_PER_NODE): split iff count > param
 // control 1 (MAX_DEPTH):           split iff level < param (src/LpmOctree.cpp:145-180)
 __global__ __launch_bounds__(256) void tree_count(const uint64_t* __restrict__ keys, int level, int sorted_levels,
-                                                  int control, int param, int ub, int* __restrict__ lvt,
+                                                  int control, int param, int* __restrict__ lvt,
                                                   const int* __restrict__ nbeg, const int* __restrict__ nend,
-                                                  int* __restrict__ cnt, int* __restrict__ nfirst,
-                                                  int* __restrict__ nkids) {
+                                                  int* __restrict__ cnt, int* __restrict__ blk,
+                                                  int* __restrict__ nfirst, int* __restrict__ nkids) {
+    __shared__ int s_w[4];
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
     const int j = threadIdx.x & 7;
-    if (t >= ub) return;  // whole lane groups leave together
     const int start = lvt[2 * level], count = lvt[2 * level + 1];
-    if (t >= count) {
-        if (j == 0) cnt[t] = 0;
-        return;
-    }
-    const int i = start + t;
-    const int b = nbeg[i], e = nend[i];
-    const bool want = (control == 0) ? (e - b > param) : (level < param && e > b);
-    bool search = false;
-    if (want) {
-        if (level >= kMaxDepth) {
-            if (j == 0) atomicOr(lvt + kLvTooDeep, 1);         // cannot split further
-        } else if (level >= sorted_levels) {
-            if (j == 0) atomicOr(lvt + kLvKeys, 1);            // keys sorted too coarsely
-        } else {
-            search = true;
+    int c = 0;
+    if (t < count) {  // whole lane groups: the group's shuffles see all eight lanes
+        const int i = start + t;
+        const int b = nbeg[i], e = nend[i];
+        const bool want = (control == 0) ? (e - b > param) : (level < param && e > b);
+        bool search = false;
+        if (want) {
+            if (level >= kMaxDepth) {
+                if (j == 0) atomicOr(lvt + kLvTooDeep, 1);         // cannot split further
+            } else if (level >= sorted_levels) {
+                if (j == 0) atomicOr(lvt + kLvKeys, 1);            // keys sorted too coarsely
+            } else {
+                search = true;
+            }
+        }
+        int lo, hi;
+        child_interval(keys, b, e, level, search, lo, hi);
+        const unsigned grp = unsigned(__ballot(search && hi > lo) >> (threadIdx.x & 56)) & 0xffu;
+        c = __popc(grp);
+        if (j == 0) {
+            cnt[t] = c;
+            nfirst[i] = -1;
+            nkids[i] = 0;
+        }
+        // leaves and their moment chunks, counted over the wave's active lanes by
+        // ballots before one atomic each (the deep levels have thousands of
+        // leaves on the same two words); a leaf of several chunks (leaf_max >
+        // kChunk) adds the chunks beyond its first itself
+        const bool is_leaf = j == 0 && c == 0;
+        const uint64_t lm = __ballot(is_leaf);
+        if (lm != 0) {
+            const int nch = is_leaf ? (e - b + kChunk - 1) / kChunk : 0;
+            const uint64_t cm = __ballot(nch > 0);
+            const int slot = blockIdx.x % kLvSpread;
+            if (nch > 1) atomicAdd(lvt + kLvChunks + slot, nch - 1);
+            if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(true))) {
+                atomicAdd(lvt + kLvLeaves + slot, __popcll(lm));
+                atomicAdd(lvt + kLvChunks + slot, __popcll(cm));
+            }
         }
     }
-    int lo, hi;
-    child_interval(keys, b, e, level, search, lo, hi);
-    const unsigned grp = unsigned(__ballot(search && hi > lo) >> (threadIdx.x & 56)) & 0xffu;
-    const int c = __popc(grp);
-    if (j == 0) {
-        cnt[t] = c;
-        nfirst[i] = -1;
-        nkids[i] = 0;
-    }
-    // leaves and their moment chunks, counted over the wave's active lanes by
-    // ballots before one atomic each (the deep levels have thousands of
-    // leaves on the same two words); a leaf of several chunks (leaf_max >
-    // kChunk) adds the chunks beyond its first itself
-    const bool is_leaf = j == 0 && c == 0;
-    const uint64_t lm = __ballot(is_leaf);
-    if (lm == 0) return;
-    const int nch = is_leaf ? (e - b + kChunk - 1) / kChunk : 0;
-    const uint64_t cm = __ballot(nch > 0);
-    const int slot = blockIdx.x % kLvSpread;
-    if (nch > 1) atomicAdd(lvt + kLvChunks + slot, nch - 1);
-    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(true))) {
-        atomicAdd(lvt + kLvLeaves + slot, __popcll(lm));
-        atomicAdd(lvt + kLvChunks + slot, __popcll(cm));
-    }
+    // children of the block's 32 nodes (lane 8k holds node k's count)
+    int v = (j == 0) ? c : 0;
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// Children of the nodes of `level` at first + count + off[t] (off = exclusive
-// scan of cnt), and the next level's entry of the level table; 8 lanes per
-// node, lane j writes child j.
+// Children of the nodes of `level`, written compactly at first + count +
+// (children of the earlier nodes), and the next level's entry of the level
+// table; 8 lanes per node, lane j writes child j.  The offset of a block's
+// first node is the sum of the earlier blocks' blk words.
 __global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ keys, int level, int* lvt,
-                                                 const int* __restrict__ cnt, const int* __restrict__ off, int* nbeg,
-                                                 int* nend, int* nfirst, int* nkids, int* nlevel, int* nparent,
-                                                 uint64_t* nprefix, int ub) {
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-    const int j = threadIdx.x & 7;
+                                                 const int* __restrict__ cnt, const int* __restrict__ blk,
+                                                 int* nbeg, int* nend, int* nfirst, int* nkids, int* nlevel,
+                                                 int* nparent, uint64_t* nprefix) {
+    __shared__ int s_w[4];
+    __shared__ int s_c[32];
     const int start = lvt[2 * level], count = lvt[2 * level + 1];
-    if (t == 0 && j == 0) {
-        lvt[2 * level + 2] = start + count;
-        lvt[2 * level + 3] = count > 0 ? off[count - 1] + cnt[count - 1] : 0;
+    if (count == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            lvt[2 * level + 2] = start;
+            lvt[2 * level + 3] = 0;
+        }
+        return;
     }
-    if (t >= ub) return;  // whole lane groups leave together
-    const bool split = t < count && cnt[t] > 0;
+    if (int(blockIdx.x) * 32 >= count) return;  // the whole block: no node of this level
+    // children of the earlier blocks
+    int a = 0;
+    for (int k = threadIdx.x; k < int(blockIdx.x); k += blockDim.x) a += blk[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int j = threadIdx.x & 7, slot = threadIdx.x >> 3;
+    const int c = t < count ? cnt[t] : 0;
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = a;
+    if (j == 0) s_c[slot] = c;
+    __syncthreads();
+    int off = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    for (int k = 0; k < slot; ++k) off += s_c[k];
+    if (t == count - 1 && j == 0) {
+        lvt[2 * level + 2] = start + count;
+        lvt[2 * level + 3] = off + c;
+    }
+    const bool split = t < count && c > 0;
     const int i = start + t;
     const int b = split ? nbeg[i] : 0, e = split ? nend[i] : 0;
     int lo, hi;
@@ -179,10 +210,10 @@ __global__ __launch_bounds__(256) void tree_emit(const uint64_t* __restrict__ ke
     const bool mine = split && hi > lo;
     const unsigned grp = unsigned(__ballot(mine) >> (threadIdx.x & 56)) & 0xffu;
     if (!split) return;
-    const int first = start + count + off[t];
+    const int first = start + count + off;
     if (j == 0) {
         nfirst[i] = first;
-        nkids[i] = cnt[t];
+        nkids[i] = c;
     }
     if (!mine) return;
     const int k = first + __popc(grp & ((1u << j) - 1u));
@@ -288,51 +319,6 @@ __global__ __launch_bounds__(256) void iota_i32(int* v, int64_t n) {
     if (i < n) v[i] = int(i);
 }
 
-// Source radius of every leaf about its box centroid: the largest distance
-// of its sources (one block per node; internal nodes are set by
-// tree_radius_up).
-__global__ __launch_bounds__(256) void tree_radius_leaf(int nn, const int* __restrict__ nkids,
-                                                        const int* __restrict__ nbeg, const int* __restrict__ nend,
-                                                        const double4* __restrict__ ncent,
-                                                        const double4* __restrict__ spos, double* __restrict__ nrad) {
-    __shared__ double red[256];
-    const int i = blockIdx.x;
-    if (i >= nn || nkids[i] != 0) return;
-    const double4 c = ncent[i];
-    double r2 = 0.0;
-    for (int q = nbeg[i] + threadIdx.x; q < nend[i]; q += blockDim.x) {
-        const double4 s = spos[q];
-        const double dx = s.x - c.x, dy = s.y - c.y, dz = s.z - c.z;
-        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
-    }
-    red[threadIdx.x] = r2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) nrad[i] = sqrt(red[0]);
-}
-
-// Internal nodes of one level: r = max over children of |c_child - c| + r_child.
-__global__ __launch_bounds__(256) void tree_radius_up(int first, int count, const int* __restrict__ nfirst,
-                                                      const int* __restrict__ nkids,
-                                                      const double4* __restrict__ ncent, double* __restrict__ nrad) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int i = first + t;
-    const int nk = nkids[i];
-    if (nk == 0) return;
-    const double4 c = ncent[i];
-    double r = 0.0;
-    for (int k = nfirst[i]; k < nfirst[i] + nk; ++k) {
-        const double4 ck = ncent[k];
-        const double dx = ck.x - c.x, dy = ck.y - c.y, dz = ck.z - c.z;
-        r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz) + nrad[k]);
-    }
-    nrad[i] = r;
-}
-
 __global__ __launch_bounds__(256) void gather_pos(const double* __restrict__ x, const double* __restrict__ y,
                                                   const double* __restrict__ z, const int* __restrict__ perm,
                                                   int64_t n, double4* __restrict__ out) {
@@ -384,7 +370,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
 
 static_assert(kLvTotal <= kPinWords, "the build table is read through the pinned words");
 constexpr int kFastLevels = 10;  // key levels sorted before the first level loop (30 bits: 4 radix passes)
-constexpr int kLevelGroup = 4;   // levels built between two reads of the build table
+constexpr int64_t kBuildNodeBudget = int64_t(16) << 20;  // node bound of the levels built between two table reads
 
 }  // namespace
 
@@ -423,11 +409,8 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     LPM_CHECK(t->tmp2.reserve(nalloc * sizeof(uint64_t)));
     LPM_CHECK(t->tmp3.reserve(size_t(2 * nalloc + 2) * sizeof(int)));  // children counts | their scan
     LPM_CHECK(t->lvt.reserve(kLvTotal * sizeof(int)));
-    size_t scan_bytes = 0;
     int* cnt = t->tmp3.as<int>();
     int* off = cnt + nalloc + 1;
-    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, int(nalloc), st));
-    LPM_CHECK(t->scan_tmp.reserve(scan_bytes + 256));
     int* lvt = t->lvt.as<int>();
     LPM_CHECK(grow_nodes(c, t, std::max<int64_t>(1024, 4 * (n / std::max(1, leaf_max)) + 64)));
     LPM_HIP(c, hipMemsetAsync(lvt, 0, kLvTotal * sizeof(int), st));
@@ -435,46 +418,56 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         hipLaunchKernelGGL(tree_keys, dim3(blocks(n)), dim3(256), 0, st, x, y, z, n, blo[0], bhi[0], blo[1], bhi[1],
                            blo[2], bhi[2], t->tmp0.as<uint64_t>(), t->tmp1.as<int>(), lvt);
 
-    // Level by level: count children, scan, emit -- kLevelGroup levels between
-    // reads of the build table.  The keys are first sorted on their top
-    // kFastLevels levels only; a node below them that must split sends the
-    // build round again with fully sorted keys.
+    // Level by level: count children, then emit them (offsets from the
+    // per-block child counts) -- every level up to the sorted key levels in
+    // one run of launches, grids sized by the node bound of each level, the
+    // build table read once at the end.  The keys are first sorted on their
+    // top kFastLevels levels only; a node below them that must split sends
+    // the build round again with fully sorted keys.  A run whose node bound
+    // would pass kBuildNodeBudget stops at that level and reads the table
+    // before going on.
     const int64_t split_bound = (control == 0) ? std::min<int64_t>(n, 8 * (n / (int64_t(leaf_max) + 1))) : n;
-    auto bound_after = [&](int64_t known, int steps) {  // node count bound `steps` levels below a known level
-        int64_t b = known;
-        for (int s2 = 0; s2 < steps; ++s2) b = std::min<int64_t>(split_bound, 8 * b);
-        return b;
-    };
     int* h = c->pin;
     int sorted_levels = std::min(kFastLevels, kMaxDepth);
     int64_t nn = 1;
+    int* blk = off;  // per-block child counts (at most one word per 32 nodes)
     for (;;) {
         if (n > 0)
             LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->keys.as<uint64_t>(), t->tmp1.as<int>(),
                                  t->perm.as<int>(), n, 3 * (kMaxDepth - sorted_levels)));
         hipLaunchKernelGGL(tree_root, dim3(1), dim3(1), 0, st, int(n), t->nbeg.as<int>(), t->nend.as<int>(),
                            t->nlevel.as<int>(), t->nparent.as<int>(), t->nprefix.as<uint64_t>(), lvt);
+        // deepest level that may hold nodes to split
+        const int last = std::min(sorted_levels, control == 1 ? std::max(0, std::min(param, kMaxDepth)) : kMaxDepth);
         int lk = 0;
         int64_t ck = 1;
         nn = 1;
         bool resort = false;
         for (;;) {
-            const int G = std::min(kLevelGroup, kMaxDepth + 1 - lk);
-            int64_t need = nn;
-            for (int j = 1; j <= G; ++j) need += bound_after(ck, j);
+            // levels lk .. lk + G - 1 in this run; need = the nodes so far plus
+            // the bound of each level's children
+            int G = 0;
+            int64_t need = nn, b = ck;
+            while (lk + G <= last) {
+                const int64_t nb = std::min<int64_t>(split_bound, 8 * b);
+                if (G > 0 && need + nb > kBuildNodeBudget) break;
+                need += nb;
+                b = nb;
+                ++G;
+            }
             LPM_CHECK(grow_nodes(c, t, need));
-            for (int j = 0; j < G; ++j) {
-                const int level = lk + j;
-                const int ub = int(std::max<int64_t>(1, j == 0 ? ck : bound_after(ck, j)));
-                hipLaunchKernelGGL(tree_count, dim3(blocks(int64_t(ub) * 8)), dim3(256), 0, st, t->keys.as<uint64_t>(), level,
-                                   sorted_levels, control, param, ub, lvt, t->nbeg.as<int>(), t->nend.as<int>(), cnt,
+            int64_t ub = ck;
+            for (int g2 = 0; g2 < G; ++g2) {
+                const int level = lk + g2;
+                const unsigned nbk = blocks(std::max<int64_t>(1, ub) * 8);
+                hipLaunchKernelGGL(tree_count, dim3(nbk), dim3(256), 0, st, t->keys.as<uint64_t>(), level,
+                                   sorted_levels, control, param, lvt, t->nbeg.as<int>(), t->nend.as<int>(), cnt, blk,
                                    t->nfirst.as<int>(), t->nkids.as<int>());
-                size_t sb = scan_bytes;
-                LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, cnt, off, ub, st));
-                hipLaunchKernelGGL(tree_emit, dim3(blocks(int64_t(ub) * 8)), dim3(256), 0, st, t->keys.as<uint64_t>(),
-                                   level, lvt, cnt, off, t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(),
+                hipLaunchKernelGGL(tree_emit, dim3(nbk), dim3(256), 0, st, t->keys.as<uint64_t>(), level, lvt, cnt,
+                                   blk, t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(),
                                    t->nkids.as<int>(), t->nlevel.as<int>(), t->nparent.as<int>(),
-                                   t->nprefix.as<uint64_t>(), ub);
+                                   t->nprefix.as<uint64_t>());
+                ub = std::min<int64_t>(split_bound, 8 * ub);
             }
             LPM_HIP(c, hipGetLastError());
             LPM_HIP(c, hipMemcpyAsync(h, lvt, kLvTotal * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -485,7 +478,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             lk += G;
             ck = h[2 * lk + 1];
             nn = int64_t(h[2 * lk]) + ck;
-            if (ck == 0 || lk > kMaxDepth) break;
+            if (ck == 0 || lk > last) break;
         }
         if (!resort) break;
         sorted_levels = kMaxDepth;
@@ -534,29 +527,25 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
                            t->spos.as<double4>());
     }
+    // node source radii: leaves with the strengths, internal nodes in the
+    // upward pass of the moments
     LPM_CHECK(t->nrad.reserve(size_t(nn) * sizeof(double)));
-    LPM_HIP(c, hipMemsetAsync(t->nrad.ptr, 0, size_t(nn) * sizeof(double), st));
-    hipLaunchKernelGGL(tree_radius_leaf, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
-                       t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
-                       t->nrad.as<double>());
-    for (int l = depth - 1; l >= 0; --l) {
-        const int64_t f = t->level_start[l], lc = t->level_start[l + 1] - f;
-        hipLaunchKernelGGL(tree_radius_up, dim3(blocks(lc)), dim3(256), 0, st, int(f), int(lc),
-                           t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), t->nrad.as<double>());
-    }
-    // leaf chunk list of the moment kernel
+    // leaf chunk list of the moment kernel, needed only when a leaf holds
+    // more than kChunk particles (else the kernel takes one block per node)
     const int64_t nch = t->nch;
-    LPM_CHECK(t->chunks.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
-    int* ch = t->chunks.as<int>();
-    int *cn = ch, *cb = ch + nch, *ce = ch + 2 * nch, *n0 = ch + 3 * nch, *nc = n0 + nn;
-    hipLaunchKernelGGL(chunk_count, dim3(blocks(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(), t->nbeg.as<int>(),
-                       t->nend.as<int>(), nc);
-    size_t sb = 0;
-    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, nc, n0, int(nn), st));
-    LPM_CHECK(t->scan_tmp.reserve(sb + 256));  // the level scans have finished (table read above)
-    LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, nc, n0, int(nn), st));
-    hipLaunchKernelGGL(chunk_fill, dim3(blocks(nn)), dim3(256), 0, st, int(nn), n0, nc, t->nbeg.as<int>(),
-                       t->nend.as<int>(), cn, cb, ce);
+    if (nch != t->nleaves) {
+        LPM_CHECK(t->chunks.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
+        int* ch = t->chunks.as<int>();
+        int *cn = ch, *cb = ch + nch, *ce = ch + 2 * nch, *n0 = ch + 3 * nch, *nc = n0 + nn;
+        hipLaunchKernelGGL(chunk_count, dim3(blocks(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
+                           t->nbeg.as<int>(), t->nend.as<int>(), nc);
+        size_t sb = 0;
+        LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, nc, n0, int(nn), st));
+        LPM_CHECK(t->scan_tmp.reserve(sb + 256));
+        LPM_HIP(c, hipcub::DeviceScan::ExclusiveSum(t->scan_tmp.ptr, sb, nc, n0, int(nn), st));
+        hipLaunchKernelGGL(chunk_fill, dim3(blocks(nn)), dim3(256), 0, st, int(nn), n0, nc, t->nbeg.as<int>(),
+                           t->nend.as<int>(), cn, cb, ce);
+    }
     LPM_HIP(c, hipGetLastError());
     if (n > 0) {
         t->sx = x;

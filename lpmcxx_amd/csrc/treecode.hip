@@ -93,26 +93,41 @@ __device__ inline void store_moment(const MomOut& o, int64_t node, int p, int nt
 }
 
 constexpr int kMomThreads = 256;
+// LDS row of the leaf moment kernel: a chunk's particles plus two pad
+// doubles, so that rows read at the same particle by different lanes start
+// four banks apart
+constexpr int kMomRow = kChunk + 2;
 
-// P2M of one leaf chunk: particle powers h^a staged in LDS, then G groups of
-// nterm threads (G = 256 / nterm, at least 1) each sum the terms over a
-// contiguous part of the chunk; the group partials are added in group order.
+// P2M of one leaf chunk, order P known at compile time: the particles'
+// powers h^a (a < P, per axis) and gamma y (or gamma) staged in LDS as rows
+// over the chunk's particles, then G groups of nterm threads (G = 256 /
+// nterm, at least 1) each sum their term over a contiguous part of the
+// chunk, four particles per step (ds_read2 pairs of consecutive particles);
+// the group partials are added in group order.
 // direct: every leaf is one chunk, so the sum is the leaf's moments (all three
 // forms are stored); otherwise the chunk partial goes to `part`.
-template <int NC>
+template <int NC, int P>
 __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __restrict__ chunk_node,
                                                                   const int* __restrict__ chunk_beg,
                                                                   const int* __restrict__ chunk_end,
+                                                                  const int* __restrict__ nkids,
+                                                                  const int* __restrict__ nbeg,
+                                                                  const int* __restrict__ nend,
                                                                   const double4* __restrict__ ncent,
-                                                                  const double4* __restrict__ spos, int p, int nterm,
-                                                                  double sign, double* __restrict__ part, MomOut out,
+                                                                  const double4* __restrict__ spos, double sign,
+                                                                  double* __restrict__ part, MomOut out,
                                                                   int direct) {
+    constexpr int nterm = P * (P + 1) * (P + 2) / 6;
+    constexpr int G = kMomThreads / nterm > 1 ? kMomThreads / nterm : 1;
     extern __shared__ double lds[];
-    double* pw = lds;                       // [kChunk][3][p] powers
-    double* gy = lds + kChunk * 3 * p;      // [kChunk][NC]  gamma * y, or gamma
-    double* red = gy + kChunk * NC;         // [G][nterm][NC] group partials (G > 1)
+    double* pw = lds;                       // [3][P][kMomRow] powers
+    double* gy = lds + 3 * P * kMomRow;     // [NC][kMomRow]   gamma * y, or gamma
+    double* red = gy + NC * kMomRow;        // [G][nterm][NC]  group partials (G > 1)
+    // chunk_node == nullptr: one block per node, every leaf one chunk
     const int ch = blockIdx.x;
-    const int node = chunk_node[ch], b = chunk_beg[ch], e = chunk_end[ch];
+    if (!chunk_node && nkids[ch] != 0) return;  // the whole block
+    const int node = chunk_node ? chunk_node[ch] : ch;
+    const int b = chunk_node ? chunk_beg[ch] : nbeg[ch], e = chunk_node ? chunk_end[ch] : nend[ch];
     const int cnt = e - b;
     const double4 c = ncent[node];
     for (int q = threadIdx.x; q < kChunk; q += blockDim.x) {
@@ -122,34 +137,47 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
             yx = s.x; yy = s.y; yz = s.z; g = s.w;
             hx = sign * (s.x - c.x); hy = sign * (s.y - c.y); hz = sign * (s.z - c.z);
         }
-        double* w = pw + q * 3 * p;
         double ax = 1.0, ay = 1.0, az = 1.0;
-        for (int a = 0; a < p; ++a) {
-            w[a] = ax; w[p + a] = ay; w[2 * p + a] = az;
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            pw[a * kMomRow + q] = ax;
+            pw[(P + a) * kMomRow + q] = ay;
+            pw[(2 * P + a) * kMomRow + q] = az;
             ax *= hx; ay *= hy; az *= hz;
         }
         if (NC == 3) {
-            gy[3 * q + 0] = g * yx;
-            gy[3 * q + 1] = g * yy;
-            gy[3 * q + 2] = g * yz;
+            gy[q] = g * yx;
+            gy[kMomRow + q] = g * yy;
+            gy[2 * kMomRow + q] = g * yz;
         } else {
             gy[q] = g;
         }
     }
     __syncthreads();
-    const int G = max(1, kMomThreads / nterm);
     const double irho = 1.0 / c.w;
     for (int it = threadIdx.x; it < G * nterm; it += blockDim.x) {
         const int grp = it / nterm, t = it - grp * nterm;
         int k1, k2, k3;
-        lex_decode(p, t, k1, k2, k3);
+        lex_decode(P, t, k1, k2, k3);
         const int q0 = grp * cnt / G, q1 = (grp + 1) * cnt / G;
+        const double* wx = pw + k1 * kMomRow + q0;
+        const double* wy = pw + (P + k2) * kMomRow + q0;
+        const double* wz = pw + (2 * P + k3) * kMomRow + q0;
+        const double* gq = gy + q0;
         double m[NC] = {};
-        for (int q = q0; q < q1; ++q) {
-            const double* w = pw + q * 3 * p;
-            const double P = w[k1] * w[p + k2] * w[2 * p + k3];
+        int n = q1 - q0;
+        for (; n >= 4; n -= 4, wx += 4, wy += 4, wz += 4, gq += 4) {
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc) m[cc] += gy[NC * q + cc] * P;
+            for (int u = 0; u < 4; ++u) {
+                const double pv = wx[u] * wy[u] * wz[u];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) m[cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[cc]);
+            }
+        }
+        for (int u = 0; u < n; ++u) {
+            const double pv = wx[u] * wy[u] * wz[u];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) m[cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[cc]);
         }
         if (G > 1) {
 #pragma unroll
@@ -157,7 +185,7 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
         } else {
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
-                if (direct) store_moment<NC>(out, node, p, nterm, t, cc, m[cc], irho);
+                if (direct) store_moment<NC>(out, node, P, nterm, t, cc, m[cc], irho);
                 else part[(int64_t(ch) * nterm + t) * NC + cc] = m[cc];
             }
         }
@@ -167,9 +195,21 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
     for (int it = threadIdx.x; it < nterm * NC; it += blockDim.x) {
         double v = red[it];
         for (int grp = 1; grp < G; ++grp) v += red[grp * nterm * NC + it];
-        if (direct) store_moment<NC>(out, node, p, nterm, it / NC, it % NC, v, irho);
+        if (direct) store_moment<NC>(out, node, P, nterm, it / NC, it % NC, v, irho);
         else part[int64_t(ch) * nterm * NC + it] = v;
     }
+}
+
+// nch blocks: one per chunk of the list cn/cb/ce, or (cn == nullptr) one per node
+template <int NC, int P>
+void launch_moments_chunk(hipStream_t st, int64_t nch, const int* cn, const int* cb, const int* ce, const int* nkids,
+                          const int* nbeg, const int* nend, const double4* ncent, const double4* spos, double sign,
+                          double* part, MomOut out, int direct) {
+    constexpr int nterm = P * (P + 1) * (P + 2) / 6;
+    constexpr int G = kMomThreads / nterm > 1 ? kMomThreads / nterm : 1;
+    const size_t lds = (size_t(kMomRow) * (3 * P + NC) + (G > 1 ? size_t(G) * nterm * NC : 0)) * sizeof(double);
+    hipLaunchKernelGGL((tree_moments_chunk<NC, P>), dim3(unsigned(nch)), dim3(kMomThreads), lds, st, cn, cb, ce,
+                       nkids, nbeg, nend, ncent, spos, sign, part, out, direct);
 }
 
 // chunk partials of every leaf summed in chunk order (leaves of more than one
@@ -191,51 +231,87 @@ __global__ __launch_bounds__(256) void tree_moments_reduce(int nn, int p, int nt
     store_moment<NC>(out, node, p, nterm, rem / NC, rem % NC, s, 1.0 / ncent[node].w);
 }
 
-struct BinomTab {
-    double v[kMaxOrder][kMaxOrder];
-    constexpr BinomTab() : v() {
-        for (int n = 0; n < kMaxOrder; ++n) {
-            v[n][0] = 1.0;
-            for (int k = 1; k <= n; ++k) v[n][k] = v[n - 1][k - 1] + (k < n ? v[n - 1][k] : 0.0);
-        }
-    }
-};
-__constant__ BinomTab c_binom{};
-
-__host__ __device__ constexpr int tet(int m) { return m * (m + 1) * (m + 2) / 6; }
-
-// lex position of (a, b, c) among |k| < p
-__device__ inline int lex_pos(int p, int a, int b, int c) {
-    return tet(p) - tet(p - a) + b * (p - a) - b * (b - 1) / 2 + c;
-}
-
 // Upward pass for the internal nodes of one level (moments about the box
 // centroid, h = sign (y - c)): with d = sign (c_child - c),
 //   m_k(node) = sum_children sum_{j <= k} C(k, j) d^(k - j) m_j(child),
 // the same sums as the particle-by-particle definition of the leaves,
 // regrouped.  Since (h + d)^k factorizes over the axes, the translation is
-// applied as three one-dimensional binomial passes (x, then y, then z) on the
-// children's moments staged in LDS: O(p) work per term instead of O(p^3).
-// One block per node; up to `gmax` children are translated at once (one
-// thread per (child, term)), then added to the node's sum in child order.
-template <int NC>
-__global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, double sign, int gmax,
-                                                const int* __restrict__ nfirst, const int* __restrict__ nkids,
-                                                const double4* __restrict__ ncent, MomOut out) {
-    extern __shared__ double lds[];
-    const int S = NC * nterm;
-    double* acc = lds;                 // [S]
-    double* A = acc + S;               // [gmax][S]
-    double* B = A + gmax * S;          // [gmax][S]
-    double* dd = B + gmax * S;         // [gmax][3] child offsets
+// applied as three one-dimensional shifts (x, then y, then z).  Along one
+// axis the terms with the other two exponents fixed form a line m[0..L-1]
+// (L <= P), and m'[e] = sum_{a <= e} C(e, a) d^(e - a) m[a] is L - 1 sweeps
+// of m[e] += d m[e - 1] (e from the top down), Pascal's rule, done in
+// registers.  One block per node: up to G children staged in LDS at once,
+// one thread per (child, line, component), the children then added to the
+// node's sum in child order.  The node's source radius (max over children
+// of |c_child - c| + r_child) is set on the way.
+template <int P>
+__device__ __forceinline__ int lex_at(int a, int b, int c) {  // lex position of (a, b, c), |k| < P
+    return P * (P + 1) * (P + 2) / 6 - (P - a) * (P - a + 1) * (P - a + 2) / 6 + b * (P - a) - b * (b - 1) / 2 + c;
+}
+
+// line l (l < P (P + 1) / 2) of the pairs (u, v) with u + v < P, u major
+template <int P>
+__device__ __forceinline__ void line_pair(int l, int& u, int& v) {
+    u = 0;
+    while (l >= P - u) { l -= P - u; ++u; }
+    v = l;
+}
+
+// children staged per round of tree_m2m: G S doubles plus the node's sum within 64 KB
+template <int NC, int P>
+constexpr int m2m_group() {
+    constexpr int S = NC * P * (P + 1) * (P + 2) / 6;
+    constexpr int g = (8192 - S - 32) / S;
+    return g > 8 ? 8 : (g < 1 ? 1 : g);
+}
+
+template <int NC, int P, int AXIS>
+__device__ __forceinline__ void shift_lines(double* A, const double* dd, int ng) {
+    constexpr int S = NC * P * (P + 1) * (P + 2) / 6;
+    constexpr int NL = P * (P + 1) / 2;
+    for (int task = threadIdx.x; task < ng * NL * NC; task += blockDim.x) {
+        const int k = task / (NL * NC), rem = task - k * NL * NC;
+        const int l = rem / NC, cc = rem - l * NC;
+        int u, v;
+        line_pair<P>(l, u, v);
+        const int L = P - u - v;
+        const double d = dd[3 * k + AXIS];
+        double* buf = A + k * S;
+        // position of exponent e along AXIS (the other two are u, v in axis order)
+        auto pos = [&](int e) {
+            return AXIS == 0 ? lex_at<P>(e, u, v) : AXIS == 1 ? lex_at<P>(u, e, v) : lex_at<P>(u, v, e);
+        };
+        double m[P];
+#pragma unroll
+        for (int e = 0; e < P; ++e) m[e] = e < L ? buf[NC * pos(e) + cc] : 0.0;
+#pragma unroll
+        for (int r = 1; r < P; ++r)
+#pragma unroll
+            for (int e = P - 1; e >= r; --e)
+                if (e < L) m[e] = __builtin_fma(d, m[e - 1], m[e]);
+#pragma unroll
+        for (int e = 0; e < P; ++e)
+            if (e < L) buf[NC * pos(e) + cc] = m[e];
+    }
+}
+
+template <int NC, int P>
+__global__ __launch_bounds__(256) void tree_m2m(int first, double sign, const int* __restrict__ nfirst,
+                                                const int* __restrict__ nkids, const double4* __restrict__ ncent,
+                                                double* __restrict__ nrad, MomOut out) {
+    constexpr int S = NC * P * (P + 1) * (P + 2) / 6;
+    constexpr int G = m2m_group<NC, P>();
+    __shared__ double acc[S];
+    __shared__ double A[G * S];
+    __shared__ double dd[3 * G];
     const int i = first + blockIdx.x;
     const int nk = nkids[i];
-    if (nk == 0) return;
+    if (nk == 0) return;  // the whole block
     const double4 c = ncent[i];
     const int f = nfirst[i];
     for (int q = threadIdx.x; q < S; q += blockDim.x) acc[q] = 0.0;
-    for (int c0 = 0; c0 < nk; c0 += gmax) {
-        const int ng = min(gmax, nk - c0);
+    for (int c0 = 0; c0 < nk; c0 += G) {
+        const int ng = min(G, nk - c0);
         __syncthreads();
         // children are consecutive nodes: their moments are one contiguous block
         const double* mc = out.lex + int64_t(f + c0) * S;
@@ -246,40 +322,37 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, int p, int nterm, dou
             dd[threadIdx.x] = sign * (ax == 0 ? cc.x - c.x : (ax == 1 ? cc.y - c.y : cc.z - c.z));
         }
         __syncthreads();
-        for (int axis = 0; axis < 3; ++axis) {
-            const double* src = (axis == 1) ? B : A;
-            double* dst = (axis == 1) ? A : B;
-            for (int it = threadIdx.x; it < ng * nterm; it += blockDim.x) {
-                const int k = it / nterm, t = it - k * nterm;
-                int k1, k2, k3;
-                lex_decode(p, t, k1, k2, k3);
-                const int kk = axis == 0 ? k1 : (axis == 1 ? k2 : k3);
-                const double d = dd[3 * k + axis];
-                const double* sk = src + k * S;
-                double sm[NC] = {};
-                double pw = 1.0;
-                for (int a = kk; a >= 0; --a) {
-                    const int j = axis == 0 ? lex_pos(p, a, k2, k3) : (axis == 1 ? lex_pos(p, k1, a, k3)
-                                                                                 : lex_pos(p, k1, k2, a));
-                    const double w = c_binom.v[kk][a] * pw;
-#pragma unroll
-                    for (int q = 0; q < NC; ++q) sm[q] = fma(w, sk[NC * j + q], sm[q]);
-                    pw *= d;
-                }
-#pragma unroll
-                for (int q = 0; q < NC; ++q) dst[k * S + NC * t + q] = sm[q];
-            }
-            __syncthreads();
-        }
+        shift_lines<NC, P, 0>(A, dd, ng);
+        __syncthreads();
+        shift_lines<NC, P, 1>(A, dd, ng);
+        __syncthreads();
+        shift_lines<NC, P, 2>(A, dd, ng);
+        __syncthreads();
         for (int q = threadIdx.x; q < S; q += blockDim.x) {
             double v = acc[q];
-            for (int k = 0; k < ng; ++k) v += B[k * S + q];
+            for (int k = 0; k < ng; ++k) v += A[k * S + q];
             acc[q] = v;
         }
     }
+    if (threadIdx.x == 0) {
+        double rad = 0.0;
+        for (int k = f; k < f + nk; ++k) {
+            const double4 ck = ncent[k];
+            const double dx = ck.x - c.x, dy = ck.y - c.y, dz = ck.z - c.z;
+            rad = fmax(rad, sqrt(dx * dx + dy * dy + dz * dz) + nrad[k]);
+        }
+        nrad[i] = rad;
+    }
     __syncthreads();
     const double irho = 1.0 / c.w;
-    for (int q = threadIdx.x; q < S; q += blockDim.x) store_moment<NC>(out, i, p, nterm, q / NC, q % NC, acc[q], irho);
+    for (int q = threadIdx.x; q < S; q += blockDim.x) store_moment<NC>(out, i, P, S / NC, q / NC, q % NC, acc[q], irho);
+}
+
+template <int NC, int P>
+void launch_m2m(hipStream_t st, int64_t f, int64_t cnt, double sign, const int* nfirst, const int* nkids,
+                const double4* ncent, double* nrad, MomOut out) {
+    hipLaunchKernelGGL((tree_m2m<NC, P>), dim3(unsigned(cnt)), dim3(256), 0, st, int(f), sign, nfirst, nkids, ncent,
+                       nrad, out);
 }
 
 // Norms of the degree-0, 1 and 2 moment shells (graded positions 0, 1..3,
@@ -298,23 +371,36 @@ __global__ __launch_bounds__(256) void tree_node_stats(int nn, int p, int nterm,
     nstat[i] = make_float4(float(sqrt(sh[0])), float(sqrt(sh[1])), float(sqrt(sh[2])), float(nend[i] - nbeg[i]));
 }
 
-__global__ __launch_bounds__(256) void tree_set_gamma(const double* __restrict__ gamma, const int* __restrict__ perm,
-                                                      int64_t n, double4* __restrict__ spos) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) spos[i].w = gamma[perm[i]];
-}
-
-// FP32 offsets of every particle from its leaf's centroid (FP32C near field).
-__global__ __launch_bounds__(256) void tree_leaf_offsets(int nn, const int* __restrict__ nkids,
-                                                         const int* __restrict__ nbeg, const int* __restrict__ nend,
-                                                         const double4* __restrict__ ncent,
-                                                         const double4* __restrict__ spos, float4* __restrict__ soff) {
+// Strengths of the sorted particles (gamma[perm[q]] into spos[q].w), their
+// FP32 offsets from their leaf's centroid (FP32C near field) and the leaf's
+// source radius (largest distance of its particles from the centroid), one
+// block per leaf: every particle belongs to exactly one leaf.
+__global__ __launch_bounds__(256) void tree_leaf_strengths(int nn, const int* __restrict__ nkids,
+                                                           const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                                           const double4* __restrict__ ncent,
+                                                           const double* __restrict__ gamma,
+                                                           const int* __restrict__ perm, double4* __restrict__ spos,
+                                                           float4* __restrict__ soff, double* __restrict__ nrad) {
+    __shared__ double s_r2[4];
     const int i = blockIdx.x;
-    if (i >= nn || nkids[i] != 0) return;
+    if (i >= nn || nkids[i] != 0) return;  // the whole block
     const double4 c = ncent[i];
+    double r2 = 0.0;
     for (int q = nbeg[i] + threadIdx.x; q < nend[i]; q += blockDim.x) {
-        const double4 s = spos[q];
+        double4 s = spos[q];
+        s.w = gamma[perm[q]];
+        spos[q] = s;
         soff[q] = make_float4(float(s.x - c.x), float(s.y - c.y), float(s.z - c.z), float(s.w));
+        const double dx = s.x - c.x, dy = s.y - c.y, dz = s.z - c.z;
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    if ((threadIdx.x & 63) == 0) s_r2[threadIdx.x >> 6] = r2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) r2 = fmax(r2, s_r2[w]);
+        nrad[i] = sqrt(r2);
     }
 }
 
@@ -349,13 +435,31 @@ __device__ inline int key_lcp(uint64_t a, uint64_t b) {
     return (62 - (63 - __clzll(x))) / 3;
 }
 
+// hist[l] += adjacent sorted keys with a common prefix of l levels.  Sorted
+// keys mostly share deep prefixes, so the few busy bins would queue every
+// lane's add: the lanes of a wave are counted per value by ballots (one LDS
+// add per value and wave), and a grid of at most kHistBlocks blocks strides
+// over the keys, so that few global adds reach the bins.
+constexpr int kHistBlocks = 512;
 __global__ __launch_bounds__(256) void batch_lcp_hist(const uint64_t* __restrict__ k, int64_t n,
                                                       unsigned* __restrict__ hist) {
     __shared__ unsigned h[kMaxDepth + 1];
     if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= 1 && i < n) atomicAdd(&h[key_lcp(k[i - 1], k[i])], 1u);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    // the loop bound is wave-uniform: every lane of a wave takes the same trips
+    const int64_t base = int64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63);
+    for (int64_t w0 = base; w0 < n; w0 += stride) {
+        const int64_t i = w0 + (threadIdx.x & 63);
+        const int v = (i >= 1 && i < n) ? key_lcp(k[i - 1], k[i]) : -1;
+        uint64_t left = __ballot(v >= 0);
+        while (left) {  // one pass per distinct value in the wave
+            const int u = __builtin_amdgcn_readlane(v, __builtin_ctzll(left));
+            const uint64_t m = __ballot(v == u);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&h[u], unsigned(__popcll(m)));
+            left &= ~m;
+        }
+    }
     __syncthreads();
     if (threadIdx.x <= kMaxDepth && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
@@ -556,40 +660,54 @@ static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, 
     hipStream_t st = c->stream;
     const int nterm = nterms(p);
     const int64_t nn = t->nn;
-    // chunk list of the leaves, made by the build (internal nodes come from the upward pass)
+    // chunk list of the leaves, made by the build when a leaf holds more than
+    // one chunk (internal nodes come from the upward pass)
     const int64_t nch = t->nch;
-    const int* d_cn = t->chunks.as<int>();
-    const int* d_cb = d_cn + nch;
-    const int* d_ce = d_cb + nch;
-    const int* d_n0 = d_ce + nch;
-    const int* d_nc = d_n0 + nn;
-    LPM_CHECK(t->mpart.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * NC * sizeof(double)));
+    // every leaf one chunk: the chunk sums are the leaf moments, one block per node
+    const int direct = (nch == t->nleaves) ? 1 : 0;
+    const int* d_cn = direct ? nullptr : t->chunks.as<int>();
+    const int* d_cb = direct ? nullptr : d_cn + nch;
+    const int* d_ce = direct ? nullptr : d_cb + nch;
+    const int* d_n0 = direct ? nullptr : d_ce + nch;
+    const int* d_nc = direct ? nullptr : d_n0 + nn;
+    if (!direct) LPM_CHECK(t->mpart.reserve(size_t(std::max<int64_t>(nch, 1)) * nterm * NC * sizeof(double)));
     LPM_CHECK(mom.reserve(size_t(nn) * nterm * NC * sizeof(double)));
     LPM_CHECK(momg.reserve(size_t(nn) * nterm * NC * sizeof(double)));
     LPM_CHECK(mom32.reserve(size_t(nn) * nterm * NC * sizeof(float)));
     const MomOut out{mom.as<double>(), momg.as<double>(), mom32.as<float>()};
-    // every leaf one chunk: the chunk sums are the leaf moments
-    const int direct = (nch == t->nleaves) ? 1 : 0;
+    const int64_t nblk = direct ? nn : nch;
     if (nch > 0) {
-        const int G = std::max(1, kMomThreads / nterm);
-        const size_t lds = (size_t(kChunk) * (3 * p + NC) + (G > 1 ? size_t(G) * nterm * NC : 0)) * sizeof(double);
-        hipLaunchKernelGGL(tree_moments_chunk<NC>, dim3(unsigned(nch)), dim3(kMomThreads), lds, st, d_cn, d_cb, d_ce,
-                           t->ncent.as<double4>(), t->spos.as<double4>(), p, nterm, double(sign), t->mpart.as<double>(),
-                           out, direct);
+        switch (p) {
+#define LPM_CASE(PP)                                                                                          \
+    case PP:                                                                                                  \
+        launch_moments_chunk<NC, PP>(st, nblk, d_cn, d_cb, d_ce, t->nkids.as<int>(), t->nbeg.as<int>(),       \
+                                     t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),        \
+                                     double(sign), t->mpart.as<double>(), out, direct);                      \
+        break;
+            LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
+            LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
+#undef LPM_CASE
+            default: return fail(c, LPM_ERR_INVALID_ARG, "moment order %d unsupported", p);
+        }
     }
     if (!direct)
         hipLaunchKernelGGL(tree_moments_reduce<NC>, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), p, nterm,
                            t->nkids.as<int>(), d_n0, d_nc, t->ncent.as<double4>(), t->mpart.as<double>(), out);
-    // upward pass, deepest internal level first; up to gmax children per
-    // block within 64 KB of LDS
-    const int S = NC * nterm;
-    const int gmax = std::max(1, std::min(8, (8192 - S - 24) / (2 * S)));
-    const size_t m2m_lds = size_t(S + 2 * gmax * S + 3 * gmax) * sizeof(double);
+    // upward pass, deepest internal level first
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(tree_m2m<NC>, dim3(unsigned(cnt)), dim3(256), m2m_lds, st, int(f), p, nterm, double(sign),
-                           gmax, t->nfirst.as<int>(), t->nkids.as<int>(), t->ncent.as<double4>(), out);
+        switch (p) {
+#define LPM_CASE(PP)                                                                                      \
+    case PP:                                                                                              \
+        launch_m2m<NC, PP>(st, f, cnt, double(sign), t->nfirst.as<int>(), t->nkids.as<int>(),             \
+                           t->ncent.as<double4>(), t->nrad.as<double>(), out);                            \
+        break;
+            LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
+            LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
+#undef LPM_CASE
+            default: return fail(c, LPM_ERR_INVALID_ARG, "moment order %d unsupported", p);
+        }
     }
     return LPM_OK;
 }
@@ -598,12 +716,9 @@ static int set_strengths(lpm_ctx* c, const double* gamma) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
-    if (t->ns > 0)
-        hipLaunchKernelGGL(tree_set_gamma, dim3(blocks(t->ns)), dim3(256), 0, st, gamma, t->perm.as<int>(), t->ns,
-                           t->spos.as<double4>());
-    hipLaunchKernelGGL(tree_leaf_offsets, dim3(unsigned(t->nn)), dim3(256), 0, st, int(t->nn), t->nkids.as<int>(),
-                       t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),
-                       t->soff32.as<float4>());
+    hipLaunchKernelGGL(tree_leaf_strengths, dim3(unsigned(t->nn)), dim3(256), 0, st, int(t->nn), t->nkids.as<int>(),
+                       t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), gamma, t->perm.as<int>(),
+                       t->spos.as<double4>(), t->soff32.as<float4>(), t->nrad.as<double>());
     return LPM_OK;
 }
 
@@ -713,7 +828,8 @@ int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const do
     void* tmp = w + off_t;
     const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
     LPM_HIP(c, hipMemsetAsync(hist, 0, 128, st));
-    hipLaunchKernelGGL(batch_lcp_hist, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, hist);
+    hipLaunchKernelGGL(batch_lcp_hist, dim3(unsigned(std::min<int64_t>(kHistBlocks, blocks(nt)))), dim3(256), 0, st,
+                       keys, nt, hist);
     hipLaunchKernelGGL(batch_level, dim3(1), dim3(1), 0, st, hist, want, sorted_levels, level);
     hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, level, cs);
     size_t b1 = tmp_bytes;

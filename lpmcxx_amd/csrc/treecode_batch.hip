@@ -172,8 +172,10 @@ __device__ __forceinline__ void far_node(int p, int n, int nterm, double t0, dou
 }
 
 // Leaf [b, e) summed directly at every lane's target; the sources are
-// wave-uniform loads.
-template <bool F32>
+// wave-uniform loads.  CHECK = false (FP32 only): no target of the batch
+// comes within 1e-6 of the leaf's sources, so no pair meets the zero rule
+// and its test is left out.
+template <bool F32, bool CHECK = true>
 __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, double t2, const double4& c, double d2,
                                           float eps2v, const double4* __restrict__ spos,
                                           const float4* __restrict__ soff, double& s0, double& s1, double& s2,
@@ -185,7 +187,7 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
             const float dx = q0 - o.x, dy = q1 - o.y, dz = q2 - o.z;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2v)));
             // the reference's |den| <= 1e-13 rule: |d|^2 + 2 delta^2 <= 2e-13 contributes 0
-            const float wgt = (r2 > 2e-13f) ? o.w * __builtin_amdgcn_rcpf(r2) : 0.f;
+            const float wgt = (!CHECK || r2 > 2e-13f) ? o.w * __builtin_amdgcn_rcpf(r2) : 0.f;
             a0 = fmaf(wgt, dx, a0);
             a1 = fmaf(wgt, dy, a1);
             a2 = fmaf(wgt, dz, a2);
@@ -452,6 +454,9 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
 // without the model's registers and LDS, and tree_list_finish scales and
 // scatters.
 constexpr int kNodeBits = 27;
+// near-list entry: leaf | kNearCheck when a target of the batch may come
+// within 1e-6 of one of the leaf's sources (the zero rule must be tested)
+constexpr int kNearCheck = 1 << 30;
 
 // planning-stack entry of an internal node: first child | child count << kNodeBits
 __device__ inline int stack_entry(int first, int kids) { return first | (kids << kNodeBits); }
@@ -518,10 +523,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     int* stl = ls.st + bi * ls.st_cap;
     int nfar = 0, nhi = 0, nnear = 0, nstl = 0;
     unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0, nst = 0, nterms_st = 0;
+    unsigned l_terms = 0, l_terms_hi = 0, l_np = 0, l_terms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int sp = 0;
     if (tv.nkids[0] == 0) {  // the root is a leaf
-        if (lane == 0 && ls.near_cap > 0) near[0] = 0;
+        if (lane == 0 && ls.near_cap > 0) near[0] = 0 | kNearCheck;
         nnear = 1;
         near_pairs = unsigned(tv.nend[0] - tv.nbeg[0]);
     } else {
@@ -571,7 +577,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         const bool has = row < tot;
         const int chr = firstc + row - base;
         bool prop = false;
-        double r3 = 0.0;  // the pair's ratio (r_n + r_b) / s on the sphere
+        bool near_chk = true;  // a leaf child's sources may come within 1e-6 of a target
+        // stereographic geometry test of the pair, taken before the model
+        // call so that its node load overlaps the features: the plane ratio
+        // no worse (by sv.ratio) than the sphere ratio (r_n + r_b) / s
+        bool st_geo = false;
         float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
         // every node-table word of the child is loaded here, in one round
         int ckids = 0, cfirst = 0, cb = 0, ce = 0;
@@ -584,9 +594,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
             const double sd = sqrt(dx * dx + dy * dy + dz * dz);
             const double rn = tv.nrad[chr];
+            near_chk = !(sd - rn - bt.r > 1e-6);
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
-                r3 = (rn + bt.r) / sd;
+                if (st_on) {
+                    const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + chr];
+                    const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
+                    const double dist = sqrt(er * er + ei * ei);
+                    const double rz = gz.z + sb.r;
+                    st_geo = rz < sv.ratio * ((rn + bt.r) / sd) * dist && rz < 0.95 * dist;
+                }
                 float fe[kIn];
                 pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
                 x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
@@ -608,18 +625,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         }
         const bool mine = lane < 16 && has;
         const bool leaf = mine && ckids == 0;
-        // an accepted pair whose plane ratio is no worse (by sv.ratio) than its
-        // sphere ratio becomes a stereographic entry of order p + extra
-        bool is_st = false;
-        int pz = 0;
-        if (st_on && mine && p > 0) {
-            const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + chr];
-            const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
-            const double dist = sqrt(er * er + ei * ei);
-            const double rz = gz.z + sb.r;
-            is_st = rz < sv.ratio * r3 * dist && rz < 0.95 * dist;
-            pz = min(p + sv.extra, sv.maxp);
-        }
+        // an accepted pair that passes the stereographic geometry test
+        // becomes a stereographic entry of order p + extra
+        const bool is_st = st_geo && mine && p > 0;
+        const int pz = is_st ? min(p + sv.extra, sv.maxp) : 0;
         const bool is_lo = mine && p > 0 && p <= kLowOrder && !is_st, is_hi = mine && p > kLowOrder && !is_st;
         const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
         const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi), m_st = __ballot(is_st);
@@ -635,33 +644,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         }
         if (nstl + c_st <= ls.st_cap && is_st) stl[nstl + __popcll(m_st & below)] = chr | (pz << kNodeBits);
         if (nnear + c_near <= ls.near_cap && is_near)
-            near[nnear + __popcll(m_near & below)] = chr;
+            near[nnear + __popcll(m_near & below)] = chr | (near_chk ? kNearCheck : 0);
         if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = stack_entry(cfirst, ckids);
         nfar += c_lo;
         nhi += c_hi;
         nstl += c_st;
         nnear += c_near;
         sp += c_desc;
-        // terms of the accepted children and sources of the near leaves
-        // (lanes 0..15: an xor butterfly over 16 lanes sums them)
-        unsigned t_all = ((is_lo || is_hi) && !is_st) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
-        unsigned t_hi = is_hi ? t_all : 0u;
-        unsigned np = is_near ? unsigned(ce - cb) : 0u;
-        unsigned t_st = is_st ? unsigned(pz + 1) : 0u;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            t_all += __shfl_xor(t_all, o, 64);
-            t_hi += __shfl_xor(t_hi, o, 64);
-            np += __shfl_xor(np, o, 64);
-            t_st += __shfl_xor(t_st, o, 64);
-        }
-        nterms += __builtin_amdgcn_readfirstlane(t_all);
-        nterms_hi += __builtin_amdgcn_readfirstlane(t_hi);
-        near_pairs += __builtin_amdgcn_readfirstlane(np);
-        nterms_st += __builtin_amdgcn_readfirstlane(t_st);
+        // terms of the accepted children and sources of the near leaves, kept
+        // per lane (lanes 0..15) and summed over the wave after the walk
+        const unsigned t_all = ((is_lo || is_hi) && !is_st) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
+        l_terms += t_all;
+        l_terms_hi += is_hi ? t_all : 0u;
+        l_np += is_near ? unsigned(ce - cb) : 0u;
+        l_terms_st += is_st ? unsigned(pz + 1) : 0u;
         nst += __popcll(__ballot(is_st));
         wave_sync();
     }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        l_terms += __shfl_xor(l_terms, o, 64);
+        l_terms_hi += __shfl_xor(l_terms_hi, o, 64);
+        l_np += __shfl_xor(l_np, o, 64);
+        l_terms_st += __shfl_xor(l_terms_st, o, 64);
+    }
+    nterms = __builtin_amdgcn_readfirstlane(l_terms);
+    nterms_hi = __builtin_amdgcn_readfirstlane(l_terms_hi);
+    near_pairs += __builtin_amdgcn_readfirstlane(l_np);
+    nterms_st = __builtin_amdgcn_readfirstlane(l_terms_st);
     if (lane == 0) {
         // a batch whose lists do not fit is evaluated as empty and flagged with
         // the capacities it needs; the caller plans again with larger lists
@@ -712,9 +722,14 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
         const int* near = ls.near + bi * ls.near_cap;
         const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
         for (int j = 0; j < nnear; ++j) {
-            const int n = __builtin_amdgcn_readfirstlane(near[j]);
-            near_leaf<F32>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
-                           v0, v1, v2);
+            const int ent = __builtin_amdgcn_readfirstlane(near[j]);
+            const int n = ent & ~kNearCheck;
+            if (!F32 || (ent & kNearCheck))
+                near_leaf<F32, true>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff,
+                                     s0, s1, s2, v0, v1, v2);
+            else
+                near_leaf<F32, false>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff,
+                                      s0, s1, s2, v0, v1, v2);
         }
     }
     for (int j = j0; j < j1; ++j) {
@@ -726,9 +741,7 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     }
     cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
     if (STEREO && LOW) {
-        // stereographic entries in the batch's projection, software-pipelined:
-        // entry j + 1's node data and entry j + 2's list word are requested
-        // before entry j is summed
+        // stereographic entries in the batch's projection
         const int* stl = ls.st + bi * ls.st_cap;
         const int nst = __builtin_amdgcn_readfirstlane(ls.st_cnt[bi]);
         if (nst > 0) {
