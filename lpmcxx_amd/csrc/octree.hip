@@ -31,31 +31,43 @@
 namespace lpm {
 namespace {
 
+// kKeyBlocks blocks stride over the sources, so that the range of |x|^2
+// folds into the build table with few atomics
+constexpr int kKeyBlocks = 2048;
 __global__ __launch_bounds__(256) void tree_keys(const double* __restrict__ x, const double* __restrict__ y,
                                                  const double* __restrict__ z, int64_t n, double x0, double x1,
                                                  double y0, double y1, double z0, double z1,
                                                  uint64_t* __restrict__ keys, int* __restrict__ vals,
                                                  int* __restrict__ lvt) {
     int* outside = lvt + kLvOutside;
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const double px = valid ? x[i] : 0.0, py = valid ? y[i] : 0.0, pz = valid ? z[i] : 0.0;
-    r2_block_range(px * px + py * py + pz * pz, valid,
-                   reinterpret_cast<unsigned long long*>(lvt + kLvR2) + 2 * (blockIdx.x % kR2Slots));
-    if (!valid) return;
-    if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
-    double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
-    uint64_t key = 0;
-    for (int l = 0; l < kMaxDepth; ++l) {
-        const double mx = 0.5 * (xl + xh), my = 0.5 * (yl + yh), mz = 0.5 * (zl + zh);
-        const int bx = px >= mx, by = py >= my, bz = pz >= mz;
-        if (bx) xl = mx; else xh = mx;
-        if (by) yl = my; else yh = my;
-        if (bz) zl = mz; else zh = mz;
-        key = (key << 3) | uint64_t(bx | (by << 1) | (bz << 2));
+    double lo = 0.0, hi = 0.0;
+    bool any = false;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double px = x[i], py = y[i], pz = z[i];
+        const double r2 = px * px + py * py + pz * pz;
+        // NaN must win both ends: compare on the bit patterns the words use
+        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(r2));
+        if (!any || bits < static_cast<unsigned long long>(__double_as_longlong(lo))) lo = r2;
+        if (!any || bits > static_cast<unsigned long long>(__double_as_longlong(hi))) hi = r2;
+        any = true;
+        if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
+        double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
+        uint64_t key = 0;
+        for (int l = 0; l < kMaxDepth; ++l) {
+            const double mx = 0.5 * (xl + xh), my = 0.5 * (yl + yh), mz = 0.5 * (zl + zh);
+            const int bx = px >= mx, by = py >= my, bz = pz >= mz;
+            if (bx) xl = mx; else xh = mx;
+            if (by) yl = my; else yh = my;
+            if (bz) zl = mz; else zh = mz;
+            key = (key << 3) | uint64_t(bx | (by << 1) | (bz << 2));
+        }
+        keys[i] = key;
+        vals[i] = int(i);
     }
-    keys[i] = key;
-    vals[i] = int(i);
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(lvt + kLvR2) + 2 * (blockIdx.x % kR2Slots);
+    r2_block_range(lo, any, w);
+    __syncthreads();  // the second call reuses the block's reduction words
+    r2_block_range(hi, any, w);
 }
 
 __device__ inline int child_code(uint64_t key, int level) {  // child index taken at depth level+1
@@ -295,11 +307,13 @@ __global__ __launch_bounds__(256) void tree_leaf_keys(int nn, const int* __restr
 
 // Leaf-local ascending source order without a sort: block per leaf (at most
 // kRankMax sources), each source's position is the number of the leaf's
-// sources with a smaller index.
+// sources with a smaller index; the source's position goes to spos there.
 constexpr int kRankMax = 1024;
 __global__ __launch_bounds__(256) void tree_leaf_rank(int nn, const int* __restrict__ nkids,
                                                       const int* __restrict__ nbeg, const int* __restrict__ nend,
-                                                      const int* __restrict__ perm_in, int* __restrict__ perm_out) {
+                                                      const int* __restrict__ perm_in, int* __restrict__ perm_out,
+                                                      const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z, double4* __restrict__ spos) {
     __shared__ int v[kRankMax];
     const int i = blockIdx.x;
     if (i >= nn || nkids[i] != 0) return;
@@ -307,10 +321,11 @@ __global__ __launch_bounds__(256) void tree_leaf_rank(int nn, const int* __restr
     for (int t = threadIdx.x; t < m; t += blockDim.x) v[t] = perm_in[b + t];
     __syncthreads();
     for (int t = threadIdx.x; t < m; t += blockDim.x) {
-        const int x = v[t];
+        const int s = v[t];
         int r = 0;
-        for (int k = 0; k < m; ++k) r += v[k] < x ? 1 : 0;
-        perm_out[b + r] = x;
+        for (int k = 0; k < m; ++k) r += v[k] < s ? 1 : 0;
+        perm_out[b + r] = s;
+        spos[b + r] = make_double4(x[s], y[s], z[s], 0.0);
     }
 }
 
@@ -415,8 +430,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     LPM_CHECK(grow_nodes(c, t, std::max<int64_t>(1024, 4 * (n / std::max(1, leaf_max)) + 64)));
     LPM_HIP(c, hipMemsetAsync(lvt, 0, kLvTotal * sizeof(int), st));
     if (n > 0)
-        hipLaunchKernelGGL(tree_keys, dim3(blocks(n)), dim3(256), 0, st, x, y, z, n, blo[0], bhi[0], blo[1], bhi[1],
-                           blo[2], bhi[2], t->tmp0.as<uint64_t>(), t->tmp1.as<int>(), lvt);
+        hipLaunchKernelGGL(tree_keys, dim3(unsigned(std::min<int64_t>(kKeyBlocks, blocks(n)))), dim3(256), 0, st, x, y,
+                           z, n, blo[0], bhi[0], blo[1], bhi[1], blo[2], bhi[2], t->tmp0.as<uint64_t>(),
+                           t->tmp1.as<int>(), lvt);
 
     // Level by level: count children, then emit them (offsets from the
     // per-block child counts) -- every level up to the sorted key levels in
@@ -512,9 +528,11 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         // perm = leaf order, kperm = the order of the sorted keys.  Leaves of
         // at most kRankMax sources (max-per-node control): ranks within the
         // leaf; else keys truncated to the leaf depth, sorted from source order.
-        if (control == 0 && leaf_max <= kRankMax) {
+        const bool ranked = control == 0 && leaf_max <= kRankMax;
+        if (ranked) {
             hipLaunchKernelGGL(tree_leaf_rank, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
-                               t->nbeg.as<int>(), t->nend.as<int>(), t->perm.as<int>(), t->kperm.as<int>());
+                               t->nbeg.as<int>(), t->nend.as<int>(), t->perm.as<int>(), t->kperm.as<int>(), x, y, z,
+                               t->spos.as<double4>());
         } else {
             hipLaunchKernelGGL(tree_leaf_keys, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
                                t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
@@ -524,8 +542,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
                                  t->kperm.as<int>(), n, std::min(3 * (kMaxDepth - depth), 3 * kMaxDepth - 1)));
         }
         std::swap(t->perm, t->kperm);
-        hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
-                           t->spos.as<double4>());
+        if (!ranked)
+            hipLaunchKernelGGL(gather_pos, dim3(blocks(n)), dim3(256), 0, st, x, y, z, t->perm.as<int>(), n,
+                               t->spos.as<double4>());
     }
     // node source radii: leaves with the strengths, internal nodes in the
     // upward pass of the moments

@@ -92,6 +92,46 @@ __device__ inline void store_moment(const MomOut& o, int64_t node, int p, int nt
     o.scaled[g] = float(v * sc);
 }
 
+// Graded position and degree of every lex term |k| < P (packed g | deg << 16),
+// for the kernels that know P at compile time.
+template <int P>
+struct LexInfo {
+    static constexpr int NT = P * (P + 1) * (P + 2) / 6;
+    int gd[NT] = {};
+    constexpr LexInfo() {
+        int t = 0;
+        for (int a = 0; a < P; ++a)
+            for (int b = 0; a + b < P; ++b)
+                for (int c = 0; a + b + c < P; ++c) gd[t++] = graded_index(a, b, c) | ((a + b + c) << 16);
+    }
+};
+template <int P>
+inline constexpr LexInfo<P> kLexInfo{};
+
+// store_moment with P known: ipow[d] = irho^d (d < P) formed by repeated
+// multiplication, the same products as store_moment's loop
+template <int NC, int P>
+__device__ inline void store_moment_p(const MomOut& o, int64_t node, int t, int cc, double v, const double* ipow) {
+    constexpr int NT = P * (P + 1) * (P + 2) / 6;
+    const int gd = kLexInfo<P>.gd[t];
+    const int64_t base = node * NT * NC;
+    const int64_t g = base + NC * (gd & 0xffff) + cc;
+    o.lex[base + NC * t + cc] = v;
+    o.graded[g] = v;
+    o.scaled[g] = float(v * ipow[gd >> 16]);
+}
+
+// ipow[d] = irho^d, d < P (one thread)
+template <int P>
+__device__ inline void inverse_powers(double irho, double* ipow) {
+    double x = 1.0;
+#pragma unroll
+    for (int d = 0; d < P; ++d) {
+        ipow[d] = x;
+        x *= irho;
+    }
+}
+
 constexpr int kMomThreads = 256;
 // LDS row of the leaf moment kernel: a chunk's particles plus two pad
 // doubles, so that rows read at the same particle by different lanes start
@@ -123,20 +163,29 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
     double* pw = lds;                       // [3][P][kMomRow] powers
     double* gy = lds + 3 * P * kMomRow;     // [NC][kMomRow]   gamma * y, or gamma
     double* red = gy + NC * kMomRow;        // [G][nterm][NC]  group partials (G > 1)
-    // chunk_node == nullptr: one block per node, every leaf one chunk
+    // chunk_node == nullptr: one block per node, every leaf one chunk (the
+    // node's words are loaded together, then internal nodes leave)
     const int ch = blockIdx.x;
-    if (!chunk_node && nkids[ch] != 0) return;  // the whole block
-    const int node = chunk_node ? chunk_node[ch] : ch;
-    const int b = chunk_node ? chunk_beg[ch] : nbeg[ch], e = chunk_node ? chunk_end[ch] : nend[ch];
+    int node = ch, b, e;
+    double4 c;
+    if (!chunk_node) {
+        const int nk = nkids[ch];
+        b = nbeg[ch];
+        e = nend[ch];
+        c = ncent[ch];
+        if (nk != 0) return;  // the whole block
+    } else {
+        node = chunk_node[ch];
+        b = chunk_beg[ch];
+        e = chunk_end[ch];
+        c = ncent[node];
+    }
     const int cnt = e - b;
-    const double4 c = ncent[node];
-    for (int q = threadIdx.x; q < kChunk; q += blockDim.x) {
-        double hx = 0.0, hy = 0.0, hz = 0.0, g = 0.0, yx = 0.0, yy = 0.0, yz = 0.0;
-        if (q < cnt) {
-            const double4 s = spos[b + q];
-            yx = s.x; yy = s.y; yz = s.z; g = s.w;
-            hx = sign * (s.x - c.x); hy = sign * (s.y - c.y); hz = sign * (s.z - c.z);
-        }
+    __shared__ double s_ipow[P];
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {  // the sums read rows q < cnt only
+        const double4 s = spos[b + q];
+        const double yx = s.x, yy = s.y, yz = s.z, g = s.w;
+        const double hx = sign * (s.x - c.x), hy = sign * (s.y - c.y), hz = sign * (s.z - c.z);
         double ax = 1.0, ay = 1.0, az = 1.0;
 #pragma unroll
         for (int a = 0; a < P; ++a) {
@@ -153,8 +202,8 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
             gy[q] = g;
         }
     }
+    if (threadIdx.x == 0) inverse_powers<P>(1.0 / c.w, s_ipow);
     __syncthreads();
-    const double irho = 1.0 / c.w;
     for (int it = threadIdx.x; it < G * nterm; it += blockDim.x) {
         const int grp = it / nterm, t = it - grp * nterm;
         int k1, k2, k3;
@@ -185,7 +234,7 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
         } else {
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
-                if (direct) store_moment<NC>(out, node, P, nterm, t, cc, m[cc], irho);
+                if (direct) store_moment_p<NC, P>(out, node, t, cc, m[cc], s_ipow);
                 else part[(int64_t(ch) * nterm + t) * NC + cc] = m[cc];
             }
         }
@@ -195,7 +244,7 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
     for (int it = threadIdx.x; it < nterm * NC; it += blockDim.x) {
         double v = red[it];
         for (int grp = 1; grp < G; ++grp) v += red[grp * nterm * NC + it];
-        if (direct) store_moment<NC>(out, node, P, nterm, it / NC, it % NC, v, irho);
+        if (direct) store_moment_p<NC, P>(out, node, it / NC, it % NC, v, s_ipow);
         else part[int64_t(ch) * nterm * NC + it] = v;
     }
 }
@@ -304,6 +353,7 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, double sign, const in
     __shared__ double acc[S];
     __shared__ double A[G * S];
     __shared__ double dd[3 * G];
+    static_assert(3 * G >= P, "dd also holds the P inverse powers of the stores");
     const int i = first + blockIdx.x;
     const int nk = nkids[i];
     if (nk == 0) return;  // the whole block
@@ -335,6 +385,7 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, double sign, const in
         }
     }
     if (threadIdx.x == 0) {
+        inverse_powers<P>(1.0 / c.w, dd);  // dd is free now: it holds irho^d for the stores
         double rad = 0.0;
         for (int k = f; k < f + nk; ++k) {
             const double4 ck = ncent[k];
@@ -344,8 +395,7 @@ __global__ __launch_bounds__(256) void tree_m2m(int first, double sign, const in
         nrad[i] = rad;
     }
     __syncthreads();
-    const double irho = 1.0 / c.w;
-    for (int q = threadIdx.x; q < S; q += blockDim.x) store_moment<NC>(out, i, P, S / NC, q / NC, q % NC, acc[q], irho);
+    for (int q = threadIdx.x; q < S; q += blockDim.x) store_moment_p<NC, P>(out, i, q / NC, q % NC, acc[q], dd);
 }
 
 template <int NC, int P>
@@ -374,14 +424,13 @@ __global__ __launch_bounds__(256) void tree_node_stats(int nn, int p, int nterm,
 // Strengths of the sorted particles (gamma[perm[q]] into spos[q].w), their
 // FP32 offsets from their leaf's centroid (FP32C near field) and the leaf's
 // source radius (largest distance of its particles from the centroid), one
-// block per leaf: every particle belongs to exactly one leaf.
-__global__ __launch_bounds__(256) void tree_leaf_strengths(int nn, const int* __restrict__ nkids,
+// wave per leaf: every particle belongs to exactly one leaf.
+__global__ __launch_bounds__(64) void tree_leaf_strengths(int nn, const int* __restrict__ nkids,
                                                            const int* __restrict__ nbeg, const int* __restrict__ nend,
                                                            const double4* __restrict__ ncent,
                                                            const double* __restrict__ gamma,
                                                            const int* __restrict__ perm, double4* __restrict__ spos,
                                                            float4* __restrict__ soff, double* __restrict__ nrad) {
-    __shared__ double s_r2[4];
     const int i = blockIdx.x;
     if (i >= nn || nkids[i] != 0) return;  // the whole block
     const double4 c = ncent[i];
@@ -396,12 +445,7 @@ __global__ __launch_bounds__(256) void tree_leaf_strengths(int nn, const int* __
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
-    if ((threadIdx.x & 63) == 0) s_r2[threadIdx.x >> 6] = r2;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < int(blockDim.x >> 6); ++w) r2 = fmax(r2, s_r2[w]);
-        nrad[i] = sqrt(r2);
-    }
+    if (threadIdx.x == 0) nrad[i] = sqrt(r2);
 }
 
 // ---------------------------------------------------------------- target ordering
@@ -654,8 +698,12 @@ __global__ __launch_bounds__(256) void tree_eval(const double* __restrict__ tx, 
 // P2M over the build's leaf chunks, chunk reduction (leaves of several
 // chunks), upward pass level by level; every kernel stores the lex, graded
 // and scaled FP32 forms of the moments it makes.  NC components.
-template <int NC>
-static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, DevBuf& mom32) {
+// between_passes (may be empty) is called after the leaf moments are
+// enqueued and before the upward pass (the stereographic moments go to the
+// second stream there, behind the first stream's longest kernel).
+template <int NC, class Hook>
+static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, DevBuf& mom32,
+                        Hook between_passes) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     const int nterm = nterms(p);
@@ -693,6 +741,7 @@ static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, 
     if (!direct)
         hipLaunchKernelGGL(tree_moments_reduce<NC>, dim3(blocks(nn * nterm * NC)), dim3(256), 0, st, int(nn), p, nterm,
                            t->nkids.as<int>(), d_n0, d_nc, t->ncent.as<double4>(), t->mpart.as<double>(), out);
+    LPM_CHECK(between_passes());
     // upward pass, deepest internal level first
     for (int l = int(t->level_start.size()) - 3; l >= 0; --l) {
         const int64_t f = t->level_start[l], cnt = t->level_start[l + 1] - f;
@@ -716,7 +765,7 @@ static int set_strengths(lpm_ctx* c, const double* gamma) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     LPM_CHECK(t->soff32.reserve(size_t(std::max<int64_t>(t->ns, 1)) * sizeof(float4)));
-    hipLaunchKernelGGL(tree_leaf_strengths, dim3(unsigned(t->nn)), dim3(256), 0, st, int(t->nn), t->nkids.as<int>(),
+    hipLaunchKernelGGL(tree_leaf_strengths, dim3(unsigned(t->nn)), dim3(64), 0, st, int(t->nn), t->nkids.as<int>(),
                        t->nbeg.as<int>(), t->nend.as<int>(), t->ncent.as<double4>(), gamma, t->perm.as<int>(),
                        t->spos.as<double4>(), t->soff32.as<float4>(), t->nrad.as<double>());
     return LPM_OK;
@@ -728,19 +777,22 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     LPM_CHECK(t->nstat.reserve(size_t(t->nn) * sizeof(float4)));
     LPM_CHECK(set_strengths(c, gamma));
     t->stereo_terms = 0;
-    if (c->stereo) {
-        // the stereographic moments on the second stream, beside the Cartesian ones
-        if (!c->aux) {
-            LPM_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
-            LPM_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-            LPM_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        }
-        LPM_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+    if (c->stereo && !c->aux) {
+        LPM_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        LPM_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        LPM_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    // the stereographic moments on the second stream, beside the Cartesian
+    // ones: forked once the strengths are set, enqueued after the leaf
+    // moments so that those start first
+    if (c->stereo) LPM_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+    LPM_CHECK(moments_pass<3>(c, p, sign, t->mom, t->momg, t->mom32, [&]() -> int {
+        if (!c->stereo) return LPM_OK;
         LPM_HIP(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
         LPM_CHECK(tree_stereo_moments(c, p, c->aux));
         LPM_HIP(c, hipEventRecord(c->ev_join, c->aux));
-    }
-    LPM_CHECK(moments_pass<3>(c, p, sign, t->mom, t->momg, t->mom32));
+        return LPM_OK;
+    }));
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(t->nn)), dim3(256), 0, c->stream, int(t->nn), p, nterm,
                        t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
@@ -753,7 +805,7 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
 int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
     Tree* t = c->tree;
     LPM_CHECK(set_strengths(c, gamma));
-    LPM_CHECK(moments_pass<1>(c, p, 1, t->pmom, t->pmomg, t->pmom32));
+    LPM_CHECK(moments_pass<1>(c, p, 1, t->pmom, t->pmomg, t->pmom32, []() -> int { return LPM_OK; }));
     LPM_HIP(c, hipGetLastError());
     t->psi_order = p;
     return LPM_OK;
