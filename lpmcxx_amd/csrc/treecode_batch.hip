@@ -116,6 +116,30 @@ __device__ inline void pair_features(float* f, double rn, double rb, double s, f
     for (int k = kFeatures; k < kIn; ++k) f[k] = 0.f;
 }
 
+// Features 4g .. 4g+3 of the pair (the lane's input group g of the model
+// call), each formed by the same operations as in pair_features (so the
+// values are identical), with one log2 per feature slot instead of all
+// seven: the slot's log argument is chosen by g first.
+__device__ inline void pair_features_group(int g, float& x0, float& x1, float& x2, float& x3, double rn, double rb,
+                                           double s, float4 st, bool leaf, int nvalid, double u_ref) {
+    const float sf = float(s);
+    const float is = __builtin_amdgcn_rcpf(sf);
+    const float rnf = float(fmax(rn, 1e-12 * s)), rbf = float(fmax(rb, 1e-12 * s));
+    const float m0 = st.x, m1 = st.y, m2 = st.z;
+    const float mr = m0 * rnf;
+    const bool g0 = g == 0, g1 = g == 1, g2 = g == 2;
+    // slot 0: log2(r_n/s) | log2(M1/(M0 r_n)) | log2 s | 0
+    const float a0 = g0 ? rnf * is : g1 ? m1 * __builtin_amdgcn_rcpf(mr) : sf;
+    x0 = (g0 || g2 || (g1 && mr > 0.f)) ? flog2(a0) : (g1 ? -100.f : 0.f);
+    // slot 1: log2(r_b/s) | log2(M2/(M0 r_n^2)) | leaf flag | 0
+    const float a1 = g0 ? rbf * is : m2 * __builtin_amdgcn_rcpf(mr * rnf);
+    x1 = (g0 || (g1 && mr > 0.f)) ? flog2(a1) : (g1 ? -100.f : (g2 && leaf ? 1.f : 0.f));
+    // slot 2: (r_n + r_b)/s | log2(npart) | 0 | 0
+    x2 = g0 ? (rnf + rbf) * is : g1 ? flog2(st.w) : 0.f;
+    // slot 3: log2(M0/(u_ref s)) | nvalid/64 | 0 | 0
+    x3 = g0 ? flog2(m0 * is * __builtin_amdgcn_rcpf(float(u_ref))) : g1 ? float(nvalid) * (1.f / 64.f) : 0.f;
+}
+
 template <typename T, int... Ps>
 __device__ __forceinline__ void far_switch(std::integer_sequence<int, Ps...>, int p, T d0, T d1, T d2, T dd,
                                            const T* __restrict__ mm, T& S0, T& S1, T& S2) {
@@ -598,18 +622,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
                 if (st_on) {
+                    // rz / dist < ratio (r_n + r_b) / s and rz < 0.95 dist, squared
+                    // (both sides are >= 0; rho = +inf fails both)
                     const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + chr];
                     const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
-                    const double dist = sqrt(er * er + ei * ei);
+                    const double dist2 = er * er + ei * ei;
                     const double rz = gz.z + sb.r;
-                    st_geo = rz < sv.ratio * ((rn + bt.r) / sd) * dist && rz < 0.95 * dist;
+                    const double lhs = rz * sd, rhs = sv.ratio * (rn + bt.r);
+                    st_geo = lhs * lhs < rhs * rhs * dist2 && rz * rz < 0.9025 * dist2;
                 }
-                float fe[kIn];
-                pair_features(fe, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
-                x0 = g == 0 ? fe[0] : g == 1 ? fe[4] : g == 2 ? fe[8] : fe[12];
-                x1 = g == 0 ? fe[1] : g == 1 ? fe[5] : g == 2 ? fe[9] : fe[13];
-                x2 = g == 0 ? fe[2] : g == 1 ? fe[6] : g == 2 ? fe[10] : fe[14];
-                x3 = g == 0 ? fe[3] : g == 1 ? fe[7] : g == 2 ? fe[11] : fe[15];
+                pair_features_group(g, x0, x1, x2, x3, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
                 const float4 mu = s_norm[0][g], iv = s_norm[1][g], lo = s_norm[2][g], hi = s_norm[3][g];
                 x0 = (fminf(fmaxf(x0, lo.x), hi.x) - mu.x) * iv.x;
                 x1 = (fminf(fmaxf(x1, lo.y), hi.y) - mu.y) * iv.y;
@@ -699,9 +721,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
 // order <= kLowOrder, written to acc[3][nt] (sorted positions, FP64); else
 // the far entries of order kLowOrder + 1 .. MAXP, added to acc.  The order is
 // wave-uniform per entry.
+// Output: acc[3][nt] at the sorted positions (LOW writes, else adds), or,
+// when ux is not NULL (LOW with no high-order pass), the scaled velocity
+// straight to ux/uy/uz at the target's own index.
 template <int MAXP, bool LOW, bool F32, bool STEREO = false>
 __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, StereoView sv, double d2,
-                                                       float eps2v, double* __restrict__ acc) {
+                                                       float eps2v, double* __restrict__ acc, double mult,
+                                                       double* __restrict__ ux, double* __restrict__ uy,
+                                                       double* __restrict__ uz) {
     const int64_t wi = wave_index();
     if (wi >= tg.nb) return;
     const int64_t bi = __builtin_amdgcn_readfirstlane(ls.order[wi]);
@@ -767,7 +794,11 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
         }
     }
     if (valid) {
-        if (LOW) {
+        if (LOW && ux) {
+            ux[orig] = s0 * mult;
+            uy[orig] = s1 * mult;
+            uz[orig] = s2 * mult;
+        } else if (LOW) {
             acc[pos] = s0;
             acc[tg.n + pos] = s1;
             acc[2 * tg.n + pos] = s2;
@@ -779,13 +810,33 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     }
 }
 
-// cost 0 and the identity order for every wave of the launch (the planning
-// pass sets the costs of the real batches)
-__global__ __launch_bounds__(256) void list_order_init(int64_t n, unsigned* __restrict__ cost, int* __restrict__ idx) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    cost[i] = 0u;
-    idx[i] = int(i);
+// The waves of the list kernels take the batches dearest first (an LPT
+// order): a counting sort of the planning pass's cost estimates on 32 bins
+// of log2(cost), dearest bin first, in one block.  Wave indices at or past
+// the batch count (no batch) go last.  Within a bin the order is that of the
+// block's LDS atomics: it only decides which wave evaluates which batch, not
+// any result.
+constexpr int kCostBins = 32;
+__device__ inline int cost_bin(unsigned c) { return c == 0u ? kCostBins - 1 : min(kCostBins - 1, int(__clz(c))); }
+
+__global__ __launch_bounds__(1024) void list_order_bins(int64_t n, const int* __restrict__ nbd,
+                                                        const unsigned* __restrict__ cost, int* __restrict__ order) {
+    __shared__ unsigned cnt[kCostBins], pos[kCostBins];
+    if (threadIdx.x < kCostBins) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t nb = min<int64_t>(n, *nbd);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[i < nb ? cost_bin(cost[i]) : kCostBins - 1], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned s = 0;
+        for (int b = 0; b < kCostBins; ++b) {
+            pos[b] = s;
+            s += cnt[b];
+        }
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        order[atomicAdd(&pos[i < nb ? cost_bin(cost[i]) : kCostBins - 1], 1u)] = int(i);
 }
 
 __global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ perm, int64_t n,
@@ -1104,16 +1155,9 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     // if one did not, grow the lists to what it needed and do it again
     for (int attempt = 0;; ++attempt) {
         const int st_cap = sv.on ? c->st_cap : 0;
-        size_t sort_bytes = 0;
-        {
-            unsigned* k0 = nullptr;
-            int* v0 = nullptr;
-            LPM_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, k0, k0, v0, v0, int(nbatch),
-                                                                    0, 32, st));
-        }
         const size_t bytes =
-            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + 4) * sizeof(int) + 64 +
-            size_t(3 * nt) * sizeof(double) + sort_bytes + 256;
+            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + 2) * sizeof(int) + 64 +
+            size_t(3 * nt) * sizeof(double) + 256;
         LPM_CHECK(c->tl_work.reserve(bytes));
         double* acc = c->tl_work.as<double>();
         Lists ls;
@@ -1125,32 +1169,26 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         ls.near_cnt = ls.far_hi_cnt + nbatch;
         ls.st_cnt = ls.near_cnt + nbatch;
         ls.overflow = ls.st_cnt + nbatch;
-        unsigned* cost_out = reinterpret_cast<unsigned*>(ls.overflow + 16);
-        ls.cost = cost_out + nbatch;
-        int* order_in = reinterpret_cast<int*>(ls.cost + nbatch);
-        int* order = order_in + nbatch;
-        void* sort_tmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(order + nbatch) + 255) & ~uintptr_t(255));
+        ls.cost = reinterpret_cast<unsigned*>(ls.overflow + 16);
+        int* order = reinterpret_cast<int*>(ls.cost + nbatch);
         ls.order = order;
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
         ls.st_cap = st_cap;
         LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 4 * sizeof(int), st));
         if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
-        hipLaunchKernelGGL(list_order_init, dim3(unsigned((nbatch + 255) / 256)), dim3(256), 0, st, nbatch, ls.cost,
-                           order_in);
         if (sv.on)
             hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
         else
             hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
-        {
-            size_t sb = sort_bytes;
-            LPM_HIP(c, hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sb, ls.cost, cost_out, order_in, order,
-                                                                    int(nbatch), 0, 32, st));
-        }
+        hipLaunchKernelGGL(list_order_bins, dim3(1), dim3(1024), 0, st, nbatch, tg.nbd, ls.cost, order);
+        // with no order above kLowOrder the low-order kernel writes the result
+        const bool direct_out = max_order <= kLowOrder;
         auto lists = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc, mult,
+                               direct_out ? ux : nullptr, uy, uz);
         };
         // the timed launch is the low-order list kernel, the one that dominates
         mark_begin(c);
@@ -1167,8 +1205,9 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
                 else lists(tree_eval_lists<12, false, false>);
             }
         }
-        hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult,
-                           ux, uy, uz);
+        if (!direct_out)
+            hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc,
+                               mult, ux, uy, uz);
         LPM_HIP(c, hipGetLastError());
         LPM_HIP(c, hipMemcpyAsync(c->pin, ls.overflow, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         LPM_HIP(c, hipStreamSynchronize(st));
