@@ -175,29 +175,29 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
     const float* b2 = b1 + kHid;
     const float* b3 = b2 + kHid;
     bf16x4 h1[4], h2[4];
+    // the biases enter as the accumulators' initial values (the MFMA result
+    // layout is the bias layout: lane (g, c) holds outputs 4g..4g+3)
+    auto bias = [&](const float* bv) {
+        const float4 b = *reinterpret_cast<const float4*>(bv + 4 * g);
+        return f32x4{b.x, b.y, b.z, b.w};
+    };
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(as_bf16x4(f1[nt * 64 + lane]), x, acc);
-        const float4 b = *reinterpret_cast<const float4*>(b1 + nt * 16 + 4 * g);
-        h1[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
-                            fmaxf(acc[3] + b.w, 0.f));
+        const f32x4 acc = mfma16(as_bf16x4(f1[nt * 64 + lane]), x, bias(b1 + nt * 16));
+        h1[nt] = cvt_bf16x4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
     }
     const bf16x8 k1[2] = {join_bf16(h1[0], h1[1]), join_bf16(h1[2], h1[3])};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = bias(b2 + nt * 16);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) acc = mfma32(f2[(2 * nt + kb) * 64 + lane], k1[kb], acc);
-        const float4 b = *reinterpret_cast<const float4*>(b2 + nt * 16 + 4 * g);
-        h2[nt] = cvt_bf16x4(fmaxf(acc[0] + b.x, 0.f), fmaxf(acc[1] + b.y, 0.f), fmaxf(acc[2] + b.z, 0.f),
-                            fmaxf(acc[3] + b.w, 0.f));
+        h2[nt] = cvt_bf16x4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
     }
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc = bias(b3);
     acc = mfma32(f3[lane], join_bf16(h2[0], h2[1]), acc);
     acc = mfma32(f3[64 + lane], join_bf16(h2[2], h2[3]), acc);
-    const float4 b = *reinterpret_cast<const float4*>(b3 + 4 * g);
-    const float z[4] = {acc[0] + b.x, acc[1] + b.y, acc[2] + b.z, acc[3] + b.w};
+    const float z[4] = {acc[0], acc[1], acc[2], acc[3]};
     float v = -3.0e38f;
     int best = 0;
 #pragma unroll

@@ -721,14 +721,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
 // order <= kLowOrder, written to acc[3][nt] (sorted positions, FP64); else
 // the far entries of order kLowOrder + 1 .. MAXP, added to acc.  The order is
 // wave-uniform per entry.
-// Output: acc[3][nt] at the sorted positions (LOW writes, else adds), or,
-// when ux is not NULL (LOW with no high-order pass), the scaled velocity
-// straight to ux/uy/uz at the target's own index.
+// Output: acc[3][nt] at the sorted positions (LOW writes, else adds): a
+// wave's 64 targets are consecutive there, so its stores fill whole lines
+// (tree_list_finish then scatters to the targets' own indices).
 template <int MAXP, bool LOW, bool F32, bool STEREO = false>
 __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, Lists ls, StereoView sv, double d2,
-                                                       float eps2v, double* __restrict__ acc, double mult,
-                                                       double* __restrict__ ux, double* __restrict__ uy,
-                                                       double* __restrict__ uz) {
+                                                       float eps2v, double* __restrict__ acc) {
     const int64_t wi = wave_index();
     if (wi >= tg.nb) return;
     const int64_t bi = __builtin_amdgcn_readfirstlane(ls.order[wi]);
@@ -794,11 +792,7 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
         }
     }
     if (valid) {
-        if (LOW && ux) {
-            ux[orig] = s0 * mult;
-            uy[orig] = s1 * mult;
-            uz[orig] = s2 * mult;
-        } else if (LOW) {
+        if (LOW) {
             acc[pos] = s0;
             acc[tg.n + pos] = s1;
             acc[2 * tg.n + pos] = s2;
@@ -819,13 +813,28 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
 constexpr int kCostBins = 32;
 __device__ inline int cost_bin(unsigned c) { return c == 0u ? kCostBins - 1 : min(kCostBins - 1, int(__clz(c))); }
 
+// The lanes of a wave are grouped by bin with ballots, so that each
+// distinct bin of a wave takes one LDS atomic (the costs crowd into a few
+// bins).  The loop bounds are wave-uniform.
 __global__ __launch_bounds__(1024) void list_order_bins(int64_t n, const int* __restrict__ nbd,
                                                         const unsigned* __restrict__ cost, int* __restrict__ order) {
     __shared__ unsigned cnt[kCostBins], pos[kCostBins];
     if (threadIdx.x < kCostBins) cnt[threadIdx.x] = 0u;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
     const int64_t nb = min<int64_t>(n, *nbd);
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[i < nb ? cost_bin(cost[i]) : kCostBins - 1], 1u);
+    for (int64_t w0 = threadIdx.x & ~63; w0 < n; w0 += blockDim.x) {
+        const int64_t i = w0 + lane;
+        const int bin = i < n ? (i < nb ? cost_bin(cost[i]) : kCostBins - 1) : -1;
+        uint64_t left = __ballot(bin >= 0);
+        while (left) {
+            const int u = __builtin_amdgcn_readlane(bin, __builtin_ctzll(left));
+            const uint64_t m = __ballot(bin == u);
+            if (lane == __builtin_ctzll(m)) atomicAdd(&cnt[u], unsigned(__popcll(m)));
+            left &= ~m;
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned s = 0;
@@ -835,8 +844,21 @@ __global__ __launch_bounds__(1024) void list_order_bins(int64_t n, const int* __
         }
     }
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-        order[atomicAdd(&pos[i < nb ? cost_bin(cost[i]) : kCostBins - 1], 1u)] = int(i);
+    for (int64_t w0 = threadIdx.x & ~63; w0 < n; w0 += blockDim.x) {
+        const int64_t i = w0 + lane;
+        const int bin = i < n ? (i < nb ? cost_bin(cost[i]) : kCostBins - 1) : -1;
+        uint64_t left = __ballot(bin >= 0);
+        while (left) {
+            const int u = __builtin_amdgcn_readlane(bin, __builtin_ctzll(left));
+            const uint64_t m = __ballot(bin == u);
+            const int leader = __builtin_ctzll(m);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&pos[u], unsigned(__popcll(m)));
+            base = __shfl(base, leader, 64);
+            if (bin == u) order[base + __popcll(m & below)] = int(i);
+            left &= ~m;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ perm, int64_t n,
@@ -1184,11 +1206,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
             hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
         hipLaunchKernelGGL(list_order_bins, dim3(1), dim3(1024), 0, st, nbatch, tg.nbd, ls.cost, order);
-        // with no order above kLowOrder the low-order kernel writes the result
-        const bool direct_out = max_order <= kLowOrder;
         auto lists = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc, mult,
-                               direct_out ? ux : nullptr, uy, uz);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
         };
         // the timed launch is the low-order list kernel, the one that dominates
         mark_begin(c);
@@ -1205,9 +1224,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
                 else lists(tree_eval_lists<12, false, false>);
             }
         }
-        if (!direct_out)
-            hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc,
-                               mult, ux, uy, uz);
+        hipLaunchKernelGGL(tree_list_finish, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tperm, nt, acc, mult,
+                           ux, uy, uz);
         LPM_HIP(c, hipGetLastError());
         LPM_HIP(c, hipMemcpyAsync(c->pin, ls.overflow, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         LPM_HIP(c, hipStreamSynchronize(st));
