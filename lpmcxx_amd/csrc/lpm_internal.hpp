@@ -41,6 +41,23 @@ constexpr int kStatSlots = 32;
 constexpr int kStatStride = 16;  // 64-bit words per copy
 constexpr size_t kStatBytes = size_t(kStatSlots) * kStatStride * sizeof(unsigned long long);
 
+// Words that follow the evaluation counters in device memory and are cleared
+// together with them (one fill per evaluation): the target batching's
+// histogram of adjacent key prefixes and its batch count, the interaction
+// lists' overflow flag and needed capacities, and the batches per cost bin of
+// the learned evaluation.
+struct BatchHeader {
+    unsigned hist[32];
+    int nsel;
+    int overflow[4];
+    unsigned bins[32];
+};
+constexpr size_t kHeaderBytes = 512;
+static_assert(sizeof(BatchHeader) <= kHeaderBytes, "batch header");
+inline BatchHeader* batch_header(unsigned long long* stats) {
+    return reinterpret_cast<BatchHeader*>(reinterpret_cast<char*>(stats) + kStatBytes);
+}
+
 }  // namespace lpm
 
 struct lpm_ctx {
@@ -150,19 +167,21 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
                      const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz,
                      int64_t* mac_out, double radius);
 // Morton order of the targets in the tree's root box (tperm: sorted position
-// -> target index, device); also clears the evaluation counters.  *levels
-// (nullable) receives the number of top key levels by which the sorted keys
-// are ordered: below them, neighbouring keys keep the input order.
+// -> target index, device; *keys, nullable: the sorted keys); also clears the
+// evaluation counters and the batch header.  *levels (nullable) receives the
+// number of top key levels by which the sorted keys are ordered: below them,
+// neighbouring keys keep the input order.  Targets that are the tree's
+// sources use the build's sorted keys and order in place (read only).
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
-                          int** tperm, int* levels = nullptr);
+                          const int** tperm, int* levels = nullptr, const uint64_t** keys = nullptr);
 // Cell-aligned target batches: the Morton-sorted targets split at the
 // boundaries of the octree cells of level L (the shallowest level with at
 // least nt / batch_cell_targets occupied cells, and at most the number of
 // key levels the sort ordered) and cut into runs of at most 64.  Enqueued without waiting: bstart[b] = first sorted position of batch
 // b, bstart[nb] = nt; the count nb stays in device memory (*nb_dev) and
 // *nb_bound >= nb bounds it for launch sizes.
-int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
-                        int** bstart, int64_t* nb_bound, const int** nb_dev);
+int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                        const int** tperm, int** bstart, int64_t* nb_bound, const int** nb_dev);
 // waits for the batches and reads their count
 int read_batch_count(lpm_ctx* c, const int* nb_dev, int64_t* nb);
 // batch-MAC treecode (treecode_batch.hip): prec LPM_F64, or LPM_F32C with

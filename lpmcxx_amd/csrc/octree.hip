@@ -447,6 +447,10 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     int sorted_levels = std::min(kFastLevels, kMaxDepth);
     int64_t nn = 1;
     int* blk = off;  // per-block child counts (at most one word per 32 nodes)
+    // The first run of levels stops at the depth of the previous build (a
+    // rebuild of moved particles mostly keeps it): levels below it would be
+    // launched empty.  A deeper tree continues in the next run.
+    int run_cap = t->depth > 0 ? t->depth : kMaxDepth;
     for (;;) {
         if (n > 0)
             LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->keys.as<uint64_t>(), t->tmp1.as<int>(),
@@ -464,7 +468,9 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             // the bound of each level's children
             int G = 0;
             int64_t need = nn, b = ck;
-            while (lk + G <= last) {
+            const int run_last = std::min(last, run_cap);
+            run_cap = kMaxDepth;
+            while (lk + G <= run_last) {
                 const int64_t nb = std::min<int64_t>(split_bound, 8 * b);
                 if (G > 0 && need + nb > kBuildNodeBudget) break;
                 need += nb;

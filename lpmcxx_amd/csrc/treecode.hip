@@ -508,40 +508,136 @@ __global__ __launch_bounds__(256) void batch_lcp_hist(const uint64_t* __restrict
     if (threadIdx.x <= kMaxDepth && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// cs[i] = i where a level-L cell starts, else 0 (then a running max gives the cell start)
-__global__ __launch_bounds__(256) void batch_cell_start(const uint64_t* __restrict__ k, int64_t n,
-                                                        const int* __restrict__ level, int* __restrict__ cs) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int L = *level;
-    if (i < n) cs[i] = (i == 0 || key_lcp(k[i - 1], k[i]) < L) ? int(i) : 0;
-}
-
-__global__ __launch_bounds__(256) void batch_flags(const int* __restrict__ cs, int64_t n,
-                                                   unsigned char* __restrict__ f) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) f[i] = ((i - cs[i]) % kWave) == 0;
-}
-
 // The batching level L: the shallowest level with at least `want` occupied
 // cells (hist[l] = adjacent sorted keys whose common prefix is l levels), and
 // at most max_level, the key levels by which the keys are sorted.  Up to
 // max_level every cell is one run of the sorted keys, so the cells of level L
 // number fewer than 8 want (fewer than want when max_level caps L).  Below
 // it, neighbouring keys are in input order and a cell would fall apart into
-// runs of single targets.  One thread.
-__global__ void batch_level(const unsigned* __restrict__ hist, int64_t want, int max_level,
-                            int* __restrict__ level) {
+// runs of single targets.
+__device__ inline int batching_level(const unsigned* hist, int64_t want, int max_level) {
     int L = max_level;
     int64_t cells = 1;
     for (int l = 0; l < max_level; ++l) {
         if (cells >= want) { L = l; break; }
         cells += hist[l];
     }
-    *level = L;
+    return L;
 }
 
-// bstart[nb] = n closes the last batch.  One thread.
-__global__ void batch_close(int* __restrict__ bs, const int* __restrict__ nb, int n) { bs[*nb] = n; }
+// Top L levels of a 63-bit key (L <= kMaxDepth).
+__device__ inline uint64_t key_prefix(uint64_t k, int L) { return L == 0 ? 0 : k >> (3 * (kMaxDepth - L)); }
+
+// Batch starts of one tile of kBatchTile sorted positions, kBatchPer
+// consecutive positions per thread: a position starts a batch when it lies a
+// multiple of 64 after the start of its level-L cell.  The cell start carried
+// into the tile is the first position with the tile's first prefix (prefixes
+// of at most the sorted levels are non-decreasing), found by a 256-way search
+// (a few rounds of one key per thread); inside the tile a running maximum of
+// the cell starts continues it.  Returns the thread's flags (bit j: position
+// first + j).
+constexpr int kBatchPer = 8;
+constexpr int kBatchTile = 256 * kBatchPer;
+using BatchScan = hipcub::BlockScan<int, 256>;
+
+__device__ inline unsigned batch_tile_flags(const uint64_t* __restrict__ k, int n, int L, int& first,
+                                            BatchScan::TempStorage& tmp) {
+    const int t0 = blockIdx.x * kBatchTile;
+    first = t0 + threadIdx.x * kBatchPer;
+    // the first position in [lo, hi] whose prefix is not below p0 (t0 is one);
+    // every thread holds the same lo and hi, so the loop is uniform
+    const uint64_t p0 = key_prefix(k[t0], L);
+    int lo = 0, hi = t0;
+    while (lo < hi) {
+        const int step = (hi - lo + 255) / 256;
+        const int m = lo + int(threadIdx.x) * step;
+        // the probes below p0 are the first cnt (prefixes are non-decreasing)
+        const int cnt = __syncthreads_count(m < hi && key_prefix(k[m], L) < p0);
+        if (cnt == 0) {
+            hi = lo;
+        } else {
+            hi = min(hi, lo + cnt * step);
+            lo = lo + (cnt - 1) * step + 1;
+        }
+    }
+    const int carry = lo;
+    int st[kBatchPer];
+    int last = -1;
+    uint64_t prev = (first >= 1 && first <= n) ? key_prefix(k[first - 1], L) : 0;
+#pragma unroll
+    for (int j = 0; j < kBatchPer; ++j) {
+        const int i = first + j;
+        st[j] = -1;
+        if (i < n) {
+            const uint64_t cur = key_prefix(k[i], L);
+            if (i == 0 || cur != prev) st[j] = i;
+            prev = cur;
+        }
+        last = max(last, st[j]);
+    }
+    int run = 0;
+    BatchScan(tmp).ExclusiveScan(last, run, carry, hipcub::Max());
+    unsigned flags = 0;
+#pragma unroll
+    for (int j = 0; j < kBatchPer; ++j) {
+        const int i = first + j;
+        run = max(run, st[j]);
+        if (i < n && ((i - run) & (kWave - 1)) == 0) flags |= 1u << j;
+    }
+    return flags;
+}
+
+// Batch count of every tile.
+__global__ __launch_bounds__(256) void batch_tile_count(const uint64_t* __restrict__ k, int n, int64_t want,
+                                                        int max_level, const BatchHeader* __restrict__ hdr,
+                                                        int* __restrict__ tcnt) {
+    __shared__ BatchScan::TempStorage tmp;
+    const int L = batching_level(hdr->hist, want, max_level);
+    int first;
+    const unsigned flags = batch_tile_flags(k, n, L, first, tmp);
+    __syncthreads();  // tmp is reused
+    int ex = 0, tot = 0;
+    BatchScan(tmp).ExclusiveSum(__popc(flags), ex, tot);
+    if (threadIdx.x == 0) tcnt[blockIdx.x] = tot;
+}
+
+// Exclusive sums of the tile counts (in place), the batch count, and
+// bstart[nb] = n.  One block.
+__global__ __launch_bounds__(256) void batch_tile_scan(int ntile, int n, BatchHeader* __restrict__ hdr,
+                                                       int* __restrict__ tcnt, int* __restrict__ bs) {
+    __shared__ BatchScan::TempStorage tmp;
+    int run = 0;
+    for (int b0 = 0; b0 < ntile; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < ntile ? tcnt[b] : 0;
+        int ex = 0, sum = 0;
+        BatchScan(tmp).ExclusiveSum(v, ex, sum);
+        if (b < ntile) tcnt[b] = run + ex;
+        run += sum;
+        __syncthreads();  // tmp is reused
+    }
+    if (threadIdx.x == 0) {
+        hdr->nsel = run;
+        bs[run] = n;
+    }
+}
+
+// Writes the batch starts of every tile at its offset.
+__global__ __launch_bounds__(256) void batch_tile_emit(const uint64_t* __restrict__ k, int n, int64_t want,
+                                                       int max_level, const BatchHeader* __restrict__ hdr,
+                                                       const int* __restrict__ tcnt, int* __restrict__ bs) {
+    __shared__ BatchScan::TempStorage tmp;
+    const int L = batching_level(hdr->hist, want, max_level);
+    int first;
+    const unsigned flags = batch_tile_flags(k, n, L, first, tmp);
+    __syncthreads();  // tmp is reused
+    int ex = 0;
+    BatchScan(tmp).ExclusiveSum(__popc(flags), ex);
+    int o = tcnt[blockIdx.x] + ex;
+#pragma unroll
+    for (int j = 0; j < kBatchPer; ++j)
+        if (flags & (1u << j)) bs[o++] = first + j;
+}
 
 // ---------------------------------------------------------------- evaluation
 __device__ inline uint64_t uniform64(uint64_t v) {
@@ -814,26 +910,29 @@ int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
 constexpr int kTargetLevels = 12;  // key levels that order the targets
 
 int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
-                          int** tperm, int* levels) {
+                          const int** tperm, int* levels, const uint64_t** keys) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     const int64_t na = std::max<int64_t>(nt, 1);
-    LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + kStatBytes + 128));
+    LPM_CHECK(c->ds_tgt.reserve(size_t(na) * (sizeof(uint64_t) * 2 + sizeof(int) * 2) + kStatBytes + kHeaderBytes +
+                                128));
     uint64_t* k_in = c->ds_tgt.as<uint64_t>();
     uint64_t* k_out = k_in + na;
     int* v_in = reinterpret_cast<int*>(k_out + na);
     int* v_out = v_in + na;
     const uintptr_t sa = (reinterpret_cast<uintptr_t>(v_out + na) + 127) & ~uintptr_t(127);
     c->stats_ptr = reinterpret_cast<unsigned long long*>(sa);
-    LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
+    // the counters and the batch header in one fill
+    LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes + kHeaderBytes, st));
     *tperm = v_out;
+    if (keys) *keys = k_out;
     if (levels) *levels = kTargetLevels;
     if (nt == 0) return LPM_OK;
     if (nt == t->ns && tx == t->sx && ty == t->sy && tz == t->sz) {
-        // the targets are the tree's sources: their keys are sorted already
+        // the targets are the tree's sources: the build's keys are sorted already
         if (levels) *levels = t->sorted_levels;
-        LPM_HIP(c, hipMemcpyAsync(k_out, t->keys.ptr, size_t(nt) * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-        LPM_HIP(c, hipMemcpyAsync(v_out, t->kperm.ptr, size_t(nt) * sizeof(int), hipMemcpyDeviceToDevice, st));
+        *tperm = t->kperm.as<int>();
+        if (keys) *keys = t->keys.as<uint64_t>();
         return LPM_OK;
     }
     hipLaunchKernelGGL(target_keys, dim3(blocks(nt)), dim3(256), 0, st, tx, ty, tz, nt, t->root_box[0], t->root_box[1],
@@ -851,53 +950,36 @@ int sort_targets_for_tree(lpm_ctx* c, const double* tx, const double* ty, const 
     return LPM_OK;
 }
 
-int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt, int** tperm,
-                        int** bstart, int64_t* nb_bound, const int** nb_dev) {
+int tree_target_batches(lpm_ctx* c, const double* tx, const double* ty, const double* tz, int64_t nt,
+                        const int** tperm, int** bstart, int64_t* nb_bound, const int** nb_dev) {
     int sorted_levels = 0;
-    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm, &sorted_levels));
+    const uint64_t* keys = nullptr;
+    LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, tperm, &sorted_levels, &keys));
     *nb_bound = 0;
+    BatchHeader* hdr = batch_header(c->stats_ptr);
+    *nb_dev = &hdr->nsel;
     if (nt == 0) return LPM_OK;
     hipStream_t st = c->stream;
-    const uint64_t* keys = c->ds_tgt.as<uint64_t>() + nt;  // sorted keys (k_out)
-    // workspace: hist[32] | L | nsel | cs[nt] | flags[nt] | bstart[nt + 1] | scan/select temp
-    size_t scan_bytes = 0, sel_bytes = 0;
-    int* cs0 = nullptr;
-    LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(nullptr, scan_bytes, cs0, cs0, hipcub::Max(), int(nt), st));
-    LPM_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<int>(0),
-                                             static_cast<unsigned char*>(nullptr), cs0, static_cast<int*>(nullptr),
-                                             int(nt), st));
-    const size_t tmp_bytes = std::max(scan_bytes, sel_bytes) + 256;
-    const size_t off_cs = 256, off_f = off_cs + size_t(nt) * 4, off_b = off_f + ((size_t(nt) + 255) & ~size_t(255)),
-                 off_t = off_b + size_t(nt) * 4 + 256;
-    LPM_CHECK(c->tb_work.reserve(off_t + tmp_bytes));
-    char* w = c->tb_work.as<char>();
-    unsigned* hist = reinterpret_cast<unsigned*>(w);
-    int* level = reinterpret_cast<int*>(w + 128);
-    int* nsel = reinterpret_cast<int*>(w + 132);
-    int* cs = reinterpret_cast<int*>(w + off_cs);
-    unsigned char* f = reinterpret_cast<unsigned char*>(w + off_f);
-    int* bs = reinterpret_cast<int*>(w + off_b);
-    void* tmp = w + off_t;
+    // workspace: per-tile batch counts (then their exclusive sums) | bstart[nt + 1]
+    const int ntile = int((nt + kBatchTile - 1) / kBatchTile);
+    const size_t off_b = (size_t(ntile) * sizeof(int) + 255) & ~size_t(255);
+    LPM_CHECK(c->tb_work.reserve(off_b + size_t(nt + 1) * sizeof(int)));
+    int* tcnt = c->tb_work.as<int>();
+    int* bs = reinterpret_cast<int*>(c->tb_work.as<char>() + off_b);
     const int64_t want = std::max<int64_t>(1, nt / c->batch_cell_targets);
-    LPM_HIP(c, hipMemsetAsync(hist, 0, 128, st));
     hipLaunchKernelGGL(batch_lcp_hist, dim3(unsigned(std::min<int64_t>(kHistBlocks, blocks(nt)))), dim3(256), 0, st,
-                       keys, nt, hist);
-    hipLaunchKernelGGL(batch_level, dim3(1), dim3(1), 0, st, hist, want, sorted_levels, level);
-    hipLaunchKernelGGL(batch_cell_start, dim3(blocks(nt)), dim3(256), 0, st, keys, nt, level, cs);
-    size_t b1 = tmp_bytes;
-    LPM_HIP(c, hipcub::DeviceScan::InclusiveScan(tmp, b1, cs, cs, hipcub::Max(), int(nt), st));
-    hipLaunchKernelGGL(batch_flags, dim3(blocks(nt)), dim3(256), 0, st, cs, nt, f);
-    size_t b2 = tmp_bytes;
-    LPM_HIP(c, hipcub::DeviceSelect::Flagged(tmp, b2, hipcub::CountingInputIterator<int>(0), f, bs, nsel, int(nt),
-                                             st));
-    hipLaunchKernelGGL(batch_close, dim3(1), dim3(1), 0, st, bs, nsel, int(nt));
+                       keys, nt, hdr->hist);
+    hipLaunchKernelGGL(batch_tile_count, dim3(unsigned(ntile)), dim3(256), 0, st, keys, int(nt), want, sorted_levels,
+                       static_cast<const BatchHeader*>(hdr), tcnt);
+    hipLaunchKernelGGL(batch_tile_scan, dim3(1), dim3(256), 0, st, ntile, int(nt), hdr, tcnt, bs);
+    hipLaunchKernelGGL(batch_tile_emit, dim3(unsigned(ntile)), dim3(256), 0, st, keys, int(nt), want, sorted_levels,
+                       static_cast<const BatchHeader*>(hdr), static_cast<const int*>(tcnt), bs);
     LPM_HIP(c, hipGetLastError());
     // every level-L cell of s targets makes ceil(s / 64) batches; the level
     // above L has fewer than `want` cells, so L has fewer than 8 want (L is
     // never below the sorted key levels, where that would not hold)
     *bstart = bs;
     *nb_bound = std::min<int64_t>(nt, (nt + kWave - 1) / kWave + 8 * want);
-    *nb_dev = nsel;
     return LPM_OK;
 }
 
@@ -936,7 +1018,7 @@ int tree_eval_device(lpm_ctx* c, int prec, int series, double theta, double delt
                      int64_t* mac_out, double radius) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
-    int* v_out = nullptr;
+    const int* v_out = nullptr;
     LPM_CHECK(sort_targets_for_tree(c, tx, ty, tz, nt, &v_out));
     if (nt == 0) { if (mac_out) *mac_out = 0; return LPM_OK; }
     unsigned long long* d_mac = c->stats_ptr;

@@ -500,14 +500,44 @@ struct Lists {
     int* st_cnt;
     int far_cap, near_cap, st_cap;
     int* overflow;  // [0] a batch overflowed, [1] far, [2] near and [3] stereographic capacity it needs
-    unsigned* cost;    // estimated vector instructions of each batch's lists (planning pass)
-    const int* order;  // batches by decreasing cost: wave w of the list kernels takes batch order[w]
+    // the batches by estimated cost (vector instructions of their lists):
+    // bins[k] batches in bin k, listed at queue[k * qcap ...]
+    unsigned* bins;
+    int* queue;
+    int64_t qcap;
 };
 
 // Vector instructions per near pair, Cartesian term, Cartesian entry,
 // stereographic entry and complex term: the list kernels' costs, used to run
 // the dearest batches first so that the cheap ones fill the end of the launch.
 constexpr unsigned kCostNear = 14, kCostTerm = 5, kCostFar = 35, kCostSt = 24, kCostStTerm = 4;
+
+// The waves of the list kernels take the batches dearest first (an LPT
+// order): the planning pass files each batch under one of 32 bins of
+// log2(its cost estimate), and wave w of a list kernel takes the w-th batch
+// counting from the dearest bin.  Within a bin the order is that of the
+// filing atomics: it only decides which wave evaluates which batch, not any
+// result.
+constexpr int kCostBins = 32;
+static_assert(kCostBins <= 32, "bins of the batch header");
+__device__ inline int cost_bin(unsigned c) { return c == 0u ? kCostBins - 1 : min(kCostBins - 1, int(__clz(c))); }
+
+// Batch of wave w (wave-uniform), or -1 past the last filed batch.
+__device__ inline int64_t list_batch(const Lists& ls, int64_t w) {
+    const int lane = threadIdx.x & 63;
+    const unsigned c = lane < kCostBins ? ls.bins[lane] : 0u;
+    unsigned incl = c;  // inclusive sum over the bins up to this lane's
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    // the bins that end at or before w are the first b (sums are non-decreasing)
+    const int b = __popcll(__ballot(int64_t(incl) <= w));
+    if (b >= kCostBins) return -1;
+    const unsigned before = __shfl(incl - c, b, 64);
+    return __builtin_amdgcn_readfirstlane(ls.queue[int64_t(b) * ls.qcap + (w - int64_t(before))]);
+}
 
 // six waves per SIMD (76 VGPRs, 78 with STEREO, no spills): the traversal
 // waits on node loads.  STEREO: accepted pairs may become stereographic
@@ -704,7 +734,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         ls.st_cnt[bi] = ovf ? 0 : nstl;
         const unsigned long long cost = kCostNear * near_pairs + kCostTerm * nterms +
                                         kCostFar * unsigned(nfar + nhi) + kCostSt * nst + kCostStTerm * nterms_st;
-        ls.cost[bi] = ovf ? 0u : unsigned(cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull);
+        const int bin = cost_bin(ovf ? 0u : unsigned(cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull));
+        ls.queue[int64_t(bin) * ls.qcap + atomicAdd(&ls.bins[bin], 1u)] = int(bi);
         if (ovf) {
             atomicOr(ls.overflow, 1);
             atomicMax(ls.overflow + 1, nfar + nhi);
@@ -729,8 +760,8 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
                                                        float eps2v, double* __restrict__ acc) {
     const int64_t wi = wave_index();
     if (wi >= tg.nb) return;
-    const int64_t bi = __builtin_amdgcn_readfirstlane(ls.order[wi]);
-    if (!is_batch(tg, bi)) return;
+    const int64_t bi = list_batch(ls, wi);
+    if (bi < 0) return;
     const int* far = ls.far + bi * ls.far_cap;
     int j0 = 0, j1 = __builtin_amdgcn_readfirstlane(ls.far_cnt[bi]);
     if (!LOW) {
@@ -804,62 +835,6 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     }
 }
 
-// The waves of the list kernels take the batches dearest first (an LPT
-// order): a counting sort of the planning pass's cost estimates on 32 bins
-// of log2(cost), dearest bin first, in one block.  Wave indices at or past
-// the batch count (no batch) go last.  Within a bin the order is that of the
-// block's LDS atomics: it only decides which wave evaluates which batch, not
-// any result.
-constexpr int kCostBins = 32;
-__device__ inline int cost_bin(unsigned c) { return c == 0u ? kCostBins - 1 : min(kCostBins - 1, int(__clz(c))); }
-
-// The lanes of a wave are grouped by bin with ballots, so that each
-// distinct bin of a wave takes one LDS atomic (the costs crowd into a few
-// bins).  The loop bounds are wave-uniform.
-__global__ __launch_bounds__(1024) void list_order_bins(int64_t n, const int* __restrict__ nbd,
-                                                        const unsigned* __restrict__ cost, int* __restrict__ order) {
-    __shared__ unsigned cnt[kCostBins], pos[kCostBins];
-    if (threadIdx.x < kCostBins) cnt[threadIdx.x] = 0u;
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const uint64_t below = (1ull << lane) - 1ull;
-    const int64_t nb = min<int64_t>(n, *nbd);
-    for (int64_t w0 = threadIdx.x & ~63; w0 < n; w0 += blockDim.x) {
-        const int64_t i = w0 + lane;
-        const int bin = i < n ? (i < nb ? cost_bin(cost[i]) : kCostBins - 1) : -1;
-        uint64_t left = __ballot(bin >= 0);
-        while (left) {
-            const int u = __builtin_amdgcn_readlane(bin, __builtin_ctzll(left));
-            const uint64_t m = __ballot(bin == u);
-            if (lane == __builtin_ctzll(m)) atomicAdd(&cnt[u], unsigned(__popcll(m)));
-            left &= ~m;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned s = 0;
-        for (int b = 0; b < kCostBins; ++b) {
-            pos[b] = s;
-            s += cnt[b];
-        }
-    }
-    __syncthreads();
-    for (int64_t w0 = threadIdx.x & ~63; w0 < n; w0 += blockDim.x) {
-        const int64_t i = w0 + lane;
-        const int bin = i < n ? (i < nb ? cost_bin(cost[i]) : kCostBins - 1) : -1;
-        uint64_t left = __ballot(bin >= 0);
-        while (left) {
-            const int u = __builtin_amdgcn_readlane(bin, __builtin_ctzll(left));
-            const uint64_t m = __ballot(bin == u);
-            const int leader = __builtin_ctzll(m);
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(&pos[u], unsigned(__popcll(m)));
-            base = __shfl(base, leader, 64);
-            if (bin == u) order[base + __popcll(m & below)] = int(i);
-            left &= ~m;
-        }
-    }
-}
 
 __global__ __launch_bounds__(256) void tree_list_finish(const int* __restrict__ perm, int64_t n,
                                                         const double* __restrict__ acc, double mult,
@@ -1073,7 +1048,8 @@ constexpr double kPi = 3.1415926535897932384626433832795027975;
 int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double delta, const double* tx,
                            const double* ty, const double* tz, int64_t nt, double* ux, double* uy, double* uz) {
     Tree* t = c->tree;
-    int *tperm = nullptr, *bstart = nullptr;
+    const int* tperm = nullptr;
+    int* bstart = nullptr;
     int64_t nbatch = 0;
     const int* nbd = nullptr;
     LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch, &nbd));
@@ -1108,7 +1084,8 @@ int tree_eval_batch_device(lpm_ctx* c, int prec, int order, double theta, double
 int tree_eval_psi_device(lpm_ctx* c, int prec, int order, double theta, const double* tx, const double* ty,
                          const double* tz, int64_t nt, double* psi) {
     Tree* t = c->tree;
-    int *tperm = nullptr, *bstart = nullptr;
+    const int* tperm = nullptr;
+    int* bstart = nullptr;
     int64_t nbatch = 0;
     const int* nbd = nullptr;
     LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch, &nbd));
@@ -1146,7 +1123,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     if (t->nn >= (int64_t(1) << kNodeBits)) return fail(c, LPM_ERR_INVALID_ARG, "too many tree nodes");
-    int *tperm = nullptr, *bstart = nullptr;
+    const int* tperm = nullptr;
+    int* bstart = nullptr;
     int64_t nbatch = 0;
     const int* nbd = nullptr;
     LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch, &nbd));
@@ -1178,8 +1156,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     for (int attempt = 0;; ++attempt) {
         const int st_cap = sv.on ? c->st_cap : 0;
         const size_t bytes =
-            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + 2) * sizeof(int) + 64 +
-            size_t(3 * nt) * sizeof(double) + 256;
+            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + kCostBins) *
+                sizeof(int) + size_t(3 * nt) * sizeof(double) + 256;
         LPM_CHECK(c->tl_work.reserve(bytes));
         double* acc = c->tl_work.as<double>();
         Lists ls;
@@ -1190,22 +1168,27 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         ls.far_hi_cnt = ls.far_cnt + nbatch;
         ls.near_cnt = ls.far_hi_cnt + nbatch;
         ls.st_cnt = ls.near_cnt + nbatch;
-        ls.overflow = ls.st_cnt + nbatch;
-        ls.cost = reinterpret_cast<unsigned*>(ls.overflow + 16);
-        int* order = reinterpret_cast<int*>(ls.cost + nbatch);
-        ls.order = order;
+        // the overflow words and the cost bins sit in the batch header,
+        // cleared with the counters
+        BatchHeader* hdr = batch_header(c->stats_ptr);
+        ls.overflow = hdr->overflow;
+        ls.bins = hdr->bins;
+        ls.queue = ls.st_cnt + nbatch;
+        ls.qcap = nbatch;
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
         ls.st_cap = st_cap;
-        LPM_HIP(c, hipMemsetAsync(ls.overflow, 0, 4 * sizeof(int), st));
-        if (attempt > 0) LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
+        if (attempt > 0) {
+            LPM_HIP(c, hipMemsetAsync(hdr->overflow, 0, sizeof(hdr->overflow), st));
+            LPM_HIP(c, hipMemsetAsync(hdr->bins, 0, sizeof(hdr->bins), st));
+            LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
+        }
         if (sv.on)
             hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
         else
             hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
-        hipLaunchKernelGGL(list_order_bins, dim3(1), dim3(1024), 0, st, nbatch, tg.nbd, ls.cost, order);
         auto lists = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
         };
@@ -1267,7 +1250,8 @@ static int tree_labels_device(lpm_ctx* c, double theta_max, double delta, double
                               int64_t* n_out) {
     Tree* t = c->tree;
     hipStream_t st = c->stream;
-    int *tperm = nullptr, *bstart = nullptr;
+    const int* tperm = nullptr;
+    int* bstart = nullptr;
     int64_t nbatch = 0;
     const int* nbd = nullptr;
     LPM_CHECK(tree_target_batches(c, tx, ty, tz, nt, &tperm, &bstart, &nbatch, &nbd));
