@@ -323,9 +323,10 @@ __device__ inline double uniform_d(double v) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 
-__device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t0, double t1, double t2) {
+// The batch in projection proj (wave-uniform).
+__device__ inline StereoBatch stereo_batch_in(int proj, const Batch& bt, bool valid, double t0, double t1, double t2) {
     StereoBatch s;
-    s.proj = __builtin_amdgcn_readfirstlane(bt.cz >= 0.0 ? 0 : 1);
+    s.proj = proj;
     const bool good = stereo_point(s.proj, t0, t1, t2, s.zr, s.zi) || !valid;
     s.ok = __ballot(!good) == 0ull;
     const double inv = 1.0 / double(bt.nvalid);
@@ -334,6 +335,29 @@ __device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t
     const double er = s.zr - s.cr, ei = s.zi - s.ci;
     s.r = uniform_d(sqrt(wmax(valid ? er * er + ei * ei : 0.0)));
     return s;
+}
+
+// The batch in its own projection: from the pole opposite its centre's
+// hemisphere, which keeps its targets within 90 degrees of the projection
+// centre.  A pair that fails the geometry test there may pass it in the other
+// projection (a node near the far pole is distorted in the first, near the
+// centre of the second).
+__device__ inline StereoBatch stereo_batch(const Batch& bt, bool valid, double t0, double t1, double t2) {
+    return stereo_batch_in(__builtin_amdgcn_readfirstlane(bt.cz >= 0.0 ? 0 : 1), bt, valid, t0, t1, t2);
+}
+
+// Stereographic geometry test of the pair (node n, batch sb): the plane ratio
+// no worse (by sv.ratio) than the sphere ratio (r_n + r_b) / s, and the plane
+// series convergent (radius below 0.95 of the distance); squared, so no FP64
+// division or square root (both sides are >= 0; rho = +inf fails both).
+__device__ inline bool stereo_fits(const StereoView& sv, const StereoBatch& sb, int n, double sd, double rn,
+                                   double rb) {
+    const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + n];
+    const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
+    const double dist2 = er * er + ei * ei;
+    const double rz = gz.z + sb.r;
+    const double lhs = rz * sd, rhs = sv.ratio * (rn + rb);
+    return lhs * lhs < rhs * rhs * dist2 && rz * rz < 0.9025 * dist2;
 }
 
 // Node data of one stereographic entry: plane centre and radius (FP64) and
@@ -497,7 +521,8 @@ struct Lists {
     int* far_cnt;     // low-order far entries (front)
     int* far_hi_cnt;  // high-order far entries (back)
     int* near_cnt;
-    int* st_cnt;
+    int* st_cnt;      // stereographic entries in the batch's own projection (front)
+    int* st_o_cnt;    // stereographic entries in the other projection (back)
     int far_cap, near_cap, st_cap;
     int* overflow;  // [0] a batch overflowed, [1] far, [2] near and [3] stereographic capacity it needs
     // the batches by estimated cost (vector instructions of their lists):
@@ -569,13 +594,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     double t0, t1, t2;
     load_target(tg, bi, valid, orig, t0, t1, t2);
     const Batch bt = batch_of(valid, t0, t1, t2);
-    StereoBatch sb{};
-    if (STEREO) sb = stereo_batch(bt, valid, t0, t1, t2);
-    const bool st_on = STEREO && sb.ok;
+    StereoBatch sb{}, so{};
+    if (STEREO) {
+        sb = stereo_batch(bt, valid, t0, t1, t2);
+        so = stereo_batch_in(1 - sb.proj, bt, valid, t0, t1, t2);
+    }
+    const bool st_on = STEREO && sb.ok, so_on = STEREO && so.ok;
     int* far = ls.far + bi * ls.far_cap;
     int* near = ls.near + bi * ls.near_cap;
     int* stl = ls.st + bi * ls.st_cap;
-    int nfar = 0, nhi = 0, nnear = 0, nstl = 0;
+    int nfar = 0, nhi = 0, nnear = 0, nstl = 0, nso = 0;
     unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0, nst = 0, nterms_st = 0;
     unsigned l_terms = 0, l_terms_hi = 0, l_np = 0, l_terms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -632,10 +660,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         const int chr = firstc + row - base;
         bool prop = false;
         bool near_chk = true;  // a leaf child's sources may come within 1e-6 of a target
-        // stereographic geometry test of the pair, taken before the model
-        // call so that its node load overlaps the features: the plane ratio
-        // no worse (by sv.ratio) than the sphere ratio (r_n + r_b) / s
-        bool st_geo = false;
+        // stereographic geometry test of the pair in the batch's projection,
+        // else in the other one, taken before the model call so that its node
+        // loads overlap the features
+        bool st_geo = false, so_geo = false;
         float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
         // every node-table word of the child is loaded here, in one round
         int ckids = 0, cfirst = 0, cb = 0, ce = 0;
@@ -651,16 +679,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             near_chk = !(sd - rn - bt.r > 1e-6);
             prop = rn + bt.r < theta_max * sd;
             if (prop) {
-                if (st_on) {
-                    // rz / dist < ratio (r_n + r_b) / s and rz < 0.95 dist, squared
-                    // (both sides are >= 0; rho = +inf fails both)
-                    const double4 gz = sv.sgeo[int64_t(sb.proj) * sv.nn + chr];
-                    const double er = sb.cr - gz.x, ei = sb.ci - gz.y;
-                    const double dist2 = er * er + ei * ei;
-                    const double rz = gz.z + sb.r;
-                    const double lhs = rz * sd, rhs = sv.ratio * (rn + bt.r);
-                    st_geo = lhs * lhs < rhs * rhs * dist2 && rz * rz < 0.9025 * dist2;
-                }
+                if (st_on) st_geo = stereo_fits(sv, sb, chr, sd, rn, bt.r);
+                if (!st_geo && so_on) so_geo = stereo_fits(sv, so, chr, sd, rn, bt.r);
                 pair_features_group(g, x0, x1, x2, x3, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
                 const float4 mu = s_norm[0][g], iv = s_norm[1][g], lo = s_norm[2][g], hi = s_norm[3][g];
                 x0 = (fminf(fmaxf(x0, lo.x), hi.x) - mu.x) * iv.x;
@@ -679,14 +699,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         const bool leaf = mine && ckids == 0;
         // an accepted pair that passes the stereographic geometry test
         // becomes a stereographic entry of order p + extra
-        const bool is_st = st_geo && mine && p > 0;
-        const int pz = is_st ? min(p + sv.extra, sv.maxp) : 0;
-        const bool is_lo = mine && p > 0 && p <= kLowOrder && !is_st, is_hi = mine && p > kLowOrder && !is_st;
+        const bool is_st = st_geo && mine && p > 0, is_so = so_geo && mine && p > 0;
+        const int pz = (is_st || is_so) ? min(p + sv.extra, sv.maxp) : 0;
+        const bool cart = mine && p > 0 && !is_st && !is_so;
+        const bool is_lo = cart && p <= kLowOrder, is_hi = cart && p > kLowOrder;
         const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
-        const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi), m_st = __ballot(is_st);
+        const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi), m_st = __ballot(is_st), m_so = __ballot(is_so);
         const uint64_t m_near = __ballot(is_near), m_desc = __ballot(is_desc);
         const int c_lo = __popcll(m_lo), c_hi = __popcll(m_hi), c_near = __popcll(m_near), c_desc = __popcll(m_desc);
-        const int c_st = __popcll(m_st);
+        const int c_st = __popcll(m_st), c_so = __popcll(m_so);
         // entries are written only while the batch's lists fit (else the
         // batch is reported as overflowing at the end)
         if (nfar + c_lo + nhi + c_hi <= ls.far_cap) {
@@ -694,23 +715,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             if (is_lo) far[nfar + __popcll(m_lo & below)] = e;
             if (is_hi) far[ls.far_cap - 1 - nhi - __popcll(m_hi & below)] = e;
         }
-        if (nstl + c_st <= ls.st_cap && is_st) stl[nstl + __popcll(m_st & below)] = chr | (pz << kNodeBits);
+        if (nstl + c_st + nso + c_so <= ls.st_cap) {
+            if (is_st) stl[nstl + __popcll(m_st & below)] = chr | (pz << kNodeBits);
+            if (is_so) stl[ls.st_cap - 1 - nso - __popcll(m_so & below)] = chr | (pz << kNodeBits);
+        }
         if (nnear + c_near <= ls.near_cap && is_near)
             near[nnear + __popcll(m_near & below)] = chr | (near_chk ? kNearCheck : 0);
         if (is_desc) s_node[w][sp + c_desc - 1 - __popcll(m_desc & below)] = stack_entry(cfirst, ckids);
         nfar += c_lo;
         nhi += c_hi;
         nstl += c_st;
+        nso += c_so;
         nnear += c_near;
         sp += c_desc;
         // terms of the accepted children and sources of the near leaves, kept
         // per lane (lanes 0..15) and summed over the wave after the walk
-        const unsigned t_all = ((is_lo || is_hi) && !is_st) ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
+        const unsigned t_all = cart ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
         l_terms += t_all;
         l_terms_hi += is_hi ? t_all : 0u;
         l_np += is_near ? unsigned(ce - cb) : 0u;
-        l_terms_st += is_st ? unsigned(pz + 1) : 0u;
-        nst += __popcll(__ballot(is_st));
+        l_terms_st += (is_st || is_so) ? unsigned(pz + 1) : 0u;
+        nst += unsigned(c_st + c_so);
         wave_sync();
     }
 #pragma unroll
@@ -727,11 +752,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     if (lane == 0) {
         // a batch whose lists do not fit is evaluated as empty and flagged with
         // the capacities it needs; the caller plans again with larger lists
-        const bool ovf = nfar + nhi > ls.far_cap || nnear > ls.near_cap || nstl > ls.st_cap;
+        const bool ovf = nfar + nhi > ls.far_cap || nnear > ls.near_cap || nstl + nso > ls.st_cap;
         ls.far_cnt[bi] = ovf ? 0 : nfar;
         ls.far_hi_cnt[bi] = ovf ? 0 : nhi;
         ls.near_cnt[bi] = ovf ? 0 : nnear;
         ls.st_cnt[bi] = ovf ? 0 : nstl;
+        ls.st_o_cnt[bi] = ovf ? 0 : nso;
         const unsigned long long cost = kCostNear * near_pairs + kCostTerm * nterms +
                                         kCostFar * unsigned(nfar + nhi) + kCostSt * nst + kCostStTerm * nterms_st;
         const int bin = cost_bin(ovf ? 0u : unsigned(cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull));
@@ -740,12 +766,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             atomicOr(ls.overflow, 1);
             atomicMax(ls.overflow + 1, nfar + nhi);
             atomicMax(ls.overflow + 2, nnear);
-            atomicMax(ls.overflow + 3, nstl);
+            atomicMax(ls.overflow + 3, nstl + nso);
         }
     }
     const unsigned long long nv = unsigned(bt.nvalid);
-    flush_stats(stats, (unsigned long long)(nfar + nhi + nstl) * nv, near_pairs * nv, nterms * nv, ncalls,
+    flush_stats(stats, (unsigned long long)(nfar + nhi + nstl + nso) * nv, near_pairs * nv, nterms * nv, ncalls,
                 (unsigned long long)nhi * nv, nterms_hi * nv, nst * nv, nterms_st * nv);
+}
+
+// The stereographic entries e[0 .. n) of a batch in projection sb.proj,
+// their velocity added to (s0, s1, s2): FP32 sums over runs of kStRun
+// entries, each run then added in FP64.
+__device__ inline void stereo_entries(const StereoView& sv, const StereoBatch& sb, const int* __restrict__ e, int n,
+                                      double& s0, double& s1, double& s2) {
+    constexpr int kStRun = 16;
+    double Sr = 0.0, Si = 0.0, Gs = 0.0;
+    float sr = 0.f, si = 0.f, gs = 0.f;
+    const int64_t base = int64_t(sb.proj) * sv.nn;
+    const unsigned mask = (1u << kNodeBits) - 1u;
+    for (int j = 0; j < n; ++j) {
+        const unsigned en = unsigned(__builtin_amdgcn_readfirstlane(e[j]));
+        const StNode cur = stereo_fetch(sv, base + (en & mask));
+        stereo_sum(cur, int(en >> kNodeBits), base + (en & mask), sv, sb.zr, sb.zi, sr, si, gs);
+        if ((j % kStRun) == kStRun - 1 || j + 1 == n) {
+            Sr += double(sr);
+            Si += double(si);
+            Gs += double(gs);
+            sr = si = gs = 0.f;
+        }
+    }
+    stereo_add(sb.proj, sb.zr, sb.zi, Sr, Si, Gs, s0, s1, s2);
 }
 
 // The batch's interaction list.  LOW: near leaves and the far entries of
@@ -797,29 +847,18 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     }
     cross_add(t0, t1, t2, v0, v1, v2, s0, s1, s2);
     if (STEREO && LOW) {
-        // stereographic entries in the batch's projection
+        // stereographic entries: in the batch's projection from the front of
+        // the list, in the other projection from the back
         const int* stl = ls.st + bi * ls.st_cap;
         const int nst = __builtin_amdgcn_readfirstlane(ls.st_cnt[bi]);
-        if (nst > 0) {
-            const StereoBatch sb = stereo_batch(batch_of(valid, t0, t1, t2), valid, t0, t1, t2);
-            // FP32 sums over runs of kStRun entries, each run then added in FP64
-            constexpr int kStRun = 16;
-            double Sr = 0.0, Si = 0.0, Gs = 0.0;
-            float sr = 0.f, si = 0.f, gs = 0.f;
-            const int64_t base = int64_t(sb.proj) * sv.nn;
-            const unsigned mask = (1u << kNodeBits) - 1u;
-            for (int j = 0; j < nst; ++j) {
-                const unsigned e = unsigned(__builtin_amdgcn_readfirstlane(stl[j]));
-                const StNode cur = stereo_fetch(sv, base + (e & mask));
-                stereo_sum(cur, int(e >> kNodeBits), base + (e & mask), sv, sb.zr, sb.zi, sr, si, gs);
-                if ((j % kStRun) == kStRun - 1 || j + 1 == nst) {
-                    Sr += double(sr);
-                    Si += double(si);
-                    Gs += double(gs);
-                    sr = si = gs = 0.f;
-                }
-            }
-            stereo_add(sb.proj, sb.zr, sb.zi, Sr, Si, Gs, s0, s1, s2);
+        const int nso = __builtin_amdgcn_readfirstlane(ls.st_o_cnt[bi]);
+        if (nst + nso > 0) {
+            const Batch bt = batch_of(valid, t0, t1, t2);
+            const StereoBatch sb = stereo_batch(bt, valid, t0, t1, t2);
+            if (nst > 0) stereo_entries(sv, sb, stl, nst, s0, s1, s2);
+            if (nso > 0)
+                stereo_entries(sv, stereo_batch_in(1 - sb.proj, bt, valid, t0, t1, t2), stl + ls.st_cap - nso, nso,
+                               s0, s1, s2);
         }
     }
     if (valid) {
@@ -1156,7 +1195,7 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     for (int attempt = 0;; ++attempt) {
         const int st_cap = sv.on ? c->st_cap : 0;
         const size_t bytes =
-            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 4 + kCostBins) *
+            size_t(nbatch) * (size_t(c->far_cap) + size_t(c->near_cap) + size_t(st_cap) + 5 + kCostBins) *
                 sizeof(int) + size_t(3 * nt) * sizeof(double) + 256;
         LPM_CHECK(c->tl_work.reserve(bytes));
         double* acc = c->tl_work.as<double>();
@@ -1173,7 +1212,8 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
         BatchHeader* hdr = batch_header(c->stats_ptr);
         ls.overflow = hdr->overflow;
         ls.bins = hdr->bins;
-        ls.queue = ls.st_cnt + nbatch;
+        ls.st_o_cnt = ls.st_cnt + nbatch;
+        ls.queue = ls.st_o_cnt + nbatch;
         ls.qcap = nbatch;
         ls.far_cap = c->far_cap;
         ls.near_cap = c->near_cap;
