@@ -318,7 +318,7 @@ def set_stereo(ctx, args, on: bool) -> None:
     off for the other solvers, whose moments would otherwise include the
     stereographic ones."""
     import lpmcxx_amd as L
-    ctx.set_option(L.LPM_OPT_STEREO, 1 if (on and not args.no_stereo) else 0)
+    ctx.set_option(L.LPM_OPT_STEREO, args.stereo_mode if (on and not args.no_stereo) else 0)
     ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, args.stereo_extra)
     ctx.set_option(L.LPM_OPT_STEREO_RATIO, args.stereo_ratio)
 
@@ -671,13 +671,14 @@ def measured_traffic(kernel: str, args) -> tuple:
     """HBM bytes per launch of `kernel` from the committed counter summary of
     the default level-9 learned step (FETCH_SIZE / WRITE_SIZE passes scaled by
     a 256 MiB copy); None for other workloads."""
-    defaults = ((args.level, args.leaf, args.theta_max, args.max_order, args.learned_cell, args.tau, args.no_stereo,
-                 args.stereo_extra, args.stereo_ratio) == (9, 128, 0.92, 7, 1024, 7e-7, False, 1, 2.0))
     f = ROOT / "profiles" / "tree_eval_lists_l9_traffic.json"
-    if not defaults or not f.exists():
+    if not f.exists():
         return None, None
     d = json.loads(f.read_text())
-    if d.get("kernel") != kernel:
+    mine = {"level": args.level, "leaf": args.leaf, "theta_max": args.theta_max, "max_order": args.max_order,
+            "cell": args.learned_cell, "tau": args.tau, "stereo_mode": 0 if args.no_stereo else args.stereo_mode,
+            "stereo_extra": args.stereo_extra, "stereo_ratio": args.stereo_ratio}
+    if d.get("kernel") != kernel or d.get("settings") != mine:
         return None, None
     return d["traffic_bytes_per_launch"], "profiles/tree_eval_lists_l9_traffic.md"
 
@@ -689,13 +690,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--level", type=int, default=9)
     # learned treecode
-    ap.add_argument("--leaf", type=int, default=128)
+    ap.add_argument("--leaf", type=int, default=None, help="default 192 (stereographic), 128 (Cartesian)")
     ap.add_argument("--theta-max", type=float, default=None, help="default 0.92 (stereographic), 0.85 (Cartesian)")
     ap.add_argument("--tau", type=float, default=None,
-                    help="per-pair error bound of the labels (default 7e-7 with the stereographic far field, "
+                    help="per-pair error bound of the labels (default 3e-6 with the stereographic far field, "
                          "3e-7 without)")
     ap.add_argument("--no-stereo", action="store_true",
                     help="Cartesian far field only (no LPM_OPT_STEREO in the learned evaluation)")
+    ap.add_argument("--stereo-mode", type=int, default=2, choices=[1, 2],
+                    help="LPM_OPT_STEREO of the learned evaluation: 2 = stereographic far field only (Cartesian "
+                         "moments of order 3 for the model's features), 1 = with the Cartesian series beside it")
     ap.add_argument("--stereo-extra", type=int, default=1, help="LPM_OPT_STEREO_EXTRA_ORDER")
     ap.add_argument("--stereo-ratio", type=float, default=2.0, help="LPM_OPT_STEREO_RATIO")
     ap.add_argument("--max-order", type=int, default=7)
@@ -721,7 +725,7 @@ def main() -> int:
     ap.add_argument("--no-rk4-l10", action="store_true", help="skip the level-10 RK4 comparison")
     ap.add_argument("--l10-level", type=int, default=10)
     ap.add_argument("--l10-steps", type=int, default=10)
-    ap.add_argument("--l10-tau", type=float, default=None, help="default 5e-7 (stereographic), 1.5e-7 (Cartesian)")
+    ap.add_argument("--l10-tau", type=float, default=None, help="default 1.5e-6 (stereographic), 1.5e-7 (Cartesian)")
     ap.add_argument("--l10-label-stride", type=int, default=64)
     # the fastest plain FP64 setting within 1e-5 at level 10 on the Rossby-Haurwitz field (p 6, theta 0.35
     # gives 1.00e-5 there)
@@ -735,16 +739,18 @@ def main() -> int:
     ap.add_argument("--psi-leaf", type=int, default=256)
     ap.add_argument("--psi-order", type=int, default=6)
     ap.add_argument("--psi-theta", type=float, default=0.4)
-    ap.add_argument("--bve-tau", type=float, default=None, help="default 6e-7 (stereographic), 2e-7 (Cartesian)")
+    ap.add_argument("--bve-tau", type=float, default=None, help="default 2e-6 (stereographic), 2e-7 (Cartesian)")
     args = ap.parse_args()
     if args.theta_max is None:
         args.theta_max = 0.85 if args.no_stereo else 0.92
+    if args.leaf is None:
+        args.leaf = 128 if args.no_stereo else 192
     if args.tau is None:
-        args.tau = 3e-7 if args.no_stereo else 7e-7
+        args.tau = 3e-7 if args.no_stereo else 3e-6
     if args.l10_tau is None:
-        args.l10_tau = 1.5e-7 if args.no_stereo else 5e-7
+        args.l10_tau = 1.5e-7 if args.no_stereo else 1.5e-6
     if args.bve_tau is None:
-        args.bve_tau = 2e-7 if args.no_stereo else 6e-7
+        args.bve_tau = 2e-7 if args.no_stereo else 2e-6
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2

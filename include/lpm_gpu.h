@@ -241,9 +241,16 @@ int lpm_tree_export_moments(lpm_ctx* ctx, double* moments);
  * the pair's plane ratio (rho_n + r_b) / |z_b - z_c| is at most
  * LPM_OPT_STEREO_RATIO (default 1.0) times its ratio (r_n + r_b) / s on the
  * sphere.  Every target batch uses the projection from the pole opposite its
- * hemisphere.  P + 1 complex terms replace the (p + 1)(p + 2)(p + 3)/6
- * Cartesian terms of the pair.  Options take effect at the next
- * lpm_tree_moments* call (the stereographic moments) and evaluation. */
+ * hemisphere; a pair that fails the ratio test there is tested in the other
+ * projection.  P + 1 complex terms replace the (p + 1)(p + 2)(p + 3)/6
+ * Cartesian terms of the pair.  LPM_OPT_STEREO 2 (stereographic only): the
+ * moments call makes the Cartesian moments only up to order 3 (the degrees 0
+ * to 2 that the order model reads) beside the stereographic ones of order p,
+ * and a learned evaluation (FP32C, delta = 0, max_order <= p) lets an
+ * accepted pair that neither projection takes descend instead of using the
+ * Cartesian series; the other evaluations then need moments made without it.
+ * Options take effect at the next lpm_tree_moments* call (the stereographic
+ * moments) and evaluation. */
 #define LPM_OPT_STEREO 2
 #define LPM_OPT_STEREO_EXTRA_ORDER 3
 #define LPM_OPT_STEREO_RATIO 4

@@ -164,7 +164,8 @@ int lpm_set_option(lpm_ctx* c, int option, double value) {
             c->batch_cell_targets = int(value);
             return LPM_OK;
         case LPM_OPT_STEREO:
-            if (!(value == 0.0 || value == 1.0)) return fail(c, LPM_ERR_INVALID_ARG, "LPM_OPT_STEREO is 0 or 1");
+            if (!(value == 0.0 || value == 1.0 || value == 2.0))
+                return fail(c, LPM_ERR_INVALID_ARG, "LPM_OPT_STEREO is 0, 1 or 2");
             c->stereo = int(value);
             return LPM_OK;
         case LPM_OPT_STEREO_EXTRA_ORDER:

@@ -74,6 +74,7 @@ struct Tree {
     // stereographic node data (lpm_stereo.hpp), [2 projections][nn]; made
     // with the moments when LPM_OPT_STEREO is set
     int stereo_terms = 0;  // terms per node and projection, 0 = none
+    int stereo_order = 0;  // the Cartesian order p they serve (entries of order p + extra)
     DevBuf sgeo;          // double4: centre (re, im), source radius about it (+inf: unusable), unused
     DevBuf smom;          // double2 [.][kStereoTerms]: a_k
     DevBuf smom32;        // float2  [.][kStereoTerms]: a_k / rho^k

@@ -187,6 +187,7 @@ int tree_stereo_moments(lpm_ctx* c, int p, hipStream_t st) {
     }
     LPM_HIP(c, hipGetLastError());
     t->stereo_terms = T;
+    t->stereo_order = p;
     return LPM_OK;
 }
 

@@ -867,12 +867,20 @@ static int set_strengths(lpm_ctx* c, const double* gamma) {
     return LPM_OK;
 }
 
-int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
+// Cartesian order of the moments made under LPM_OPT_STEREO 2: the degrees 0 to
+// 2 that the order model's moment-shell features read.
+constexpr int kFeatureOrder = 3;
+
+int tree_moments_device(lpm_ctx* c, const double* gamma, int p_far, int sign) {
     Tree* t = c->tree;
+    // stereographic only: the far field of order p_far comes from the
+    // stereographic moments, the Cartesian ones serve the features
+    const int p = c->stereo == 2 ? std::min(p_far, kFeatureOrder) : p_far;
     const int nterm = nterms(p);
     LPM_CHECK(t->nstat.reserve(size_t(t->nn) * sizeof(float4)));
     LPM_CHECK(set_strengths(c, gamma));
     t->stereo_terms = 0;
+    t->stereo_order = 0;
     if (c->stereo && !c->aux) {
         LPM_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         LPM_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -885,7 +893,7 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p, int sign) {
     LPM_CHECK(moments_pass<3>(c, p, sign, t->mom, t->momg, t->mom32, [&]() -> int {
         if (!c->stereo) return LPM_OK;
         LPM_HIP(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-        LPM_CHECK(tree_stereo_moments(c, p, c->aux));
+        LPM_CHECK(tree_stereo_moments(c, p_far, c->aux));
         LPM_HIP(c, hipEventRecord(c->ev_join, c->aux));
         return LPM_OK;
     }));

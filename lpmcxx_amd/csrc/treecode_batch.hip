@@ -305,6 +305,7 @@ struct StereoView {
     int extra;             // order P = p + extra
     int maxp;              // highest P the moments allow
     double ratio;          // plane ratio must be <= ratio x the sphere ratio
+    int cart_max;          // highest order of a Cartesian entry (the Cartesian moments')
 };
 
 // The batch's projection and plane centre / radius; ok is false when a target
@@ -701,9 +702,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         // becomes a stereographic entry of order p + extra
         const bool is_st = st_geo && mine && p > 0, is_so = so_geo && mine && p > 0;
         const int pz = (is_st || is_so) ? min(p + sv.extra, sv.maxp) : 0;
-        const bool cart = mine && p > 0 && !is_st && !is_so;
+        // a pair for which neither the stereographic nor the Cartesian series
+        // of its order is at hand (stereographic-only moments) is refused
+        const bool cart = mine && p > 0 && !is_st && !is_so && p <= sv.cart_max;
         const bool is_lo = cart && p <= kLowOrder, is_hi = cart && p > kLowOrder;
-        const bool is_near = mine && p == 0 && leaf, is_desc = mine && p == 0 && !leaf;
+        const bool refused = mine && !cart && !is_st && !is_so;
+        const bool is_near = refused && leaf, is_desc = refused && !leaf;
         const uint64_t m_lo = __ballot(is_lo), m_hi = __ballot(is_hi), m_st = __ballot(is_st), m_so = __ballot(is_so);
         const uint64_t m_near = __ballot(is_near), m_desc = __ballot(is_desc);
         const int c_lo = __popcll(m_lo), c_hi = __popcll(m_hi), c_near = __popcll(m_near), c_desc = __popcll(m_desc);
@@ -1184,7 +1188,7 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
     const bool f32 = prec == LPM_F32C;
     // stereographic far entries: FP32C with delta = 0 and the option set
     StereoView sv{t->sgeo.as<double4>(), t->smom32.as<float2>(), t->nn, 0, c->stereo_extra, t->stereo_terms - 1,
-                  c->stereo_ratio};
+                  c->stereo_ratio, t->order};
     if (c->stereo && f32 && d2 == 0.0) {
         if (t->stereo_terms == 0)
             return fail(c, LPM_ERR_STATE, "no stereographic moments (lpm_tree_moments after setting LPM_OPT_STEREO)");
@@ -1401,7 +1405,13 @@ static int check_learned(lpm_ctx* c, int prec, double theta_max, int margin, int
         return fail(c, LPM_ERR_INVALID_ARG, "max_order must be in 1..%d", kMaxLearnedOrder);
     if (margin < 0 || !(u_ref > 0.0) || !(delta >= 0.0) || !(theta_max >= 0.0) || !std::isfinite(theta_max))
         return fail(c, LPM_ERR_INVALID_ARG, "invalid learned-evaluation arguments");
-    LPM_CHECK(check_tree_for_batch(c, max_order));
+    // the far field of a stereographic FP32C evaluation may come from the
+    // stereographic moments alone (LPM_OPT_STEREO 2)
+    const Tree* t = c->tree;
+    const bool st_far = t && c->stereo && t->stereo_terms > 0 && prec == LPM_F32C && delta == 0.0;
+    LPM_CHECK(check_tree_for_batch(c, st_far ? std::min(max_order, t->order) : max_order));
+    if (st_far && std::max(t->order, t->stereo_order) < max_order)
+        return fail(c, LPM_ERR_STATE, "moments of order >= %d needed", max_order);
     if (!model_weights(c)) return fail(c, LPM_ERR_STATE, "no order model (lpm_model_init or lpm_model_set_params)");
     return check_targets(c, tx, ty, tz, nt, ux, uy, uz);
 }

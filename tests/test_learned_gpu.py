@@ -394,7 +394,44 @@ def test_stereo_is_off_for_fp64_and_smoothing(ctx, mesh):
         with pytest.raises(L.LpmError):
             ctx.set_option(L.LPM_OPT_STEREO_EXTRA_ORDER, 15)
         with pytest.raises(L.LpmError):
-            ctx.set_option(L.LPM_OPT_STEREO, 2)
+            ctx.set_option(L.LPM_OPT_STEREO, 3)
+    finally:
+        _stereo(ctx, False)
+
+
+def test_stereo_only_moments(ctx, mesh):
+    """LPM_OPT_STEREO 2 makes the Cartesian moments only up to order 3 (the
+    model's features) beside stereographic moments of the requested order.  A
+    constant model accepting every pair past theta_max = 0.4 at order 6 then
+    leaves no Cartesian entry (a pair that neither projection takes descends),
+    the result is no less accurate against the direct sum than with the full
+    Cartesian moments (LPM_OPT_STEREO 1), and the evaluations that need the
+    Cartesian series of order 6 refuse these moments."""
+    m, src, g, tgt = _field(mesh, 5, 64, 8)
+    ud = np.stack(O.bs_direct(tgt, src, g))
+    u_ref = LN.u_ref_of(g)
+    res = {}
+    try:
+        for mode in (1, 2):
+            _stereo(ctx, True, 1, 2.0)
+            ctx.set_option(L.LPM_OPT_STEREO, mode)
+            _gpu_tree(ctx, m, src, g, 64, 8)
+            LN.constant_model(ctx, 6)
+            u = np.stack(LN.evaluate(ctx, tgt, 0.4, 0, 8, u_ref, prec=L.LPM_F32C))
+            res[mode] = (O.rel_l2(u, ud), T.stats(ctx))
+        (e1, s1), (e2, s2) = res[1], res[2]
+        print(f"\nstereo 1: {e1:.2e} (far {s1['far']}, stereographic {s1['far_stereo']}, near {s1['near']}); "
+              f"stereo 2: {e2:.2e} (far {s2['far']}, stereographic {s2['far_stereo']}, near {s2['near']})")
+        assert s1["far"] > s1["far_stereo"] > 0.9 * s1["far"]
+        assert s2["far"] == s2["far_stereo"] and s2["terms"] == 0
+        assert s2["near"] >= s1["near"]
+        assert e2 <= 1.05 * e1
+        with pytest.raises(L.LpmError):
+            T.evaluate_batch(ctx, tgt, 0.4, 6)
+        with pytest.raises(L.LpmError):
+            LN.evaluate(ctx, tgt, 0.4, 0, 6, u_ref, prec=L.LPM_F64)
+        # orders up to 3 are still served by the Cartesian moments
+        LN.evaluate(ctx, tgt, 0.4, 0, 3, u_ref, prec=L.LPM_F64)
     finally:
         _stereo(ctx, False)
 
