@@ -385,6 +385,7 @@ int sort_pairs(lpm_ctx* c, Tree* t, const K* kin, K* kout, const V* vin, V* vout
 
 static_assert(kLvTotal <= kPinWords, "the build table is read through the pinned words");
 constexpr int kFastLevels = 10;  // key levels sorted before the first level loop (30 bits: 4 radix passes)
+constexpr int kShortLevels = 8;  // 24 bits: 3 radix passes
 constexpr int64_t kBuildNodeBudget = int64_t(16) << 20;  // node bound of the levels built between two table reads
 
 }  // namespace
@@ -444,7 +445,11 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     // before going on.
     const int64_t split_bound = (control == 0) ? std::min<int64_t>(n, 8 * (n / (int64_t(leaf_max) + 1))) : n;
     int* h = c->pin;
-    int sorted_levels = std::min(kFastLevels, kMaxDepth);
+    // key levels sorted at first: kFastLevels, or 8 (24 bits, three radix
+    // passes) when the previous build's tree was at most 7 levels deep (a
+    // rebuild of moved particles mostly keeps its depth); a node below them
+    // that must split sends the build round again with more levels sorted
+    int sorted_levels = (t->depth > 0 && t->depth < kShortLevels) ? kShortLevels : kFastLevels;
     int64_t nn = 1;
     int* blk = off;  // per-block child counts (at most one word per 32 nodes)
     // The first run of levels stops at the depth of the previous build (a
@@ -503,7 +508,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             if (ck == 0 || lk > last) break;
         }
         if (!resort) break;
-        sorted_levels = kMaxDepth;
+        sorted_levels = sorted_levels < kFastLevels ? kFastLevels : kMaxDepth;
     }
     int depth = 0;
     t->level_start.assign(1, 0);
@@ -558,7 +563,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     // leaf chunk list of the moment kernel, needed only when a leaf holds
     // more than kChunk particles (else the kernel takes one block per node)
     const int64_t nch = t->nch;
-    if (nch != t->nleaves) {
+    if (nch != t->nleaves && !(control == 0 && leaf_max <= kDirectLeafMax)) {
         LPM_CHECK(t->chunks.reserve(size_t(3 * std::max<int64_t>(nch, 1) + 2 * nn) * sizeof(int)));
         int* ch = t->chunks.as<int>();
         int *cn = ch, *cb = ch + nch, *ce = ch + 2 * nch, *n0 = ch + 3 * nch, *nc = n0 + nn;

@@ -138,14 +138,16 @@ constexpr int kMomThreads = 256;
 // four banks apart
 constexpr int kMomRow = kChunk + 2;
 
-// P2M of one leaf chunk, order P known at compile time: the particles'
-// powers h^a (a < P, per axis) and gamma y (or gamma) staged in LDS as rows
-// over the chunk's particles, then G groups of nterm threads (G = 256 /
-// nterm, at least 1) each sum their term over a contiguous part of the
-// chunk, four particles per step (ds_read2 pairs of consecutive particles);
-// the group partials are added in group order.
-// direct: every leaf is one chunk, so the sum is the leaf's moments (all three
-// forms are stored); otherwise the chunk partial goes to `part`.
+// P2M of one leaf (or one leaf chunk), order P known at compile time: the
+// particles' powers h^a (a < P, per axis) and gamma y (or gamma) staged in
+// LDS as rows over up to kChunk particles at a time, then G groups of nterm
+// threads (G = 256 / nterm, at least 1) each sum their term over a contiguous
+// part of the staged particles, four per step (ds_read2 pairs of consecutive
+// particles), in registers across the leaf's chunks; the group partials are
+// added in group order.
+// chunk_node == nullptr: one block per node, a leaf's particles staged chunk
+// by chunk (all three forms of its moments are stored); otherwise one block
+// per entry of the chunk list, whose partial goes to `part`.
 template <int NC, int P>
 __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __restrict__ chunk_node,
                                                                   const int* __restrict__ chunk_beg,
@@ -155,20 +157,19 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
                                                                   const int* __restrict__ nend,
                                                                   const double4* __restrict__ ncent,
                                                                   const double4* __restrict__ spos, double sign,
-                                                                  double* __restrict__ part, MomOut out,
-                                                                  int direct) {
+                                                                  double* __restrict__ part, MomOut out) {
     constexpr int nterm = P * (P + 1) * (P + 2) / 6;
     constexpr int G = kMomThreads / nterm > 1 ? kMomThreads / nterm : 1;
+    constexpr int KT = (G * nterm + kMomThreads - 1) / kMomThreads;  // tasks per thread
     extern __shared__ double lds[];
     double* pw = lds;                       // [3][P][kMomRow] powers
     double* gy = lds + 3 * P * kMomRow;     // [NC][kMomRow]   gamma * y, or gamma
     double* red = gy + NC * kMomRow;        // [G][nterm][NC]  group partials (G > 1)
-    // chunk_node == nullptr: one block per node, every leaf one chunk (the
-    // node's words are loaded together, then internal nodes leave)
     const int ch = blockIdx.x;
+    const bool direct = chunk_node == nullptr;
     int node = ch, b, e;
     double4 c;
-    if (!chunk_node) {
+    if (direct) {
         const int nk = nkids[ch];
         b = nbeg[ch];
         e = nend[ch];
@@ -180,72 +181,88 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
         e = chunk_end[ch];
         c = ncent[node];
     }
-    const int cnt = e - b;
     __shared__ double s_ipow[P];
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {  // the sums read rows q < cnt only
-        const double4 s = spos[b + q];
-        const double yx = s.x, yy = s.y, yz = s.z, g = s.w;
-        const double hx = sign * (s.x - c.x), hy = sign * (s.y - c.y), hz = sign * (s.z - c.z);
-        double ax = 1.0, ay = 1.0, az = 1.0;
-#pragma unroll
-        for (int a = 0; a < P; ++a) {
-            pw[a * kMomRow + q] = ax;
-            pw[(P + a) * kMomRow + q] = ay;
-            pw[(2 * P + a) * kMomRow + q] = az;
-            ax *= hx; ay *= hy; az *= hz;
-        }
-        if (NC == 3) {
-            gy[q] = g * yx;
-            gy[kMomRow + q] = g * yy;
-            gy[2 * kMomRow + q] = g * yz;
-        } else {
-            gy[q] = g;
-        }
-    }
     if (threadIdx.x == 0) inverse_powers<P>(1.0 / c.w, s_ipow);
-    __syncthreads();
-    for (int it = threadIdx.x; it < G * nterm; it += blockDim.x) {
-        const int grp = it / nterm, t = it - grp * nterm;
-        int k1, k2, k3;
-        lex_decode(P, t, k1, k2, k3);
-        const int q0 = grp * cnt / G, q1 = (grp + 1) * cnt / G;
-        const double* wx = pw + k1 * kMomRow + q0;
-        const double* wy = pw + (P + k2) * kMomRow + q0;
-        const double* wz = pw + (2 * P + k3) * kMomRow + q0;
-        const double* gq = gy + q0;
-        double m[NC] = {};
-        int n = q1 - q0;
-        for (; n >= 4; n -= 4, wx += 4, wy += 4, wz += 4, gq += 4) {
+    double m[KT][NC] = {};
+    for (int c0 = b; c0 < e; c0 += kChunk) {
+        const int cnt = min(kChunk, e - c0);
+        if (c0 != b) __syncthreads();  // the previous chunk's rows are read
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {  // the sums read rows q < cnt only
+            const double4 s = spos[c0 + q];
+            const double yx = s.x, yy = s.y, yz = s.z, g = s.w;
+            const double hx = sign * (s.x - c.x), hy = sign * (s.y - c.y), hz = sign * (s.z - c.z);
+            double ax = 1.0, ay = 1.0, az = 1.0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int a = 0; a < P; ++a) {
+                pw[a * kMomRow + q] = ax;
+                pw[(P + a) * kMomRow + q] = ay;
+                pw[(2 * P + a) * kMomRow + q] = az;
+                ax *= hx; ay *= hy; az *= hz;
+            }
+            if (NC == 3) {
+                gy[q] = g * yx;
+                gy[kMomRow + q] = g * yy;
+                gy[2 * kMomRow + q] = g * yz;
+            } else {
+                gy[q] = g;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const int it = threadIdx.x + k * kMomThreads;
+            if (it >= G * nterm) break;
+            const int grp = it / nterm, t = it - grp * nterm;
+            int k1, k2, k3;
+            lex_decode(P, t, k1, k2, k3);
+            const int q0 = grp * cnt / G, q1 = (grp + 1) * cnt / G;
+            const double* wx = pw + k1 * kMomRow + q0;
+            const double* wy = pw + (P + k2) * kMomRow + q0;
+            const double* wz = pw + (2 * P + k3) * kMomRow + q0;
+            const double* gq = gy + q0;
+            int n = q1 - q0;
+            for (; n >= 4; n -= 4, wx += 4, wy += 4, wz += 4, gq += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double pv = wx[u] * wy[u] * wz[u];
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) m[k][cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[k][cc]);
+                }
+            }
+            for (int u = 0; u < n; ++u) {
                 const double pv = wx[u] * wy[u] * wz[u];
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) m[cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[cc]);
-            }
-        }
-        for (int u = 0; u < n; ++u) {
-            const double pv = wx[u] * wy[u] * wz[u];
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) m[cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[cc]);
-        }
-        if (G > 1) {
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) red[(grp * nterm + t) * NC + cc] = m[cc];
-        } else {
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-                if (direct) store_moment_p<NC, P>(out, node, t, cc, m[cc], s_ipow);
-                else part[(int64_t(ch) * nterm + t) * NC + cc] = m[cc];
+                for (int cc = 0; cc < NC; ++cc) m[k][cc] = __builtin_fma(gq[cc * kMomRow + u], pv, m[k][cc]);
             }
         }
     }
-    if (G == 1) return;
-    __syncthreads();
-    for (int it = threadIdx.x; it < nterm * NC; it += blockDim.x) {
-        double v = red[it];
-        for (int grp = 1; grp < G; ++grp) v += red[grp * nterm * NC + it];
-        if (direct) store_moment_p<NC, P>(out, node, it / NC, it % NC, v, s_ipow);
-        else part[int64_t(ch) * nterm * NC + it] = v;
+    if (G > 1) {
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const int it = threadIdx.x + k * kMomThreads;
+            if (it >= G * nterm) break;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) red[it * NC + cc] = m[k][cc];
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < nterm * NC; it += blockDim.x) {
+            double v = red[it];
+            for (int grp = 1; grp < G; ++grp) v += red[grp * nterm * NC + it];
+            if (direct) store_moment_p<NC, P>(out, node, it / NC, it % NC, v, s_ipow);
+            else part[int64_t(ch) * nterm * NC + it] = v;
+        }
+    } else {
+        __syncthreads();  // s_ipow
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const int t = threadIdx.x + k * kMomThreads;
+            if (t >= nterm) break;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                if (direct) store_moment_p<NC, P>(out, node, t, cc, m[k][cc], s_ipow);
+                else part[(int64_t(ch) * nterm + t) * NC + cc] = m[k][cc];
+            }
+        }
     }
 }
 
@@ -253,12 +270,12 @@ __global__ __launch_bounds__(kMomThreads) void tree_moments_chunk(const int* __r
 template <int NC, int P>
 void launch_moments_chunk(hipStream_t st, int64_t nch, const int* cn, const int* cb, const int* ce, const int* nkids,
                           const int* nbeg, const int* nend, const double4* ncent, const double4* spos, double sign,
-                          double* part, MomOut out, int direct) {
+                          double* part, MomOut out) {
     constexpr int nterm = P * (P + 1) * (P + 2) / 6;
     constexpr int G = kMomThreads / nterm > 1 ? kMomThreads / nterm : 1;
     const size_t lds = (size_t(kMomRow) * (3 * P + NC) + (G > 1 ? size_t(G) * nterm * NC : 0)) * sizeof(double);
     hipLaunchKernelGGL((tree_moments_chunk<NC, P>), dim3(unsigned(nch)), dim3(kMomThreads), lds, st, cn, cb, ce,
-                       nkids, nbeg, nend, ncent, spos, sign, part, out, direct);
+                       nkids, nbeg, nend, ncent, spos, sign, part, out);
 }
 
 // chunk partials of every leaf summed in chunk order (leaves of more than one
@@ -807,8 +824,9 @@ static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, 
     // chunk list of the leaves, made by the build when a leaf holds more than
     // one chunk (internal nodes come from the upward pass)
     const int64_t nch = t->nch;
-    // every leaf one chunk: the chunk sums are the leaf moments, one block per node
-    const int direct = (nch == t->nleaves) ? 1 : 0;
+    // one block per node when every leaf is one chunk or the leaves are small
+    // enough for a block to stage chunk after chunk
+    const int direct = (nch == t->nleaves || (t->leaf_max > 0 && t->leaf_max <= kDirectLeafMax)) ? 1 : 0;
     const int* d_cn = direct ? nullptr : t->chunks.as<int>();
     const int* d_cb = direct ? nullptr : d_cn + nch;
     const int* d_ce = direct ? nullptr : d_cb + nch;
@@ -826,7 +844,7 @@ static int moments_pass(lpm_ctx* c, int p, int sign, DevBuf& mom, DevBuf& momg, 
     case PP:                                                                                                  \
         launch_moments_chunk<NC, PP>(st, nblk, d_cn, d_cb, d_ce, t->nkids.as<int>(), t->nbeg.as<int>(),       \
                                      t->nend.as<int>(), t->ncent.as<double4>(), t->spos.as<double4>(),        \
-                                     double(sign), t->mpart.as<double>(), out, direct);                      \
+                                     double(sign), t->mpart.as<double>(), out);                              \
         break;
             LPM_CASE(1) LPM_CASE(2) LPM_CASE(3) LPM_CASE(4) LPM_CASE(5) LPM_CASE(6)
             LPM_CASE(7) LPM_CASE(8) LPM_CASE(9) LPM_CASE(10) LPM_CASE(11) LPM_CASE(12)
