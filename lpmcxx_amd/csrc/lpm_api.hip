@@ -136,6 +136,7 @@ void lpm_destroy(lpm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     tree_free(c->tree);
     model_free(c->model);
     c->tree = nullptr;
@@ -209,6 +210,7 @@ const char* lpm_last_error(const lpm_ctx* c) { return c ? c->err.c_str() : ""; }
 int lpm_synchronize(lpm_ctx* c) {
     if (!c) return LPM_ERR_NULL_PTR;
     LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->aux) LPM_HIP(c, hipStreamSynchronize(c->aux));
     return LPM_OK;
 }
 

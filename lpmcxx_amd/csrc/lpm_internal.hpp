@@ -97,6 +97,10 @@ struct lpm_ctx {
     // stereographic moments run beside the Cartesian ones (made on first use)
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the second stream's last work (ev_join) is not yet ordered before the
+    // first stream's: the learned evaluation joins it just before it reads
+    // the stereographic node data, anything that rewrites the tree at its start
+    bool aux_pending = false;
 };
 
 namespace lpm {
@@ -121,6 +125,13 @@ inline void mark_end(lpm_ctx* c) {
     }
 }
 int hip_fail(lpm_ctx* c, hipError_t e, const char* what);
+
+// Orders the second stream's pending work before the context stream's next work.
+inline hipError_t join_aux(lpm_ctx* c) {
+    if (!c->aux_pending) return hipSuccess;
+    c->aux_pending = false;
+    return hipStreamWaitEvent(c->stream, c->ev_join, 0);
+}
 
 #define LPM_HIP(ctx, call)                                  \
     do {                                                    \

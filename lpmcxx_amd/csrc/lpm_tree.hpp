@@ -12,6 +12,8 @@ constexpr int kMaxDepth = LPM_TREE_MAX_DEPTH;  // 21 levels x 3 bits = 63-bit ke
 constexpr int kMaxOrder = 16;                  // series order limit (|k| < p)
 constexpr int kChunk = 128;                    // leaf particles staged at once by the leaf moment kernel
 constexpr int kDirectLeafMax = 4 * kChunk;     // leaves up to this size take one moment block each
+constexpr int kTopLevels = 4;                  // levels laid out from per-cell counts (tree_top)
+constexpr int kTopCells = 1 << (3 * kTopLevels);
 
 // Build table in device memory (int32): first node and node count of levels
 // 0 .. kMaxDepth + 1, then flags and counters written by the build kernels.

@@ -32,38 +32,64 @@ namespace lpm {
 namespace {
 
 // kKeyBlocks blocks stride over the sources, so that the range of |x|^2
-// folds into the build table with few atomics
+// folds into the build table with few atomics.  The sources are also counted
+// per cell of level kTopLevels (kTopCells cells in key order), so that
+// tree_top lays out the first levels of the tree without searching the
+// sorted keys.
 constexpr int kKeyBlocks = 2048;
 __global__ __launch_bounds__(256) void tree_keys(const double* __restrict__ x, const double* __restrict__ y,
                                                  const double* __restrict__ z, int64_t n, double x0, double x1,
                                                  double y0, double y1, double z0, double z1,
                                                  uint64_t* __restrict__ keys, int* __restrict__ vals,
-                                                 int* __restrict__ lvt) {
+                                                 int* __restrict__ lvt, unsigned* __restrict__ top_hist) {
+    __shared__ unsigned s_hist[kTopCells];
+    for (int b = threadIdx.x; b < kTopCells; b += blockDim.x) s_hist[b] = 0u;
+    __syncthreads();
     int* outside = lvt + kLvOutside;
     double lo = 0.0, hi = 0.0;
     bool any = false;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const double px = x[i], py = y[i], pz = z[i];
-        const double r2 = px * px + py * py + pz * pz;
-        // NaN must win both ends: compare on the bit patterns the words use
-        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(r2));
-        if (!any || bits < static_cast<unsigned long long>(__double_as_longlong(lo))) lo = r2;
-        if (!any || bits > static_cast<unsigned long long>(__double_as_longlong(hi))) hi = r2;
-        any = true;
-        if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
-        double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
-        uint64_t key = 0;
-        for (int l = 0; l < kMaxDepth; ++l) {
-            const double mx = 0.5 * (xl + xh), my = 0.5 * (yl + yh), mz = 0.5 * (zl + zh);
-            const int bx = px >= mx, by = py >= my, bz = pz >= mz;
-            if (bx) xl = mx; else xh = mx;
-            if (by) yl = my; else yh = my;
-            if (bz) zl = mz; else zh = mz;
-            key = (key << 3) | uint64_t(bx | (by << 1) | (bz << 2));
+    const int lane = threadIdx.x & 63;
+    // the loop bound is wave-uniform (whole waves take the same trips) for the ballots
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t w0 = int64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63); w0 < n; w0 += stride) {
+        const int64_t i = w0 + lane;
+        int cell = -1;
+        if (i < n) {
+            const double px = x[i], py = y[i], pz = z[i];
+            const double r2 = px * px + py * py + pz * pz;
+            // NaN must win both ends: compare on the bit patterns the words use
+            const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(r2));
+            if (!any || bits < static_cast<unsigned long long>(__double_as_longlong(lo))) lo = r2;
+            if (!any || bits > static_cast<unsigned long long>(__double_as_longlong(hi))) hi = r2;
+            any = true;
+            if (!(x0 <= px && px < x1 && y0 <= py && py < y1 && z0 <= pz && pz < z1)) atomicOr(outside, 1);
+            double xl = x0, xh = x1, yl = y0, yh = y1, zl = z0, zh = z1;
+            uint64_t key = 0;
+            for (int l = 0; l < kMaxDepth; ++l) {
+                const double mx = 0.5 * (xl + xh), my = 0.5 * (yl + yh), mz = 0.5 * (zl + zh);
+                const int bx = px >= mx, by = py >= my, bz = pz >= mz;
+                if (bx) xl = mx; else xh = mx;
+                if (by) yl = my; else yh = my;
+                if (bz) zl = mz; else zh = mz;
+                key = (key << 3) | uint64_t(bx | (by << 1) | (bz << 2));
+            }
+            keys[i] = key;
+            vals[i] = int(i);
+            cell = int(key >> (3 * (kMaxDepth - kTopLevels)));
         }
-        keys[i] = key;
-        vals[i] = int(i);
+        // one LDS add per distinct cell of the wave (neighbouring sources
+        // mostly share a cell)
+        uint64_t left = __ballot(cell >= 0);
+        while (left) {
+            const int u = __builtin_amdgcn_readlane(cell, __builtin_ctzll(left));
+            const uint64_t m = __ballot(cell == u);
+            if (lane == __builtin_ctzll(m)) atomicAdd(&s_hist[u], unsigned(__popcll(m)));
+            left &= ~m;
+        }
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kTopCells; b += blockDim.x)
+        if (s_hist[b]) atomicAdd(&top_hist[b], s_hist[b]);
     unsigned long long* w = reinterpret_cast<unsigned long long*>(lvt + kLvR2) + 2 * (blockIdx.x % kR2Slots);
     r2_block_range(lo, any, w);
     __syncthreads();  // the second call reuses the block's reduction words
@@ -84,19 +110,145 @@ __device__ inline int lower_code(const uint64_t* keys, int b, int e, int level, 
     return b;
 }
 
-// Root node and an empty level table (level 0 = the root).  One thread.
-__global__ void tree_root(int n, int* nbeg, int* nend, int* nlevel, int* nparent, uint64_t* nprefix,
-                          int* __restrict__ lvt) {
-    nbeg[0] = 0;
-    nend[0] = n;
-    nlevel[0] = 0;
-    nparent[0] = -1;
-    nprefix[0] = 0;
-    for (int w = 0; w < kLvWords; ++w) lvt[w] = 0;
-    lvt[1] = 1;
-    lvt[kLvTooDeep] = 0;
-    lvt[kLvKeys] = 0;
-    for (int w = 0; w < 2 * kLvSpread; ++w) lvt[kLvLeaves + w] = 0;
+// Levels 0 .. kTopLevels of the tree from the per-cell counts of tree_keys.
+// Cell b of level l covers the sorted positions [start, start + count), the
+// starts being the prefix sums of the level-kTopLevels counts (the keys are
+// sorted on at least kTopLevels levels, so a cell is one run of them).  A
+// cell is a node when its parent cell is a node that splits (control 0:
+// more than param sources; control 1: level below param), and the nodes of
+// each level are numbered in key order after those of the levels above (BFS
+// order, children consecutive): the same table the level-by-level build
+// makes, without searching the keys.  Also resets the level table, the flags
+// and the leaf / chunk counters, and counts the leaves of levels below
+// kTopLevels (those of level kTopLevels are counted by tree_count).  One
+// block.
+constexpr int kTopThreads = 1024;
+__host__ __device__ constexpr int top_offset(int l) { return ((1 << (3 * l)) - 1) / 7; }  // cells above level l
+constexpr int kTopAll = top_offset(kTopLevels + 1);  // cells of levels 0 .. kTopLevels
+static_assert(kTopCells == 1 << (3 * kTopLevels) && kTopCells == kTopThreads * 4, "four level-kTopLevels cells per thread");
+
+__global__ __launch_bounds__(kTopThreads) void tree_top(int n, int control, int param,
+                                                        const unsigned* __restrict__ top_hist, int* __restrict__ lvt,
+                                                        int* __restrict__ nbeg, int* __restrict__ nend,
+                                                        int* __restrict__ nfirst, int* __restrict__ nkids,
+                                                        int* __restrict__ nlevel, int* __restrict__ nparent,
+                                                        uint64_t* __restrict__ nprefix) {
+    using Scan = hipcub::BlockScan<int, kTopThreads>;
+    __shared__ typename Scan::TempStorage tmp;
+    __shared__ int s_cnt[kTopAll], s_start[kTopAll], s_idx[kTopAll];
+    __shared__ int s_lv[kTopLevels + 2];
+    __shared__ int s_leaves, s_chunks;
+    const int tid = threadIdx.x;
+    for (int w = tid; w < kLvWords; w += kTopThreads) lvt[w] = 0;
+    for (int w = tid; w < 2 * kLvSpread; w += kTopThreads) lvt[kLvLeaves + w] = 0;
+    if (tid == 0) {
+        lvt[kLvTooDeep] = 0;
+        lvt[kLvKeys] = 0;
+        s_leaves = 0;
+        s_chunks = 0;
+        s_lv[0] = 0;
+        s_lv[1] = 1;  // the root
+    }
+    // counts and starts of the deepest top level: four cells per thread
+    constexpr int o4 = top_offset(kTopLevels);
+    int c4[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        c4[j] = int(top_hist[4 * tid + j]);
+        sum += c4[j];
+    }
+    int ex = 0;
+    Scan(tmp).ExclusiveSum(sum, ex);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        s_cnt[o4 + 4 * tid + j] = c4[j];
+        s_start[o4 + 4 * tid + j] = ex;
+        ex += c4[j];
+    }
+    __syncthreads();
+    // the levels above: sums of eight children, start of the first
+    for (int l = kTopLevels - 1; l >= 0; --l) {
+        const int o = top_offset(l), oc = top_offset(l + 1);
+        for (int b = tid; b < (1 << (3 * l)); b += kTopThreads) {
+            int c = 0;
+            for (int j = 0; j < 8; ++j) c += s_cnt[oc + 8 * b + j];
+            s_cnt[o + b] = c;
+            s_start[o + b] = s_start[oc + 8 * b];
+        }
+        __syncthreads();
+    }
+    // nodes, level by level from the root
+    auto splits = [&](int c, int l) { return control == 0 ? c > param : (l < param && c > 0); };
+    if (tid == 0) s_idx[0] = 0;
+    for (int l = 1; l <= kTopLevels; ++l) {
+        const int o = top_offset(l), op = top_offset(l - 1), ncell = 1 << (3 * l);
+        __syncthreads();  // s_idx of level l - 1, s_lv[l]
+        // cells 4 tid .. 4 tid + 3 (levels with fewer cells use the first threads)
+        int flag[4], f = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * tid + j;
+            flag[j] = 0;
+            if (b < ncell) {
+                const int pb = b >> 3;
+                flag[j] = s_cnt[o + b] > 0 && s_idx[op + pb] >= 0 && splits(s_cnt[op + pb], l - 1);
+            }
+            f += flag[j];
+        }
+        int rank = 0, total = 0;
+        __syncthreads();  // tmp
+        Scan(tmp).ExclusiveSum(f, rank, total);
+        const int first = s_lv[l];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * tid + j;
+            if (b < ncell) s_idx[o + b] = flag[j] ? first + rank : -1;
+            rank += flag[j];
+        }
+        if (tid == 0) s_lv[l + 1] = first + total;
+    }
+    __syncthreads();
+    // node words; the children of a node of level l < kTopLevels are its
+    // nonempty child cells, consecutive in the numbering
+    for (int l = 0; l <= kTopLevels; ++l) {
+        const int o = top_offset(l), ncell = 1 << (3 * l);
+        for (int b = tid; b < ncell; b += kTopThreads) {
+            const int i = s_idx[o + b];
+            if (i < 0) continue;
+            const int c = s_cnt[o + b], st = s_start[o + b];
+            nbeg[i] = st;
+            nend[i] = st + c;
+            nlevel[i] = l;
+            nparent[i] = l == 0 ? -1 : s_idx[top_offset(l - 1) + (b >> 3)];
+            nprefix[i] = uint64_t(b);
+            int kf = -1, nk = 0;
+            if (l < kTopLevels && splits(c, l)) {
+                const int oc = top_offset(l + 1);
+                for (int j = 0; j < 8; ++j) {
+                    const int ci = s_idx[oc + 8 * b + j];
+                    if (ci >= 0) {
+                        if (nk == 0) kf = ci;
+                        ++nk;
+                    }
+                }
+            }
+            nfirst[i] = kf;
+            nkids[i] = nk;
+            if (l < kTopLevels && nk == 0) {
+                atomicAdd(&s_leaves, 1);
+                atomicAdd(&s_chunks, (c + kChunk - 1) / kChunk);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid <= kTopLevels) {
+        lvt[2 * tid] = s_lv[tid];
+        lvt[2 * tid + 1] = s_lv[tid + 1] - s_lv[tid];
+    }
+    if (tid == 0) {
+        lvt[kLvLeaves] = s_leaves;
+        lvt[kLvChunks] = s_chunks;
+    }
 }
 
 // The child intervals of node [b, e) at `level`, found by the 8 lanes of an
@@ -405,6 +557,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     if (!c->tree) c->tree = new Tree();
     Tree* t = c->tree;
     hipStream_t st = c->stream;
+    LPM_HIP(c, join_aux(c));  // the second stream may still read the old tree
     t->ns = n;
     t->nn = 0;
     t->sx = t->sy = t->sz = nullptr;  // set once the build has succeeded
@@ -424,18 +577,21 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     LPM_CHECK(t->tmp1.reserve(nalloc * sizeof(int) + 64));   // unsorted values
     LPM_CHECK(t->tmp2.reserve(nalloc * sizeof(uint64_t)));
     LPM_CHECK(t->tmp3.reserve(size_t(2 * nalloc + 2) * sizeof(int)));  // children counts | their scan
-    LPM_CHECK(t->lvt.reserve(kLvTotal * sizeof(int)));
+    // the build table, then the per-cell counts of the top levels
+    LPM_CHECK(t->lvt.reserve((kLvTotal + kTopCells) * sizeof(int)));
     int* cnt = t->tmp3.as<int>();
     int* off = cnt + nalloc + 1;
     int* lvt = t->lvt.as<int>();
-    LPM_CHECK(grow_nodes(c, t, std::max<int64_t>(1024, 4 * (n / std::max(1, leaf_max)) + 64)));
-    LPM_HIP(c, hipMemsetAsync(lvt, 0, kLvTotal * sizeof(int), st));
+    unsigned* top_hist = reinterpret_cast<unsigned*>(lvt + kLvTotal);
+    LPM_CHECK(grow_nodes(c, t, std::max<int64_t>(std::max(1024, kTopAll), 4 * (n / std::max(1, leaf_max)) + 64)));
+    LPM_HIP(c, hipMemsetAsync(lvt, 0, (kLvTotal + kTopCells) * sizeof(int), st));
     if (n > 0)
         hipLaunchKernelGGL(tree_keys, dim3(unsigned(std::min<int64_t>(kKeyBlocks, blocks(n)))), dim3(256), 0, st, x, y,
                            z, n, blo[0], bhi[0], blo[1], bhi[1], blo[2], bhi[2], t->tmp0.as<uint64_t>(),
-                           t->tmp1.as<int>(), lvt);
+                           t->tmp1.as<int>(), lvt, top_hist);
 
-    // Level by level: count children, then emit them (offsets from the
+    // Levels 0 .. kTopLevels from the top-level cell counts (tree_top); then
+    // level by level: count children, then emit them (offsets from the
     // per-block child counts) -- every level up to the sorted key levels in
     // one run of launches, grids sized by the node bound of each level, the
     // build table read once at the end.  The keys are first sorted on their
@@ -455,18 +611,22 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     // The first run of levels stops at the depth of the previous build (a
     // rebuild of moved particles mostly keeps it): levels below it would be
     // launched empty.  A deeper tree continues in the next run.
-    int run_cap = t->depth > 0 ? t->depth : kMaxDepth;
+    int run_cap = t->depth > 0 ? std::max(t->depth, kTopLevels) : kMaxDepth;
     for (;;) {
         if (n > 0)
             LPM_CHECK(sort_pairs(c, t, t->tmp0.as<uint64_t>(), t->keys.as<uint64_t>(), t->tmp1.as<int>(),
                                  t->perm.as<int>(), n, 3 * (kMaxDepth - sorted_levels)));
-        hipLaunchKernelGGL(tree_root, dim3(1), dim3(1), 0, st, int(n), t->nbeg.as<int>(), t->nend.as<int>(),
-                           t->nlevel.as<int>(), t->nparent.as<int>(), t->nprefix.as<uint64_t>(), lvt);
+        hipLaunchKernelGGL(tree_top, dim3(1), dim3(kTopThreads), 0, st, int(n), control, param,
+                           static_cast<const unsigned*>(top_hist), lvt, t->nbeg.as<int>(), t->nend.as<int>(),
+                           t->nfirst.as<int>(), t->nkids.as<int>(), t->nlevel.as<int>(), t->nparent.as<int>(),
+                           t->nprefix.as<uint64_t>());
         // deepest level that may hold nodes to split
         const int last = std::min(sorted_levels, control == 1 ? std::max(0, std::min(param, kMaxDepth)) : kMaxDepth);
-        int lk = 0;
-        int64_t ck = 1;
-        nn = 1;
+        // the count kernels start at the deepest top level (its nodes may
+        // split); its node count is known on the host only by its bound
+        int lk = kTopLevels;
+        int64_t ck = std::min<int64_t>(std::min<int64_t>(kTopCells, std::max<int64_t>(n, 1)), std::max<int64_t>(split_bound, 1));
+        nn = top_offset(kTopLevels) + ck;
         bool resort = false;
         for (;;) {
             // levels lk .. lk + G - 1 in this run; need = the nodes so far plus

@@ -891,6 +891,7 @@ constexpr int kFeatureOrder = 3;
 
 int tree_moments_device(lpm_ctx* c, const double* gamma, int p_far, int sign) {
     Tree* t = c->tree;
+    LPM_HIP(c, join_aux(c));
     // stereographic only: the far field of order p_far comes from the
     // stereographic moments, the Cartesian ones serve the features
     const int p = c->stereo == 2 ? std::min(p_far, kFeatureOrder) : p_far;
@@ -918,7 +919,9 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p_far, int sign) {
     hipLaunchKernelGGL(tree_node_stats, dim3(blocks(t->nn)), dim3(256), 0, c->stream, int(t->nn), p, nterm,
                        t->momg.as<double>(), t->nbeg.as<int>(), t->nend.as<int>(), t->nstat.as<float4>());
     LPM_HIP(c, hipGetLastError());
-    if (c->stereo) LPM_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    // only the learned evaluation reads the stereographic data: it joins the
+    // second stream after its target batching, which runs beside it
+    if (c->stereo) c->aux_pending = true;
     t->order = p;
     t->moment_sign = sign;
     return LPM_OK;
@@ -926,6 +929,7 @@ int tree_moments_device(lpm_ctx* c, const double* gamma, int p_far, int sign) {
 
 int tree_moments_psi_device(lpm_ctx* c, const double* gamma, int p) {
     Tree* t = c->tree;
+    LPM_HIP(c, join_aux(c));
     LPM_CHECK(set_strengths(c, gamma));
     LPM_CHECK(moments_pass<1>(c, p, 1, t->pmom, t->pmomg, t->pmom32, []() -> int { return LPM_OK; }));
     LPM_HIP(c, hipGetLastError());

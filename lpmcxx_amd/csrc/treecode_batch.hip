@@ -1227,6 +1227,7 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
             LPM_HIP(c, hipMemsetAsync(hdr->bins, 0, sizeof(hdr->bins), st));
             LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
         }
+        LPM_HIP(c, join_aux(c));  // the stereographic node data
         if (sv.on)
             hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
                                max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
