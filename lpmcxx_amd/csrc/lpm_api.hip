@@ -152,6 +152,7 @@ void lpm_destroy(lpm_ctx* c) {
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_build) (void)hipEventDestroy(c->ev_build);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -101,6 +101,8 @@ struct lpm_ctx {
     // first stream's: the learned evaluation joins it just before it reads
     // the stereographic node data, anything that rewrites the tree at its start
     bool aux_pending = false;
+    // recorded after the build table's copy to the host (the build waits for it alone)
+    hipEvent_t ev_build = nullptr;
 };
 
 namespace lpm {

@@ -413,15 +413,26 @@ __global__ __launch_bounds__(256) void chunk_fill(int nn, const int* __restrict_
     }
 }
 
+// Node count of a finished build table: the start of the first empty level
+// (the emit of the level above writes it).  While levels are still to come,
+// the entry after the last written one is still 0, and so is the result.
+__device__ inline int table_nodes(const int* lvt) {
+    for (int l = 1; l <= kMaxDepth + 1; ++l)
+        if (lvt[2 * l + 1] == 0) return lvt[2 * l];
+    return 0;
+}
+
 // Box bounds by the FP64 bisection recursion, centroid 0.5*(max+min), and
 // maxRadius = max over corners (xmin,ymin,zmin), (xmin,ymax,zmin), ... in the
-// order of src/LpmBox3d.cpp:55-66 of distance(centroid, corner).
-__global__ __launch_bounds__(256) void tree_geom(int nn, double x0, double y0, double z0, double x1, double y1,
-                                                 double z1, const int* __restrict__ nlevel,
+// order of src/LpmBox3d.cpp:55-66 of distance(centroid, corner).  The node
+// count comes from the build table, so the kernel can be enqueued before the
+// host has read it.
+__global__ __launch_bounds__(256) void tree_geom(const int* __restrict__ lvt, double x0, double y0, double z0,
+                                                 double x1, double y1, double z1, const int* __restrict__ nlevel,
                                                  const uint64_t* __restrict__ nprefix, double4* __restrict__ ncent,
                                                  double* __restrict__ nbox) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nn) return;
+    if (i >= table_nodes(lvt)) return;
     const int L = nlevel[i];
     const uint64_t pre = nprefix[i];
     double b[6] = {x0, y0, z0, x1, y1, z1};
@@ -461,23 +472,27 @@ __global__ __launch_bounds__(256) void tree_leaf_keys(int nn, const int* __restr
 // kRankMax sources), each source's position is the number of the leaf's
 // sources with a smaller index; the source's position goes to spos there.
 constexpr int kRankMax = 1024;
-__global__ __launch_bounds__(256) void tree_leaf_rank(int nn, const int* __restrict__ nkids,
+__global__ __launch_bounds__(256) void tree_leaf_rank(const int* __restrict__ lvt, const int* __restrict__ nkids,
                                                       const int* __restrict__ nbeg, const int* __restrict__ nend,
                                                       const int* __restrict__ perm_in, int* __restrict__ perm_out,
                                                       const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z, double4* __restrict__ spos) {
     __shared__ int v[kRankMax];
-    const int i = blockIdx.x;
-    if (i >= nn || nkids[i] != 0) return;
-    const int b = nbeg[i], m = nend[i] - b;
-    for (int t = threadIdx.x; t < m; t += blockDim.x) v[t] = perm_in[b + t];
-    __syncthreads();
-    for (int t = threadIdx.x; t < m; t += blockDim.x) {
-        const int s = v[t];
-        int r = 0;
-        for (int k = 0; k < m; ++k) r += v[k] < s ? 1 : 0;
-        perm_out[b + r] = s;
-        spos[b + r] = make_double4(x[s], y[s], z[s], 0.0);
+    const int nn = table_nodes(lvt);
+    // blocks stride over the nodes (the grid is sized before the node count is known)
+    for (int i = blockIdx.x; i < nn; i += gridDim.x) {
+        if (nkids[i] != 0) continue;
+        const int b = nbeg[i], m = nend[i] - b;
+        __syncthreads();  // the previous leaf's ranks are read
+        for (int t = threadIdx.x; t < m; t += blockDim.x) v[t] = perm_in[b + t];
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+            const int s = v[t];
+            int r = 0;
+            for (int k = 0; k < m; ++k) r += v[k] < s ? 1 : 0;
+            perm_out[b + r] = s;
+            spos[b + r] = make_double4(x[s], y[s], z[s], 0.0);
+        }
     }
 }
 
@@ -558,6 +573,7 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     Tree* t = c->tree;
     hipStream_t st = c->stream;
     LPM_HIP(c, join_aux(c));  // the second stream may still read the old tree
+    const int64_t prev_nn = t->nn;
     t->ns = n;
     t->nn = 0;
     t->sx = t->sy = t->sz = nullptr;  // set once the build has succeeded
@@ -600,6 +616,12 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
     // would pass kBuildNodeBudget stops at that level and reads the table
     // before going on.
     const int64_t split_bound = (control == 0) ? std::min<int64_t>(n, 8 * (n / (int64_t(leaf_max) + 1))) : n;
+    // leaves of at most kRankMax sources (max-per-node control) get their
+    // ascending source order by ranks within the leaf; others by keys
+    // truncated to the leaf depth, sorted from source order
+    const bool ranked = control == 0 && leaf_max <= kRankMax;
+    const int64_t rank_grid = std::max<int64_t>(4096, prev_nn + prev_nn / 4);
+    if (!c->ev_build) LPM_HIP(c, hipEventCreateWithFlags(&c->ev_build, hipEventDisableTiming));
     int* h = c->pin;
     // key levels sorted at first: kFastLevels, or 8 (24 bits, three radix
     // passes) when the previous build's tree was at most 7 levels deep (a
@@ -658,7 +680,24 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             }
             LPM_HIP(c, hipGetLastError());
             LPM_HIP(c, hipMemcpyAsync(h, lvt, kLvTotal * sizeof(int), hipMemcpyDeviceToHost, st));
-            LPM_HIP(c, hipStreamSynchronize(st));
+            LPM_HIP(c, hipEventRecord(c->ev_build, st));
+            // the node geometry and the leaves' source order of the tree as the
+            // table stands: the whole tree when this run finished it (the
+            // kernels find nothing to do otherwise), computed while the host
+            // waits for the table
+            LPM_CHECK(t->ncent.reserve(size_t(t->cap) * sizeof(double4)));
+            LPM_CHECK(t->nbox.reserve(size_t(t->cap) * 6 * sizeof(double)));
+            hipLaunchKernelGGL(tree_geom, dim3(blocks(t->cap)), dim3(256), 0, st, static_cast<const int*>(lvt), blo[0],
+                               blo[1], blo[2], bhi[0], bhi[1], bhi[2], t->nlevel.as<int>(), t->nprefix.as<uint64_t>(),
+                               t->ncent.as<double4>(), t->nbox.as<double>());
+            // one block per node of the previous tree (and a few more), striding
+            if (ranked && n > 0)
+                hipLaunchKernelGGL(tree_leaf_rank, dim3(unsigned(std::min<int64_t>(t->cap, rank_grid))), dim3(256), 0,
+                                   st, static_cast<const int*>(lvt),
+                                   t->nkids.as<int>(), t->nbeg.as<int>(), t->nend.as<int>(), t->perm.as<int>(),
+                                   t->kperm.as<int>(), x, y, z, t->spos.as<double4>());
+            LPM_HIP(c, hipGetLastError());
+            LPM_HIP(c, hipEventSynchronize(c->ev_build));
             if (h[kLvOutside]) return fail(c, LPM_ERR_INVALID_ARG, "a source lies outside the root box");
             if (h[kLvTooDeep]) return fail(c, LPM_ERR_TREE_DEPTH, "tree deeper than %d levels (coincident sources)", kMaxDepth);
             if (h[kLvKeys]) { resort = true; break; }
@@ -689,22 +728,12 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
         t->nch += h[kLvChunks + w];
     }
 
-    LPM_CHECK(t->ncent.reserve(size_t(nn) * sizeof(double4)));
-    LPM_CHECK(t->nbox.reserve(size_t(nn) * 6 * sizeof(double)));
-    hipLaunchKernelGGL(tree_geom, dim3(blocks(nn)), dim3(256), 0, st, int(nn), blo[0], blo[1], blo[2], bhi[0], bhi[1],
-                       bhi[2], t->nlevel.as<int>(), t->nprefix.as<uint64_t>(), t->ncent.as<double4>(),
-                       t->nbox.as<double>());
+    // tree_geom (and, for ranked leaves, tree_leaf_rank) ran on the finished
+    // table after the last run
     if (n > 0) {
         // leaf-local ascending source order goes to kperm, then the two swap:
-        // perm = leaf order, kperm = the order of the sorted keys.  Leaves of
-        // at most kRankMax sources (max-per-node control): ranks within the
-        // leaf; else keys truncated to the leaf depth, sorted from source order.
-        const bool ranked = control == 0 && leaf_max <= kRankMax;
-        if (ranked) {
-            hipLaunchKernelGGL(tree_leaf_rank, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
-                               t->nbeg.as<int>(), t->nend.as<int>(), t->perm.as<int>(), t->kperm.as<int>(), x, y, z,
-                               t->spos.as<double4>());
-        } else {
+        // perm = leaf order, kperm = the order of the sorted keys
+        if (!ranked) {
             hipLaunchKernelGGL(tree_leaf_keys, dim3(unsigned(nn)), dim3(256), 0, st, int(nn), t->nkids.as<int>(),
                                t->nbeg.as<int>(), t->nend.as<int>(), t->nlevel.as<int>(), t->keys.as<uint64_t>(),
                                t->perm.as<int>(), t->tmp0.as<uint64_t>());
