@@ -145,6 +145,27 @@ __device__ inline bf16x4 cvt_bf16x4(float a, float b, float c, float d) {
     return r;
 }
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU of four FP32 values as BF16: rounded first, then the negative ones
+// (sign bit set: negative as 16-bit integers, -0 included) raised to +0 by a
+// packed 16-bit integer max, two values per instruction.  For finite values
+// this is max(x, 0) rounded; an FP32 max would first quiet its operand (the
+// MFMA results are not known to be canonical), two instructions per value.
+__device__ inline bf16x4 relu_bf16x4(float a, float b, float c, float d) {
+    const bf16x2v lo = __builtin_convertvector((f32x2v){a, b}, bf16x2v);
+    const bf16x2v hi = __builtin_convertvector((f32x2v){c, d}, bf16x2v);
+    const s16x2 z = {0, 0};
+    const s16x2 l = __builtin_elementwise_max(__builtin_bit_cast(s16x2, lo), z);
+    const s16x2 h = __builtin_elementwise_max(__builtin_bit_cast(s16x2, hi), z);
+    bf16x4 r;
+    r[0] = l[0];
+    r[1] = l[1];
+    r[2] = h[0];
+    r[3] = h[1];
+    return r;
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 
@@ -184,7 +205,7 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const f32x4 acc = mfma16(as_bf16x4(f1[nt * 64 + lane]), x, bias(b1 + nt * 16));
-        h1[nt] = cvt_bf16x4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+        h1[nt] = relu_bf16x4(acc[0], acc[1], acc[2], acc[3]);
     }
     const bf16x8 k1[2] = {join_bf16(h1[0], h1[1]), join_bf16(h1[2], h1[3])};
 #pragma unroll
@@ -192,7 +213,7 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
         f32x4 acc = bias(b2 + nt * 16);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) acc = mfma32(f2[(2 * nt + kb) * 64 + lane], k1[kb], acc);
-        h2[nt] = cvt_bf16x4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+        h2[nt] = relu_bf16x4(acc[0], acc[1], acc[2], acc[3]);
     }
     f32x4 acc = bias(b3);
     acc = mfma32(f3[lane], join_bf16(h2[0], h2[1]), acc);

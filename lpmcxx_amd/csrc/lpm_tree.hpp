@@ -54,6 +54,7 @@ struct Tree {
     // node table (device)
     DevBuf nbeg, nend, nfirst, nkids, nlevel, nparent;  // int32 [cap]
     DevBuf nprefix;   // uint64 [cap] key prefix (top 3*level bits)
+    DevBuf nodew;     // int4 [nn] begin, end, first child, child count (one load in the evaluation kernels)
     DevBuf ncent;     // double4 [nn] centroid + maxRadius
     DevBuf nbox;      // double [nn][6] lo3 hi3
     DevBuf nrad;      // double [nn] source radius about the box centroid: exact at leaves,

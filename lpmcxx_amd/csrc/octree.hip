@@ -427,12 +427,17 @@ __device__ inline int table_nodes(const int* lvt) {
 // order of src/LpmBox3d.cpp:55-66 of distance(centroid, corner).  The node
 // count comes from the build table, so the kernel can be enqueued before the
 // host has read it.
+// Also packs each node's begin, end, first child and child count into one
+// int4 (nodew).
 __global__ __launch_bounds__(256) void tree_geom(const int* __restrict__ lvt, double x0, double y0, double z0,
                                                  double x1, double y1, double z1, const int* __restrict__ nlevel,
-                                                 const uint64_t* __restrict__ nprefix, double4* __restrict__ ncent,
-                                                 double* __restrict__ nbox) {
+                                                 const uint64_t* __restrict__ nprefix, const int* __restrict__ nbeg,
+                                                 const int* __restrict__ nend, const int* __restrict__ nfirst,
+                                                 const int* __restrict__ nkids, double4* __restrict__ ncent,
+                                                 double* __restrict__ nbox, int4* __restrict__ nodew) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= table_nodes(lvt)) return;
+    nodew[i] = make_int4(nbeg[i], nend[i], nfirst[i], nkids[i]);
     const int L = nlevel[i];
     const uint64_t pre = nprefix[i];
     double b[6] = {x0, y0, z0, x1, y1, z1};
@@ -560,7 +565,7 @@ constexpr int64_t kBuildNodeBudget = int64_t(16) << 20;  // node bound of the le
 void tree_free(Tree* t) {
     if (!t) return;
     for (DevBuf* b : {&t->perm, &t->kperm, &t->spos, &t->keys, &t->nbeg, &t->nend, &t->nfirst, &t->nkids, &t->nlevel,
-                      &t->nparent, &t->nprefix, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32,
+                      &t->nparent, &t->nprefix, &t->nodew, &t->ncent, &t->nbox, &t->nrad, &t->mom, &t->momg, &t->mom32,
                       &t->soff32, &t->nstat, &t->mpart, &t->pmom, &t->pmomg, &t->pmom32, &t->tmp0, &t->tmp1,
                       &t->tmp2, &t->tmp3, &t->sort_tmp, &t->scan_tmp, &t->lvt, &t->chunks})
         b->release();
@@ -687,9 +692,11 @@ int tree_build_device(lpm_ctx* c, const double* x, const double* y, const double
             // waits for the table
             LPM_CHECK(t->ncent.reserve(size_t(t->cap) * sizeof(double4)));
             LPM_CHECK(t->nbox.reserve(size_t(t->cap) * 6 * sizeof(double)));
+            LPM_CHECK(t->nodew.reserve(size_t(t->cap) * sizeof(int4)));
             hipLaunchKernelGGL(tree_geom, dim3(blocks(t->cap)), dim3(256), 0, st, static_cast<const int*>(lvt), blo[0],
                                blo[1], blo[2], bhi[0], bhi[1], bhi[2], t->nlevel.as<int>(), t->nprefix.as<uint64_t>(),
-                               t->ncent.as<double4>(), t->nbox.as<double>());
+                               t->nbeg.as<int>(), t->nend.as<int>(), t->nfirst.as<int>(), t->nkids.as<int>(),
+                               t->ncent.as<double4>(), t->nbox.as<double>(), t->nodew.as<int4>());
             // one block per node of the previous tree (and a few more), striding
             if (ranked && n > 0)
                 hipLaunchKernelGGL(tree_leaf_rank, dim3(unsigned(std::min<int64_t>(t->cap, rank_grid))), dim3(256), 0,

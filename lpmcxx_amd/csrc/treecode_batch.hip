@@ -244,6 +244,7 @@ struct TreeView {
     const double4* ncent;
     const double* nrad;
     const int *nfirst, *nkids, *nbeg, *nend;
+    const int4* nodew;  // the four words above, packed
     const float4* nstat;
     const double4* spos;
     const float4* soff;
@@ -565,17 +566,20 @@ __device__ inline int64_t list_batch(const Lists& ls, int64_t w) {
     return __builtin_amdgcn_readfirstlane(ls.queue[int64_t(b) * ls.qcap + (w - int64_t(before))]);
 }
 
-// six waves per SIMD (76 VGPRs, 78 with STEREO, no spills): the traversal
-// waits on node loads.  STEREO: accepted pairs may become stereographic
-// entries (the batch's plane centre and radius are kept in scalar registers).
+// Five waves per SIMD (96 registers with the accumulation registers the
+// compiler uses as spill space; a six-wave bound spilled to scratch and was
+// no faster): the traversal waits on node loads.  STEREO: accepted pairs may
+// become stereographic entries (the batch's plane centre and radius in both
+// projections are kept in scalar registers).
 template <bool STEREO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
+__global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
                                                          int max_order, double u_ref,
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
                                                          Lists ls, StereoView sv,
                                                          unsigned long long* __restrict__ stats) {
     __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackPlan];
+    __shared__ int s_row[4][16];  // child node of each row of a model call
     // the input normalization in LDS (mean, inv_std, lo, hi; 4 inputs per
     // float4): a lane reads the four of its input group per model call
     // instead of the kernel holding 64 values in scalar registers
@@ -609,12 +613,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
     unsigned l_terms = 0, l_terms_hi = 0, l_np = 0, l_terms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
     int sp = 0;
-    if (tv.nkids[0] == 0) {  // the root is a leaf
+    const int4 root = tv.nodew[0];
+    if (root.w == 0) {  // the root is a leaf
         if (lane == 0 && ls.near_cap > 0) near[0] = 0 | kNearCheck;
         nnear = 1;
-        near_pairs = unsigned(tv.nend[0] - tv.nbeg[0]);
+        near_pairs = unsigned(root.y - root.x);
     } else {
-        s_node[w][0] = stack_entry(tv.nfirst[0], tv.nkids[0]);
+        s_node[w][0] = stack_entry(root.z, root.w);
         sp = 1;
     }
     // The stack holds internal nodes to descend into, as their first child and
@@ -632,33 +637,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
             kc = ent >> kNodeBits;
             kf = ent & ((1 << kNodeBits) - 1);
         }
-        // rows [rb[j], rb[j] + children) belong to popped entry j
-        int rb[8], rf[8];
-        int m = 0, tot = 0;
+        // the popped entries are those whose children fit in the rows so far
+        // (inclusive sums of the child counts over lanes 0..7, a prefix of
+        // the lanes); entry j's children take rows incl - kc .. incl - 1,
+        // and s_row maps each row to its child node
+        int incl = kc;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int cj = __builtin_amdgcn_readlane(kc, j);
-            rf[j] = __builtin_amdgcn_readlane(kf, j);
-            const bool take = j < ncand && j == m && tot + cj <= 16;
-            rb[j] = take ? tot : (1 << 20);
-            if (take) {
-                tot += cj;
-                ++m;
-            }
+        for (int o = 1; o < 8; o <<= 1) {
+            const int v = __shfl_up(incl, o, 8);
+            if ((lane & 7) >= o) incl += v;
         }
+        const bool take = lane < ncand && incl <= 16;
+        const int m = __popcll(__ballot(take));
+        const int tot = __shfl(incl, m - 1, 64);
+        if (take) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (k < kc) s_row[w][incl - kc + k] = kf + k;
+        }
+        wave_sync();
         sp -= m;
         // lane l works on row r = l % 16; the lanes of one row hold its
         // inputs 4g..4g+3 (g = l / 16)
         const int row = lane & 15, g = lane >> 4;
-        int base = rb[0], firstc = rf[0];
-#pragma unroll
-        for (int j = 1; j < 8; ++j)
-            if (rb[j] <= row) {
-                base = rb[j];
-                firstc = rf[j];
-            }
         const bool has = row < tot;
-        const int chr = firstc + row - base;
+        const int chr = has ? s_row[w][row] : 0;
         bool prop = false;
         bool near_chk = true;  // a leaf child's sources may come within 1e-6 of a target
         // stereographic geometry test of the pair in the batch's projection,
@@ -669,10 +672,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
         // every node-table word of the child is loaded here, in one round
         int ckids = 0, cfirst = 0, cb = 0, ce = 0;
         if (has) {
-            ckids = tv.nkids[chr];
-            cfirst = tv.nfirst[chr];
-            cb = tv.nbeg[chr];
-            ce = tv.nend[chr];
+            const int4 nw = tv.nodew[chr];
+            cb = nw.x;
+            ce = nw.y;
+            cfirst = nw.z;
+            ckids = nw.w;
             const double4 cc = tv.ncent[chr];
             const double dx = bt.cx - cc.x, dy = bt.cy - cc.y, dz = bt.cz - cc.z;
             const double sd = sqrt(dx * dx + dy * dy + dz * dz);
@@ -684,10 +688,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void t
                 if (!st_geo && so_on) so_geo = stereo_fits(sv, so, chr, sd, rn, bt.r);
                 pair_features_group(g, x0, x1, x2, x3, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
                 const float4 mu = s_norm[0][g], iv = s_norm[1][g], lo = s_norm[2][g], hi = s_norm[3][g];
-                x0 = (fminf(fmaxf(x0, lo.x), hi.x) - mu.x) * iv.x;
-                x1 = (fminf(fmaxf(x1, lo.y), hi.y) - mu.y) * iv.y;
-                x2 = (fminf(fmaxf(x2, lo.z), hi.z) - mu.z) * iv.z;
-                x3 = (fminf(fmaxf(x3, lo.w), hi.w) - mu.w) * iv.w;
+                // clamp to [lo, hi] (lo <= hi): the median of the three
+                x0 = (__builtin_amdgcn_fmed3f(x0, lo.x, hi.x) - mu.x) * iv.x;
+                x1 = (__builtin_amdgcn_fmed3f(x1, lo.y, hi.y) - mu.y) * iv.y;
+                x2 = (__builtin_amdgcn_fmed3f(x2, lo.z, hi.z) - mu.z) * iv.z;
+                x3 = (__builtin_amdgcn_fmed3f(x3, lo.w, hi.w) - mu.w) * iv.w;
             }
         }
         int p = 0;
@@ -834,12 +839,13 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
         for (int j = 0; j < nnear; ++j) {
             const int ent = __builtin_amdgcn_readfirstlane(near[j]);
             const int n = ent & ~kNearCheck;
+            const int4 nw = tv.nodew[n];
             if (!F32 || (ent & kNearCheck))
-                near_leaf<F32, true>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff,
-                                     s0, s1, s2, v0, v1, v2);
+                near_leaf<F32, true>(nw.x, nw.y, t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
+                                     v0, v1, v2);
             else
-                near_leaf<F32, false>(tv.nbeg[n], tv.nend[n], t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff,
-                                      s0, s1, s2, v0, v1, v2);
+                near_leaf<F32, false>(nw.x, nw.y, t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
+                                      v0, v1, v2);
         }
     }
     for (int j = j0; j < j1; ++j) {
@@ -1075,6 +1081,7 @@ TreeView view(const Tree* t) {
     v.nkids = t->nkids.as<int>();
     v.nbeg = t->nbeg.as<int>();
     v.nend = t->nend.as<int>();
+    v.nodew = t->nodew.as<int4>();
     v.nstat = t->nstat.as<float4>();
     v.spos = t->spos.as<double4>();
     v.soff = t->soff32.as<float4>();
