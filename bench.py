@@ -693,7 +693,7 @@ def main() -> int:
     ap.add_argument("--leaf", type=int, default=None, help="default 192 (stereographic), 128 (Cartesian)")
     ap.add_argument("--theta-max", type=float, default=None, help="default 0.92 (stereographic), 0.85 (Cartesian)")
     ap.add_argument("--tau", type=float, default=None,
-                    help="per-pair error bound of the labels (default 3e-6 with the stereographic far field, "
+                    help="per-pair error bound of the labels (default 2.5e-6 with the stereographic far field, "
                          "3e-7 without)")
     ap.add_argument("--no-stereo", action="store_true",
                     help="Cartesian far field only (no LPM_OPT_STEREO in the learned evaluation)")
@@ -746,7 +746,7 @@ def main() -> int:
     if args.leaf is None:
         args.leaf = 128 if args.no_stereo else 192
     if args.tau is None:
-        args.tau = 3e-7 if args.no_stereo else 3e-6
+        args.tau = 3e-7 if args.no_stereo else 2.5e-6
     if args.l10_tau is None:
         args.l10_tau = 1.5e-7 if args.no_stereo else 1.5e-6
     if args.bve_tau is None:
