@@ -529,7 +529,8 @@ void TreeSum::ensureMoments(int order) {
     }
     _uref /= 4.0 * PI;
     const bool stereo = _learned && _trained && _model.stereoExtra >= 0;
-    _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO, stereo ? 1.0 : 0.0), "TreeSum::computeMeshMoments");
+    _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO, stereo ? (_model.stereoOnly ? 2.0 : 1.0) : 0.0),
+                "TreeSum::computeMeshMoments");
     if (stereo) {
         _gpu->check(lpm_set_option(_gpu->get(), LPM_OPT_STEREO_EXTRA_ORDER, double(_model.stereoExtra)),
                     "TreeSum::computeMeshMoments");

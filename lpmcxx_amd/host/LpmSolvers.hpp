@@ -316,11 +316,14 @@ struct OrderModelConfig {
     scalar_type keepRate = 0.3;   // fraction of its candidate pairs kept
     uint64_t seed = 1;
     // stereographic far field (LPM_OPT_STEREO) in meshSolve: -1 = off, else the
-    // extra order of the plane series (the bench uses 1 with tau 7e-7,
-    // thetaMax 0.92 and stereoRatio 2.0); leaves are then priced as
-    // stereographic entries
+    // extra order of the plane series (the bench uses 1 with stereoOnly,
+    // tau 2.5e-6, thetaMax 0.92, stereoRatio 2.0 and leaves of 192); leaves are
+    // then priced as stereographic entries
     int stereoExtra = -1;
     scalar_type stereoRatio = 2.0;
+    // LPM_OPT_STEREO 2: the stereographic far field alone (Cartesian moments of
+    // order 3 for the model's features, pairs no projection takes descend)
+    bool stereoOnly = false;
 };
 
 class TreeSum : public Tree {

@@ -87,7 +87,7 @@ int main() {
         // batch acceptance and the learned order model through the same API
         CubedSphereSeed cseed;
         auto pm = std::make_shared<MeshedParticles>(cseed, 6, true, 1.0, 0);
-        for (const char* n : {"velocity", "batchVelocity", "learnedVelocity", "stereoVelocity"}) {
+        for (const char* n : {"velocity", "batchVelocity", "learnedVelocity", "stereoVelocity", "stereoOnlyVelocity"}) {
             pm->createVertexField(n, "length/time", 3);
             pm->createFaceField(n, "length/time", 3);
         }
@@ -119,12 +119,22 @@ int main() {
         int macS = 0;
         st.meshSolve(mpiVerts, mpiFaces, 0.0, macS);
         const double serr = maxRelErr(*pm->getFaceFieldPtr("stereoVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
+        // and with the stereographic far field alone (LPM_OPT_STEREO 2)
+        TreeSum so(8, 64, pm, kernel, "stereoOnlyVelocity", "vorticity");
+        cfg.stereoOnly = true;
+        so.trainOrderModel(cfg);
+        so.setPrecision(LPM_F32C);
+        int macO = 0;
+        so.meshSolve(mpiVerts, mpiFaces, 0.0, macO);
+        const double oerr = maxRelErr(*pm->getFaceFieldPtr("stereoOnlyVelocity"), *pm->getFaceFieldPtr("velocity"), *pm, true);
         std::printf("cubed level 6: batch tree vs direct %.3e (far pairs %d), learned vs direct %.3e (far pairs %d, "
-                    "training loss %.3f), learned FP32C with the stereographic far field %.3e (far pairs %d)\n",
-                    berr, macB, lerr, macL, loss, serr, macS);
+                    "training loss %.3f), learned FP32C with the stereographic far field %.3e (far pairs %d), "
+                    "stereographic only %.3e (far pairs %d)\n",
+                    berr, macB, lerr, macL, loss, serr, macS, oerr, macO);
         CHECK(berr < 1e-4 && macB > 0);
         CHECK(lerr < 1e-3 && macL > 0);
         CHECK(serr < 1e-3 && macS > 0);
+        CHECK(oerr < 1e-3 && macO > 0);
         CHECK(loss < 1.5f);
     }
     {
