@@ -82,10 +82,13 @@ __global__ __launch_bounds__(256) void stereo_leaf(int nn, int T, const int* __r
     for (int k = 0; k < kStereoTerms; ++k) a[k] = make_double2(0.0, 0.0);
     double rho2 = 0.0;
     const int e = nend[n];
-    for (int j = nbeg[n] + l; ok && j < e; j += kLeafLanes) {
+    // every source is summed (a pole point only spoils the result, which is
+    // then discarded), so the loads of successive sources do not wait for
+    // each other's pole test
+    for (int j = nbeg[n] + l; j < e; j += kLeafLanes) {
         const double4 s = spos[j];
         double zr, zi;
-        ok = stereo_point(proj, s.x, s.y, s.z, zr, zi);
+        ok = stereo_point(proj, s.x, s.y, s.z, zr, zi) && ok;
         const double dr = zr - cr, di = zi - ci;
         rho2 = fmax(rho2, dr * dr + di * di);
         double pr = s.w, pim = 0.0;  // g (z - z_c)^k
