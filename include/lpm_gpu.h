@@ -106,6 +106,11 @@ void* lpm_stream(lpm_ctx* ctx);
  * ux,uy,uz : required when the velocity kernel is requested, psi when the
  *             potential kernel is requested; others may be NULL.
  * nt or ns may be 0 (outputs are then zero / untouched respectively).
+ * With LPM_F32C and 65,536 or more targets or sources, the sum runs on
+ * copies of the targets and sources sorted along the sphere (cube face, then
+ * 2D Hilbert index; workspace about 40 bytes per target and 36 per source
+ * beyond the unsorted sum's), so its speed does not depend on the caller's
+ * order; outputs are in the caller's order.
  */
 int lpm_direct_sum(lpm_ctx* ctx, int kernels, int prec, double radius, double delta,
                    const double* tx, const double* ty, const double* tz, int64_t nt,
@@ -126,7 +131,10 @@ int lpm_direct_sum_dev(lpm_ctx* ctx, int kernels, int prec, double radius, doubl
  * until the process ends), so it calls no hipMalloc / hipFree.  Fits when
  * 256 + ns x 48 (LPM_F32C; 32 for LPM_F64[F], rounded up to 256 sources and 256
  * bytes) + nt x 32 bytes <= 32 MiB, e.g. 98,304 sources with up to 900,000
- * targets; larger problems return LPM_ERR_INVALID_ARG.  When the partials of
+ * targets; larger problems return LPM_ERR_INVALID_ARG.  Unlike
+ * lpm_direct_sum_dev it does not sort large LPM_F32C problems: its speed
+ * needs targets and sources in a spatially coherent order (mesh order is).
+ * When the partials of
  * every split do not fit, fewer splits are used (the sum over splits then
  * runs in another order: results agree with lpm_direct_sum_dev to rounding).
  * With LPM_F32C it waits once for the stream to check the sphere

@@ -42,8 +42,13 @@
 // and the reference's zero rule |R^2 - x.y + delta^2| <= 1e-13, i.e.
 // |d|^2 + 2 delta^2 <= 2e-13, applies.  All other tiles run the plain
 // 11-instruction FP32 pair loop.  Inputs whose order is not spatially coherent
-// make every tile near: results stay accurate, at FP64 speed.
+// would make every tile near (accurate, at FP64 speed): direct_sum_device
+// therefore sums large FP32C problems on copies sorted along the sphere (cube
+// face, then Hilbert index), whatever the caller's order.  The no-allocation
+// entry keeps the caller's order.
 #include "lpm_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -514,6 +519,113 @@ int ds_launch(lpm_ctx* c, const DsPlan& P, int kernels, int prec, double radius,
     return LPM_OK;
 }
 
+// ---------------------------------------------------------------- spatial order (FP32C)
+// The FP32C loop needs waves of targets and tiles of sources that are small
+// in space; in an order that is not (shuffled particles), every tile is near
+// every wave and runs the FP64 loop.  Large FP32C sums therefore run on
+// copies of the targets and sources sorted by a key that follows the sphere:
+// the cube face the point projects to (largest coordinate) and the 2D
+// Hilbert index of its gnomonic coordinates on that face (13 bits a side);
+// the outputs are scattered back.  A Hilbert curve has no jumps, so runs of
+// consecutive keys stay compact on the sphere.
+
+constexpr int kHilbertBits = 13;
+
+__global__ __launch_bounds__(256) void ds_order_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, int64_t n,
+                                                     uint32_t* __restrict__ key, int* __restrict__ val) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double X = x[i], Y = y[i], Z = z[i];
+    const double ax = fabs(X), ay = fabs(Y), az = fabs(Z);
+    uint32_t face;
+    double a, b, m;
+    if (ax >= ay && ax >= az) {
+        face = X >= 0.0 ? 0u : 1u;
+        a = Y, b = Z, m = ax;
+    } else if (ay >= az) {
+        face = Y >= 0.0 ? 2u : 3u;
+        a = Z, b = X, m = ay;
+    } else {
+        face = Z >= 0.0 ? 4u : 5u;
+        a = X, b = Y, m = az;
+    }
+    constexpr uint32_t side = 1u << kHilbertBits;
+    auto q = [&](double v) {
+        const double u = m > 0.0 ? (v / m + 1.0) * (0.5 * side) : 0.0;  // [0, side] on the face
+        return uint32_t(u < 0.0 ? 0.0 : (u > side - 1 ? double(side - 1) : u));
+    };
+    uint32_t hx = q(a), hy = q(b), d = 0;
+    for (uint32_t s = side >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (hx & s) ? 1u : 0u, ry = (hy & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {  // rotate the quadrant
+            if (rx == 1) {
+                hx = side - 1 - hx;
+                hy = side - 1 - hy;
+            }
+            const uint32_t t = hx;
+            hx = hy;
+            hy = t;
+        }
+    }
+    key[i] = (face << (2 * kHilbertBits)) | d;
+    val[i] = int(i);
+}
+
+// position of every source in the sorted order
+__global__ __launch_bounds__(256) void ds_order_inverse(const int* __restrict__ perm, int64_t n, int* __restrict__ inv) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < n) inv[perm[j]] = int(j);
+}
+
+__global__ __launch_bounds__(256) void ds_order_gather(const int* __restrict__ perm, int64_t n,
+                                                       const double* __restrict__ a, const double* __restrict__ b,
+                                                       const double* __restrict__ c, const double* __restrict__ d,
+                                                       double* __restrict__ as, double* __restrict__ bs,
+                                                       double* __restrict__ cs, double* __restrict__ ds,
+                                                       const int64_t* __restrict__ skip, const int* __restrict__ inv,
+                                                       int64_t* __restrict__ skip_s) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = perm[i];
+    as[i] = a[k];
+    bs[i] = b[k];
+    cs[i] = c[k];
+    if (d) ds[i] = d[k];
+    if (skip) skip_s[i] = skip[k] >= 0 ? int64_t(inv[skip[k]]) : skip[k];
+}
+
+__global__ __launch_bounds__(256) void ds_order_scatter(const int* __restrict__ perm, int64_t n,
+                                                        const double* __restrict__ u, const double* __restrict__ v,
+                                                        const double* __restrict__ w, const double* __restrict__ p,
+                                                        double* ux, double* uy, double* uz, double* psi) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = perm[i];
+    if (ux) {
+        ux[k] = u[i];
+        uy[k] = v[i];
+        uz[k] = w[i];
+    }
+    if (psi) psi[k] = p[i];
+}
+
+// sums with at least this many targets or sources take the sorted copies
+// (the sort's counts are 32-bit: larger sums keep the caller's order)
+constexpr int64_t kOrderMin = 65536;
+
+constexpr int kKeyBits = 2 * kHilbertBits + 3;
+
+int sort_order(lpm_ctx* c, const double* x, const double* y, const double* z, int64_t n, uint32_t* kin,
+               uint32_t* kout, int* vin, int* vout, void* tmp, size_t tmp_bytes) {
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(ds_order_keys, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, x, y, z, n, kin, vin);
+    size_t b = tmp_bytes;
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, b, kin, kout, vin, vout, int(n), 0, kKeyBits, st));
+    return LPM_OK;
+}
+
 }  // namespace
 
 int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double delta, const double* tx,
@@ -527,8 +639,52 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
         LPM_CHECK(c->ds_misc.reserve(64));
         LPM_CHECK(c->ds_src.reserve(P.src_bytes));
     }
-    return ds_launch(c, P, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz, psi,
-                     c->ds_src.ptr, c->ds_part.as<double>(), c->ds_misc.as<double>());
+    if (prec != LPM_F32C || ns == 0 || std::max(nt, ns) < kOrderMin || std::max(nt, ns) > INT32_MAX)
+        return ds_launch(c, P, kernels, prec, radius, delta, tx, ty, tz, nt, sx, sy, sz, gamma, ns, skip, ux, uy, uz,
+                         psi, c->ds_src.ptr, c->ds_part.as<double>(), c->ds_misc.as<double>());
+    // sorted copies: keys and values of targets and sources, the source
+    // positions, the copies, the outputs, the sort's temporary storage
+    hipStream_t st = c->stream;
+    const int64_t nm = std::max(nt, ns);
+    size_t tmp_bytes = 0;
+    LPM_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, static_cast<uint32_t*>(nullptr),
+                                                  static_cast<uint32_t*>(nullptr), static_cast<int*>(nullptr),
+                                                  static_cast<int*>(nullptr), int(nm), 0, kKeyBits, st));
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_kin = 0, o_kout = o_kin + al(4 * nm), o_vin = o_kout + al(4 * nm), o_pt = o_vin + al(4 * nm),
+                 o_ps = o_pt + al(4 * nt), o_inv = o_ps + al(4 * ns), o_t = o_inv + al(4 * ns),
+                 o_s = o_t + al(3 * 8 * nt), o_sk = o_s + al(4 * 8 * ns), o_out = o_sk + al(8 * nt),
+                 o_tmp = o_out + al(4 * 8 * nt), total = o_tmp + al(tmp_bytes);
+    LPM_CHECK(c->ds_ord.reserve(total));
+    char* w = c->ds_ord.as<char>();
+    uint32_t *kin = reinterpret_cast<uint32_t*>(w + o_kin), *kout = reinterpret_cast<uint32_t*>(w + o_kout);
+    int *vin = reinterpret_cast<int*>(w + o_vin), *pt = reinterpret_cast<int*>(w + o_pt);
+    int *ps = reinterpret_cast<int*>(w + o_ps), *inv = reinterpret_cast<int*>(w + o_inv);
+    double* T = reinterpret_cast<double*>(w + o_t);
+    double* S = reinterpret_cast<double*>(w + o_s);
+    int64_t* sk = skip ? reinterpret_cast<int64_t*>(w + o_sk) : nullptr;
+    double* O = reinterpret_cast<double*>(w + o_out);
+    LPM_CHECK(sort_order(c, tx, ty, tz, nt, kin, kout, vin, pt, w + o_tmp, tmp_bytes));
+    LPM_CHECK(sort_order(c, sx, sy, sz, ns, kin, kout, vin, ps, w + o_tmp, tmp_bytes));
+    const unsigned bs = unsigned((ns + 255) / 256), bt = unsigned((nt + 255) / 256);
+    hipLaunchKernelGGL(ds_order_inverse, dim3(bs), dim3(256), 0, st, static_cast<const int*>(ps), ns, inv);
+    hipLaunchKernelGGL(ds_order_gather, dim3(bs), dim3(256), 0, st, static_cast<const int*>(ps), ns, sx, sy, sz, gamma,
+                       S, S + ns, S + 2 * ns, S + 3 * ns, static_cast<const int64_t*>(nullptr),
+                       static_cast<const int*>(nullptr), static_cast<int64_t*>(nullptr));
+    hipLaunchKernelGGL(ds_order_gather, dim3(bt), dim3(256), 0, st, static_cast<const int*>(pt), nt, tx, ty, tz,
+                       static_cast<const double*>(nullptr), T, T + nt, T + 2 * nt, static_cast<double*>(nullptr), skip,
+                       static_cast<const int*>(inv), sk);
+    const bool vel = (kernels & LPM_K_BIOT_SAVART_SPHERE) != 0;
+    const bool pot = (kernels & LPM_K_GREENS_SPHERE) != 0;
+    LPM_CHECK(ds_launch(c, P, kernels, prec, radius, delta, T, T + nt, T + 2 * nt, nt, S, S + ns, S + 2 * ns, S + 3 * ns,
+                        ns, sk, vel ? O : nullptr, vel ? O + nt : nullptr, vel ? O + 2 * nt : nullptr,
+                        pot ? O + 3 * nt : nullptr, c->ds_src.ptr, c->ds_part.as<double>(), c->ds_misc.as<double>()));
+    hipLaunchKernelGGL(ds_order_scatter, dim3(bt), dim3(256), 0, st, static_cast<const int*>(pt), nt,
+                       static_cast<const double*>(O), static_cast<const double*>(O + nt),
+                       static_cast<const double*>(O + 2 * nt), static_cast<const double*>(O + 3 * nt),
+                       vel ? ux : nullptr, vel ? uy : nullptr, vel ? uz : nullptr, pot ? psi : nullptr);
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
 }
 
 // ---------------------------------------------------------------- static scratch

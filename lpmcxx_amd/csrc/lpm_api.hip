@@ -140,7 +140,7 @@ void lpm_destroy(lpm_ctx* c) {
     tree_free(c->tree);
     model_free(c->model);
     c->tree = nullptr;
-    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->h_in, &c->h_out, &c->rk_work,
+    for (DevBuf* b : {&c->ds_src, &c->ds_tgt, &c->ds_part, &c->ds_misc, &c->ds_ord, &c->h_in, &c->h_out, &c->rk_work,
                       &c->tb_work, &c->tl_work, &c->mesh_work, &c->lab_work, &c->r2w})
         b->release();
     if (c->ev_beg) (void)hipEventDestroy(c->ev_beg);

@@ -66,6 +66,8 @@ struct lpm_ctx {
     std::string err;
     // direct-sum workspace
     lpm::DevBuf ds_src, ds_tgt, ds_part, ds_misc;
+    // sorted copies of the FP32C direct sum's targets and sources
+    lpm::DevBuf ds_ord;
     // staging for the host-pointer entry points
     lpm::DevBuf h_in, h_out;
     // RK4 stage storage

@@ -188,3 +188,36 @@ def test_argument_errors(ctx):
     st = lib.lpm_direct_sum(ctx.h, VEL, 0, 1.0, 0.0, None, None, None, 5, None, None, None, None, 5,
                             None, None, None, None, None)
     assert st == -2
+
+
+def test_f32c_shuffled_order(ctx, mesh):
+    """FP32C on the cubed level 7 particles in a random order: lpm_direct_sum
+    sorts targets and sources by position, so the result equals the mesh-order
+    sum to FP32C round-off, the self skip follows the shuffled sources, and the
+    pair kernel runs at the mesh-order speed (no tile falls back to FP64)."""
+    m = mesh(L.LPM_SEED_CUBED, 7)
+    tgt, src, g, skip = _setup(m)
+    rng = np.random.default_rng(5)
+    pt, ps = rng.permutation(len(tgt[0])), rng.permutation(len(src[0]))
+    inv_ps = np.empty_like(ps)
+    inv_ps[ps] = np.arange(len(ps))
+    tgt_s = tuple(t[pt] for t in tgt)
+    src_s = tuple(s[ps] for s in src)
+    sk = skip[pt]
+    skip_s = np.where(sk >= 0, inv_ps[np.maximum(sk, 0)], -1)
+    u0, p0 = ctx.direct_sum(tgt, src, g, kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip)
+    ms0 = ctx.last_kernel_ms()
+    u1, p1 = ctx.direct_sum(tgt_s, src_s, g[ps], kernels=VEL | PSI, prec=L.LPM_F32C, skip=skip_s)
+    ms1 = ctx.last_kernel_ms()
+    back = np.empty_like(pt)
+    back[pt] = np.arange(len(pt))
+    u1 = tuple(u[back] for u in u1)
+    p1 = p1[back]
+    e_u, e_p = O.rel_l2(u1, u0), O.rel_l2(p1, p0)
+    print(f"\ncubed L7 f32c shuffled vs mesh order: rel-L2 velocity {e_u:.2e} streamfunction {e_p:.2e}; "
+          f"pair kernel {ms1:.3f} ms shuffled, {ms0:.3f} ms in mesh order")
+    assert e_u <= 1e-6 and e_p <= 1e-6
+    idx = np.sort(rng.choice(len(tgt[0]), 512, replace=False))
+    u_ref = O.bs_direct(tuple(t[idx] for t in tgt), src, g, skip=skip[idx])
+    assert O.rel_l2(tuple(u[idx] for u in u1), u_ref) <= 1e-5
+    assert ms1 <= 1.5 * ms0 + 0.05
