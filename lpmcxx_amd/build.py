@@ -34,6 +34,11 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I", str(ROOT / "include"),
 ]
+# Per-file additions.  treecode_batch.hip: MFMA accumulators and results in
+# ordinary VGPRs (the planning pass converts every MFMA result to BF16 with
+# VALU instructions; in accumulation registers each result first needs a copy
+# out, about 36 instructions per model call).
+HIP_FILE_FLAGS = {"treecode_batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare",
              "-I", str(ROOT / "include"), "-I", str(HOST)]
 
@@ -50,7 +55,7 @@ def build_gpu(jobs: int = 8) -> Path:
     srcs = sorted(CSRC.glob("*.hip"))
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(o)]) for s, o in zip(srcs, objs)]
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, *HIP_FILE_FLAGS.get(s.name, []), "-c", str(s), "-o", str(o)]) for s, o in zip(srcs, objs)]
         for f in futs:
             f.result()
     out = LIB / "liblpm_gpu.so"

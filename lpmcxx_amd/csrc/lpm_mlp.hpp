@@ -145,6 +145,11 @@ __device__ inline bf16x4 cvt_bf16x4(float a, float b, float c, float d) {
     return r;
 }
 
+// two floats to BF16 (round to nearest even), a in the low half
+__device__ inline unsigned cvt_bf16x2(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2v){a, b}, bf16x2v));
+}
+
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 // ReLU of four FP32 values as BF16: rounded first, then the negative ones
@@ -231,6 +236,81 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
         if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
     }
     return best;
+}
+
+// Inference for up to 16 RT rows in RT row tiles (tile t: rows 16 t ..
+// 16 t + 15), with the layout and operations of mlp_classify_regs per tile:
+// x[t] holds lane (g, c)'s inputs 4g..4g+3 of row 16 t + c.  Each weight
+// fragment is read from LDS once and applied to every tile.  Returns in lane
+// l the argmax class of row l (for RT = 1, of row l % 16 in every lane).
+template <int RT>
+__device__ inline int mlp_classify_tiles(const unsigned char* model, const bf16x4 (&x)[RT]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const uint2* f1 = reinterpret_cast<const uint2*>(model);
+    const uint4* f2 = reinterpret_cast<const uint4*>(model + kFragW1);
+    const uint4* f3 = reinterpret_cast<const uint4*>(model + kFragW1 + kFragW2);
+    const float* b1 = reinterpret_cast<const float*>(model + kFragBytes);
+    const float* b2 = b1 + kHid;
+    const float* b3 = b2 + kHid;
+    auto bias = [&](const float* bv) {
+        const float4 b = *reinterpret_cast<const float4*>(bv + 4 * g);
+        return f32x4{b.x, b.y, b.z, b.w};
+    };
+    bf16x4 h1[RT][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const bf16x4 w = as_bf16x4(f1[nt * 64 + lane]);
+        const f32x4 b = bias(b1 + nt * 16);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const f32x4 acc = mfma16(w, x[t], b);
+            h1[t][nt] = relu_bf16x4(acc[0], acc[1], acc[2], acc[3]);
+        }
+    }
+    f32x4 out[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) out[t] = bias(b3);
+    // layer 2 in two halves of two output tiles each; each half is the K
+    // block of layer 3 that consumes it
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        bf16x4 h2[RT][2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int nt = 2 * hf + j;
+            f32x4 acc[RT];
+#pragma unroll
+            for (int t = 0; t < RT; ++t) acc[t] = bias(b2 + nt * 16);
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                const uint4 w = f2[(2 * nt + kb) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) acc[t] = mfma32(w, join_bf16(h1[t][2 * kb], h1[t][2 * kb + 1]), acc[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < RT; ++t) h2[t][j] = relu_bf16x4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+        }
+        const uint4 w = f3[hf * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) out[t] = mfma32(w, join_bf16(h2[t][0], h2[t][1]), out[t]);
+    }
+    int mine = 0;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        float v = -3.0e38f;
+        int best = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * g + i < kClasses && out[t][i] > v) { v = out[t][i]; best = 4 * g + i; }
+#pragma unroll
+        for (int o = 16; o < 64; o <<= 1) {
+            const float ov = __shfl_xor(v, o, 64);
+            const int ob = __shfl_xor(best, o, 64);
+            if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
+        }
+        if (RT == 1 || t == g) mine = best;
+    }
+    return mine;
 }
 
 }  // namespace lpm

@@ -26,6 +26,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 namespace lpm {
@@ -482,19 +483,27 @@ __global__ __launch_bounds__(256) void tree_leaf_rank(const int* __restrict__ lv
                                                       const int* __restrict__ perm_in, int* __restrict__ perm_out,
                                                       const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z, double4* __restrict__ spos) {
-    __shared__ int v[kRankMax];
+    __shared__ __align__(16) int v[kRankMax];
     const int nn = table_nodes(lvt);
     // blocks stride over the nodes (the grid is sized before the node count is known)
     for (int i = blockIdx.x; i < nn; i += gridDim.x) {
         if (nkids[i] != 0) continue;
         const int b = nbeg[i], m = nend[i] - b;
+        const int m4 = (m + 3) & ~3;
         __syncthreads();  // the previous leaf's ranks are read
-        for (int t = threadIdx.x; t < m; t += blockDim.x) v[t] = perm_in[b + t];
+        // padded to a multiple of 4 with INT_MAX, which ranks below no index
+        for (int t = threadIdx.x; t < m4; t += blockDim.x) v[t] = t < m ? perm_in[b + t] : INT_MAX;
         __syncthreads();
+        const int4* v4 = reinterpret_cast<const int4*>(v);
         for (int t = threadIdx.x; t < m; t += blockDim.x) {
             const int s = v[t];
             int r = 0;
-            for (int k = 0; k < m; ++k) r += v[k] < s ? 1 : 0;
+            // four indices per LDS read, several reads in flight
+#pragma unroll 4
+            for (int k = 0; k < m4 / 4; ++k) {
+                const int4 q = v4[k];
+                r += (q.x < s ? 1 : 0) + (q.y < s ? 1 : 0) + (q.z < s ? 1 : 0) + (q.w < s ? 1 : 0);
+            }
             perm_out[b + r] = s;
             spos[b + r] = make_double4(x[s], y[s], z[s], 0.0);
         }

@@ -40,12 +40,10 @@ namespace lpm {
 namespace {
 
 constexpr int kStackB = 160;  // >= 7 * kMaxDepth + 1
-// The planning pass pops the top stack entries whose children fit in 16 rows
-// (at least one) and pushes up to 16 children.  Seen as segments (the
-// children pushed by one call), every segment below the top has lost an entry
-// (at most 15 left) and the segments' smallest depths increase up the stack
-// (at most kMaxDepth segments).
-constexpr int kStackPlan = 15 * kMaxDepth + 16;
+// Planning-stack entries per wave (the bound for a given number of rows per
+// model call is plan_stack_bound): with the model, the rows' inputs and the
+// row map, a block of four waves stays within a fifth of the CU's LDS.
+constexpr int kStackPlan = 640;
 constexpr int kLabelOrder = 12;
 constexpr int kMaxLearnedOrder = 12;
 
@@ -114,30 +112,6 @@ __device__ inline void pair_features(float* f, double rn, double rb, double s, f
     f[9] = leaf ? 1.f : 0.f;
 #pragma unroll
     for (int k = kFeatures; k < kIn; ++k) f[k] = 0.f;
-}
-
-// Features 4g .. 4g+3 of the pair (the lane's input group g of the model
-// call), each formed by the same operations as in pair_features (so the
-// values are identical), with one log2 per feature slot instead of all
-// seven: the slot's log argument is chosen by g first.
-__device__ inline void pair_features_group(int g, float& x0, float& x1, float& x2, float& x3, double rn, double rb,
-                                           double s, float4 st, bool leaf, int nvalid, double u_ref) {
-    const float sf = float(s);
-    const float is = __builtin_amdgcn_rcpf(sf);
-    const float rnf = float(fmax(rn, 1e-12 * s)), rbf = float(fmax(rb, 1e-12 * s));
-    const float m0 = st.x, m1 = st.y, m2 = st.z;
-    const float mr = m0 * rnf;
-    const bool g0 = g == 0, g1 = g == 1, g2 = g == 2;
-    // slot 0: log2(r_n/s) | log2(M1/(M0 r_n)) | log2 s | 0
-    const float a0 = g0 ? rnf * is : g1 ? m1 * __builtin_amdgcn_rcpf(mr) : sf;
-    x0 = (g0 || g2 || (g1 && mr > 0.f)) ? flog2(a0) : (g1 ? -100.f : 0.f);
-    // slot 1: log2(r_b/s) | log2(M2/(M0 r_n^2)) | leaf flag | 0
-    const float a1 = g0 ? rbf * is : m2 * __builtin_amdgcn_rcpf(mr * rnf);
-    x1 = (g0 || (g1 && mr > 0.f)) ? flog2(a1) : (g1 ? -100.f : (g2 && leaf ? 1.f : 0.f));
-    // slot 2: (r_n + r_b)/s | log2(npart) | 0 | 0
-    x2 = g0 ? (rnf + rbf) * is : g1 ? flog2(st.w) : 0.f;
-    // slot 3: log2(M0/(u_ref s)) | nvalid/64 | 0 | 0
-    x3 = g0 ? flog2(m0 * is * __builtin_amdgcn_rcpf(float(u_ref))) : g1 ? float(nvalid) * (1.f / 64.f) : 0.f;
 }
 
 template <typename T, int... Ps>
@@ -566,32 +540,43 @@ __device__ inline int64_t list_batch(const Lists& ls, int64_t w) {
     return __builtin_amdgcn_readfirstlane(ls.queue[int64_t(b) * ls.qcap + (w - int64_t(before))]);
 }
 
-// Five waves per SIMD (96 registers with the accumulation registers the
-// compiler uses as spill space; a six-wave bound spilled to scratch and was
-// no faster): the traversal waits on node loads.  STEREO: accepted pairs may
-// become stereographic entries (the batch's plane centre and radius in both
-// projections are kept in scalar registers).
-template <bool STEREO>
+// Rows of one model call: the planning pass pops the top stack entries
+// whose children fit in ROWS rows (at least one entry: a node has at most 8
+// children) and pushes up to ROWS children.  Seen as segments (the children
+// pushed by one call), every segment below the top has lost an entry (at
+// most ROWS - 1 left) and the segments' smallest depths increase up the
+// stack, so at most (ROWS - 1) d + ROWS entries are held for a tree of depth
+// d.  The host takes the largest ROWS whose bound fits kStackPlan.
+__host__ __device__ constexpr int plan_stack_bound(int rows, int depth) { return (rows - 1) * depth + rows; }
+static_assert(plan_stack_bound(16, kMaxDepth) <= kStackPlan, "16 rows fit any tree");
+
+// One lane per row: lane l computes the geometry and the ten features of
+// row l's child, the rows go through the model in ROWS / 16 tiles of 16, and
+// lane l files its child.  Five waves per SIMD: the traversal waits on node
+// loads.  STEREO: accepted pairs may become stereographic entries (the
+// batch's plane centre and radius in both projections are kept in scalar
+// registers).
+template <bool STEREO, int ROWS>
 __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv, double theta_max, int margin,
                                                          int max_order, double u_ref,
                                                          const unsigned short* __restrict__ wts, MlpNorm norm,
                                                          Lists ls, StereoView sv,
                                                          unsigned long long* __restrict__ stats) {
+    static_assert(ROWS == 16 || ROWS == 32 || ROWS == 64, "rows of a model call");
+    constexpr int RT = ROWS / 16;
     __shared__ __align__(16) unsigned char s_model[kModelLds];
     __shared__ int s_node[4][kStackPlan];
-    __shared__ int s_row[4][16];  // child node of each row of a model call
-    // the input normalization in LDS (mean, inv_std, lo, hi; 4 inputs per
-    // float4): a lane reads the four of its input group per model call
-    // instead of the kernel holding 64 values in scalar registers
-    __shared__ float4 s_norm[4][kIn / 4];
+    __shared__ int s_row[4][ROWS];  // child node of each row of a model call
+    // the BF16 model inputs of each row, read back in the MFMA operand layout
+    __shared__ __align__(16) unsigned s_feat[4][ROWS][kIn / 2];
+    // the input normalization (mean, inv_std, lo, hi) of each input
+    __shared__ float4 s_norm[kIn];
     mlp_stage_weights(wts, s_model);
-    if (threadIdx.x < 4 * kIn) {
-        const int q = threadIdx.x / kIn, k = threadIdx.x % kIn;
-        const float* src = q == 0 ? norm.mean : q == 1 ? norm.inv_std : q == 2 ? norm.lo : norm.hi;
-        reinterpret_cast<float*>(&s_norm[q][0])[k] = src[k];
-    }
+    if (threadIdx.x < kIn)
+        s_norm[threadIdx.x] = make_float4(norm.mean[threadIdx.x], norm.inv_std[threadIdx.x], norm.lo[threadIdx.x],
+                                          norm.hi[threadIdx.x]);
     __syncthreads();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
     const int64_t bi = wave_index();
     if (!is_batch(tg, bi)) return;
     bool valid;
@@ -612,6 +597,15 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
     unsigned long long near_pairs = 0, nterms = 0, nterms_hi = 0, ncalls = 0, nst = 0, nterms_st = 0;
     unsigned l_terms = 0, l_terms_hi = 0, l_np = 0, l_terms_st = 0;
     const uint64_t below = (1ull << lane) - 1ull;
+    // normalized inputs kFeatures .. kIn - 1 (always 0 before normalization)
+    // as BF16 pairs, the same for every row
+    unsigned pad[(kIn - kFeatures) / 2];
+#pragma unroll
+    for (int k = kFeatures; k < kIn; k += 2) {
+        const float4 a = s_norm[k], b = s_norm[k + 1];
+        pad[(k - kFeatures) / 2] = cvt_bf16x2((__builtin_amdgcn_fmed3f(0.f, a.z, a.w) - a.x) * a.y,
+                                              (__builtin_amdgcn_fmed3f(0.f, b.z, b.w) - b.x) * b.y);
+    }
     int sp = 0;
     const int4 root = tv.nodew[0];
     if (root.w == 0) {  // the root is a leaf
@@ -623,33 +617,31 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         sp = 1;
     }
     // The stack holds internal nodes to descend into, as their first child and
-    // child count (stack_entry), so a pop needs no node-table load.  The top entries whose
-    // children fit in the 16 rows of one model call are popped together (at
-    // least one: a node has at most 8 children, so two always fit) and their
-    // children classified at once, rows in stack order; lane l < 16 then files
-    // its child: accepted -> far list (order p), leaf -> near list, else ->
-    // the stack.
+    // child count (stack_entry), so a pop needs no node-table load.  The top
+    // entries whose children fit in the ROWS rows of one model call are
+    // popped together, rows in stack order; lane l then files row l's child:
+    // accepted -> far list (order p), leaf -> near list, else -> the stack.
     while (sp > 0) {
-        const int ncand = min(sp, 8);
+        // lane j < ncand holds the j-th entry from the top; incl = its
+        // inclusive prefix sum of child counts (counts 0..8, summed bit by bit
+        // over the lanes with ballots)
+        const int ncand = min(sp, ROWS);
         int kc = 0, kf = 0;
         if (lane < ncand) {
             const int ent = s_node[w][sp - 1 - lane];
             kc = ent >> kNodeBits;
             kf = ent & ((1 << kNodeBits) - 1);
         }
-        // the popped entries are those whose children fit in the rows so far
-        // (inclusive sums of the child counts over lanes 0..7, a prefix of
-        // the lanes); entry j's children take rows incl - kc .. incl - 1,
-        // and s_row maps each row to its child node
-        int incl = kc;
+        int incl = 0;
 #pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            const int v = __shfl_up(incl, o, 8);
-            if ((lane & 7) >= o) incl += v;
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t mb = __ballot((kc >> b) & 1);
+            incl += __popcll(mb & (below | (1ull << lane))) << b;
         }
-        const bool take = lane < ncand && incl <= 16;
+        // the taken entries are a prefix of the lanes (incl is non-decreasing)
+        const bool take = lane < ncand && incl <= ROWS;
         const int m = __popcll(__ballot(take));
-        const int tot = __shfl(incl, m - 1, 64);
+        const int tot = __builtin_amdgcn_readlane(incl, m - 1);
         if (take) {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
@@ -657,20 +649,18 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         }
         wave_sync();
         sp -= m;
-        // lane l works on row r = l % 16; the lanes of one row hold its
-        // inputs 4g..4g+3 (g = l / 16)
-        const int row = lane & 15, g = lane >> 4;
-        const bool has = row < tot;
-        const int chr = has ? s_row[w][row] : 0;
+        const bool has = lane < tot;
+        const int chr = has ? s_row[w][lane] : 0;
         bool prop = false;
         bool near_chk = true;  // a leaf child's sources may come within 1e-6 of a target
         // stereographic geometry test of the pair in the batch's projection,
-        // else in the other one, taken before the model call so that its node
-        // loads overlap the features
+        // else in the other one
         bool st_geo = false, so_geo = false;
-        float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
         // every node-table word of the child is loaded here, in one round
         int ckids = 0, cfirst = 0, cb = 0, ce = 0;
+        unsigned xin[kIn / 2];
+#pragma unroll
+        for (int k = 0; k < kIn / 2; ++k) xin[k] = 0u;
         if (has) {
             const int4 nw = tv.nodew[chr];
             cb = nw.x;
@@ -686,22 +676,40 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
             if (prop) {
                 if (st_on) st_geo = stereo_fits(sv, sb, chr, sd, rn, bt.r);
                 if (!st_geo && so_on) so_geo = stereo_fits(sv, so, chr, sd, rn, bt.r);
-                pair_features_group(g, x0, x1, x2, x3, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
-                const float4 mu = s_norm[0][g], iv = s_norm[1][g], lo = s_norm[2][g], hi = s_norm[3][g];
-                // clamp to [lo, hi] (lo <= hi): the median of the three
-                x0 = (__builtin_amdgcn_fmed3f(x0, lo.x, hi.x) - mu.x) * iv.x;
-                x1 = (__builtin_amdgcn_fmed3f(x1, lo.y, hi.y) - mu.y) * iv.y;
-                x2 = (__builtin_amdgcn_fmed3f(x2, lo.z, hi.z) - mu.z) * iv.z;
-                x3 = (__builtin_amdgcn_fmed3f(x3, lo.w, hi.w) - mu.w) * iv.w;
+                float f[kIn];
+                pair_features(f, rn, bt.r, sd, tv.nstat[chr], ckids == 0, bt.nvalid, u_ref);
+#pragma unroll
+                for (int k = 0; k < kFeatures; k += 2) {
+                    const float4 a = s_norm[k], b = s_norm[k + 1];
+                    // clamp to [lo, hi] (lo <= hi): the median of the three
+                    xin[k / 2] = cvt_bf16x2((__builtin_amdgcn_fmed3f(f[k], a.z, a.w) - a.x) * a.y,
+                                            (__builtin_amdgcn_fmed3f(f[k + 1], b.z, b.w) - b.x) * b.y);
+                }
+#pragma unroll
+                for (int k = kFeatures; k < kIn; k += 2) xin[k / 2] = pad[(k - kFeatures) / 2];
             }
         }
         int p = 0;
         if (__ballot(prop)) {
-            const int q = mlp_classify_regs(s_model, cvt_bf16x4(x0, x1, x2, x3));
+            if (lane < ROWS) {
+                uint4* dst = reinterpret_cast<uint4*>(&s_feat[w][lane][0]);
+                dst[0] = make_uint4(xin[0], xin[1], xin[2], xin[3]);
+                dst[1] = make_uint4(xin[4], xin[5], xin[6], xin[7]);
+            }
+            wave_sync();
+            // lane (g, c) of tile t: inputs 4g .. 4g+3 of row 16 t + c
+            bf16x4 x[RT];
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const uint2 v = *reinterpret_cast<const uint2*>(&s_feat[w][16 * t + c16][2 * g]);
+                x[t] = as_bf16x4(v);
+            }
+            const int q = mlp_classify_tiles<RT>(s_model, x);
             ++ncalls;
             if (prop && q > 0 && q <= max_order) p = min(q + margin, max_order);
+            wave_sync();  // the next call's rows overwrite s_feat
         }
-        const bool mine = lane < 16 && has;
+        const bool mine = has;
         const bool leaf = mine && ckids == 0;
         // an accepted pair that passes the stereographic geometry test
         // becomes a stereographic entry of order p + extra
@@ -738,7 +746,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         nnear += c_near;
         sp += c_desc;
         // terms of the accepted children and sources of the near leaves, kept
-        // per lane (lanes 0..15) and summed over the wave after the walk
+        // per lane and summed over the wave after the walk
         const unsigned t_all = cart ? unsigned(p * (p + 1) * (p + 2) / 6) : 0u;
         l_terms += t_all;
         l_terms_hi += is_hi ? t_all : 0u;
@@ -748,7 +756,7 @@ __global__ __launch_bounds__(256) void tree_plan_learned(Targets tg, TreeView tv
         wave_sync();
     }
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
+    for (int o = 1; o < 64; o <<= 1) {
         l_terms += __shfl_xor(l_terms, o, 64);
         l_terms_hi += __shfl_xor(l_terms_hi, o, 64);
         l_np += __shfl_xor(l_np, o, 64);
@@ -1235,12 +1243,23 @@ int tree_eval_learned_device(lpm_ctx* c, int prec, double theta_max, int margin,
             LPM_HIP(c, hipMemsetAsync(c->stats_ptr, 0, kStatBytes, st));
         }
         LPM_HIP(c, join_aux(c));  // the stereographic node data
-        if (sv.on)
-            hipLaunchKernelGGL(tree_plan_learned<true>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
-                               max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
-        else
-            hipLaunchKernelGGL(tree_plan_learned<false>, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin,
-                               max_order, u_ref, w, norm, ls, sv, c->stats_ptr);
+        auto plan = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, theta_max, margin, max_order, u_ref, w, norm,
+                               ls, sv, c->stats_ptr);
+        };
+        // the most rows per model call whose stack bound fits this tree
+        const int rows = plan_stack_bound(64, t->depth) <= kStackPlan   ? 64
+                         : plan_stack_bound(32, t->depth) <= kStackPlan ? 32
+                                                                        : 16;
+        if (sv.on) {
+            if (rows == 64) plan(tree_plan_learned<true, 64>);
+            else if (rows == 32) plan(tree_plan_learned<true, 32>);
+            else plan(tree_plan_learned<true, 16>);
+        } else {
+            if (rows == 64) plan(tree_plan_learned<false, 64>);
+            else if (rows == 32) plan(tree_plan_learned<false, 32>);
+            else plan(tree_plan_learned<false, 16>);
+        }
         auto lists = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, tg, tv, ls, sv, d2, eps2v, acc);
         };
