@@ -190,8 +190,30 @@ __device__ __forceinline__ void near_leaf(int b, int e, double t0, double t1, do
             a1 = fmaf(wgt, dy, a1);
             a2 = fmaf(wgt, dz, a2);
         };
-#pragma unroll 4
-        for (int j = b; j < e; ++j) pair(soff[j]);
+        // four sources a step, the next four requested before the current
+        // four are summed
+        const float4* src = soff + b;
+        const int n = e - b;
+        int j = 0;
+        if (n >= 8) {
+            float4 c0 = src[0], c1 = src[1], c2 = src[2], c3 = src[3];
+            for (j = 4; j + 4 <= n; j += 4) {
+                const float4 n0 = src[j], n1 = src[j + 1], n2 = src[j + 2], n3 = src[j + 3];
+                pair(c0);
+                pair(c1);
+                pair(c2);
+                pair(c3);
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+            pair(c0);
+            pair(c1);
+            pair(c2);
+            pair(c3);
+        }
+        for (; j < n; ++j) pair(src[j]);
         // (t x y) gamma / den = -2 t x d gamma / (|d|^2 + 2 delta^2): -2 d
         // gamma / den joins v, crossed with t once per target
         v0 -= 2.0 * double(a0);
@@ -801,10 +823,18 @@ __device__ inline void stereo_entries(const StereoView& sv, const StereoBatch& s
     float sr = 0.f, si = 0.f, gs = 0.f;
     const int64_t base = int64_t(sb.proj) * sv.nn;
     const unsigned mask = (1u << kNodeBits) - 1u;
+    // the next entry's node data is requested before the current entry is
+    // summed, the entry after it one step earlier still
+    unsigned en = unsigned(__builtin_amdgcn_readfirstlane(e[0]));
+    unsigned en1 = unsigned(__builtin_amdgcn_readfirstlane(e[n > 1 ? 1 : 0]));
+    StNode cur = stereo_fetch(sv, base + (en & mask));
     for (int j = 0; j < n; ++j) {
-        const unsigned en = unsigned(__builtin_amdgcn_readfirstlane(e[j]));
-        const StNode cur = stereo_fetch(sv, base + (en & mask));
+        const unsigned en2 = unsigned(__builtin_amdgcn_readfirstlane(e[j + 2 < n ? j + 2 : n - 1]));
+        const StNode nxt = stereo_fetch(sv, base + (en1 & mask));
         stereo_sum(cur, int(en >> kNodeBits), base + (en & mask), sv, sb.zr, sb.zi, sr, si, gs);
+        cur = nxt;
+        en = en1;
+        en1 = en2;
         if ((j % kStRun) == kStRun - 1 || j + 1 == n) {
             Sr += double(sr);
             Si += double(si);
@@ -844,16 +874,31 @@ __global__ __launch_bounds__(256) void tree_eval_lists(Targets tg, TreeView tv, 
     if (LOW) {
         const int* near = ls.near + bi * ls.near_cap;
         const int nnear = __builtin_amdgcn_readfirstlane(ls.near_cnt[bi]);
+        // the next leaf's entry, range and centre are requested before the
+        // current leaf is summed
+        int ent = nnear > 0 ? __builtin_amdgcn_readfirstlane(near[0]) : 0;
+        int4 nw = tv.nodew[ent & ~kNearCheck];
+        double4 cc = tv.ncent[ent & ~kNearCheck];
         for (int j = 0; j < nnear; ++j) {
-            const int ent = __builtin_amdgcn_readfirstlane(near[j]);
-            const int n = ent & ~kNearCheck;
-            const int4 nw = tv.nodew[n];
-            if (!F32 || (ent & kNearCheck))
-                near_leaf<F32, true>(nw.x, nw.y, t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
-                                     v0, v1, v2);
+            const int ent1 = __builtin_amdgcn_readfirstlane(near[j + 1 < nnear ? j + 1 : j]);
+            const int4 nw1 = tv.nodew[ent1 & ~kNearCheck];
+            const double4 cc1 = tv.ncent[ent1 & ~kNearCheck];
+            bool chk = !F32 || (ent & kNearCheck);
+            if (F32 && chk) {
+                // a source within 1e-6 of a target needs the target within
+                // r_n + 1e-6 of the leaf centre (FP32 offsets: 2e-6)
+                const float q0 = float(t0 - cc.x), q1 = float(t1 - cc.y), q2 = float(t2 - cc.z);
+                const float lim = float(tv.nrad[ent & ~kNearCheck]) + 2e-6f;
+                chk = __ballot(valid && fmaf(q0, q0, fmaf(q1, q1, q2 * q2)) <= lim * lim) != 0ull;
+            }
+            if (chk)
+                near_leaf<F32, true>(nw.x, nw.y, t0, t1, t2, cc, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1, v2);
             else
-                near_leaf<F32, false>(nw.x, nw.y, t0, t1, t2, tv.ncent[n], d2, eps2v, tv.spos, tv.soff, s0, s1, s2,
-                                      v0, v1, v2);
+                near_leaf<F32, false>(nw.x, nw.y, t0, t1, t2, cc, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1,
+                                      v2);
+            ent = ent1;
+            nw = nw1;
+            cc = cc1;
         }
     }
     for (int j = j0; j < j1; ++j) {
