@@ -240,11 +240,12 @@ __device__ inline int mlp_classify_regs(const unsigned char* model, bf16x4 x) {
 
 // Inference for up to 16 RT rows in RT row tiles (tile t: rows 16 t ..
 // 16 t + 15), with the layout and operations of mlp_classify_regs per tile:
-// x[t] holds lane (g, c)'s inputs 4g..4g+3 of row 16 t + c.  Each weight
-// fragment is read from LDS once and applied to every tile.  Returns in lane
-// l the argmax class of row l (for RT = 1, of row l % 16 in every lane).
-template <int RT>
-__device__ inline int mlp_classify_tiles(const unsigned char* model, const bf16x4 (&x)[RT]) {
+// x[t] holds lane (g, c)'s inputs 4g..4g+3 of row 16 t + c.  The tiles go
+// through the layers in groups of GRP: each weight fragment is read from LDS
+// once per group and applied to the group's tiles.  Returns in lane l the
+// argmax class of row l (for RT = 1, of row l % 16 in every lane).
+template <int GRP>
+__device__ inline void mlp_tile_group(const unsigned char* model, const bf16x4* x, int t0, int& mine) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint2* f1 = reinterpret_cast<const uint2*>(model);
     const uint4* f2 = reinterpret_cast<const uint4*>(model + kFragW1);
@@ -256,47 +257,46 @@ __device__ inline int mlp_classify_tiles(const unsigned char* model, const bf16x
         const float4 b = *reinterpret_cast<const float4*>(bv + 4 * g);
         return f32x4{b.x, b.y, b.z, b.w};
     };
-    bf16x4 h1[RT][4];
+    bf16x4 h1[GRP][4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const bf16x4 w = as_bf16x4(f1[nt * 64 + lane]);
         const f32x4 b = bias(b1 + nt * 16);
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
+        for (int t = 0; t < GRP; ++t) {
             const f32x4 acc = mfma16(w, x[t], b);
             h1[t][nt] = relu_bf16x4(acc[0], acc[1], acc[2], acc[3]);
         }
     }
-    f32x4 out[RT];
+    f32x4 out[GRP];
 #pragma unroll
-    for (int t = 0; t < RT; ++t) out[t] = bias(b3);
+    for (int t = 0; t < GRP; ++t) out[t] = bias(b3);
     // layer 2 in two halves of two output tiles each; each half is the K
     // block of layer 3 that consumes it
 #pragma unroll
     for (int hf = 0; hf < 2; ++hf) {
-        bf16x4 h2[RT][2];
+        bf16x4 h2[GRP][2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int nt = 2 * hf + j;
-            f32x4 acc[RT];
+            f32x4 acc[GRP];
 #pragma unroll
-            for (int t = 0; t < RT; ++t) acc[t] = bias(b2 + nt * 16);
+            for (int t = 0; t < GRP; ++t) acc[t] = bias(b2 + nt * 16);
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
                 const uint4 w = f2[(2 * nt + kb) * 64 + lane];
 #pragma unroll
-                for (int t = 0; t < RT; ++t) acc[t] = mfma32(w, join_bf16(h1[t][2 * kb], h1[t][2 * kb + 1]), acc[t]);
+                for (int t = 0; t < GRP; ++t) acc[t] = mfma32(w, join_bf16(h1[t][2 * kb], h1[t][2 * kb + 1]), acc[t]);
             }
 #pragma unroll
-            for (int t = 0; t < RT; ++t) h2[t][j] = relu_bf16x4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+            for (int t = 0; t < GRP; ++t) h2[t][j] = relu_bf16x4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
         }
         const uint4 w = f3[hf * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < RT; ++t) out[t] = mfma32(w, join_bf16(h2[t][0], h2[t][1]), out[t]);
+        for (int t = 0; t < GRP; ++t) out[t] = mfma32(w, join_bf16(h2[t][0], h2[t][1]), out[t]);
     }
-    int mine = 0;
 #pragma unroll
-    for (int t = 0; t < RT; ++t) {
+    for (int t = 0; t < GRP; ++t) {
         float v = -3.0e38f;
         int best = 0;
 #pragma unroll
@@ -308,8 +308,17 @@ __device__ inline int mlp_classify_tiles(const unsigned char* model, const bf16x
             const int ob = __shfl_xor(best, o, 64);
             if (ov > v || (ov == v && ob < best)) { v = ov; best = ob; }
         }
-        if (RT == 1 || t == g) mine = best;
+        if (t0 + t == g) mine = best;
     }
+}
+
+template <int RT, int GRP = RT>
+__device__ inline int mlp_classify_tiles(const unsigned char* model, const bf16x4 (&x)[RT]) {
+    static_assert(RT % GRP == 0, "whole groups of tiles");
+    if (RT == 1) return mlp_classify_regs(model, x[0]);
+    int mine = 0;
+#pragma unroll
+    for (int t0 = 0; t0 < RT; t0 += GRP) mlp_tile_group<GRP>(model, x + t0, t0, mine);
     return mine;
 }
 
