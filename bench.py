@@ -686,8 +686,10 @@ def measured_traffic(kernel: str, args) -> tuple:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    # 20 timed steps: a learned step is ~2 ms, so fewer leave the timing to a
+    # handful of launches' jitter
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=9)
     # learned treecode
     ap.add_argument("--leaf", type=int, default=None, help="default 192 (stereographic), 128 (Cartesian)")
