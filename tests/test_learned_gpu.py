@@ -295,6 +295,47 @@ def test_learned_constant_model_all_list_kernels(ctx, mesh, order):
         assert d < (1e-13 if prec == L.LPM_F64 else 1e-6)
 
 
+def _clustered(cap, nu=20000, nc=4096, seed=3):
+    """nu uniform points on the unit sphere and nc in a cap of angular radius cap."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(nu, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    ax = np.array([0.3, -0.5, 0.81])
+    ax /= np.linalg.norm(ax)
+    t1 = np.cross(ax, [1.0, 0.0, 0.0])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(ax, t1)
+    r, ph = cap * np.sqrt(rng.random(nc)), 2.0 * np.pi * rng.random(nc)
+    c = ax + (r * np.cos(ph))[:, None] * t1 + (r * np.sin(ph))[:, None] * t2
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    P = np.vstack([u, c])
+    return tuple(np.ascontiguousarray(P[:, k]) for k in range(3)), rng.normal(size=len(P)) * 1e-4
+
+
+@pytest.mark.parametrize("cap,leaf,min_depth", [(1e-3, 32, 10), (2e-5, 16, 20)])
+def test_learned_deep_tree_fewer_rows_per_call(ctx, cap, leaf, min_depth):
+    """A tight cluster deepens the tree, and the planning pass then classifies
+    fewer children per model call (64 rows up to depth 9, 32 up to depth 19,
+    16 beyond: the bound on its stack).  With a constant model it must still
+    reproduce the fixed-order batch treecode: the same far and near pairs and
+    terms, the same velocity (FP64 within 1e-13, FP32C within 1e-6)."""
+    src, g = _clustered(cap)
+    nn, depth = T.build(ctx, src, float(np.sqrt(src[0] ** 2 + src[1] ** 2 + src[2] ** 2).max()), leaf)
+    print(f"\ncluster cap {cap}, leaf {leaf}: {nn} nodes, depth {depth}")
+    assert depth >= min_depth
+    T.moments(ctx, g, 8, 1)
+    u_ref = LN.u_ref_of(g)
+    for prec in (L.LPM_F64, L.LPM_F32C):
+        LN.constant_model(ctx, 6)
+        ul = LN.evaluate(ctx, src, 0.6, 0, 8, u_ref, prec=prec)
+        sl = T.stats(ctx)
+        ub = T.evaluate_batch(ctx, src, 0.6, 6, prec=prec)
+        sb = T.stats(ctx)
+        assert sl["model"] > 0
+        assert (sl["far"], sl["near"], sl["terms"]) == (sb["far"], sb["near"], sb["terms"])
+        assert O.rel_max(np.stack(ul), np.stack(ub)) < (1e-13 if prec == L.LPM_F64 else 1e-6)
+
+
 def test_learned_planning_clamps_inputs_to_the_model_range(ctx, mesh):
     """A one-target batch has radius 0, so log2(r_b / s) is -39.9, far below
     anything in training.  A hand-made model answers order 7 when
