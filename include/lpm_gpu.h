@@ -101,8 +101,9 @@ void* lpm_stream(lpm_ctx* ctx);
  * gamma_j  : source strength (vorticity x area in the mesh solve).
  * tgt_self_src : per-target source index excluded from the potential sum
  *             (the face-target self pair of src/LpmDirectSum.cpp:52-75), -1 for
- *             none; NULL means no exclusions.  The velocity needs none: its
- *             self pair is exactly zero.
+ *             none (any other value outside [0, ns) is invalid); NULL means no
+ *             exclusions.  The velocity needs none: its self pair is exactly
+ *             zero.
  * ux,uy,uz : required when the velocity kernel is requested, psi when the
  *             potential kernel is requested; others may be NULL.
  * nt or ns may be 0 (outputs are then zero / untouched respectively).

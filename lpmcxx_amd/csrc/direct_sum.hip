@@ -585,7 +585,7 @@ __global__ __launch_bounds__(256) void ds_order_gather(const int* __restrict__ p
                                                        double* __restrict__ as, double* __restrict__ bs,
                                                        double* __restrict__ cs, double* __restrict__ ds,
                                                        const int64_t* __restrict__ skip, const int* __restrict__ inv,
-                                                       int64_t* __restrict__ skip_s) {
+                                                       int64_t ns, int64_t* __restrict__ skip_s) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int k = perm[i];
@@ -593,7 +593,10 @@ __global__ __launch_bounds__(256) void ds_order_gather(const int* __restrict__ p
     bs[i] = b[k];
     cs[i] = c[k];
     if (d) ds[i] = d[k];
-    if (skip) skip_s[i] = skip[k] >= 0 ? int64_t(inv[skip[k]]) : skip[k];
+    if (skip) {
+        const int64_t q = skip[k];
+        skip_s[i] = (q >= 0 && q < ns) ? int64_t(inv[q]) : int64_t(-1);  // a source index, or none
+    }
 }
 
 __global__ __launch_bounds__(256) void ds_order_scatter(const int* __restrict__ perm, int64_t n,
@@ -670,10 +673,10 @@ int direct_sum_device(lpm_ctx* c, int kernels, int prec, double radius, double d
     hipLaunchKernelGGL(ds_order_inverse, dim3(bs), dim3(256), 0, st, static_cast<const int*>(ps), ns, inv);
     hipLaunchKernelGGL(ds_order_gather, dim3(bs), dim3(256), 0, st, static_cast<const int*>(ps), ns, sx, sy, sz, gamma,
                        S, S + ns, S + 2 * ns, S + 3 * ns, static_cast<const int64_t*>(nullptr),
-                       static_cast<const int*>(nullptr), static_cast<int64_t*>(nullptr));
+                       static_cast<const int*>(nullptr), ns, static_cast<int64_t*>(nullptr));
     hipLaunchKernelGGL(ds_order_gather, dim3(bt), dim3(256), 0, st, static_cast<const int*>(pt), nt, tx, ty, tz,
                        static_cast<const double*>(nullptr), T, T + nt, T + 2 * nt, static_cast<double*>(nullptr), skip,
-                       static_cast<const int*>(inv), sk);
+                       static_cast<const int*>(inv), ns, sk);
     const bool vel = (kernels & LPM_K_BIOT_SAVART_SPHERE) != 0;
     const bool pot = (kernels & LPM_K_GREENS_SPHERE) != 0;
     LPM_CHECK(ds_launch(c, P, kernels, prec, radius, delta, T, T + nt, T + 2 * nt, nt, S, S + ns, S + 2 * ns, S + 3 * ns,
