@@ -336,6 +336,45 @@ def test_learned_deep_tree_fewer_rows_per_call(ctx, cap, leaf, min_depth):
         assert O.rel_max(np.stack(ul), np.stack(ub)) < (1e-13 if prec == L.LPM_F64 else 1e-6)
 
 
+def test_learned_zero_rule_near_duplicates(ctx, mesh):
+    """Sources with near-duplicates: 24 partners 2e-7 away (inside the
+    reference's zero rule |d|^2 / 2 <= 1e-13: the pair contributes nothing)
+    and 24 partners 1e-5 away (a full contribution), targets = sources.
+    Class 0 (every pair summed in the near field): FP64 within 1e-12 and
+    FP32C within 1e-3 of the oracle's direct sum with the same rule; class 6
+    (far + near): FP32C equal to the fixed-order batch treecode to 1e-6."""
+    m = mesh(L.LPM_SEED_CUBED, 5)
+    P = np.stack([m.cx, m.cy, m.cz], 1)
+    rng = np.random.default_rng(8)
+    pick = rng.choice(len(P), 48, replace=False)
+    extra = []
+    for i, d in zip(pick, [2e-7] * 24 + [1e-5] * 24):
+        t = np.cross(P[i], rng.normal(size=3))
+        q = P[i] + d * t / np.linalg.norm(t)
+        extra.append(q / np.linalg.norm(q))
+    Q = np.vstack([P, np.array(extra)])
+    src = tuple(np.ascontiguousarray(Q[:, k]) for k in range(3))
+    g = np.concatenate([m.area, np.full(len(extra), m.area.mean())]) * rng.uniform(0.5, 1.5, len(Q))
+    T.build(ctx, src, float(np.sqrt((Q ** 2).sum(1)).max()), 32)
+    T.moments(ctx, g, 8, 1)
+    u_ref = LN.u_ref_of(g)
+    ud = np.stack(O.bs_direct(src, src, g))
+    LN.constant_model(ctx, 0)
+    u64 = np.stack(LN.evaluate(ctx, src, 0.6, 0, 8, u_ref, prec=L.LPM_F64))
+    u32 = np.stack(LN.evaluate(ctx, src, 0.6, 0, 8, u_ref, prec=L.LPM_F32C))
+    assert T.stats(ctx)["far"] == 0
+    e64, e32 = O.rel_max(u64, ud), O.rel_l2(u32, ud)
+    print(f"\nnear duplicates, all pairs near: FP64 rel max {e64:.2e}, FP32C rel-L2 {e32:.2e}")
+    assert e64 < 1e-12 and e32 < 1e-3
+    LN.constant_model(ctx, 6)
+    ul = np.stack(LN.evaluate(ctx, src, 0.6, 0, 8, u_ref, prec=L.LPM_F32C))
+    sl = T.stats(ctx)
+    ub = np.stack(T.evaluate_batch(ctx, src, 0.6, 6, prec=L.LPM_F32C))
+    sb = T.stats(ctx)
+    assert (sl["far"], sl["near"]) == (sb["far"], sb["near"])
+    assert O.rel_max(ul, ub) < 1e-6
+
+
 def test_learned_planning_clamps_inputs_to_the_model_range(ctx, mesh):
     """A one-target batch has radius 0, so log2(r_b / s) is -39.9, far below
     anything in training.  A hand-made model answers order 7 when
