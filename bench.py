@@ -722,7 +722,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-generalization", action="store_true", help="skip the held-out field evaluation")
-    ap.add_argument("--no-direct64", action="store_true", help="judge errors against the FP32C direct sum")
+    ap.add_argument("--no-direct64", action="store_true", help="measure errors against the FP32C direct sum")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--no-rk4-l10", action="store_true", help="skip the level-10 RK4 comparison")
     ap.add_argument("--l10-level", type=int, default=10)
