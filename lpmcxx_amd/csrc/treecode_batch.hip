@@ -1000,16 +1000,31 @@ __global__ __launch_bounds__(256) void tree_eval_psi(Targets tg, TreeView tv, co
             const int b = tv.nbeg[n], e = tv.nend[n];
             if constexpr (F32) {
                 const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
-                float l2 = 0.f, gs = 0.f;
-                for (int j = b; j < e; ++j) {
-                    const float4 o = tv.soff[j];
-                    const float ex = q0 - o.x, ey = q1 - o.y, ez = q2 - o.z;
-                    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                    const bool ok = r2 > 0.f;
-                    l2 = fmaf(ok ? o.w : 0.f, __builtin_amdgcn_logf(ok ? r2 : 1.f), l2);
-                    gs += ok ? o.w : 0.f;
+                // a target that is one of the leaf's sources (a zero-distance
+                // pair, left out) lies within r_n of the leaf centre (FP32
+                // offsets: 2e-6 of margin); other waves sum every pair and
+                // take the leaf's strength sum from its degree-0 moment
+                const float lim = float(rn) + 2e-6f;
+                if (uniform(__ballot(valid && fmaf(q0, q0, fmaf(q1, q1, q2 * q2)) <= lim * lim) != 0ull)) {
+                    float l2 = 0.f, gs = 0.f;
+                    for (int j = b; j < e; ++j) {
+                        const float4 o = tv.soff[j];
+                        const float ex = q0 - o.x, ey = q1 - o.y, ez = q2 - o.z;
+                        const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                        const bool ok = r2 > 0.f;
+                        l2 = fmaf(ok ? o.w : 0.f, __builtin_amdgcn_logf(ok ? r2 : 1.f), l2);
+                        gs += ok ? o.w : 0.f;
+                    }
+                    acc += kLn2 * (double(l2) - double(gs));
+                } else {
+                    float l2 = 0.f;
+                    for (int j = b; j < e; ++j) {
+                        const float4 o = tv.soff[j];
+                        const float ex = q0 - o.x, ey = q1 - o.y, ez = q2 - o.z;
+                        l2 = fmaf(o.w, __builtin_amdgcn_logf(fmaf(ex, ex, fmaf(ey, ey, ez * ez))), l2);
+                    }
+                    acc += kLn2 * (double(l2) - pmomg[int64_t(n) * nterm]);
                 }
-                acc += kLn2 * (double(l2) - double(gs));
             } else {
                 for (int j = b; j < e; ++j) {
                     const double4 y = tv.spos[j];
