@@ -471,7 +471,18 @@ __global__ __launch_bounds__(256) void tree_eval_batch(Targets tg, TreeView tv, 
         const int nk = tv.nkids[n];
         if (nk == 0) {
             const int b = tv.nbeg[n], e = tv.nend[n];
-            near_leaf<F32>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1, v2);
+            // FP32: the zero-rule test only when some target of the wave lies
+            // within r_n + 1e-6 of the leaf centre (FP32 offsets: 2e-6)
+            bool chk = true;
+            if (F32) {
+                const float q0 = float(t0 - c.x), q1 = float(t1 - c.y), q2 = float(t2 - c.z);
+                const float lim = float(rn) + 2e-6f;
+                chk = __ballot(valid && fmaf(q0, q0, fmaf(q1, q1, q2 * q2)) <= lim * lim) != 0ull;
+            }
+            if (chk)
+                near_leaf<F32, true>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1, v2);
+            else
+                near_leaf<F32, false>(b, e, t0, t1, t2, c, d2, eps2v, tv.spos, tv.soff, s0, s1, s2, v0, v1, v2);
             nnear += unsigned(e - b);
         } else {
             const int f = tv.nfirst[n];
