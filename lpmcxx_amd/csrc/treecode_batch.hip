@@ -585,8 +585,9 @@ static_assert(plan_stack_bound(16, kMaxDepth) <= kStackPlan, "16 rows fit any tr
 
 // One lane per row: lane l computes the geometry and the ten features of
 // row l's child, the rows go through the model in ROWS / 16 tiles of 16, and
-// lane l files its child.  Five waves per SIMD: the traversal waits on node
-// loads.  STEREO: accepted pairs may become stereographic entries (the
+// lane l files its child.  Four waves per SIMD with 64 or 32 rows (the
+// register footprint of the model's row tiles; an occupancy bound spills),
+// five with 16.  STEREO: accepted pairs may become stereographic entries (the
 // batch's plane centre and radius in both projections are kept in scalar
 // registers).
 template <bool STEREO, int ROWS>
